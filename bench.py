@@ -1,0 +1,161 @@
+#!/usr/bin/env python3
+"""Benchmark of the MAC step on one MI355X.
+
+Workload: 192x256x192 cells, dx 0.5 mm, dt 0.25 ms, three seeded blood blobs
+(radius 7+-1 cells, w in [0.3, 0.8] m/s), 3-level V-cycle with 2+2 red-black
+sweeps and 8 cycles per step, plain FP32.  `value` is FP32 steps/s with all
+fields resident in HBM.  The same line carries the mixed-precision run
+(FP64 refinement to ||r||/||b|| <= 1e-8, FP16 level>=2 storage) and its
+speed-up, the smoother's roofline figures (HIP events on the solver's
+stream) and the FP64 CPU oracle timed on this host as the CPU baseline.
+
+python bench.py [--gpus 1] [--steps K] [--warmup W]
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(ROOT, "drop-of-blood_amd"))
+
+HBM_PEAK_GBPS = 8000.0
+# algorithmic bytes per level-0 cell of one red-black colour pass: read p and
+# rhs, write p (FP32)
+SMOOTH_BYTES_PER_CELL = 12.0
+
+
+def log(msg):
+    print(msg, file=sys.stderr, flush=True)
+
+
+def run_gpu(workload, steps, warmup, profile_steps=3):
+    from mac3d import MacSolver
+    from mac3d.lib import config_for
+    cfg = config_for(workload)
+    with MacSolver(cfg) as s:
+        s.init_blobs(workload.blobs)
+        if warmup:
+            s.step_nostats(workload.dt, warmup)
+        t0 = time.perf_counter()
+        st = s.step(workload.dt, steps)  # blocks until the stream has drained
+        wall = time.perf_counter() - t0
+        # stats add one divergence-norm pass after the timed kernels; the
+        # kernel span of the K steps is st["ms"]
+        prof = {}
+        if profile_steps:
+            s.profile(True)
+            s.step_nostats(workload.dt, profile_steps)
+            from mac3d.lib import KC
+            for name in KC:
+                ms, n = s.profile_read(name)
+                if n:
+                    prof[name] = (ms, n)
+            s.profile(False)
+        return st, wall, prof
+
+
+def cpu_baseline(workload, steps=1):
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle
+    threads = oracle.set_threads(int(os.environ.get("OMP_NUM_THREADS", "0") or 0))
+    o = oracle.Oracle(workload.nx, workload.ny, workload.nz, workload.dx, levels=workload.levels,
+                      nu1=workload.nu1, nu2=workload.nu2, cycles=workload.cycles)
+    o.init_blobs(workload.blobs)
+    t0 = time.perf_counter()
+    o.step(workload.dt, steps)
+    dt = time.perf_counter() - t0
+    o.close()
+    return steps / dt, threads, dt
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=40)
+    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--seed", type=int, default=1)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-mixed", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}): the MAC step is "
+              "built and measured per GPU; multi-GPU slab decomposition is design-only.",
+              file=sys.stderr)
+        sys.exit(2)
+
+    from mac3d import scenarios
+    w32 = scenarios.workload("ref", args.seed)
+    log(f"[bench] fp32 {w32.nx}x{w32.ny}x{w32.nz} steps={args.steps} warmup={args.warmup}")
+    st32, wall32, prof = run_gpu(w32, args.steps, args.warmup)
+    sps32 = args.steps / wall32
+    log(f"[bench] fp32 {sps32:.2f} steps/s (kernel span {st32['ms']:.1f} ms) "
+        f"max_div {st32['max_div']:.3e}")
+
+    mixed = None
+    if not args.no_mixed:
+        wm = scenarios.workload("mixed", args.seed)
+        log("[bench] mixed")
+        stm, wallm, profm = run_gpu(wm, args.steps, args.warmup, profile_steps=0)
+        spsm = args.steps / wallm
+        mixed = {"steps_per_s": round(spsm, 3), "speedup_vs_fp32": round(spsm / sps32, 3),
+                 "vcycles_per_step": round(stm["vcycles"] / args.steps, 3),
+                 "max_div": stm["max_div"], "rel_res": stm["rel_res"],
+                 "converged": bool(stm["converged"])}
+        log(f"[bench] mixed {spsm:.2f} steps/s, {mixed['vcycles_per_step']} cycles/step")
+
+    roof = None
+    if "SMOOTH_L0" in prof:
+        ms, n = prof["SMOOTH_L0"]
+        avg_s = ms / n / 1e3
+        bytes_launch = SMOOTH_BYTES_PER_CELL * w32.cells
+        ach = bytes_launch / avg_s / 1e9
+        roof = {"bound": "hbm", "achieved": round(ach, 1), "peak": HBM_PEAK_GBPS, "unit": "GB/s",
+                "frac": round(ach / HBM_PEAK_GBPS, 4), "traffic": None,
+                "kernel": "k_rbgs (level-0 red-black colour pass)",
+                "avg_launch_us": round(avg_s * 1e6, 2), "launches": n,
+                "bytes_per_launch": bytes_launch}
+        traffic_file = os.path.join(ROOT, "profiles", "rbgs_l0_192x256x192_traffic.json")
+        if os.path.exists(traffic_file):
+            with open(traffic_file) as f:
+                roof["traffic"] = json.load(f).get("bytes_per_launch")
+    kernel_ms = {k: round(v[0] / v[1], 4) for k, v in prof.items()}
+
+    cpu = None
+    if not args.no_cpu:
+        log("[bench] cpu baseline (FP64 oracle, 1 step at full size)")
+        cps, threads, secs = cpu_baseline(w32, 1)
+        cpu = {"value": round(cps, 5), "unit": "steps/s", "cores": threads, "kind": "port",
+               "sample": f"1 step (8 V-cycles) of the 192x256x192 workload, FP64 C oracle, "
+                         f"OpenMP {threads} threads, {secs:.1f} s"}
+
+    line = {
+        "metric": "MAC steps/sec @192x256x192 (mixed vs FP32, div<=1e-6); HBM GB/s % peak",
+        "value": round(sps32, 3),
+        "unit": "steps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(1e3 * wall32 / args.steps, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "fp32",
+        "data": "synthetic (seeded blobs)",
+        "config": {"workload": "192x256x192 MAC step, 3 seeded blobs, FP32, 3-level V-cycle "
+                               "2+2 RBGS x 8 cycles", "nx": w32.nx, "ny": w32.ny, "nz": w32.nz,
+                   "dx": w32.dx, "dt": w32.dt, "seed": args.seed, "parallelism": "single GPU"},
+        "max_div": st32["max_div"],
+        "mixed": mixed,
+        "roofline": roof,
+        "kernel_avg_ms": kernel_ms,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

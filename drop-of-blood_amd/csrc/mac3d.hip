@@ -1,0 +1,966 @@
+// mac3d.hip — context, step orchestration and C ABI of the MAC step.
+//
+// The context owns all device buffers and one HIP stream (default priority).
+// Host code only sequences kernel launches on that stream; every field
+// operation runs in a hand-written kernel of step_kernels.hpp,
+// mg_kernels.hpp or mixed_kernels.hpp.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <new>
+#include <string>
+#include <type_traits>
+#include <vector>
+
+#include "../../include/mac3d.h"
+#include "common.hpp"
+#include "mg_kernels.hpp"
+#include "mixed_kernels.hpp"
+#include "step_kernels.hpp"
+#include "profile.hpp"
+
+using namespace mac3d;
+
+namespace {
+
+enum { TF16 = 0, TF32 = 1, TF64 = 2 };
+const int kMaxLev = 8;
+
+size_t type_size(int t) { return t == TF16 ? 2 : (t == TF32 ? 4 : 8); }
+
+struct Buf {
+  void *ptr = nullptr;
+  int type = TF32;
+  size_t n = 0;
+};
+
+template <class F> void with_type(int t, F &&f) {
+  switch (t) {
+    case TF16: f((half_t *)nullptr); break;
+    case TF32: f((float *)nullptr); break;
+    default: f((double *)nullptr); break;
+  }
+}
+
+// step fields are FP32 or FP64 only
+template <class F> void with_step_type(int t, F &&f) {
+  if (t == TF64) f((double *)nullptr); else f((float *)nullptr);
+}
+
+struct Ctx {
+  mac3d_config cfg;
+  int L = 0;
+  int prec = 0;
+  int t0 = TF32;  // type of the level-0 step fields
+  Dims d[kMaxLev + 1];
+  size_t n[kMaxLev + 1];
+  hipStream_t stream = nullptr;
+  std::string err;
+  // level-0 step fields
+  Buf u, v, w, u2, v2, w2, p, dye, dye2, dpred, cu, cv, cw, rhs;
+  // multigrid: x, b, r per level (level 0 x/b alias p/rhs except in mixed)
+  Buf mgx[kMaxLev + 1], mgb[kMaxLev + 1], mgr[kMaxLev + 1];
+  // mixed-mode FP32 L0 correction and its right-hand side
+  Buf e0, r0;
+  // coarse solve
+  double *cq = nullptr;  // qx | qy | qz | lx | ly | lz
+  size_t cq_off[6] = {0, 0, 0, 0, 0, 0};
+  double *cwork = nullptr;
+  // reductions
+  double *part = nullptr;
+  size_t part_n = 0;
+  double *red = nullptr;     // device fold output (3 doubles)
+  double *red_host = nullptr; // pinned
+  // staging for upload/download
+  double *stage = nullptr;
+  mac3d::BlobDev *blobs = nullptr;
+  int blobs_cap = 0;
+  Profiler prof;
+  bool dead = false;
+};
+
+#define HIPCHK(call)                                                     \
+  do {                                                                   \
+    hipError_t e_ = (call);                                              \
+    if (e_ != hipSuccess) {                                              \
+      ctx->err = std::string(#call) + ": " + hipGetErrorString(e_);      \
+      return MAC3D_EHIP;                                                 \
+    }                                                                    \
+  } while (0)
+
+#define LAUNCHCHK()                                                      \
+  do {                                                                   \
+    hipError_t e_ = hipGetLastError();                                   \
+    if (e_ != hipSuccess) {                                              \
+      ctx->err = std::string("kernel launch: ") + hipGetErrorString(e_); \
+      return MAC3D_EHIP;                                                 \
+    }                                                                    \
+  } while (0)
+
+int fail(Ctx *ctx, int code, const std::string &msg) {
+  ctx->err = msg;
+  return code;
+}
+
+dim3 cell_grid(const Dims &d) { return dim3((d.nx + 63) / 64, (d.ny + 3) / 4, d.nz); }
+dim3 half_grid(const Dims &d) { return dim3(((d.nx + 1) / 2 + 63) / 64, (d.ny + 3) / 4, d.nz); }
+const dim3 kCellBlock(64, 4, 1);
+
+double top_coeff(int level) {
+  double s = (double)(1 << level);
+  return 2.0 * s / (s + 1.0);
+}
+double level_h2(int level) {
+  double s = (double)(1 << level);
+  return s * s;
+}
+
+int alloc(Ctx *ctx, Buf &b, int type, size_t n) {
+  b.type = type;
+  b.n = n;
+  hipError_t e = hipMalloc(&b.ptr, type_size(type) * n);
+  if (e != hipSuccess) {
+    b.ptr = nullptr;
+    return fail(ctx, MAC3D_ENOMEM, std::string("hipMalloc: ") + hipGetErrorString(e));
+  }
+  e = hipMemsetAsync(b.ptr, 0, type_size(type) * n, ctx->stream);
+  if (e != hipSuccess) return fail(ctx, MAC3D_EHIP, hipGetErrorString(e));
+  return MAC3D_OK;
+}
+
+void release(Buf &b) {
+  if (b.ptr) (void)hipFree(b.ptr);
+  b.ptr = nullptr;
+}
+
+// Symmetric tridiagonal 1-D operator of one axis and its eigen-pairs by
+// cyclic Jacobi rotations (n <= a few hundred, host FP64, once per context).
+void axis_eigen(int n, bool lid, double topc, std::vector<double> &Q, std::vector<double> &lam) {
+  std::vector<double> A(n * n, 0.0);
+  for (int i = 0; i < n; ++i) {
+    double dg = (i > 0 ? 1.0 : 0.0);
+    if (i < n - 1) dg += 1.0; else if (lid) dg += topc;
+    A[i * n + i] = dg;
+    if (i > 0) A[i * n + i - 1] = A[(i - 1) * n + i] = -1.0;
+  }
+  Q.assign(n * n, 0.0);
+  for (int i = 0; i < n; ++i) Q[i * n + i] = 1.0;
+  for (int sweep = 0; sweep < 100; ++sweep) {
+    double off = 0.0;
+    for (int p = 0; p < n; ++p)
+      for (int q = p + 1; q < n; ++q) off += A[p * n + q] * A[p * n + q];
+    if (off < 1e-30) break;
+    for (int p = 0; p < n; ++p)
+      for (int q = p + 1; q < n; ++q) {
+        double apq = A[p * n + q];
+        if (std::fabs(apq) < 1e-300) continue;
+        double theta = (A[q * n + q] - A[p * n + p]) / (2.0 * apq);
+        double t = (theta >= 0 ? 1.0 : -1.0) / (std::fabs(theta) + std::sqrt(theta * theta + 1.0));
+        double c = 1.0 / std::sqrt(t * t + 1.0), s = t * c;
+        for (int r = 0; r < n; ++r) {  // A <- A J
+          double arp = A[r * n + p], arq = A[r * n + q];
+          A[r * n + p] = c * arp - s * arq;
+          A[r * n + q] = s * arp + c * arq;
+        }
+        for (int r = 0; r < n; ++r) {  // A <- J^T A
+          double apr = A[p * n + r], aqr = A[q * n + r];
+          A[p * n + r] = c * apr - s * aqr;
+          A[q * n + r] = s * apr + c * aqr;
+        }
+        for (int r = 0; r < n; ++r) {  // Q <- Q J
+          double qrp = Q[r * n + p], qrq = Q[r * n + q];
+          Q[r * n + p] = c * qrp - s * qrq;
+          Q[r * n + q] = s * qrp + c * qrq;
+        }
+      }
+  }
+  lam.resize(n);
+  for (int i = 0; i < n; ++i) lam[i] = A[i * n + i];
+}
+
+// ---------------------------------------------------------------------------
+// kernel launchers
+int launch_face_to_centre(Ctx *ctx) {
+  const Dims &d = ctx->d[0];
+  ProfScope ps(ctx->prof, ctx->stream, KC_F2C);
+  with_step_type(ctx->t0, [&](auto tag) {
+    using T = std::remove_pointer_t<decltype(tag)>;
+    hipLaunchKernelGGL(k_face_to_centre<T>, cell_grid(d), kCellBlock, 0, ctx->stream, d,
+                       (const T *)ctx->u.ptr, (const T *)ctx->v.ptr, (const T *)ctx->w.ptr,
+                       (T *)ctx->cu.ptr, (T *)ctx->cv.ptr, (T *)ctx->cw.ptr);
+  });
+  LAUNCHCHK();
+  return MAC3D_OK;
+}
+
+StepParams step_params(const Ctx *ctx, double dt) {
+  const mac3d_config &c = ctx->cfg;
+  StepParams P;
+  P.dt = dt; P.dx = c.dx; P.rho_w = c.rho_w; P.mu_w = c.mu_w; P.rho_b = c.rho_b;
+  P.tau_y = c.tau_y; P.mu_inf = c.mu_inf; P.g = c.g; P.gamma_reg = c.gamma_reg;
+  P.shear_eps2 = c.shear_eps2; P.alpha_max = c.alpha_max; P.vel_damp = c.vel_damp;
+  P.maccormack = c.maccormack;
+  return P;
+}
+
+void swap_vel(Ctx *ctx) {
+  std::swap(ctx->u, ctx->u2);
+  std::swap(ctx->v, ctx->v2);
+  std::swap(ctx->w, ctx->w2);
+}
+
+int launch_advect(Ctx *ctx, double dt) {
+  const Dims &d = ctx->d[0];
+  double dom[3] = {d.nx * ctx->cfg.dx, d.ny * ctx->cfg.dx, d.nz * ctx->cfg.dx};
+  {
+    ProfScope ps(ctx->prof, ctx->stream, KC_ADVECT);
+    with_step_type(ctx->t0, [&](auto tag) {
+      using T = std::remove_pointer_t<decltype(tag)>;
+      hipLaunchKernelGGL(k_advect_mac<T>, cell_grid(d), kCellBlock, 0, ctx->stream, d,
+                         (T)(dt / dom[0]), (T)(dt / dom[1]), (T)(dt / dom[2]),
+                         (T)ctx->cfg.vel_damp, (const T *)ctx->u.ptr, (const T *)ctx->v.ptr,
+                         (const T *)ctx->w.ptr, (const T *)ctx->cu.ptr, (const T *)ctx->cv.ptr,
+                         (const T *)ctx->cw.ptr, (T *)ctx->u2.ptr, (T *)ctx->v2.ptr,
+                         (T *)ctx->w2.ptr);
+    });
+  }
+  LAUNCHCHK();
+  swap_vel(ctx);
+  return MAC3D_OK;
+}
+
+int launch_casson(Ctx *ctx, double dt) {
+  const Dims &d = ctx->d[0];
+  StepParams P = step_params(ctx, dt);
+  {
+    ProfScope ps(ctx->prof, ctx->stream, KC_CASSON);
+    with_step_type(ctx->t0, [&](auto tag) {
+      using T = std::remove_pointer_t<decltype(tag)>;
+      hipLaunchKernelGGL(k_casson<T>, cell_grid(d), kCellBlock, 0, ctx->stream, d, P,
+                         (const T *)ctx->u.ptr, (const T *)ctx->v.ptr, (const T *)ctx->w.ptr,
+                         (const T *)ctx->dye.ptr, (T *)ctx->u2.ptr, (T *)ctx->v2.ptr,
+                         (T *)ctx->w2.ptr);
+    });
+  }
+  LAUNCHCHK();
+  swap_vel(ctx);
+  return MAC3D_OK;
+}
+
+int launch_divergence(Ctx *ctx) {
+  const Dims &d = ctx->d[0];
+  ProfScope ps(ctx->prof, ctx->stream, KC_DIV);
+  with_step_type(ctx->t0, [&](auto tag) {
+    using T = std::remove_pointer_t<decltype(tag)>;
+    with_step_type(ctx->rhs.type, [&](auto rtag) {
+      using R = std::remove_pointer_t<decltype(rtag)>;
+      hipLaunchKernelGGL((k_divergence<T, R>), cell_grid(d), kCellBlock, 0, ctx->stream, d,
+                         (const T *)ctx->u.ptr, (const T *)ctx->v.ptr, (const T *)ctx->w.ptr,
+                         (R *)ctx->rhs.ptr);
+    });
+  });
+  LAUNCHCHK();
+  return MAC3D_OK;
+}
+
+int launch_project(Ctx *ctx) {
+  const Dims &d = ctx->d[0];
+  ProfScope ps(ctx->prof, ctx->stream, KC_PROJECT);
+  with_step_type(ctx->t0, [&](auto tag) {
+    using T = std::remove_pointer_t<decltype(tag)>;
+    with_step_type(ctx->p.type, [&](auto ptag) {
+      using Pt = std::remove_pointer_t<decltype(ptag)>;
+      hipLaunchKernelGGL((k_project<T, Pt>), cell_grid(d), kCellBlock, 0, ctx->stream, d,
+                         (T *)ctx->u.ptr, (T *)ctx->v.ptr, (T *)ctx->w.ptr,
+                         (const Pt *)ctx->p.ptr);
+    });
+  });
+  LAUNCHCHK();
+  return MAC3D_OK;
+}
+
+int launch_dye_predict(Ctx *ctx, double dt) {
+  const Dims &d = ctx->d[0];
+  double dom[3] = {d.nx * ctx->cfg.dx, d.ny * ctx->cfg.dx, d.nz * ctx->cfg.dx};
+  ProfScope ps(ctx->prof, ctx->stream, KC_DYE);
+  with_step_type(ctx->t0, [&](auto tag) {
+    using T = std::remove_pointer_t<decltype(tag)>;
+    hipLaunchKernelGGL(k_dye_predict<T>, cell_grid(d), kCellBlock, 0, ctx->stream, d,
+                       (T)(dt / dom[0]), (T)(dt / dom[1]), (T)(dt / dom[2]),
+                       (const T *)ctx->cu.ptr, (const T *)ctx->cv.ptr, (const T *)ctx->cw.ptr,
+                       (const T *)ctx->dye.ptr, (T *)ctx->dpred.ptr);
+  });
+  LAUNCHCHK();
+  return MAC3D_OK;
+}
+
+int launch_dye_resolve(Ctx *ctx, double dt) {
+  const Dims &d = ctx->d[0];
+  double dom[3] = {d.nx * ctx->cfg.dx, d.ny * ctx->cfg.dx, d.nz * ctx->cfg.dx};
+  {
+    ProfScope ps(ctx->prof, ctx->stream, KC_DYE);
+    with_step_type(ctx->t0, [&](auto tag) {
+      using T = std::remove_pointer_t<decltype(tag)>;
+      hipLaunchKernelGGL(k_dye_resolve<T>, cell_grid(d), kCellBlock, 0, ctx->stream, d,
+                         (T)(dt / dom[0]), (T)(dt / dom[1]), (T)(dt / dom[2]),
+                         (T)ctx->cfg.maccormack, (const T *)ctx->cu.ptr, (const T *)ctx->cv.ptr,
+                         (const T *)ctx->cw.ptr, (const T *)ctx->dye.ptr,
+                         (const T *)ctx->dpred.ptr, (T *)ctx->dye2.ptr);
+    });
+  }
+  LAUNCHCHK();
+  std::swap(ctx->dye, ctx->dye2);
+  return MAC3D_OK;
+}
+
+// ---- multigrid
+int mg_pass(Ctx *ctx, int l, int parity, bool zero, double bscale) {
+  const Dims &d = ctx->d[l];
+  ProfScope ps(ctx->prof, ctx->stream, l == 0 ? KC_SMOOTH0 : KC_MG);
+  with_type(ctx->mgx[l].type, [&](auto tag) {
+    using S = std::remove_pointer_t<decltype(tag)>;
+    using C = typename compute_of<S>::type;
+    if (zero)
+      hipLaunchKernelGGL((k_rbgs<S, true>), half_grid(d), kCellBlock, 0, ctx->stream, d,
+                         (S *)ctx->mgx[l].ptr, (const S *)ctx->mgb[l].ptr, (C)level_h2(l),
+                         (C)(l == 0 ? 1.0 : top_coeff(l)), (C)bscale, parity);
+    else
+      hipLaunchKernelGGL((k_rbgs<S, false>), half_grid(d), kCellBlock, 0, ctx->stream, d,
+                         (S *)ctx->mgx[l].ptr, (const S *)ctx->mgb[l].ptr, (C)level_h2(l),
+                         (C)(l == 0 ? 1.0 : top_coeff(l)), (C)bscale, parity);
+  });
+  LAUNCHCHK();
+  return MAC3D_OK;
+}
+
+int mg_fill_zero(Ctx *ctx, Buf &b) {
+  size_t n = b.n;
+  with_type(b.type, [&](auto tag) {
+    using S = std::remove_pointer_t<decltype(tag)>;
+    hipLaunchKernelGGL(k_fill<S>, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, ctx->stream,
+                       (S *)b.ptr, n, (S)0);
+  });
+  LAUNCHCHK();
+  return MAC3D_OK;
+}
+
+int mg_smooth(Ctx *ctx, int l, int sweeps, bool zero, double bscale) {
+  int rc;
+  if (zero && sweeps == 0) return mg_fill_zero(ctx, ctx->mgx[l]);
+  for (int s = 0; s < sweeps; ++s) {
+    if ((rc = mg_pass(ctx, l, 0, zero && s == 0, bscale))) return rc;
+    if ((rc = mg_pass(ctx, l, 1, false, bscale))) return rc;
+  }
+  return MAC3D_OK;
+}
+
+int mg_residual(Ctx *ctx, int l, double bscale) {
+  const Dims &d = ctx->d[l];
+  ProfScope ps(ctx->prof, ctx->stream, l == 0 ? KC_RESID0 : KC_MG);
+  with_type(ctx->mgx[l].type, [&](auto tag) {
+    using S = std::remove_pointer_t<decltype(tag)>;
+    using C = typename compute_of<S>::type;
+    hipLaunchKernelGGL(k_residual<S>, cell_grid(d), kCellBlock, 0, ctx->stream, d,
+                       (const S *)ctx->mgx[l].ptr, (const S *)ctx->mgb[l].ptr,
+                       (S *)ctx->mgr[l].ptr, (C)level_h2(l),
+                       (C)(l == 0 ? 1.0 : top_coeff(l)), (C)bscale);
+  });
+  LAUNCHCHK();
+  return MAC3D_OK;
+}
+
+int mg_restrict(Ctx *ctx, int l) {
+  const Dims &df = ctx->d[l], &dc = ctx->d[l + 1];
+  ProfScope ps(ctx->prof, ctx->stream, l == 0 ? KC_RESID0 : KC_MG);
+  with_type(ctx->mgr[l].type, [&](auto ftag) {
+    using Sf = std::remove_pointer_t<decltype(ftag)>;
+    with_type(ctx->mgb[l + 1].type, [&](auto ctag) {
+      using Sc = std::remove_pointer_t<decltype(ctag)>;
+      hipLaunchKernelGGL((k_restrict<Sf, Sc>), cell_grid(dc), kCellBlock, 0, ctx->stream, df, dc,
+                         (const Sf *)ctx->mgr[l].ptr, (Sc *)ctx->mgb[l + 1].ptr);
+    });
+  });
+  LAUNCHCHK();
+  return MAC3D_OK;
+}
+
+int mg_prolong(Ctx *ctx, int l) {
+  const Dims &df = ctx->d[l], &dc = ctx->d[l + 1];
+  ProfScope ps(ctx->prof, ctx->stream, l == 0 ? KC_PROLONG0 : KC_MG);
+  with_type(ctx->mgx[l + 1].type, [&](auto ctag) {
+    using Sc = std::remove_pointer_t<decltype(ctag)>;
+    with_type(ctx->mgx[l].type, [&](auto ftag) {
+      using Sf = std::remove_pointer_t<decltype(ftag)>;
+      hipLaunchKernelGGL((k_prolong_add<Sc, Sf>), cell_grid(df), kCellBlock, 0, ctx->stream, df,
+                         dc, (const Sc *)ctx->mgx[l + 1].ptr, (Sf *)ctx->mgx[l].ptr);
+    });
+  });
+  LAUNCHCHK();
+  return MAC3D_OK;
+}
+
+int mg_coarse(Ctx *ctx) {
+  int L = ctx->L;
+  const Dims &d = ctx->d[L];
+  const double *qx = ctx->cq + ctx->cq_off[0], *qy = ctx->cq + ctx->cq_off[1],
+               *qz = ctx->cq + ctx->cq_off[2], *lx = ctx->cq + ctx->cq_off[3],
+               *ly = ctx->cq + ctx->cq_off[4], *lz = ctx->cq + ctx->cq_off[5];
+  double h2 = level_h2(L);
+  ProfScope ps(ctx->prof, ctx->stream, KC_COARSE);
+  bool c64 = ctx->prec != MAC3D_PREC_FP32;
+  with_type(ctx->mgx[L].type, [&](auto tag) {
+    using S = std::remove_pointer_t<decltype(tag)>;
+    if (c64) {
+      hipLaunchKernelGGL((k_coarse_xz_fwd<S, double>), dim3(d.ny), dim3(256), 0, ctx->stream, d,
+                         (const S *)ctx->mgb[L].ptr, qx, qz, ctx->cwork);
+      hipLaunchKernelGGL((k_coarse_y<double>), dim3(d.nz), dim3(256), 0, ctx->stream, d, qy, lx,
+                         ly, lz, h2, ctx->cwork);
+      hipLaunchKernelGGL((k_coarse_xz_inv<S, double>), dim3(d.ny), dim3(256), 0, ctx->stream, d,
+                         (const double *)ctx->cwork, qx, qz, (S *)ctx->mgx[L].ptr);
+    } else {
+      hipLaunchKernelGGL((k_coarse_xz_fwd<S, float>), dim3(d.ny), dim3(256), 0, ctx->stream, d,
+                         (const S *)ctx->mgb[L].ptr, qx, qz, ctx->cwork);
+      hipLaunchKernelGGL((k_coarse_y<float>), dim3(d.nz), dim3(256), 0, ctx->stream, d, qy, lx,
+                         ly, lz, h2, ctx->cwork);
+      hipLaunchKernelGGL((k_coarse_xz_inv<S, float>), dim3(d.ny), dim3(256), 0, ctx->stream, d,
+                         (const double *)ctx->cwork, qx, qz, (S *)ctx->mgx[L].ptr);
+    }
+  });
+  LAUNCHCHK();
+  return MAC3D_OK;
+}
+
+// One V-cycle on (mgx[0], mgb[0]) with the schedule of
+// BloodDropEngine.ts:1942-2059: smooth / residual / restrict down to the
+// coarsest level, exact coarse solve, prolong / smooth back up.  Levels >= 1
+// start from a zero guess.  zero0: level 0 also starts from zero.
+int vcycle(Ctx *ctx, bool zero0, double bscale0) {
+  int rc, L = ctx->L;
+  int nu1 = ctx->cfg.nu1, nu2 = ctx->cfg.nu2;
+  if ((rc = mg_smooth(ctx, 0, nu1, zero0, bscale0))) return rc;
+  if ((rc = mg_residual(ctx, 0, bscale0))) return rc;
+  if ((rc = mg_restrict(ctx, 0))) return rc;
+  for (int l = 1; l < L; ++l) {
+    if ((rc = mg_smooth(ctx, l, nu1, true, 1.0))) return rc;
+    if ((rc = mg_residual(ctx, l, 1.0))) return rc;
+    if ((rc = mg_restrict(ctx, l))) return rc;
+  }
+  if ((rc = mg_coarse(ctx))) return rc;
+  for (int l = L - 1; l >= 1; --l) {
+    if ((rc = mg_prolong(ctx, l))) return rc;
+    if ((rc = mg_smooth(ctx, l, nu2, false, 1.0))) return rc;
+  }
+  if ((rc = mg_prolong(ctx, 0))) return rc;
+  if ((rc = mg_smooth(ctx, 0, nu2, false, bscale0))) return rc;
+  return MAC3D_OK;
+}
+
+size_t cell_blocks(const Dims &d) {
+  dim3 g = cell_grid(d);
+  return (size_t)g.x * g.y * g.z;
+}
+
+// Mixed-mode residual in FP64 with its norms, read back to the host.
+int residual64(Ctx *ctx, double *maxr, double *rr, double *bb) {
+  const Dims &d = ctx->d[0];
+  {
+    ProfScope ps(ctx->prof, ctx->stream, KC_RESID64);
+    hipLaunchKernelGGL(k_residual64, cell_grid(d), kCellBlock, 0, ctx->stream, d,
+                       (const double *)ctx->p.ptr, (const double *)ctx->rhs.ptr,
+                       (float *)ctx->r0.ptr, ctx->part);
+    hipLaunchKernelGGL(k_fold3, dim3(1), dim3(256), 0, ctx->stream, (const double *)ctx->part,
+                       cell_blocks(d), ctx->red);
+  }
+  LAUNCHCHK();
+  HIPCHK(hipMemcpyAsync(ctx->red_host, ctx->red, 3 * sizeof(double), hipMemcpyDeviceToHost,
+                        ctx->stream));
+  HIPCHK(hipStreamSynchronize(ctx->stream));
+  *maxr = ctx->red_host[0];
+  *rr = ctx->red_host[1];
+  *bb = ctx->red_host[2];
+  return MAC3D_OK;
+}
+
+int update64(Ctx *ctx, double scale) {
+  size_t n = ctx->n[0];
+  ProfScope ps(ctx->prof, ctx->stream, KC_RESID64);
+  hipLaunchKernelGGL(k_update64, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, ctx->stream, n,
+                     (double *)ctx->p.ptr, (const float *)ctx->e0.ptr, scale);
+  LAUNCHCHK();
+  return MAC3D_OK;
+}
+
+int div_norms(Ctx *ctx, double *mx, double *l2) {
+  const Dims &d = ctx->d[0];
+  with_step_type(ctx->t0, [&](auto tag) {
+    using T = std::remove_pointer_t<decltype(tag)>;
+    hipLaunchKernelGGL(k_div_partials<T>, cell_grid(d), kCellBlock, 0, ctx->stream, d,
+                       (const T *)ctx->u.ptr, (const T *)ctx->v.ptr, (const T *)ctx->w.ptr,
+                       ctx->part);
+  });
+  hipLaunchKernelGGL(k_fold_partials, dim3(1), dim3(256), 0, ctx->stream,
+                     (const double *)ctx->part, cell_blocks(d), ctx->red);
+  LAUNCHCHK();
+  HIPCHK(hipMemcpyAsync(ctx->red_host, ctx->red, 2 * sizeof(double), hipMemcpyDeviceToHost,
+                        ctx->stream));
+  HIPCHK(hipStreamSynchronize(ctx->stream));
+  *mx = ctx->red_host[0];
+  *l2 = std::sqrt(ctx->red_host[1] / (double)ctx->n[0]);
+  return MAC3D_OK;
+}
+
+// Pressure solve of one step.  FP32/FP64: a fixed number of V-cycles on
+// (P, RHS) warm-started from the previous step's P.  Mixed: FP64 iterative
+// refinement around FP32 V-cycles until ||r||/||b|| <= ir_tol.
+int pressure_solve(Ctx *ctx, int *cycles, int *iters, double *rel, int *conv) {
+  int rc;
+  if (ctx->prec != MAC3D_PREC_MIXED) {
+    for (int c = 0; c < ctx->cfg.cycles; ++c)
+      if ((rc = vcycle(ctx, false, 1.0))) return rc;
+    *cycles += ctx->cfg.cycles;
+    *iters = ctx->cfg.cycles;
+    *rel = -1.0;
+    *conv = 1;
+    return MAC3D_OK;
+  }
+  double mr, rr, bb;
+  int it = 0;
+  *conv = 0;
+  for (;;) {
+    if ((rc = residual64(ctx, &mr, &rr, &bb))) return rc;
+    double r = bb > 0.0 ? std::sqrt(rr / bb) : 0.0;
+    *rel = r;
+    if (r <= ctx->cfg.ir_tol || mr == 0.0) { *conv = 1; break; }
+    if (it >= ctx->cfg.ir_max_iters) break;
+    if ((rc = vcycle(ctx, true, 1.0 / mr))) return rc;
+    if ((rc = update64(ctx, mr))) return rc;
+    ++it;
+  }
+  *cycles += it;
+  *iters = it;
+  return MAC3D_OK;
+}
+
+int one_step(Ctx *ctx, double dt, int *cycles, int *iters, double *rel, int *conv) {
+  int rc;
+  if ((rc = launch_face_to_centre(ctx))) return rc;
+  if ((rc = launch_advect(ctx, dt))) return rc;
+  if ((rc = launch_casson(ctx, dt))) return rc;
+  if ((rc = launch_divergence(ctx))) return rc;
+  if ((rc = pressure_solve(ctx, cycles, iters, rel, conv))) return rc;
+  if ((rc = launch_project(ctx))) return rc;
+  if ((rc = launch_face_to_centre(ctx))) return rc;
+  if ((rc = launch_dye_predict(ctx, dt))) return rc;
+  if ((rc = launch_dye_resolve(ctx, dt))) return rc;
+  return MAC3D_OK;
+}
+
+Buf *field_buf(Ctx *ctx, int field, int level) {
+  if (level < 0 || level > ctx->L) return nullptr;
+  bool l0 = level == 0;
+  switch (field) {
+    case MAC3D_FIELD_U: return l0 ? &ctx->u : nullptr;
+    case MAC3D_FIELD_V: return l0 ? &ctx->v : nullptr;
+    case MAC3D_FIELD_W: return l0 ? &ctx->w : nullptr;
+    case MAC3D_FIELD_P: return l0 ? &ctx->p : nullptr;
+    case MAC3D_FIELD_DYE: return l0 ? &ctx->dye : nullptr;
+    case MAC3D_FIELD_CU: return l0 ? &ctx->cu : nullptr;
+    case MAC3D_FIELD_CV: return l0 ? &ctx->cv : nullptr;
+    case MAC3D_FIELD_CW: return l0 ? &ctx->cw : nullptr;
+    case MAC3D_FIELD_RHS: return l0 ? &ctx->rhs : nullptr;
+    case MAC3D_FIELD_DYE_PRED: return l0 ? &ctx->dpred : nullptr;
+    case MAC3D_FIELD_MGX: return &ctx->mgx[level];
+    case MAC3D_FIELD_MGB: return &ctx->mgb[level];
+    case MAC3D_FIELD_MGR: return &ctx->mgr[level];
+  }
+  return nullptr;
+}
+
+int valid_config(const mac3d_config *c, std::string &why) {
+  if (c->nx <= 0 || c->ny <= 0 || c->nz <= 0) { why = "grid sizes must be positive"; return 0; }
+  if (c->levels < 1 || c->levels > kMaxLev) { why = "levels must be in [1, 8]"; return 0; }
+  int m = 1 << c->levels;
+  if (c->nx % m || c->ny % m || c->nz % m) {
+    why = "every grid size must divide by 2^levels";
+    return 0;
+  }
+  int cx = c->nx / m, cy = c->ny / m, cz = c->nz / m;
+  if (cx * cz > COARSE_MAX_PLANE || cx * cy > COARSE_MAX_PLANE) {
+    why = "coarsest level too large for the exact solve (nx*nz and nx*ny at the coarsest "
+          "level must be <= 2048): add levels";
+    return 0;
+  }
+  if (c->nu1 < 0 || c->nu2 < 0 || c->cycles < 0 || c->ir_max_iters < 0) {
+    why = "sweep and cycle counts must be >= 0";
+    return 0;
+  }
+  if (c->precision < 0 || c->precision > 2) { why = "unknown precision"; return 0; }
+  if (!(c->dx > 0) || !std::isfinite(c->dx)) { why = "dx must be finite and > 0"; return 0; }
+  if (!(c->rho_w > 0) || !(c->rho_b > 0)) { why = "densities must be > 0"; return 0; }
+  if ((size_t)c->nx * c->ny * c->nz > (size_t)1 << 31) { why = "grid too large"; return 0; }
+  return 1;
+}
+
+int create_impl(Ctx *ctx, const mac3d_config *cfg) {
+  int rc;
+  ctx->cfg = *cfg;
+  ctx->L = cfg->levels;
+  ctx->prec = cfg->precision;
+  ctx->t0 = ctx->prec == MAC3D_PREC_FP64 ? TF64 : TF32;
+  HIPCHK(hipSetDevice(cfg->device));
+  HIPCHK(hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking));
+  for (int l = 0; l <= ctx->L; ++l) {
+    ctx->d[l] = Dims{cfg->nx >> l, cfg->ny >> l, cfg->nz >> l};
+    ctx->n[l] = ctx->d[l].cells();
+  }
+  size_t n0 = ctx->n[0];
+  int t0 = ctx->t0;
+  Buf *f0[] = {&ctx->u, &ctx->v, &ctx->w, &ctx->u2, &ctx->v2, &ctx->w2, &ctx->dye,
+               &ctx->dye2, &ctx->dpred, &ctx->cu, &ctx->cv, &ctx->cw};
+  for (Buf *b : f0)
+    if ((rc = alloc(ctx, *b, t0, n0))) return rc;
+  bool mixed = ctx->prec == MAC3D_PREC_MIXED;
+  if ((rc = alloc(ctx, ctx->p, mixed ? TF64 : t0, n0))) return rc;
+  if ((rc = alloc(ctx, ctx->rhs, mixed ? TF64 : t0, n0))) return rc;
+  if (mixed) {
+    if ((rc = alloc(ctx, ctx->e0, TF32, n0))) return rc;
+    if ((rc = alloc(ctx, ctx->r0, TF32, n0))) return rc;
+    ctx->mgx[0] = ctx->e0;
+    ctx->mgb[0] = ctx->r0;
+  } else {
+    ctx->mgx[0] = ctx->p;
+    ctx->mgb[0] = ctx->rhs;
+  }
+  if ((rc = alloc(ctx, ctx->mgr[0], ctx->mgx[0].type, n0))) return rc;
+  for (int l = 1; l <= ctx->L; ++l) {
+    int t = mixed ? (l >= 2 ? TF16 : TF32) : t0;
+    if ((rc = alloc(ctx, ctx->mgx[l], t, ctx->n[l]))) return rc;
+    if ((rc = alloc(ctx, ctx->mgb[l], t, ctx->n[l]))) return rc;
+    if ((rc = alloc(ctx, ctx->mgr[l], t, ctx->n[l]))) return rc;
+  }
+  // coarse eigen-decomposition
+  const Dims &dc = ctx->d[ctx->L];
+  std::vector<double> qx, qy, qz, lx, ly, lz;
+  axis_eigen(dc.nx, false, 0.0, qx, lx);
+  axis_eigen(dc.ny, true, ctx->L == 0 ? 1.0 : top_coeff(ctx->L), qy, ly);
+  axis_eigen(dc.nz, false, 0.0, qz, lz);
+  std::vector<double> all;
+  std::vector<double> *parts[6] = {&qx, &qy, &qz, &lx, &ly, &lz};
+  for (int a = 0; a < 6; ++a) {
+    ctx->cq_off[a] = all.size();
+    all.insert(all.end(), parts[a]->begin(), parts[a]->end());
+  }
+  HIPCHK(hipMalloc(&ctx->cq, all.size() * sizeof(double)));
+  HIPCHK(hipMemcpyAsync(ctx->cq, all.data(), all.size() * sizeof(double), hipMemcpyHostToDevice,
+                        ctx->stream));
+  HIPCHK(hipMalloc(&ctx->cwork, ctx->n[ctx->L] * sizeof(double)));
+  ctx->part_n = cell_blocks(ctx->d[0]);
+  HIPCHK(hipMalloc(&ctx->part, 3 * ctx->part_n * sizeof(double)));
+  HIPCHK(hipMalloc(&ctx->red, 4 * sizeof(double)));
+  HIPCHK(hipHostMalloc(&ctx->red_host, 4 * sizeof(double), hipHostMallocDefault));
+  HIPCHK(hipMalloc(&ctx->stage, n0 * sizeof(double)));
+  HIPCHK(hipStreamSynchronize(ctx->stream));
+  if ((rc = ctx->prof.init(ctx->err))) return rc;
+  return MAC3D_OK;
+}
+
+void destroy_impl(Ctx *ctx) {
+  if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
+  Buf *f0[] = {&ctx->u, &ctx->v, &ctx->w, &ctx->u2, &ctx->v2, &ctx->w2, &ctx->dye,
+               &ctx->dye2, &ctx->dpred, &ctx->cu, &ctx->cv, &ctx->cw, &ctx->p, &ctx->rhs,
+               &ctx->e0, &ctx->r0, &ctx->mgr[0]};
+  for (Buf *b : f0) release(*b);
+  for (int l = 1; l <= ctx->L; ++l) {
+    release(ctx->mgx[l]);
+    release(ctx->mgb[l]);
+    release(ctx->mgr[l]);
+  }
+  if (ctx->cq) (void)hipFree(ctx->cq);
+  if (ctx->cwork) (void)hipFree(ctx->cwork);
+  if (ctx->part) (void)hipFree(ctx->part);
+  if (ctx->red) (void)hipFree(ctx->red);
+  if (ctx->red_host) (void)hipHostFree(ctx->red_host);
+  if (ctx->stage) (void)hipFree(ctx->stage);
+  if (ctx->blobs) (void)hipFree(ctx->blobs);
+  ctx->prof.destroy();
+  if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
+  ctx->stream = nullptr;
+}
+
+thread_local std::string g_create_err;
+
+}  // namespace
+
+// ===========================================================================
+// C ABI
+extern "C" {
+
+int mac3d_default_config(mac3d_config *c) {
+  if (!c) return MAC3D_EINVAL;
+  std::memset(c, 0, sizeof(*c));
+  c->nx = 32; c->ny = 48; c->nz = 32;
+  c->levels = 3; c->nu1 = 2; c->nu2 = 2; c->cycles = 8;
+  c->precision = MAC3D_PREC_FP32;
+  c->ir_max_iters = 30;
+  c->device = 0;
+  c->dx = 1e-3;
+  c->rho_w = 1000.0; c->mu_w = 1e-3;
+  c->rho_b = 1060.0; c->tau_y = 0.005; c->mu_inf = 0.0035;
+  c->g = 9.81;
+  c->gamma_reg = 0.40;       // fluid.ts:461
+  c->shear_eps2 = 1e-6;      // fluid.ts:454
+  c->alpha_max = 1.0 / 6.0;  // FTCS stability bound of the 7-point update
+  c->vel_damp = 1.0;
+  c->maccormack = 0.45 * 0.995;  // advect.ts:202 with concSharpness 0.995
+  c->ir_tol = 1e-8;
+  return MAC3D_OK;
+}
+
+int mac3d_create(const mac3d_config *cfg, void **out) {
+  if (!cfg || !out) return MAC3D_EINVAL;
+  *out = nullptr;
+  std::string why;
+  if (!valid_config(cfg, why)) {
+    g_create_err = why;
+    return MAC3D_EINVAL;
+  }
+  Ctx *ctx = new (std::nothrow) Ctx();
+  if (!ctx) return MAC3D_ENOMEM;
+  int rc = create_impl(ctx, cfg);
+  if (rc) {
+    g_create_err = ctx->err;
+    destroy_impl(ctx);
+    delete ctx;
+    return rc;
+  }
+  *out = ctx;
+  return MAC3D_OK;
+}
+
+int mac3d_destroy(void *h) {
+  if (!h) return MAC3D_OK;
+  Ctx *ctx = (Ctx *)h;
+  destroy_impl(ctx);
+  delete ctx;
+  return MAC3D_OK;
+}
+
+int mac3d_upload(void *h, int field, int level, const void *host, int dtype, size_t count) {
+  if (!h) return MAC3D_EINVAL;
+  Ctx *ctx = (Ctx *)h;
+  Buf *b = field_buf(ctx, field, level);
+  if (!b || !host) return fail(ctx, MAC3D_EINVAL, "unknown field/level or null host pointer");
+  if (dtype != MAC3D_F32 && dtype != MAC3D_F64) return fail(ctx, MAC3D_EINVAL, "unknown dtype");
+  if (count != b->n) return fail(ctx, MAC3D_EINVAL, "element count does not match the field");
+  size_t hsz = dtype == MAC3D_F32 ? 4 : 8;
+  HIPCHK(hipMemcpyAsync(ctx->stage, host, hsz * count, hipMemcpyHostToDevice, ctx->stream));
+  with_type(b->type, [&](auto tag) {
+    using S = std::remove_pointer_t<decltype(tag)>;
+    dim3 g((unsigned)((count + 255) / 256));
+    if (dtype == MAC3D_F32)
+      hipLaunchKernelGGL((k_convert<float, S>), g, dim3(256), 0, ctx->stream,
+                         (const float *)ctx->stage, (S *)b->ptr, count);
+    else
+      hipLaunchKernelGGL((k_convert<double, S>), g, dim3(256), 0, ctx->stream,
+                         (const double *)ctx->stage, (S *)b->ptr, count);
+  });
+  LAUNCHCHK();
+  HIPCHK(hipStreamSynchronize(ctx->stream));
+  return MAC3D_OK;
+}
+
+int mac3d_download(void *h, int field, int level, void *host, int dtype, size_t count) {
+  if (!h) return MAC3D_EINVAL;
+  Ctx *ctx = (Ctx *)h;
+  Buf *b = field_buf(ctx, field, level);
+  if (!b || !host) return fail(ctx, MAC3D_EINVAL, "unknown field/level or null host pointer");
+  if (dtype != MAC3D_F32 && dtype != MAC3D_F64) return fail(ctx, MAC3D_EINVAL, "unknown dtype");
+  if (count != b->n) return fail(ctx, MAC3D_EINVAL, "element count does not match the field");
+  size_t hsz = dtype == MAC3D_F32 ? 4 : 8;
+  with_type(b->type, [&](auto tag) {
+    using S = std::remove_pointer_t<decltype(tag)>;
+    dim3 g((unsigned)((count + 255) / 256));
+    if (dtype == MAC3D_F32)
+      hipLaunchKernelGGL((k_convert<S, float>), g, dim3(256), 0, ctx->stream,
+                         (const S *)b->ptr, (float *)ctx->stage, count);
+    else
+      hipLaunchKernelGGL((k_convert<S, double>), g, dim3(256), 0, ctx->stream,
+                         (const S *)b->ptr, (double *)ctx->stage, count);
+  });
+  LAUNCHCHK();
+  HIPCHK(hipMemcpyAsync(host, ctx->stage, hsz * count, hipMemcpyDeviceToHost, ctx->stream));
+  HIPCHK(hipStreamSynchronize(ctx->stream));
+  return MAC3D_OK;
+}
+
+int mac3d_init_blobs(void *h, const mac3d_blob *blobs, int nblobs) {
+  if (!h) return MAC3D_EINVAL;
+  Ctx *ctx = (Ctx *)h;
+  if (nblobs < 0 || (nblobs > 0 && !blobs)) return fail(ctx, MAC3D_EINVAL, "bad blob list");
+  for (int q = 0; q < nblobs; ++q) {
+    const mac3d_blob &b = blobs[q];
+    if (!std::isfinite(b.cx) || !std::isfinite(b.cy) || !std::isfinite(b.cz) ||
+        !std::isfinite(b.w0) || !(b.radius > 0.5) || !std::isfinite(b.radius))
+      return fail(ctx, MAC3D_EINVAL, "blob fields must be finite with radius > 0.5");
+  }
+  if (nblobs > ctx->blobs_cap) {
+    if (ctx->blobs) HIPCHK(hipFree(ctx->blobs));
+    ctx->blobs = nullptr;
+    HIPCHK(hipMalloc(&ctx->blobs, sizeof(BlobDev) * nblobs));
+    ctx->blobs_cap = nblobs;
+  }
+  std::vector<BlobDev> hb(nblobs);
+  for (int q = 0; q < nblobs; ++q)
+    hb[q] = BlobDev{blobs[q].cx, blobs[q].cy, blobs[q].cz, blobs[q].radius, blobs[q].w0};
+  if (nblobs)
+    HIPCHK(hipMemcpyAsync(ctx->blobs, hb.data(), sizeof(BlobDev) * nblobs, hipMemcpyHostToDevice,
+                          ctx->stream));
+  size_t n0 = ctx->n[0];
+  Buf *zero[] = {&ctx->u, &ctx->w, &ctx->p, &ctx->e0, &ctx->cu, &ctx->cv, &ctx->cw};
+  for (Buf *b : zero)
+    if (b->ptr) HIPCHK(hipMemsetAsync(b->ptr, 0, type_size(b->type) * n0, ctx->stream));
+  const Dims &d = ctx->d[0];
+  with_step_type(ctx->t0, [&](auto tag) {
+    using T = std::remove_pointer_t<decltype(tag)>;
+    hipLaunchKernelGGL(k_init_blobs<T>, cell_grid(d), kCellBlock, 0, ctx->stream, d,
+                       (const BlobDev *)ctx->blobs, nblobs, (T *)ctx->dye.ptr, (T *)ctx->v.ptr);
+  });
+  LAUNCHCHK();
+  HIPCHK(hipStreamSynchronize(ctx->stream));
+  return MAC3D_OK;
+}
+
+int mac3d_step(void *h, double dt, int nsteps, mac3d_stats *stats) {
+  if (!h) return MAC3D_EINVAL;
+  Ctx *ctx = (Ctx *)h;
+  if (!std::isfinite(dt) || !(dt > 0.0)) return fail(ctx, MAC3D_EINVAL, "dt must be finite and > 0");
+  if (nsteps < 0) return fail(ctx, MAC3D_EINVAL, "nsteps must be >= 0");
+  hipEvent_t e0, e1;
+  HIPCHK(hipEventCreate(&e0));
+  HIPCHK(hipEventCreate(&e1));
+  HIPCHK(hipEventRecord(e0, ctx->stream));
+  int cycles = 0, iters = 0, conv = 1, rc;
+  double rel = -1.0;
+  for (int s = 0; s < nsteps; ++s) {
+    if ((rc = one_step(ctx, dt, &cycles, &iters, &rel, &conv))) {
+      (void)hipEventDestroy(e0);
+      (void)hipEventDestroy(e1);
+      return rc;
+    }
+    if (ctx->prof.pending() > 1024) {
+      HIPCHK(hipStreamSynchronize(ctx->stream));
+      ctx->prof.collect();
+    }
+  }
+  HIPCHK(hipEventRecord(e1, ctx->stream));
+  HIPCHK(hipEventSynchronize(e1));
+  float ms = 0.f;
+  HIPCHK(hipEventElapsedTime(&ms, e0, e1));
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  ctx->prof.collect();
+  if (stats) {
+    stats->steps = nsteps;
+    stats->vcycles = cycles;
+    stats->ir_iters = iters;
+    stats->converged = conv;
+    stats->rel_res = rel;
+    stats->ms = ms;
+    double mx = 0.0, l2 = 0.0;
+    if ((rc = div_norms(ctx, &mx, &l2))) return rc;
+    stats->max_div = mx;
+    stats->l2_div = l2;
+    stats->max_div_rate = mx / ctx->cfg.dx;
+  }
+  return MAC3D_OK;
+}
+
+int mac3d_stage(void *h, int stage, double dt) {
+  if (!h) return MAC3D_EINVAL;
+  Ctx *ctx = (Ctx *)h;
+  if (!std::isfinite(dt) || dt < 0.0) return fail(ctx, MAC3D_EINVAL, "dt must be finite and >= 0");
+  int rc;
+  switch (stage) {
+    case MAC3D_STAGE_FACE_TO_CENTRE: rc = launch_face_to_centre(ctx); break;
+    case MAC3D_STAGE_ADVECT: rc = launch_advect(ctx, dt); break;
+    case MAC3D_STAGE_CASSON: rc = launch_casson(ctx, dt); break;
+    case MAC3D_STAGE_DIVERGENCE: rc = launch_divergence(ctx); break;
+    case MAC3D_STAGE_VCYCLE: rc = vcycle(ctx, false, 1.0); break;
+    case MAC3D_STAGE_PROJECT: rc = launch_project(ctx); break;
+    case MAC3D_STAGE_DYE_PREDICT: rc = launch_dye_predict(ctx, dt); break;
+    case MAC3D_STAGE_DYE_RESOLVE: rc = launch_dye_resolve(ctx, dt); break;
+    default: return fail(ctx, MAC3D_EINVAL, "unknown stage");
+  }
+  if (rc) return rc;
+  HIPCHK(hipStreamSynchronize(ctx->stream));
+  ctx->prof.collect();
+  return MAC3D_OK;
+}
+
+int mac3d_mg_op(void *h, int op, int level, int count) {
+  if (!h) return MAC3D_EINVAL;
+  Ctx *ctx = (Ctx *)h;
+  if (level < 0 || level > ctx->L) return fail(ctx, MAC3D_EINVAL, "level out of range");
+  if (count < 0) return fail(ctx, MAC3D_EINVAL, "count must be >= 0");
+  int rc;
+  switch (op) {
+    case MAC3D_MG_SMOOTH: rc = mg_smooth(ctx, level, count, false, 1.0); break;
+    case MAC3D_MG_RESIDUAL: rc = mg_residual(ctx, level, 1.0); break;
+    case MAC3D_MG_RESTRICT:
+      if (level >= ctx->L) return fail(ctx, MAC3D_EINVAL, "restrict needs level < levels");
+      rc = mg_restrict(ctx, level);
+      break;
+    case MAC3D_MG_PROLONG:
+      if (level >= ctx->L) return fail(ctx, MAC3D_EINVAL, "prolong needs level < levels");
+      rc = mg_prolong(ctx, level);
+      break;
+    case MAC3D_MG_COARSE:
+      if (level != ctx->L) return fail(ctx, MAC3D_EINVAL, "coarse solve runs on the coarsest level");
+      rc = mg_coarse(ctx);
+      break;
+    case MAC3D_MG_ZERO: rc = mg_fill_zero(ctx, ctx->mgx[level]); break;
+    default: return fail(ctx, MAC3D_EINVAL, "unknown multigrid op");
+  }
+  if (rc) return rc;
+  HIPCHK(hipStreamSynchronize(ctx->stream));
+  ctx->prof.collect();
+  return MAC3D_OK;
+}
+
+int mac3d_div_norms(void *h, double *max_div, double *l2_div) {
+  if (!h || !max_div || !l2_div) return MAC3D_EINVAL;
+  return div_norms((Ctx *)h, max_div, l2_div);
+}
+
+int mac3d_profile(void *h, int enable) {
+  if (!h) return MAC3D_EINVAL;
+  Ctx *ctx = (Ctx *)h;
+  HIPCHK(hipStreamSynchronize(ctx->stream));
+  ctx->prof.collect();
+  ctx->prof.reset();
+  ctx->prof.enabled = enable != 0;
+  return MAC3D_OK;
+}
+
+int mac3d_profile_read(void *h, int kclass, double *ms_total, int64_t *launches) {
+  if (!h || !ms_total || !launches) return MAC3D_EINVAL;
+  Ctx *ctx = (Ctx *)h;
+  if (kclass < 0 || kclass >= KC_COUNT) return fail(ctx, MAC3D_EINVAL, "unknown kernel class");
+  HIPCHK(hipStreamSynchronize(ctx->stream));
+  ctx->prof.collect();
+  *ms_total = ctx->prof.ms[kclass];
+  *launches = ctx->prof.count[kclass];
+  return MAC3D_OK;
+}
+
+const char *mac3d_last_error(void *h) {
+  if (!h) return g_create_err.c_str();
+  return ((Ctx *)h)->err.c_str();
+}
+
+const char *mac3d_version(void) { return "mac3d 0.1.0 (gfx950, abi 1)"; }
+
+}  // extern "C"

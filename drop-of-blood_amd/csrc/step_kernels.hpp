@@ -1,0 +1,399 @@
+// step_kernels.hpp — the non-solver kernels of one MAC step.
+//
+// One thread per cell, launched as (64, 4, 1) blocks over (x, y) with z on
+// grid.z, so a wavefront covers 64 consecutive x cells of one row: every
+// plain load/store is a coalesced 256 B (FP32) row segment.
+#pragma once
+
+#include "common.hpp"
+
+namespace mac3d {
+
+struct StepParams {
+  double dt, dx;
+  double rho_w, mu_w, rho_b, tau_y, mu_inf, g;
+  double gamma_reg, shear_eps2, alpha_max, vel_damp, maccormack;
+};
+
+#define MAC3D_CELL_PROLOGUE                                  \
+  int i = blockIdx.x * blockDim.x + threadIdx.x;             \
+  int j = blockIdx.y * blockDim.y + threadIdx.y;             \
+  int k = blockIdx.z;                                        \
+  if (i >= d.nx || j >= d.ny) return;                        \
+  size_t c = idx3(d, i, j, k);                               \
+  const size_t sx = 1, sy = (size_t)d.nx, sz = (size_t)d.nx * d.ny; \
+  (void)sx; (void)sy; (void)sz;
+
+// FLUID_MAC_TO_CENTER (src/shaders/fluid.ts:917-943): centre velocity is the
+// mean of the two faces of each axis, solid wall faces read as 0.
+template <class T>
+__global__ void __launch_bounds__(256)
+k_face_to_centre(Dims d, const T *__restrict__ u, const T *__restrict__ v,
+                 const T *__restrict__ w, T *__restrict__ cu,
+                 T *__restrict__ cv, T *__restrict__ cw) {
+  MAC3D_CELL_PROLOGUE
+  T uR = (i >= d.nx - 1) ? T(0) : u[c];
+  T uL = (i == 0) ? T(0) : u[c - sx];
+  T vU = v[c];
+  T vD = (j == 0) ? T(0) : v[c - sy];
+  T wF = (k >= d.nz - 1) ? T(0) : w[c];
+  T wB = (k == 0) ? T(0) : w[c - sz];
+  cu[c] = T(0.5) * (uL + uR);
+  cv[c] = T(0.5) * (vD + vU);
+  cw[c] = T(0.5) * (wB + wF);
+}
+
+// traceBackUVW (fluid.ts:169-174): RK2 midpoint backtrace on the centred
+// velocity in normalised coordinates, clamped to [0.002, 0.998].
+template <class T>
+__device__ inline void trace_back(const Dims &d, const T *__restrict__ cu,
+                                  const T *__restrict__ cv,
+                                  const T *__restrict__ cw, const T sc[3],
+                                  const T f[3], T out[3]) {
+  const T lo = T(0.002), hi = T(0.998);
+  T p0x = clampv(f[0], lo, hi), p0y = clampv(f[1], lo, hi), p0z = clampv(f[2], lo, hi);
+  T v0x = sample_cell(cu, d, p0x, p0y, p0z);
+  T v0y = sample_cell(cv, d, p0x, p0y, p0z);
+  T v0z = sample_cell(cw, d, p0x, p0y, p0z);
+  T mx = clampv(f[0] - T(0.5) * (v0x * sc[0]), lo, hi);
+  T my = clampv(f[1] - T(0.5) * (v0y * sc[1]), lo, hi);
+  T mz = clampv(f[2] - T(0.5) * (v0z * sc[2]), lo, hi);
+  T vmx = sample_cell(cu, d, mx, my, mz);
+  T vmy = sample_cell(cv, d, mx, my, mz);
+  T vmz = sample_cell(cw, d, mx, my, mz);
+  out[0] = clampv(f[0] - vmx * sc[0], lo, hi);
+  out[1] = clampv(f[1] - vmy * sc[1], lo, hi);
+  out[2] = clampv(f[2] - vmz * sc[2], lo, hi);
+}
+
+// FLUID_ADVECT_MAC (fluid.ts:144-192, 323-325): each staggered component is
+// traced back from its own face and sampled with the -1/2 texel shift of
+// sampleMacU/V/W (fluid.ts:154-167), then multiplied by the damping factor;
+// the two solid wall faces stay 0.  sc = dt / domain size per axis.
+template <class T>
+__global__ void __launch_bounds__(256)
+k_advect_mac(Dims d, T sc0, T sc1, T sc2, T damp, const T *__restrict__ u,
+             const T *__restrict__ v, const T *__restrict__ w,
+             const T *__restrict__ cu, const T *__restrict__ cv,
+             const T *__restrict__ cw, T *__restrict__ uo, T *__restrict__ vo,
+             T *__restrict__ wo) {
+  MAC3D_CELL_PROLOGUE
+  const T sc[3] = {sc0, sc1, sc2};
+  const T inx = T(1) / (T)d.nx, iny = T(1) / (T)d.ny, inz = T(1) / (T)d.nz;
+  const T lo = T(0.001), hi = T(0.999);
+  T b[3];
+  {
+    T f[3] = {((T)i + T(1)) / (T)d.nx, ((T)j + T(0.5)) / (T)d.ny, ((T)k + T(0.5)) / (T)d.nz};
+    trace_back(d, cu, cv, cw, sc, f, b);
+    T vx = sample_cell(u, d, clampv(b[0] - T(0.5) * inx, lo, hi), clampv(b[1], lo, hi),
+                       clampv(b[2], lo, hi)) * damp;
+    uo[c] = (i >= d.nx - 1) ? T(0) : vx;
+  }
+  {
+    T f[3] = {((T)i + T(0.5)) / (T)d.nx, ((T)j + T(1)) / (T)d.ny, ((T)k + T(0.5)) / (T)d.nz};
+    trace_back(d, cu, cv, cw, sc, f, b);
+    vo[c] = sample_cell(v, d, clampv(b[0], lo, hi), clampv(b[1] - T(0.5) * iny, lo, hi),
+                        clampv(b[2], lo, hi)) * damp;
+  }
+  {
+    T f[3] = {((T)i + T(0.5)) / (T)d.nx, ((T)j + T(0.5)) / (T)d.ny, ((T)k + T(1)) / (T)d.nz};
+    trace_back(d, cu, cv, cw, sc, f, b);
+    T vz = sample_cell(w, d, clampv(b[0], lo, hi), clampv(b[1], lo, hi),
+                       clampv(b[2] - T(0.5) * inz, lo, hi)) * damp;
+    wo[c] = (k >= d.nz - 1) ? T(0) : vz;
+  }
+}
+
+// Neighbour face values with the workload's wall rules: no-slip solid walls
+// (tangential ghost = -value, normal wall face = 0) and a free-slip lid
+// (ghost = value).  At most one index is out of range per call.
+template <class T>
+__device__ inline T face_u(const Dims &d, const T *__restrict__ u, int i, int j, int k) {
+  if (i < 0 || i >= d.nx) return T(0);
+  if (j < 0) return -u[idx3(d, i, 0, k)];
+  if (j >= d.ny) return u[idx3(d, i, d.ny - 1, k)];
+  if (k < 0) return -u[idx3(d, i, j, 0)];
+  if (k >= d.nz) return -u[idx3(d, i, j, d.nz - 1)];
+  return u[idx3(d, i, j, k)];
+}
+template <class T>
+__device__ inline T face_v(const Dims &d, const T *__restrict__ v, int i, int j, int k) {
+  if (j < 0) return T(0);
+  if (j >= d.ny) return v[idx3(d, i, d.ny - 1, k)];
+  if (i < 0) return -v[idx3(d, 0, j, k)];
+  if (i >= d.nx) return -v[idx3(d, d.nx - 1, j, k)];
+  if (k < 0) return -v[idx3(d, i, j, 0)];
+  if (k >= d.nz) return -v[idx3(d, i, j, d.nz - 1)];
+  return v[idx3(d, i, j, k)];
+}
+template <class T>
+__device__ inline T face_w(const Dims &d, const T *__restrict__ w, int i, int j, int k) {
+  if (k < 0 || k >= d.nz) return T(0);
+  if (i < 0) return -w[idx3(d, 0, j, k)];
+  if (i >= d.nx) return -w[idx3(d, d.nx - 1, j, k)];
+  if (j < 0) return -w[idx3(d, i, 0, k)];
+  if (j >= d.ny) return w[idx3(d, i, d.ny - 1, k)];
+  return w[idx3(d, i, j, k)];
+}
+
+// FLUID_CASSON_FORCES (fluid.ts:377-511) parts (1) and (3): Casson
+// shear-thinning explicit viscous update and Boussinesq buoyancy.
+//   gamma  = |+-1 neighbour differences| / (2h)          (fluid.ts:451-454)
+//   mu_eff = (sqrt(mu_inf(c)) + sqrt(tau_y(c) / max(gamma + reg, reg)))^2
+//                                                        (fluid.ts:458-462)
+//   u     += alpha (sum of 6 neighbours - 6 u), alpha = mu_eff/rho dt/h^2
+//                                                        (fluid.ts:464-465)
+//   v     += -g (rho_b - rho_w)/rho_w * dye_face * dt    (fluid.ts:477-486)
+// with SI Casson parameters blended by the dye fraction c.
+template <class T>
+__global__ void __launch_bounds__(256)
+k_casson(Dims d, StepParams P, const T *__restrict__ u, const T *__restrict__ v,
+         const T *__restrict__ w, const T *__restrict__ dye, T *__restrict__ uo,
+         T *__restrict__ vo, T *__restrict__ wo) {
+  MAC3D_CELL_PROLOGUE
+  const T inv2h = T(1.0 / (2.0 * P.dx));
+  const T dtoh2 = T(P.dt / (P.dx * P.dx));
+  const T buoy = T(-P.g * (P.rho_b - P.rho_w) / P.rho_w * P.dt);
+  T m0 = u[c], m1 = v[c], m2 = w[c];
+  T xp0 = face_u(d, u, i + 1, j, k), xp1 = face_v(d, v, i + 1, j, k), xp2 = face_w(d, w, i + 1, j, k);
+  T xn0 = face_u(d, u, i - 1, j, k), xn1 = face_v(d, v, i - 1, j, k), xn2 = face_w(d, w, i - 1, j, k);
+  T yp0 = face_u(d, u, i, j + 1, k), yp1 = face_v(d, v, i, j + 1, k), yp2 = face_w(d, w, i, j + 1, k);
+  T yn0 = face_u(d, u, i, j - 1, k), yn1 = face_v(d, v, i, j - 1, k), yn2 = face_w(d, w, i, j - 1, k);
+  T zp0 = face_u(d, u, i, j, k + 1), zp1 = face_v(d, v, i, j, k + 1), zp2 = face_w(d, w, i, j, k + 1);
+  T zn0 = face_u(d, u, i, j, k - 1), zn1 = face_v(d, v, i, j, k - 1), zn2 = face_w(d, w, i, j, k - 1);
+  T ss = T(0);
+  {
+    T a = xp0 - xn0, b = yp0 - yn0, e = zp0 - zn0;
+    ss += a * a + b * b + e * e;
+    a = xp1 - xn1; b = yp1 - yn1; e = zp1 - zn1;
+    ss += a * a + b * b + e * e;
+    a = xp2 - xn2; b = yp2 - yn2; e = zp2 - zn2;
+    ss += a * a + b * b + e * e;
+  }
+  T gam = inv2h * sqrt(ss + T(P.shear_eps2));
+  T cc = clampv(dye[c], T(0), T(1));
+  T sq_muinf = sqrt(T(P.mu_w) + cc * T(P.mu_inf - P.mu_w));
+  T tau = T(P.tau_y) * cc * cc;
+  T reg = T(P.gamma_reg);
+  T gr = gam + reg;
+  T yterm = sqrt(tau / (gr > reg ? gr : reg));
+  T mu_eff = (sq_muinf + yterm) * (sq_muinf + yterm);
+  T rho = T(P.rho_w) + cc * T(P.rho_b - P.rho_w);
+  T alpha = clampv(mu_eff / rho * dtoh2, T(0), T(P.alpha_max));
+  T o0 = m0 + alpha * ((xp0 + xn0 + yp0 + yn0 + zp0 + zn0) - T(6) * m0);
+  T o1 = m1 + alpha * ((xp1 + xn1 + yp1 + yn1 + zp1 + zn1) - T(6) * m1);
+  T o2 = m2 + alpha * ((xp2 + xn2 + yp2 + yn2 + zp2 + zn2) - T(6) * m2);
+  T dyef = T(0.5) * (dye[c] + dye[(j + 1 < d.ny) ? c + sy : c]);
+  o1 += buoy * dyef;
+  uo[c] = (i >= d.nx - 1) ? T(0) : o0;
+  vo[c] = o1;
+  wo[c] = (k >= d.nz - 1) ? T(0) : o2;
+}
+
+// FLUID_MAC_RHS (fluid.ts:517-555) with a flat lid (p_top = 0):
+// D = (uR - uL) + (vT - vB) + (wF - wB), solid wall faces 0, top face free.
+template <class T, class R>
+__global__ void __launch_bounds__(256)
+k_divergence(Dims d, const T *__restrict__ u, const T *__restrict__ v,
+             const T *__restrict__ w, R *__restrict__ out) {
+  MAC3D_CELL_PROLOGUE
+  typedef typename compute_of<R>::type C;
+  C uR = (i >= d.nx - 1) ? C(0) : (C)u[c];
+  C uL = (i == 0) ? C(0) : (C)u[c - sx];
+  C vT = (C)v[c];
+  C vB = (j == 0) ? C(0) : (C)v[c - sy];
+  C wF = (k >= d.nz - 1) ? C(0) : (C)w[c];
+  C wB = (k == 0) ? C(0) : (C)w[c - sz];
+  out[c] = (R)((uR - uL) + (vT - vB) + (wF - wB));
+}
+
+// FLUID_MAC_PROJECT (fluid.ts:863-911) with a flat lid: face -= p(+1) - p,
+// solid wall faces 0, top face uses the zero ghost above the top row.
+template <class T, class Pt>
+__global__ void __launch_bounds__(256)
+k_project(Dims d, T *__restrict__ u, T *__restrict__ v, T *__restrict__ w,
+          const Pt *__restrict__ p) {
+  MAC3D_CELL_PROLOGUE
+  typedef typename compute_of<Pt>::type C;
+  C pc = (C)p[c];
+  if (i < d.nx - 1) u[c] = (T)((C)u[c] - ((C)p[c + sx] - pc)); else u[c] = T(0);
+  C pu = (j < d.ny - 1) ? (C)p[c + sy] : C(0);
+  v[c] = (T)((C)v[c] - (pu - pc));
+  if (k < d.nz - 1) w[c] = (T)((C)w[c] - ((C)p[c + sz] - pc)); else w[c] = T(0);
+}
+
+// CONC_PREDICT (advect.ts:28-72): RK2 semi-Lagrangian predictor of the dye
+// on the centred velocity, normalised coordinates clamped to [0.001, 0.999].
+template <class T>
+__device__ inline void dye_back(const Dims &d, const T *__restrict__ cu,
+                                const T *__restrict__ cv, const T *__restrict__ cw,
+                                const T sc[3], const T uvw[3], T v0[3], T back[3]) {
+  const T lo = T(0.001), hi = T(0.999);
+  v0[0] = sample_cell(cu, d, uvw[0], uvw[1], uvw[2]);
+  v0[1] = sample_cell(cv, d, uvw[0], uvw[1], uvw[2]);
+  v0[2] = sample_cell(cw, d, uvw[0], uvw[1], uvw[2]);
+  T mx = clampv(uvw[0] - T(0.5) * (v0[0] * sc[0]), lo, hi);
+  T my = clampv(uvw[1] - T(0.5) * (v0[1] * sc[1]), lo, hi);
+  T mz = clampv(uvw[2] - T(0.5) * (v0[2] * sc[2]), lo, hi);
+  T vmx = sample_cell(cu, d, mx, my, mz);
+  T vmy = sample_cell(cv, d, mx, my, mz);
+  T vmz = sample_cell(cw, d, mx, my, mz);
+  back[0] = clampv(uvw[0] - vmx * sc[0], lo, hi);
+  back[1] = clampv(uvw[1] - vmy * sc[1], lo, hi);
+  back[2] = clampv(uvw[2] - vmz * sc[2], lo, hi);
+}
+
+template <class T>
+__global__ void __launch_bounds__(256)
+k_dye_predict(Dims d, T sc0, T sc1, T sc2, const T *__restrict__ cu,
+              const T *__restrict__ cv, const T *__restrict__ cw,
+              const T *__restrict__ dye, T *__restrict__ pred) {
+  MAC3D_CELL_PROLOGUE
+  const T sc[3] = {sc0, sc1, sc2};
+  T uvw[3] = {((T)i + T(0.5)) / (T)d.nx, ((T)j + T(0.5)) / (T)d.ny, ((T)k + T(0.5)) / (T)d.nz};
+  T v0[3], back[3];
+  dye_back(d, cu, cv, cw, sc, uvw, v0, back);
+  pred[c] = sample_cell(dye, d, back[0], back[1], back[2]);
+}
+
+// CONC_RESOLVE's MacCormack core (advect.ts:196-222): forward estimate from
+// the predictor at uvw + v0 dt, corrected value pred + s (orig - estimate),
+// clamped to the min/max of the 8 source texels around the backtrace point.
+template <class T>
+__global__ void __launch_bounds__(256)
+k_dye_resolve(Dims d, T sc0, T sc1, T sc2, T strength, const T *__restrict__ cu,
+              const T *__restrict__ cv, const T *__restrict__ cw,
+              const T *__restrict__ dye, const T *__restrict__ pred,
+              T *__restrict__ out) {
+  MAC3D_CELL_PROLOGUE
+  const T sc[3] = {sc0, sc1, sc2};
+  const T lo = T(0.001), hi = T(0.999);
+  T uvw[3] = {((T)i + T(0.5)) / (T)d.nx, ((T)j + T(0.5)) / (T)d.ny, ((T)k + T(0.5)) / (T)d.nz};
+  T v0[3], back[3];
+  dye_back(d, cu, cv, cw, sc, uvw, v0, back);
+  T fx = clampv(uvw[0] + v0[0] * sc[0], lo, hi);
+  T fy = clampv(uvw[1] + v0[1] * sc[1], lo, hi);
+  T fz = clampv(uvw[2] + v0[2] * sc[2], lo, hi);
+  T est = sample_cell(pred, d, fx, fy, fz);
+  T corr = pred[c] + strength * (dye[c] - est);
+  int bi = (int)floor(back[0] * (T)d.nx - T(0.5));
+  int bj = (int)floor(back[1] * (T)d.ny - T(0.5));
+  int bk = (int)floor(back[2] * (T)d.nz - T(0.5));
+  int i0 = clampi(bi, 0, d.nx - 1), i1 = clampi(bi + 1, 0, d.nx - 1);
+  int j0 = clampi(bj, 0, d.ny - 1), j1 = clampi(bj + 1, 0, d.ny - 1);
+  int k0 = clampi(bk, 0, d.nz - 1), k1 = clampi(bk + 1, 0, d.nz - 1);
+  T s0 = dye[idx3(d, i0, j0, k0)], s1 = dye[idx3(d, i1, j0, k0)];
+  T s2 = dye[idx3(d, i0, j1, k0)], s3 = dye[idx3(d, i1, j1, k0)];
+  T s4 = dye[idx3(d, i0, j0, k1)], s5 = dye[idx3(d, i1, j0, k1)];
+  T s6 = dye[idx3(d, i0, j1, k1)], s7 = dye[idx3(d, i1, j1, k1)];
+  T mn = fmin(fmin(fmin(s0, s1), fmin(s2, s3)), fmin(fmin(s4, s5), fmin(s6, s7)));
+  T mx = fmax(fmax(fmax(s0, s1), fmax(s2, s3)), fmax(fmax(s4, s5), fmax(s6, s7)));
+  out[c] = clampv(corr, mn, mx);
+}
+
+// Seeded blob initial condition: dye at cell centres and -w0 on V faces,
+// each the sum over blobs of 1 - smoothstep(r - 1/2, r + 1/2, distance).
+struct BlobDev {
+  double cx, cy, cz, radius, w0;
+};
+
+template <class T>
+__device__ inline T smoothstep_v(T e0, T e1, T x) {
+  T t = clampv((x - e0) / (e1 - e0), T(0), T(1));
+  return t * t * (T(3) - T(2) * t);
+}
+
+template <class T>
+__global__ void __launch_bounds__(256)
+k_init_blobs(Dims d, const BlobDev *__restrict__ blobs, int nb, T *__restrict__ dye,
+             T *__restrict__ v) {
+  MAC3D_CELL_PROLOGUE
+  double dy = 0.0, vv = 0.0;
+  for (int q = 0; q < nb; ++q) {
+    BlobDev b = blobs[q];
+    double ex = i + 0.5 - b.cx, ey = j + 0.5 - b.cy, ez = k + 0.5 - b.cz;
+    double r = sqrt(ex * ex + ey * ey + ez * ez);
+    dy += 1.0 - smoothstep_v(b.radius - 0.5, b.radius + 0.5, r);
+    double fy = j + 1.0 - b.cy;
+    double rf = sqrt(ex * ex + fy * fy + ez * ez);
+    vv -= b.w0 * (1.0 - smoothstep_v(b.radius - 0.5, b.radius + 0.5, rf));
+  }
+  dye[c] = (T)dy;
+  v[c] = (T)vv;
+}
+
+// Divergence norms: per-block max |D| and sum D^2 in FP64, then one block
+// folds the partials in block order (deterministic).
+template <class T>
+__global__ void __launch_bounds__(256)
+k_div_partials(Dims d, const T *__restrict__ u, const T *__restrict__ v,
+               const T *__restrict__ w, double *__restrict__ part) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  int j = blockIdx.y * blockDim.y + threadIdx.y;
+  int k = blockIdx.z;
+  double mx = 0.0, ss = 0.0;
+  if (i < d.nx && j < d.ny) {
+    size_t c = idx3(d, i, j, k);
+    const size_t sy = (size_t)d.nx, sz = (size_t)d.nx * d.ny;
+    double uR = (i >= d.nx - 1) ? 0.0 : (double)u[c];
+    double uL = (i == 0) ? 0.0 : (double)u[c - 1];
+    double vT = (double)v[c];
+    double vB = (j == 0) ? 0.0 : (double)v[c - sy];
+    double wF = (k >= d.nz - 1) ? 0.0 : (double)w[c];
+    double wB = (k == 0) ? 0.0 : (double)w[c - sz];
+    double D = (uR - uL) + (vT - vB) + (wF - wB);
+    mx = fabs(D);
+    ss = D * D;
+  }
+  mx = wave_max(mx);
+  ss = wave_sum(ss);
+  __shared__ double smx[4], sss[4];
+  int t = threadIdx.y * blockDim.x + threadIdx.x;
+  if ((t & 63) == 0) { smx[t >> 6] = mx; sss[t >> 6] = ss; }
+  __syncthreads();
+  if (t == 0) {
+    double a = smx[0], b = sss[0];
+    for (int q = 1; q < 4; ++q) { a = smx[q] > a ? smx[q] : a; b += sss[q]; }
+    size_t blk = ((size_t)blockIdx.z * gridDim.y + blockIdx.y) * gridDim.x + blockIdx.x;
+    part[2 * blk] = a;
+    part[2 * blk + 1] = b;
+  }
+}
+
+// Fold n (max, sum) pairs into out[0] = max, out[1] = sum; one block of 256.
+__global__ void __launch_bounds__(256)
+k_fold_partials(const double *__restrict__ part, size_t n, double *__restrict__ out) {
+  double mx = 0.0, ss = 0.0;
+  for (size_t q = threadIdx.x; q < n; q += 256) {
+    double a = part[2 * q];
+    mx = a > mx ? a : mx;
+    ss += part[2 * q + 1];
+  }
+  mx = wave_max(mx);
+  ss = wave_sum(ss);
+  __shared__ double smx[4], sss[4];
+  int t = threadIdx.x;
+  if ((t & 63) == 0) { smx[t >> 6] = mx; sss[t >> 6] = ss; }
+  __syncthreads();
+  if (t == 0) {
+    double a = smx[0], b = sss[0];
+    for (int q = 1; q < 4; ++q) { a = smx[q] > a ? smx[q] : a; b += sss[q]; }
+    out[0] = a;
+    out[1] = b;
+  }
+}
+
+// dtype conversion between a staging buffer and a field
+template <class A, class B>
+__global__ void __launch_bounds__(256) k_convert(const A *__restrict__ src, B *__restrict__ dst, size_t n) {
+  size_t t = (size_t)blockIdx.x * 256 + threadIdx.x;
+  if (t < n) dst[t] = (B)src[t];
+}
+
+template <class T>
+__global__ void __launch_bounds__(256) k_fill(T *__restrict__ dst, size_t n, T val) {
+  size_t t = (size_t)blockIdx.x * 256 + threadIdx.x;
+  if (t < n) dst[t] = val;
+}
+
+}  // namespace mac3d

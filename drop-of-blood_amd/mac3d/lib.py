@@ -1,0 +1,217 @@
+"""ctypes binding of include/mac3d.h and the MacSolver host class."""
+from __future__ import annotations
+
+import ctypes as C
+import os
+from typing import Optional, Sequence
+
+import numpy as np
+
+PKG = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+LIB_PATH = os.path.join(PKG, "lib", "libmac3d.so")
+
+F32, F64 = 0, 1
+PREC = {"fp32": 0, "mixed": 1, "fp64": 2}
+FIELD = dict(U=0, V=1, W=2, P=3, DYE=4, CU=5, CV=6, CW=7, RHS=8, MGX=9, MGB=10, MGR=11,
+             DYE_PRED=12)
+STAGE = dict(FACE_TO_CENTRE=0, ADVECT=1, CASSON=2, DIVERGENCE=3, VCYCLE=4, PROJECT=5,
+             DYE_PREDICT=6, DYE_RESOLVE=7)
+MG = dict(SMOOTH=0, RESIDUAL=1, RESTRICT=2, PROLONG=3, COARSE=4, ZERO=5)
+KC = dict(FACE_TO_CENTRE=0, ADVECT=1, CASSON=2, DIVERGENCE=3, SMOOTH_L0=4, RESTRICT_L0=5,
+          PROLONG_L0=6, MG_COARSE_LEVELS=7, COARSE_SOLVE=8, PROJECT=9, DYE=10, REFINE64=11)
+STATUS = {0: "OK", 1: "EINVAL", 2: "EHIP", 3: "ENOMEM", 4: "ESTATE"}
+
+
+class Config(C.Structure):
+    _fields_ = [(n, C.c_int32) for n in ("nx", "ny", "nz", "levels", "nu1", "nu2", "cycles",
+                                          "precision", "ir_max_iters", "device")] + \
+               [(n, C.c_double) for n in ("dx", "rho_w", "mu_w", "rho_b", "tau_y", "mu_inf", "g",
+                                           "gamma_reg", "shear_eps2", "alpha_max", "vel_damp",
+                                           "maccormack", "ir_tol")]
+
+
+class Blob(C.Structure):
+    _fields_ = [(n, C.c_double) for n in ("cx", "cy", "cz", "radius", "w0")]
+
+
+class Stats(C.Structure):
+    _fields_ = [(n, C.c_int32) for n in ("steps", "vcycles", "ir_iters", "converged")] + \
+               [(n, C.c_double) for n in ("max_div", "l2_div", "max_div_rate", "rel_res", "ms")]
+
+    def as_dict(self):
+        return {n: getattr(self, n) for n, _ in self._fields_}
+
+
+class Mac3dError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"mac3d {STATUS.get(code, code)}: {msg}")
+        self.code = code
+
+
+_lib = None
+EXPORTS = ("mac3d_default_config", "mac3d_create", "mac3d_destroy", "mac3d_upload",
+           "mac3d_download", "mac3d_init_blobs", "mac3d_step", "mac3d_stage", "mac3d_mg_op",
+           "mac3d_div_norms", "mac3d_profile", "mac3d_profile_read", "mac3d_last_error",
+           "mac3d_version")
+
+
+def load_library(path: Optional[str] = None):
+    """Load libmac3d.so (raises if it is not built: no fallback path)."""
+    global _lib
+    if _lib is not None and path is None:
+        return _lib
+    p = path or LIB_PATH
+    if not os.path.exists(p):
+        raise ImportError(f"{p} is not built; run __graft_entry__.build()")
+    L = C.CDLL(p)
+    vp = C.c_void_p
+    L.mac3d_default_config.argtypes = [C.POINTER(Config)]
+    L.mac3d_create.argtypes = [C.POINTER(Config), C.POINTER(vp)]
+    L.mac3d_destroy.argtypes = [vp]
+    L.mac3d_upload.argtypes = [vp, C.c_int, C.c_int, vp, C.c_int, C.c_size_t]
+    L.mac3d_download.argtypes = [vp, C.c_int, C.c_int, vp, C.c_int, C.c_size_t]
+    L.mac3d_init_blobs.argtypes = [vp, C.POINTER(Blob), C.c_int]
+    L.mac3d_step.argtypes = [vp, C.c_double, C.c_int, C.POINTER(Stats)]
+    L.mac3d_stage.argtypes = [vp, C.c_int, C.c_double]
+    L.mac3d_mg_op.argtypes = [vp, C.c_int, C.c_int, C.c_int]
+    L.mac3d_div_norms.argtypes = [vp, C.POINTER(C.c_double), C.POINTER(C.c_double)]
+    L.mac3d_profile.argtypes = [vp, C.c_int]
+    L.mac3d_profile_read.argtypes = [vp, C.c_int, C.POINTER(C.c_double), C.POINTER(C.c_int64)]
+    L.mac3d_last_error.argtypes = [vp]
+    L.mac3d_last_error.restype = C.c_char_p
+    L.mac3d_version.restype = C.c_char_p
+    for n in EXPORTS:
+        if n not in ("mac3d_last_error", "mac3d_version"):
+            getattr(L, n).restype = C.c_int
+    if path is None:
+        _lib = L
+    return L
+
+
+def default_config(**kw) -> Config:
+    L = load_library()
+    c = Config()
+    L.mac3d_default_config(C.byref(c))
+    for k, v in kw.items():
+        if not hasattr(c, k):
+            raise AttributeError(k)
+        setattr(c, k, v)
+    return c
+
+
+class MacSolver:
+    """One simulation on one GPU: step(dt, n) and field buffers.
+
+    Mirrors the TypeScript MacSolver (js/MacSolver.js); arrays are numpy
+    (nz, ny, nx) with x fastest, as the reference's textures are laid out.
+    """
+
+    def __init__(self, config: Optional[Config] = None, **kw):
+        self.L = load_library()
+        self.cfg = config if config is not None else default_config(**kw)
+        h = C.c_void_p()
+        rc = self.L.mac3d_create(C.byref(self.cfg), C.byref(h))
+        if rc:
+            raise Mac3dError(rc, self.L.mac3d_last_error(None).decode())
+        self.h = h
+        self.shape = (self.cfg.nz, self.cfg.ny, self.cfg.nx)
+
+    # -- lifetime
+    def close(self):
+        if getattr(self, "h", None):
+            self.L.mac3d_destroy(self.h)
+            self.h = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _check(self, rc):
+        if rc:
+            raise Mac3dError(rc, self.L.mac3d_last_error(self.h).decode())
+
+    # -- fields
+    def level_shape(self, level: int):
+        nz, ny, nx = self.shape
+        return (nz >> level, ny >> level, nx >> level)
+
+    def download(self, field, level: int = 0, dtype=np.float64) -> np.ndarray:
+        fid = FIELD[field] if isinstance(field, str) else field
+        out = np.empty(self.level_shape(level), dtype=dtype)
+        dt = F64 if out.dtype == np.float64 else F32
+        self._check(self.L.mac3d_download(self.h, fid, level, out.ctypes.data, dt, out.size))
+        return out
+
+    def upload(self, field, arr, level: int = 0):
+        fid = FIELD[field] if isinstance(field, str) else field
+        a = np.ascontiguousarray(arr)
+        if a.dtype not in (np.float32, np.float64):
+            a = a.astype(np.float64)
+        dt = F64 if a.dtype == np.float64 else F32
+        self._check(self.L.mac3d_upload(self.h, fid, level, a.ctypes.data, dt, a.size))
+
+    @property
+    def u(self): return self.download("U")
+
+    @property
+    def v(self): return self.download("V")
+
+    @property
+    def w(self): return self.download("W")
+
+    @property
+    def p(self): return self.download("P")
+
+    @property
+    def dye(self): return self.download("DYE")
+
+    # -- stepping
+    def init_blobs(self, blobs: Sequence):
+        arr = (Blob * len(blobs))(*[Blob(b.cx, b.cy, b.cz, b.radius, b.w0) for b in blobs])
+        self._check(self.L.mac3d_init_blobs(self.h, arr, len(blobs)))
+
+    def step(self, dt: float, n: int = 1) -> dict:
+        st = Stats()
+        self._check(self.L.mac3d_step(self.h, dt, n, C.byref(st)))
+        return st.as_dict()
+
+    def step_nostats(self, dt: float, n: int = 1):
+        self._check(self.L.mac3d_step(self.h, dt, n, None))
+
+    def stage(self, stage, dt: float = 0.0):
+        sid = STAGE[stage] if isinstance(stage, str) else stage
+        self._check(self.L.mac3d_stage(self.h, sid, dt))
+
+    def mg(self, op, level: int, count: int = 1):
+        oid = MG[op] if isinstance(op, str) else op
+        self._check(self.L.mac3d_mg_op(self.h, oid, level, count))
+
+    def div_norms(self):
+        a, b = C.c_double(), C.c_double()
+        self._check(self.L.mac3d_div_norms(self.h, C.byref(a), C.byref(b)))
+        return a.value, b.value
+
+    def profile(self, enable: bool):
+        self._check(self.L.mac3d_profile(self.h, int(enable)))
+
+    def profile_read(self, kclass) -> tuple:
+        kid = KC[kclass] if isinstance(kclass, str) else kclass
+        ms, n = C.c_double(), C.c_int64()
+        self._check(self.L.mac3d_profile_read(self.h, kid, C.byref(ms), C.byref(n)))
+        return ms.value, n.value
+
+
+def config_for(workload, **kw) -> Config:
+    """Config of a scenarios.Workload."""
+    return default_config(nx=workload.nx, ny=workload.ny, nz=workload.nz, dx=workload.dx,
+                          levels=workload.levels, nu1=workload.nu1, nu2=workload.nu2,
+                          cycles=workload.cycles, precision=workload.precision,
+                          ir_tol=workload.ir_tol, **kw)

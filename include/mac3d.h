@@ -1,0 +1,190 @@
+/*
+ * mac3d.h — C ABI of the MI355X staggered-MAC Navier–Stokes step.
+ *
+ * One context owns every device buffer of one simulation and one HIP stream.
+ * All entry points return an int status (MAC3D_OK = 0); on failure the
+ * context's last-error string says why (mac3d_last_error).  Host buffers
+ * belong to the caller: upload/download copy synchronously and never keep
+ * the pointer.  Calls on one context must be serialised by the caller.
+ *
+ * Grid convention (same as the reference's textures): cell (i,j,k) with x
+ * fastest, flat index (k*ny + j)*nx + i, y is the vertical axis.  The face
+ * components are stored per cell at the +1/2 face: U(i,j,k) is the x-face
+ * between cells i and i+1, V the y-face above cell j, W the z-face in front
+ * of cell k (src/shaders/fluid.ts:186-188, FLUID_INIT_VEL fluid.ts:95-97).
+ * U at i = nx-1 and W at k = nz-1 are solid walls (always 0); V at j = ny-1
+ * is the free-slip lid, left free (fluid.ts:323-325).
+ *
+ * Reference interface being replaced: the WebGPU compute pass recorded by
+ * BloodDropEngine.encodePhysicsStep (src/engine/BloodDropEngine.ts:1855-2069)
+ * for the MAC portion (:1916-2069), driven per substep by renderFrame
+ * (:2605) and seekToSimTime (:2142-2179).
+ */
+#ifndef MAC3D_H
+#define MAC3D_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define MAC3D_ABI_VERSION 1
+
+/* status codes */
+#define MAC3D_OK 0
+#define MAC3D_EINVAL 1   /* bad argument: sizes, NaN/negative dt, unknown id */
+#define MAC3D_EHIP 2     /* a HIP runtime call failed */
+#define MAC3D_ENOMEM 3   /* device allocation failed */
+#define MAC3D_ESTATE 4   /* call not valid in the context's current state */
+
+/* precision modes of the step */
+#define MAC3D_PREC_FP32 0  /* plain FP32: fixed V-cycle count per step */
+#define MAC3D_PREC_MIXED 1 /* FP32 V-cycles, FP16 level>=2 storage, FP64 IR */
+#define MAC3D_PREC_FP64 2  /* FP64 everywhere (gradient checks, references) */
+
+/* host dtypes for upload/download */
+#define MAC3D_F32 0
+#define MAC3D_F64 1
+
+/* field ids (level 0 unless noted) */
+#define MAC3D_FIELD_U 0    /* x-face velocity (m/s) */
+#define MAC3D_FIELD_V 1    /* y-face velocity (m/s), vertical */
+#define MAC3D_FIELD_W 2    /* z-face velocity (m/s) */
+#define MAC3D_FIELD_P 3    /* dimensionless pressure p~ = dt p/(rho h) */
+#define MAC3D_FIELD_DYE 4  /* blood dye fraction */
+#define MAC3D_FIELD_CU 5   /* cell-centred velocity x */
+#define MAC3D_FIELD_CV 6   /* cell-centred velocity y */
+#define MAC3D_FIELD_CW 7   /* cell-centred velocity z */
+#define MAC3D_FIELD_RHS 8  /* Poisson right-hand side = MAC divergence */
+#define MAC3D_FIELD_MGX 9  /* multigrid unknown of a level (level 0 = P) */
+#define MAC3D_FIELD_MGB 10 /* multigrid right-hand side of a level */
+#define MAC3D_FIELD_MGR 11 /* multigrid residual of a level */
+#define MAC3D_FIELD_DYE_PRED 12 /* dye semi-Lagrangian predictor */
+#define MAC3D_FIELD_COUNT 13
+
+/* single stages of the step, for kernel-level parity tests */
+#define MAC3D_STAGE_FACE_TO_CENTRE 0 /* CU,CV,CW <- U,V,W */
+#define MAC3D_STAGE_ADVECT 1         /* U,V,W <- advect(U,V,W; CU,CV,CW) */
+#define MAC3D_STAGE_CASSON 2         /* U,V,W <- Casson viscosity + buoyancy */
+#define MAC3D_STAGE_DIVERGENCE 3     /* RHS <- D(U,V,W) */
+#define MAC3D_STAGE_VCYCLE 4         /* P <- one V-cycle on (P, RHS) */
+#define MAC3D_STAGE_PROJECT 5        /* U,V,W <- U,V,W - G(P) */
+#define MAC3D_STAGE_DYE_PREDICT 6    /* DYE_PRED <- semi-Lagrangian(DYE) */
+#define MAC3D_STAGE_DYE_RESOLVE 7    /* DYE <- MacCormack + limiter */
+#define MAC3D_STAGE_COUNT 8
+
+/* single multigrid operators on one level, for parity tests */
+#define MAC3D_MG_SMOOTH 0   /* count RBGS sweeps on (MGX, MGB) */
+#define MAC3D_MG_RESIDUAL 1 /* MGR <- MGB - L(MGX) */
+#define MAC3D_MG_RESTRICT 2 /* MGB[l+1] <- R(MGR[l]) */
+#define MAC3D_MG_PROLONG 3  /* MGX[l] += P(MGX[l+1]) */
+#define MAC3D_MG_COARSE 4   /* MGX[L] <- exact solve of L(x) = MGB[L] */
+#define MAC3D_MG_ZERO 5     /* MGX[l] <- 0 */
+
+/* kernel classes timed by mac3d_profile (HIP events on the stream) */
+#define MAC3D_KC_FACE_TO_CENTRE 0
+#define MAC3D_KC_ADVECT 1
+#define MAC3D_KC_CASSON 2
+#define MAC3D_KC_DIVERGENCE 3
+#define MAC3D_KC_SMOOTH_L0 4       /* one RBGS colour pass on level 0 */
+#define MAC3D_KC_RESTRICT_L0 5     /* level-0 residual and restriction */
+#define MAC3D_KC_PROLONG_L0 6      /* prolongation onto level 0 */
+#define MAC3D_KC_MG_COARSE_LEVELS 7 /* all multigrid work on levels >= 1 */
+#define MAC3D_KC_COARSE_SOLVE 8    /* exact coarsest-level solve */
+#define MAC3D_KC_PROJECT 9
+#define MAC3D_KC_DYE 10            /* dye predictor + resolve */
+#define MAC3D_KC_REFINE64 11       /* mixed: FP64 residual/update */
+#define MAC3D_KC_COUNT 12
+
+typedef struct mac3d_config {
+  int32_t nx, ny, nz;   /* level-0 cells; each must divide by 2^levels */
+  int32_t levels;       /* coarsenings below level 0 (3 => L0..L3) */
+  int32_t nu1, nu2;     /* pre-/post-smoothing RBGS sweeps, every level */
+  int32_t cycles;       /* V-cycles per step (FP32 and FP64 modes) */
+  int32_t precision;    /* MAC3D_PREC_* */
+  int32_t ir_max_iters; /* mixed: cap on refinement iterations per step */
+  int32_t device;       /* HIP device ordinal */
+  double dx;            /* cell size (m) */
+  double rho_w, mu_w;   /* ambient water density (kg/m^3), viscosity (Pa s) */
+  double rho_b;         /* blood density (kg/m^3) */
+  double tau_y, mu_inf; /* Casson yield stress (Pa), infinite-shear visc. */
+  double g;             /* gravity (m/s^2), acts along -y */
+  double gamma_reg;     /* Casson shear-rate regulariser (1/s) */
+  double shear_eps2;    /* (m/s)^2 added under the strain-rate root */
+  double alpha_max;     /* explicit viscous stability clamp on nu dt/h^2 */
+  double vel_damp;      /* velocity multiplier after advection */
+  double maccormack;    /* dye MacCormack correction strength */
+  double ir_tol;        /* mixed: relative divergence target ||r||/||b|| */
+} mac3d_config;
+
+/* one spherical blood blob of the seeded initial condition (cell units:
+ * cell (i,j,k) has its centre at (i+0.5, j+0.5, k+0.5)) */
+typedef struct mac3d_blob {
+  double cx, cy, cz; /* centre */
+  double radius;     /* radius; dye edge is a smoothstep over one cell */
+  double w0;         /* initial downward speed (m/s), applied to V faces */
+} mac3d_blob;
+
+typedef struct mac3d_stats {
+  int32_t steps;      /* steps completed by the call */
+  int32_t vcycles;    /* V-cycles run by the call, all steps */
+  int32_t ir_iters;   /* mixed: refinement iterations of the last step */
+  int32_t converged;  /* mixed: last step met ir_tol */
+  double max_div;     /* max |D(u)| after the last projection (m/s) */
+  double l2_div;      /* RMS of D(u) after the last projection (m/s) */
+  double max_div_rate;/* max |D(u)|/h (1/s) */
+  double rel_res;     /* ||r||_2/||b||_2 of the last pressure solve */
+  double ms;          /* wall time of the call (ms) */
+} mac3d_stats;
+
+/* replaces: constructor texture/pipeline setup, BloodDropEngine.ts:545-589,
+ * 861-897, 1100-1260, and the SimUniforms parameter block (common.ts:26-43,
+ * uniformLayout.ts:56-190) for the MAC step */
+int mac3d_default_config(mac3d_config *cfg);
+int mac3d_create(const mac3d_config *cfg, void **ctx_out);
+/* replaces: BloodDropEngine.destroy (BloodDropEngine.ts:2671-2736);
+ * idempotent on NULL */
+int mac3d_destroy(void *ctx);
+
+/* replaces: queue.writeBuffer / texture readback (BloodDropEngine.ts:2602,
+ * verify.mjs:560-606); count = elements of that field at that level */
+int mac3d_upload(void *ctx, int field, int level, const void *host, int dtype,
+                 size_t count);
+int mac3d_download(void *ctx, int field, int level, void *host, int dtype,
+                   size_t count);
+
+/* replaces: resetSimulation's FLUID_INIT_VEL + FLUID_CLEAR_R32 dispatches
+ * (BloodDropEngine.ts:1734-1744, fluid.ts:80-138) with the seeded blob
+ * initial condition: dye and V from the blobs, U = W = P = 0 */
+int mac3d_init_blobs(void *ctx, const mac3d_blob *blobs, int nblobs);
+
+/* replaces: encodePhysicsStep's MAC portion (BloodDropEngine.ts:1916-2069)
+ * plus CONC_PREDICT/CONC_RESOLVE's dye core (advect.ts:28-72,196-222),
+ * nsteps times at fixed dt, blocking until the stream completes.
+ * dt must be finite and > 0 (EINVAL otherwise, state untouched). */
+int mac3d_step(void *ctx, double dt, int nsteps, mac3d_stats *stats);
+
+/* one stage of the step (MAC3D_STAGE_*), blocking */
+int mac3d_stage(void *ctx, int stage, double dt);
+/* one multigrid operator (MAC3D_MG_*) on a level, blocking */
+int mac3d_mg_op(void *ctx, int op, int level, int count);
+/* max|D|, RMS(D) of the current velocity, FP64 reduction, blocking */
+int mac3d_div_norms(void *ctx, double *max_div, double *l2_div);
+
+/* kernel timing (no reference counterpart; the reference reads pass-level
+ * timestamp queries, BloodDropEngine.ts:411-439,2400-2456).  enable != 0
+ * clears the totals and brackets every launch with HIP events on the
+ * context's stream; read returns the class's total ms and launch count. */
+int mac3d_profile(void *ctx, int enable);
+int mac3d_profile_read(void *ctx, int kclass, double *ms_total, int64_t *launches);
+
+const char *mac3d_last_error(void *ctx);
+const char *mac3d_version(void);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* MAC3D_H */

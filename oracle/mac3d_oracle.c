@@ -1,0 +1,775 @@
+/*
+ * mac3d_oracle.c — FP64 CPU restatement of drop-of-blood's MAC step.
+ *
+ * TEST INFRASTRUCTURE ONLY: the checker for the HIP path.  Only tests, the
+ * smoke check and the CPU-baseline leg of the benchmark load it.
+ *
+ * Each function cites the reference kernel it restates.  Changes the MI355X
+ * workload makes to the reference scene are marked "WORKLOAD:" where they
+ * occur; the restatement is otherwise operation-for-operation:
+ *   - WebGPU textureSampleLevel with a linear clamp-to-edge sampler
+ *     (BloodDropEngine.ts:497-505) is restated with exact FP64 weights;
+ *   - velocity is held in FP64 here (rgba16float in the reference,
+ *     BloodDropEngine.ts:550-553).
+ * OpenMP parallelises the k loops; every loop body is independent (red-black
+ * colour passes only read the other colour), so results do not depend on the
+ * thread count.  Norm reductions sum per-plane partials in plane order.
+ */
+#include "mac3d_oracle.h"
+
+#include <math.h>
+#include <stdlib.h>
+#include <string.h>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+#define MAXLEV 8
+
+enum { F_U, F_V, F_W, F_P, F_DYE, F_CU, F_CV, F_CW, F_RHS, F_MGX, F_MGB,
+       F_MGR, F_DYE_PRED, F_COUNT };
+enum { ST_F2C, ST_ADVECT, ST_CASSON, ST_DIV, ST_VCYCLE, ST_PROJECT,
+       ST_DYE_PRED, ST_DYE_RES };
+enum { MG_SMOOTH, MG_RESIDUAL, MG_RESTRICT, MG_PROLONG, MG_COARSE, MG_ZERO };
+
+typedef struct { int nx, ny, nz; } dims3;
+
+struct orc_state {
+  orc_params p;
+  dims3 d[MAXLEV + 1];
+  long n[MAXLEV + 1];
+  double *u, *v, *w, *pr, *dye, *cu, *cv, *cw, *rhs, *dpred;
+  double *u2, *v2, *w2;
+  double *mgx[MAXLEV + 1], *mgb[MAXLEV + 1], *mgr[MAXLEV + 1];
+  double *cg[4]; /* coarse-solve CG vectors */
+};
+
+#define IDX(D, i, j, k) ((((long)(k)) * (D).ny + (j)) * (long)(D).nx + (i))
+
+static double clampd(double x, double lo, double hi) {
+  return x < lo ? lo : (x > hi ? hi : x);
+}
+static int clampi(int x, int lo, int hi) { return x < lo ? lo : (x > hi ? hi : x); }
+static double mixd(double a, double b, double t) { return a * (1.0 - t) + b * t; }
+
+double orc_smoothstep(double e0, double e1, double x) {
+  /* WGSL smoothstep (used at fluid.ts:74, advect.ts and the blob edge) */
+  double t = clampd((x - e0) / (e1 - e0), 0.0, 1.0);
+  return t * t * (3.0 - 2.0 * t);
+}
+
+int orc_set_threads(int n) {
+#ifdef _OPENMP
+  if (n > 0) omp_set_num_threads(n);
+  return omp_get_max_threads();
+#else
+  (void)n;
+  return 1;
+#endif
+}
+
+/* textureSampleLevel(tex, linear clamp-to-edge sampler, uvw): texel space
+ * t = uvw*N - 0.5, floor/frac, clamped texel indices, x then y then z lerp.
+ * Used by sampleMacU/V/W and traceBackUVW (fluid.ts:154-174) and the dye
+ * predictor (advect.ts:52-69). */
+static double sample_cell(const double *f, dims3 d, double ux, double uy, double uz) {
+  double tx = ux * d.nx - 0.5, ty = uy * d.ny - 0.5, tz = uz * d.nz - 0.5;
+  double bx = floor(tx), by = floor(ty), bz = floor(tz);
+  double fx = tx - bx, fy = ty - by, fz = tz - bz;
+  int i0 = clampi((int)bx, 0, d.nx - 1), i1 = clampi((int)bx + 1, 0, d.nx - 1);
+  int j0 = clampi((int)by, 0, d.ny - 1), j1 = clampi((int)by + 1, 0, d.ny - 1);
+  int k0 = clampi((int)bz, 0, d.nz - 1), k1 = clampi((int)bz + 1, 0, d.nz - 1);
+  double c000 = f[IDX(d, i0, j0, k0)], c100 = f[IDX(d, i1, j0, k0)];
+  double c010 = f[IDX(d, i0, j1, k0)], c110 = f[IDX(d, i1, j1, k0)];
+  double c001 = f[IDX(d, i0, j0, k1)], c101 = f[IDX(d, i1, j0, k1)];
+  double c011 = f[IDX(d, i0, j1, k1)], c111 = f[IDX(d, i1, j1, k1)];
+  double x00 = mixd(c000, c100, fx), x10 = mixd(c010, c110, fx);
+  double x01 = mixd(c001, c101, fx), x11 = mixd(c011, c111, fx);
+  return mixd(mixd(x00, x10, fy), mixd(x01, x11, fy), fz);
+}
+
+double orc_sample_cell(const double *f, int nx, int ny, int nz, double ux,
+                       double uy, double uz) {
+  dims3 d = {nx, ny, nz};
+  return sample_cell(f, d, ux, uy, uz);
+}
+
+/* ---- face -> centre: FLUID_MAC_TO_CENTER (fluid.ts:917-943); the
+ * scene-specific top-row stagnation gate (fluid.ts:944-959) is not part of
+ * the workload. */
+static void face_to_centre(orc_state *s) {
+  dims3 d = s->d[0];
+#pragma omp parallel for
+  for (int k = 0; k < d.nz; ++k)
+    for (int j = 0; j < d.ny; ++j)
+      for (int i = 0; i < d.nx; ++i) {
+        long c = IDX(d, i, j, k);
+        double uR = (i >= d.nx - 1) ? 0.0 : s->u[c];
+        double uL = (i == 0) ? 0.0 : s->u[IDX(d, i - 1, j, k)];
+        double vU = s->v[c];
+        double vD = (j == 0) ? 0.0 : s->v[IDX(d, i, j - 1, k)];
+        double wF = (k >= d.nz - 1) ? 0.0 : s->w[c];
+        double wB = (k == 0) ? 0.0 : s->w[IDX(d, i, j, k - 1)];
+        s->cu[c] = 0.5 * (uL + uR);
+        s->cv[c] = 0.5 * (vD + vU);
+        s->cw[c] = 0.5 * (wB + wF);
+      }
+}
+
+/* traceBackUVW (fluid.ts:169-174): RK2 midpoint backtrace on the
+ * cell-centred velocity, uvw clamped to [0.002, 0.998]. */
+static void trace_back(const orc_state *s, double dt, const double uvw[3],
+                       double out[3]) {
+  dims3 d = s->d[0];
+  double dom[3] = {d.nx * s->p.dx, d.ny * s->p.dx, d.nz * s->p.dx};
+  double p0[3], mid[3], v0[3], vm[3];
+  for (int a = 0; a < 3; ++a) p0[a] = clampd(uvw[a], 0.002, 0.998);
+  v0[0] = sample_cell(s->cu, d, p0[0], p0[1], p0[2]);
+  v0[1] = sample_cell(s->cv, d, p0[0], p0[1], p0[2]);
+  v0[2] = sample_cell(s->cw, d, p0[0], p0[1], p0[2]);
+  for (int a = 0; a < 3; ++a) mid[a] = clampd(uvw[a] - 0.5 * (v0[a] * dt) / dom[a], 0.002, 0.998);
+  vm[0] = sample_cell(s->cu, d, mid[0], mid[1], mid[2]);
+  vm[1] = sample_cell(s->cv, d, mid[0], mid[1], mid[2]);
+  vm[2] = sample_cell(s->cw, d, mid[0], mid[1], mid[2]);
+  for (int a = 0; a < 3; ++a) out[a] = clampd(uvw[a] - (vm[a] * dt) / dom[a], 0.002, 0.998);
+}
+
+/* ---- FLUID_ADVECT_MAC (fluid.ts:144-192, 323-325): semi-Lagrangian RK2
+ * advection of each staggered component, sampled with the -1/2 texel shift
+ * of sampleMacU/V/W (fluid.ts:154-167), times velDamp.  WORKLOAD: the scene
+ * forcing blocks (drop necks :198-285, Marangoni/kinematic top :287-310,
+ * sponges :312-321) are not part of the workload. */
+static void advect_mac(orc_state *s, double dt) {
+  dims3 d = s->d[0];
+  double damp = s->p.vel_damp;
+#pragma omp parallel for
+  for (int k = 0; k < d.nz; ++k)
+    for (int j = 0; j < d.ny; ++j)
+      for (int i = 0; i < d.nx; ++i) {
+        long c = IDX(d, i, j, k);
+        double fX[3] = {(i + 1.0) / d.nx, (j + 0.5) / d.ny, (k + 0.5) / d.nz};
+        double fY[3] = {(i + 0.5) / d.nx, (j + 1.0) / d.ny, (k + 0.5) / d.nz};
+        double fZ[3] = {(i + 0.5) / d.nx, (j + 0.5) / d.ny, (k + 1.0) / d.nz};
+        double b[3];
+        trace_back(s, dt, fX, b);
+        double vx = sample_cell(s->u, d, clampd(b[0] - 0.5 / d.nx, 0.001, 0.999),
+                                clampd(b[1], 0.001, 0.999), clampd(b[2], 0.001, 0.999)) * damp;
+        trace_back(s, dt, fY, b);
+        double vy = sample_cell(s->v, d, clampd(b[0], 0.001, 0.999),
+                                clampd(b[1] - 0.5 / d.ny, 0.001, 0.999), clampd(b[2], 0.001, 0.999)) * damp;
+        trace_back(s, dt, fZ, b);
+        double vz = sample_cell(s->w, d, clampd(b[0], 0.001, 0.999), clampd(b[1], 0.001, 0.999),
+                                clampd(b[2] - 0.5 / d.nz, 0.001, 0.999)) * damp;
+        if (i >= d.nx - 1) vx = 0.0;
+        if (k >= d.nz - 1) vz = 0.0;
+        s->u2[c] = vx; s->v2[c] = vy; s->w2[c] = vz;
+      }
+  double *t;
+  t = s->u; s->u = s->u2; s->u2 = t;
+  t = s->v; s->v = s->v2; s->v2 = t;
+  t = s->w; s->w = s->w2; s->w2 = t;
+}
+
+/* Neighbour face values with the workload's wall rules (used by the viscous
+ * stencil).  WORKLOAD: the reference clamps indices to the edge
+ * (loadMac, fluid.ts:387-392), i.e. zero-gradient walls; the workload has
+ * no-slip walls (tangential ghost = -value, normal wall face = 0) and a
+ * free-slip lid (ghost = value). */
+static double U_at(const orc_state *s, int i, int j, int k) {
+  dims3 d = s->d[0];
+  if (i < 0 || i >= d.nx) return 0.0;
+  if (j < 0) return -s->u[IDX(d, i, 0, k)];
+  if (j >= d.ny) return s->u[IDX(d, i, d.ny - 1, k)];
+  if (k < 0) return -s->u[IDX(d, i, j, 0)];
+  if (k >= d.nz) return -s->u[IDX(d, i, j, d.nz - 1)];
+  return s->u[IDX(d, i, j, k)];
+}
+static double V_at(const orc_state *s, int i, int j, int k) {
+  dims3 d = s->d[0];
+  if (j < 0) return 0.0;
+  if (j >= d.ny) return s->v[IDX(d, i, d.ny - 1, k)];
+  if (i < 0) return -s->v[IDX(d, 0, j, k)];
+  if (i >= d.nx) return -s->v[IDX(d, d.nx - 1, j, k)];
+  if (k < 0) return -s->v[IDX(d, i, j, 0)];
+  if (k >= d.nz) return -s->v[IDX(d, i, j, d.nz - 1)];
+  return s->v[IDX(d, i, j, k)];
+}
+static double W_at(const orc_state *s, int i, int j, int k) {
+  dims3 d = s->d[0];
+  if (k < 0 || k >= d.nz) return 0.0;
+  if (i < 0) return -s->w[IDX(d, 0, j, k)];
+  if (i >= d.nx) return -s->w[IDX(d, d.nx - 1, j, k)];
+  if (j < 0) return -s->w[IDX(d, i, 0, k)];
+  if (j >= d.ny) return s->w[IDX(d, i, d.ny - 1, k)];
+  return s->w[IDX(d, i, j, k)];
+}
+
+/* ---- FLUID_CASSON_FORCES (fluid.ts:377-511), parts (1) and (3):
+ * strain-rate proxy from +-1 neighbour differences (fluid.ts:451-454),
+ * Casson effective viscosity with the max(gamma+reg, reg) floor
+ * (fluid.ts:458-462), explicit update mac += alpha*(sum6 - 6 mac)
+ * (fluid.ts:464-465), buoyancy on y-faces from the face-sampled dye
+ * (fluid.ts:477-486), wall faces zeroed (fluid.ts:506-507).
+ * WORKLOAD: SI Casson parameters blended by dye c in [0,1]:
+ *   mu_inf(c) = mu_w + c (mu_inf - mu_w), tau_y(c) = tau_y c^2 (the
+ *   reference's c^2 yield law, fluid.ts:460), rho(c) = rho_w + c (rho_b -
+ *   rho_w), gamma = |diff|/(2h) (the reference's 50 = 1/(2*0.01 m)),
+ *   alpha = clamp(mu_eff/rho dt/h^2, 0, alpha_max);
+ *   linear Boussinesq a_y = -g (rho_b - rho_w)/rho_w * dye_face in place of
+ *   the tanh-saturated surface-gated term; vorticity confinement (2) and
+ *   unfurling forcing (4) and the +-1.8 clamp are not part of the workload. */
+static void casson_buoyancy(orc_state *s, double dt) {
+  dims3 d = s->d[0];
+  const orc_params *P = &s->p;
+  double h = P->dx;
+  double inv2h = 1.0 / (2.0 * h);
+  double dtoh2 = dt / (h * h);
+  double buoy = -P->g * (P->rho_b - P->rho_w) / P->rho_w * dt;
+#pragma omp parallel for
+  for (int k = 0; k < d.nz; ++k)
+    for (int j = 0; j < d.ny; ++j)
+      for (int i = 0; i < d.nx; ++i) {
+        long c = IDX(d, i, j, k);
+        double m[3] = {s->u[c], s->v[c], s->w[c]};
+        double xp[3] = {U_at(s, i + 1, j, k), V_at(s, i + 1, j, k), W_at(s, i + 1, j, k)};
+        double xn[3] = {U_at(s, i - 1, j, k), V_at(s, i - 1, j, k), W_at(s, i - 1, j, k)};
+        double yp[3] = {U_at(s, i, j + 1, k), V_at(s, i, j + 1, k), W_at(s, i, j + 1, k)};
+        double yn[3] = {U_at(s, i, j - 1, k), V_at(s, i, j - 1, k), W_at(s, i, j - 1, k)};
+        double zp[3] = {U_at(s, i, j, k + 1), V_at(s, i, j, k + 1), W_at(s, i, j, k + 1)};
+        double zn[3] = {U_at(s, i, j, k - 1), V_at(s, i, j, k - 1), W_at(s, i, j, k - 1)};
+        double ss = 0.0;
+        for (int a = 0; a < 3; ++a) {
+          double du = xp[a] - xn[a], dv = yp[a] - yn[a], dw = zp[a] - zn[a];
+          ss += du * du + dv * dv + dw * dw;
+        }
+        double gam = inv2h * sqrt(ss + P->shear_eps2);
+        double cc = clampd(s->dye[c], 0.0, 1.0);
+        double sq_muinf = sqrt(P->mu_w + cc * (P->mu_inf - P->mu_w));
+        double tau = P->tau_y * cc * cc;
+        double yterm = sqrt(tau / fmax(gam + P->gamma_reg, P->gamma_reg));
+        double mu_eff = (sq_muinf + yterm) * (sq_muinf + yterm);
+        double rho = P->rho_w + cc * (P->rho_b - P->rho_w);
+        double alpha = clampd(mu_eff / rho * dtoh2, 0.0, P->alpha_max);
+        double out[3];
+        for (int a = 0; a < 3; ++a) {
+          double lap = (xp[a] + xn[a] + yp[a] + yn[a] + zp[a] + zn[a]) - 6.0 * m[a];
+          out[a] = m[a] + alpha * lap;
+        }
+        int jp = j + 1 < d.ny ? j + 1 : d.ny - 1;
+        double dye_face = 0.5 * (s->dye[c] + s->dye[IDX(d, i, jp, k)]);
+        out[1] += buoy * dye_face;
+        if (i >= d.nx - 1) out[0] = 0.0;
+        if (k >= d.nz - 1) out[2] = 0.0;
+        s->u2[c] = out[0]; s->v2[c] = out[1]; s->w2[c] = out[2];
+      }
+  double *t;
+  t = s->u; s->u = s->u2; s->u2 = t;
+  t = s->v; s->v = s->v2; s->v2 = t;
+  t = s->w; s->w = s->w2; s->w2 = t;
+}
+
+/* ---- FLUID_MAC_RHS (fluid.ts:517-555): D = (uR-uL)+(vT-vB)+(wF-wB),
+ * solid wall faces 0, top face free.  WORKLOAD: flat lid, p_top = 0
+ * (evalTopDirichletPressure fluid.ts:16-78 needs the wave solver). */
+static void divergence(const orc_state *s, const double *u, const double *v,
+                       const double *w, double *out) {
+  dims3 d = s->d[0];
+#pragma omp parallel for
+  for (int k = 0; k < d.nz; ++k)
+    for (int j = 0; j < d.ny; ++j)
+      for (int i = 0; i < d.nx; ++i) {
+        long c = IDX(d, i, j, k);
+        double uR = (i >= d.nx - 1) ? 0.0 : u[c];
+        double uL = (i == 0) ? 0.0 : u[IDX(d, i - 1, j, k)];
+        double vT = v[c];
+        double vB = (j == 0) ? 0.0 : v[IDX(d, i, j - 1, k)];
+        double wF = (k >= d.nz - 1) ? 0.0 : w[c];
+        double wB = (k == 0) ? 0.0 : w[IDX(d, i, j, k - 1)];
+        out[c] = (uR - uL) + (vT - vB) + (wF - wB);
+      }
+}
+
+/* ---- evalMgStencil (fluid.ts:561-603): neighbour sum with Neumann walls
+ * (neighbour and diagonal term dropped) and the top face always in the
+ * diagonal (Dirichlet 0 above the top row).  WORKLOAD: on level l the top
+ * row's diagonal term is topc(l) = 2^(l+1)/(2^l + 1) instead of 1, which puts
+ * the zero-pressure plane half a level-0 cell above the top row on every
+ * level (linear ghost extrapolation); level 0 keeps the reference's 1.  With
+ * the reference's 1 on every level the plane rises by one coarse cell per
+ * level and the V-cycle stalls near 0.7 per cycle. */
+static double level_h2(int l) { double s = (double)(1 << l); return s * s; }
+
+double orc_top_coeff(int level) {
+  double s = (double)(1 << level);
+  return 2.0 * s / (s + 1.0);
+}
+
+static void stencil(const double *x, dims3 d, double topc, int i, int j, int k,
+                    double *sum, double *diag) {
+  double sn = 0.0, dg = (j < d.ny - 1) ? 1.0 : topc;
+  if (i > 0) { sn += x[IDX(d, i - 1, j, k)]; dg += 1.0; }
+  if (i < d.nx - 1) { sn += x[IDX(d, i + 1, j, k)]; dg += 1.0; }
+  if (j > 0) { sn += x[IDX(d, i, j - 1, k)]; dg += 1.0; }
+  if (j < d.ny - 1) { sn += x[IDX(d, i, j + 1, k)]; }
+  if (k > 0) { sn += x[IDX(d, i, j, k - 1)]; dg += 1.0; }
+  if (k < d.nz - 1) { sn += x[IDX(d, i, j, k + 1)]; dg += 1.0; }
+  *sum = sn;
+  *diag = dg;
+}
+
+
+/* ---- MG_RBGS_RED / MG_RBGS_BLACK (fluid.ts:645-681): colour
+ * (i+j+k)&1 == parity, x <- (sum - h^2 b)/diag.  The reference ping-pongs
+ * two textures per pass, copying the other colour; updating in place is the
+ * same map because a colour only reads the other colour. */
+static void rbgs_pass(double *x, const double *b, dims3 d, double h2, double topc,
+                      int parity) {
+#pragma omp parallel for
+  for (int k = 0; k < d.nz; ++k)
+    for (int j = 0; j < d.ny; ++j)
+      for (int i = ((parity + j + k) & 1); i < d.nx; i += 2) {
+        double sn, dg;
+        stencil(x, d, topc, i, j, k, &sn, &dg);
+        x[IDX(d, i, j, k)] = (sn - h2 * b[IDX(d, i, j, k)]) / dg;
+      }
+}
+
+static void smooth(double *x, const double *b, dims3 d, int l, int sweeps) {
+  double h2 = level_h2(l), topc = orc_top_coeff(l);
+  for (int s = 0; s < sweeps; ++s) {
+    rbgs_pass(x, b, d, h2, topc, 0);
+    rbgs_pass(x, b, d, h2, topc, 1);
+  }
+}
+
+/* ---- MG_RESIDUAL (fluid.ts:744-770): r = b - (sum - diag x)/h^2 */
+static void residual(const double *x, const double *b, double *r, dims3 d, int l) {
+  double h2 = level_h2(l), topc = orc_top_coeff(l);
+#pragma omp parallel for
+  for (int k = 0; k < d.nz; ++k)
+    for (int j = 0; j < d.ny; ++j)
+      for (int i = 0; i < d.nx; ++i) {
+        double sn, dg;
+        stencil(x, d, topc, i, j, k, &sn, &dg);
+        long c = IDX(d, i, j, k);
+        r[c] = b[c] - (sn - dg * x[c]) / h2;
+      }
+}
+
+/* ---- MG_RESTRICT (fluid.ts:775-800): coarse = 1/8 sum of the 2x2x2
+ * children */
+static void restrict_op(const double *rf, dims3 df, double *bc, dims3 dc) {
+#pragma omp parallel for
+  for (int k = 0; k < dc.nz; ++k)
+    for (int j = 0; j < dc.ny; ++j)
+      for (int i = 0; i < dc.nx; ++i) {
+        double sum = 0.0;
+        for (int dz = 0; dz < 2; ++dz)
+          for (int dy = 0; dy < 2; ++dy)
+            for (int dx = 0; dx < 2; ++dx)
+              sum += rf[IDX(df, 2 * i + dx, 2 * j + dy, 2 * k + dz)];
+        bc[IDX(dc, i, j, k)] = sum * 0.125;
+      }
+}
+
+/* ---- MG_PROLONG_ADD (fluid.ts:806-855): cell-centred trilinear
+ * interpolation of the coarse error, indices clamped at the walls and 0
+ * above the top row (loadCoarseErr :814-821). */
+static double coarse_err(const double *e, dims3 dc, int i, int j, int k) {
+  if (j >= dc.ny) return 0.0;
+  i = clampi(i, 0, dc.nx - 1);
+  j = clampi(j, 0, dc.ny - 1);
+  k = clampi(k, 0, dc.nz - 1);
+  return e[IDX(dc, i, j, k)];
+}
+
+static void prolong_add(double *xf, dims3 df, const double *ec, dims3 dc) {
+#pragma omp parallel for
+  for (int k = 0; k < df.nz; ++k)
+    for (int j = 0; j < df.ny; ++j)
+      for (int i = 0; i < df.nx; ++i) {
+        double t[3] = {(i + 0.5) / df.nx * dc.nx - 0.5, (j + 0.5) / df.ny * dc.ny - 0.5,
+                       (k + 0.5) / df.nz * dc.nz - 0.5};
+        int b[3];
+        double f[3];
+        for (int a = 0; a < 3; ++a) {
+          double fl = floor(t[a]);
+          b[a] = (int)fl;
+          f[a] = t[a] - fl;
+        }
+        double c000 = coarse_err(ec, dc, b[0], b[1], b[2]);
+        double c100 = coarse_err(ec, dc, b[0] + 1, b[1], b[2]);
+        double c010 = coarse_err(ec, dc, b[0], b[1] + 1, b[2]);
+        double c110 = coarse_err(ec, dc, b[0] + 1, b[1] + 1, b[2]);
+        double c001 = coarse_err(ec, dc, b[0], b[1], b[2] + 1);
+        double c101 = coarse_err(ec, dc, b[0] + 1, b[1], b[2] + 1);
+        double c011 = coarse_err(ec, dc, b[0], b[1] + 1, b[2] + 1);
+        double c111 = coarse_err(ec, dc, b[0] + 1, b[1] + 1, b[2] + 1);
+        double x00 = mixd(c000, c100, f[0]), x10 = mixd(c010, c110, f[0]);
+        double x01 = mixd(c001, c101, f[0]), x11 = mixd(c011, c111, f[0]);
+        double corr = mixd(mixd(x00, x10, f[1]), mixd(x01, x11, f[1]), f[2]);
+        xf[IDX(df, i, j, k)] += corr;
+      }
+}
+
+/* ---- coarsest-level solve.  The reference runs 6 RBGS sweeps from zero in
+ * one workgroup (MG_COARSE_SOLVE_L3, fluid.ts:687-739).  WORKLOAD: an exact
+ * FP64 solve of (sum - diag x) = h^2 b.  The oracle solves it with
+ * Jacobi-preconditioned conjugate gradients to a 1e-14 relative residual,
+ * a method independent of the device's eigen-decomposition solve. */
+static void apply_A(const double *x, double *y, dims3 d, double topc) {
+  /* A = diag - neighbours (symmetric positive definite) */
+#pragma omp parallel for
+  for (int k = 0; k < d.nz; ++k)
+    for (int j = 0; j < d.ny; ++j)
+      for (int i = 0; i < d.nx; ++i) {
+        double sn, dg;
+        stencil(x, d, topc, i, j, k, &sn, &dg);
+        y[IDX(d, i, j, k)] = dg * x[IDX(d, i, j, k)] - sn;
+      }
+}
+
+static void coarse_solve(orc_state *s, int L) {
+  dims3 d = s->d[L];
+  long n = s->n[L];
+  double h2 = level_h2(L), topc = orc_top_coeff(L);
+  double *x = s->mgx[L], *r = s->cg[0], *z = s->cg[1], *q = s->cg[2], *Aq = s->cg[3];
+  const double *b = s->mgb[L];
+  /* A x = -h^2 b */
+  double bn = 0.0;
+  for (long c = 0; c < n; ++c) { x[c] = 0.0; r[c] = -h2 * b[c]; bn += r[c] * r[c]; }
+  if (bn == 0.0) return;
+  for (int k = 0; k < d.nz; ++k)
+    for (int j = 0; j < d.ny; ++j)
+      for (int i = 0; i < d.nx; ++i) {
+        double sn, dg;
+        stencil(r, d, topc, i, j, k, &sn, &dg);
+        long c = IDX(d, i, j, k);
+        z[c] = r[c] / dg;
+      }
+  memcpy(q, z, sizeof(double) * n);
+  double rz = 0.0;
+  for (long c = 0; c < n; ++c) rz += r[c] * z[c];
+  for (int it = 0; it < 20000; ++it) {
+    apply_A(q, Aq, d, topc);
+    double qAq = 0.0;
+    for (long c = 0; c < n; ++c) qAq += q[c] * Aq[c];
+    double a = rz / qAq;
+    double rn = 0.0;
+    for (long c = 0; c < n; ++c) { x[c] += a * q[c]; r[c] -= a * Aq[c]; rn += r[c] * r[c]; }
+    if (rn <= 1e-28 * bn) break;
+    double rz2 = 0.0;
+    for (int k = 0; k < d.nz; ++k)
+      for (int j = 0; j < d.ny; ++j)
+        for (int i = 0; i < d.nx; ++i) {
+          double sn, dg;
+          stencil(r, d, topc, i, j, k, &sn, &dg);
+          long c = IDX(d, i, j, k);
+          z[c] = r[c] / dg;
+          rz2 += r[c] * z[c];
+        }
+    double beta = rz2 / rz;
+    rz = rz2;
+    for (long c = 0; c < n; ++c) q[c] = z[c] + beta * q[c];
+  }
+}
+
+/* ---- one V-cycle: the dispatch schedule of BloodDropEngine.ts:1942-2059
+ * (smooth, residual, restrict down; coarse solve; prolong, smooth up; the
+ * level>=1 unknowns start from zero as MG_RBGS_RED_ZERO, fluid.ts:605-641).
+ * WORKLOAD: nu1/nu2 sweeps on every level (reference: 1/1 on L0, 2/2 below),
+ * exact coarse solve. */
+static void vcycle(orc_state *s) {
+  int L = s->p.levels;
+  smooth(s->pr, s->rhs, s->d[0], 0, s->p.nu1);
+  residual(s->pr, s->rhs, s->mgr[0], s->d[0], 0);
+  restrict_op(s->mgr[0], s->d[0], s->mgb[1], s->d[1]);
+  for (int l = 1; l < L; ++l) {
+    memset(s->mgx[l], 0, sizeof(double) * s->n[l]);
+    smooth(s->mgx[l], s->mgb[l], s->d[l], l, s->p.nu1);
+    residual(s->mgx[l], s->mgb[l], s->mgr[l], s->d[l], l);
+    restrict_op(s->mgr[l], s->d[l], s->mgb[l + 1], s->d[l + 1]);
+  }
+  coarse_solve(s, L);
+  for (int l = L - 1; l >= 1; --l) {
+    prolong_add(s->mgx[l], s->d[l], s->mgx[l + 1], s->d[l + 1]);
+    smooth(s->mgx[l], s->mgb[l], s->d[l], l, s->p.nu2);
+  }
+  prolong_add(s->pr, s->d[0], s->mgx[1], s->d[1]);
+  smooth(s->pr, s->rhs, s->d[0], 0, s->p.nu2);
+}
+
+static double rel_residual(orc_state *s) {
+  residual(s->pr, s->rhs, s->mgr[0], s->d[0], 0);
+  double rn = 0.0, bn = 0.0;
+  for (long c = 0; c < s->n[0]; ++c) {
+    rn += s->mgr[0][c] * s->mgr[0][c];
+    bn += s->rhs[c] * s->rhs[c];
+  }
+  return bn > 0.0 ? sqrt(rn / bn) : 0.0;
+}
+
+/* ---- FLUID_MAC_PROJECT (fluid.ts:863-911): face -= p(+1) - p, wall faces
+ * 0, top face uses the lid ghost (p_top = 0 for the flat lid).  WORKLOAD: the
+ * reference's +-2 clamp (lattice units) is not part of the workload. */
+static void project(orc_state *s) {
+  dims3 d = s->d[0];
+#pragma omp parallel for
+  for (int k = 0; k < d.nz; ++k)
+    for (int j = 0; j < d.ny; ++j)
+      for (int i = 0; i < d.nx; ++i) {
+        long c = IDX(d, i, j, k);
+        double pc = s->pr[c];
+        if (i < d.nx - 1) s->u[c] -= s->pr[IDX(d, i + 1, j, k)] - pc; else s->u[c] = 0.0;
+        if (j < d.ny - 1) s->v[c] -= s->pr[IDX(d, i, j + 1, k)] - pc; else s->v[c] -= 0.0 - pc;
+        if (k < d.nz - 1) s->w[c] -= s->pr[IDX(d, i, j, k + 1)] - pc; else s->w[c] = 0.0;
+      }
+}
+
+/* ---- CONC_PREDICT (advect.ts:28-72): RK2 semi-Lagrangian predictor on the
+ * centred velocity, uvw clamped to [0.001, 0.999].  WORKLOAD: the dye is a
+ * cell field of the MAC grid; the static / single-stage speed shortcuts
+ * (advect.ts:54-63) and the wall/top zeroing (advect.ts:42-45) are not part
+ * of the workload. */
+static void dye_predict(orc_state *s, double dt) {
+  dims3 d = s->d[0];
+  double dom[3] = {d.nx * s->p.dx, d.ny * s->p.dx, d.nz * s->p.dx};
+#pragma omp parallel for
+  for (int k = 0; k < d.nz; ++k)
+    for (int j = 0; j < d.ny; ++j)
+      for (int i = 0; i < d.nx; ++i) {
+        double uvw[3] = {(i + 0.5) / d.nx, (j + 0.5) / d.ny, (k + 0.5) / d.nz};
+        double v0[3] = {sample_cell(s->cu, d, uvw[0], uvw[1], uvw[2]),
+                        sample_cell(s->cv, d, uvw[0], uvw[1], uvw[2]),
+                        sample_cell(s->cw, d, uvw[0], uvw[1], uvw[2])};
+        double mid[3], back[3];
+        for (int a = 0; a < 3; ++a) mid[a] = clampd(uvw[a] - 0.5 * (v0[a] * dt) / dom[a], 0.001, 0.999);
+        double vm[3] = {sample_cell(s->cu, d, mid[0], mid[1], mid[2]),
+                        sample_cell(s->cv, d, mid[0], mid[1], mid[2]),
+                        sample_cell(s->cw, d, mid[0], mid[1], mid[2])};
+        for (int a = 0; a < 3; ++a) back[a] = clampd(uvw[a] - (vm[a] * dt) / dom[a], 0.001, 0.999);
+        s->dpred[IDX(d, i, j, k)] = sample_cell(s->dye, d, back[0], back[1], back[2]);
+      }
+}
+
+/* ---- CONC_RESOLVE's MacCormack core (advect.ts:196-222): forward
+ * estimate from the predictor at uvw + v0 dt, correction
+ * pred + strength (orig - estimate), clamped to the min/max of the 8 source
+ * texels around the backtrace point.  WORKLOAD: the diffusion/dilution and
+ * parcel/bed/oxygenation parts (advect.ts:224-296) are not part of it. */
+static void dye_resolve(orc_state *s, double dt) {
+  dims3 d = s->d[0];
+  double dom[3] = {d.nx * s->p.dx, d.ny * s->p.dx, d.nz * s->p.dx};
+  double mc = s->p.maccormack;
+  double *out = s->u2; /* scratch */
+#pragma omp parallel for
+  for (int k = 0; k < d.nz; ++k)
+    for (int j = 0; j < d.ny; ++j)
+      for (int i = 0; i < d.nx; ++i) {
+        long c = IDX(d, i, j, k);
+        double uvw[3] = {(i + 0.5) / d.nx, (j + 0.5) / d.ny, (k + 0.5) / d.nz};
+        double v0[3] = {sample_cell(s->cu, d, uvw[0], uvw[1], uvw[2]),
+                        sample_cell(s->cv, d, uvw[0], uvw[1], uvw[2]),
+                        sample_cell(s->cw, d, uvw[0], uvw[1], uvw[2])};
+        double mid[3], back[3], fwd[3];
+        for (int a = 0; a < 3; ++a) mid[a] = clampd(uvw[a] - 0.5 * (v0[a] * dt) / dom[a], 0.001, 0.999);
+        double vm[3] = {sample_cell(s->cu, d, mid[0], mid[1], mid[2]),
+                        sample_cell(s->cv, d, mid[0], mid[1], mid[2]),
+                        sample_cell(s->cw, d, mid[0], mid[1], mid[2])};
+        for (int a = 0; a < 3; ++a) {
+          back[a] = clampd(uvw[a] - (vm[a] * dt) / dom[a], 0.001, 0.999);
+          fwd[a] = clampd(uvw[a] + (v0[a] * dt) / dom[a], 0.001, 0.999);
+        }
+        double est = sample_cell(s->dpred, d, fwd[0], fwd[1], fwd[2]);
+        double corr = s->dpred[c] + mc * (s->dye[c] - est);
+        int bi = (int)floor(back[0] * d.nx - 0.5);
+        int bj = (int)floor(back[1] * d.ny - 0.5);
+        int bk = (int)floor(back[2] * d.nz - 0.5);
+        double lo = 1e300, hi = -1e300;
+        for (int dz = 0; dz < 2; ++dz)
+          for (int dy = 0; dy < 2; ++dy)
+            for (int dx = 0; dx < 2; ++dx) {
+              double sv = s->dye[IDX(d, clampi(bi + dx, 0, d.nx - 1), clampi(bj + dy, 0, d.ny - 1),
+                                     clampi(bk + dz, 0, d.nz - 1))];
+              lo = fmin(lo, sv);
+              hi = fmax(hi, sv);
+            }
+        out[c] = clampd(corr, lo, hi);
+      }
+  memcpy(s->dye, out, sizeof(double) * s->n[0]);
+}
+
+/* ---- state ---- */
+orc_state *orc_create(const orc_params *p) {
+  if (p->levels < 1 || p->levels > MAXLEV) return NULL;
+  orc_state *s = (orc_state *)calloc(1, sizeof(orc_state));
+  s->p = *p;
+  for (int l = 0; l <= p->levels; ++l) {
+    s->d[l].nx = p->nx >> l;
+    s->d[l].ny = p->ny >> l;
+    s->d[l].nz = p->nz >> l;
+    s->n[l] = (long)s->d[l].nx * s->d[l].ny * s->d[l].nz;
+  }
+  long n = s->n[0];
+  double **f0[] = {&s->u, &s->v, &s->w, &s->pr, &s->dye, &s->cu, &s->cv, &s->cw,
+                   &s->rhs, &s->dpred, &s->u2, &s->v2, &s->w2};
+  for (unsigned a = 0; a < sizeof(f0) / sizeof(f0[0]); ++a) *f0[a] = (double *)calloc(n, sizeof(double));
+  s->mgx[0] = s->pr;
+  s->mgb[0] = s->rhs;
+  s->mgr[0] = (double *)calloc(n, sizeof(double));
+  for (int l = 1; l <= p->levels; ++l) {
+    s->mgx[l] = (double *)calloc(s->n[l], sizeof(double));
+    s->mgb[l] = (double *)calloc(s->n[l], sizeof(double));
+    s->mgr[l] = (double *)calloc(s->n[l], sizeof(double));
+  }
+  for (int a = 0; a < 4; ++a) s->cg[a] = (double *)calloc(s->n[p->levels], sizeof(double));
+  return s;
+}
+
+void orc_destroy(orc_state *s) {
+  if (!s) return;
+  double *f0[] = {s->u, s->v, s->w, s->pr, s->dye, s->cu, s->cv, s->cw, s->rhs, s->dpred,
+                  s->u2, s->v2, s->w2, s->mgr[0]};
+  for (unsigned a = 0; a < sizeof(f0) / sizeof(f0[0]); ++a) free(f0[a]);
+  for (int l = 1; l <= s->p.levels; ++l) { free(s->mgx[l]); free(s->mgb[l]); free(s->mgr[l]); }
+  for (int a = 0; a < 4; ++a) free(s->cg[a]);
+  free(s);
+}
+
+double *orc_field(orc_state *s, int field, int level) {
+  if (level < 0 || level > s->p.levels) return NULL;
+  switch (field) {
+    case F_U: return s->u;
+    case F_V: return s->v;
+    case F_W: return s->w;
+    case F_P: return s->pr;
+    case F_DYE: return s->dye;
+    case F_CU: return s->cu;
+    case F_CV: return s->cv;
+    case F_CW: return s->cw;
+    case F_RHS: return s->rhs;
+    case F_MGX: return s->mgx[level];
+    case F_MGB: return s->mgb[level];
+    case F_MGR: return s->mgr[level];
+    case F_DYE_PRED: return s->dpred;
+  }
+  return NULL;
+}
+
+long orc_field_len(orc_state *s, int field, int level) {
+  if (field == F_MGX || field == F_MGB || field == F_MGR) return s->n[level];
+  return s->n[0];
+}
+
+/* Seeded blob initial condition (no reference counterpart: the workload
+ * replaces FLUID_INIT_VEL, fluid.ts:80-121).  dye = sum over blobs of
+ * 1 - smoothstep(r - 1/2, r + 1/2, |x - c|) at cell centres; V on y-faces
+ * (x = i+1/2, y = j+1, z = k+1/2) = -sum w0 * (same profile at the face);
+ * U = W = P = 0. */
+void orc_init_blobs(orc_state *s, const orc_blob *b, int nb) {
+  dims3 d = s->d[0];
+  long n = s->n[0];
+  memset(s->u, 0, sizeof(double) * n);
+  memset(s->w, 0, sizeof(double) * n);
+  memset(s->pr, 0, sizeof(double) * n);
+#pragma omp parallel for
+  for (int k = 0; k < d.nz; ++k)
+    for (int j = 0; j < d.ny; ++j)
+      for (int i = 0; i < d.nx; ++i) {
+        double dy = 0.0, vv = 0.0;
+        for (int q = 0; q < nb; ++q) {
+          double ex = i + 0.5 - b[q].cx, ey = j + 0.5 - b[q].cy, ez = k + 0.5 - b[q].cz;
+          double r = sqrt(ex * ex + ey * ey + ez * ez);
+          dy += 1.0 - orc_smoothstep(b[q].radius - 0.5, b[q].radius + 0.5, r);
+          double fy = j + 1.0 - b[q].cy;
+          double rf = sqrt(ex * ex + fy * fy + ez * ez);
+          vv -= b[q].w0 * (1.0 - orc_smoothstep(b[q].radius - 0.5, b[q].radius + 0.5, rf));
+        }
+        long c = IDX(d, i, j, k);
+        s->dye[c] = dy;
+        s->v[c] = vv;
+      }
+}
+
+void orc_stage(orc_state *s, int stage, double dt) {
+  switch (stage) {
+    case ST_F2C: face_to_centre(s); break;
+    case ST_ADVECT: advect_mac(s, dt); break;
+    case ST_CASSON: casson_buoyancy(s, dt); break;
+    case ST_DIV: divergence(s, s->u, s->v, s->w, s->rhs); break;
+    case ST_VCYCLE: vcycle(s); break;
+    case ST_PROJECT: project(s); break;
+    case ST_DYE_PRED: dye_predict(s, dt); break;
+    case ST_DYE_RES: dye_resolve(s, dt); break;
+  }
+}
+
+void orc_mg_op(orc_state *s, int op, int level, int count) {
+  int L = s->p.levels;
+  if (level < 0 || level > L) return;
+  dims3 d = s->d[level];
+  switch (op) {
+    case MG_SMOOTH: smooth(s->mgx[level], s->mgb[level], d, level, count); break;
+    case MG_RESIDUAL: residual(s->mgx[level], s->mgb[level], s->mgr[level], d, level); break;
+    case MG_RESTRICT:
+      if (level < L) restrict_op(s->mgr[level], d, s->mgb[level + 1], s->d[level + 1]);
+      break;
+    case MG_PROLONG:
+      if (level < L) prolong_add(s->mgx[level], d, s->mgx[level + 1], s->d[level + 1]);
+      break;
+    case MG_COARSE:
+      if (level == L) coarse_solve(s, L);
+      break;
+    case MG_ZERO: memset(s->mgx[level], 0, sizeof(double) * s->n[level]); break;
+  }
+}
+
+/* The step order of encodePhysicsStep (BloodDropEngine.ts:1916-2069) and
+ * pass B's dye transport (:2096-2108): advect, Casson + buoyancy, RHS,
+ * V-cycles, project, face->centre, dye.  The face->centre reconstruction
+ * before advection (:1922-1925) equals the one after the previous step's
+ * projection (:2067-2069), so it is computed once per step here. */
+int orc_step(orc_state *s, double dt, int nsteps, double *rel_res) {
+  int cycles = 0;
+  double rr = 0.0;
+  for (int n = 0; n < nsteps; ++n) {
+    face_to_centre(s);
+    advect_mac(s, dt);
+    casson_buoyancy(s, dt);
+    divergence(s, s->u, s->v, s->w, s->rhs);
+    if (s->p.solve_to_tol) {
+      rr = rel_residual(s);
+      int it = 0;
+      while (rr > s->p.tol && it < s->p.max_cycles) {
+        vcycle(s);
+        ++it;
+        rr = rel_residual(s);
+      }
+      cycles += it;
+    } else {
+      for (int c = 0; c < s->p.cycles; ++c) vcycle(s);
+      cycles += s->p.cycles;
+      if (rel_res) rr = rel_residual(s);
+    }
+    project(s);
+    face_to_centre(s);
+    dye_predict(s, dt);
+    dye_resolve(s, dt);
+  }
+  if (rel_res) *rel_res = rr;
+  return cycles;
+}
+
+void orc_div_norms(orc_state *s, double *max_div, double *l2_div) {
+  dims3 d = s->d[0];
+  double *D = s->mgr[0];
+  divergence(s, s->u, s->v, s->w, D);
+  double mx = 0.0, ss = 0.0;
+  for (long c = 0; c < s->n[0]; ++c) {
+    mx = fmax(mx, fabs(D[c]));
+    ss += D[c] * D[c];
+  }
+  (void)d;
+  *max_div = mx;
+  *l2_div = sqrt(ss / (double)s->n[0]);
+}

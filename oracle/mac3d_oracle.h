@@ -1,0 +1,58 @@
+/*
+ * mac3d_oracle.h — FP64 CPU restatement of the reference's MAC step.
+ *
+ * TEST INFRASTRUCTURE ONLY.  This library is the checker that the HIP path is
+ * compared against (tests, the smoke check, the CPU-baseline leg of the
+ * benchmark).  Nothing in the product links, loads or calls it.
+ *
+ * Every function follows one WGSL kernel of the reference, cited at its
+ * definition in mac3d_oracle.c, with the documented changes the MI355X
+ * workload makes (SI Casson/Boussinesq, no-slip walls, exact coarse solve).
+ * Arrays are cell-major with x fastest: idx = (k*ny + j)*nx + i.
+ */
+#ifndef MAC3D_ORACLE_H
+#define MAC3D_ORACLE_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct orc_params {
+  int nx, ny, nz;
+  int levels, nu1, nu2, cycles;
+  int solve_to_tol; /* 0: fixed cycles; 1: cycles until ||r||/||b|| <= tol */
+  int max_cycles;
+  double dx, rho_w, mu_w, rho_b, tau_y, mu_inf, g;
+  double gamma_reg, shear_eps2, alpha_max, vel_damp, maccormack, tol;
+} orc_params;
+
+typedef struct orc_blob {
+  double cx, cy, cz, radius, w0;
+} orc_blob;
+
+/* state of one simulation held by the oracle */
+typedef struct orc_state orc_state;
+
+orc_state *orc_create(const orc_params *p);
+void orc_destroy(orc_state *s);
+/* field ids match MAC3D_FIELD_* of include/mac3d.h */
+double *orc_field(orc_state *s, int field, int level);
+long orc_field_len(orc_state *s, int field, int level);
+void orc_init_blobs(orc_state *s, const orc_blob *b, int nb);
+/* returns V-cycles run; *rel_res gets the last ||r||/||b|| */
+int orc_step(orc_state *s, double dt, int nsteps, double *rel_res);
+void orc_stage(orc_state *s, int stage, double dt);
+void orc_mg_op(orc_state *s, int op, int level, int count);
+void orc_div_norms(orc_state *s, double *max_div, double *l2_div);
+int orc_set_threads(int n);
+
+/* helpers exposed for the pinning tests */
+double orc_sample_cell(const double *f, int nx, int ny, int nz, double ux,
+                       double uy, double uz);
+double orc_smoothstep(double e0, double e1, double x);
+double orc_top_coeff(int level);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,154 @@
+"""ctypes front end of the FP64 oracle (oracle/liborc.so).
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, the smoke check and the
+CPU-baseline leg of bench.py as the checker of the HIP path; the product never
+imports it.  Field and stage ids are those of include/mac3d.h.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB = os.path.join(HERE, "liborc.so")
+
+F_U, F_V, F_W, F_P, F_DYE, F_CU, F_CV, F_CW, F_RHS, F_MGX, F_MGB, F_MGR, F_DYE_PRED = range(13)
+ST_F2C, ST_ADVECT, ST_CASSON, ST_DIV, ST_VCYCLE, ST_PROJECT, ST_DYE_PRED, ST_DYE_RES = range(8)
+MG_SMOOTH, MG_RESIDUAL, MG_RESTRICT, MG_PROLONG, MG_COARSE, MG_ZERO = range(6)
+
+
+class OrcParams(C.Structure):
+    _fields_ = [(n, C.c_int) for n in ("nx", "ny", "nz", "levels", "nu1", "nu2", "cycles",
+                                        "solve_to_tol", "max_cycles")] + \
+               [(n, C.c_double) for n in ("dx", "rho_w", "mu_w", "rho_b", "tau_y", "mu_inf", "g",
+                                           "gamma_reg", "shear_eps2", "alpha_max", "vel_damp",
+                                           "maccormack", "tol")]
+
+
+class OrcBlob(C.Structure):
+    _fields_ = [(n, C.c_double) for n in ("cx", "cy", "cz", "radius", "w0")]
+
+
+def build():
+    subprocess.run(["make", "-s", "-C", HERE], check=True)
+
+
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB):
+            build()
+        L = C.CDLL(LIB)
+        L.orc_create.restype = C.c_void_p
+        L.orc_create.argtypes = [C.POINTER(OrcParams)]
+        L.orc_destroy.argtypes = [C.c_void_p]
+        L.orc_field.restype = C.POINTER(C.c_double)
+        L.orc_field.argtypes = [C.c_void_p, C.c_int, C.c_int]
+        L.orc_field_len.restype = C.c_long
+        L.orc_field_len.argtypes = [C.c_void_p, C.c_int, C.c_int]
+        L.orc_init_blobs.argtypes = [C.c_void_p, C.POINTER(OrcBlob), C.c_int]
+        L.orc_step.restype = C.c_int
+        L.orc_step.argtypes = [C.c_void_p, C.c_double, C.c_int, C.POINTER(C.c_double)]
+        L.orc_stage.argtypes = [C.c_void_p, C.c_int, C.c_double]
+        L.orc_mg_op.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_int]
+        L.orc_div_norms.argtypes = [C.c_void_p, C.POINTER(C.c_double), C.POINTER(C.c_double)]
+        L.orc_set_threads.restype = C.c_int
+        L.orc_set_threads.argtypes = [C.c_int]
+        L.orc_sample_cell.restype = C.c_double
+        L.orc_sample_cell.argtypes = [C.POINTER(C.c_double), C.c_int, C.c_int, C.c_int,
+                                      C.c_double, C.c_double, C.c_double]
+        L.orc_smoothstep.restype = C.c_double
+        L.orc_smoothstep.argtypes = [C.c_double, C.c_double, C.c_double]
+        _lib = L
+    return _lib
+
+
+def set_threads(n: int) -> int:
+    return lib().orc_set_threads(n)
+
+
+DEFAULTS = dict(rho_w=1000.0, mu_w=1e-3, rho_b=1060.0, tau_y=0.005, mu_inf=0.0035, g=9.81,
+                gamma_reg=0.40, shear_eps2=1e-6, alpha_max=1.0 / 6.0, vel_damp=1.0,
+                maccormack=0.45 * 0.995, tol=1e-8)
+
+
+class Oracle:
+    """One FP64 simulation on the CPU."""
+
+    def __init__(self, nx, ny, nz, dx, levels=3, nu1=2, nu2=2, cycles=8, solve_to_tol=False,
+                 max_cycles=50, **kw):
+        p = dict(DEFAULTS)
+        p.update(kw)
+        self.params = OrcParams(nx=nx, ny=ny, nz=nz, levels=levels, nu1=nu1, nu2=nu2,
+                                cycles=cycles, solve_to_tol=int(solve_to_tol),
+                                max_cycles=max_cycles, dx=dx, **p)
+        self.shape = (nz, ny, nx)
+        self.levels = levels
+        self.h = lib().orc_create(C.byref(self.params))
+        if not self.h:
+            raise ValueError("oracle: bad parameters")
+
+    def close(self):
+        if self.h:
+            lib().orc_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def level_shape(self, level):
+        nz, ny, nx = self.shape
+        return (nz >> level, ny >> level, nx >> level)
+
+    def view(self, field, level=0):
+        """numpy view (nz, ny, nx) of an oracle field; writes go through."""
+        n = lib().orc_field_len(self.h, field, level)
+        ptr = lib().orc_field(self.h, field, level)
+        shape = self.level_shape(level) if field in (F_MGX, F_MGB, F_MGR) else self.shape
+        return np.ctypeslib.as_array(ptr, shape=(n,)).reshape(shape)
+
+    def get(self, field, level=0):
+        return self.view(field, level).copy()
+
+    def set(self, field, arr, level=0):
+        v = self.view(field, level)
+        v[...] = np.asarray(arr, dtype=np.float64).reshape(v.shape)
+
+    def init_blobs(self, blobs):
+        arr = (OrcBlob * len(blobs))(*[OrcBlob(b.cx, b.cy, b.cz, b.radius, b.w0) for b in blobs])
+        lib().orc_init_blobs(self.h, arr, len(blobs))
+
+    def step(self, dt, nsteps=1):
+        rr = C.c_double(0.0)
+        cycles = lib().orc_step(self.h, dt, nsteps, C.byref(rr))
+        return cycles, rr.value
+
+    def stage(self, stage, dt=0.0):
+        lib().orc_stage(self.h, stage, dt)
+
+    def mg(self, op, level, count=1):
+        lib().orc_mg_op(self.h, op, level, count)
+
+    def div_norms(self):
+        a, b = C.c_double(), C.c_double()
+        lib().orc_div_norms(self.h, C.byref(a), C.byref(b))
+        return a.value, b.value
+
+
+def sample_cell(f, ux, uy, uz):
+    f = np.ascontiguousarray(f, dtype=np.float64)
+    nz, ny, nx = f.shape
+    return lib().orc_sample_cell(f.ctypes.data_as(C.POINTER(C.c_double)), nx, ny, nz, ux, uy, uz)
+
+
+def smoothstep(e0, e1, x):
+    return lib().orc_smoothstep(e0, e1, x)

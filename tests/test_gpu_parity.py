@@ -1,0 +1,166 @@
+"""HIP path vs the FP64 oracle, through the C ABI (needs an MI355X).
+
+Tolerances: FP32 kernels against FP64 arithmetic, relative L2 <= 1e-5 per
+field (the workload's parity bar); the FP64 device mode against the oracle
+<= 1e-11.
+"""
+import numpy as np
+import pytest
+
+import oracle
+from conftest import rel_l2
+from gpu_helpers import pair, copy_state
+
+pytestmark = pytest.mark.gpu
+
+TOL32 = 1e-5
+TOL64 = 1e-11
+
+
+def _cmp(s, o, fields, tol, level=0):
+    for f in fields:
+        fid = getattr(oracle, "F_" + f)
+        a = s.download(f, level)
+        b = o.get(fid, level)
+        e = rel_l2(a, b)
+        assert e <= tol, f"{f} level {level}: rel L2 {e:.3e} > {tol:.1e}"
+
+
+@pytest.mark.parametrize("prec,tol", [(0, TOL32), (2, TOL64)])
+@pytest.mark.parametrize("stage,fields", [
+    ("FACE_TO_CENTRE", ("CU", "CV", "CW")),
+    ("ADVECT", ("U", "V", "W")),
+    ("CASSON", ("U", "V", "W")),
+    ("DIVERGENCE", ("RHS",)),
+    ("PROJECT", ("U", "V", "W")),
+])
+def test_stage(stage, fields, prec, tol):
+    w, s, o = pair(precision=prec)
+    with s:
+        # centred velocity feeds advection; set it in both
+        o.stage(oracle.ST_F2C)
+        s.stage("FACE_TO_CENTRE")
+        if stage in ("DIVERGENCE", "PROJECT"):
+            # a projected field has D(u) ~ 0: advect and force it first so the
+            # divergence and the projection act on an O(1) right-hand side
+            o.stage(oracle.ST_ADVECT, w.dt)
+            o.stage(oracle.ST_CASSON, w.dt)
+            copy_state(o, s, ("U", "V", "W"))
+        if stage == "PROJECT":
+            o.stage(oracle.ST_DIV)
+            o.stage(oracle.ST_VCYCLE)
+            copy_state(o, s, ("P",))
+        o.stage(getattr(oracle, {"FACE_TO_CENTRE": "ST_F2C", "ADVECT": "ST_ADVECT",
+                                  "CASSON": "ST_CASSON", "DIVERGENCE": "ST_DIV",
+                                  "PROJECT": "ST_PROJECT"}[stage]), w.dt)
+        s.stage(stage, w.dt)
+        _cmp(s, o, fields, tol)
+
+
+@pytest.mark.parametrize("prec,tol", [(0, TOL32), (2, TOL64)])
+def test_dye_stages(prec, tol):
+    w, s, o = pair(precision=prec)
+    with s:
+        o.stage(oracle.ST_F2C)
+        s.stage("FACE_TO_CENTRE")
+        o.stage(oracle.ST_DYE_PRED, w.dt)
+        s.stage("DYE_PREDICT", w.dt)
+        _cmp(s, o, ("DYE_PRED",), tol)
+        o.stage(oracle.ST_DYE_RES, w.dt)
+        s.stage("DYE_RESOLVE", w.dt)
+        _cmp(s, o, ("DYE",), tol)
+
+
+@pytest.mark.parametrize("prec,tol", [(0, 2e-6), (2, TOL64)])
+@pytest.mark.parametrize("level", [0, 1, 2])
+def test_mg_smooth_residual_restrict_prolong(level, prec, tol):
+    w, s, o = pair(precision=prec, warm_steps=0)
+    rng = np.random.default_rng(level)
+    with s:
+        shp = s.level_shape(level)
+        x = rng.standard_normal(shp)
+        b = rng.standard_normal(shp)
+        for dev_f, orc_f, arr in (("MGX", oracle.F_MGX, x), ("MGB", oracle.F_MGB, b)):
+            s.upload(dev_f, arr, level)
+            o.set(orc_f, arr, level)
+        s.mg("SMOOTH", level, 2)
+        o.mg(oracle.MG_SMOOTH, level, 2)
+        _cmp(s, o, ("MGX",), tol, level)
+        s.mg("RESIDUAL", level)
+        o.mg(oracle.MG_RESIDUAL, level)
+        _cmp(s, o, ("MGR",), tol * 10, level)
+        s.mg("RESTRICT", level)
+        o.mg(oracle.MG_RESTRICT, level)
+        _cmp(s, o, ("MGB",), tol * 10, level + 1)
+        xc = rng.standard_normal(s.level_shape(level + 1))
+        s.upload("MGX", xc, level + 1)
+        o.set(oracle.F_MGX, xc, level + 1)
+        s.mg("PROLONG", level)
+        o.mg(oracle.MG_PROLONG, level)
+        _cmp(s, o, ("MGX",), tol, level)
+
+
+@pytest.mark.parametrize("prec,tol", [(0, 1e-5), (2, 1e-11)])
+@pytest.mark.parametrize("name", ["tiny", "adjoint"])
+def test_coarse_solve_exact(name, prec, tol):
+    w, s, o = pair(name, precision=prec, warm_steps=0)
+    L = w.levels
+    rng = np.random.default_rng(7)
+    with s:
+        b = rng.standard_normal(s.level_shape(L))
+        s.upload("MGB", b, L)
+        o.set(oracle.F_MGB, b, L)
+        s.mg("COARSE", L)
+        o.mg(oracle.MG_COARSE, L)
+        _cmp(s, o, ("MGX",), tol, L)
+
+
+@pytest.mark.parametrize("prec,tol", [(0, TOL32), (2, 1e-10)])
+def test_vcycle(prec, tol):
+    w, s, o = pair(precision=prec)
+    with s:
+        o.stage(oracle.ST_F2C); o.stage(oracle.ST_ADVECT, w.dt); o.stage(oracle.ST_CASSON, w.dt)
+        o.stage(oracle.ST_DIV)
+        copy_state(o, s, ("U", "V", "W", "P"))
+        s.stage("DIVERGENCE")
+        for _ in range(3):
+            o.stage(oracle.ST_VCYCLE)
+            s.stage("VCYCLE")
+        _cmp(s, o, ("P",), tol)
+
+
+@pytest.mark.parametrize("prec,tol", [(0, TOL32), (2, 1e-10)])
+def test_full_step_tiny(prec, tol):
+    """The tiny workload: 32x48x32, 20 steps, dt 0.5 ms, from the seeded blob."""
+    w, s, o = pair(precision=prec, warm_steps=0)
+    with s:
+        st = s.step(w.dt, w.steps)
+        o.step(w.dt, w.steps)
+        _cmp(s, o, ("U", "V", "W", "P", "DYE"), tol)
+        assert st["max_div"] <= 1e-6, st
+        omax, _ = o.div_norms()
+        assert omax <= 1e-9
+
+
+def test_mixed_tiny():
+    """Mixed precision (FP16 level>=2 storage, FP64 refinement to 1e-8)
+    against the FP64 oracle solving to the same tolerance."""
+    w, s, o = pair(precision=1, warm_steps=0)
+    with s:
+        st = s.step(w.dt, w.steps)
+        o.step(w.dt, w.steps)
+        assert st["converged"] == 1 and st["rel_res"] <= 1e-8, st
+        assert st["max_div"] <= 1e-6, st
+        _cmp(s, o, ("U", "V", "W", "DYE"), TOL32)
+        _cmp(s, o, ("P",), 1e-5)
+
+
+def test_step_rejects_bad_dt():
+    w, s, o = pair(warm_steps=0)
+    with s:
+        before = s.download("V")
+        from mac3d import Mac3dError
+        for bad in (float("nan"), -1e-3, 0.0, float("inf")):
+            with pytest.raises(Mac3dError):
+                s.step(bad, 1)
+        assert np.array_equal(before, s.download("V"))
