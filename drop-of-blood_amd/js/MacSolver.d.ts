@@ -1,0 +1,34 @@
+// Type declarations of the MAC step host (js/MacSolver.js) and its N-API addon.
+export interface MacConfig {
+  nx?: number; ny?: number; nz?: number;
+  levels?: number; nu1?: number; nu2?: number; cycles?: number;
+  precision?: 0 | 1 | 2; ir_max_iters?: number; device?: number;
+  dx?: number; rho_w?: number; mu_w?: number; rho_b?: number;
+  tau_y?: number; mu_inf?: number; g?: number; gamma_reg?: number;
+  shear_eps2?: number; alpha_max?: number; vel_damp?: number;
+  maccormack?: number; ir_tol?: number;
+}
+export interface Blob { cx: number; cy: number; cz: number; radius: number; w0: number; }
+export interface StepStats {
+  steps: number; vcycles: number; irIters: number; converged: number;
+  maxDiv: number; l2Div: number; maxDivRate: number; relRes: number; ms: number;
+}
+export declare const FIELD: Readonly<{ U: 0; V: 1; W: 2; P: 3; DYE: 4; CU: 5; CV: 6; CW: 7;
+  RHS: 8; MGX: 9; MGB: 10; MGR: 11; DYE_PRED: 12 }>;
+export declare const PRECISION: Readonly<{ fp32: 0; mixed: 1; fp64: 2 }>;
+export declare function version(): string;
+export declare class MacSolver {
+  constructor(config?: MacConfig);
+  readonly config: Required<MacConfig>;
+  readonly cells: number;
+  readonly u: Float32Array; readonly v: Float32Array; readonly w: Float32Array;
+  readonly p: Float32Array; readonly dye: Float32Array;
+  destroyed: boolean;
+  initBlobs(blobs: Blob[]): void;
+  step(dt: number, n?: number): StepStats;
+  sync(): this;
+  write(name: keyof typeof FIELD, data: Float32Array | Float64Array): void;
+  read<T extends Float32Array | Float64Array>(name: keyof typeof FIELD, out: T): T;
+  divNorms(): { maxDiv: number; l2Div: number };
+  destroy(): void;
+}

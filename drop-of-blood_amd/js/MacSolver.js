@@ -1,0 +1,113 @@
+// @ts-check
+'use strict';
+/**
+ * TypeScript-checked host of the MI355X MAC step (Node N-API over the C ABI).
+ *
+ * Stands where the reference's WebGPU compute pass stands: the engine used to
+ * record the MAC kernels in encodePhysicsStep (src/engine/BloodDropEngine.ts:
+ * 1855-2069) and step them per substep from renderFrame / seekToSimTime
+ * (:2605, :2142-2179).  A MacSolver keeps the src/sim solver shape: a
+ * `step(dt)` method like MultiDropSystem.step (src/sim/dropKinematics.ts:604)
+ * and reusable Float32Array field buffers like SimUniformPacker's
+ * (src/sim/uniformLayout.ts:52-54).
+ *
+ * Errors: the addon throws an Error carrying the library's status and
+ * message; a non-finite or non-positive dt throws before any state changes.
+ */
+const path = require('path');
+
+const addon = require(path.join(__dirname, '..', 'lib', 'mac3d.node'));
+
+const FIELD = Object.freeze({ U: 0, V: 1, W: 2, P: 3, DYE: 4, CU: 5, CV: 6, CW: 7, RHS: 8,
+  MGX: 9, MGB: 10, MGR: 11, DYE_PRED: 12 });
+const PRECISION = Object.freeze({ fp32: 0, mixed: 1, fp64: 2 });
+
+/**
+ * @typedef {{nx?:number, ny?:number, nz?:number, levels?:number, nu1?:number,
+ *   nu2?:number, cycles?:number, precision?:number, ir_max_iters?:number,
+ *   device?:number, dx?:number, rho_w?:number, mu_w?:number, rho_b?:number,
+ *   tau_y?:number, mu_inf?:number, g?:number, gamma_reg?:number,
+ *   shear_eps2?:number, alpha_max?:number, vel_damp?:number,
+ *   maccormack?:number, ir_tol?:number}} MacConfig
+ * @typedef {{steps:number, vcycles:number, irIters:number, converged:number,
+ *   maxDiv:number, l2Div:number, maxDivRate:number, relRes:number, ms:number}} StepStats
+ */
+
+class MacSolver {
+  /** @param {MacConfig} [config] */
+  constructor(config = {}) {
+    /** @type {Required<MacConfig>} */
+    this.config = Object.assign(addon.defaultConfig(), config);
+    this._h = addon.create(this.config);
+    const { nx, ny, nz } = this.config;
+    this.cells = nx * ny * nz;
+    /** field buffers, refreshed by sync(); x fastest, (k*ny + j)*nx + i */
+    this.u = new Float32Array(this.cells);
+    this.v = new Float32Array(this.cells);
+    this.w = new Float32Array(this.cells);
+    this.p = new Float32Array(this.cells);
+    this.dye = new Float32Array(this.cells);
+    this.destroyed = false;
+  }
+
+  _live() {
+    if (this.destroyed) throw new Error('MacSolver: destroyed');
+    return this._h;
+  }
+
+  /** @param {{cx:number, cy:number, cz:number, radius:number, w0:number}[]} blobs */
+  initBlobs(blobs) {
+    addon.initBlobs(this._live(), blobs);
+  }
+
+  /**
+   * Advance n steps of fixed dt (seconds); blocks until the GPU is done.
+   * @param {number} dt @param {number} [n]
+   * @returns {StepStats}
+   */
+  step(dt, n = 1) {
+    if (!Number.isFinite(dt) || dt <= 0) throw new RangeError('dt must be finite and > 0');
+    return addon.step(this._live(), dt, n);
+  }
+
+  /** Copy the device fields into the reusable buffers. */
+  sync() {
+    const h = this._live();
+    addon.download(h, FIELD.U, 0, this.u);
+    addon.download(h, FIELD.V, 0, this.v);
+    addon.download(h, FIELD.W, 0, this.w);
+    addon.download(h, FIELD.P, 0, this.p);
+    addon.download(h, FIELD.DYE, 0, this.dye);
+    return this;
+  }
+
+  /**
+   * @param {keyof typeof FIELD} name
+   * @param {Float32Array|Float64Array} data
+   */
+  write(name, data) {
+    addon.upload(this._live(), FIELD[name], 0, data);
+  }
+
+  /**
+   * @param {keyof typeof FIELD} name
+   * @param {Float32Array|Float64Array} out
+   */
+  read(name, out) {
+    addon.download(this._live(), FIELD[name], 0, out);
+    return out;
+  }
+
+  divNorms() {
+    return addon.divNorms(this._live());
+  }
+
+  /** Idempotent, like the reference engine's destroy() guard. */
+  destroy() {
+    if (this.destroyed) return;
+    addon.destroy(this._h);
+    this.destroyed = true;
+  }
+}
+
+module.exports = { MacSolver, FIELD, PRECISION, version: addon.version };

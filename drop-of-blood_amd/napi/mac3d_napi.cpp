@@ -1,0 +1,263 @@
+// mac3d_napi.cpp — Node N-API addon: a thin shim from JavaScript onto the
+// C ABI of include/mac3d.h.  No compute here: every call forwards to
+// libmac3d.so; a non-zero status becomes a thrown JS Error carrying
+// mac3d_last_error (the reference throws from create and on device errors,
+// src/engine/BloodDropEngine.ts:337-345,1465-1476).
+#include <node_api.h>
+
+#include <cmath>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "mac3d.h"
+
+namespace {
+
+#define NAPI_CALL(env, call)                                   \
+  do {                                                         \
+    if ((call) != napi_ok) {                                   \
+      napi_throw_error((env), nullptr, "N-API call failed: " #call); \
+      return nullptr;                                          \
+    }                                                          \
+  } while (0)
+
+napi_value throw_status(napi_env env, void *ctx, int rc) {
+  std::string msg = std::string("mac3d status ") + std::to_string(rc) + ": " + mac3d_last_error(ctx);
+  napi_throw_error(env, nullptr, msg.c_str());
+  return nullptr;
+}
+
+bool get_args(napi_env env, napi_callback_info info, size_t want, napi_value *argv) {
+  size_t argc = want;
+  if (napi_get_cb_info(env, info, &argc, argv, nullptr, nullptr) != napi_ok || argc < want) {
+    napi_throw_type_error(env, nullptr, "missing arguments");
+    return false;
+  }
+  return true;
+}
+
+void *get_handle(napi_env env, napi_value v) {
+  void *p = nullptr;
+  if (napi_get_value_external(env, v, &p) != napi_ok || !p) {
+    napi_throw_type_error(env, nullptr, "expected a live solver handle");
+    return nullptr;
+  }
+  return p;
+}
+
+double get_double(napi_env env, napi_value v) {
+  double d = NAN;
+  napi_get_value_double(env, v, &d);
+  return d;
+}
+
+int get_int(napi_env env, napi_value v) {
+  int32_t i = 0;
+  napi_get_value_int32(env, v, &i);
+  return i;
+}
+
+bool prop_num(napi_env env, napi_value obj, const char *name, double *out) {
+  bool has = false;
+  if (napi_has_named_property(env, obj, name, &has) != napi_ok || !has) return false;
+  napi_value v;
+  napi_get_named_property(env, obj, name, &v);
+  napi_valuetype t;
+  napi_typeof(env, v, &t);
+  if (t != napi_number) return false;
+  napi_get_value_double(env, v, out);
+  return true;
+}
+
+void set_num(napi_env env, napi_value obj, const char *name, double v) {
+  napi_value n;
+  napi_create_double(env, v, &n);
+  napi_set_named_property(env, obj, name, n);
+}
+
+#define CONFIG_INT_FIELDS(X) X(nx) X(ny) X(nz) X(levels) X(nu1) X(nu2) X(cycles) \
+  X(precision) X(ir_max_iters) X(device)
+#define CONFIG_DBL_FIELDS(X) X(dx) X(rho_w) X(mu_w) X(rho_b) X(tau_y) X(mu_inf) X(g) \
+  X(gamma_reg) X(shear_eps2) X(alpha_max) X(vel_damp) X(maccormack) X(ir_tol)
+
+napi_value config_to_js(napi_env env, const mac3d_config &c) {
+  napi_value o;
+  napi_create_object(env, &o);
+#define X(f) set_num(env, o, #f, (double)c.f);
+  CONFIG_INT_FIELDS(X)
+  CONFIG_DBL_FIELDS(X)
+#undef X
+  return o;
+}
+
+void config_from_js(napi_env env, napi_value o, mac3d_config &c) {
+  double v;
+#define X(f) if (prop_num(env, o, #f, &v)) c.f = (int32_t)v;
+  CONFIG_INT_FIELDS(X)
+#undef X
+#define X(f) if (prop_num(env, o, #f, &v)) c.f = v;
+  CONFIG_DBL_FIELDS(X)
+#undef X
+}
+
+napi_value DefaultConfig(napi_env env, napi_callback_info) {
+  mac3d_config c;
+  mac3d_default_config(&c);
+  return config_to_js(env, c);
+}
+
+napi_value Create(napi_env env, napi_callback_info info) {
+  napi_value argv[1];
+  if (!get_args(env, info, 1, argv)) return nullptr;
+  mac3d_config c;
+  mac3d_default_config(&c);
+  config_from_js(env, argv[0], c);
+  void *ctx = nullptr;
+  int rc = mac3d_create(&c, &ctx);
+  if (rc) return throw_status(env, nullptr, rc);
+  napi_value h;
+  NAPI_CALL(env, napi_create_external(env, ctx, nullptr, nullptr, &h));
+  return h;
+}
+
+napi_value Destroy(napi_env env, napi_callback_info info) {
+  napi_value argv[1];
+  if (!get_args(env, info, 1, argv)) return nullptr;
+  void *ctx = get_handle(env, argv[0]);
+  if (!ctx) return nullptr;
+  mac3d_destroy(ctx);
+  return nullptr;
+}
+
+// (handle, field, level, typedArray) for upload and download
+napi_value Transfer(napi_env env, napi_callback_info info, bool up) {
+  napi_value argv[4];
+  if (!get_args(env, info, 4, argv)) return nullptr;
+  void *ctx = get_handle(env, argv[0]);
+  if (!ctx) return nullptr;
+  int field = get_int(env, argv[1]), level = get_int(env, argv[2]);
+  bool is_ta = false;
+  napi_is_typedarray(env, argv[3], &is_ta);
+  if (!is_ta) {
+    napi_throw_type_error(env, nullptr, "expected a Float32Array or Float64Array");
+    return nullptr;
+  }
+  napi_typedarray_type tt;
+  size_t len = 0, off = 0;
+  void *data = nullptr;
+  napi_value ab;
+  NAPI_CALL(env, napi_get_typedarray_info(env, argv[3], &tt, &len, &data, &ab, &off));
+  int dtype;
+  if (tt == napi_float32_array) dtype = MAC3D_F32;
+  else if (tt == napi_float64_array) dtype = MAC3D_F64;
+  else {
+    napi_throw_type_error(env, nullptr, "expected a Float32Array or Float64Array");
+    return nullptr;
+  }
+  int rc = up ? mac3d_upload(ctx, field, level, data, dtype, len)
+              : mac3d_download(ctx, field, level, data, dtype, len);
+  if (rc) return throw_status(env, ctx, rc);
+  return nullptr;
+}
+napi_value Upload(napi_env env, napi_callback_info info) { return Transfer(env, info, true); }
+napi_value Download(napi_env env, napi_callback_info info) { return Transfer(env, info, false); }
+
+napi_value InitBlobs(napi_env env, napi_callback_info info) {
+  napi_value argv[2];
+  if (!get_args(env, info, 2, argv)) return nullptr;
+  void *ctx = get_handle(env, argv[0]);
+  if (!ctx) return nullptr;
+  uint32_t n = 0;
+  NAPI_CALL(env, napi_get_array_length(env, argv[1], &n));
+  std::vector<mac3d_blob> blobs(n);
+  for (uint32_t q = 0; q < n; ++q) {
+    napi_value o;
+    NAPI_CALL(env, napi_get_element(env, argv[1], q, &o));
+    mac3d_blob &b = blobs[q];
+    b.cx = b.cy = b.cz = b.radius = b.w0 = NAN;
+    prop_num(env, o, "cx", &b.cx);
+    prop_num(env, o, "cy", &b.cy);
+    prop_num(env, o, "cz", &b.cz);
+    prop_num(env, o, "radius", &b.radius);
+    prop_num(env, o, "w0", &b.w0);
+  }
+  int rc = mac3d_init_blobs(ctx, blobs.data(), (int)n);
+  if (rc) return throw_status(env, ctx, rc);
+  return nullptr;
+}
+
+napi_value Step(napi_env env, napi_callback_info info) {
+  napi_value argv[3];
+  if (!get_args(env, info, 3, argv)) return nullptr;
+  void *ctx = get_handle(env, argv[0]);
+  if (!ctx) return nullptr;
+  mac3d_stats st;
+  std::memset(&st, 0, sizeof(st));
+  int rc = mac3d_step(ctx, get_double(env, argv[1]), get_int(env, argv[2]), &st);
+  if (rc) return throw_status(env, ctx, rc);
+  napi_value o;
+  NAPI_CALL(env, napi_create_object(env, &o));
+  set_num(env, o, "steps", st.steps);
+  set_num(env, o, "vcycles", st.vcycles);
+  set_num(env, o, "irIters", st.ir_iters);
+  set_num(env, o, "converged", st.converged);
+  set_num(env, o, "maxDiv", st.max_div);
+  set_num(env, o, "l2Div", st.l2_div);
+  set_num(env, o, "maxDivRate", st.max_div_rate);
+  set_num(env, o, "relRes", st.rel_res);
+  set_num(env, o, "ms", st.ms);
+  return o;
+}
+
+napi_value Stage(napi_env env, napi_callback_info info) {
+  napi_value argv[3];
+  if (!get_args(env, info, 3, argv)) return nullptr;
+  void *ctx = get_handle(env, argv[0]);
+  if (!ctx) return nullptr;
+  int rc = mac3d_stage(ctx, get_int(env, argv[1]), get_double(env, argv[2]));
+  if (rc) return throw_status(env, ctx, rc);
+  return nullptr;
+}
+
+napi_value DivNorms(napi_env env, napi_callback_info info) {
+  napi_value argv[1];
+  if (!get_args(env, info, 1, argv)) return nullptr;
+  void *ctx = get_handle(env, argv[0]);
+  if (!ctx) return nullptr;
+  double mx = 0, l2 = 0;
+  int rc = mac3d_div_norms(ctx, &mx, &l2);
+  if (rc) return throw_status(env, ctx, rc);
+  napi_value o;
+  NAPI_CALL(env, napi_create_object(env, &o));
+  set_num(env, o, "maxDiv", mx);
+  set_num(env, o, "l2Div", l2);
+  return o;
+}
+
+napi_value Version(napi_env env, napi_callback_info) {
+  napi_value s;
+  NAPI_CALL(env, napi_create_string_utf8(env, mac3d_version(), NAPI_AUTO_LENGTH, &s));
+  return s;
+}
+
+napi_value Init(napi_env env, napi_value exports) {
+  napi_property_descriptor props[] = {
+      {"defaultConfig", nullptr, DefaultConfig, nullptr, nullptr, nullptr, napi_default, nullptr},
+      {"create", nullptr, Create, nullptr, nullptr, nullptr, napi_default, nullptr},
+      {"destroy", nullptr, Destroy, nullptr, nullptr, nullptr, napi_default, nullptr},
+      {"upload", nullptr, Upload, nullptr, nullptr, nullptr, napi_default, nullptr},
+      {"download", nullptr, Download, nullptr, nullptr, nullptr, napi_default, nullptr},
+      {"initBlobs", nullptr, InitBlobs, nullptr, nullptr, nullptr, napi_default, nullptr},
+      {"step", nullptr, Step, nullptr, nullptr, nullptr, napi_default, nullptr},
+      {"stage", nullptr, Stage, nullptr, nullptr, nullptr, napi_default, nullptr},
+      {"divNorms", nullptr, DivNorms, nullptr, nullptr, nullptr, napi_default, nullptr},
+      {"version", nullptr, Version, nullptr, nullptr, nullptr, napi_default, nullptr},
+  };
+  napi_define_properties(env, exports, sizeof(props) / sizeof(props[0]), props);
+  return exports;
+}
+
+}  // namespace
+
+NAPI_MODULE(NODE_GYP_MODULE_NAME, Init)

@@ -1,0 +1,24 @@
+// @ts-check
+'use strict';
+// Runs the tiny workload through the TypeScript-checked host (MacSolver over
+// the N-API addon) and prints the fields' checksums and step stats as JSON.
+const path = require('path');
+const root = path.join(__dirname, '..', '..', 'drop-of-blood_amd', 'js');
+const { MacSolver } = require(path.join(root, 'MacSolver.js'));
+const { workload } = require(path.join(root, 'scenarios.js'));
+
+const steps = Number(process.argv[2] || 4);
+const w = workload('tiny', 1);
+const s = new MacSolver({ nx: w.nx, ny: w.ny, nz: w.nz, dx: w.dx, levels: w.levels,
+  cycles: w.cycles, precision: w.precision });
+s.initBlobs(w.blobs);
+let bad = null;
+try { s.step(NaN, 1); } catch (e) { bad = String(e.message); }
+const st = s.step(w.dt, steps);
+s.sync();
+const sum = (a) => { let t = 0; for (let q = 0; q < a.length; q++) t += a[q] * a[q]; return Math.sqrt(t); };
+const out = { stats: st, norms: { u: sum(s.u), v: sum(s.v), w: sum(s.w), p: sum(s.p), dye: sum(s.dye) },
+  badDt: bad };
+s.destroy();
+s.destroy();
+console.log(JSON.stringify(out));

@@ -1,0 +1,34 @@
+"""The TypeScript-checked host (js/MacSolver.js over the N-API addon) steps
+the tiny workload on the GPU and agrees with the FP64 oracle."""
+import json
+import os
+import shutil
+import subprocess
+
+import numpy as np
+import pytest
+
+import oracle
+from mac3d import scenarios
+
+pytestmark = pytest.mark.gpu
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+@pytest.mark.skipif(shutil.which("node") is None, reason="node not installed")
+def test_node_host_steps_tiny():
+    steps = 4
+    r = subprocess.run(["node", os.path.join(ROOT, "tests", "js", "step_tiny.js"), str(steps)],
+                       capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stderr
+    out = json.loads(r.stdout.strip().splitlines()[-1])
+    assert out["badDt"] is not None and "dt" in out["badDt"]
+    assert out["stats"]["steps"] == steps and out["stats"]["maxDiv"] <= 1e-6
+    w = scenarios.workload("tiny", 1)
+    o = oracle.Oracle(w.nx, w.ny, w.nz, w.dx, levels=w.levels, cycles=w.cycles)
+    o.init_blobs(w.blobs)
+    o.step(w.dt, steps)
+    for name, fid in (("u", oracle.F_U), ("v", oracle.F_V), ("w", oracle.F_W),
+                      ("p", oracle.F_P), ("dye", oracle.F_DYE)):
+        want = np.linalg.norm(o.get(fid))
+        assert abs(out["norms"][name] - want) <= 1e-5 * want + 1e-12, name
