@@ -61,6 +61,33 @@ __device__ inline T sample_cell(const T *__restrict__ f, const Dims &d, T ux,
   return mixv(mixv(x00, x10, fy), mixv(x01, x11, fy), fz);
 }
 
+// The same trilinear sample of three fields at one point (the cell-centred
+// velocity components): index and weight arithmetic shared.
+template <class T>
+__device__ inline void sample_cell3(const T *__restrict__ f0, const T *__restrict__ f1,
+                                    const T *__restrict__ f2, const Dims &d, T ux, T uy,
+                                    T uz, T out[3]) {
+  T tx = ux * (T)d.nx - T(0.5), ty = uy * (T)d.ny - T(0.5), tz = uz * (T)d.nz - T(0.5);
+  T bx = floor(tx), by = floor(ty), bz = floor(tz);
+  T fx = tx - bx, fy = ty - by, fz = tz - bz;
+  int ib = (int)bx, jb = (int)by, kb = (int)bz;
+  int i0 = clampi(ib, 0, d.nx - 1), i1 = clampi(ib + 1, 0, d.nx - 1);
+  int j0 = clampi(jb, 0, d.ny - 1), j1 = clampi(jb + 1, 0, d.ny - 1);
+  int k0 = clampi(kb, 0, d.nz - 1), k1 = clampi(kb + 1, 0, d.nz - 1);
+  size_t r00 = ((size_t)k0 * d.ny + j0) * d.nx, r10 = ((size_t)k0 * d.ny + j1) * d.nx;
+  size_t r01 = ((size_t)k1 * d.ny + j0) * d.nx, r11 = ((size_t)k1 * d.ny + j1) * d.nx;
+  const T *fs[3] = {f0, f1, f2};
+#pragma unroll
+  for (int a = 0; a < 3; ++a) {
+    const T *f = fs[a];
+    T x00 = mixv(f[r00 + i0], f[r00 + i1], fx);
+    T x10 = mixv(f[r10 + i0], f[r10 + i1], fx);
+    T x01 = mixv(f[r01 + i0], f[r01 + i1], fx);
+    T x11 = mixv(f[r11 + i0], f[r11 + i1], fx);
+    out[a] = mixv(mixv(x00, x10, fy), mixv(x01, x11, fy), fz);
+  }
+}
+
 // wavefront (64 lanes) reductions
 template <class T> __device__ inline T wave_sum(T v) {
 #pragma unroll

@@ -10,6 +10,7 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <map>
 #include <new>
 #include <string>
 #include <type_traits>
@@ -61,10 +62,25 @@ struct Ctx {
   std::string err;
   // level-0 step fields
   Buf u, v, w, u2, v2, w2, p, dye, dye2, dpred, cu, cv, cw, rhs;
-  // multigrid: x, b, r per level (level 0 x/b alias p/rhs except in mixed)
-  Buf mgx[kMaxLev + 1], mgb[kMaxLev + 1], mgr[kMaxLev + 1];
+  // multigrid: x, b, r per level (level 0 x/b alias p/rhs except in mixed);
+  // sweeps run out of place between mgx and its alternate mgxa
+  Buf mgx[kMaxLev + 1], mgxa[kMaxLev + 1], mgb[kMaxLev + 1], mgr[kMaxLev + 1];
   // mixed-mode FP32 L0 correction and its right-hand side
   Buf e0, r0;
+  // right-hand-side scale of level 0: [0] = multiplier, [1] = its inverse
+  double *scale = nullptr;
+  // every device allocation, freed at destroy (the Bufs above are views)
+  std::vector<void *> allocs;
+  // home pointers of the ping-pong buffers (parity 0)
+  void *u_home = nullptr, *dye_home = nullptr, *mgx_home[kMaxLev + 1] = {};
+  // captured launch sequences, keyed by kind and buffer parities
+  struct Graph {
+    hipGraphExec_t exec = nullptr;
+    double dt = 0.0;
+    uint64_t after = 0;  // parity bits after the sequence
+  };
+  std::map<uint64_t, Graph> graphs;
+  bool use_graphs = true;
   // coarse solve
   double *cq = nullptr;  // qx | qy | qz | lx | ly | lz
   size_t cq_off[6] = {0, 0, 0, 0, 0, 0};
@@ -106,7 +122,6 @@ int fail(Ctx *ctx, int code, const std::string &msg) {
 }
 
 dim3 cell_grid(const Dims &d) { return dim3((d.nx + 63) / 64, (d.ny + 3) / 4, d.nz); }
-dim3 half_grid(const Dims &d) { return dim3(((d.nx + 1) / 2 + 63) / 64, (d.ny + 3) / 4, d.nz); }
 const dim3 kCellBlock(64, 4, 1);
 
 double top_coeff(int level) {
@@ -118,22 +133,24 @@ double level_h2(int level) {
   return s * s;
 }
 
-int alloc(Ctx *ctx, Buf &b, int type, size_t n) {
-  b.type = type;
-  b.n = n;
-  hipError_t e = hipMalloc(&b.ptr, type_size(type) * n);
+int dev_alloc(Ctx *ctx, void **p, size_t bytes) {
+  hipError_t e = hipMalloc(p, bytes);
   if (e != hipSuccess) {
-    b.ptr = nullptr;
+    *p = nullptr;
     return fail(ctx, MAC3D_ENOMEM, std::string("hipMalloc: ") + hipGetErrorString(e));
   }
-  e = hipMemsetAsync(b.ptr, 0, type_size(type) * n, ctx->stream);
-  if (e != hipSuccess) return fail(ctx, MAC3D_EHIP, hipGetErrorString(e));
+  ctx->allocs.push_back(*p);
   return MAC3D_OK;
 }
 
-void release(Buf &b) {
-  if (b.ptr) (void)hipFree(b.ptr);
-  b.ptr = nullptr;
+int alloc(Ctx *ctx, Buf &b, int type, size_t n) {
+  b.type = type;
+  b.n = n;
+  int rc = dev_alloc(ctx, &b.ptr, type_size(type) * n);
+  if (rc) return rc;
+  hipError_t e = hipMemsetAsync(b.ptr, 0, type_size(type) * n, ctx->stream);
+  if (e != hipSuccess) return fail(ctx, MAC3D_EHIP, hipGetErrorString(e));
+  return MAC3D_OK;
 }
 
 // Symmetric tridiagonal 1-D operator of one axis and its eigen-pairs by
@@ -317,22 +334,46 @@ int launch_dye_resolve(Ctx *ctx, double dt) {
 }
 
 // ---- multigrid
-int mg_pass(Ctx *ctx, int l, int parity, bool zero, double bscale) {
+void swap_x(Ctx *ctx, int l) {
+  std::swap(ctx->mgx[l], ctx->mgxa[l]);
+  if (l == 0 && ctx->prec != MAC3D_PREC_MIXED) ctx->p = ctx->mgx[0];
+}
+
+// z-chunk of the sweep kernel: about three 512-thread blocks per CU
+// (256 CUs) in one round, chunks of at least 8 planes
+int sweep_cz(const Dims &d) {
+  int cols = ((d.nx + SW_TX - 1) / SW_TX) * ((d.ny + SW_TY - 1) / SW_TY);
+  int chunks = std::max(1, 768 / cols);
+  int cz = (d.nz + chunks - 1) / chunks;
+  return std::max(cz, std::min(d.nz, 8));
+}
+
+// One red-black sweep on level l, mgx -> mgxa, then the two swap roles.
+// Level 0's right-hand side is multiplied by scale[0] (mixed mode: 1/max|r|).
+int mg_sweep(Ctx *ctx, int l, bool zero) {
   const Dims &d = ctx->d[l];
-  ProfScope ps(ctx->prof, ctx->stream, l == 0 ? KC_SMOOTH0 : KC_MG);
-  with_type(ctx->mgx[l].type, [&](auto tag) {
-    using S = std::remove_pointer_t<decltype(tag)>;
-    using C = typename compute_of<S>::type;
-    if (zero)
-      hipLaunchKernelGGL((k_rbgs<S, true>), half_grid(d), kCellBlock, 0, ctx->stream, d,
-                         (S *)ctx->mgx[l].ptr, (const S *)ctx->mgb[l].ptr, (C)level_h2(l),
-                         (C)(l == 0 ? 1.0 : top_coeff(l)), (C)bscale, parity);
-    else
-      hipLaunchKernelGGL((k_rbgs<S, false>), half_grid(d), kCellBlock, 0, ctx->stream, d,
-                         (S *)ctx->mgx[l].ptr, (const S *)ctx->mgb[l].ptr, (C)level_h2(l),
-                         (C)(l == 0 ? 1.0 : top_coeff(l)), (C)bscale, parity);
-  });
+  int cz = sweep_cz(d);
+  int ncx = (d.nx + SW_TX - 1) / SW_TX, ncy = (d.ny + SW_TY - 1) / SW_TY;
+  dim3 grid(ncx * ncy * ((d.nz + cz - 1) / cz));
+  const double *sc = (l == 0) ? ctx->scale : nullptr;
+  {
+    ProfScope ps(ctx->prof, ctx->stream, l == 0 ? KC_SMOOTH0 : KC_MG);
+    with_type(ctx->mgx[l].type, [&](auto tag) {
+      using S = std::remove_pointer_t<decltype(tag)>;
+      using C = typename compute_of<S>::type;
+      C h2 = (C)level_h2(l), tc = (C)(l == 0 ? 1.0 : top_coeff(l));
+      if (zero)
+        hipLaunchKernelGGL((k_rbgs_sweep<S, true>), grid, dim3(SW_TX, SW_TY), 0, ctx->stream, d,
+                           (const S *)ctx->mgx[l].ptr, (S *)ctx->mgxa[l].ptr,
+                           (const S *)ctx->mgb[l].ptr, h2, tc, sc, cz, ncx, ncy);
+      else
+        hipLaunchKernelGGL((k_rbgs_sweep<S, false>), grid, dim3(SW_TX, SW_TY), 0, ctx->stream, d,
+                           (const S *)ctx->mgx[l].ptr, (S *)ctx->mgxa[l].ptr,
+                           (const S *)ctx->mgb[l].ptr, h2, tc, sc, cz, ncx, ncy);
+    });
+  }
   LAUNCHCHK();
+  swap_x(ctx, l);
   return MAC3D_OK;
 }
 
@@ -347,17 +388,15 @@ int mg_fill_zero(Ctx *ctx, Buf &b) {
   return MAC3D_OK;
 }
 
-int mg_smooth(Ctx *ctx, int l, int sweeps, bool zero, double bscale) {
+int mg_smooth(Ctx *ctx, int l, int sweeps, bool zero) {
   int rc;
   if (zero && sweeps == 0) return mg_fill_zero(ctx, ctx->mgx[l]);
-  for (int s = 0; s < sweeps; ++s) {
-    if ((rc = mg_pass(ctx, l, 0, zero && s == 0, bscale))) return rc;
-    if ((rc = mg_pass(ctx, l, 1, false, bscale))) return rc;
-  }
+  for (int s = 0; s < sweeps; ++s)
+    if ((rc = mg_sweep(ctx, l, zero && s == 0))) return rc;
   return MAC3D_OK;
 }
 
-int mg_residual(Ctx *ctx, int l, double bscale) {
+int mg_residual(Ctx *ctx, int l) {
   const Dims &d = ctx->d[l];
   ProfScope ps(ctx->prof, ctx->stream, l == 0 ? KC_RESID0 : KC_MG);
   with_type(ctx->mgx[l].type, [&](auto tag) {
@@ -366,7 +405,7 @@ int mg_residual(Ctx *ctx, int l, double bscale) {
     hipLaunchKernelGGL(k_residual<S>, cell_grid(d), kCellBlock, 0, ctx->stream, d,
                        (const S *)ctx->mgx[l].ptr, (const S *)ctx->mgb[l].ptr,
                        (S *)ctx->mgr[l].ptr, (C)level_h2(l),
-                       (C)(l == 0 ? 1.0 : top_coeff(l)), (C)bscale);
+                       (C)(l == 0 ? 1.0 : top_coeff(l)), l == 0 ? (const double *)ctx->scale : nullptr);
   });
   LAUNCHCHK();
   return MAC3D_OK;
@@ -436,25 +475,26 @@ int mg_coarse(Ctx *ctx) {
 // One V-cycle on (mgx[0], mgb[0]) with the schedule of
 // BloodDropEngine.ts:1942-2059: smooth / residual / restrict down to the
 // coarsest level, exact coarse solve, prolong / smooth back up.  Levels >= 1
-// start from a zero guess.  zero0: level 0 also starts from zero.
-int vcycle(Ctx *ctx, bool zero0, double bscale0) {
+// start from a zero guess.  zero0: level 0 also starts from zero.  Level 0's
+// right-hand side is scaled by scale[0] (1 outside the refinement loop).
+int vcycle(Ctx *ctx, bool zero0) {
   int rc, L = ctx->L;
   int nu1 = ctx->cfg.nu1, nu2 = ctx->cfg.nu2;
-  if ((rc = mg_smooth(ctx, 0, nu1, zero0, bscale0))) return rc;
-  if ((rc = mg_residual(ctx, 0, bscale0))) return rc;
+  if ((rc = mg_smooth(ctx, 0, nu1, zero0))) return rc;
+  if ((rc = mg_residual(ctx, 0))) return rc;
   if ((rc = mg_restrict(ctx, 0))) return rc;
   for (int l = 1; l < L; ++l) {
-    if ((rc = mg_smooth(ctx, l, nu1, true, 1.0))) return rc;
-    if ((rc = mg_residual(ctx, l, 1.0))) return rc;
+    if ((rc = mg_smooth(ctx, l, nu1, true))) return rc;
+    if ((rc = mg_residual(ctx, l))) return rc;
     if ((rc = mg_restrict(ctx, l))) return rc;
   }
   if ((rc = mg_coarse(ctx))) return rc;
   for (int l = L - 1; l >= 1; --l) {
     if ((rc = mg_prolong(ctx, l))) return rc;
-    if ((rc = mg_smooth(ctx, l, nu2, false, 1.0))) return rc;
+    if ((rc = mg_smooth(ctx, l, nu2, false))) return rc;
   }
   if ((rc = mg_prolong(ctx, 0))) return rc;
-  if ((rc = mg_smooth(ctx, 0, nu2, false, bscale0))) return rc;
+  if ((rc = mg_smooth(ctx, 0, nu2, false))) return rc;
   return MAC3D_OK;
 }
 
@@ -463,8 +503,10 @@ size_t cell_blocks(const Dims &d) {
   return (size_t)g.x * g.y * g.z;
 }
 
-// Mixed-mode residual in FP64 with its norms, read back to the host.
-int residual64(Ctx *ctx, double *maxr, double *rr, double *bb) {
+// Mixed-mode residual in FP64 and its norms (enqueued only: the norms land
+// in pinned host memory once the stream reaches the copy), and the scale of
+// the next correction cycle.
+int residual64(Ctx *ctx) {
   const Dims &d = ctx->d[0];
   {
     ProfScope ps(ctx->prof, ctx->stream, KC_RESID64);
@@ -472,23 +514,20 @@ int residual64(Ctx *ctx, double *maxr, double *rr, double *bb) {
                        (const double *)ctx->p.ptr, (const double *)ctx->rhs.ptr,
                        (float *)ctx->r0.ptr, ctx->part);
     hipLaunchKernelGGL(k_fold3, dim3(1), dim3(256), 0, ctx->stream, (const double *)ctx->part,
-                       cell_blocks(d), ctx->red);
+                       cell_blocks(d), ctx->red, ctx->scale);
   }
   LAUNCHCHK();
   HIPCHK(hipMemcpyAsync(ctx->red_host, ctx->red, 3 * sizeof(double), hipMemcpyDeviceToHost,
                         ctx->stream));
-  HIPCHK(hipStreamSynchronize(ctx->stream));
-  *maxr = ctx->red_host[0];
-  *rr = ctx->red_host[1];
-  *bb = ctx->red_host[2];
   return MAC3D_OK;
 }
 
-int update64(Ctx *ctx, double scale) {
+int update64(Ctx *ctx) {
   size_t n = ctx->n[0];
   ProfScope ps(ctx->prof, ctx->stream, KC_RESID64);
   hipLaunchKernelGGL(k_update64, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, ctx->stream, n,
-                     (double *)ctx->p.ptr, (const float *)ctx->e0.ptr, scale);
+                     (double *)ctx->p.ptr, (const float *)ctx->mgx[0].ptr,
+                     (const double *)ctx->scale);
   LAUNCHCHK();
   return MAC3D_OK;
 }
@@ -512,50 +551,135 @@ int div_norms(Ctx *ctx, double *mx, double *l2) {
   return MAC3D_OK;
 }
 
-// Pressure solve of one step.  FP32/FP64: a fixed number of V-cycles on
-// (P, RHS) warm-started from the previous step's P.  Mixed: FP64 iterative
-// refinement around FP32 V-cycles until ||r||/||b|| <= ir_tol.
-int pressure_solve(Ctx *ctx, int *cycles, int *iters, double *rel, int *conv) {
+// ---- launch sequences and their graphs
+// Parity bits of the ping-pong buffers: bit 0 velocity, bit 1 dye, bit 2+l
+// multigrid level l.  A captured sequence is valid for the parities it was
+// captured at and leaves a fixed set of parities behind.
+uint64_t parity_bits(const Ctx *ctx) {
+  uint64_t b = 0;
+  if (ctx->u.ptr != ctx->u_home) b |= 1;
+  if (ctx->dye.ptr != ctx->dye_home) b |= 2;
+  for (int l = 0; l <= ctx->L; ++l)
+    if (ctx->mgx[l].ptr != ctx->mgx_home[l]) b |= (uint64_t)4 << l;
+  return b;
+}
+
+void set_parity_bits(Ctx *ctx, uint64_t b) {
+  if (((ctx->u.ptr != ctx->u_home) ? 1u : 0u) != (b & 1)) swap_vel(ctx);
+  if (((ctx->dye.ptr != ctx->dye_home) ? 1u : 0u) != ((b >> 1) & 1)) std::swap(ctx->dye, ctx->dye2);
+  for (int l = 0; l <= ctx->L; ++l)
+    if (((ctx->mgx[l].ptr != ctx->mgx_home[l]) ? 1u : 0u) != ((b >> (2 + l)) & 1)) swap_x(ctx, l);
+}
+
+// Run a launch sequence: eagerly while profiling, otherwise through a graph
+// captured on first use for the current parities and dt.
+template <class F>
+int run_seq(Ctx *ctx, int kind, double dt, F &&enqueue) {
+  if (!ctx->use_graphs || ctx->prof.enabled) return enqueue();
+  uint64_t before = parity_bits(ctx);
+  uint64_t key = ((uint64_t)kind << 32) | before;
+  auto it = ctx->graphs.find(key);
+  if (it == ctx->graphs.end() || it->second.dt != dt) {
+    if (it != ctx->graphs.end()) {
+      (void)hipGraphExecDestroy(it->second.exec);
+      ctx->graphs.erase(it);
+    }
+    HIPCHK(hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
+    int rc = enqueue();
+    hipGraph_t g = nullptr;
+    hipError_t e = hipStreamEndCapture(ctx->stream, &g);
+    if (rc) {
+      if (g) (void)hipGraphDestroy(g);
+      set_parity_bits(ctx, before);
+      return rc;
+    }
+    if (e != hipSuccess) return fail(ctx, MAC3D_EHIP, std::string("graph capture: ") + hipGetErrorString(e));
+    Ctx::Graph gr;
+    gr.dt = dt;
+    gr.after = parity_bits(ctx);
+    hipError_t ei = hipGraphInstantiate(&gr.exec, g, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(g);
+    if (ei != hipSuccess) return fail(ctx, MAC3D_EHIP, std::string("graph instantiate: ") + hipGetErrorString(ei));
+    set_parity_bits(ctx, before);
+    it = ctx->graphs.emplace(key, gr).first;
+  }
+  HIPCHK(hipGraphLaunch(it->second.exec, ctx->stream));
+  set_parity_bits(ctx, it->second.after);
+  return MAC3D_OK;
+}
+
+void drop_graphs(Ctx *ctx) {
+  for (auto &kv : ctx->graphs) (void)hipGraphExecDestroy(kv.second.exec);
+  ctx->graphs.clear();
+}
+
+enum { SEQ_STEP = 1, SEQ_PRE = 2, SEQ_ITER = 3, SEQ_POST = 4 };
+
+int seq_pre(Ctx *ctx, double dt) {
+  int rc;
+  if ((rc = launch_face_to_centre(ctx))) return rc;
+  if ((rc = launch_advect(ctx, dt))) return rc;
+  if ((rc = launch_casson(ctx, dt))) return rc;
+  return launch_divergence(ctx);
+}
+
+int seq_post(Ctx *ctx, double dt) {
+  int rc;
+  if ((rc = launch_project(ctx))) return rc;
+  if ((rc = launch_face_to_centre(ctx))) return rc;
+  if ((rc = launch_dye_predict(ctx, dt))) return rc;
+  return launch_dye_resolve(ctx, dt);
+}
+
+// One step.  FP32/FP64: a fixed number of V-cycles on (P, RHS), warm-started
+// from the previous step's P, all in one captured sequence.  Mixed: FP64
+// iterative refinement around FP32 correction V-cycles until
+// ||r||/||b|| <= ir_tol; each iteration is one captured sequence ending in
+// the residual norms, which the host reads to decide whether to go on.
+int one_step(Ctx *ctx, double dt, int *cycles, int *iters, double *rel, int *conv) {
   int rc;
   if (ctx->prec != MAC3D_PREC_MIXED) {
-    for (int c = 0; c < ctx->cfg.cycles; ++c)
-      if ((rc = vcycle(ctx, false, 1.0))) return rc;
+    rc = run_seq(ctx, SEQ_STEP, dt, [&]() {
+      int r;
+      if ((r = seq_pre(ctx, dt))) return r;
+      for (int c = 0; c < ctx->cfg.cycles; ++c)
+        if ((r = vcycle(ctx, false))) return r;
+      return seq_post(ctx, dt);
+    });
+    if (rc) return rc;
     *cycles += ctx->cfg.cycles;
     *iters = ctx->cfg.cycles;
     *rel = -1.0;
     *conv = 1;
     return MAC3D_OK;
   }
-  double mr, rr, bb;
+  rc = run_seq(ctx, SEQ_PRE, dt, [&]() {
+    int r;
+    if ((r = seq_pre(ctx, dt))) return r;
+    return residual64(ctx);
+  });
+  if (rc) return rc;
   int it = 0;
   *conv = 0;
   for (;;) {
-    if ((rc = residual64(ctx, &mr, &rr, &bb))) return rc;
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    double mr = ctx->red_host[0], rr = ctx->red_host[1], bb = ctx->red_host[2];
     double r = bb > 0.0 ? std::sqrt(rr / bb) : 0.0;
     *rel = r;
     if (r <= ctx->cfg.ir_tol || mr == 0.0) { *conv = 1; break; }
     if (it >= ctx->cfg.ir_max_iters) break;
-    if ((rc = vcycle(ctx, true, 1.0 / mr))) return rc;
-    if ((rc = update64(ctx, mr))) return rc;
+    rc = run_seq(ctx, SEQ_ITER, 0.0, [&]() {
+      int q;
+      if ((q = vcycle(ctx, true))) return q;
+      if ((q = update64(ctx))) return q;
+      return residual64(ctx);
+    });
+    if (rc) return rc;
     ++it;
   }
   *cycles += it;
   *iters = it;
-  return MAC3D_OK;
-}
-
-int one_step(Ctx *ctx, double dt, int *cycles, int *iters, double *rel, int *conv) {
-  int rc;
-  if ((rc = launch_face_to_centre(ctx))) return rc;
-  if ((rc = launch_advect(ctx, dt))) return rc;
-  if ((rc = launch_casson(ctx, dt))) return rc;
-  if ((rc = launch_divergence(ctx))) return rc;
-  if ((rc = pressure_solve(ctx, cycles, iters, rel, conv))) return rc;
-  if ((rc = launch_project(ctx))) return rc;
-  if ((rc = launch_face_to_centre(ctx))) return rc;
-  if ((rc = launch_dye_predict(ctx, dt))) return rc;
-  if ((rc = launch_dye_resolve(ctx, dt))) return rc;
-  return MAC3D_OK;
+  return run_seq(ctx, SEQ_POST, dt, [&]() { return seq_post(ctx, dt); });
 }
 
 Buf *field_buf(Ctx *ctx, int field, int level) {
@@ -634,18 +758,23 @@ int create_impl(Ctx *ctx, const mac3d_config *cfg) {
     ctx->mgx[0] = ctx->p;
     ctx->mgb[0] = ctx->rhs;
   }
+  if ((rc = alloc(ctx, ctx->mgxa[0], ctx->mgx[0].type, n0))) return rc;
   if ((rc = alloc(ctx, ctx->mgr[0], ctx->mgx[0].type, n0))) return rc;
   for (int l = 1; l <= ctx->L; ++l) {
     int t = mixed ? (l >= 2 ? TF16 : TF32) : t0;
     if ((rc = alloc(ctx, ctx->mgx[l], t, ctx->n[l]))) return rc;
+    if ((rc = alloc(ctx, ctx->mgxa[l], t, ctx->n[l]))) return rc;
     if ((rc = alloc(ctx, ctx->mgb[l], t, ctx->n[l]))) return rc;
     if ((rc = alloc(ctx, ctx->mgr[l], t, ctx->n[l]))) return rc;
   }
+  ctx->u_home = ctx->u.ptr;
+  ctx->dye_home = ctx->dye.ptr;
+  for (int l = 0; l <= ctx->L; ++l) ctx->mgx_home[l] = ctx->mgx[l].ptr;
   // coarse eigen-decomposition
   const Dims &dc = ctx->d[ctx->L];
   std::vector<double> qx, qy, qz, lx, ly, lz;
   axis_eigen(dc.nx, false, 0.0, qx, lx);
-  axis_eigen(dc.ny, true, ctx->L == 0 ? 1.0 : top_coeff(ctx->L), qy, ly);
+  axis_eigen(dc.ny, true, top_coeff(ctx->L), qy, ly);
   axis_eigen(dc.nz, false, 0.0, qz, lz);
   std::vector<double> all;
   std::vector<double> *parts[6] = {&qx, &qy, &qz, &lx, &ly, &lz};
@@ -653,15 +782,18 @@ int create_impl(Ctx *ctx, const mac3d_config *cfg) {
     ctx->cq_off[a] = all.size();
     all.insert(all.end(), parts[a]->begin(), parts[a]->end());
   }
-  HIPCHK(hipMalloc(&ctx->cq, all.size() * sizeof(double)));
+  if ((rc = dev_alloc(ctx, (void **)&ctx->cq, all.size() * sizeof(double)))) return rc;
   HIPCHK(hipMemcpyAsync(ctx->cq, all.data(), all.size() * sizeof(double), hipMemcpyHostToDevice,
                         ctx->stream));
-  HIPCHK(hipMalloc(&ctx->cwork, ctx->n[ctx->L] * sizeof(double)));
+  if ((rc = dev_alloc(ctx, (void **)&ctx->cwork, ctx->n[ctx->L] * sizeof(double)))) return rc;
   ctx->part_n = cell_blocks(ctx->d[0]);
-  HIPCHK(hipMalloc(&ctx->part, 3 * ctx->part_n * sizeof(double)));
-  HIPCHK(hipMalloc(&ctx->red, 4 * sizeof(double)));
+  if ((rc = dev_alloc(ctx, (void **)&ctx->part, 3 * ctx->part_n * sizeof(double)))) return rc;
+  if ((rc = dev_alloc(ctx, (void **)&ctx->red, 4 * sizeof(double)))) return rc;
+  if ((rc = dev_alloc(ctx, (void **)&ctx->scale, 2 * sizeof(double)))) return rc;
   HIPCHK(hipHostMalloc(&ctx->red_host, 4 * sizeof(double), hipHostMallocDefault));
-  HIPCHK(hipMalloc(&ctx->stage, n0 * sizeof(double)));
+  if ((rc = dev_alloc(ctx, (void **)&ctx->stage, n0 * sizeof(double)))) return rc;
+  hipLaunchKernelGGL(k_fill<double>, dim3(1), dim3(256), 0, ctx->stream, ctx->scale, (size_t)2, 1.0);
+  LAUNCHCHK();
   HIPCHK(hipStreamSynchronize(ctx->stream));
   if ((rc = ctx->prof.init(ctx->err))) return rc;
   return MAC3D_OK;
@@ -669,25 +801,23 @@ int create_impl(Ctx *ctx, const mac3d_config *cfg) {
 
 void destroy_impl(Ctx *ctx) {
   if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
-  Buf *f0[] = {&ctx->u, &ctx->v, &ctx->w, &ctx->u2, &ctx->v2, &ctx->w2, &ctx->dye,
-               &ctx->dye2, &ctx->dpred, &ctx->cu, &ctx->cv, &ctx->cw, &ctx->p, &ctx->rhs,
-               &ctx->e0, &ctx->r0, &ctx->mgr[0]};
-  for (Buf *b : f0) release(*b);
-  for (int l = 1; l <= ctx->L; ++l) {
-    release(ctx->mgx[l]);
-    release(ctx->mgb[l]);
-    release(ctx->mgr[l]);
-  }
-  if (ctx->cq) (void)hipFree(ctx->cq);
-  if (ctx->cwork) (void)hipFree(ctx->cwork);
-  if (ctx->part) (void)hipFree(ctx->part);
-  if (ctx->red) (void)hipFree(ctx->red);
+  drop_graphs(ctx);
+  for (void *p : ctx->allocs) (void)hipFree(p);
+  ctx->allocs.clear();
   if (ctx->red_host) (void)hipHostFree(ctx->red_host);
-  if (ctx->stage) (void)hipFree(ctx->stage);
+  ctx->red_host = nullptr;
   if (ctx->blobs) (void)hipFree(ctx->blobs);
+  ctx->blobs = nullptr;
   ctx->prof.destroy();
   if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
   ctx->stream = nullptr;
+}
+
+// test entry points run level 0 with an unscaled right-hand side
+int reset_scale(Ctx *ctx) {
+  hipLaunchKernelGGL(k_fill<double>, dim3(1), dim3(256), 0, ctx->stream, ctx->scale, (size_t)2, 1.0);
+  LAUNCHCHK();
+  return MAC3D_OK;
 }
 
 thread_local std::string g_create_err;
@@ -888,7 +1018,7 @@ int mac3d_stage(void *h, int stage, double dt) {
     case MAC3D_STAGE_ADVECT: rc = launch_advect(ctx, dt); break;
     case MAC3D_STAGE_CASSON: rc = launch_casson(ctx, dt); break;
     case MAC3D_STAGE_DIVERGENCE: rc = launch_divergence(ctx); break;
-    case MAC3D_STAGE_VCYCLE: rc = vcycle(ctx, false, 1.0); break;
+    case MAC3D_STAGE_VCYCLE: rc = reset_scale(ctx) ? MAC3D_EHIP : vcycle(ctx, false); break;
     case MAC3D_STAGE_PROJECT: rc = launch_project(ctx); break;
     case MAC3D_STAGE_DYE_PREDICT: rc = launch_dye_predict(ctx, dt); break;
     case MAC3D_STAGE_DYE_RESOLVE: rc = launch_dye_resolve(ctx, dt); break;
@@ -907,8 +1037,10 @@ int mac3d_mg_op(void *h, int op, int level, int count) {
   if (count < 0) return fail(ctx, MAC3D_EINVAL, "count must be >= 0");
   int rc;
   switch (op) {
-    case MAC3D_MG_SMOOTH: rc = mg_smooth(ctx, level, count, false, 1.0); break;
-    case MAC3D_MG_RESIDUAL: rc = mg_residual(ctx, level, 1.0); break;
+    case MAC3D_MG_SMOOTH:
+      rc = reset_scale(ctx) ? MAC3D_EHIP : mg_smooth(ctx, level, count, false);
+      break;
+    case MAC3D_MG_RESIDUAL: rc = reset_scale(ctx) ? MAC3D_EHIP : mg_residual(ctx, level); break;
     case MAC3D_MG_RESTRICT:
       if (level >= ctx->L) return fail(ctx, MAC3D_EINVAL, "restrict needs level < levels");
       rc = mg_restrict(ctx, level);

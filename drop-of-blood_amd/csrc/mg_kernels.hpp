@@ -29,60 +29,15 @@ __device__ inline void mg_stencil(const S *__restrict__ x, const Dims &d, C topc
   diag = dg;
 }
 
-// One colour pass of red-black Gauss-Seidel, in place
-// (MG_RBGS_RED / MG_RBGS_BLACK, fluid.ts:645-681): cells with
-// (i+j+k)&1 == parity get x = (sum - h^2 b bscale)/diag.  Threads map to the
-// cells of one colour: thread t of a row takes i = 2t + ((parity+j+k)&1).
-// zero_guess: the other colour is known to be 0 and is written as 0, this
-// colour's neighbours are not read (MG_RBGS_RED_ZERO, fluid.ts:605-641).
-template <class S, bool ZERO>
-__global__ void __launch_bounds__(256)
-k_rbgs(Dims d, S *__restrict__ x, const S *__restrict__ b,
-       typename compute_of<S>::type h2, typename compute_of<S>::type topc,
-       typename compute_of<S>::type bscale, int parity) {
-  typedef typename compute_of<S>::type C;
-  int t = blockIdx.x * blockDim.x + threadIdx.x;
-  int j = blockIdx.y * blockDim.y + threadIdx.y;
-  int k = blockIdx.z;
-  if (j >= d.ny) return;
-  int off = (parity + j + k) & 1;
-  int i = 2 * t + off;
-  if (ZERO) {
-    // both colours of this thread's pair are written: the pair is (2t, 2t+1)
-    int ia = 2 * t;
-    if (ia >= d.nx) return;
-    for (int q = 0; q < 2; ++q) {
-      int ii = ia + q;
-      if (ii >= d.nx) break;
-      size_t c = idx3(d, ii, j, k);
-      if (((ii + j + k) & 1) == parity) {
-        C dg = (j < d.ny - 1) ? C(1) : topc;
-        if (ii > 0) dg += C(1);
-        if (ii < d.nx - 1) dg += C(1);
-        if (j > 0) dg += C(1);
-        if (k > 0) dg += C(1);
-        if (k < d.nz - 1) dg += C(1);
-        x[c] = store_as<S, C>((-h2 * ((C)b[c] * bscale)) / dg);
-      } else {
-        x[c] = S(0);
-      }
-    }
-    return;
-  }
-  if (i >= d.nx) return;
-  size_t c = idx3(d, i, j, k);
-  C sn, dg;
-  mg_stencil<S, C>(x, d, topc, i, j, k, c, sn, dg);
-  x[c] = store_as<S, C>((sn - h2 * ((C)b[c] * bscale)) / dg);
-}
-
-// MG_RESIDUAL (fluid.ts:744-770): r = b bscale - (sum - diag x)/h^2
+// MG_RESIDUAL (fluid.ts:744-770): r = b scale - (sum - diag x)/h^2, with
+// scale = scale[0] (nullable: 1)
 template <class S>
 __global__ void __launch_bounds__(256)
 k_residual(Dims d, const S *__restrict__ x, const S *__restrict__ b, S *__restrict__ r,
            typename compute_of<S>::type h2, typename compute_of<S>::type topc,
-           typename compute_of<S>::type bscale) {
+           const double *__restrict__ scale) {
   typedef typename compute_of<S>::type C;
+  const C bscale = scale ? (C)scale[0] : C(1);
   int i = blockIdx.x * blockDim.x + threadIdx.x;
   int j = blockIdx.y * blockDim.y + threadIdx.y;
   int k = blockIdx.z;
@@ -252,6 +207,209 @@ k_coarse_xz_inv(Dims d, const double *__restrict__ work, const double *__restric
     C acc = C(0);
     for (int a = 0; a < nx; ++a) acc += (C)qx[m * nx + a] * s1[n * nx + a];
     x[idx3(d, m, j, n)] = store_as<S, C>(acc);
+  }
+}
+
+}  // namespace mac3d
+
+namespace mac3d {
+
+// ---------------------------------------------------------------------------
+// Red-black Gauss-Seidel sweep (MG_RBGS_RED then MG_RBGS_BLACK,
+// fluid.ts:645-681): cells with (i+j+k)&1 == 0 get x = (sum - h^2 b)/diag from
+// the old values, then the (i+j+k)&1 == 1 cells from the new red values.  Both
+// colour passes run in one kernel, out of place (xin -> xout): 12 B per FP32
+// cell per sweep instead of 24 for two in-place colour passes.  Each block owns a TX x TY column tile and marches
+// over a z-chunk [k0, k1).  Planes of xin are staged in LDS with a two-cell
+// ring; red is recomputed on the tile plus a one-cell ring (what the tile's
+// black cells read), so no block waits on another.  The arithmetic of each
+// update uses the neighbour order of mg_stencil, so the result equals two
+// in-place colour passes bit for bit.  Registers prefetch the next plane.
+// ZERO: the sweep starts from a zero guess (MG_RBGS_RED_ZERO, fluid.ts:605-641)
+// and never reads xin.  scale (nullable): the right-hand side is b * scale[0].
+#define SW_TX 64
+#define SW_TY 8
+#define SW_EX (SW_TX + 4)
+#define SW_EY (SW_TY + 4)
+#define SW_RX (SW_TX + 2)
+#define SW_RY (SW_TY + 2)
+#define SW_THREADS (SW_TX * SW_TY)
+#define SW_PL (SW_EX * SW_EY)  // 816 staged values per xin plane
+#define SW_BL (SW_RX * SW_RY)  // 660 staged values per b plane
+#define SW_PPT ((SW_PL + SW_THREADS - 1) / SW_THREADS)  // 2 per thread
+#define SW_BPT ((SW_BL + SW_THREADS - 1) / SW_THREADS)  // 2 per thread
+
+// Blocks are numbered on a 1-D grid and remapped so that each of the 8 XCDs
+// (blocks are dealt round-robin to XCDs) gets a contiguous run of tiles: the
+// halo planes and rows a tile shares with its neighbours then sit in the
+// same XCD's L2.
+__device__ inline int xcd_remap(int b, int nblocks) {
+  if (nblocks % 8) return b;
+  return (b % 8) * (nblocks / 8) + b / 8;
+}
+
+template <class S, bool ZERO>
+__global__ void __launch_bounds__(SW_THREADS)
+k_rbgs_sweep(Dims d, const S *__restrict__ xin, S *__restrict__ xout, const S *__restrict__ b,
+             typename compute_of<S>::type h2, typename compute_of<S>::type topc,
+             const double *__restrict__ scale, int cz, int ncx, int ncy) {
+  typedef typename compute_of<S>::type C;
+  __shared__ C sp[4][SW_EY][SW_EX];
+  __shared__ C sb[2][SW_RY][SW_RX];
+  const int nx = d.nx, ny = d.ny, nz = d.nz;
+  const int lb = xcd_remap(blockIdx.x, gridDim.x);
+  const int x0 = (lb % ncx) * SW_TX, y0 = ((lb / ncx) % ncy) * SW_TY;
+  const int k0 = (lb / (ncx * ncy)) * cz;
+  const int k1 = min(k0 + cz, nz);
+  const int tid = threadIdx.y * SW_TX + threadIdx.x;
+  const size_t plane = (size_t)nx * ny;
+  const C bs = scale ? (C)scale[0] : C(1);
+
+  // plane k of xin (two-cell ring) / of b (one-cell ring) into registers
+  auto fetch_p = [&](int k, C (&pre)[SW_PPT]) {
+#pragma unroll
+    for (int q = 0; q < SW_PPT; ++q) {
+      int t = tid + q * SW_THREADS;
+      C v = C(0);
+      if (t < SW_PL && k >= 0 && k < nz) {
+        int gx = x0 - 2 + t % SW_EX, gy = y0 - 2 + t / SW_EX;
+        if (gx >= 0 && gx < nx && gy >= 0 && gy < ny) v = (C)xin[k * plane + (size_t)gy * nx + gx];
+      }
+      pre[q] = v;
+    }
+  };
+  auto store_p = [&](int k, const C (&pre)[SW_PPT]) {
+#pragma unroll
+    for (int q = 0; q < SW_PPT; ++q) {
+      int t = tid + q * SW_THREADS;
+      if (t < SW_PL) sp[k & 3][t / SW_EX][t % SW_EX] = pre[q];
+    }
+  };
+  auto fetch_b = [&](int k, C (&pre)[SW_BPT]) {
+#pragma unroll
+    for (int q = 0; q < SW_BPT; ++q) {
+      int t = tid + q * SW_THREADS;
+      C v = C(0);
+      if (t < SW_BL && k >= 0 && k < nz) {
+        int gx = x0 - 1 + t % SW_RX, gy = y0 - 1 + t / SW_RX;
+        if (gx >= 0 && gx < nx && gy >= 0 && gy < ny) v = (C)b[k * plane + (size_t)gy * nx + gx] * bs;
+      }
+      pre[q] = v;
+    }
+  };
+  auto store_b = [&](int k, const C (&pre)[SW_BPT]) {
+#pragma unroll
+    for (int q = 0; q < SW_BPT; ++q) {
+      int t = tid + q * SW_THREADS;
+      if (t < SW_BL) sb[k & 1][t / SW_RX][t % SW_RX] = pre[q];
+    }
+  };
+  // diagonal of evalMgStencil at (gx, gy, k)
+  auto diag_at = [&](int gx, int gy, int k) {
+    C dg = (gy < ny - 1) ? C(1) : topc;
+    if (gx > 0) dg += C(1);
+    if (gx < nx - 1) dg += C(1);
+    if (gy > 0) dg += C(1);
+    if (k > 0) dg += C(1);
+    if (k < nz - 1) dg += C(1);
+    return dg;
+  };
+  // red cells of plane k on the tile plus one-cell ring, in LDS
+  auto red = [&](int k) {
+    if (k < 0 || k >= nz) return;
+    for (int t = tid; t < SW_BL; t += SW_THREADS) {
+      int c = t % SW_RX, r = t / SW_RX;
+      int gx = x0 - 1 + c, gy = y0 - 1 + r;
+      if (gx < 0 || gx >= nx || gy < 0 || gy >= ny || ((gx + gy + k) & 1)) continue;
+      C bb = sb[k & 1][r][c];
+      C v;
+      if (ZERO) {
+        v = (-h2 * bb) / diag_at(gx, gy, k);
+      } else {
+        C sn = C(0), dg = (gy < ny - 1) ? C(1) : topc;
+        if (gx > 0) { sn += sp[k & 3][r + 1][c]; dg += C(1); }
+        if (gx < nx - 1) { sn += sp[k & 3][r + 1][c + 2]; dg += C(1); }
+        if (gy > 0) { sn += sp[k & 3][r][c + 1]; dg += C(1); }
+        if (gy < ny - 1) { sn += sp[k & 3][r + 2][c + 1]; }
+        if (k > 0) { sn += sp[(k - 1) & 3][r + 1][c + 1]; dg += C(1); }
+        if (k < nz - 1) { sn += sp[(k + 1) & 3][r + 1][c + 1]; dg += C(1); }
+        v = (sn - h2 * bb) / dg;
+      }
+      sp[k & 3][r + 1][c + 1] = v;
+    }
+  };
+  // black cells of plane k on the tile, written out with the red cells
+  const int tx = threadIdx.x, ty = threadIdx.y;
+  const int gx = x0 + tx, gy = y0 + ty;
+  const bool inside = gx < nx && gy < ny;
+  auto black_and_write = [&](int k) {
+    if (!inside) return;
+    C v;
+    if ((gx + gy + k) & 1) {
+      C sn = C(0), dg = (gy < ny - 1) ? C(1) : topc;
+      const int r = ty + 2, c = tx + 2;
+      if (gx > 0) { sn += sp[k & 3][r][c - 1]; dg += C(1); }
+      if (gx < nx - 1) { sn += sp[k & 3][r][c + 1]; dg += C(1); }
+      if (gy > 0) { sn += sp[k & 3][r - 1][c]; dg += C(1); }
+      if (gy < ny - 1) { sn += sp[k & 3][r + 1][c]; }
+      if (k > 0) { sn += sp[(k - 1) & 3][r][c]; dg += C(1); }
+      if (k < nz - 1) { sn += sp[(k + 1) & 3][r][c]; dg += C(1); }
+      v = (sn - h2 * sb[k & 1][ty + 1][tx + 1]) / dg;
+    } else {
+      v = sp[k & 3][ty + 2][tx + 2];
+    }
+    xout[k * plane + (size_t)gy * nx + gx] = store_as<S, C>(v);
+  };
+
+  // Register ring of depth two: at the start of iteration k, ring k&1 holds
+  // plane k+2 of xin and plane k+1 of b; the plane four ahead is fetched into
+  // it as soon as it is stored, so two planes of loads are in flight.
+  C pA[SW_PPT], pB[SW_PPT], bA[SW_BPT], bB[SW_BPT];
+  auto iter = [&](int k, C (&pr)[SW_PPT], C (&br)[SW_BPT]) {
+    if (!ZERO) store_p(k + 2, pr);
+    store_b(k + 1, br);
+    __syncthreads();
+    if (!ZERO) fetch_p(k + 4, pr);
+    fetch_b(k + 3, br);
+    red(k + 1);
+    __syncthreads();
+    black_and_write(k);
+    __syncthreads();
+  };
+
+  // prologue: planes k0-2 .. k0+1 of xin, b planes k0-1, k0; red on k0-1, k0
+  {
+    C t0[SW_PPT], t1[SW_PPT], t2[SW_PPT], t3[SW_PPT], u0[SW_BPT], u1[SW_BPT];
+    if (!ZERO) {
+      fetch_p(k0 - 2, t0);
+      fetch_p(k0 - 1, t1);
+      fetch_p(k0, t2);
+      fetch_p(k0 + 1, t3);
+    }
+    fetch_b(k0 - 1, u0);
+    fetch_b(k0, u1);
+    if (!ZERO) {
+      store_p(k0 - 2, t0);
+      store_p(k0 - 1, t1);
+      store_p(k0, t2);
+      store_p(k0 + 1, t3);
+    }
+    store_b(k0 - 1, u0);
+    store_b(k0, u1);
+  }
+  if (!ZERO) {
+    fetch_p(k0 + 2, pA);
+    fetch_p(k0 + 3, pB);
+  }
+  fetch_b(k0 + 1, bA);
+  fetch_b(k0 + 2, bB);
+  __syncthreads();
+  red(k0 - 1);
+  red(k0);
+  __syncthreads();
+  for (int k = k0; k < k1; k += 2) {
+    iter(k, pA, bA);
+    if (k + 1 < k1) iter(k + 1, pB, bB);
   }
 }
 

@@ -48,9 +48,11 @@ k_residual64(Dims d, const double *__restrict__ p, const double *__restrict__ b,
   }
 }
 
-// Fold n (max, sum, sum) triples; one block of 256.
+// Fold n (max, sum, sum) triples; one block of 256.  Also sets the scale of
+// the next correction V-cycle: scale[0] = 1/max|r|, scale[1] = max|r|.
 __global__ void __launch_bounds__(256)
-k_fold3(const double *__restrict__ part, size_t n, double *__restrict__ out) {
+k_fold3(const double *__restrict__ part, size_t n, double *__restrict__ out,
+        double *__restrict__ scale) {
   double mx = 0.0, e = 0.0, f = 0.0;
   for (size_t q = threadIdx.x; q < n; q += 256) {
     double a = part[3 * q];
@@ -71,14 +73,19 @@ k_fold3(const double *__restrict__ part, size_t n, double *__restrict__ out) {
     out[0] = a;
     out[1] = g;
     out[2] = h;
+    // right-hand-side scale of the next correction cycle and its inverse
+    scale[0] = a > 0.0 ? 1.0 / a : 1.0;
+    scale[1] = a;
   }
 }
 
-// p64 += scale * e (correction of one refinement iteration)
+// p64 += max|r| * e: the correction of one refinement iteration, solved
+// for the right-hand side r / max|r|
 __global__ void __launch_bounds__(256)
-k_update64(size_t n, double *__restrict__ p, const float *__restrict__ e, double scale) {
+k_update64(size_t n, double *__restrict__ p, const float *__restrict__ e,
+           const double *__restrict__ scale) {
   size_t t = (size_t)blockIdx.x * 256 + threadIdx.x;
-  if (t < n) p[t] += scale * (double)e[t];
+  if (t < n) p[t] += scale[1] * (double)e[t];
 }
 
 }  // namespace mac3d

@@ -51,19 +51,16 @@ __device__ inline void trace_back(const Dims &d, const T *__restrict__ cu,
                                   const T *__restrict__ cw, const T sc[3],
                                   const T f[3], T out[3]) {
   const T lo = T(0.002), hi = T(0.998);
-  T p0x = clampv(f[0], lo, hi), p0y = clampv(f[1], lo, hi), p0z = clampv(f[2], lo, hi);
-  T v0x = sample_cell(cu, d, p0x, p0y, p0z);
-  T v0y = sample_cell(cv, d, p0x, p0y, p0z);
-  T v0z = sample_cell(cw, d, p0x, p0y, p0z);
-  T mx = clampv(f[0] - T(0.5) * (v0x * sc[0]), lo, hi);
-  T my = clampv(f[1] - T(0.5) * (v0y * sc[1]), lo, hi);
-  T mz = clampv(f[2] - T(0.5) * (v0z * sc[2]), lo, hi);
-  T vmx = sample_cell(cu, d, mx, my, mz);
-  T vmy = sample_cell(cv, d, mx, my, mz);
-  T vmz = sample_cell(cw, d, mx, my, mz);
-  out[0] = clampv(f[0] - vmx * sc[0], lo, hi);
-  out[1] = clampv(f[1] - vmy * sc[1], lo, hi);
-  out[2] = clampv(f[2] - vmz * sc[2], lo, hi);
+  T v0[3], vm[3];
+  sample_cell3(cu, cv, cw, d, clampv(f[0], lo, hi), clampv(f[1], lo, hi),
+               clampv(f[2], lo, hi), v0);
+  T mx = clampv(f[0] - T(0.5) * (v0[0] * sc[0]), lo, hi);
+  T my = clampv(f[1] - T(0.5) * (v0[1] * sc[1]), lo, hi);
+  T mz = clampv(f[2] - T(0.5) * (v0[2] * sc[2]), lo, hi);
+  sample_cell3(cu, cv, cw, d, mx, my, mz, vm);
+  out[0] = clampv(f[0] - vm[0] * sc[0], lo, hi);
+  out[1] = clampv(f[1] - vm[1] * sc[1], lo, hi);
+  out[2] = clampv(f[2] - vm[2] * sc[2], lo, hi);
 }
 
 // FLUID_ADVECT_MAC (fluid.ts:144-192, 323-325): each staggered component is
@@ -229,18 +226,15 @@ __device__ inline void dye_back(const Dims &d, const T *__restrict__ cu,
                                 const T *__restrict__ cv, const T *__restrict__ cw,
                                 const T sc[3], const T uvw[3], T v0[3], T back[3]) {
   const T lo = T(0.001), hi = T(0.999);
-  v0[0] = sample_cell(cu, d, uvw[0], uvw[1], uvw[2]);
-  v0[1] = sample_cell(cv, d, uvw[0], uvw[1], uvw[2]);
-  v0[2] = sample_cell(cw, d, uvw[0], uvw[1], uvw[2]);
+  sample_cell3(cu, cv, cw, d, uvw[0], uvw[1], uvw[2], v0);
   T mx = clampv(uvw[0] - T(0.5) * (v0[0] * sc[0]), lo, hi);
   T my = clampv(uvw[1] - T(0.5) * (v0[1] * sc[1]), lo, hi);
   T mz = clampv(uvw[2] - T(0.5) * (v0[2] * sc[2]), lo, hi);
-  T vmx = sample_cell(cu, d, mx, my, mz);
-  T vmy = sample_cell(cv, d, mx, my, mz);
-  T vmz = sample_cell(cw, d, mx, my, mz);
-  back[0] = clampv(uvw[0] - vmx * sc[0], lo, hi);
-  back[1] = clampv(uvw[1] - vmy * sc[1], lo, hi);
-  back[2] = clampv(uvw[2] - vmz * sc[2], lo, hi);
+  T vm[3];
+  sample_cell3(cu, cv, cw, d, mx, my, mz, vm);
+  back[0] = clampv(uvw[0] - vm[0] * sc[0], lo, hi);
+  back[1] = clampv(uvw[1] - vm[1] * sc[1], lo, hi);
+  back[2] = clampv(uvw[2] - vm[2] * sc[2], lo, hi);
 }
 
 template <class T>

@@ -164,3 +164,22 @@ def test_step_rejects_bad_dt():
             with pytest.raises(Mac3dError):
                 s.step(bad, 1)
         assert np.array_equal(before, s.download("V"))
+
+
+@pytest.mark.parametrize("prec", [0, 1])
+def test_graph_replays_match_eager(prec):
+    """Steps replayed from captured graphs (the default path) against the
+    same steps launched eagerly (profiling switches graphs off), one step per
+    call with downloads in between: every replay is bit-identical."""
+    w, s, o = pair(precision=prec, warm_steps=0)
+    from mac3d import MacSolver
+    from mac3d.lib import config_for
+    with s, MacSolver(config_for(w)) as e:
+        e.init_blobs(w.blobs)
+        e.profile(True)
+        for step in range(6):
+            s.step(w.dt, 1)
+            e.step(w.dt, 1)
+            for f in ("U", "V", "W", "P", "DYE"):
+                a, b = s.download(f), e.download(f)
+                assert np.array_equal(a, b), (step, f)
