@@ -21,6 +21,7 @@
 #include "mg_kernels.hpp"
 #include "mixed_kernels.hpp"
 #include "step_kernels.hpp"
+#include "adjoint_kernels.hpp"
 #include "profile.hpp"
 
 using namespace mac3d;
@@ -81,6 +82,19 @@ struct Ctx {
   };
   std::map<uint64_t, Graph> graphs;
   bool use_graphs = true;
+  // adjoint work space (allocated on first use)
+  struct Adj {
+    bool ready = false;
+    void *c1[3] = {}, *vela[3] = {}, *lvel[3] = {}, *lvelc[3] = {}, *lvela[3] = {}, *lnew[3] = {};
+    void *lc[3] = {}, *ldye = nullptr, *ldyen = nullptr, *lpred = nullptr, *target = nullptr;
+    Buf lx, lxa, lb;  // adjoint pressure solve on level 0
+    double *part = nullptr, *acc = nullptr;  // partial sums; acc[0] = d loss / d rho_b
+    size_t part_n = 0;
+    // states (u, v, w, p, dye) of the checkpoints and of one segment
+    std::vector<void *> ckpt, seg;
+    int ckpt_cap = 0, seg_cap = 0;
+  } adj;
+  bool adj_solving = false;
   // coarse solve
   double *cq = nullptr;  // qx | qy | qz | lx | ly | lz
   size_t cq_off[6] = {0, 0, 0, 0, 0, 0};
@@ -336,10 +350,10 @@ int launch_dye_resolve(Ctx *ctx, double dt) {
 // ---- multigrid
 void swap_x(Ctx *ctx, int l) {
   std::swap(ctx->mgx[l], ctx->mgxa[l]);
-  if (l == 0 && ctx->prec != MAC3D_PREC_MIXED) ctx->p = ctx->mgx[0];
+  if (l == 0 && ctx->prec != MAC3D_PREC_MIXED && !ctx->adj_solving) ctx->p = ctx->mgx[0];
 }
 
-// z-chunk of the sweep kernel: about three 512-thread blocks per CU
+// z-chunk of the level-0 sweep kernel: about three 512-thread blocks per CU
 // (256 CUs) in one round, chunks of at least 8 planes
 int sweep_cz(const Dims &d) {
   int cols = ((d.nx + SW_TX - 1) / SW_TX) * ((d.ny + SW_TY - 1) / SW_TY);
@@ -348,32 +362,51 @@ int sweep_cz(const Dims &d) {
   return std::max(cz, std::min(d.nz, 8));
 }
 
-// One red-black sweep on level l, mgx -> mgxa, then the two swap roles.
-// Level 0's right-hand side is multiplied by scale[0] (mixed mode: 1/max|r|).
+// One red-black sweep on level l.  Level 0: the fused out-of-place kernel,
+// mgx -> mgxa, then the two swap roles; its right-hand side is multiplied by
+// scale[0] (mixed mode: 1/max|r|).  Coarser levels: two in-place colour passes.
 int mg_sweep(Ctx *ctx, int l, bool zero) {
   const Dims &d = ctx->d[l];
-  int cz = sweep_cz(d);
-  int ncx = (d.nx + SW_TX - 1) / SW_TX, ncy = (d.ny + SW_TY - 1) / SW_TY;
-  dim3 grid(ncx * ncy * ((d.nz + cz - 1) / cz));
-  const double *sc = (l == 0) ? ctx->scale : nullptr;
-  {
-    ProfScope ps(ctx->prof, ctx->stream, l == 0 ? KC_SMOOTH0 : KC_MG);
+  if (l > 0) {
+    ProfScope ps(ctx->prof, ctx->stream, KC_MG);
+    dim3 grid(((d.nx + 1) / 2 + 63) / 64, (d.ny + 3) / 4, d.nz);
     with_type(ctx->mgx[l].type, [&](auto tag) {
       using S = std::remove_pointer_t<decltype(tag)>;
       using C = typename compute_of<S>::type;
-      C h2 = (C)level_h2(l), tc = (C)(l == 0 ? 1.0 : top_coeff(l));
+      C h2 = (C)level_h2(l), tc = (C)top_coeff(l);
       if (zero)
-        hipLaunchKernelGGL((k_rbgs_sweep<S, true>), grid, dim3(SW_TX, SW_TY), 0, ctx->stream, d,
-                           (const S *)ctx->mgx[l].ptr, (S *)ctx->mgxa[l].ptr,
-                           (const S *)ctx->mgb[l].ptr, h2, tc, sc, cz, ncx, ncy);
+        hipLaunchKernelGGL((k_rbgs_pass<S, true>), grid, kCellBlock, 0, ctx->stream, d,
+                           (S *)ctx->mgx[l].ptr, (const S *)ctx->mgb[l].ptr, h2, tc, 0);
       else
-        hipLaunchKernelGGL((k_rbgs_sweep<S, false>), grid, dim3(SW_TX, SW_TY), 0, ctx->stream, d,
-                           (const S *)ctx->mgx[l].ptr, (S *)ctx->mgxa[l].ptr,
-                           (const S *)ctx->mgb[l].ptr, h2, tc, sc, cz, ncx, ncy);
+        hipLaunchKernelGGL((k_rbgs_pass<S, false>), grid, kCellBlock, 0, ctx->stream, d,
+                           (S *)ctx->mgx[l].ptr, (const S *)ctx->mgb[l].ptr, h2, tc, 0);
+      hipLaunchKernelGGL((k_rbgs_pass<S, false>), grid, kCellBlock, 0, ctx->stream, d,
+                         (S *)ctx->mgx[l].ptr, (const S *)ctx->mgb[l].ptr, h2, tc, 1);
+    });
+    LAUNCHCHK();
+    return MAC3D_OK;
+  }
+  int cz = sweep_cz(d);
+  int ncx = (d.nx + SW_TX - 1) / SW_TX, ncy = (d.ny + SW_TY - 1) / SW_TY;
+  dim3 grid(ncx * ncy * ((d.nz + cz - 1) / cz));
+  const double *sc = ctx->scale;
+  {
+    ProfScope ps(ctx->prof, ctx->stream, KC_SMOOTH0);
+    with_step_type(ctx->mgx[0].type, [&](auto tag) {
+      using S = std::remove_pointer_t<decltype(tag)>;
+      using C = typename compute_of<S>::type;
+      if (zero)
+        hipLaunchKernelGGL((k_rbgs_sweep<S, true>), grid, dim3(SW_THREADS), 0, ctx->stream, d,
+                           (const S *)ctx->mgx[0].ptr, (S *)ctx->mgxa[0].ptr,
+                           (const S *)ctx->mgb[0].ptr, (C)1, (C)1, sc, cz, ncx, ncy);
+      else
+        hipLaunchKernelGGL((k_rbgs_sweep<S, false>), grid, dim3(SW_THREADS), 0, ctx->stream, d,
+                           (const S *)ctx->mgx[0].ptr, (S *)ctx->mgxa[0].ptr,
+                           (const S *)ctx->mgb[0].ptr, (C)1, (C)1, sc, cz, ncx, ncy);
     });
   }
   LAUNCHCHK();
-  swap_x(ctx, l);
+  swap_x(ctx, 0);
   return MAC3D_OK;
 }
 
@@ -820,6 +853,354 @@ int reset_scale(Ctx *ctx) {
   return MAC3D_OK;
 }
 
+int init_blobs_impl(Ctx *ctx, const mac3d_blob *blobs, int nblobs) {
+  if (nblobs < 0 || (nblobs > 0 && !blobs)) return fail(ctx, MAC3D_EINVAL, "bad blob list");
+  for (int q = 0; q < nblobs; ++q) {
+    const mac3d_blob &b = blobs[q];
+    if (!std::isfinite(b.cx) || !std::isfinite(b.cy) || !std::isfinite(b.cz) ||
+        !std::isfinite(b.w0) || !(b.radius > 0.5) || !std::isfinite(b.radius))
+      return fail(ctx, MAC3D_EINVAL, "blob fields must be finite with radius > 0.5");
+  }
+  if (nblobs > ctx->blobs_cap) {
+    if (ctx->blobs) HIPCHK(hipFree(ctx->blobs));
+    ctx->blobs = nullptr;
+    HIPCHK(hipMalloc(&ctx->blobs, sizeof(BlobDev) * nblobs));
+    ctx->blobs_cap = nblobs;
+  }
+  std::vector<BlobDev> hb(nblobs);
+  for (int q = 0; q < nblobs; ++q)
+    hb[q] = BlobDev{blobs[q].cx, blobs[q].cy, blobs[q].cz, blobs[q].radius, blobs[q].w0};
+  if (nblobs)
+    HIPCHK(hipMemcpyAsync(ctx->blobs, hb.data(), sizeof(BlobDev) * nblobs, hipMemcpyHostToDevice,
+                          ctx->stream));
+  size_t n0 = ctx->n[0];
+  Buf *zero[] = {&ctx->u, &ctx->w, &ctx->p, &ctx->e0, &ctx->cu, &ctx->cv, &ctx->cw};
+  for (Buf *b : zero)
+    if (b->ptr) HIPCHK(hipMemsetAsync(b->ptr, 0, type_size(b->type) * n0, ctx->stream));
+  const Dims &d = ctx->d[0];
+  with_step_type(ctx->t0, [&](auto tag) {
+    using T = std::remove_pointer_t<decltype(tag)>;
+    hipLaunchKernelGGL(k_init_blobs<T>, cell_grid(d), kCellBlock, 0, ctx->stream, d,
+                       (const BlobDev *)ctx->blobs, nblobs, (T *)ctx->dye.ptr, (T *)ctx->v.ptr);
+  });
+  LAUNCHCHK();
+  HIPCHK(hipStreamSynchronize(ctx->stream));
+  return MAC3D_OK;
+}
+
+
+// ===========================================================================
+// Adjoint of the step (no reference counterpart): reverse sweep with
+// checkpoints every K steps, states recomputed in between.
+enum { SV_U, SV_V, SV_W, SV_P, SV_DYE };
+
+int adj_alloc(Ctx *ctx) {
+  Ctx::Adj &A = ctx->adj;
+  if (A.ready) return MAC3D_OK;
+  int rc;
+  size_t n0 = ctx->n[0], bytes = type_size(ctx->t0) * n0;
+  void **arrs[] = {&A.c1[0], &A.c1[1], &A.c1[2], &A.vela[0], &A.vela[1], &A.vela[2],
+                   &A.lvel[0], &A.lvel[1], &A.lvel[2], &A.lvelc[0], &A.lvelc[1], &A.lvelc[2],
+                   &A.lvela[0], &A.lvela[1], &A.lvela[2], &A.lnew[0], &A.lnew[1], &A.lnew[2],
+                   &A.lc[0], &A.lc[1], &A.lc[2], &A.ldye, &A.ldyen, &A.lpred, &A.target};
+  for (void **a : arrs)
+    if ((rc = dev_alloc(ctx, a, bytes))) return rc;
+  int tx = ctx->mgx[0].type;
+  if ((rc = alloc(ctx, A.lx, tx, n0))) return rc;
+  if ((rc = alloc(ctx, A.lxa, tx, n0))) return rc;
+  if ((rc = alloc(ctx, A.lb, tx, n0))) return rc;
+  A.part_n = std::max(cell_blocks(ctx->d[0]), (size_t)1024 * 5 * 16);
+  if ((rc = dev_alloc(ctx, (void **)&A.part, A.part_n * sizeof(double)))) return rc;
+  if ((rc = dev_alloc(ctx, (void **)&A.acc, 4 * sizeof(double)))) return rc;
+  A.ready = true;
+  return MAC3D_OK;
+}
+
+size_t state_bytes(const Ctx *ctx, int f) {
+  return type_size(f == SV_P ? ctx->p.type : ctx->t0) * ctx->n[0];
+}
+
+int states_alloc(Ctx *ctx, std::vector<void *> &v, int &cap, int count) {
+  int rc;
+  while (cap < count) {
+    for (int f = 0; f < 5; ++f) {
+      void *p = nullptr;
+      if ((rc = dev_alloc(ctx, &p, state_bytes(ctx, f)))) return rc;
+      v.push_back(p);
+    }
+    ++cap;
+  }
+  return MAC3D_OK;
+}
+
+void *state_field(Ctx *ctx, int f) {
+  switch (f) {
+    case SV_U: return ctx->u.ptr;
+    case SV_V: return ctx->v.ptr;
+    case SV_W: return ctx->w.ptr;
+    case SV_P: return ctx->p.ptr;
+    default: return ctx->dye.ptr;
+  }
+}
+
+int save_state(Ctx *ctx, void *const *dst) {
+  for (int f = 0; f < 5; ++f)
+    HIPCHK(hipMemcpyAsync(dst[f], state_field(ctx, f), state_bytes(ctx, f),
+                          hipMemcpyDeviceToDevice, ctx->stream));
+  return MAC3D_OK;
+}
+
+int load_state(Ctx *ctx, void *const *src) {
+  for (int f = 0; f < 5; ++f)
+    HIPCHK(hipMemcpyAsync(state_field(ctx, f), src[f], state_bytes(ctx, f),
+                          hipMemcpyDeviceToDevice, ctx->stream));
+  return MAC3D_OK;
+}
+
+int zero_dev(Ctx *ctx, void *p, size_t bytes) {
+  HIPCHK(hipMemsetAsync(p, 0, bytes, ctx->stream));
+  return MAC3D_OK;
+}
+
+// lam_rhs = L^-1 lam_p with the step's V-cycle (L is symmetric): cycles
+// V-cycles from zero on the adjoint buffers; returns the solution buffer.
+int adj_pressure_solve(Ctx *ctx, Buf *sol) {
+  Ctx::Adj &A = ctx->adj;
+  Buf sx = ctx->mgx[0], sxa = ctx->mgxa[0], sb = ctx->mgb[0];
+  ctx->adj_solving = true;
+  ctx->mgx[0] = A.lx;
+  ctx->mgxa[0] = A.lxa;
+  ctx->mgb[0] = A.lb;
+  int rc = reset_scale(ctx);
+  if (!rc) rc = mg_fill_zero(ctx, ctx->mgx[0]);
+  for (int c = 0; c < std::max(ctx->cfg.cycles, 1) && !rc; ++c) rc = vcycle(ctx, false);
+  *sol = ctx->mgx[0];
+  // keep the pair's identities for the next solve
+  A.lx = ctx->mgx[0];
+  A.lxa = ctx->mgxa[0];
+  ctx->mgx[0] = sx;
+  ctx->mgxa[0] = sxa;
+  ctx->mgb[0] = sb;
+  ctx->adj_solving = false;
+  return rc;
+}
+
+// One reverse step.  The context holds the step's input state X_n (a copy of
+// src); lvel/ldye hold the adjoint of X_{n+1} on entry and of X_n on exit.
+int adj_step(Ctx *ctx, double dt, void *const *src) {
+  Ctx::Adj &A = ctx->adj;
+  const Dims &d = ctx->d[0];
+  size_t n0 = ctx->n[0], tb = type_size(ctx->t0) * n0;
+  int rc, cyc = 0, it = 0, conv = 0;
+  double rel = 0;
+  // forward recompute of the stage inputs
+  if ((rc = load_state(ctx, src))) return rc;
+  if ((rc = launch_face_to_centre(ctx))) return rc;
+  void *cc[3] = {ctx->cu.ptr, ctx->cv.ptr, ctx->cw.ptr};
+  for (int a = 0; a < 3; ++a)
+    HIPCHK(hipMemcpyAsync(A.c1[a], cc[a], tb, hipMemcpyDeviceToDevice, ctx->stream));
+  if ((rc = launch_advect(ctx, dt))) return rc;
+  void *va[3] = {ctx->u.ptr, ctx->v.ptr, ctx->w.ptr};
+  for (int a = 0; a < 3; ++a)
+    HIPCHK(hipMemcpyAsync(A.vela[a], va[a], tb, hipMemcpyDeviceToDevice, ctx->stream));
+  if ((rc = launch_casson(ctx, dt))) return rc;
+  if ((rc = launch_divergence(ctx))) return rc;
+  if (ctx->prec == MAC3D_PREC_MIXED) {
+    bool g = ctx->use_graphs;
+    ctx->use_graphs = false;
+    // the refinement loop of one_step without its pre/post stages
+    if ((rc = residual64(ctx))) return rc;
+    for (;;) {
+      HIPCHK(hipStreamSynchronize(ctx->stream));
+      double mr = ctx->red_host[0], rr = ctx->red_host[1], bb = ctx->red_host[2];
+      double r = bb > 0.0 ? std::sqrt(rr / bb) : 0.0;
+      if (r <= ctx->cfg.ir_tol || mr == 0.0 || it >= ctx->cfg.ir_max_iters) break;
+      if ((rc = vcycle(ctx, true)) || (rc = update64(ctx)) || (rc = residual64(ctx))) return rc;
+      ++it;
+    }
+    ctx->use_graphs = g;
+  } else {
+    for (int c = 0; c < ctx->cfg.cycles; ++c)
+      if ((rc = vcycle(ctx, false))) return rc;
+  }
+  (void)cyc; (void)conv; (void)rel;
+  if ((rc = launch_project(ctx))) return rc;
+  if ((rc = launch_face_to_centre(ctx))) return rc;  // C2
+  if ((rc = launch_dye_predict(ctx, dt))) return rc;
+  double dom[3] = {d.nx * ctx->cfg.dx, d.ny * ctx->cfg.dx, d.nz * ctx->cfg.dx};
+  StepParams P = step_params(ctx, dt);
+  // reverse
+  if ((rc = zero_dev(ctx, A.lpred, tb)) || (rc = zero_dev(ctx, A.ldyen, tb))) return rc;
+  for (int a = 0; a < 3; ++a)
+    if ((rc = zero_dev(ctx, A.lc[a], tb))) return rc;
+  with_step_type(ctx->t0, [&](auto tag) {
+    using T = std::remove_pointer_t<decltype(tag)>;
+    T s0 = (T)(dt / dom[0]), s1 = (T)(dt / dom[1]), s2 = (T)(dt / dom[2]);
+    const T *dyen = (const T *)src[SV_DYE];
+    hipLaunchKernelGGL(k_adj_dye_resolve<T>, cell_grid(d), kCellBlock, 0, ctx->stream, d, s0, s1,
+                       s2, (T)ctx->cfg.maccormack, (const T *)ctx->cu.ptr, (const T *)ctx->cv.ptr,
+                       (const T *)ctx->cw.ptr, dyen, (const T *)ctx->dpred.ptr,
+                       (const T *)A.ldye, (T *)A.ldyen, (T *)A.lpred, (T *)A.lc[0],
+                       (T *)A.lc[1], (T *)A.lc[2]);
+    hipLaunchKernelGGL(k_adj_dye_predict<T>, cell_grid(d), kCellBlock, 0, ctx->stream, d, s0, s1,
+                       s2, (const T *)ctx->cu.ptr, (const T *)ctx->cv.ptr, (const T *)ctx->cw.ptr,
+                       dyen, (const T *)A.lpred, (T *)A.ldyen, (T *)A.lc[0], (T *)A.lc[1],
+                       (T *)A.lc[2]);
+    hipLaunchKernelGGL(k_adj_face_to_centre<T>, cell_grid(d), kCellBlock, 0, ctx->stream, d,
+                       (const T *)A.lc[0], (const T *)A.lc[1], (const T *)A.lc[2],
+                       (T *)A.lvel[0], (T *)A.lvel[1], (T *)A.lvel[2]);
+  });
+  LAUNCHCHK();
+  // lam_p = D(lam') then the adjoint pressure solve
+  with_step_type(ctx->t0, [&](auto tag) {
+    using T = std::remove_pointer_t<decltype(tag)>;
+    with_step_type(A.lb.type, [&](auto rtag) {
+      using R = std::remove_pointer_t<decltype(rtag)>;
+      hipLaunchKernelGGL((k_adj_project_p<T, R>), cell_grid(d), kCellBlock, 0, ctx->stream, d,
+                         (const T *)A.lvel[0], (const T *)A.lvel[1], (const T *)A.lvel[2],
+                         (R *)A.lb.ptr);
+    });
+  });
+  LAUNCHCHK();
+  Buf lrhs;
+  if ((rc = adj_pressure_solve(ctx, &lrhs))) return rc;
+  with_step_type(ctx->t0, [&](auto tag) {
+    using T = std::remove_pointer_t<decltype(tag)>;
+    with_step_type(lrhs.type, [&](auto rtag) {
+      using R = std::remove_pointer_t<decltype(rtag)>;
+      hipLaunchKernelGGL((k_adj_div<T, R>), cell_grid(d), kCellBlock, 0, ctx->stream, d,
+                         (const T *)A.lvel[0], (const T *)A.lvel[1], (const T *)A.lvel[2],
+                         (const R *)lrhs.ptr, (T *)A.lvelc[0], (T *)A.lvelc[1], (T *)A.lvelc[2]);
+    });
+  });
+  LAUNCHCHK();
+  for (int a = 0; a < 3; ++a)
+    if ((rc = zero_dev(ctx, A.lvela[a], tb)) || (rc = zero_dev(ctx, A.lnew[a], tb)) ||
+        (rc = zero_dev(ctx, A.lc[a], tb)))
+      return rc;
+  with_step_type(ctx->t0, [&](auto tag) {
+    using T = std::remove_pointer_t<decltype(tag)>;
+    const T *dyen = (const T *)src[SV_DYE];
+    hipLaunchKernelGGL(k_adj_casson<T>, cell_grid(d), kCellBlock, 0, ctx->stream, d, P,
+                       (const T *)A.vela[0], (const T *)A.vela[1], (const T *)A.vela[2], dyen,
+                       (const T *)A.lvelc[0], (const T *)A.lvelc[1], (const T *)A.lvelc[2],
+                       (T *)A.lvela[0], (T *)A.lvela[1], (T *)A.lvela[2], (T *)A.ldyen, A.part);
+    hipLaunchKernelGGL(k_sum_into, dim3(1), dim3(256), 0, ctx->stream, (const double *)A.part,
+                       cell_blocks(d), A.acc);
+    T s0 = (T)(dt / dom[0]), s1 = (T)(dt / dom[1]), s2 = (T)(dt / dom[2]);
+    hipLaunchKernelGGL(k_adj_advect<T>, cell_grid(d), kCellBlock, 0, ctx->stream, d, s0, s1, s2,
+                       (T)ctx->cfg.vel_damp, (const T *)src[SV_U], (const T *)src[SV_V],
+                       (const T *)src[SV_W], (const T *)A.c1[0], (const T *)A.c1[1],
+                       (const T *)A.c1[2], (const T *)A.lvela[0], (const T *)A.lvela[1],
+                       (const T *)A.lvela[2], (T *)A.lnew[0], (T *)A.lnew[1], (T *)A.lnew[2],
+                       (T *)A.lc[0], (T *)A.lc[1], (T *)A.lc[2]);
+    hipLaunchKernelGGL(k_adj_face_to_centre<T>, cell_grid(d), kCellBlock, 0, ctx->stream, d,
+                       (const T *)A.lc[0], (const T *)A.lc[1], (const T *)A.lc[2],
+                       (T *)A.lnew[0], (T *)A.lnew[1], (T *)A.lnew[2]);
+  });
+  LAUNCHCHK();
+  for (int a = 0; a < 3; ++a) std::swap(A.lvel[a], A.lnew[a]);
+  std::swap(A.ldye, A.ldyen);
+  return MAC3D_OK;
+}
+
+int adjoint_impl(Ctx *ctx, const mac3d_blob *blobs, int nb, double dt, int nsteps, int K,
+                 const void *target, int dtype, mac3d_grad *out) {
+  int rc;
+  Ctx::Adj &A = ctx->adj;
+  if ((rc = adj_alloc(ctx))) return rc;
+  int nck = (nsteps + K - 1) / K;
+  if ((rc = states_alloc(ctx, A.ckpt, A.ckpt_cap, std::max(nck, 1)))) return rc;
+  if ((rc = states_alloc(ctx, A.seg, A.seg_cap, K))) return rc;
+  size_t n0 = ctx->n[0], tb = type_size(ctx->t0) * n0;
+  // forward with checkpoints
+  if ((rc = init_blobs_impl(ctx, blobs, nb))) return rc;
+  if ((rc = save_state(ctx, &A.ckpt[0]))) return rc;
+  int cyc = 0, it = 0, conv = 0;
+  double rel = 0;
+  for (int n = 0; n < nsteps; ++n) {
+    if ((rc = one_step(ctx, dt, &cyc, &it, &rel, &conv))) return rc;
+    if ((n + 1) % K == 0 && n + 1 < nsteps)
+      if ((rc = save_state(ctx, &A.ckpt[5 * ((n + 1) / K)]))) return rc;
+  }
+  // loss and seed: lam_dye = dye_N - target
+  HIPCHK(hipMemcpyAsync(ctx->stage, target, (dtype == MAC3D_F32 ? 4 : 8) * n0,
+                        hipMemcpyHostToDevice, ctx->stream));
+  dim3 g1((unsigned)((n0 + 255) / 256));
+  with_step_type(ctx->t0, [&](auto tag) {
+    using T = std::remove_pointer_t<decltype(tag)>;
+    if (dtype == MAC3D_F32)
+      hipLaunchKernelGGL((k_convert<float, T>), g1, dim3(256), 0, ctx->stream,
+                         (const float *)ctx->stage, (T *)A.target, n0);
+    else
+      hipLaunchKernelGGL((k_convert<double, T>), g1, dim3(256), 0, ctx->stream,
+                         (const double *)ctx->stage, (T *)A.target, n0);
+  });
+  LAUNCHCHK();
+  double *lossp = nullptr;
+  if ((rc = dev_alloc(ctx, (void **)&lossp, (g1.x + 4) * sizeof(double)))) return rc;
+  with_step_type(ctx->t0, [&](auto tag) {
+    using T = std::remove_pointer_t<decltype(tag)>;
+    hipLaunchKernelGGL(k_sub_sq<T>, g1, dim3(256), 0, ctx->stream, n0, (const T *)ctx->dye.ptr,
+                       (const T *)A.target, (T *)A.ldye, lossp);
+  });
+  LAUNCHCHK();
+  std::vector<double> lp(g1.x);
+  HIPCHK(hipMemcpyAsync(lp.data(), lossp, g1.x * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+  for (int a = 0; a < 3; ++a)
+    if ((rc = zero_dev(ctx, A.lvel[a], tb))) return rc;
+  HIPCHK(hipMemsetAsync(A.acc, 0, 4 * sizeof(double), ctx->stream));
+  HIPCHK(hipStreamSynchronize(ctx->stream));
+  double loss = 0.0;
+  for (double x : lp) loss += x;
+  loss *= 0.5;
+  // reverse sweep over the segments
+  for (int sgi = nck - 1; sgi >= 0; --sgi) {
+    int n0s = sgi * K, len = std::min(K, nsteps - n0s);
+    if ((rc = load_state(ctx, &A.ckpt[5 * sgi]))) return rc;
+    for (int sgs = 0; sgs < len; ++sgs) {
+      if ((rc = save_state(ctx, &A.seg[5 * sgs]))) return rc;
+      if (sgs + 1 < len)
+        if ((rc = one_step(ctx, dt, &cyc, &it, &rel, &conv))) return rc;
+    }
+    for (int sgs = len - 1; sgs >= 0; --sgs)
+      if ((rc = adj_step(ctx, dt, &A.seg[5 * sgs]))) return rc;
+  }
+  // parameters of the initial condition
+  std::vector<BlobDev> hb(nb);
+  for (int q = 0; q < nb; ++q)
+    hb[q] = BlobDev{blobs[q].cx, blobs[q].cy, blobs[q].cz, blobs[q].radius, blobs[q].w0};
+  HIPCHK(hipMemcpyAsync(ctx->blobs, hb.data(), sizeof(BlobDev) * nb, hipMemcpyHostToDevice,
+                        ctx->stream));
+  const int nblk = 1024, npar = 5 * nb;
+  with_step_type(ctx->t0, [&](auto tag) {
+    using T = std::remove_pointer_t<decltype(tag)>;
+    hipLaunchKernelGGL(k_adj_init<T>, dim3(nblk), dim3(256), 4 * npar * sizeof(double),
+                       ctx->stream, ctx->d[0], (const BlobDev *)ctx->blobs, nb,
+                       (const T *)A.ldye, (const T *)A.lvel[1], A.part);
+  });
+  LAUNCHCHK();
+  std::vector<double> hp((size_t)nblk * npar);
+  double acc[4];
+  HIPCHK(hipMemcpyAsync(hp.data(), A.part, hp.size() * sizeof(double), hipMemcpyDeviceToHost,
+                        ctx->stream));
+  HIPCHK(hipMemcpyAsync(acc, A.acc, 4 * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+  HIPCHK(hipStreamSynchronize(ctx->stream));
+  std::memset(out, 0, sizeof(*out));
+  out->loss = loss;
+  out->d_rho_b = acc[0];
+  out->nblobs = nb;
+  out->steps = nsteps;
+  for (int q = 0; q < npar; ++q) {
+    double s = 0.0;
+    for (int b = 0; b < nblk; ++b) s += hp[(size_t)b * npar + q];
+    out->d_blob[q / 5][q % 5] = s;
+  }
+  HIPCHK(hipFree(lossp));
+  ctx->allocs.erase(std::find(ctx->allocs.begin(), ctx->allocs.end(), (void *)lossp));
+  return MAC3D_OK;
+}
+
 thread_local std::string g_create_err;
 
 }  // namespace
@@ -928,39 +1309,7 @@ int mac3d_download(void *h, int field, int level, void *host, int dtype, size_t 
 
 int mac3d_init_blobs(void *h, const mac3d_blob *blobs, int nblobs) {
   if (!h) return MAC3D_EINVAL;
-  Ctx *ctx = (Ctx *)h;
-  if (nblobs < 0 || (nblobs > 0 && !blobs)) return fail(ctx, MAC3D_EINVAL, "bad blob list");
-  for (int q = 0; q < nblobs; ++q) {
-    const mac3d_blob &b = blobs[q];
-    if (!std::isfinite(b.cx) || !std::isfinite(b.cy) || !std::isfinite(b.cz) ||
-        !std::isfinite(b.w0) || !(b.radius > 0.5) || !std::isfinite(b.radius))
-      return fail(ctx, MAC3D_EINVAL, "blob fields must be finite with radius > 0.5");
-  }
-  if (nblobs > ctx->blobs_cap) {
-    if (ctx->blobs) HIPCHK(hipFree(ctx->blobs));
-    ctx->blobs = nullptr;
-    HIPCHK(hipMalloc(&ctx->blobs, sizeof(BlobDev) * nblobs));
-    ctx->blobs_cap = nblobs;
-  }
-  std::vector<BlobDev> hb(nblobs);
-  for (int q = 0; q < nblobs; ++q)
-    hb[q] = BlobDev{blobs[q].cx, blobs[q].cy, blobs[q].cz, blobs[q].radius, blobs[q].w0};
-  if (nblobs)
-    HIPCHK(hipMemcpyAsync(ctx->blobs, hb.data(), sizeof(BlobDev) * nblobs, hipMemcpyHostToDevice,
-                          ctx->stream));
-  size_t n0 = ctx->n[0];
-  Buf *zero[] = {&ctx->u, &ctx->w, &ctx->p, &ctx->e0, &ctx->cu, &ctx->cv, &ctx->cw};
-  for (Buf *b : zero)
-    if (b->ptr) HIPCHK(hipMemsetAsync(b->ptr, 0, type_size(b->type) * n0, ctx->stream));
-  const Dims &d = ctx->d[0];
-  with_step_type(ctx->t0, [&](auto tag) {
-    using T = std::remove_pointer_t<decltype(tag)>;
-    hipLaunchKernelGGL(k_init_blobs<T>, cell_grid(d), kCellBlock, 0, ctx->stream, d,
-                       (const BlobDev *)ctx->blobs, nblobs, (T *)ctx->dye.ptr, (T *)ctx->v.ptr);
-  });
-  LAUNCHCHK();
-  HIPCHK(hipStreamSynchronize(ctx->stream));
-  return MAC3D_OK;
+  return init_blobs_impl((Ctx *)h, blobs, nblobs);
 }
 
 int mac3d_step(void *h, double dt, int nsteps, mac3d_stats *stats) {
@@ -1006,6 +1355,22 @@ int mac3d_step(void *h, double dt, int nsteps, mac3d_stats *stats) {
     stats->max_div_rate = mx / ctx->cfg.dx;
   }
   return MAC3D_OK;
+}
+
+int mac3d_adjoint(void *h, const mac3d_blob *blobs, int nblobs, double dt, int nsteps,
+                  int checkpoint_every, const void *target, int dtype, mac3d_grad *out) {
+  if (!h) return MAC3D_EINVAL;
+  Ctx *ctx = (Ctx *)h;
+  if (!out || !target) return fail(ctx, MAC3D_EINVAL, "null target or result");
+  if (nblobs < 1 || nblobs > MAC3D_MAX_BLOBS || !blobs)
+    return fail(ctx, MAC3D_EINVAL, "need 1..MAC3D_MAX_BLOBS blobs");
+  if (!std::isfinite(dt) || !(dt > 0.0)) return fail(ctx, MAC3D_EINVAL, "dt must be finite and > 0");
+  if (nsteps < 1 || checkpoint_every < 1)
+    return fail(ctx, MAC3D_EINVAL, "nsteps and checkpoint_every must be >= 1");
+  if (dtype != MAC3D_F32 && dtype != MAC3D_F64) return fail(ctx, MAC3D_EINVAL, "unknown dtype");
+  int rc = adjoint_impl(ctx, blobs, nblobs, dt, nsteps, checkpoint_every, target, dtype, out);
+  ctx->prof.collect();
+  return rc;
 }
 
 int mac3d_stage(void *h, int stage, double dt) {

@@ -215,29 +215,72 @@ k_coarse_xz_inv(Dims d, const double *__restrict__ work, const double *__restric
 namespace mac3d {
 
 // ---------------------------------------------------------------------------
-// Red-black Gauss-Seidel sweep (MG_RBGS_RED then MG_RBGS_BLACK,
-// fluid.ts:645-681): cells with (i+j+k)&1 == 0 get x = (sum - h^2 b)/diag from
-// the old values, then the (i+j+k)&1 == 1 cells from the new red values.  Both
-// colour passes run in one kernel, out of place (xin -> xout): 12 B per FP32
-// cell per sweep instead of 24 for two in-place colour passes.  Each block owns a TX x TY column tile and marches
-// over a z-chunk [k0, k1).  Planes of xin are staged in LDS with a two-cell
-// ring; red is recomputed on the tile plus a one-cell ring (what the tile's
-// black cells read), so no block waits on another.  The arithmetic of each
-// update uses the neighbour order of mg_stencil, so the result equals two
-// in-place colour passes bit for bit.  Registers prefetch the next plane.
-// ZERO: the sweep starts from a zero guess (MG_RBGS_RED_ZERO, fluid.ts:605-641)
-// and never reads xin.  scale (nullable): the right-hand side is b * scale[0].
+// One colour pass of red-black Gauss-Seidel in place (MG_RBGS_RED /
+// MG_RBGS_BLACK, fluid.ts:645-681): cells with (i+j+k)&1 == parity get
+// x = (sum - h^2 b)/diag.  Used on the coarse levels, where a grid holds too
+// few planes for the z-marching sweep to pay.  Thread t of a row takes the
+// pair (2t, 2t+1) and updates the one of this colour.  ZERO: the sweep starts
+// from a zero guess (MG_RBGS_RED_ZERO, fluid.ts:605-641): the other colour is
+// written as 0 and no neighbour is read.
+template <class S, bool ZERO>
+__global__ void __launch_bounds__(256)
+k_rbgs_pass(Dims d, S *__restrict__ x, const S *__restrict__ b,
+            typename compute_of<S>::type h2, typename compute_of<S>::type topc, int parity) {
+  typedef typename compute_of<S>::type C;
+  int t = blockIdx.x * blockDim.x + threadIdx.x;
+  int j = blockIdx.y * blockDim.y + threadIdx.y;
+  int k = blockIdx.z;
+  int ia = 2 * t;
+  if (j >= d.ny || ia >= d.nx) return;
+  int i = ia + ((ia + j + k + parity) & 1);
+  if (ZERO) {
+    int io = ia + ((ia + j + k + parity + 1) & 1);
+    if (io < d.nx) x[idx3(d, io, j, k)] = S(0);
+  }
+  if (i >= d.nx) return;
+  size_t c = idx3(d, i, j, k);
+  C sn, dg;
+  if (ZERO) {
+    sn = C(0);
+    dg = (j < d.ny - 1) ? C(1) : topc;
+    if (i > 0) dg += C(1);
+    if (i < d.nx - 1) dg += C(1);
+    if (j > 0) dg += C(1);
+    if (k > 0) dg += C(1);
+    if (k < d.nz - 1) dg += C(1);
+  } else {
+    mg_stencil<S, C>(x, d, topc, i, j, k, c, sn, dg);
+  }
+  x[c] = store_as<S, C>((sn - h2 * (C)b[c]) / dg);
+}
+
+// ---------------------------------------------------------------------------
+// Level-0 red-black sweep: both colour passes in one kernel, out of place
+// (xin -> xout): 12 B per FP32 cell per sweep instead of 24 for two in-place
+// colour passes.  Each 512-thread block owns a 64 x 16 column tile and
+// marches over a z-chunk [k0, k1).  Planes of xin and b are staged in LDS
+// with a two-cell ring (8-byte pair loads: x0 - 2 is even); red is recomputed
+// on the tile plus a one-cell ring (what the tile's black cells read), so no
+// block waits on another.  Threads work on (even, odd) x-pairs, each pair
+// holding one cell of each colour, so no lane idles on a parity test.
+// Registers prefetch two planes ahead.  ZERO: zero initial guess (xin is not
+// read).  scale (nullable): the right-hand side is b * scale[0].
 #define SW_TX 64
-#define SW_TY 8
-#define SW_EX (SW_TX + 4)
+#define SW_TY 16
+#define SW_EX (SW_TX + 4)                 // staged row, two-cell ring
 #define SW_EY (SW_TY + 4)
-#define SW_RX (SW_TX + 2)
-#define SW_RY (SW_TY + 2)
-#define SW_THREADS (SW_TX * SW_TY)
-#define SW_PL (SW_EX * SW_EY)  // 816 staged values per xin plane
-#define SW_BL (SW_RX * SW_RY)  // 660 staged values per b plane
-#define SW_PPT ((SW_PL + SW_THREADS - 1) / SW_THREADS)  // 2 per thread
-#define SW_BPT ((SW_BL + SW_THREADS - 1) / SW_THREADS)  // 2 per thread
+#define SW_THREADS 512
+#define SW_NV (SW_EX / 2 * SW_EY)         // 680 staged pairs per plane
+#define SW_LPT ((SW_NV + SW_THREADS - 1) / SW_THREADS)   // 2
+#define SW_RP ((SW_TX + 2) / 2 * (SW_TY + 2))           // 594 red pairs
+#define SW_RPT ((SW_RP + SW_THREADS - 1) / SW_THREADS)  // 2
+
+template <class S> struct vec2_of;
+template <> struct vec2_of<float> { typedef float2 type; };
+template <> struct vec2_of<double> { typedef double2 type; };
+
+__device__ inline float fast_rcp(float x) { return __builtin_amdgcn_rcpf(x); }
+__device__ inline double fast_rcp(double x) { return 1.0 / x; }
 
 // Blocks are numbered on a 1-D grid and remapped so that each of the 8 XCDs
 // (blocks are dealt round-robin to XCDs) gets a contiguous run of tiles: the
@@ -254,123 +297,119 @@ k_rbgs_sweep(Dims d, const S *__restrict__ xin, S *__restrict__ xout, const S *_
              typename compute_of<S>::type h2, typename compute_of<S>::type topc,
              const double *__restrict__ scale, int cz, int ncx, int ncy) {
   typedef typename compute_of<S>::type C;
+  typedef typename vec2_of<S>::type V2;
   __shared__ C sp[4][SW_EY][SW_EX];
-  __shared__ C sb[2][SW_RY][SW_RX];
+  __shared__ C sb[2][SW_EY][SW_EX];
   const int nx = d.nx, ny = d.ny, nz = d.nz;
   const int lb = xcd_remap(blockIdx.x, gridDim.x);
   const int x0 = (lb % ncx) * SW_TX, y0 = ((lb / ncx) % ncy) * SW_TY;
   const int k0 = (lb / (ncx * ncy)) * cz;
   const int k1 = min(k0 + cz, nz);
-  const int tid = threadIdx.y * SW_TX + threadIdx.x;
+  const int tid = threadIdx.x;
   const size_t plane = (size_t)nx * ny;
   const C bs = scale ? (C)scale[0] : C(1);
 
-  // plane k of xin (two-cell ring) / of b (one-cell ring) into registers
-  auto fetch_p = [&](int k, C (&pre)[SW_PPT]) {
+  // staged pairs of this thread (fixed over the march)
+  int st_r[SW_LPT], st_c[SW_LPT];
+  long st_g[SW_LPT];  // in-plane offset, -1 when outside the grid or the stage
 #pragma unroll
-    for (int q = 0; q < SW_PPT; ++q) {
-      int t = tid + q * SW_THREADS;
-      C v = C(0);
-      if (t < SW_PL && k >= 0 && k < nz) {
-        int gx = x0 - 2 + t % SW_EX, gy = y0 - 2 + t / SW_EX;
-        if (gx >= 0 && gx < nx && gy >= 0 && gy < ny) v = (C)xin[k * plane + (size_t)gy * nx + gx];
-      }
+  for (int q = 0; q < SW_LPT; ++q) {
+    int v = tid + q * SW_THREADS;
+    st_r[q] = v / (SW_EX / 2);
+    st_c[q] = 2 * (v % (SW_EX / 2));
+    int gx = x0 - 2 + st_c[q], gy = y0 - 2 + st_r[q];
+    st_g[q] = (v < SW_NV && gx >= 0 && gx < nx && gy >= 0 && gy < ny) ? (long)gy * nx + gx : -1;
+  }
+  auto fetch = [&](const S *__restrict__ f, int k, V2 (&pre)[SW_LPT]) {
+#pragma unroll
+    for (int q = 0; q < SW_LPT; ++q) {
+      V2 v;
+      v.x = S(0);
+      v.y = S(0);
+      if (st_g[q] >= 0 && k >= 0 && k < nz) v = *(const V2 *)(f + k * plane + st_g[q]);
       pre[q] = v;
     }
   };
-  auto store_p = [&](int k, const C (&pre)[SW_PPT]) {
+  auto stage_p = [&](int k, const V2 (&pre)[SW_LPT]) {
 #pragma unroll
-    for (int q = 0; q < SW_PPT; ++q) {
-      int t = tid + q * SW_THREADS;
-      if (t < SW_PL) sp[k & 3][t / SW_EX][t % SW_EX] = pre[q];
-    }
-  };
-  auto fetch_b = [&](int k, C (&pre)[SW_BPT]) {
-#pragma unroll
-    for (int q = 0; q < SW_BPT; ++q) {
-      int t = tid + q * SW_THREADS;
-      C v = C(0);
-      if (t < SW_BL && k >= 0 && k < nz) {
-        int gx = x0 - 1 + t % SW_RX, gy = y0 - 1 + t / SW_RX;
-        if (gx >= 0 && gx < nx && gy >= 0 && gy < ny) v = (C)b[k * plane + (size_t)gy * nx + gx] * bs;
+    for (int q = 0; q < SW_LPT; ++q)
+      if (tid + q * SW_THREADS < SW_NV) {
+        sp[k & 3][st_r[q]][st_c[q]] = (C)pre[q].x;
+        sp[k & 3][st_r[q]][st_c[q] + 1] = (C)pre[q].y;
       }
-      pre[q] = v;
-    }
   };
-  auto store_b = [&](int k, const C (&pre)[SW_BPT]) {
+  auto stage_b = [&](int k, const V2 (&pre)[SW_LPT]) {
 #pragma unroll
-    for (int q = 0; q < SW_BPT; ++q) {
-      int t = tid + q * SW_THREADS;
-      if (t < SW_BL) sb[k & 1][t / SW_RX][t % SW_RX] = pre[q];
-    }
+    for (int q = 0; q < SW_LPT; ++q)
+      if (tid + q * SW_THREADS < SW_NV) {
+        sb[k & 1][st_r[q]][st_c[q]] = (C)pre[q].x * bs;
+        sb[k & 1][st_r[q]][st_c[q] + 1] = (C)pre[q].y * bs;
+      }
   };
-  // diagonal of evalMgStencil at (gx, gy, k)
-  auto diag_at = [&](int gx, int gy, int k) {
-    C dg = (gy < ny - 1) ? C(1) : topc;
-    if (gx > 0) dg += C(1);
-    if (gx < nx - 1) dg += C(1);
-    if (gy > 0) dg += C(1);
-    if (k > 0) dg += C(1);
-    if (k < nz - 1) dg += C(1);
-    return dg;
+
+  // red pairs of this thread on the tile plus one-cell ring: left cell at
+  // grid (rgx, rgy), stage (rr, rc); flags per cell of the pair
+  int rr[SW_RPT], rc[SW_RPT], rgx[SW_RPT], rgy[SW_RPT];
+  bool rv[SW_RPT];
+#pragma unroll
+  for (int q = 0; q < SW_RPT; ++q) {
+    int v = tid + q * SW_THREADS;
+    int row = v / ((SW_TX + 2) / 2), pc = v % ((SW_TX + 2) / 2);
+    rr[q] = row + 1;
+    rc[q] = 2 * pc + 1;
+    rgx[q] = x0 - 1 + 2 * pc;
+    rgy[q] = y0 - 1 + row;
+    rv[q] = v < SW_RP && rgy[q] >= 0 && rgy[q] < ny;
+  }
+  // stencil update of the cell at stage (r, c), grid (gx, gy, k)
+  // (nbrs false: red cells of a zero-guess sweep, whose neighbours are 0)
+  auto update = [&](int r, int c, int gx, int gy, int k, bool nbrs) {
+    C sn = C(0), dg = (gy < ny - 1) ? C(1) : topc;
+    if (gx > 0) { if (nbrs) sn += sp[k & 3][r][c - 1]; dg += C(1); }
+    if (gx < nx - 1) { if (nbrs) sn += sp[k & 3][r][c + 1]; dg += C(1); }
+    if (gy > 0) { if (nbrs) sn += sp[k & 3][r - 1][c]; dg += C(1); }
+    if (gy < ny - 1) { if (nbrs) sn += sp[k & 3][r + 1][c]; }
+    if (k > 0) { if (nbrs) sn += sp[(k - 1) & 3][r][c]; dg += C(1); }
+    if (k < nz - 1) { if (nbrs) sn += sp[(k + 1) & 3][r][c]; dg += C(1); }
+    return (sn - h2 * sb[k & 1][r][c]) * fast_rcp(dg);
   };
-  // red cells of plane k on the tile plus one-cell ring, in LDS
   auto red = [&](int k) {
     if (k < 0 || k >= nz) return;
-    for (int t = tid; t < SW_BL; t += SW_THREADS) {
-      int c = t % SW_RX, r = t / SW_RX;
-      int gx = x0 - 1 + c, gy = y0 - 1 + r;
-      if (gx < 0 || gx >= nx || gy < 0 || gy >= ny || ((gx + gy + k) & 1)) continue;
-      C bb = sb[k & 1][r][c];
-      C v;
-      if (ZERO) {
-        v = (-h2 * bb) / diag_at(gx, gy, k);
-      } else {
-        C sn = C(0), dg = (gy < ny - 1) ? C(1) : topc;
-        if (gx > 0) { sn += sp[k & 3][r + 1][c]; dg += C(1); }
-        if (gx < nx - 1) { sn += sp[k & 3][r + 1][c + 2]; dg += C(1); }
-        if (gy > 0) { sn += sp[k & 3][r][c + 1]; dg += C(1); }
-        if (gy < ny - 1) { sn += sp[k & 3][r + 2][c + 1]; }
-        if (k > 0) { sn += sp[(k - 1) & 3][r + 1][c + 1]; dg += C(1); }
-        if (k < nz - 1) { sn += sp[(k + 1) & 3][r + 1][c + 1]; dg += C(1); }
-        v = (sn - h2 * bb) / dg;
-      }
-      sp[k & 3][r + 1][c + 1] = v;
+#pragma unroll
+    for (int q = 0; q < SW_RPT; ++q) {
+      if (!rv[q]) continue;
+      int w = (rgx[q] + rgy[q] + k) & 1;  // 0: left cell red, 1: right cell red
+      int gx = rgx[q] + w;
+      if (gx < 0 || gx >= nx) continue;
+      sp[k & 3][rr[q]][rc[q] + w] = update(rr[q], rc[q] + w, gx, rgy[q], k, !ZERO);
     }
   };
-  // black cells of plane k on the tile, written out with the red cells
-  const int tx = threadIdx.x, ty = threadIdx.y;
-  const int gx = x0 + tx, gy = y0 + ty;
-  const bool inside = gx < nx && gy < ny;
+  // black pair of this thread on the tile
+  const int bty = tid / (SW_TX / 2), bpx = tid % (SW_TX / 2);
+  const int bgx = x0 + 2 * bpx, bgy = y0 + bty;
+  const int br = bty + 2, bc = 2 * bpx + 2;
+  const bool binside = bgx < nx && bgy < ny;
   auto black_and_write = [&](int k) {
-    if (!inside) return;
-    C v;
-    if ((gx + gy + k) & 1) {
-      C sn = C(0), dg = (gy < ny - 1) ? C(1) : topc;
-      const int r = ty + 2, c = tx + 2;
-      if (gx > 0) { sn += sp[k & 3][r][c - 1]; dg += C(1); }
-      if (gx < nx - 1) { sn += sp[k & 3][r][c + 1]; dg += C(1); }
-      if (gy > 0) { sn += sp[k & 3][r - 1][c]; dg += C(1); }
-      if (gy < ny - 1) { sn += sp[k & 3][r + 1][c]; }
-      if (k > 0) { sn += sp[(k - 1) & 3][r][c]; dg += C(1); }
-      if (k < nz - 1) { sn += sp[(k + 1) & 3][r][c]; dg += C(1); }
-      v = (sn - h2 * sb[k & 1][ty + 1][tx + 1]) / dg;
-    } else {
-      v = sp[k & 3][ty + 2][tx + 2];
-    }
-    xout[k * plane + (size_t)gy * nx + gx] = store_as<S, C>(v);
+    if (!binside) return;
+    int w = (bgx + bgy + k + 1) & 1;  // 0: left cell black, 1: right cell black
+    C vb = update(br, bc + w, bgx + w, bgy, k, true);
+    C vr = sp[k & 3][br][bc + 1 - w];
+    V2 o;
+    o.x = store_as<S, C>(w ? vr : vb);
+    o.y = store_as<S, C>(w ? vb : vr);
+    *(V2 *)(xout + k * plane + (size_t)bgy * nx + bgx) = o;
   };
 
   // Register ring of depth two: at the start of iteration k, ring k&1 holds
   // plane k+2 of xin and plane k+1 of b; the plane four ahead is fetched into
-  // it as soon as it is stored, so two planes of loads are in flight.
-  C pA[SW_PPT], pB[SW_PPT], bA[SW_BPT], bB[SW_BPT];
-  auto iter = [&](int k, C (&pr)[SW_PPT], C (&br)[SW_BPT]) {
-    if (!ZERO) store_p(k + 2, pr);
-    store_b(k + 1, br);
+  // it as soon as it is staged, so two planes of loads are in flight.
+  V2 pA[SW_LPT], pB[SW_LPT], bA[SW_LPT], bB[SW_LPT];
+  auto iter = [&](int k, V2 (&pr)[SW_LPT], V2 (&br_)[SW_LPT]) {
+    if (!ZERO) stage_p(k + 2, pr);
+    stage_b(k + 1, br_);
     __syncthreads();
-    if (!ZERO) fetch_p(k + 4, pr);
-    fetch_b(k + 3, br);
+    if (!ZERO) fetch(xin, k + 4, pr);
+    fetch(b, k + 3, br_);
     red(k + 1);
     __syncthreads();
     black_and_write(k);
@@ -379,30 +418,30 @@ k_rbgs_sweep(Dims d, const S *__restrict__ xin, S *__restrict__ xout, const S *_
 
   // prologue: planes k0-2 .. k0+1 of xin, b planes k0-1, k0; red on k0-1, k0
   {
-    C t0[SW_PPT], t1[SW_PPT], t2[SW_PPT], t3[SW_PPT], u0[SW_BPT], u1[SW_BPT];
+    V2 t0[SW_LPT], t1[SW_LPT], t2[SW_LPT], t3[SW_LPT], u0[SW_LPT], u1[SW_LPT];
     if (!ZERO) {
-      fetch_p(k0 - 2, t0);
-      fetch_p(k0 - 1, t1);
-      fetch_p(k0, t2);
-      fetch_p(k0 + 1, t3);
+      fetch(xin, k0 - 2, t0);
+      fetch(xin, k0 - 1, t1);
+      fetch(xin, k0, t2);
+      fetch(xin, k0 + 1, t3);
     }
-    fetch_b(k0 - 1, u0);
-    fetch_b(k0, u1);
+    fetch(b, k0 - 1, u0);
+    fetch(b, k0, u1);
     if (!ZERO) {
-      store_p(k0 - 2, t0);
-      store_p(k0 - 1, t1);
-      store_p(k0, t2);
-      store_p(k0 + 1, t3);
+      stage_p(k0 - 2, t0);
+      stage_p(k0 - 1, t1);
+      stage_p(k0, t2);
+      stage_p(k0 + 1, t3);
     }
-    store_b(k0 - 1, u0);
-    store_b(k0, u1);
+    stage_b(k0 - 1, u0);
+    stage_b(k0, u1);
   }
   if (!ZERO) {
-    fetch_p(k0 + 2, pA);
-    fetch_p(k0 + 3, pB);
+    fetch(xin, k0 + 2, pA);
+    fetch(xin, k0 + 3, pB);
   }
-  fetch_b(k0 + 1, bA);
-  fetch_b(k0 + 2, bB);
+  fetch(b, k0 + 1, bA);
+  fetch(b, k0 + 2, bB);
   __syncthreads();
   red(k0 - 1);
   red(k0);

@@ -29,6 +29,9 @@ export declare class MacSolver {
   sync(): this;
   write(name: keyof typeof FIELD, data: Float32Array | Float64Array): void;
   read<T extends Float32Array | Float64Array>(name: keyof typeof FIELD, out: T): T;
+  backward(target: Float32Array | Float64Array,
+           run: { blobs: Blob[]; dt: number; steps: number; checkpointEvery?: number }):
+    { loss: number; dRhoB: number; dBlobs: Blob[] };
   divNorms(): { maxDiv: number; l2Div: number };
   destroy(): void;
 }

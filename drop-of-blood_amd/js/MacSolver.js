@@ -98,6 +98,23 @@ class MacSolver {
     return out;
   }
 
+  /**
+   * Gradient of L = 1/2 ||dye_N - target||^2 for a run of `steps` steps of
+   * `dt` from the blob initial condition, by the hand-written adjoint with a
+   * checkpoint every `checkpointEvery` steps (recomputed in between).
+   * Leaves the solver at an unspecified state of that run.
+   * @param {Float32Array|Float64Array} target one dye value per cell
+   * @param {{blobs:{cx:number, cy:number, cz:number, radius:number, w0:number}[],
+   *   dt:number, steps:number, checkpointEvery?:number}} run
+   * @returns {{loss:number, dRhoB:number,
+   *   dBlobs:{cx:number, cy:number, cz:number, radius:number, w0:number}[]}}
+   */
+  backward(target, run) {
+    if (!Number.isFinite(run.dt) || run.dt <= 0) throw new RangeError('dt must be finite and > 0');
+    return addon.adjoint(this._live(), run.blobs, run.dt, run.steps, run.checkpointEvery || 10,
+      target);
+  }
+
   divNorms() {
     return addon.divNorms(this._live());
   }

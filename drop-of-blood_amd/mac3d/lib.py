@@ -42,6 +42,18 @@ class Stats(C.Structure):
         return {n: getattr(self, n) for n, _ in self._fields_}
 
 
+MAX_BLOBS = 16
+
+
+class Grad(C.Structure):
+    _fields_ = [("loss", C.c_double), ("d_rho_b", C.c_double), ("nblobs", C.c_int32),
+                ("steps", C.c_int32), ("d_blob", (C.c_double * 5) * MAX_BLOBS)]
+
+    def as_dict(self):
+        return {"loss": self.loss, "d_rho_b": self.d_rho_b, "steps": self.steps,
+                "d_blob": [[self.d_blob[q][p] for p in range(5)] for q in range(self.nblobs)]}
+
+
 class Mac3dError(RuntimeError):
     def __init__(self, code, msg):
         super().__init__(f"mac3d {STATUS.get(code, code)}: {msg}")
@@ -51,8 +63,8 @@ class Mac3dError(RuntimeError):
 _lib = None
 EXPORTS = ("mac3d_default_config", "mac3d_create", "mac3d_destroy", "mac3d_upload",
            "mac3d_download", "mac3d_init_blobs", "mac3d_step", "mac3d_stage", "mac3d_mg_op",
-           "mac3d_div_norms", "mac3d_profile", "mac3d_profile_read", "mac3d_last_error",
-           "mac3d_version")
+           "mac3d_div_norms", "mac3d_profile", "mac3d_profile_read", "mac3d_adjoint",
+           "mac3d_last_error", "mac3d_version")
 
 
 def load_library(path: Optional[str] = None):
@@ -77,6 +89,8 @@ def load_library(path: Optional[str] = None):
     L.mac3d_div_norms.argtypes = [vp, C.POINTER(C.c_double), C.POINTER(C.c_double)]
     L.mac3d_profile.argtypes = [vp, C.c_int]
     L.mac3d_profile_read.argtypes = [vp, C.c_int, C.POINTER(C.c_double), C.POINTER(C.c_int64)]
+    L.mac3d_adjoint.argtypes = [vp, C.POINTER(Blob), C.c_int, C.c_double, C.c_int, C.c_int, vp,
+                                C.c_int, C.POINTER(Grad)]
     L.mac3d_last_error.argtypes = [vp]
     L.mac3d_last_error.restype = C.c_char_p
     L.mac3d_version.restype = C.c_char_p
@@ -185,6 +199,19 @@ class MacSolver:
 
     def step_nostats(self, dt: float, n: int = 1):
         self._check(self.L.mac3d_step(self.h, dt, n, None))
+
+    def adjoint(self, blobs: Sequence, dt: float, steps: int, target, checkpoint_every: int = 10):
+        """Gradient of 1/2 ||dye_N - target||^2 with respect to the blob
+        parameters (cx, cy, cz, radius, w0 per blob) and rho_b, by the
+        hand-written adjoint with checkpoints every `checkpoint_every` steps."""
+        arr = (Blob * len(blobs))(*[Blob(b.cx, b.cy, b.cz, b.radius, b.w0) for b in blobs])
+        t = np.ascontiguousarray(target, dtype=np.float64)
+        if t.size != self.cfg.nx * self.cfg.ny * self.cfg.nz:
+            raise ValueError("target must have one value per cell")
+        g = Grad()
+        self._check(self.L.mac3d_adjoint(self.h, arr, len(blobs), dt, steps, checkpoint_every,
+                                         t.ctypes.data, F64, C.byref(g)))
+        return g.as_dict()
 
     def stage(self, stage, dt: float = 0.0):
         sid = STAGE[stage] if isinstance(stage, str) else stage

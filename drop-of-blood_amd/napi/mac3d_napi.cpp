@@ -235,6 +235,63 @@ napi_value DivNorms(napi_env env, napi_callback_info info) {
   return o;
 }
 
+// (handle, blobs[], dt, steps, checkpointEvery, target Float32Array|Float64Array)
+napi_value Adjoint(napi_env env, napi_callback_info info) {
+  napi_value argv[6];
+  if (!get_args(env, info, 6, argv)) return nullptr;
+  void *ctx = get_handle(env, argv[0]);
+  if (!ctx) return nullptr;
+  uint32_t n = 0;
+  NAPI_CALL(env, napi_get_array_length(env, argv[1], &n));
+  std::vector<mac3d_blob> blobs(n);
+  for (uint32_t q = 0; q < n; ++q) {
+    napi_value o;
+    NAPI_CALL(env, napi_get_element(env, argv[1], q, &o));
+    mac3d_blob &b = blobs[q];
+    b.cx = b.cy = b.cz = b.radius = b.w0 = NAN;
+    prop_num(env, o, "cx", &b.cx);
+    prop_num(env, o, "cy", &b.cy);
+    prop_num(env, o, "cz", &b.cz);
+    prop_num(env, o, "radius", &b.radius);
+    prop_num(env, o, "w0", &b.w0);
+  }
+  napi_typedarray_type tt;
+  size_t len = 0, off = 0;
+  void *data = nullptr;
+  napi_value ab;
+  bool is_ta = false;
+  napi_is_typedarray(env, argv[5], &is_ta);
+  if (!is_ta) {
+    napi_throw_type_error(env, nullptr, "target must be a Float32Array or Float64Array");
+    return nullptr;
+  }
+  NAPI_CALL(env, napi_get_typedarray_info(env, argv[5], &tt, &len, &data, &ab, &off));
+  if (tt != napi_float32_array && tt != napi_float64_array) {
+    napi_throw_type_error(env, nullptr, "target must be a Float32Array or Float64Array");
+    return nullptr;
+  }
+  mac3d_grad g;
+  std::memset(&g, 0, sizeof(g));
+  int rc = mac3d_adjoint(ctx, blobs.data(), (int)n, get_double(env, argv[2]), get_int(env, argv[3]),
+                         get_int(env, argv[4]), data,
+                         tt == napi_float32_array ? MAC3D_F32 : MAC3D_F64, &g);
+  if (rc) return throw_status(env, ctx, rc);
+  napi_value o, arr;
+  NAPI_CALL(env, napi_create_object(env, &o));
+  set_num(env, o, "loss", g.loss);
+  set_num(env, o, "dRhoB", g.d_rho_b);
+  NAPI_CALL(env, napi_create_array_with_length(env, n, &arr));
+  const char *names[5] = {"cx", "cy", "cz", "radius", "w0"};
+  for (uint32_t q = 0; q < n; ++q) {
+    napi_value e;
+    NAPI_CALL(env, napi_create_object(env, &e));
+    for (int p = 0; p < 5; ++p) set_num(env, e, names[p], g.d_blob[q][p]);
+    NAPI_CALL(env, napi_set_element(env, arr, q, e));
+  }
+  NAPI_CALL(env, napi_set_named_property(env, o, "dBlobs", arr));
+  return o;
+}
+
 napi_value Version(napi_env env, napi_callback_info) {
   napi_value s;
   NAPI_CALL(env, napi_create_string_utf8(env, mac3d_version(), NAPI_AUTO_LENGTH, &s));
@@ -252,6 +309,7 @@ napi_value Init(napi_env env, napi_value exports) {
       {"step", nullptr, Step, nullptr, nullptr, nullptr, napi_default, nullptr},
       {"stage", nullptr, Stage, nullptr, nullptr, nullptr, napi_default, nullptr},
       {"divNorms", nullptr, DivNorms, nullptr, nullptr, nullptr, napi_default, nullptr},
+      {"adjoint", nullptr, Adjoint, nullptr, nullptr, nullptr, napi_default, nullptr},
       {"version", nullptr, Version, nullptr, nullptr, nullptr, napi_default, nullptr},
   };
   napi_define_properties(env, exports, sizeof(props) / sizeof(props[0]), props);

@@ -139,6 +139,17 @@ typedef struct mac3d_stats {
   double ms;          /* wall time of the call (ms) */
 } mac3d_stats;
 
+#define MAC3D_MAX_BLOBS 16
+
+/* gradient of L = 1/2 ||dye_N - target||^2 (adjoint of the whole run) */
+typedef struct mac3d_grad {
+  double loss;
+  double d_rho_b;                     /* dL/d rho_b (density excess; rho_w fixed) */
+  int32_t nblobs;
+  int32_t steps;
+  double d_blob[MAC3D_MAX_BLOBS][5];  /* dL/d (cx, cy, cz, radius, w0) per blob */
+} mac3d_grad;
+
 /* replaces: constructor texture/pipeline setup, BloodDropEngine.ts:545-589,
  * 861-897, 1100-1260, and the SimUniforms parameter block (common.ts:26-43,
  * uniformLayout.ts:56-190) for the MAC step */
@@ -165,6 +176,16 @@ int mac3d_init_blobs(void *ctx, const mac3d_blob *blobs, int nblobs);
  * nsteps times at fixed dt, blocking until the stream completes.
  * dt must be finite and > 0 (EINVAL otherwise, state untouched). */
 int mac3d_step(void *ctx, double dt, int nsteps, mac3d_stats *stats);
+
+/* Adjoint of nsteps steps from the blob initial condition (no reference
+ * counterpart: the reference has no differentiation).  Runs the forward
+ * pass keeping the state every checkpoint_every steps, computes
+ * L = 1/2 ||dye_N - target||^2, and sweeps back step by step, recomputing
+ * each segment from its checkpoint; returns dL/d(blob parameters) and
+ * dL/d rho_b.  target: level-0 dye field (count = cells, dtype F32/F64).
+ * Leaves the context at an unspecified state of the run. */
+int mac3d_adjoint(void *ctx, const mac3d_blob *blobs, int nblobs, double dt, int nsteps,
+                  int checkpoint_every, const void *target, int dtype, mac3d_grad *out);
 
 /* one stage of the step (MAC3D_STAGE_*), blocking */
 int mac3d_stage(void *ctx, int stage, double dt);

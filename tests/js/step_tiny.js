@@ -22,3 +22,15 @@ const out = { stats: st, norms: { u: sum(s.u), v: sum(s.v), w: sum(s.w), p: sum(
 s.destroy();
 s.destroy();
 console.log(JSON.stringify(out));
+
+// adjoint through the TypeScript host: gradient of the tiny run's dye
+// against a zero target must be finite and match the C ABI's (checked by the
+// pytest driver)
+if (process.argv[3] === 'backward') {
+  const s2 = new MacSolver({ nx: w.nx, ny: w.ny, nz: w.nz, dx: w.dx, levels: w.levels,
+    cycles: w.cycles, precision: 2 });
+  const target = new Float64Array(s2.cells);
+  const g = s2.backward(target, { blobs: w.blobs, dt: w.dt, steps, checkpointEvery: 2 });
+  s2.destroy();
+  console.log(JSON.stringify({ backward: g }));
+}

@@ -32,3 +32,21 @@ def test_node_host_steps_tiny():
                       ("p", oracle.F_P), ("dye", oracle.F_DYE)):
         want = np.linalg.norm(o.get(fid))
         assert abs(out["norms"][name] - want) <= 1e-5 * want + 1e-12, name
+
+
+@pytest.mark.skipif(shutil.which("node") is None, reason="node not installed")
+def test_node_backward_matches_c_abi():
+    from mac3d import MacSolver
+    from mac3d.lib import config_for
+    steps = 3
+    r = subprocess.run(["node", os.path.join(ROOT, "tests", "js", "step_tiny.js"), str(steps),
+                        "backward"], capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stderr
+    js = json.loads(r.stdout.strip().splitlines()[-1])["backward"]
+    w = scenarios.scaled(scenarios.workload("tiny", 1), precision=2)
+    with MacSolver(config_for(w)) as s:
+        g = s.adjoint(w.blobs, w.dt, steps, np.zeros((w.nz, w.ny, w.nx)), checkpoint_every=2)
+    assert abs(js["loss"] - g["loss"]) <= 1e-12 * g["loss"]
+    for p, n in enumerate(["cx", "cy", "cz", "radius", "w0"]):
+        assert abs(js["dBlobs"][0][n] - g["d_blob"][0][p]) <= 1e-9 * abs(g["d_blob"][0][p]) + 1e-12
+    assert abs(js["dRhoB"] - g["d_rho_b"]) <= 1e-9 * abs(g["d_rho_b"]) + 1e-15
