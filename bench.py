@@ -23,8 +23,8 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, os.path.join(ROOT, "drop-of-blood_amd"))
 
 HBM_PEAK_GBPS = 8000.0
-# algorithmic bytes per level-0 cell of one red-black colour pass: read p and
-# rhs, write p (FP32)
+# algorithmic bytes per level-0 cell of one red-black sweep (both colours in
+# one kernel): read p and rhs, write p (FP32)
 SMOOTH_BYTES_PER_CELL = 12.0
 
 
@@ -58,6 +58,17 @@ def run_gpu(workload, steps, warmup, profile_steps=3):
         return st, wall, prof
 
 
+def copy_probe(workload, times=5):
+    """k_convert<float, float> over one field: reads and writes 4 B per cell."""
+    from mac3d import MacSolver
+    from mac3d.lib import config_for
+    import numpy as np
+    with MacSolver(config_for(workload)) as s:
+        out = np.empty((workload.nz, workload.ny, workload.nx), dtype=np.float32)
+        for _ in range(times):
+            s.download_into("U", out)
+
+
 def cpu_baseline(workload, steps=1):
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle
@@ -80,6 +91,9 @@ def main():
     ap.add_argument("--seed", type=int, default=1)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-mixed", action="store_true")
+    ap.add_argument("--copy-probe", action="store_true",
+                    help="after the run, 5 device copies of one FP32 field (a plain copy "
+                         "kernel of known size, to calibrate HBM counters)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}): the MAC step is "
@@ -115,7 +129,7 @@ def main():
         ach = bytes_launch / avg_s / 1e9
         roof = {"bound": "hbm", "achieved": round(ach, 1), "peak": HBM_PEAK_GBPS, "unit": "GB/s",
                 "frac": round(ach / HBM_PEAK_GBPS, 4), "traffic": None,
-                "kernel": "k_rbgs (level-0 red-black colour pass)",
+                "kernel": "k_rbgs_sweep<float> (level-0 red-black sweep, both colours)",
                 "avg_launch_us": round(avg_s * 1e6, 2), "launches": n,
                 "bytes_per_launch": bytes_launch}
         traffic_file = os.path.join(ROOT, "profiles", "rbgs_l0_192x256x192_traffic.json")
@@ -123,6 +137,8 @@ def main():
             with open(traffic_file) as f:
                 roof["traffic"] = json.load(f).get("bytes_per_launch")
     kernel_ms = {k: round(v[0] / v[1], 4) for k, v in prof.items()}
+    if args.copy_probe:
+        copy_probe(w32)
 
     cpu = None
     if not args.no_cpu:

@@ -59,6 +59,32 @@ __device__ inline void tri_grad(const T *__restrict__ f, const Dims &d, const Tr
   g[2] = dz * (T)d.nz;
 }
 
+// d sample / d position of the three packed centre-velocity components:
+// g[a][b] = d v_a / d position_b
+template <class T>
+__device__ inline void tri_grad4(const typename vec4_of<T>::type *__restrict__ f, const Dims &d,
+                                 const Tri<T> &t, T g[3][3]) {
+  auto c000 = f[t.r00 + t.i0], c100 = f[t.r00 + t.i1], c010 = f[t.r10 + t.i0], c110 = f[t.r10 + t.i1];
+  auto c001 = f[t.r01 + t.i0], c101 = f[t.r01 + t.i1], c011 = f[t.r11 + t.i0], c111 = f[t.r11 + t.i1];
+  T ey = T(1) - t.fy, ez = T(1) - t.fz;
+#define MAC3D_G(C, a)                                                                    \
+  {                                                                                      \
+    T dx = ey * ez * (c100.C - c000.C) + t.fy * ez * (c110.C - c010.C) +                 \
+           ey * t.fz * (c101.C - c001.C) + t.fy * t.fz * (c111.C - c011.C);              \
+    T x00 = mixv(c000.C, c100.C, t.fx), x10 = mixv(c010.C, c110.C, t.fx);                \
+    T x01 = mixv(c001.C, c101.C, t.fx), x11 = mixv(c011.C, c111.C, t.fx);                \
+    T dy = ez * (x10 - x00) + t.fz * (x11 - x01);                                        \
+    T dz = mixv(x01, x11, t.fy) - mixv(x00, x10, t.fy);                                  \
+    g[a][0] = dx * (T)d.nx;                                                              \
+    g[a][1] = dy * (T)d.ny;                                                              \
+    g[a][2] = dz * (T)d.nz;                                                              \
+  }
+  MAC3D_G(x, 0)
+  MAC3D_G(y, 1)
+  MAC3D_G(z, 2)
+#undef MAC3D_G
+}
+
 template <class T>
 __device__ inline void tri_scatter(T *__restrict__ lam, const Tri<T> &t, T val) {
   T ex = T(1) - t.fx, ey = T(1) - t.fy, ez = T(1) - t.fz;
@@ -135,8 +161,8 @@ k_adj_div(Dims d, const T *__restrict__ lu1, const T *__restrict__ lv1, const T 
 // the centred velocity at the cell.
 template <class T>
 __global__ void __launch_bounds__(256)
-k_adj_dye_resolve(Dims d, T sc0, T sc1, T sc2, T strength, const T *__restrict__ cu,
-                  const T *__restrict__ cv, const T *__restrict__ cw, const T *__restrict__ dye,
+k_adj_dye_resolve(Dims d, T sc0, T sc1, T sc2, T strength,
+                  const typename vec4_of<T>::type *__restrict__ cvel, const T *__restrict__ dye,
                   const T *__restrict__ pred, const T *__restrict__ ldye_out,
                   T *__restrict__ ldye, T *__restrict__ lpred, T *__restrict__ lcu,
                   T *__restrict__ lcv, T *__restrict__ lcw) {
@@ -147,7 +173,7 @@ k_adj_dye_resolve(Dims d, T sc0, T sc1, T sc2, T strength, const T *__restrict__
   const T lo = T(0.001), hi = T(0.999);
   T uvw[3] = {((T)i + T(0.5)) / (T)d.nx, ((T)j + T(0.5)) / (T)d.ny, ((T)k + T(0.5)) / (T)d.nz};
   T v0[3], back[3];
-  dye_back(d, cu, cv, cw, sc, uvw, v0, back);
+  dye_back(d, cvel, sc, uvw, v0, back);
   T fr[3], f[3];
   for (int a = 0; a < 3; ++a) {
     fr[a] = uvw[a] + v0[a] * sc[a];
@@ -195,20 +221,20 @@ k_adj_dye_resolve(Dims d, T sc0, T sc1, T sc2, T strength, const T *__restrict__
 // lo/hi: the clamp of the trace points; shift: the sample-point shift per
 // axis (MAC faces) with its own clamp sl/sh.
 template <class T>
-__device__ inline void adj_trace(const Dims &d, const T *__restrict__ cu, const T *__restrict__ cv,
-                                 const T *__restrict__ cw, const T sc[3], const T f[3],
-                                 bool clamp_p0, T lo, T hi, const T gback[3],
-                                 T *__restrict__ lcu, T *__restrict__ lcv, T *__restrict__ lcw) {
+__device__ inline void adj_trace(const Dims &d, const typename vec4_of<T>::type *__restrict__ cvel,
+                                 const T sc[3], const T f[3], bool clamp_p0, T lo, T hi,
+                                 const T gback[3], T *__restrict__ lcu, T *__restrict__ lcv,
+                                 T *__restrict__ lcw) {
   T p0[3], v0[3], midr[3], mid[3], vm[3], backr[3];
   for (int a = 0; a < 3; ++a) p0[a] = clamp_p0 ? clampv(f[a], lo, hi) : f[a];
   Tri<T> t0 = tri_setup(d, p0[0], p0[1], p0[2]);
-  sample_cell3(cu, cv, cw, d, p0[0], p0[1], p0[2], v0);
+  sample_vec<T>(cvel, d, p0[0], p0[1], p0[2], v0);
   for (int a = 0; a < 3; ++a) {
     midr[a] = f[a] - T(0.5) * (v0[a] * sc[a]);
     mid[a] = clampv(midr[a], lo, hi);
   }
   Tri<T> tm = tri_setup(d, mid[0], mid[1], mid[2]);
-  sample_cell3(cu, cv, cw, d, mid[0], mid[1], mid[2], vm);
+  sample_vec<T>(cvel, d, mid[0], mid[1], mid[2], vm);
   T gvm[3];
   for (int a = 0; a < 3; ++a) {
     backr[a] = f[a] - vm[a] * sc[a];
@@ -218,16 +244,10 @@ __device__ inline void adj_trace(const Dims &d, const T *__restrict__ cu, const 
   tri_scatter(lcv, tm, gvm[1]);
   tri_scatter(lcw, tm, gvm[2]);
   // d vm_a / d mid_b
-  T gmid[3] = {T(0), T(0), T(0)};
-  {
-    T ga[3];
-    tri_grad(cu, d, tm, ga);
-    for (int b = 0; b < 3; ++b) gmid[b] += gvm[0] * ga[b];
-    tri_grad(cv, d, tm, ga);
-    for (int b = 0; b < 3; ++b) gmid[b] += gvm[1] * ga[b];
-    tri_grad(cw, d, tm, ga);
-    for (int b = 0; b < 3; ++b) gmid[b] += gvm[2] * ga[b];
-  }
+  T ga[3][3];
+  tri_grad4<T>(cvel, d, tm, ga);
+  T gmid[3];
+  for (int b = 0; b < 3; ++b) gmid[b] = gvm[0] * ga[0][b] + gvm[1] * ga[1][b] + gvm[2] * ga[2][b];
   T gv0[3];
   for (int a = 0; a < 3; ++a) gv0[a] = -T(0.5) * gmid[a] * dclamp(midr[a], lo, hi) * sc[a];
   tri_scatter(lcu, t0, gv0[0]);
@@ -237,8 +257,7 @@ __device__ inline void adj_trace(const Dims &d, const T *__restrict__ cu, const 
 
 template <class T>
 __global__ void __launch_bounds__(256)
-k_adj_dye_predict(Dims d, T sc0, T sc1, T sc2, const T *__restrict__ cu,
-                  const T *__restrict__ cv, const T *__restrict__ cw,
+k_adj_dye_predict(Dims d, T sc0, T sc1, T sc2, const typename vec4_of<T>::type *__restrict__ cvel,
                   const T *__restrict__ dye, const T *__restrict__ lpred, T *__restrict__ ldye,
                   T *__restrict__ lcu, T *__restrict__ lcv, T *__restrict__ lcw) {
   MAC3D_CELL_PROLOGUE
@@ -247,13 +266,13 @@ k_adj_dye_predict(Dims d, T sc0, T sc1, T sc2, const T *__restrict__ cu,
   const T sc[3] = {sc0, sc1, sc2};
   T uvw[3] = {((T)i + T(0.5)) / (T)d.nx, ((T)j + T(0.5)) / (T)d.ny, ((T)k + T(0.5)) / (T)d.nz};
   T v0[3], back[3];
-  dye_back(d, cu, cv, cw, sc, uvw, v0, back);
+  dye_back(d, cvel, sc, uvw, v0, back);
   Tri<T> tb = tri_setup(d, back[0], back[1], back[2]);
   tri_scatter(ldye, tb, g);
   T gr[3], gb[3];
   tri_grad(dye, d, tb, gr);
   for (int a = 0; a < 3; ++a) gb[a] = g * gr[a];
-  adj_trace(d, cu, cv, cw, sc, uvw, false, T(0.001), T(0.999), gb, lcu, lcv, lcw);
+  adj_trace(d, cvel, sc, uvw, false, T(0.001), T(0.999), gb, lcu, lcv, lcw);
 }
 
 // Transpose of FLUID_ADVECT_MAC (fluid.ts:144-192): per face, the sampled
@@ -262,8 +281,8 @@ k_adj_dye_predict(Dims d, T sc0, T sc1, T sc2, const T *__restrict__ cu,
 template <class T>
 __global__ void __launch_bounds__(256)
 k_adj_advect(Dims d, T sc0, T sc1, T sc2, T damp, const T *__restrict__ u,
-             const T *__restrict__ v, const T *__restrict__ w, const T *__restrict__ cu,
-             const T *__restrict__ cv, const T *__restrict__ cw, const T *__restrict__ lout_u,
+             const T *__restrict__ v, const T *__restrict__ w,
+             const typename vec4_of<T>::type *__restrict__ cvel, const T *__restrict__ lout_u,
              const T *__restrict__ lout_v, const T *__restrict__ lout_w, T *__restrict__ lu,
              T *__restrict__ lv, T *__restrict__ lw, T *__restrict__ lcu, T *__restrict__ lcv,
              T *__restrict__ lcw) {
@@ -281,7 +300,7 @@ k_adj_advect(Dims d, T sc0, T sc1, T sc2, T damp, const T *__restrict__ u,
     f[a] = ((a == 0 ? (T)i : (a == 1 ? (T)j : (T)k)) + T(1)) /
            (T)(a == 0 ? d.nx : (a == 1 ? d.ny : d.nz));
     T back[3];
-    trace_back(d, cu, cv, cw, sc, f, back);
+    trace_back(d, cvel, sc, f, back);
     T qr[3], q[3];
     for (int b = 0; b < 3; ++b) {
       T shift = (b == a) ? T(0.5) / (T)(b == 0 ? d.nx : (b == 1 ? d.ny : d.nz)) : T(0);
@@ -293,7 +312,7 @@ k_adj_advect(Dims d, T sc0, T sc1, T sc2, T damp, const T *__restrict__ u,
     T gr[3], gb[3];
     tri_grad(fld[a], d, tq, gr);
     for (int b = 0; b < 3; ++b) gb[b] = g * gr[b] * dclamp(qr[b], lo, hi);
-    adj_trace(d, cu, cv, cw, sc, f, true, T(0.002), T(0.998), gb, lcu, lcv, lcw);
+    adj_trace(d, cvel, sc, f, true, T(0.002), T(0.998), gb, lcu, lcv, lcw);
   }
 }
 

@@ -61,6 +61,35 @@ __device__ inline T sample_cell(const T *__restrict__ f, const Dims &d, T ux,
   return mixv(mixv(x00, x10, fy), mixv(x01, x11, fy), fz);
 }
 
+// Cell-centred velocity is held as one 4-vector per cell (x, y, z, unused) so
+// a trilinear sample of all three components is eight vector loads.
+template <class T> struct vec4_of;
+template <> struct vec4_of<float> { typedef float4 type; };
+template <> struct vec4_of<double> { typedef double4 type; };
+
+template <class T>
+__device__ inline void sample_vec(const typename vec4_of<T>::type *__restrict__ f, const Dims &d,
+                                  T ux, T uy, T uz, T out[3]) {
+  T tx = ux * (T)d.nx - T(0.5), ty = uy * (T)d.ny - T(0.5), tz = uz * (T)d.nz - T(0.5);
+  T bx = floor(tx), by = floor(ty), bz = floor(tz);
+  T fx = tx - bx, fy = ty - by, fz = tz - bz;
+  int ib = (int)bx, jb = (int)by, kb = (int)bz;
+  int i0 = clampi(ib, 0, d.nx - 1), i1 = clampi(ib + 1, 0, d.nx - 1);
+  int j0 = clampi(jb, 0, d.ny - 1), j1 = clampi(jb + 1, 0, d.ny - 1);
+  int k0 = clampi(kb, 0, d.nz - 1), k1 = clampi(kb + 1, 0, d.nz - 1);
+  size_t r00 = ((size_t)k0 * d.ny + j0) * d.nx, r10 = ((size_t)k0 * d.ny + j1) * d.nx;
+  size_t r01 = ((size_t)k1 * d.ny + j0) * d.nx, r11 = ((size_t)k1 * d.ny + j1) * d.nx;
+  auto a000 = f[r00 + i0], a100 = f[r00 + i1], a010 = f[r10 + i0], a110 = f[r10 + i1];
+  auto a001 = f[r01 + i0], a101 = f[r01 + i1], a011 = f[r11 + i0], a111 = f[r11 + i1];
+#define MAC3D_TRI(C)                                                            \
+  mixv(mixv(mixv(a000.C, a100.C, fx), mixv(a010.C, a110.C, fx), fy),            \
+       mixv(mixv(a001.C, a101.C, fx), mixv(a011.C, a111.C, fx), fy), fz)
+  out[0] = MAC3D_TRI(x);
+  out[1] = MAC3D_TRI(y);
+  out[2] = MAC3D_TRI(z);
+#undef MAC3D_TRI
+}
+
 // The same trilinear sample of three fields at one point (the cell-centred
 // velocity components): index and weight arithmetic shared.
 template <class T>

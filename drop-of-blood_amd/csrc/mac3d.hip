@@ -62,7 +62,7 @@ struct Ctx {
   hipStream_t stream = nullptr;
   std::string err;
   // level-0 step fields
-  Buf u, v, w, u2, v2, w2, p, dye, dye2, dpred, cu, cv, cw, rhs;
+  Buf u, v, w, u2, v2, w2, p, dye, dye2, dpred, cvel, rhs;  // cvel: 4 per cell
   // multigrid: x, b, r per level (level 0 x/b alias p/rhs except in mixed);
   // sweeps run out of place between mgx and its alternate mgxa
   Buf mgx[kMaxLev + 1], mgxa[kMaxLev + 1], mgb[kMaxLev + 1], mgr[kMaxLev + 1];
@@ -85,7 +85,7 @@ struct Ctx {
   // adjoint work space (allocated on first use)
   struct Adj {
     bool ready = false;
-    void *c1[3] = {}, *vela[3] = {}, *lvel[3] = {}, *lvelc[3] = {}, *lvela[3] = {}, *lnew[3] = {};
+    void *c1 = nullptr, *vela[3] = {}, *lvel[3] = {}, *lvelc[3] = {}, *lvela[3] = {}, *lnew[3] = {};
     void *lc[3] = {}, *ldye = nullptr, *ldyen = nullptr, *lpred = nullptr, *target = nullptr;
     Buf lx, lxa, lb;  // adjoint pressure solve on level 0
     double *part = nullptr, *acc = nullptr;  // partial sums; acc[0] = d loss / d rho_b
@@ -221,7 +221,7 @@ int launch_face_to_centre(Ctx *ctx) {
     using T = std::remove_pointer_t<decltype(tag)>;
     hipLaunchKernelGGL(k_face_to_centre<T>, cell_grid(d), kCellBlock, 0, ctx->stream, d,
                        (const T *)ctx->u.ptr, (const T *)ctx->v.ptr, (const T *)ctx->w.ptr,
-                       (T *)ctx->cu.ptr, (T *)ctx->cv.ptr, (T *)ctx->cw.ptr);
+                       (typename vec4_of<T>::type *)ctx->cvel.ptr);
   });
   LAUNCHCHK();
   return MAC3D_OK;
@@ -253,8 +253,8 @@ int launch_advect(Ctx *ctx, double dt) {
       hipLaunchKernelGGL(k_advect_mac<T>, cell_grid(d), kCellBlock, 0, ctx->stream, d,
                          (T)(dt / dom[0]), (T)(dt / dom[1]), (T)(dt / dom[2]),
                          (T)ctx->cfg.vel_damp, (const T *)ctx->u.ptr, (const T *)ctx->v.ptr,
-                         (const T *)ctx->w.ptr, (const T *)ctx->cu.ptr, (const T *)ctx->cv.ptr,
-                         (const T *)ctx->cw.ptr, (T *)ctx->u2.ptr, (T *)ctx->v2.ptr,
+                         (const T *)ctx->w.ptr, (const typename vec4_of<T>::type *)ctx->cvel.ptr,
+                         (T *)ctx->u2.ptr, (T *)ctx->v2.ptr,
                          (T *)ctx->w2.ptr);
     });
   }
@@ -321,7 +321,7 @@ int launch_dye_predict(Ctx *ctx, double dt) {
     using T = std::remove_pointer_t<decltype(tag)>;
     hipLaunchKernelGGL(k_dye_predict<T>, cell_grid(d), kCellBlock, 0, ctx->stream, d,
                        (T)(dt / dom[0]), (T)(dt / dom[1]), (T)(dt / dom[2]),
-                       (const T *)ctx->cu.ptr, (const T *)ctx->cv.ptr, (const T *)ctx->cw.ptr,
+                       (const typename vec4_of<T>::type *)ctx->cvel.ptr,
                        (const T *)ctx->dye.ptr, (T *)ctx->dpred.ptr);
   });
   LAUNCHCHK();
@@ -337,8 +337,8 @@ int launch_dye_resolve(Ctx *ctx, double dt) {
       using T = std::remove_pointer_t<decltype(tag)>;
       hipLaunchKernelGGL(k_dye_resolve<T>, cell_grid(d), kCellBlock, 0, ctx->stream, d,
                          (T)(dt / dom[0]), (T)(dt / dom[1]), (T)(dt / dom[2]),
-                         (T)ctx->cfg.maccormack, (const T *)ctx->cu.ptr, (const T *)ctx->cv.ptr,
-                         (const T *)ctx->cw.ptr, (const T *)ctx->dye.ptr,
+                         (T)ctx->cfg.maccormack, (const typename vec4_of<T>::type *)ctx->cvel.ptr,
+                         (const T *)ctx->dye.ptr,
                          (const T *)ctx->dpred.ptr, (T *)ctx->dye2.ptr);
     });
   }
@@ -444,6 +444,25 @@ int mg_residual(Ctx *ctx, int l) {
   return MAC3D_OK;
 }
 
+int mg_residual_restrict(Ctx *ctx, int l) {
+  const Dims &df = ctx->d[l], &dc = ctx->d[l + 1];
+  ProfScope ps(ctx->prof, ctx->stream, l == 0 ? KC_RESID0 : KC_MG);
+  with_type(ctx->mgx[l].type, [&](auto ftag) {
+    using Sf = std::remove_pointer_t<decltype(ftag)>;
+    using C = typename compute_of<Sf>::type;
+    with_type(ctx->mgb[l + 1].type, [&](auto ctag) {
+      using Sc = std::remove_pointer_t<decltype(ctag)>;
+      hipLaunchKernelGGL((k_residual_restrict<Sf, Sc>), cell_grid(dc), kCellBlock, 0, ctx->stream,
+                         df, dc, (const Sf *)ctx->mgx[l].ptr, (const Sf *)ctx->mgb[l].ptr,
+                         (Sc *)ctx->mgb[l + 1].ptr, (C)level_h2(l),
+                         (C)(l == 0 ? 1.0 : top_coeff(l)),
+                         l == 0 ? (const double *)ctx->scale : nullptr);
+    });
+  });
+  LAUNCHCHK();
+  return MAC3D_OK;
+}
+
 int mg_restrict(Ctx *ctx, int l) {
   const Dims &df = ctx->d[l], &dc = ctx->d[l + 1];
   ProfScope ps(ctx->prof, ctx->stream, l == 0 ? KC_RESID0 : KC_MG);
@@ -514,12 +533,10 @@ int vcycle(Ctx *ctx, bool zero0) {
   int rc, L = ctx->L;
   int nu1 = ctx->cfg.nu1, nu2 = ctx->cfg.nu2;
   if ((rc = mg_smooth(ctx, 0, nu1, zero0))) return rc;
-  if ((rc = mg_residual(ctx, 0))) return rc;
-  if ((rc = mg_restrict(ctx, 0))) return rc;
+  if ((rc = mg_residual_restrict(ctx, 0))) return rc;
   for (int l = 1; l < L; ++l) {
     if ((rc = mg_smooth(ctx, l, nu1, true))) return rc;
-    if ((rc = mg_residual(ctx, l))) return rc;
-    if ((rc = mg_restrict(ctx, l))) return rc;
+    if ((rc = mg_residual_restrict(ctx, l))) return rc;
   }
   if ((rc = mg_coarse(ctx))) return rc;
   for (int l = L - 1; l >= 1; --l) {
@@ -724,9 +741,9 @@ Buf *field_buf(Ctx *ctx, int field, int level) {
     case MAC3D_FIELD_W: return l0 ? &ctx->w : nullptr;
     case MAC3D_FIELD_P: return l0 ? &ctx->p : nullptr;
     case MAC3D_FIELD_DYE: return l0 ? &ctx->dye : nullptr;
-    case MAC3D_FIELD_CU: return l0 ? &ctx->cu : nullptr;
-    case MAC3D_FIELD_CV: return l0 ? &ctx->cv : nullptr;
-    case MAC3D_FIELD_CW: return l0 ? &ctx->cw : nullptr;
+    case MAC3D_FIELD_CU:
+    case MAC3D_FIELD_CV:
+    case MAC3D_FIELD_CW: return l0 ? &ctx->cvel : nullptr;
     case MAC3D_FIELD_RHS: return l0 ? &ctx->rhs : nullptr;
     case MAC3D_FIELD_DYE_PRED: return l0 ? &ctx->dpred : nullptr;
     case MAC3D_FIELD_MGX: return &ctx->mgx[level];
@@ -776,9 +793,10 @@ int create_impl(Ctx *ctx, const mac3d_config *cfg) {
   size_t n0 = ctx->n[0];
   int t0 = ctx->t0;
   Buf *f0[] = {&ctx->u, &ctx->v, &ctx->w, &ctx->u2, &ctx->v2, &ctx->w2, &ctx->dye,
-               &ctx->dye2, &ctx->dpred, &ctx->cu, &ctx->cv, &ctx->cw};
+               &ctx->dye2, &ctx->dpred};
   for (Buf *b : f0)
     if ((rc = alloc(ctx, *b, t0, n0))) return rc;
+  if ((rc = alloc(ctx, ctx->cvel, t0, 4 * n0))) return rc;
   bool mixed = ctx->prec == MAC3D_PREC_MIXED;
   if ((rc = alloc(ctx, ctx->p, mixed ? TF64 : t0, n0))) return rc;
   if ((rc = alloc(ctx, ctx->rhs, mixed ? TF64 : t0, n0))) return rc;
@@ -873,10 +891,9 @@ int init_blobs_impl(Ctx *ctx, const mac3d_blob *blobs, int nblobs) {
   if (nblobs)
     HIPCHK(hipMemcpyAsync(ctx->blobs, hb.data(), sizeof(BlobDev) * nblobs, hipMemcpyHostToDevice,
                           ctx->stream));
-  size_t n0 = ctx->n[0];
-  Buf *zero[] = {&ctx->u, &ctx->w, &ctx->p, &ctx->e0, &ctx->cu, &ctx->cv, &ctx->cw};
+  Buf *zero[] = {&ctx->u, &ctx->w, &ctx->p, &ctx->e0, &ctx->cvel};
   for (Buf *b : zero)
-    if (b->ptr) HIPCHK(hipMemsetAsync(b->ptr, 0, type_size(b->type) * n0, ctx->stream));
+    if (b->ptr) HIPCHK(hipMemsetAsync(b->ptr, 0, type_size(b->type) * b->n, ctx->stream));
   const Dims &d = ctx->d[0];
   with_step_type(ctx->t0, [&](auto tag) {
     using T = std::remove_pointer_t<decltype(tag)>;
@@ -899,7 +916,8 @@ int adj_alloc(Ctx *ctx) {
   if (A.ready) return MAC3D_OK;
   int rc;
   size_t n0 = ctx->n[0], bytes = type_size(ctx->t0) * n0;
-  void **arrs[] = {&A.c1[0], &A.c1[1], &A.c1[2], &A.vela[0], &A.vela[1], &A.vela[2],
+  if ((rc = dev_alloc(ctx, &A.c1, 4 * bytes))) return rc;
+  void **arrs[] = {&A.vela[0], &A.vela[1], &A.vela[2],
                    &A.lvel[0], &A.lvel[1], &A.lvel[2], &A.lvelc[0], &A.lvelc[1], &A.lvelc[2],
                    &A.lvela[0], &A.lvela[1], &A.lvela[2], &A.lnew[0], &A.lnew[1], &A.lnew[2],
                    &A.lc[0], &A.lc[1], &A.lc[2], &A.ldye, &A.ldyen, &A.lpred, &A.target};
@@ -996,9 +1014,7 @@ int adj_step(Ctx *ctx, double dt, void *const *src) {
   // forward recompute of the stage inputs
   if ((rc = load_state(ctx, src))) return rc;
   if ((rc = launch_face_to_centre(ctx))) return rc;
-  void *cc[3] = {ctx->cu.ptr, ctx->cv.ptr, ctx->cw.ptr};
-  for (int a = 0; a < 3; ++a)
-    HIPCHK(hipMemcpyAsync(A.c1[a], cc[a], tb, hipMemcpyDeviceToDevice, ctx->stream));
+  HIPCHK(hipMemcpyAsync(A.c1, ctx->cvel.ptr, 4 * tb, hipMemcpyDeviceToDevice, ctx->stream));
   if ((rc = launch_advect(ctx, dt))) return rc;
   void *va[3] = {ctx->u.ptr, ctx->v.ptr, ctx->w.ptr};
   for (int a = 0; a < 3; ++a)
@@ -1038,12 +1054,12 @@ int adj_step(Ctx *ctx, double dt, void *const *src) {
     T s0 = (T)(dt / dom[0]), s1 = (T)(dt / dom[1]), s2 = (T)(dt / dom[2]);
     const T *dyen = (const T *)src[SV_DYE];
     hipLaunchKernelGGL(k_adj_dye_resolve<T>, cell_grid(d), kCellBlock, 0, ctx->stream, d, s0, s1,
-                       s2, (T)ctx->cfg.maccormack, (const T *)ctx->cu.ptr, (const T *)ctx->cv.ptr,
-                       (const T *)ctx->cw.ptr, dyen, (const T *)ctx->dpred.ptr,
+                       s2, (T)ctx->cfg.maccormack, (const typename vec4_of<T>::type *)ctx->cvel.ptr,
+                       dyen, (const T *)ctx->dpred.ptr,
                        (const T *)A.ldye, (T *)A.ldyen, (T *)A.lpred, (T *)A.lc[0],
                        (T *)A.lc[1], (T *)A.lc[2]);
     hipLaunchKernelGGL(k_adj_dye_predict<T>, cell_grid(d), kCellBlock, 0, ctx->stream, d, s0, s1,
-                       s2, (const T *)ctx->cu.ptr, (const T *)ctx->cv.ptr, (const T *)ctx->cw.ptr,
+                       s2, (const typename vec4_of<T>::type *)ctx->cvel.ptr,
                        dyen, (const T *)A.lpred, (T *)A.ldyen, (T *)A.lc[0], (T *)A.lc[1],
                        (T *)A.lc[2]);
     hipLaunchKernelGGL(k_adj_face_to_centre<T>, cell_grid(d), kCellBlock, 0, ctx->stream, d,
@@ -1090,8 +1106,8 @@ int adj_step(Ctx *ctx, double dt, void *const *src) {
     T s0 = (T)(dt / dom[0]), s1 = (T)(dt / dom[1]), s2 = (T)(dt / dom[2]);
     hipLaunchKernelGGL(k_adj_advect<T>, cell_grid(d), kCellBlock, 0, ctx->stream, d, s0, s1, s2,
                        (T)ctx->cfg.vel_damp, (const T *)src[SV_U], (const T *)src[SV_V],
-                       (const T *)src[SV_W], (const T *)A.c1[0], (const T *)A.c1[1],
-                       (const T *)A.c1[2], (const T *)A.lvela[0], (const T *)A.lvela[1],
+                       (const T *)src[SV_W], (const typename vec4_of<T>::type *)A.c1,
+                       (const T *)A.lvela[0], (const T *)A.lvela[1],
                        (const T *)A.lvela[2], (T *)A.lnew[0], (T *)A.lnew[1], (T *)A.lnew[2],
                        (T *)A.lc[0], (T *)A.lc[1], (T *)A.lc[2]);
     hipLaunchKernelGGL(k_adj_face_to_centre<T>, cell_grid(d), kCellBlock, 0, ctx->stream, d,
@@ -1265,9 +1281,26 @@ int mac3d_upload(void *h, int field, int level, const void *host, int dtype, siz
   Buf *b = field_buf(ctx, field, level);
   if (!b || !host) return fail(ctx, MAC3D_EINVAL, "unknown field/level or null host pointer");
   if (dtype != MAC3D_F32 && dtype != MAC3D_F64) return fail(ctx, MAC3D_EINVAL, "unknown dtype");
-  if (count != b->n) return fail(ctx, MAC3D_EINVAL, "element count does not match the field");
+  int comp = (field >= MAC3D_FIELD_CU && field <= MAC3D_FIELD_CW) ? field - MAC3D_FIELD_CU : -1;
+  if (count != (comp >= 0 ? ctx->n[0] : b->n))
+    return fail(ctx, MAC3D_EINVAL, "element count does not match the field");
   size_t hsz = dtype == MAC3D_F32 ? 4 : 8;
   HIPCHK(hipMemcpyAsync(ctx->stage, host, hsz * count, hipMemcpyHostToDevice, ctx->stream));
+  if (comp >= 0) {
+    with_step_type(b->type, [&](auto tag) {
+      using S = std::remove_pointer_t<decltype(tag)>;
+      dim3 g((unsigned)((count + 255) / 256));
+      if (dtype == MAC3D_F32)
+        hipLaunchKernelGGL((k_convert_strided<float, S>), g, dim3(256), 0, ctx->stream,
+                           (const float *)ctx->stage, (S *)b->ptr, count, 1, 0, 4, comp);
+      else
+        hipLaunchKernelGGL((k_convert_strided<double, S>), g, dim3(256), 0, ctx->stream,
+                           (const double *)ctx->stage, (S *)b->ptr, count, 1, 0, 4, comp);
+    });
+    LAUNCHCHK();
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    return MAC3D_OK;
+  }
   with_type(b->type, [&](auto tag) {
     using S = std::remove_pointer_t<decltype(tag)>;
     dim3 g((unsigned)((count + 255) / 256));
@@ -1289,8 +1322,22 @@ int mac3d_download(void *h, int field, int level, void *host, int dtype, size_t 
   Buf *b = field_buf(ctx, field, level);
   if (!b || !host) return fail(ctx, MAC3D_EINVAL, "unknown field/level or null host pointer");
   if (dtype != MAC3D_F32 && dtype != MAC3D_F64) return fail(ctx, MAC3D_EINVAL, "unknown dtype");
-  if (count != b->n) return fail(ctx, MAC3D_EINVAL, "element count does not match the field");
+  int comp = (field >= MAC3D_FIELD_CU && field <= MAC3D_FIELD_CW) ? field - MAC3D_FIELD_CU : -1;
+  if (count != (comp >= 0 ? ctx->n[0] : b->n))
+    return fail(ctx, MAC3D_EINVAL, "element count does not match the field");
   size_t hsz = dtype == MAC3D_F32 ? 4 : 8;
+  if (comp >= 0) {
+    with_step_type(b->type, [&](auto tag) {
+      using S = std::remove_pointer_t<decltype(tag)>;
+      dim3 g((unsigned)((count + 255) / 256));
+      if (dtype == MAC3D_F32)
+        hipLaunchKernelGGL((k_convert_strided<S, float>), g, dim3(256), 0, ctx->stream,
+                           (const S *)b->ptr, (float *)ctx->stage, count, 4, comp, 1, 0);
+      else
+        hipLaunchKernelGGL((k_convert_strided<S, double>), g, dim3(256), 0, ctx->stream,
+                           (const S *)b->ptr, (double *)ctx->stage, count, 4, comp, 1, 0);
+    });
+  } else
   with_type(b->type, [&](auto tag) {
     using S = std::remove_pointer_t<decltype(tag)>;
     dim3 g((unsigned)((count + 255) / 256));

@@ -66,6 +66,32 @@ k_restrict(Dims df, Dims dc, const Sf *__restrict__ rf, Sc *__restrict__ bc) {
   bc[idx3(dc, i, j, k)] = store_as<Sc, C>(s * C(0.125));
 }
 
+// MG_RESIDUAL followed by MG_RESTRICT (fluid.ts:744-800) in one pass: each
+// coarse cell sums the residuals of its 2x2x2 children, so the fine residual
+// never goes to memory.  Same arithmetic as the two kernels.
+template <class Sf, class Sc>
+__global__ void __launch_bounds__(256)
+k_residual_restrict(Dims df, Dims dc, const Sf *__restrict__ x, const Sf *__restrict__ b,
+                    Sc *__restrict__ bc, typename compute_of<Sf>::type h2,
+                    typename compute_of<Sf>::type topc, const double *__restrict__ scale) {
+  typedef typename compute_of<Sf>::type C;
+  const C bscale = scale ? (C)scale[0] : C(1);
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  int j = blockIdx.y * blockDim.y + threadIdx.y;
+  int k = blockIdx.z;
+  if (i >= dc.nx || j >= dc.ny) return;
+  C s = C(0);
+#pragma unroll
+  for (int q = 0; q < 8; ++q) {
+    int fi = 2 * i + (q & 1), fj = 2 * j + ((q >> 1) & 1), fk = 2 * k + (q >> 2);
+    size_t c = idx3(df, fi, fj, fk);
+    C sn, dg;
+    mg_stencil<Sf, C>(x, df, topc, fi, fj, fk, c, sn, dg);
+    s += (C)b[c] * bscale - (sn - dg * (C)x[c]) / h2;
+  }
+  bc[idx3(dc, i, j, k)] = store_as<Sc, C>(s * C(0.125));
+}
+
 // MG_PROLONG_ADD (fluid.ts:806-855): x_f += cell-centred trilinear
 // interpolation of the coarse correction; coarse indices clamped at the
 // walls, 0 above the top row.  Fine cell i sits at coarse coordinate

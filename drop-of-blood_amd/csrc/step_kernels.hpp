@@ -29,8 +29,7 @@ struct StepParams {
 template <class T>
 __global__ void __launch_bounds__(256)
 k_face_to_centre(Dims d, const T *__restrict__ u, const T *__restrict__ v,
-                 const T *__restrict__ w, T *__restrict__ cu,
-                 T *__restrict__ cv, T *__restrict__ cw) {
+                 const T *__restrict__ w, typename vec4_of<T>::type *__restrict__ cvel) {
   MAC3D_CELL_PROLOGUE
   T uR = (i >= d.nx - 1) ? T(0) : u[c];
   T uL = (i == 0) ? T(0) : u[c - sx];
@@ -38,26 +37,27 @@ k_face_to_centre(Dims d, const T *__restrict__ u, const T *__restrict__ v,
   T vD = (j == 0) ? T(0) : v[c - sy];
   T wF = (k >= d.nz - 1) ? T(0) : w[c];
   T wB = (k == 0) ? T(0) : w[c - sz];
-  cu[c] = T(0.5) * (uL + uR);
-  cv[c] = T(0.5) * (vD + vU);
-  cw[c] = T(0.5) * (wB + wF);
+  typename vec4_of<T>::type o;
+  o.x = T(0.5) * (uL + uR);
+  o.y = T(0.5) * (vD + vU);
+  o.z = T(0.5) * (wB + wF);
+  o.w = T(0);
+  cvel[c] = o;
 }
 
 // traceBackUVW (fluid.ts:169-174): RK2 midpoint backtrace on the centred
 // velocity in normalised coordinates, clamped to [0.002, 0.998].
 template <class T>
-__device__ inline void trace_back(const Dims &d, const T *__restrict__ cu,
-                                  const T *__restrict__ cv,
-                                  const T *__restrict__ cw, const T sc[3],
-                                  const T f[3], T out[3]) {
+__device__ inline void trace_back(const Dims &d,
+                                  const typename vec4_of<T>::type *__restrict__ cvel,
+                                  const T sc[3], const T f[3], T out[3]) {
   const T lo = T(0.002), hi = T(0.998);
   T v0[3], vm[3];
-  sample_cell3(cu, cv, cw, d, clampv(f[0], lo, hi), clampv(f[1], lo, hi),
-               clampv(f[2], lo, hi), v0);
+  sample_vec<T>(cvel, d, clampv(f[0], lo, hi), clampv(f[1], lo, hi), clampv(f[2], lo, hi), v0);
   T mx = clampv(f[0] - T(0.5) * (v0[0] * sc[0]), lo, hi);
   T my = clampv(f[1] - T(0.5) * (v0[1] * sc[1]), lo, hi);
   T mz = clampv(f[2] - T(0.5) * (v0[2] * sc[2]), lo, hi);
-  sample_cell3(cu, cv, cw, d, mx, my, mz, vm);
+  sample_vec<T>(cvel, d, mx, my, mz, vm);
   out[0] = clampv(f[0] - vm[0] * sc[0], lo, hi);
   out[1] = clampv(f[1] - vm[1] * sc[1], lo, hi);
   out[2] = clampv(f[2] - vm[2] * sc[2], lo, hi);
@@ -71,9 +71,8 @@ template <class T>
 __global__ void __launch_bounds__(256)
 k_advect_mac(Dims d, T sc0, T sc1, T sc2, T damp, const T *__restrict__ u,
              const T *__restrict__ v, const T *__restrict__ w,
-             const T *__restrict__ cu, const T *__restrict__ cv,
-             const T *__restrict__ cw, T *__restrict__ uo, T *__restrict__ vo,
-             T *__restrict__ wo) {
+             const typename vec4_of<T>::type *__restrict__ cvel, T *__restrict__ uo,
+             T *__restrict__ vo, T *__restrict__ wo) {
   MAC3D_CELL_PROLOGUE
   const T sc[3] = {sc0, sc1, sc2};
   const T inx = T(1) / (T)d.nx, iny = T(1) / (T)d.ny, inz = T(1) / (T)d.nz;
@@ -81,20 +80,20 @@ k_advect_mac(Dims d, T sc0, T sc1, T sc2, T damp, const T *__restrict__ u,
   T b[3];
   {
     T f[3] = {((T)i + T(1)) / (T)d.nx, ((T)j + T(0.5)) / (T)d.ny, ((T)k + T(0.5)) / (T)d.nz};
-    trace_back(d, cu, cv, cw, sc, f, b);
+    trace_back(d, cvel, sc, f, b);
     T vx = sample_cell(u, d, clampv(b[0] - T(0.5) * inx, lo, hi), clampv(b[1], lo, hi),
                        clampv(b[2], lo, hi)) * damp;
     uo[c] = (i >= d.nx - 1) ? T(0) : vx;
   }
   {
     T f[3] = {((T)i + T(0.5)) / (T)d.nx, ((T)j + T(1)) / (T)d.ny, ((T)k + T(0.5)) / (T)d.nz};
-    trace_back(d, cu, cv, cw, sc, f, b);
+    trace_back(d, cvel, sc, f, b);
     vo[c] = sample_cell(v, d, clampv(b[0], lo, hi), clampv(b[1] - T(0.5) * iny, lo, hi),
                         clampv(b[2], lo, hi)) * damp;
   }
   {
     T f[3] = {((T)i + T(0.5)) / (T)d.nx, ((T)j + T(0.5)) / (T)d.ny, ((T)k + T(1)) / (T)d.nz};
-    trace_back(d, cu, cv, cw, sc, f, b);
+    trace_back(d, cvel, sc, f, b);
     T vz = sample_cell(w, d, clampv(b[0], lo, hi), clampv(b[1], lo, hi),
                        clampv(b[2] - T(0.5) * inz, lo, hi)) * damp;
     wo[c] = (k >= d.nz - 1) ? T(0) : vz;
@@ -222,16 +221,15 @@ k_project(Dims d, T *__restrict__ u, T *__restrict__ v, T *__restrict__ w,
 // CONC_PREDICT (advect.ts:28-72): RK2 semi-Lagrangian predictor of the dye
 // on the centred velocity, normalised coordinates clamped to [0.001, 0.999].
 template <class T>
-__device__ inline void dye_back(const Dims &d, const T *__restrict__ cu,
-                                const T *__restrict__ cv, const T *__restrict__ cw,
+__device__ inline void dye_back(const Dims &d, const typename vec4_of<T>::type *__restrict__ cvel,
                                 const T sc[3], const T uvw[3], T v0[3], T back[3]) {
   const T lo = T(0.001), hi = T(0.999);
-  sample_cell3(cu, cv, cw, d, uvw[0], uvw[1], uvw[2], v0);
+  sample_vec<T>(cvel, d, uvw[0], uvw[1], uvw[2], v0);
   T mx = clampv(uvw[0] - T(0.5) * (v0[0] * sc[0]), lo, hi);
   T my = clampv(uvw[1] - T(0.5) * (v0[1] * sc[1]), lo, hi);
   T mz = clampv(uvw[2] - T(0.5) * (v0[2] * sc[2]), lo, hi);
   T vm[3];
-  sample_cell3(cu, cv, cw, d, mx, my, mz, vm);
+  sample_vec<T>(cvel, d, mx, my, mz, vm);
   back[0] = clampv(uvw[0] - vm[0] * sc[0], lo, hi);
   back[1] = clampv(uvw[1] - vm[1] * sc[1], lo, hi);
   back[2] = clampv(uvw[2] - vm[2] * sc[2], lo, hi);
@@ -239,14 +237,13 @@ __device__ inline void dye_back(const Dims &d, const T *__restrict__ cu,
 
 template <class T>
 __global__ void __launch_bounds__(256)
-k_dye_predict(Dims d, T sc0, T sc1, T sc2, const T *__restrict__ cu,
-              const T *__restrict__ cv, const T *__restrict__ cw,
+k_dye_predict(Dims d, T sc0, T sc1, T sc2, const typename vec4_of<T>::type *__restrict__ cvel,
               const T *__restrict__ dye, T *__restrict__ pred) {
   MAC3D_CELL_PROLOGUE
   const T sc[3] = {sc0, sc1, sc2};
   T uvw[3] = {((T)i + T(0.5)) / (T)d.nx, ((T)j + T(0.5)) / (T)d.ny, ((T)k + T(0.5)) / (T)d.nz};
   T v0[3], back[3];
-  dye_back(d, cu, cv, cw, sc, uvw, v0, back);
+  dye_back(d, cvel, sc, uvw, v0, back);
   pred[c] = sample_cell(dye, d, back[0], back[1], back[2]);
 }
 
@@ -255,16 +252,15 @@ k_dye_predict(Dims d, T sc0, T sc1, T sc2, const T *__restrict__ cu,
 // clamped to the min/max of the 8 source texels around the backtrace point.
 template <class T>
 __global__ void __launch_bounds__(256)
-k_dye_resolve(Dims d, T sc0, T sc1, T sc2, T strength, const T *__restrict__ cu,
-              const T *__restrict__ cv, const T *__restrict__ cw,
-              const T *__restrict__ dye, const T *__restrict__ pred,
-              T *__restrict__ out) {
+k_dye_resolve(Dims d, T sc0, T sc1, T sc2, T strength,
+              const typename vec4_of<T>::type *__restrict__ cvel, const T *__restrict__ dye,
+              const T *__restrict__ pred, T *__restrict__ out) {
   MAC3D_CELL_PROLOGUE
   const T sc[3] = {sc0, sc1, sc2};
   const T lo = T(0.001), hi = T(0.999);
   T uvw[3] = {((T)i + T(0.5)) / (T)d.nx, ((T)j + T(0.5)) / (T)d.ny, ((T)k + T(0.5)) / (T)d.nz};
   T v0[3], back[3];
-  dye_back(d, cu, cv, cw, sc, uvw, v0, back);
+  dye_back(d, cvel, sc, uvw, v0, back);
   T fx = clampv(uvw[0] + v0[0] * sc[0], lo, hi);
   T fy = clampv(uvw[1] + v0[1] * sc[1], lo, hi);
   T fz = clampv(uvw[2] + v0[2] * sc[2], lo, hi);
@@ -390,4 +386,15 @@ __global__ void __launch_bounds__(256) k_fill(T *__restrict__ dst, size_t n, T v
   if (t < n) dst[t] = val;
 }
 
+}  // namespace mac3d
+
+namespace mac3d {
+// one component of the packed centre velocity <-> a plain array
+template <class A, class B>
+__global__ void __launch_bounds__(256)
+k_convert_strided(const A *__restrict__ src, B *__restrict__ dst, size_t n, int sstride,
+                  int soff, int dstride, int doff) {
+  size_t t = (size_t)blockIdx.x * 256 + threadIdx.x;
+  if (t < n) dst[t * dstride + doff] = (B)src[t * sstride + soff];
+}
 }  // namespace mac3d

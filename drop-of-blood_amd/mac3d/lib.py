@@ -164,6 +164,12 @@ class MacSolver:
         self._check(self.L.mac3d_download(self.h, fid, level, out.ctypes.data, dt, out.size))
         return out
 
+    def download_into(self, field, out: np.ndarray, level: int = 0):
+        fid = FIELD[field] if isinstance(field, str) else field
+        dt = F64 if out.dtype == np.float64 else F32
+        self._check(self.L.mac3d_download(self.h, fid, level, out.ctypes.data, dt, out.size))
+        return out
+
     def upload(self, field, arr, level: int = 0):
         fid = FIELD[field] if isinstance(field, str) else field
         a = np.ascontiguousarray(arr)

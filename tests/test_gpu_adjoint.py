@@ -10,7 +10,10 @@ The loss is only C1 in the blob parameters (smoothstep edge of the blob,
 MacCormack limiter switches, trilinear cell crossings), so a central
 difference at h = 1e-3 carries kink errors of up to a few 1e-4 relative;
 as h -> 0 the differences converge onto the adjoint, which the strict
-checks below pin at h = 1e-5.
+checks below pin at h = 1e-5 (cell units and m/s).  rho_b moves the loss
+by only ~1e-9 of itself per 1e-5 kg/m^3, where single limiter flips show,
+so its strict difference uses h = 1e-2 kg/m^3 (the loss is smooth in rho_b
+at that scale: the h = 1e-3 difference agrees to ~1e-8).
 """
 import copy
 
@@ -23,6 +26,10 @@ from mac3d.lib import config_for
 
 pytestmark = pytest.mark.gpu
 NAMES = ["cx", "cy", "cz", "radius", "w0"]
+
+
+def strict_h(name):
+    return 1e-2 if name == "rho_b" else 1e-5
 
 
 def _perturbed(blobs):
@@ -69,8 +76,9 @@ def _check(w, K, strict_rel, loose_rel, cfg_kw=None):
     rows = []
     for p, name in enumerate(NAMES + ["rho_b"]):
         ad = g["d_rho_b"] if name == "rho_b" else g["d_blob"][0][p]
-        rows.append((name, ad, _fd(lossfn, w, name, 1e-5, cfg_kw), _fd(lossfn, w, name, 1e-3, cfg_kw)))
-    print("\n" + "\n".join(f"{n:7s} adjoint {a:+.10e}  fd(1e-5) {f5:+.10e} ({abs(a - f5) / abs(f5):.1e})"
+        rows.append((name, ad, _fd(lossfn, w, name, strict_h(name), cfg_kw),
+                     _fd(lossfn, w, name, 1e-3, cfg_kw)))
+    print("\n" + "\n".join(f"{n:7s} adjoint {a:+.10e}  fd(strict) {f5:+.10e} ({abs(a - f5) / abs(f5):.1e})"
                            f"  fd(1e-3) {f3:+.10e} ({abs(a - f3) / abs(f3):.1e})"
                            for n, a, f5, f3 in rows))
     for n, a, f5, f3 in rows:
@@ -102,7 +110,7 @@ def test_adjoint_against_oracle_fd():
         return 0.5 * float(np.sum((d - target) ** 2))
 
     for name in ("cy", "w0", "rho_b"):
-        fd = _fd(oloss, w, name, 1e-5, {})
+        fd = _fd(oloss, w, name, strict_h(name), {})
         ad = g["d_rho_b"] if name == "rho_b" else g["d_blob"][0][NAMES.index(name)]
         assert abs(ad - fd) <= 1e-6 * abs(fd), (name, ad, fd)
 
