@@ -68,6 +68,13 @@ struct Ctx {
   Buf mgx[kMaxLev + 1], mgxa[kMaxLev + 1], mgb[kMaxLev + 1], mgr[kMaxLev + 1];
   // mixed-mode FP32 L0 correction and its right-hand side
   Buf e0, r0;
+  // mixed mode: FP64 pressure ring; role 0 = current, 1 = previous step,
+  // 2 = the one before, 3 = spare; hist = converged pressures held (<= 2)
+  Buf pr[4];
+  int role[4] = {0, 1, 2, 3};
+  int hist = 0;
+  double omega = 1.0;
+  int extrap = 0;
   // right-hand-side scale of level 0: [0] = multiplier, [1] = its inverse
   double *scale = nullptr;
   // every device allocation, freed at destroy (the Bufs above are views)
@@ -376,12 +383,15 @@ int mg_sweep(Ctx *ctx, int l, bool zero) {
       C h2 = (C)level_h2(l), tc = (C)top_coeff(l);
       if (zero)
         hipLaunchKernelGGL((k_rbgs_pass<S, true>), grid, kCellBlock, 0, ctx->stream, d,
-                           (S *)ctx->mgx[l].ptr, (const S *)ctx->mgb[l].ptr, h2, tc, 0);
+                           (S *)ctx->mgx[l].ptr, (const S *)ctx->mgb[l].ptr, h2, tc,
+                           (C)ctx->omega, 0);
       else
         hipLaunchKernelGGL((k_rbgs_pass<S, false>), grid, kCellBlock, 0, ctx->stream, d,
-                           (S *)ctx->mgx[l].ptr, (const S *)ctx->mgb[l].ptr, h2, tc, 0);
+                           (S *)ctx->mgx[l].ptr, (const S *)ctx->mgb[l].ptr, h2, tc,
+                           (C)ctx->omega, 0);
       hipLaunchKernelGGL((k_rbgs_pass<S, false>), grid, kCellBlock, 0, ctx->stream, d,
-                         (S *)ctx->mgx[l].ptr, (const S *)ctx->mgb[l].ptr, h2, tc, 1);
+                         (S *)ctx->mgx[l].ptr, (const S *)ctx->mgb[l].ptr, h2, tc,
+                         (C)ctx->omega, 1);
     });
     LAUNCHCHK();
     return MAC3D_OK;
@@ -398,11 +408,13 @@ int mg_sweep(Ctx *ctx, int l, bool zero) {
       if (zero)
         hipLaunchKernelGGL((k_rbgs_sweep<S, true>), grid, dim3(SW_THREADS), 0, ctx->stream, d,
                            (const S *)ctx->mgx[0].ptr, (S *)ctx->mgxa[0].ptr,
-                           (const S *)ctx->mgb[0].ptr, (C)1, (C)1, sc, cz, ncx, ncy);
+                           (const S *)ctx->mgb[0].ptr, (C)1, (C)1, (C)ctx->omega, sc, cz,
+                           ncx, ncy);
       else
         hipLaunchKernelGGL((k_rbgs_sweep<S, false>), grid, dim3(SW_THREADS), 0, ctx->stream, d,
                            (const S *)ctx->mgx[0].ptr, (S *)ctx->mgxa[0].ptr,
-                           (const S *)ctx->mgb[0].ptr, (C)1, (C)1, sc, cz, ncx, ncy);
+                           (const S *)ctx->mgb[0].ptr, (C)1, (C)1, (C)ctx->omega, sc, cz,
+                           ncx, ncy);
     });
   }
   LAUNCHCHK();
@@ -553,33 +565,65 @@ size_t cell_blocks(const Dims &d) {
   return (size_t)g.x * g.y * g.z;
 }
 
-// Mixed-mode residual in FP64 and its norms (enqueued only: the norms land
-// in pinned host memory once the stream reaches the copy), and the scale of
-// the next correction cycle.
-int residual64(Ctx *ctx) {
-  const Dims &d = ctx->d[0];
-  {
-    ProfScope ps(ctx->prof, ctx->stream, KC_RESID64);
-    hipLaunchKernelGGL(k_residual64, cell_grid(d), kCellBlock, 0, ctx->stream, d,
-                       (const double *)ctx->p.ptr, (const double *)ctx->rhs.ptr,
-                       (float *)ctx->r0.ptr, ctx->part);
-    hipLaunchKernelGGL(k_fold3, dim3(1), dim3(256), 0, ctx->stream, (const double *)ctx->part,
-                       cell_blocks(d), ctx->red, ctx->scale);
-  }
+// ---- mixed mode: FP64 pressure ring
+void sync_p(Ctx *ctx) {
+  if (ctx->prec == MAC3D_PREC_MIXED) ctx->p = ctx->pr[ctx->role[0]];
+}
+
+// fold the residual partials, set the next correction's scale, and queue the
+// norms' copy to pinned host memory
+int fold_norms(Ctx *ctx) {
+  hipLaunchKernelGGL(k_fold3, dim3(1), dim3(256), 0, ctx->stream, (const double *)ctx->part,
+                     cell_blocks(ctx->d[0]), ctx->red, ctx->scale);
   LAUNCHCHK();
   HIPCHK(hipMemcpyAsync(ctx->red_host, ctx->red, 3 * sizeof(double), hipMemcpyDeviceToHost,
                         ctx->stream));
   return MAC3D_OK;
 }
 
-int update64(Ctx *ctx) {
-  size_t n = ctx->n[0];
-  ProfScope ps(ctx->prof, ctx->stream, KC_RESID64);
-  hipLaunchKernelGGL(k_update64, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, ctx->stream, n,
-                     (double *)ctx->p.ptr, (const float *)ctx->mgx[0].ptr,
-                     (const double *)ctx->scale);
+// First residual of a step: the warm start extrapolated from the pressure
+// history (order min(extrap, hist)), formed into the spare buffer, which then
+// becomes current while the history shifts.  order 0 restarts from p.
+int mixed_begin(Ctx *ctx, int order) {
+  const Dims &d = ctx->d[0];
+  double c0 = 1.0, c1 = 0.0, c2 = 0.0;
+  if (order >= 2) { c0 = 3.0; c1 = -3.0; c2 = 1.0; }
+  else if (order == 1) { c0 = 2.0; c1 = -1.0; }
+  int *r = ctx->role;
+  {
+    ProfScope ps(ctx->prof, ctx->stream, KC_RESID64);
+    hipLaunchKernelGGL(k_extrap_residual64, cell_grid(d), kCellBlock, 0, ctx->stream, d,
+                       (const double *)ctx->pr[r[0]].ptr, (const double *)ctx->pr[r[1]].ptr,
+                       (const double *)ctx->pr[r[2]].ptr, c0, c1, c2,
+                       (const double *)ctx->rhs.ptr, (double *)ctx->pr[r[3]].ptr,
+                       (float *)ctx->r0.ptr, ctx->part);
+  }
   LAUNCHCHK();
-  return MAC3D_OK;
+  int nr[4] = {r[3], r[0], r[1], r[2]};
+  for (int q = 0; q < 4; ++q) r[q] = nr[q];
+  sync_p(ctx);
+  return fold_norms(ctx);
+}
+
+// One refinement iteration: correction V-cycle on r/max|r| from zero, then
+// p <- p + max|r| e formed into the spare buffer together with the new
+// residual; spare and current swap.
+int mixed_iter(Ctx *ctx) {
+  int rc;
+  if ((rc = vcycle(ctx, true))) return rc;
+  const Dims &d = ctx->d[0];
+  int *r = ctx->role;
+  {
+    ProfScope ps(ctx->prof, ctx->stream, KC_RESID64);
+    hipLaunchKernelGGL(k_update_residual64, cell_grid(d), kCellBlock, 0, ctx->stream, d,
+                       (const double *)ctx->pr[r[0]].ptr, (const float *)ctx->mgx[0].ptr,
+                       (const double *)ctx->scale, (const double *)ctx->rhs.ptr,
+                       (double *)ctx->pr[r[3]].ptr, (float *)ctx->r0.ptr, ctx->part);
+  }
+  LAUNCHCHK();
+  std::swap(r[0], r[3]);
+  sync_p(ctx);
+  return fold_norms(ctx);
 }
 
 int div_norms(Ctx *ctx, double *mx, double *l2) {
@@ -611,6 +655,7 @@ uint64_t parity_bits(const Ctx *ctx) {
   if (ctx->dye.ptr != ctx->dye_home) b |= 2;
   for (int l = 0; l <= ctx->L; ++l)
     if (ctx->mgx[l].ptr != ctx->mgx_home[l]) b |= (uint64_t)4 << l;
+  for (int q = 0; q < 4; ++q) b |= (uint64_t)ctx->role[q] << (16 + 2 * q);
   return b;
 }
 
@@ -619,15 +664,17 @@ void set_parity_bits(Ctx *ctx, uint64_t b) {
   if (((ctx->dye.ptr != ctx->dye_home) ? 1u : 0u) != ((b >> 1) & 1)) std::swap(ctx->dye, ctx->dye2);
   for (int l = 0; l <= ctx->L; ++l)
     if (((ctx->mgx[l].ptr != ctx->mgx_home[l]) ? 1u : 0u) != ((b >> (2 + l)) & 1)) swap_x(ctx, l);
+  for (int q = 0; q < 4; ++q) ctx->role[q] = (int)((b >> (16 + 2 * q)) & 3);
+  sync_p(ctx);
 }
 
 // Run a launch sequence: eagerly while profiling, otherwise through a graph
 // captured on first use for the current parities and dt.
 template <class F>
-int run_seq(Ctx *ctx, int kind, double dt, F &&enqueue) {
+int run_seq(Ctx *ctx, int kind, double dt, F &&enqueue, int variant = 0) {
   if (!ctx->use_graphs || ctx->prof.enabled) return enqueue();
   uint64_t before = parity_bits(ctx);
-  uint64_t key = ((uint64_t)kind << 32) | before;
+  uint64_t key = ((uint64_t)kind << 40) | ((uint64_t)variant << 32) | before;
   auto it = ctx->graphs.find(key);
   if (it == ctx->graphs.end() || it->second.dt != dt) {
     if (it != ctx->graphs.end()) {
@@ -703,11 +750,12 @@ int one_step(Ctx *ctx, double dt, int *cycles, int *iters, double *rel, int *con
     *conv = 1;
     return MAC3D_OK;
   }
+  int order = std::min(ctx->extrap, ctx->hist);
   rc = run_seq(ctx, SEQ_PRE, dt, [&]() {
     int r;
     if ((r = seq_pre(ctx, dt))) return r;
-    return residual64(ctx);
-  });
+    return mixed_begin(ctx, order);
+  }, order);
   if (rc) return rc;
   int it = 0;
   *conv = 0;
@@ -718,17 +766,13 @@ int one_step(Ctx *ctx, double dt, int *cycles, int *iters, double *rel, int *con
     *rel = r;
     if (r <= ctx->cfg.ir_tol || mr == 0.0) { *conv = 1; break; }
     if (it >= ctx->cfg.ir_max_iters) break;
-    rc = run_seq(ctx, SEQ_ITER, 0.0, [&]() {
-      int q;
-      if ((q = vcycle(ctx, true))) return q;
-      if ((q = update64(ctx))) return q;
-      return residual64(ctx);
-    });
+    rc = run_seq(ctx, SEQ_ITER, 0.0, [&]() { return mixed_iter(ctx); });
     if (rc) return rc;
     ++it;
   }
   *cycles += it;
   *iters = it;
+  ctx->hist = std::min(ctx->hist + 1, 2);
   return run_seq(ctx, SEQ_POST, dt, [&]() { return seq_post(ctx, dt); });
 }
 
@@ -772,6 +816,8 @@ int valid_config(const mac3d_config *c, std::string &why) {
     return 0;
   }
   if (c->precision < 0 || c->precision > 2) { why = "unknown precision"; return 0; }
+  if (c->extrapolate < -1 || c->extrapolate > 2) { why = "extrapolate must be -1, 0, 1 or 2"; return 0; }
+  if (!(c->sor_omega >= 0.0 && c->sor_omega < 2.0)) { why = "sor_omega must be in [0, 2)"; return 0; }
   if (!(c->dx > 0) || !std::isfinite(c->dx)) { why = "dx must be finite and > 0"; return 0; }
   if (!(c->rho_w > 0) || !(c->rho_b > 0)) { why = "densities must be > 0"; return 0; }
   if ((size_t)c->nx * c->ny * c->nz > (size_t)1 << 31) { why = "grid too large"; return 0; }
@@ -798,7 +844,13 @@ int create_impl(Ctx *ctx, const mac3d_config *cfg) {
     if ((rc = alloc(ctx, *b, t0, n0))) return rc;
   if ((rc = alloc(ctx, ctx->cvel, t0, 4 * n0))) return rc;
   bool mixed = ctx->prec == MAC3D_PREC_MIXED;
-  if ((rc = alloc(ctx, ctx->p, mixed ? TF64 : t0, n0))) return rc;
+  ctx->omega = cfg->sor_omega > 0.0 ? cfg->sor_omega : (mixed ? 1.2 : 1.0);
+  ctx->extrap = mixed ? (cfg->extrapolate < 0 ? 2 : std::min(cfg->extrapolate, 2)) : 0;
+  if (mixed) {
+    for (int q = 0; q < 4; ++q)
+      if ((rc = alloc(ctx, ctx->pr[q], TF64, n0))) return rc;
+    ctx->p = ctx->pr[0];
+  } else if ((rc = alloc(ctx, ctx->p, t0, n0))) return rc;
   if ((rc = alloc(ctx, ctx->rhs, mixed ? TF64 : t0, n0))) return rc;
   if (mixed) {
     if ((rc = alloc(ctx, ctx->e0, TF32, n0))) return rc;
@@ -872,6 +924,7 @@ int reset_scale(Ctx *ctx) {
 }
 
 int init_blobs_impl(Ctx *ctx, const mac3d_blob *blobs, int nblobs) {
+  ctx->hist = 0;
   if (nblobs < 0 || (nblobs > 0 && !blobs)) return fail(ctx, MAC3D_EINVAL, "bad blob list");
   for (int q = 0; q < nblobs; ++q) {
     const mac3d_blob &b = blobs[q];
@@ -969,6 +1022,7 @@ int save_state(Ctx *ctx, void *const *dst) {
 }
 
 int load_state(Ctx *ctx, void *const *src) {
+  ctx->hist = 0;
   for (int f = 0; f < 5; ++f)
     HIPCHK(hipMemcpyAsync(state_field(ctx, f), src[f], state_bytes(ctx, f),
                           hipMemcpyDeviceToDevice, ctx->stream));
@@ -1024,14 +1078,14 @@ int adj_step(Ctx *ctx, double dt, void *const *src) {
   if (ctx->prec == MAC3D_PREC_MIXED) {
     bool g = ctx->use_graphs;
     ctx->use_graphs = false;
-    // the refinement loop of one_step without its pre/post stages
-    if ((rc = residual64(ctx))) return rc;
+    // the refinement loop of one_step from the step's own p (no history)
+    if ((rc = mixed_begin(ctx, 0))) return rc;
     for (;;) {
       HIPCHK(hipStreamSynchronize(ctx->stream));
       double mr = ctx->red_host[0], rr = ctx->red_host[1], bb = ctx->red_host[2];
       double r = bb > 0.0 ? std::sqrt(rr / bb) : 0.0;
       if (r <= ctx->cfg.ir_tol || mr == 0.0 || it >= ctx->cfg.ir_max_iters) break;
-      if ((rc = vcycle(ctx, true)) || (rc = update64(ctx)) || (rc = residual64(ctx))) return rc;
+      if ((rc = mixed_iter(ctx))) return rc;
       ++it;
     }
     ctx->use_graphs = g;
@@ -1243,6 +1297,8 @@ int mac3d_default_config(mac3d_config *c) {
   c->vel_damp = 1.0;
   c->maccormack = 0.45 * 0.995;  // advect.ts:202 with concSharpness 0.995
   c->ir_tol = 1e-8;
+  c->extrapolate = -1;
+  c->sor_omega = 0.0;
   return MAC3D_OK;
 }
 
@@ -1284,6 +1340,7 @@ int mac3d_upload(void *h, int field, int level, const void *host, int dtype, siz
   int comp = (field >= MAC3D_FIELD_CU && field <= MAC3D_FIELD_CW) ? field - MAC3D_FIELD_CU : -1;
   if (count != (comp >= 0 ? ctx->n[0] : b->n))
     return fail(ctx, MAC3D_EINVAL, "element count does not match the field");
+  if (field == MAC3D_FIELD_P) ctx->hist = 0;  // an uploaded pressure restarts the history
   size_t hsz = dtype == MAC3D_F32 ? 4 : 8;
   HIPCHK(hipMemcpyAsync(ctx->stage, host, hsz * count, hipMemcpyHostToDevice, ctx->stream));
   if (comp >= 0) {

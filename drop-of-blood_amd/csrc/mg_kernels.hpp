@@ -247,11 +247,13 @@ namespace mac3d {
 // few planes for the z-marching sweep to pay.  Thread t of a row takes the
 // pair (2t, 2t+1) and updates the one of this colour.  ZERO: the sweep starts
 // from a zero guess (MG_RBGS_RED_ZERO, fluid.ts:605-641): the other colour is
-// written as 0 and no neighbour is read.
+// written as 0 and no neighbour is read.  omega != 1: over-relaxed update
+// x + omega (gs - x) (the mixed mode's smoother; the reference's is omega 1).
 template <class S, bool ZERO>
 __global__ void __launch_bounds__(256)
 k_rbgs_pass(Dims d, S *__restrict__ x, const S *__restrict__ b,
-            typename compute_of<S>::type h2, typename compute_of<S>::type topc, int parity) {
+            typename compute_of<S>::type h2, typename compute_of<S>::type topc,
+            typename compute_of<S>::type omega, int parity) {
   typedef typename compute_of<S>::type C;
   int t = blockIdx.x * blockDim.x + threadIdx.x;
   int j = blockIdx.y * blockDim.y + threadIdx.y;
@@ -277,7 +279,12 @@ k_rbgs_pass(Dims d, S *__restrict__ x, const S *__restrict__ b,
   } else {
     mg_stencil<S, C>(x, d, topc, i, j, k, c, sn, dg);
   }
-  x[c] = store_as<S, C>((sn - h2 * (C)b[c]) / dg);
+  C v = (sn - h2 * (C)b[c]) / dg;
+  if (omega != C(1)) {
+    C old = ZERO ? C(0) : (C)x[c];
+    v = old + omega * (v - old);
+  }
+  x[c] = store_as<S, C>(v);
 }
 
 // ---------------------------------------------------------------------------
@@ -290,7 +297,8 @@ k_rbgs_pass(Dims d, S *__restrict__ x, const S *__restrict__ b,
 // block waits on another.  Threads work on (even, odd) x-pairs, each pair
 // holding one cell of each colour, so no lane idles on a parity test.
 // Registers prefetch two planes ahead.  ZERO: zero initial guess (xin is not
-// read).  scale (nullable): the right-hand side is b * scale[0].
+// read).  scale (nullable): the right-hand side is b * scale[0].  omega != 1:
+// over-relaxed updates x + omega (gs - x).
 #define SW_TX 64
 #define SW_TY 16
 #define SW_EX (SW_TX + 4)                 // staged row, two-cell ring
@@ -321,7 +329,8 @@ template <class S, bool ZERO>
 __global__ void __launch_bounds__(SW_THREADS)
 k_rbgs_sweep(Dims d, const S *__restrict__ xin, S *__restrict__ xout, const S *__restrict__ b,
              typename compute_of<S>::type h2, typename compute_of<S>::type topc,
-             const double *__restrict__ scale, int cz, int ncx, int ncy) {
+             typename compute_of<S>::type omega, const double *__restrict__ scale, int cz,
+             int ncx, int ncy) {
   typedef typename compute_of<S>::type C;
   typedef typename vec2_of<S>::type V2;
   __shared__ C sp[4][SW_EY][SW_EX];
@@ -397,7 +406,12 @@ k_rbgs_sweep(Dims d, const S *__restrict__ xin, S *__restrict__ xout, const S *_
     if (gy < ny - 1) { if (nbrs) sn += sp[k & 3][r + 1][c]; }
     if (k > 0) { if (nbrs) sn += sp[(k - 1) & 3][r][c]; dg += C(1); }
     if (k < nz - 1) { if (nbrs) sn += sp[(k + 1) & 3][r][c]; dg += C(1); }
-    return (sn - h2 * sb[k & 1][r][c]) * fast_rcp(dg);
+    C v = (sn - h2 * sb[k & 1][r][c]) * fast_rcp(dg);
+    if (omega != C(1)) {
+      C old = ZERO ? C(0) : sp[k & 3][r][c];
+      v = old + omega * (v - old);
+    }
+    return v;
   };
   auto red = [&](int k) {
     if (k < 0 || k >= nz) return;

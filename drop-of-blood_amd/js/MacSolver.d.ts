@@ -6,7 +6,7 @@ export interface MacConfig {
   dx?: number; rho_w?: number; mu_w?: number; rho_b?: number;
   tau_y?: number; mu_inf?: number; g?: number; gamma_reg?: number;
   shear_eps2?: number; alpha_max?: number; vel_damp?: number;
-  maccormack?: number; ir_tol?: number;
+  maccormack?: number; ir_tol?: number; extrapolate?: -1 | 0 | 1 | 2; sor_omega?: number;
 }
 export interface Blob { cx: number; cy: number; cz: number; radius: number; w0: number; }
 export interface StepStats {

@@ -28,7 +28,7 @@ const PRECISION = Object.freeze({ fp32: 0, mixed: 1, fp64: 2 });
  *   device?:number, dx?:number, rho_w?:number, mu_w?:number, rho_b?:number,
  *   tau_y?:number, mu_inf?:number, g?:number, gamma_reg?:number,
  *   shear_eps2?:number, alpha_max?:number, vel_damp?:number,
- *   maccormack?:number, ir_tol?:number}} MacConfig
+ *   maccormack?:number, ir_tol?:number, extrapolate?:number, sor_omega?:number}} MacConfig
  * @typedef {{steps:number, vcycles:number, irIters:number, converged:number,
  *   maxDiv:number, l2Div:number, maxDivRate:number, relRes:number, ms:number}} StepStats
  */

@@ -24,10 +24,11 @@ STATUS = {0: "OK", 1: "EINVAL", 2: "EHIP", 3: "ENOMEM", 4: "ESTATE"}
 
 class Config(C.Structure):
     _fields_ = [(n, C.c_int32) for n in ("nx", "ny", "nz", "levels", "nu1", "nu2", "cycles",
-                                          "precision", "ir_max_iters", "device")] + \
+                                          "precision", "ir_max_iters", "device",
+                                          "extrapolate")] + \
                [(n, C.c_double) for n in ("dx", "rho_w", "mu_w", "rho_b", "tau_y", "mu_inf", "g",
                                            "gamma_reg", "shear_eps2", "alpha_max", "vel_damp",
-                                           "maccormack", "ir_tol")]
+                                           "maccormack", "ir_tol", "sor_omega")]
 
 
 class Blob(C.Structure):

@@ -77,9 +77,9 @@ void set_num(napi_env env, napi_value obj, const char *name, double v) {
 }
 
 #define CONFIG_INT_FIELDS(X) X(nx) X(ny) X(nz) X(levels) X(nu1) X(nu2) X(cycles) \
-  X(precision) X(ir_max_iters) X(device)
+  X(precision) X(ir_max_iters) X(device) X(extrapolate)
 #define CONFIG_DBL_FIELDS(X) X(dx) X(rho_w) X(mu_w) X(rho_b) X(tau_y) X(mu_inf) X(g) \
-  X(gamma_reg) X(shear_eps2) X(alpha_max) X(vel_damp) X(maccormack) X(ir_tol)
+  X(gamma_reg) X(shear_eps2) X(alpha_max) X(vel_damp) X(maccormack) X(ir_tol) X(sor_omega)
 
 napi_value config_to_js(napi_env env, const mac3d_config &c) {
   napi_value o;

@@ -106,6 +106,8 @@ typedef struct mac3d_config {
   int32_t precision;    /* MAC3D_PREC_* */
   int32_t ir_max_iters; /* mixed: cap on refinement iterations per step */
   int32_t device;       /* HIP device ordinal */
+  int32_t extrapolate;  /* mixed: warm-start extrapolation order of the FP64
+                           pressure (-1: mode default 2; 0, 1, 2) */
   double dx;            /* cell size (m) */
   double rho_w, mu_w;   /* ambient water density (kg/m^3), viscosity (Pa s) */
   double rho_b;         /* blood density (kg/m^3) */
@@ -117,6 +119,8 @@ typedef struct mac3d_config {
   double vel_damp;      /* velocity multiplier after advection */
   double maccormack;    /* dye MacCormack correction strength */
   double ir_tol;        /* mixed: relative divergence target ||r||/||b|| */
+  double sor_omega;     /* smoother over-relaxation (0: mode default, 1 in the
+                           FP32/FP64 modes as the reference, 1.2 in mixed) */
 } mac3d_config;
 
 /* one spherical blood blob of the seeded initial condition (cell units:

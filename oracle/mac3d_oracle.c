@@ -323,22 +323,27 @@ static void stencil(const double *x, dims3 d, double topc, int i, int j, int k,
  * two textures per pass, copying the other colour; updating in place is the
  * same map because a colour only reads the other colour. */
 static void rbgs_pass(double *x, const double *b, dims3 d, double h2, double topc,
-                      int parity) {
+                      double omega, int parity) {
 #pragma omp parallel for
   for (int k = 0; k < d.nz; ++k)
     for (int j = 0; j < d.ny; ++j)
       for (int i = ((parity + j + k) & 1); i < d.nx; i += 2) {
         double sn, dg;
         stencil(x, d, topc, i, j, k, &sn, &dg);
-        x[IDX(d, i, j, k)] = (sn - h2 * b[IDX(d, i, j, k)]) / dg;
+        double gs = (sn - h2 * b[IDX(d, i, j, k)]) / dg;
+        /* WORKLOAD (mixed mode only): over-relaxed update x + omega (gs - x) */
+        if (omega != 1.0) gs = x[IDX(d, i, j, k)] + omega * (gs - x[IDX(d, i, j, k)]);
+        x[IDX(d, i, j, k)] = gs;
       }
 }
 
-static void smooth(double *x, const double *b, dims3 d, int l, int sweeps) {
+static double smoother_omega(const orc_state *s) { return s->p.omega > 0.0 ? s->p.omega : 1.0; }
+
+static void smooth(double *x, const double *b, dims3 d, int l, int sweeps, double omega) {
   double h2 = level_h2(l), topc = orc_top_coeff(l);
   for (int s = 0; s < sweeps; ++s) {
-    rbgs_pass(x, b, d, h2, topc, 0);
-    rbgs_pass(x, b, d, h2, topc, 1);
+    rbgs_pass(x, b, d, h2, topc, omega, 0);
+    rbgs_pass(x, b, d, h2, topc, omega, 1);
   }
 }
 
@@ -481,22 +486,22 @@ static void coarse_solve(orc_state *s, int L) {
  * exact coarse solve. */
 static void vcycle(orc_state *s) {
   int L = s->p.levels;
-  smooth(s->pr, s->rhs, s->d[0], 0, s->p.nu1);
+  smooth(s->pr, s->rhs, s->d[0], 0, s->p.nu1, smoother_omega(s));
   residual(s->pr, s->rhs, s->mgr[0], s->d[0], 0);
   restrict_op(s->mgr[0], s->d[0], s->mgb[1], s->d[1]);
   for (int l = 1; l < L; ++l) {
     memset(s->mgx[l], 0, sizeof(double) * s->n[l]);
-    smooth(s->mgx[l], s->mgb[l], s->d[l], l, s->p.nu1);
+    smooth(s->mgx[l], s->mgb[l], s->d[l], l, s->p.nu1, smoother_omega(s));
     residual(s->mgx[l], s->mgb[l], s->mgr[l], s->d[l], l);
     restrict_op(s->mgr[l], s->d[l], s->mgb[l + 1], s->d[l + 1]);
   }
   coarse_solve(s, L);
   for (int l = L - 1; l >= 1; --l) {
     prolong_add(s->mgx[l], s->d[l], s->mgx[l + 1], s->d[l + 1]);
-    smooth(s->mgx[l], s->mgb[l], s->d[l], l, s->p.nu2);
+    smooth(s->mgx[l], s->mgb[l], s->d[l], l, s->p.nu2, smoother_omega(s));
   }
   prolong_add(s->pr, s->d[0], s->mgx[1], s->d[1]);
-  smooth(s->pr, s->rhs, s->d[0], 0, s->p.nu2);
+  smooth(s->pr, s->rhs, s->d[0], 0, s->p.nu2, smoother_omega(s));
 }
 
 static double rel_residual(orc_state *s) {
@@ -709,7 +714,7 @@ void orc_mg_op(orc_state *s, int op, int level, int count) {
   if (level < 0 || level > L) return;
   dims3 d = s->d[level];
   switch (op) {
-    case MG_SMOOTH: smooth(s->mgx[level], s->mgb[level], d, level, count); break;
+    case MG_SMOOTH: smooth(s->mgx[level], s->mgb[level], d, level, count, smoother_omega(s)); break;
     case MG_RESIDUAL: residual(s->mgx[level], s->mgb[level], s->mgr[level], d, level); break;
     case MG_RESTRICT:
       if (level < L) restrict_op(s->mgr[level], d, s->mgb[level + 1], s->d[level + 1]);

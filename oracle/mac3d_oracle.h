@@ -24,6 +24,7 @@ typedef struct orc_params {
   int max_cycles;
   double dx, rho_w, mu_w, rho_b, tau_y, mu_inf, g;
   double gamma_reg, shear_eps2, alpha_max, vel_damp, maccormack, tol;
+  double omega; /* smoother over-relaxation (0 or 1: plain Gauss-Seidel) */
 } orc_params;
 
 typedef struct orc_blob {

@@ -25,7 +25,7 @@ class OrcParams(C.Structure):
                                         "solve_to_tol", "max_cycles")] + \
                [(n, C.c_double) for n in ("dx", "rho_w", "mu_w", "rho_b", "tau_y", "mu_inf", "g",
                                            "gamma_reg", "shear_eps2", "alpha_max", "vel_damp",
-                                           "maccormack", "tol")]
+                                           "maccormack", "tol", "omega")]
 
 
 class OrcBlob(C.Structure):
@@ -75,7 +75,7 @@ def set_threads(n: int) -> int:
 
 DEFAULTS = dict(rho_w=1000.0, mu_w=1e-3, rho_b=1060.0, tau_y=0.005, mu_inf=0.0035, g=9.81,
                 gamma_reg=0.40, shear_eps2=1e-6, alpha_max=1.0 / 6.0, vel_damp=1.0,
-                maccormack=0.45 * 0.995, tol=1e-8)
+                maccormack=0.45 * 0.995, tol=1e-8, omega=1.0)
 
 
 class Oracle:

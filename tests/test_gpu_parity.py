@@ -183,3 +183,25 @@ def test_graph_replays_match_eager(prec):
             for f in ("U", "V", "W", "P", "DYE"):
                 a, b = s.download(f), e.download(f)
                 assert np.array_equal(a, b), (step, f)
+
+
+@pytest.mark.parametrize("level", [0, 1])
+def test_over_relaxed_smoother_fp64(level):
+    """The mixed mode's over-relaxed red-black sweep (omega 1.2) against the
+    oracle's, in the FP64 device mode."""
+    from mac3d import MacSolver
+    from mac3d.lib import config_for
+    from mac3d import scenarios
+    w = scenarios.scaled(scenarios.workload("tiny", 1), precision=2)
+    o = oracle.Oracle(w.nx, w.ny, w.nz, w.dx, levels=w.levels, omega=1.2)
+    rng = np.random.default_rng(11 + level)
+    with MacSolver(config_for(w, sor_omega=1.2)) as s:
+        shp = s.level_shape(level)
+        x, b = rng.standard_normal(shp), rng.standard_normal(shp)
+        s.upload("MGX", x, level)
+        s.upload("MGB", b, level)
+        o.set(oracle.F_MGX, x, level)
+        o.set(oracle.F_MGB, b, level)
+        s.mg("SMOOTH", level, 2)
+        o.mg(oracle.MG_SMOOTH, level, 2)
+        assert rel_l2(s.download("MGX", level), o.get(oracle.F_MGX, level)) <= 1e-11
