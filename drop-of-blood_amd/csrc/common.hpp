@@ -117,6 +117,82 @@ __device__ inline void sample_cell3(const T *__restrict__ f0, const T *__restric
   }
 }
 
+// ---------------------------------------------------------------------------
+// Sample points in texel space kept as (integer texel, small offset) per
+// axis: t = b + o with t = uvw * N - 1/2.  The reference's normalised
+// coordinates hold a point to about N * 6e-8 texels in FP32 (1e-5 texel at
+// N = 192); an integer base plus an offset of a few texels holds it to 1e-7
+// texel.  Clamps to [lo, hi] texels and samples use the same clamp-to-edge
+// rule as sample_cell.
+template <class T> struct TPos {
+  int b[3];
+  T o[3];
+};
+
+template <class T>
+__device__ inline void tpos_clamp(TPos<T> &p, const T lo[3], const T hi[3]) {
+#pragma unroll
+  for (int a = 0; a < 3; ++a) {
+    T t = (T)p.b[a] + p.o[a];
+    if (t < lo[a] || t > hi[a]) {
+      T q = t < lo[a] ? lo[a] : hi[a];
+      T fq = floor(q);
+      p.b[a] = (int)fq;
+      p.o[a] = q - fq;
+    }
+  }
+}
+
+template <class T> struct TriIdx {
+  size_t r00, r10, r01, r11;
+  int i0, i1;
+  T fx, fy, fz;
+};
+
+template <class T>
+__device__ inline TriIdx<T> tpos_index(const Dims &d, const TPos<T> &p) {
+  TriIdx<T> t;
+  T fl0 = floor(p.o[0]), fl1 = floor(p.o[1]), fl2 = floor(p.o[2]);
+  int ib = p.b[0] + (int)fl0, jb = p.b[1] + (int)fl1, kb = p.b[2] + (int)fl2;
+  t.fx = p.o[0] - fl0;
+  t.fy = p.o[1] - fl1;
+  t.fz = p.o[2] - fl2;
+  t.i0 = clampi(ib, 0, d.nx - 1);
+  t.i1 = clampi(ib + 1, 0, d.nx - 1);
+  int j0 = clampi(jb, 0, d.ny - 1), j1 = clampi(jb + 1, 0, d.ny - 1);
+  int k0 = clampi(kb, 0, d.nz - 1), k1 = clampi(kb + 1, 0, d.nz - 1);
+  t.r00 = ((size_t)k0 * d.ny + j0) * d.nx;
+  t.r10 = ((size_t)k0 * d.ny + j1) * d.nx;
+  t.r01 = ((size_t)k1 * d.ny + j0) * d.nx;
+  t.r11 = ((size_t)k1 * d.ny + j1) * d.nx;
+  return t;
+}
+
+template <class T>
+__device__ inline T sample_tp(const T *__restrict__ f, const Dims &d, const TPos<T> &p) {
+  TriIdx<T> t = tpos_index(d, p);
+  T x00 = mixv(f[t.r00 + t.i0], f[t.r00 + t.i1], t.fx);
+  T x10 = mixv(f[t.r10 + t.i0], f[t.r10 + t.i1], t.fx);
+  T x01 = mixv(f[t.r01 + t.i0], f[t.r01 + t.i1], t.fx);
+  T x11 = mixv(f[t.r11 + t.i0], f[t.r11 + t.i1], t.fx);
+  return mixv(mixv(x00, x10, t.fy), mixv(x01, x11, t.fy), t.fz);
+}
+
+template <class T>
+__device__ inline void sample_vec_tp(const typename vec4_of<T>::type *__restrict__ f,
+                                     const Dims &d, const TPos<T> &p, T out[3]) {
+  TriIdx<T> t = tpos_index(d, p);
+  auto a000 = f[t.r00 + t.i0], a100 = f[t.r00 + t.i1], a010 = f[t.r10 + t.i0], a110 = f[t.r10 + t.i1];
+  auto a001 = f[t.r01 + t.i0], a101 = f[t.r01 + t.i1], a011 = f[t.r11 + t.i0], a111 = f[t.r11 + t.i1];
+#define MAC3D_TRI(C)                                                                    \
+  mixv(mixv(mixv(a000.C, a100.C, t.fx), mixv(a010.C, a110.C, t.fx), t.fy),              \
+       mixv(mixv(a001.C, a101.C, t.fx), mixv(a011.C, a111.C, t.fx), t.fy), t.fz)
+  out[0] = MAC3D_TRI(x);
+  out[1] = MAC3D_TRI(y);
+  out[2] = MAC3D_TRI(z);
+#undef MAC3D_TRI
+}
+
 // wavefront (64 lanes) reductions
 template <class T> __device__ inline T wave_sum(T v) {
 #pragma unroll

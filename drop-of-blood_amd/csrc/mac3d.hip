@@ -252,14 +252,12 @@ void swap_vel(Ctx *ctx) {
 
 int launch_advect(Ctx *ctx, double dt) {
   const Dims &d = ctx->d[0];
-  double dom[3] = {d.nx * ctx->cfg.dx, d.ny * ctx->cfg.dx, d.nz * ctx->cfg.dx};
   {
     ProfScope ps(ctx->prof, ctx->stream, KC_ADVECT);
     with_step_type(ctx->t0, [&](auto tag) {
       using T = std::remove_pointer_t<decltype(tag)>;
       hipLaunchKernelGGL(k_advect_mac<T>, cell_grid(d), kCellBlock, 0, ctx->stream, d,
-                         (T)(dt / dom[0]), (T)(dt / dom[1]), (T)(dt / dom[2]),
-                         (T)ctx->cfg.vel_damp, (const T *)ctx->u.ptr, (const T *)ctx->v.ptr,
+                         (T)(dt / ctx->cfg.dx), (T)ctx->cfg.vel_damp, (const T *)ctx->u.ptr, (const T *)ctx->v.ptr,
                          (const T *)ctx->w.ptr, (const typename vec4_of<T>::type *)ctx->cvel.ptr,
                          (T *)ctx->u2.ptr, (T *)ctx->v2.ptr,
                          (T *)ctx->w2.ptr);
@@ -322,12 +320,11 @@ int launch_project(Ctx *ctx) {
 
 int launch_dye_predict(Ctx *ctx, double dt) {
   const Dims &d = ctx->d[0];
-  double dom[3] = {d.nx * ctx->cfg.dx, d.ny * ctx->cfg.dx, d.nz * ctx->cfg.dx};
   ProfScope ps(ctx->prof, ctx->stream, KC_DYE);
   with_step_type(ctx->t0, [&](auto tag) {
     using T = std::remove_pointer_t<decltype(tag)>;
     hipLaunchKernelGGL(k_dye_predict<T>, cell_grid(d), kCellBlock, 0, ctx->stream, d,
-                       (T)(dt / dom[0]), (T)(dt / dom[1]), (T)(dt / dom[2]),
+                       (T)(dt / ctx->cfg.dx),
                        (const typename vec4_of<T>::type *)ctx->cvel.ptr,
                        (const T *)ctx->dye.ptr, (T *)ctx->dpred.ptr);
   });
@@ -337,14 +334,12 @@ int launch_dye_predict(Ctx *ctx, double dt) {
 
 int launch_dye_resolve(Ctx *ctx, double dt) {
   const Dims &d = ctx->d[0];
-  double dom[3] = {d.nx * ctx->cfg.dx, d.ny * ctx->cfg.dx, d.nz * ctx->cfg.dx};
   {
     ProfScope ps(ctx->prof, ctx->stream, KC_DYE);
     with_step_type(ctx->t0, [&](auto tag) {
       using T = std::remove_pointer_t<decltype(tag)>;
       hipLaunchKernelGGL(k_dye_resolve<T>, cell_grid(d), kCellBlock, 0, ctx->stream, d,
-                         (T)(dt / dom[0]), (T)(dt / dom[1]), (T)(dt / dom[2]),
-                         (T)ctx->cfg.maccormack, (const typename vec4_of<T>::type *)ctx->cvel.ptr,
+                         (T)(dt / ctx->cfg.dx), (T)ctx->cfg.maccormack, (const typename vec4_of<T>::type *)ctx->cvel.ptr,
                          (const T *)ctx->dye.ptr,
                          (const T *)ctx->dpred.ptr, (T *)ctx->dye2.ptr);
     });

@@ -63,41 +63,62 @@ __device__ inline void trace_back(const Dims &d,
   out[2] = clampv(f[2] - vm[2] * sc[2], lo, hi);
 }
 
+// traceBackUVW in texel space: the face point f (texel base + offset), RK2
+// displacement in texels v dt/h, clamps at [0.002 N - 1/2, 0.998 N - 1/2].
+template <class T>
+__device__ inline void trace_back_tp(const Dims &d,
+                                     const typename vec4_of<T>::type *__restrict__ cvel,
+                                     T dth, const TPos<T> &f, const T lo[3], const T hi[3],
+                                     TPos<T> &out) {
+  TPos<T> p0 = f;
+  tpos_clamp(p0, lo, hi);
+  T v0[3], vm[3];
+  sample_vec_tp<T>(cvel, d, p0, v0);
+  TPos<T> m = f;
+  for (int a = 0; a < 3; ++a) m.o[a] -= T(0.5) * (v0[a] * dth);
+  tpos_clamp(m, lo, hi);
+  sample_vec_tp<T>(cvel, d, m, vm);
+  out = f;
+  for (int a = 0; a < 3; ++a) out.o[a] -= vm[a] * dth;
+  tpos_clamp(out, lo, hi);
+}
+
 // FLUID_ADVECT_MAC (fluid.ts:144-192, 323-325): each staggered component is
 // traced back from its own face and sampled with the -1/2 texel shift of
 // sampleMacU/V/W (fluid.ts:154-167), then multiplied by the damping factor;
 // the two solid wall faces stay 0.  sc = dt / domain size per axis.
 template <class T>
 __global__ void __launch_bounds__(256)
-k_advect_mac(Dims d, T sc0, T sc1, T sc2, T damp, const T *__restrict__ u,
-             const T *__restrict__ v, const T *__restrict__ w,
-             const typename vec4_of<T>::type *__restrict__ cvel, T *__restrict__ uo,
-             T *__restrict__ vo, T *__restrict__ wo) {
+k_advect_mac(Dims d, T dth, T damp, const T *__restrict__ u, const T *__restrict__ v,
+             const T *__restrict__ w, const typename vec4_of<T>::type *__restrict__ cvel,
+             T *__restrict__ uo, T *__restrict__ vo, T *__restrict__ wo) {
   MAC3D_CELL_PROLOGUE
-  const T sc[3] = {sc0, sc1, sc2};
-  const T inx = T(1) / (T)d.nx, iny = T(1) / (T)d.ny, inz = T(1) / (T)d.nz;
-  const T lo = T(0.001), hi = T(0.999);
-  T b[3];
-  {
-    T f[3] = {((T)i + T(1)) / (T)d.nx, ((T)j + T(0.5)) / (T)d.ny, ((T)k + T(0.5)) / (T)d.nz};
-    trace_back(d, cvel, sc, f, b);
-    T vx = sample_cell(u, d, clampv(b[0] - T(0.5) * inx, lo, hi), clampv(b[1], lo, hi),
-                       clampv(b[2], lo, hi)) * damp;
-    uo[c] = (i >= d.nx - 1) ? T(0) : vx;
+  const T N[3] = {(T)d.nx, (T)d.ny, (T)d.nz};
+  T lo2[3], hi2[3], lo1[3], hi1[3];
+  for (int a = 0; a < 3; ++a) {
+    lo2[a] = T(0.002) * N[a] - T(0.5);
+    hi2[a] = T(0.998) * N[a] - T(0.5);
+    lo1[a] = T(0.001) * N[a] - T(0.5);
+    hi1[a] = T(0.999) * N[a] - T(0.5);
   }
-  {
-    T f[3] = {((T)i + T(0.5)) / (T)d.nx, ((T)j + T(1)) / (T)d.ny, ((T)k + T(0.5)) / (T)d.nz};
-    trace_back(d, cvel, sc, f, b);
-    vo[c] = sample_cell(v, d, clampv(b[0], lo, hi), clampv(b[1] - T(0.5) * iny, lo, hi),
-                        clampv(b[2], lo, hi)) * damp;
+  const T *fld[3] = {u, v, w};
+  T res[3];
+#pragma unroll
+  for (int a = 0; a < 3; ++a) {
+    // face +1/2 along axis a: texel (cell + 1/2) on that axis, the cell centre on the others
+    TPos<T> f;
+    f.b[0] = i; f.b[1] = j; f.b[2] = k;
+    f.o[0] = f.o[1] = f.o[2] = T(0);
+    f.o[a] = T(0.5);
+    TPos<T> b;
+    trace_back_tp<T>(d, cvel, dth, f, lo2, hi2, b);
+    b.o[a] -= T(0.5);  // the -1/2 texel shift of sampleMacU/V/W
+    tpos_clamp(b, lo1, hi1);
+    res[a] = sample_tp(fld[a], d, b) * damp;
   }
-  {
-    T f[3] = {((T)i + T(0.5)) / (T)d.nx, ((T)j + T(0.5)) / (T)d.ny, ((T)k + T(1)) / (T)d.nz};
-    trace_back(d, cvel, sc, f, b);
-    T vz = sample_cell(w, d, clampv(b[0], lo, hi), clampv(b[1], lo, hi),
-                       clampv(b[2] - T(0.5) * inz, lo, hi)) * damp;
-    wo[c] = (k >= d.nz - 1) ? T(0) : vz;
-  }
+  uo[c] = (i >= d.nx - 1) ? T(0) : res[0];
+  vo[c] = res[1];
+  wo[c] = (k >= d.nz - 1) ? T(0) : res[2];
 }
 
 // Neighbour face values with the workload's wall rules: no-slip solid walls
@@ -235,16 +256,45 @@ __device__ inline void dye_back(const Dims &d, const typename vec4_of<T>::type *
   back[2] = clampv(uvw[2] - vm[2] * sc[2], lo, hi);
 }
 
+// dye_back in texel space from the cell centre (no clamp on the first
+// sample), clamps at [0.001 N - 1/2, 0.999 N - 1/2]
+template <class T>
+__device__ inline void dye_back_tp(const Dims &d, const typename vec4_of<T>::type *__restrict__ cvel,
+                                   T dth, int i, int j, int k, const T lo[3], const T hi[3],
+                                   T v0[3], TPos<T> &back) {
+  TPos<T> c;
+  c.b[0] = i; c.b[1] = j; c.b[2] = k;
+  c.o[0] = c.o[1] = c.o[2] = T(0);
+  sample_vec_tp<T>(cvel, d, c, v0);
+  TPos<T> m = c;
+  for (int a = 0; a < 3; ++a) m.o[a] -= T(0.5) * (v0[a] * dth);
+  tpos_clamp(m, lo, hi);
+  T vm[3];
+  sample_vec_tp<T>(cvel, d, m, vm);
+  back = c;
+  for (int a = 0; a < 3; ++a) back.o[a] -= vm[a] * dth;
+  tpos_clamp(back, lo, hi);
+}
+
+template <class T>
+__device__ inline void dye_bounds(const Dims &d, T lo[3], T hi[3]) {
+  const T N[3] = {(T)d.nx, (T)d.ny, (T)d.nz};
+  for (int a = 0; a < 3; ++a) {
+    lo[a] = T(0.001) * N[a] - T(0.5);
+    hi[a] = T(0.999) * N[a] - T(0.5);
+  }
+}
+
 template <class T>
 __global__ void __launch_bounds__(256)
-k_dye_predict(Dims d, T sc0, T sc1, T sc2, const typename vec4_of<T>::type *__restrict__ cvel,
+k_dye_predict(Dims d, T dth, const typename vec4_of<T>::type *__restrict__ cvel,
               const T *__restrict__ dye, T *__restrict__ pred) {
   MAC3D_CELL_PROLOGUE
-  const T sc[3] = {sc0, sc1, sc2};
-  T uvw[3] = {((T)i + T(0.5)) / (T)d.nx, ((T)j + T(0.5)) / (T)d.ny, ((T)k + T(0.5)) / (T)d.nz};
-  T v0[3], back[3];
-  dye_back(d, cvel, sc, uvw, v0, back);
-  pred[c] = sample_cell(dye, d, back[0], back[1], back[2]);
+  T lo[3], hi[3], v0[3];
+  dye_bounds(d, lo, hi);
+  TPos<T> back;
+  dye_back_tp<T>(d, cvel, dth, i, j, k, lo, hi, v0, back);
+  pred[c] = sample_tp(dye, d, back);
 }
 
 // CONC_RESOLVE's MacCormack core (advect.ts:196-222): forward estimate from
@@ -252,23 +302,22 @@ k_dye_predict(Dims d, T sc0, T sc1, T sc2, const typename vec4_of<T>::type *__re
 // clamped to the min/max of the 8 source texels around the backtrace point.
 template <class T>
 __global__ void __launch_bounds__(256)
-k_dye_resolve(Dims d, T sc0, T sc1, T sc2, T strength,
-              const typename vec4_of<T>::type *__restrict__ cvel, const T *__restrict__ dye,
-              const T *__restrict__ pred, T *__restrict__ out) {
+k_dye_resolve(Dims d, T dth, T strength, const typename vec4_of<T>::type *__restrict__ cvel,
+              const T *__restrict__ dye, const T *__restrict__ pred, T *__restrict__ out) {
   MAC3D_CELL_PROLOGUE
-  const T sc[3] = {sc0, sc1, sc2};
-  const T lo = T(0.001), hi = T(0.999);
-  T uvw[3] = {((T)i + T(0.5)) / (T)d.nx, ((T)j + T(0.5)) / (T)d.ny, ((T)k + T(0.5)) / (T)d.nz};
-  T v0[3], back[3];
-  dye_back(d, cvel, sc, uvw, v0, back);
-  T fx = clampv(uvw[0] + v0[0] * sc[0], lo, hi);
-  T fy = clampv(uvw[1] + v0[1] * sc[1], lo, hi);
-  T fz = clampv(uvw[2] + v0[2] * sc[2], lo, hi);
-  T est = sample_cell(pred, d, fx, fy, fz);
+  T lo[3], hi[3], v0[3];
+  dye_bounds(d, lo, hi);
+  TPos<T> back;
+  dye_back_tp<T>(d, cvel, dth, i, j, k, lo, hi, v0, back);
+  TPos<T> fw;
+  fw.b[0] = i; fw.b[1] = j; fw.b[2] = k;
+  for (int a = 0; a < 3; ++a) fw.o[a] = v0[a] * dth;
+  tpos_clamp(fw, lo, hi);
+  T est = sample_tp(pred, d, fw);
   T corr = pred[c] + strength * (dye[c] - est);
-  int bi = (int)floor(back[0] * (T)d.nx - T(0.5));
-  int bj = (int)floor(back[1] * (T)d.ny - T(0.5));
-  int bk = (int)floor(back[2] * (T)d.nz - T(0.5));
+  int bi = back.b[0] + (int)floor(back.o[0]);
+  int bj = back.b[1] + (int)floor(back.o[1]);
+  int bk = back.b[2] + (int)floor(back.o[2]);
   int i0 = clampi(bi, 0, d.nx - 1), i1 = clampi(bi + 1, 0, d.nx - 1);
   int j0 = clampi(bj, 0, d.ny - 1), j1 = clampi(bj + 1, 0, d.ny - 1);
   int k0 = clampi(bk, 0, d.nz - 1), k1 = clampi(bk + 1, 0, d.nz - 1);
