@@ -92,6 +92,170 @@ k_residual_restrict(Dims df, Dims dc, const Sf *__restrict__ x, const Sf *__rest
   bc[idx3(dc, i, j, k)] = store_as<Sc, C>(s * C(0.125));
 }
 
+// The same residual-restriction with one thread per fine cell (coalesced
+// 7-point loads): a 64 x 2 x 2 block of fine cells sums to 32 coarse cells,
+// pairs in x by a lane shuffle, the 2 x 2 (y, z) by LDS.
+template <class Sf, class Sc>
+__global__ void __launch_bounds__(256)
+k_residual_restrict_t(Dims df, Dims dc, const Sf *__restrict__ x, const Sf *__restrict__ b,
+                      Sc *__restrict__ bc, typename compute_of<Sf>::type h2,
+                      typename compute_of<Sf>::type topc, const double *__restrict__ scale) {
+  typedef typename compute_of<Sf>::type C;
+  __shared__ C sr[4][64];
+  const C bscale = scale ? (C)scale[0] : C(1);
+  const int tx = threadIdx.x, ty = threadIdx.y, tz = threadIdx.z;
+  const int i = blockIdx.x * 64 + tx, j = 2 * blockIdx.y + ty, k = 2 * blockIdx.z + tz;
+  C r = C(0);
+  if (i < df.nx) {
+    size_t c = idx3(df, i, j, k);
+    C sn, dg;
+    mg_stencil<Sf, C>(x, df, topc, i, j, k, c, sn, dg);
+    r = (C)b[c] * bscale - (sn - dg * (C)x[c]) / h2;
+  }
+  r += __shfl_xor(r, 1, 64);
+  sr[ty * 2 + tz][tx] = r;
+  __syncthreads();
+  if (ty == 0 && tz == 0 && !(tx & 1)) {
+    int ci = (blockIdx.x * 64 + tx) >> 1;
+    if (ci < dc.nx) {
+      C s = sr[0][tx] + sr[1][tx] + sr[2][tx] + sr[3][tx];
+      bc[idx3(dc, ci, blockIdx.y, blockIdx.z)] = store_as<Sc, C>(s * C(0.125));
+    }
+  }
+}
+
+// Residual-restriction marching in z: a 512-thread block owns a 64 x 8 fine
+// column tile and an even z-chunk; planes of x are staged in a three-plane
+// LDS ring with a one-cell ring in x and y (each x value read about 1.3x
+// instead of ~3x by independent blocks), the 2 x 2 x 2 child sums are formed
+// in registers (z), a lane shuffle (x) and LDS (y).  Same arithmetic as
+// k_residual followed by k_restrict.
+#define RR_TX 64
+#define RR_TY 8
+#define RR_EX (RR_TX + 2)
+#define RR_EY (RR_TY + 2)
+#define RR_T (RR_TX * RR_TY)
+#define RR_PL (RR_EX * RR_EY)
+#define RR_LPT ((RR_PL + RR_T - 1) / RR_T)
+
+template <class Sf, class Sc>
+__global__ void __launch_bounds__(RR_T)
+k_residual_restrict_zm(Dims df, Dims dc, const Sf *__restrict__ x, const Sf *__restrict__ b,
+                       Sc *__restrict__ bc, typename compute_of<Sf>::type h2,
+                       typename compute_of<Sf>::type topc, const double *__restrict__ scale,
+                       int cz, int ncx, int ncy) {
+  typedef typename compute_of<Sf>::type C;
+  __shared__ C sx[3][RR_EY][RR_EX];
+  __shared__ C sred[RR_TY][RR_TX];
+  const C bscale = scale ? (C)scale[0] : C(1);
+  const int nx = df.nx, ny = df.ny, nz = df.nz;
+  const int lb = blockIdx.x;
+  const int x0 = (lb % ncx) * RR_TX, y0 = ((lb / ncx) % ncy) * RR_TY;
+  const int k0 = (lb / (ncx * ncy)) * cz, k1 = min(k0 + cz, nz);
+  const int tx = threadIdx.x, ty = threadIdx.y, tid = ty * RR_TX + tx;
+  const size_t plane = (size_t)nx * ny;
+  const int gx = x0 + tx, gy = y0 + ty;
+  const bool inside = gx < nx && gy < ny;
+  long st_g[RR_LPT];
+  int st_r[RR_LPT], st_c[RR_LPT];
+#pragma unroll
+  for (int q = 0; q < RR_LPT; ++q) {
+    int v = tid + q * RR_T;
+    st_r[q] = v / RR_EX;
+    st_c[q] = v % RR_EX;
+    int sxg = x0 - 1 + st_c[q], syg = y0 - 1 + st_r[q];
+    st_g[q] = (v < RR_PL && sxg >= 0 && sxg < nx && syg >= 0 && syg < ny) ? (long)syg * nx + sxg : -1;
+  }
+  C pre[RR_LPT];
+  auto fetch = [&](int k) {
+#pragma unroll
+    for (int q = 0; q < RR_LPT; ++q)
+      pre[q] = (st_g[q] >= 0 && k >= 0 && k < nz) ? (C)x[k * plane + st_g[q]] : C(0);
+  };
+  auto stage = [&](int k) {
+#pragma unroll
+    for (int q = 0; q < RR_LPT; ++q)
+      if (tid + q * RR_T < RR_PL) sx[(k + 3) % 3][st_r[q]][st_c[q]] = pre[q];
+  };
+  for (int k = k0 - 1; k <= k0; ++k) {
+    fetch(k);
+    stage(k);
+  }
+  fetch(k0 + 1);
+  C rsum = C(0);
+  for (int k = k0; k < k1; ++k) {
+    stage(k + 1);
+    __syncthreads();
+    fetch(k + 2);
+    if (inside) {
+      const int r = ty + 1, c = tx + 1, s0 = k % 3;
+      C sn = C(0), dg = (gy < ny - 1) ? C(1) : topc;
+      if (gx > 0) { sn += sx[s0][r][c - 1]; dg += C(1); }
+      if (gx < nx - 1) { sn += sx[s0][r][c + 1]; dg += C(1); }
+      if (gy > 0) { sn += sx[s0][r - 1][c]; dg += C(1); }
+      if (gy < ny - 1) { sn += sx[s0][r + 1][c]; }
+      if (k > 0) { sn += sx[(k + 2) % 3][r][c]; dg += C(1); }
+      if (k < nz - 1) { sn += sx[(k + 1) % 3][r][c]; dg += C(1); }
+      rsum += (C)b[k * plane + (size_t)gy * nx + gx] * bscale - (sn - dg * sx[s0][r][c]) / h2;
+    }
+    if (k & 1) {
+      C pair = rsum + __shfl_xor(rsum, 1, 64);
+      sred[ty][tx] = pair;
+      rsum = C(0);
+    }
+    __syncthreads();
+    if ((k & 1) && !(ty & 1) && !(tx & 1) && inside) {
+      C s = sred[ty][tx] + sred[ty + 1][tx];
+      bc[idx3(dc, gx >> 1, gy >> 1, k >> 1)] = store_as<Sc, C>(s * C(0.125));
+    }
+  }
+}
+
+// MG_PROLONG_ADD with one thread per 2 x 2 x 2 block of fine cells (the
+// children of one coarse cell), which share a 3 x 3 x 3 coarse
+// neighbourhood: child offset 0 along an axis takes 3/4 of the parent and
+// 1/4 of the parent's lower neighbour, offset 1 takes 3/4 and 1/4 of the
+// upper neighbour (cell-centred weights of fluid.ts:806-855; clamped at the
+// walls, 0 above the top row).
+template <class Sc, class Sf>
+__global__ void __launch_bounds__(256)
+k_prolong_add_2x(Dims df, Dims dc, const Sc *__restrict__ ec, Sf *__restrict__ xf) {
+  typedef typename compute_of<Sf>::type C;
+  int I = blockIdx.x * blockDim.x + threadIdx.x;
+  int J = blockIdx.y * blockDim.y + threadIdx.y;
+  int K = blockIdx.z;
+  if (I >= dc.nx || J >= dc.ny) return;
+  C e[3][3][3];  // [dz][dy][dx], offsets -1..1
+#pragma unroll
+  for (int a = 0; a < 3; ++a)
+#pragma unroll
+    for (int bq = 0; bq < 3; ++bq)
+#pragma unroll
+      for (int q = 0; q < 3; ++q) e[a][bq][q] = (C)coarse_at(ec, dc, I + q - 1, J + bq - 1, K + a - 1);
+  // weights along one axis for child offset o: (lower, mid, upper)
+  const C wl[2] = {C(0.25), C(0)}, wm = C(0.75), wu[2] = {C(0), C(0.25)};
+#pragma unroll
+  for (int oz = 0; oz < 2; ++oz)
+#pragma unroll
+    for (int oy = 0; oy < 2; ++oy) {
+      C yz[3];  // interpolated in z and y, per x offset
+#pragma unroll
+      for (int q = 0; q < 3; ++q) {
+        C zc[3];
+#pragma unroll
+        for (int bq = 0; bq < 3; ++bq)
+          zc[bq] = wl[oz] * e[0][bq][q] + wm * e[1][bq][q] + wu[oz] * e[2][bq][q];
+        yz[q] = wl[oy] * zc[0] + wm * zc[1] + wu[oy] * zc[2];
+      }
+      size_t base = idx3(df, 2 * I, 2 * J + oy, 2 * K + oz);
+#pragma unroll
+      for (int ox = 0; ox < 2; ++ox) {
+        C corr = wl[ox] * yz[0] + wm * yz[1] + wu[ox] * yz[2];
+        xf[base + ox] = store_as<Sf, C>((C)xf[base + ox] + corr);
+      }
+    }
+}
+
 // MG_PROLONG_ADD (fluid.ts:806-855): x_f += cell-centred trilinear
 // interpolation of the coarse correction; coarse indices clamped at the
 // walls, 0 above the top row.  Fine cell i sits at coarse coordinate
