@@ -193,6 +193,34 @@ __device__ inline void sample_vec_tp(const typename vec4_of<T>::type *__restrict
 #undef MAC3D_TRI
 }
 
+// Raw buffer resource over `bytes` bytes from `base` (stride 0; word 3 is the
+// gfx9 default, 0x00020000).  A load whose byte offset lies outside
+// [0, bytes) returns zeros: the buffer unit's range check stands in for a
+// branch, so the loads issue unconditionally and the waits on them count
+// exactly how many are still in flight.
+typedef __amdgpu_buffer_rsrc_t BufRsrc;
+__device__ inline BufRsrc buf_rsrc(const void *base, uint32_t bytes) {
+  return __builtin_amdgcn_make_buffer_rsrc(const_cast<void *>(base), (short)0, (int)bytes,
+                                           0x00020000);
+}
+template <class V> __device__ inline V buf_load(BufRsrc r, int off);
+template <> __device__ inline float buf_load<float>(BufRsrc r, int off) {
+  return __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(r, off, 0, 0));
+}
+template <> __device__ inline double buf_load<double>(BufRsrc r, int off) {
+  return __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(r, off, 0, 0));
+}
+template <> __device__ inline float2 buf_load<float2>(BufRsrc r, int off) {
+  return __builtin_bit_cast(float2, __builtin_amdgcn_raw_buffer_load_b64(r, off, 0, 0));
+}
+template <> __device__ inline double2 buf_load<double2>(BufRsrc r, int off) {
+  return __builtin_bit_cast(double2, __builtin_amdgcn_raw_buffer_load_b128(r, off, 0, 0));
+}
+// in-plane byte offset of a staged position outside the grid: with every
+// range below 2^30 bytes, it stays negative (out of range) for any plane
+// offset the marching kernels add to it
+#define MAC3D_OFF_NONE (-(1 << 30))
+
 // wavefront (64 lanes) reductions
 template <class T> __device__ inline T wave_sum(T v) {
 #pragma unroll

@@ -357,11 +357,13 @@ void swap_x(Ctx *ctx, int l) {
 
 // z-chunk of the level-0 sweep kernel: about three 512-thread blocks per CU
 // (256 CUs) in one round, chunks of at least 8 planes
-int sweep_cz(const Dims &d) {
+int sweep_cz(const Dims &d, size_t elem) {
   int cols = ((d.nx + SW_TX - 1) / SW_TX) * ((d.ny + SW_TY - 1) / SW_TY);
   int chunks = std::max(1, 768 / cols);
-  int cz = (d.nz + chunks - 1) / chunks;
-  return std::max(cz, std::min(d.nz, 8));
+  int cz = std::max((d.nz + chunks - 1) / chunks, std::min(d.nz, 8));
+  size_t pb = (size_t)d.nx * d.ny * elem;  // the chunk's buffer ranges stay below 2^30 bytes
+  while (cz > 1 && (size_t)(cz + 8) * pb >= ((size_t)1 << 30)) --cz;
+  return cz;
 }
 
 // One red-black sweep on level l.  Level 0: the fused out-of-place kernel,
@@ -391,7 +393,7 @@ int mg_sweep(Ctx *ctx, int l, bool zero) {
     LAUNCHCHK();
     return MAC3D_OK;
   }
-  int cz = sweep_cz(d);
+  int cz = sweep_cz(d, ctx->mgx[0].type == TF64 ? 8 : 4);
   int ncx = (d.nx + SW_TX - 1) / SW_TX, ncy = (d.ny + SW_TY - 1) / SW_TY;
   dim3 grid(ncx * ncy * ((d.nz + cz - 1) / cz));
   const double *sc = ctx->scale;
@@ -428,10 +430,50 @@ int mg_fill_zero(Ctx *ctx, Buf &b) {
   return MAC3D_OK;
 }
 
+// z-chunk of the two-sweep level-0 kernel: about two 512-thread blocks per CU
+// (the LDS of two fills a CU), at least 8 planes, even (the kernel's
+// unrolled halves have fixed plane parities), and a chunk's buffer ranges
+// below 2^30 bytes
+int sweep2_cz(const Dims &d) {
+  int cols = ((d.nx + TB_TX - 1) / TB_TX) * ((d.ny + TB_TY - 1) / TB_TY);
+  int chunks = std::max(1, 512 / cols);
+  int cz = std::max((d.nz + chunks - 1) / chunks, std::min(d.nz, 8));
+  cz = (cz + 1) & ~1;
+  size_t pb = (size_t)d.nx * d.ny * sizeof(float);
+  while (cz > 2 && (size_t)(cz + 16) * pb >= ((size_t)1 << 30)) cz -= 2;
+  return cz;
+}
+
+// Two red-black sweeps on level 0 in one pass (FP32 storage): mgx -> mgxa,
+// then the two swap roles.  zero: the first sweep starts from a zero guess.
+int mg_sweep2(Ctx *ctx, bool zero) {
+  const Dims &d = ctx->d[0];
+  int cz = sweep2_cz(d);
+  int ncx = (d.nx + TB_TX - 1) / TB_TX, ncy = (d.ny + TB_TY - 1) / TB_TY;
+  dim3 grid(ncx * ncy * ((d.nz + cz - 1) / cz));
+  {
+    ProfScope ps(ctx->prof, ctx->stream, KC_SMOOTH0X2);
+    auto kern = zero ? k_rbgs_sweep2<true> : k_rbgs_sweep2<false>;
+    hipLaunchKernelGGL(kern, grid, dim3(TB_T), 0, ctx->stream, d,
+                       (const float *)ctx->mgx[0].ptr, (float *)ctx->mgxa[0].ptr,
+                       (const float *)ctx->mgb[0].ptr, 1.f, 1.f, (float)ctx->omega,
+                       (const double *)ctx->scale, cz, ncx, ncy);
+  }
+  LAUNCHCHK();
+  swap_x(ctx, 0);
+  return MAC3D_OK;
+}
+
+// sweeps red-black sweeps on level l; on level 0 with FP32 storage, pairs of
+// sweeps run as one two-sweep pass
 int mg_smooth(Ctx *ctx, int l, int sweeps, bool zero) {
   int rc;
   if (zero && sweeps == 0) return mg_fill_zero(ctx, ctx->mgx[l]);
-  for (int s = 0; s < sweeps; ++s)
+  int s = 0;
+  if (l == 0 && ctx->mgx[0].type == TF32)
+    for (; s + 2 <= sweeps; s += 2)
+      if ((rc = mg_sweep2(ctx, zero && s == 0))) return rc;
+  for (; s < sweeps; ++s)
     if ((rc = mg_sweep(ctx, l, zero && s == 0))) return rc;
   return MAC3D_OK;
 }
@@ -838,6 +880,11 @@ int valid_config(const mac3d_config *c, std::string &why) {
   if (!(c->dx > 0) || !std::isfinite(c->dx)) { why = "dx must be finite and > 0"; return 0; }
   if (!(c->rho_w > 0) || !(c->rho_b > 0)) { why = "densities must be > 0"; return 0; }
   if ((size_t)c->nx * c->ny * c->nz > (size_t)1 << 31) { why = "grid too large"; return 0; }
+  if ((size_t)c->nx * c->ny > (size_t)1 << 22) {
+    why = "an x-y plane holds at most 2^22 cells (the level-0 sweeps address a chunk of "
+          "planes with 32-bit offsets)";
+    return 0;
+  }
   return 1;
 }
 
@@ -1540,6 +1587,9 @@ int mac3d_mg_op(void *h, int op, int level, int count) {
       rc = mg_coarse(ctx);
       break;
     case MAC3D_MG_ZERO: rc = mg_fill_zero(ctx, ctx->mgx[level]); break;
+    case MAC3D_MG_SMOOTH_ZERO:
+      rc = reset_scale(ctx) ? MAC3D_EHIP : mg_smooth(ctx, level, count, true);
+      break;
     default: return fail(ctx, MAC3D_EINVAL, "unknown multigrid op");
   }
   if (rc) return rc;

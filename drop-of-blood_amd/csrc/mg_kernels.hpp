@@ -10,6 +10,8 @@
 
 #include "common.hpp"
 
+#include <type_traits>
+
 namespace mac3d {
 
 // Neighbour sum and diagonal of evalMgStencil.  topc is the top row's lid
@@ -463,11 +465,15 @@ k_rbgs_pass(Dims d, S *__restrict__ x, const S *__restrict__ b,
 // Registers prefetch two planes ahead.  ZERO: zero initial guess (xin is not
 // read).  scale (nullable): the right-hand side is b * scale[0].  omega != 1:
 // over-relaxed updates x + omega (gs - x).
+#ifndef SW_TX
 #define SW_TX 64
+#endif
+#ifndef SW_TY
 #define SW_TY 16
+#endif
 #define SW_EX (SW_TX + 4)                 // staged row, two-cell ring
 #define SW_EY (SW_TY + 4)
-#define SW_THREADS 512
+#define SW_THREADS (SW_TX / 2 * SW_TY)  // one black pair per thread
 #define SW_NV (SW_EX / 2 * SW_EY)         // 680 staged pairs per plane
 #define SW_LPT ((SW_NV + SW_THREADS - 1) / SW_THREADS)   // 2
 #define SW_RP ((SW_TX + 2) / 2 * (SW_TY + 2))           // 594 red pairs
@@ -487,6 +493,36 @@ __device__ inline double fast_rcp(double x) { return 1.0 / x; }
 __device__ inline int xcd_remap(int b, int nblocks) {
   if (nblocks % 8) return b;
   return (b % 8) * (nblocks / 8) + b / 8;
+}
+
+// Red-black update of one cell, shared by the level-0 sweep kernels so that
+// one sweep and the two-sweep pass round identically: x' = (neighbour sum -
+// h^2 b) / diag (MG_RBGS_RED / MG_RBGS_BLACK, fluid.ts:645-681), the sum in
+// the stencil's order x-, x+, y-, y+, z-, z+ with 0 for neighbours outside
+// the grid, then x + omega (x' - x) when omega != 1.
+template <class C>
+__device__ inline C rb_cell_r(C xm, C xp, C ym, C yp, C zm, C zp, C bv, C old, C h2, C rdg,
+                              C omega) {
+  C sn = xm + xp;
+  sn += ym;
+  sn += yp;
+  sn += zm;
+  sn += zp;
+  C v = (sn - h2 * bv) * rdg;
+  if (omega != C(1)) v = old + omega * (v - old);
+  return v;
+}
+template <class C>
+__device__ inline C rb_cell(C xm, C xp, C ym, C yp, C zm, C zp, C bv, C old, C h2, C dg,
+                            C omega) {
+  return rb_cell_r<C>(xm, xp, ym, yp, zm, zp, bv, old, h2, fast_rcp(dg), omega);
+}
+// diagonal of evalMgStencil (fluid.ts:561-603) at (gx, gy, k): one per
+// neighbour inside the grid, the lid coefficient topc at the top row
+template <class C>
+__device__ inline C rb_diag(const Dims &d, int gx, int gy, int k, C topc) {
+  return ((gy < d.ny - 1) ? C(1) : topc) +
+         C((gx > 0) + (gx < d.nx - 1) + (gy > 0) + (k > 0) + (k < d.nz - 1));
 }
 
 template <class S, bool ZERO>
@@ -510,24 +546,27 @@ k_rbgs_sweep(Dims d, const S *__restrict__ xin, S *__restrict__ xout, const S *_
 
   // staged pairs of this thread (fixed over the march)
   int st_r[SW_LPT], st_c[SW_LPT];
-  long st_g[SW_LPT];  // in-plane offset, -1 when outside the grid or the stage
+  int st_o[SW_LPT];  // in-plane byte offset, MAC3D_OFF_NONE outside the grid or the stage
 #pragma unroll
   for (int q = 0; q < SW_LPT; ++q) {
     int v = tid + q * SW_THREADS;
     st_r[q] = v / (SW_EX / 2);
     st_c[q] = 2 * (v % (SW_EX / 2));
     int gx = x0 - 2 + st_c[q], gy = y0 - 2 + st_r[q];
-    st_g[q] = (v < SW_NV && gx >= 0 && gx < nx && gy >= 0 && gy < ny) ? (long)gy * nx + gx : -1;
+    st_o[q] = (v < SW_NV && gx >= 0 && gx < nx && gy >= 0 && gy < ny)
+                  ? (gy * nx + gx) * (int)sizeof(S) : MAC3D_OFF_NONE;
   }
-  auto fetch = [&](const S *__restrict__ f, int k, V2 (&pre)[SW_LPT]) {
+  // The planes this chunk reads, as buffer ranges: xin k0-2 .. k1+1, b
+  // k0-1 .. k1 (clipped to the grid).  Planes outside them (the grid's z
+  // walls, and the prefetches past the chunk's end) load as zeros.
+  const int pb = (int)(plane * sizeof(S));
+  const int kx0 = max(k0 - 2, 0), kx1 = min(k1 + 2, nz);
+  const int kb0 = max(k0 - 1, 0), kb1 = min(k1 + 1, nz);
+  const BufRsrc rx = buf_rsrc(xin + (size_t)kx0 * plane, (uint32_t)(kx1 - kx0) * pb);
+  const BufRsrc rb = buf_rsrc(b + (size_t)kb0 * plane, (uint32_t)(kb1 - kb0) * pb);
+  auto fetch = [&](BufRsrc r, int kbase, int k, V2 (&pre)[SW_LPT]) {
 #pragma unroll
-    for (int q = 0; q < SW_LPT; ++q) {
-      V2 v;
-      v.x = S(0);
-      v.y = S(0);
-      if (st_g[q] >= 0 && k >= 0 && k < nz) v = *(const V2 *)(f + k * plane + st_g[q]);
-      pre[q] = v;
-    }
+    for (int q = 0; q < SW_LPT; ++q) pre[q] = buf_load<V2>(r, st_o[q] + (k - kbase) * pb);
   };
   auto stage_p = [&](int k, const V2 (&pre)[SW_LPT]) {
 #pragma unroll
@@ -547,45 +586,42 @@ k_rbgs_sweep(Dims d, const S *__restrict__ xin, S *__restrict__ xout, const S *_
   };
 
   // red pairs of this thread on the tile plus one-cell ring: left cell at
-  // grid (rgx, rgy), stage (rr, rc); flags per cell of the pair
+  // grid (rgx, rgy), stage (rr, rc); rv false for pairs past the ring (parked
+  // on stage (1, 1), computed and dropped) and rows outside the grid
   int rr[SW_RPT], rc[SW_RPT], rgx[SW_RPT], rgy[SW_RPT];
   bool rv[SW_RPT];
 #pragma unroll
   for (int q = 0; q < SW_RPT; ++q) {
     int v = tid + q * SW_THREADS;
     int row = v / ((SW_TX + 2) / 2), pc = v % ((SW_TX + 2) / 2);
+    if (v >= SW_RP) row = pc = 0;
     rr[q] = row + 1;
     rc[q] = 2 * pc + 1;
     rgx[q] = x0 - 1 + 2 * pc;
     rgy[q] = y0 - 1 + row;
     rv[q] = v < SW_RP && rgy[q] >= 0 && rgy[q] < ny;
   }
-  // stencil update of the cell at stage (r, c), grid (gx, gy, k)
-  // (nbrs false: red cells of a zero-guess sweep, whose neighbours are 0)
+  // Stencil update of the cell at stage (r, c), grid (gx, gy, k).  Staged
+  // values outside the grid are zero (out-of-range loads, never updated),
+  // so the six neighbours are summed without branches, in the order of the
+  // reference stencil; only the diagonal depends on where the cell is.
+  // (nbrs false: red cells of a zero-guess sweep, whose neighbours are 0.)
   auto update = [&](int r, int c, int gx, int gy, int k, bool nbrs) {
-    C sn = C(0), dg = (gy < ny - 1) ? C(1) : topc;
-    if (gx > 0) { if (nbrs) sn += sp[k & 3][r][c - 1]; dg += C(1); }
-    if (gx < nx - 1) { if (nbrs) sn += sp[k & 3][r][c + 1]; dg += C(1); }
-    if (gy > 0) { if (nbrs) sn += sp[k & 3][r - 1][c]; dg += C(1); }
-    if (gy < ny - 1) { if (nbrs) sn += sp[k & 3][r + 1][c]; }
-    if (k > 0) { if (nbrs) sn += sp[(k - 1) & 3][r][c]; dg += C(1); }
-    if (k < nz - 1) { if (nbrs) sn += sp[(k + 1) & 3][r][c]; dg += C(1); }
-    C v = (sn - h2 * sb[k & 1][r][c]) * fast_rcp(dg);
-    if (omega != C(1)) {
-      C old = ZERO ? C(0) : sp[k & 3][r][c];
-      v = old + omega * (v - old);
-    }
-    return v;
+    C z = C(0);
+    return rb_cell<C>(nbrs ? sp[k & 3][r][c - 1] : z, nbrs ? sp[k & 3][r][c + 1] : z,
+                      nbrs ? sp[k & 3][r - 1][c] : z, nbrs ? sp[k & 3][r + 1][c] : z,
+                      nbrs ? sp[(k - 1) & 3][r][c] : z, nbrs ? sp[(k + 1) & 3][r][c] : z,
+                      sb[k & 1][r][c], ZERO ? z : sp[k & 3][r][c], h2,
+                      rb_diag<C>(d, gx, gy, k, topc), omega);
   };
   auto red = [&](int k) {
     if (k < 0 || k >= nz) return;
 #pragma unroll
     for (int q = 0; q < SW_RPT; ++q) {
-      if (!rv[q]) continue;
       int w = (rgx[q] + rgy[q] + k) & 1;  // 0: left cell red, 1: right cell red
       int gx = rgx[q] + w;
-      if (gx < 0 || gx >= nx) continue;
-      sp[k & 3][rr[q]][rc[q] + w] = update(rr[q], rc[q] + w, gx, rgy[q], k, !ZERO);
+      C v = update(rr[q], rc[q] + w, gx, rgy[q], k, !ZERO);
+      if (rv[q] && gx >= 0 && gx < nx) sp[k & 3][rr[q]][rc[q] + w] = v;
     }
   };
   // black pair of this thread on the tile
@@ -608,12 +644,14 @@ k_rbgs_sweep(Dims d, const S *__restrict__ xin, S *__restrict__ xout, const S *_
   // plane k+2 of xin and plane k+1 of b; the plane four ahead is fetched into
   // it as soon as it is staged, so two planes of loads are in flight.
   V2 pA[SW_LPT], pB[SW_LPT], bA[SW_LPT], bB[SW_LPT];
+  // ZERO: the x registers stay 0 and zeros are staged (the black update
+  // sums all six neighbours, red ones outside the grid included)
   auto iter = [&](int k, V2 (&pr)[SW_LPT], V2 (&br_)[SW_LPT]) {
-    if (!ZERO) stage_p(k + 2, pr);
+    stage_p(k + 2, pr);
     stage_b(k + 1, br_);
     __syncthreads();
-    if (!ZERO) fetch(xin, k + 4, pr);
-    fetch(b, k + 3, br_);
+    if (!ZERO) fetch(rx, kx0, k + 4, pr);
+    fetch(rb, kb0, k + 3, br_);
     red(k + 1);
     __syncthreads();
     black_and_write(k);
@@ -623,37 +661,372 @@ k_rbgs_sweep(Dims d, const S *__restrict__ xin, S *__restrict__ xout, const S *_
   // prologue: planes k0-2 .. k0+1 of xin, b planes k0-1, k0; red on k0-1, k0
   {
     V2 t0[SW_LPT], t1[SW_LPT], t2[SW_LPT], t3[SW_LPT], u0[SW_LPT], u1[SW_LPT];
-    if (!ZERO) {
-      fetch(xin, k0 - 2, t0);
-      fetch(xin, k0 - 1, t1);
-      fetch(xin, k0, t2);
-      fetch(xin, k0 + 1, t3);
+#pragma unroll
+    for (int q = 0; q < SW_LPT; ++q) {
+      t0[q].x = t0[q].y = t1[q].x = t1[q].y = S(0);
+      t2[q].x = t2[q].y = t3[q].x = t3[q].y = S(0);
+      pA[q].x = pA[q].y = pB[q].x = pB[q].y = S(0);
     }
-    fetch(b, k0 - 1, u0);
-    fetch(b, k0, u1);
     if (!ZERO) {
-      stage_p(k0 - 2, t0);
-      stage_p(k0 - 1, t1);
-      stage_p(k0, t2);
-      stage_p(k0 + 1, t3);
+      fetch(rx, kx0, k0 - 2, t0);
+      fetch(rx, kx0, k0 - 1, t1);
+      fetch(rx, kx0, k0, t2);
+      fetch(rx, kx0, k0 + 1, t3);
     }
+    fetch(rb, kb0, k0 - 1, u0);
+    fetch(rb, kb0, k0, u1);
+    stage_p(k0 - 2, t0);
+    stage_p(k0 - 1, t1);
+    stage_p(k0, t2);
+    stage_p(k0 + 1, t3);
     stage_b(k0 - 1, u0);
     stage_b(k0, u1);
   }
-  if (!ZERO) {
-    fetch(xin, k0 + 2, pA);
-    fetch(xin, k0 + 3, pB);
-  }
-  fetch(b, k0 + 1, bA);
-  fetch(b, k0 + 2, bB);
+  // (issued in the order the loop issues them, so that the loop's waits
+  // count the same loads in flight on entry as on every later pass)
+  if (!ZERO) fetch(rx, kx0, k0 + 2, pA);
+  fetch(rb, kb0, k0 + 1, bA);
+  if (!ZERO) fetch(rx, kx0, k0 + 3, pB);
+  fetch(rb, kb0, k0 + 2, bB);
   __syncthreads();
   red(k0 - 1);
   red(k0);
   __syncthreads();
-  for (int k = k0; k < k1; k += 2) {
+  // both halves unconditional in the loop body, the odd plane after it
+  int k = k0;
+  for (; k + 1 < k1; k += 2) {
     iter(k, pA, bA);
-    if (k + 1 < k1) iter(k + 1, pB, bB);
+    iter(k + 1, pB, bB);
   }
+  if (k < k1) iter(k, pA, bA);
+}
+
+
+// ---------------------------------------------------------------------------
+// Level-0 smoothing, two red-black sweeps in one pass (temporal blocking):
+// R1, B1 (sweep 1) and R2, B2 (sweep 2) march through z together, so x and b
+// are read once and x written once for two sweeps (12 B per FP32 cell per
+// pass, against 24 for two passes of k_rbgs_sweep).  A 512-thread block owns
+// a 64 x 16 column tile and a z-chunk [k0, k1).  Each colour stage is
+// recomputed on a ring one cell wider than the next stage reads (R1 on the
+// tile plus three cells, B1 plus two, R2 plus one, B2 on the tile), so no
+// block waits on another and the result is exactly two sweeps, with the cell
+// arithmetic of k_rbgs_sweep (rb_cell).
+//
+// LDS holds planes of x in a 7-slot ring (tile plus four cells, 72 x 24) and
+// b in a 5-slot ring (tile plus three cells, 72 x 22), each plane split by
+// colour: [colour][row][pair], pair = (even, odd) x-pair.  A cell's six
+// neighbours are of the other colour: x-/x+ at pairs P-1+s and P+s of its row
+// (s: the cell's side in its pair), y and z at pair P, so every stage reads
+// LDS at stride one.  Iteration j of the march runs three groups with a
+// barrier after each:
+//   B2(j) (reads slots j-1..j+1, writes plane j of xout) with R1(j+4) (reads
+//     slots j+3..j+5, writes red of j+4 and b(j+4) into its b slot);
+//   B1(j+3);
+//   R2(j+2) with the staging of x(j+6).
+// An x slot lives from its staging to B2 of the plane above, seven
+// iterations; a b slot from R1 to B2 of its plane, five.  b(j+6) and x(j+8)
+// go into registers two iterations before their use, by buffer loads whose
+// ranges cover only the planes this chunk reads: planes outside the grid (and
+// past the chunk) load as zeros with no branch.  Each stage takes the pairs
+// covering its ring and updates its colour's cell when that cell lies in the
+// ring and the grid.  ZERO: zero initial guess (x is not read, zeros are
+// staged).  scale (nullable): the right-hand side is b scale[0].  FP32 storage
+// only (an FP64 ring would not fit two blocks' LDS).
+#define TB_TX 64
+#define TB_TY 16
+#ifndef TB_T
+#define TB_T (TB_TX / 2 * TB_TY)   // 512 threads: one tile pair each
+#endif
+#define TB_PW ((TB_TX + 8) / 2)    // pairs per staged row: four-cell ring
+#define TB_EY (TB_TY + 8)
+#define TB_BY (TB_TY + 6)          // b rows: three-cell ring
+#define TB_CH (TB_EY * TB_PW)      // one colour of an x slot
+#define TB_BCH (TB_BY * TB_PW)     // one colour of a b slot
+#define TB_XR 7
+#define TB_BR 5
+#define TB_NQ ((TB_PW * TB_EY + TB_T - 1) / TB_T)  // pairs per thread in the staging and R1
+
+template <bool ZERO>
+__global__ void __launch_bounds__(TB_T)
+k_rbgs_sweep2(Dims d, const float *__restrict__ xin, float *__restrict__ xout,
+              const float *__restrict__ b, float h2, float topc, float omega,
+              const double *__restrict__ scale, int cz, int ncx, int ncy) {
+  typedef float C;
+  typedef float2 V2;
+  static_assert(TB_T == TB_TX / 2 * TB_TY, "one tile pair per thread");
+  static_assert(TB_PW * TB_EY <= 2 * TB_T && TB_PW * TB_BY <= 2 * TB_T, "two pairs per thread");
+  __shared__ C lds[TB_XR * 2 * TB_CH + TB_BR * 2 * TB_BCH];
+  char *const L = (char *)lds;
+  const int nx = d.nx, ny = d.ny, nz = d.nz;
+  const int lb = xcd_remap(blockIdx.x, gridDim.x);
+  const int x0 = (lb % ncx) * TB_TX, y0 = ((lb / ncx) % ncy) * TB_TY;
+  const int k0 = (lb / (ncx * ncy)) * cz;  // even: cz is even
+  const int k1 = min(k0 + cz, nz);
+  const int tid = threadIdx.x;
+  const C bs = scale ? (C)scale[0] : C(1);
+  const int pb = (int)((size_t)nx * ny * sizeof(C));
+
+  // pairs of the x staging (tile plus four cells) and of R1 (rows of the
+  // three-cell ring, the same columns): in-plane byte offsets
+  int xo[2], bo[2];
+#pragma unroll
+  for (int q = 0; q < 2; ++q) {
+    int v = tid + q * TB_T;
+    int row = v / TB_PW, pc = v % TB_PW;
+    int gx = x0 - 4 + 2 * pc, gy = y0 - 4 + row;
+    xo[q] = (v < TB_PW * TB_EY && gx >= 0 && gx < nx && gy >= 0 && gy < ny)
+                ? (gy * nx + gx) * (int)sizeof(C) : MAC3D_OFF_NONE;
+    gy += 1;
+    bo[q] = (v < TB_PW * TB_BY && gx >= 0 && gx < nx && gy >= 0 && gy < ny)
+                ? (gy * nx + gx) * (int)sizeof(C) : MAC3D_OFF_NONE;
+  }
+  const int kx0 = max(k0 - 4, 0), kx1 = min(k1 + 4, nz);
+  const int kb0 = max(k0 - 3, 0), kb1 = min(k1 + 3, nz);
+  const BufRsrc rx = buf_rsrc(xin + (size_t)kx0 * nx * ny, (uint32_t)(kx1 - kx0) * pb);
+  const BufRsrc rb = buf_rsrc(b + (size_t)kb0 * nx * ny, (uint32_t)(kb1 - kb0) * pb);
+  auto fetch = [&](BufRsrc r, int kbase, int k, const int (&off)[2], V2 (&pre)[2]) {
+#pragma unroll
+    for (int q = 0; q < 2; ++q) pre[q] = buf_load<V2>(r, off[q] + (k - kbase) * pb);
+  };
+
+  // ---- per-thread tables, fixed over the march ----------------------------
+  // Stage slots: 0 = B2 (the thread's tile pair), 1-2 = R1, 3-4 = B1, 5-6 =
+  // R2 (pairs tid and tid + TB_T of the stage's ring).  Per slot: byte
+  // offset of the pair in a colour array and of its x- neighbour at even
+  // planes; per slot and plane parity: 1/diag on planes with both z
+  // neighbours in the grid (srd) and on planes 0 and nz - 1 (srb), and bits
+  // 4 st + 2 par (cell valid) and 4 st + 2 par + 1 (cell
+  // on the right of its pair).  Bits 28 + q: staged pair q has red on the
+  // right at even planes; bit 30: staged pair 1 exists; bit 31: R1 pair 1
+  // exists.
+  int sofs[7], sxm[7];
+  C srd[2][7], srb[2][7];
+  unsigned bits = 0;
+  auto setup = [&](int st, int W, int X, int v, int NP) {
+    int PAD = (W + 1) & ~1, PC = TB_TX / 2 + PAD;
+    bool act = v < NP;
+    int row = v / PC, pc = v % PC;
+    int gy = y0 - W + row, gxl = x0 - PAD + 2 * pc;
+    int ip = (row + 4 - W) * TB_PW + pc + (4 - PAD) / 2;
+    sofs[st] = 4 * ip;
+#pragma unroll
+    for (int par = 0; par < 2; ++par) {
+      int sd = (gy + par + X) & 1, gx = gxl + sd;
+      if (par == 0) sxm[st] = 4 * (ip - 1 + sd);
+      bool ok = act && gx >= x0 - W && gx < x0 + TB_TX + W && gx >= 0 && gx < nx && gy >= 0 &&
+                gy < ny;
+      // rb_diag with both z neighbours in the grid (srd) and with one (srb,
+      // planes 0 and nz - 1)
+      C top = (gy < ny - 1) ? C(1) : topc;
+      int nxy = (gx > 0) + (gx < nx - 1) + (gy > 0);
+      srd[par][st] = fast_rcp(top + C(nxy + 2));
+      srb[par][st] = fast_rcp(top + C(nxy + 1));
+      bits |= (ok ? 1u : 0u) << (4 * st + 2 * par);
+      bits |= (unsigned)sd << (4 * st + 2 * par + 1);
+    }
+  };
+  setup(0, 0, 1, tid, TB_T);
+  setup(1, 3, 0, tid, TB_PW * TB_BY);
+  setup(2, 3, 0, tid + TB_T, TB_PW * TB_BY);
+  setup(3, 2, 1, tid, (TB_TX / 2 + 2) * (TB_TY + 4));
+  setup(4, 2, 1, tid + TB_T, (TB_TX / 2 + 2) * (TB_TY + 4));
+  setup(5, 1, 0, tid, (TB_TX / 2 + 2) * (TB_TY + 2));
+  setup(6, 1, 0, tid + TB_T, (TB_TX / 2 + 2) * (TB_TY + 2));
+#pragma unroll
+  for (int q = 0; q < 2; ++q) bits |= (unsigned)((y0 - 4 + (tid + q * TB_T) / TB_PW) & 1) << (28 + q);
+  if (tid + TB_T < TB_PW * TB_EY) bits |= 1u << 30;
+  if (tid + TB_T < TB_PW * TB_BY) bits |= 1u << 31;
+  const int tr = tid / (TB_TX / 2), tpc = tid % (TB_TX / 2);
+  const int tgy = y0 + tr, tgx = x0 + 2 * tpc;
+  const bool tin = tgx < nx && tgy < ny;
+  float *const tout = xout + (size_t)tgy * nx + tgx;
+
+  // ---- LDS addressing -------------------------------------------------------
+  // Slot byte bases: x slot of colour X at XS + X * TB_CH * 4, b slot of
+  // colour X at BS + X * TB_BCH * 4 (the b base one row low: b rows start
+  // one below the x slot's).
+  constexpr int XSL = 2 * TB_CH * 4, BSL = 2 * TB_BCH * 4, XCH = TB_CH * 4, BCH = TB_BCH * 4;
+  constexpr int BOFF = TB_XR * XSL - TB_PW * 4;
+  auto ld = [&](int a) { return *(const C *)(L + a); };
+  auto st_ = [&](int a, C v) { *(C *)(L + a) = v; };
+  // new value of the cell of stage slot st at plane parity par, interior
+  // plane: own colour at own, other colour at oth (same plane), om, op
+  // (planes below and above)
+  // (x- pair offset at odd planes: the cell changes side, 2 ofs - 4 - even)
+  auto calc = [&](int st, int par, bool bnd, int own, int oth, int om, int op, C bv) -> C {
+    int o = sofs[st], xm = oth + (par ? 2 * o - 4 - sxm[st] : sxm[st]);
+    return rb_cell_r<C>(ld(xm), ld(xm + 4), ld(oth + o - 4 * TB_PW), ld(oth + o + 4 * TB_PW),
+                        ld(om + o), ld(op + o), bv, ld(own + o), h2,
+                        bnd ? srb[par][st] : srd[par][st], omega);
+  };
+  // all-ones where bit k of bits is set (a sign-extending bit extract), and
+  // a select by such a mask (bitfield insert): no predicate registers
+  auto mask = [&](int k) { return ((int)(bits << (31 - k))) >> 31; };
+  auto pick = [](int m, C a, C b) {  // m ? a : b
+    return __int_as_float((__float_as_int(a) & m) | (__float_as_int(b) & ~m));
+  };
+  // Write of a stage cell: cells outside the ring or the grid write back the
+  // old value.  Lanes past a stage's pairs are not clamped: their pair index
+  // runs past the ring's rows into rows no stage of the same group writes,
+  // so the write-back is a no-op there too.
+  auto calc_w = [&](int st, int par, bool bnd, int own, int oth, int om, int op, C bv) {
+    int o = sofs[st];
+    C old = ld(own + o);
+    int xm = oth + (par ? 2 * o - 4 - sxm[st] : sxm[st]);
+    C nv = rb_cell_r<C>(ld(xm), ld(xm + 4), ld(oth + o - 4 * TB_PW), ld(oth + o + 4 * TB_PW),
+                        ld(om + o), ld(op + o), bv, old, h2, bnd ? srb[par][st] : srd[par][st],
+                        omega);
+    return pick(mask(4 * st + 2 * par), nv, old);
+  };
+
+  // the prologue's staging (plane p at x ring position (p - (k0 - 4)) mod 7)
+  const int px = k0 - 4;
+  auto stage_x_g = [&](int p, const V2 (&pre)[2]) {
+    C *red = lds + ((p - px) % TB_XR) * (2 * TB_CH), *blk = red + TB_CH;
+#pragma unroll
+    for (int q = 0; q < 2; ++q) {
+      int v = tid + q * TB_T;
+      if (v < TB_PW * TB_EY) {
+        bool rs = (y0 - 4 + v / TB_PW + p) & 1;
+        red[v] = rs ? pre[q].y : pre[q].x;
+        blk[v] = rs ? pre[q].x : pre[q].y;
+      }
+    }
+  };
+
+  // ---- the march --------------------------------------------------------------
+  const int r1lo = max(k0 - 3, 0), r1hi = min(k1 + 3, nz);
+  const int b1lo = max(k0 - 2, 0), b1hi = min(k1 + 2, nz);
+  const int r2lo = max(k0 - 1, 0), r2hi = min(k1 + 1, nz);
+  // iteration j (parity PAR, a compile-time constant); xsl[d] is the x slot
+  // base of plane j - 1 + d (plane j + 6 reuses xsl[0]), bsl[d] the b slot
+  // base of plane j + d
+  int xsl[7], bsl[5];
+  auto iter = [&](auto PARc, int j, V2 (&xr)[2], V2 (&br)[2]) {
+    constexpr int par = decltype(PARc)::value;
+    auto edge = [&](int p) { return p == 0 || p == nz - 1; };
+    // group 1: B2(j), R1(j + 4)
+    if (j >= k0) {
+      int own = xsl[1] + XCH, oth = xsl[1];
+      C vb = calc(0, par, edge(j), own, oth, xsl[0], xsl[2], ld(bsl[0] + BCH + sofs[0]));
+      C vr = ld(oth + sofs[0]);
+      if (tin) {
+        int rt = mask(2 * par + 1);
+        V2 o;
+        o.x = pick(rt, vr, vb);
+        o.y = pick(rt, vb, vr);
+        *(V2 *)(tout + (size_t)j * nx * ny) = o;
+      }
+    }
+    if (j + 4 >= r1lo && j + 4 < r1hi) {
+      const bool bnd = edge(j + 4);
+      int own = xsl[5], oth = xsl[5] + XCH, om = xsl[4] + XCH, op = xsl[6] + XCH;
+      int bR = bsl[4], bK = bsl[4] + BCH;
+      C nv[2], bred[2], bblk[2];
+#pragma unroll
+      for (int q = 0; q < 2; ++q) {
+        C b0 = br[q].x * bs, b1 = br[q].y * bs;
+        int rt = mask(4 * (1 + q) + 2 * par + 1);
+        bred[q] = pick(rt, b1, b0);
+        bblk[q] = pick(rt, b0, b1);
+        nv[q] = calc_w(1 + q, par, bnd, own, oth, om, op, bred[q]);
+      }
+#pragma unroll
+      for (int q = 0; q < 2; ++q) st_(own + sofs[1 + q], nv[q]);
+      st_(bR + sofs[1], bred[0]);
+      st_(bK + sofs[1], bblk[0]);
+      if (bits >> 31) {
+        st_(bR + sofs[2], bred[1]);
+        st_(bK + sofs[2], bblk[1]);
+      }
+    }
+    fetch(rb, kb0, j + 6, bo, br);
+    __syncthreads();
+    // group 2: B1(j + 3)
+    if (j + 3 >= b1lo && j + 3 < b1hi) {
+      constexpr int pp = par ^ 1;
+      const bool bnd = edge(j + 3);
+      int own = xsl[4] + XCH, oth = xsl[4], om = xsl[3], op = xsl[5];
+      int bK = bsl[3] + BCH;
+      C nv0 = calc_w(3, pp, bnd, own, oth, om, op, ld(bK + sofs[3]));
+      C nv1 = calc_w(4, pp, bnd, own, oth, om, op, ld(bK + sofs[4]));
+      st_(own + sofs[3], nv0);
+      st_(own + sofs[4], nv1);
+    }
+    __syncthreads();
+    // group 3: R2(j + 2), staging of x(j + 6)
+    if (j + 2 >= r2lo && j + 2 < r2hi) {
+      const bool bnd = edge(j + 2);
+      int own = xsl[3], oth = xsl[3] + XCH, om = xsl[2] + XCH, op = xsl[4] + XCH;
+      int bR = bsl[2];
+      C nv0 = calc_w(5, par, bnd, own, oth, om, op, ld(bR + sofs[5]));
+      C nv1 = calc_w(6, par, bnd, own, oth, om, op, ld(bR + sofs[6]));
+      st_(own + sofs[5], nv0);
+      st_(own + sofs[6], nv1);
+    }
+    {
+      int red = xsl[0], blk = xsl[0] + XCH;
+#pragma unroll
+      for (int q = 0; q < 2; ++q) {
+        if (q == 0 || ((bits >> 30) & 1)) {
+          int rs = mask(28 + q) ^ (par ? -1 : 0);
+          int a = 4 * (tid + q * TB_T);
+          st_(red + a, pick(rs, xr[q].y, xr[q].x));
+          st_(blk + a, pick(rs, xr[q].x, xr[q].y));
+        }
+      }
+    }
+    if (!ZERO) fetch(rx, kx0, j + 8, xo, xr);
+    __syncthreads();
+  };
+
+  // prologue: x(k0-4 .. k0-2) staged; x(k0-1), x(k0), b(k0-3), b(k0-2) in
+  // registers for the first two iterations (issued in the loop's order)
+  V2 xA[2], xB[2], bA[2], bB[2];
+  {
+    V2 t0[2], t1[2], t2[2];
+#pragma unroll
+    for (int q = 0; q < 2; ++q) xA[q] = xB[q] = t0[q] = t1[q] = t2[q] = make_float2(0.f, 0.f);
+    if (!ZERO) {
+      fetch(rx, kx0, k0 - 4, xo, t0);
+      fetch(rx, kx0, k0 - 3, xo, t1);
+      fetch(rx, kx0, k0 - 2, xo, t2);
+    }
+    fetch(rb, kb0, k0 - 3, bo, bA);
+    if (!ZERO) fetch(rx, kx0, k0 - 1, xo, xA);
+    fetch(rb, kb0, k0 - 2, bo, bB);
+    if (!ZERO) fetch(rx, kx0, k0, xo, xB);
+    stage_x_g(k0 - 4, t0);
+    stage_x_g(k0 - 3, t1);
+    stage_x_g(k0 - 2, t2);
+  }
+  __syncthreads();
+  // j = k0 - 7 is odd.  Ring slots: plane p in x slot (p - (k0 - 4)) mod 7,
+  // in b slot (p - (k0 - 3)) mod 5: at j, planes j-1 .. j+5 in x slots 3, 4,
+  // 5, 6, 0, 1, 2 and planes j .. j+4 in b slots 1, 2, 3, 4, 0.
+#pragma unroll
+  for (int q = 0; q < 7; ++q) xsl[q] = ((3 + q) % TB_XR) * XSL;
+#pragma unroll
+  for (int q = 0; q < 5; ++q) bsl[q] = BOFF + ((1 + q) % TB_BR) * BSL;
+  auto rotate = [&]() {
+    int t = xsl[0];
+#pragma unroll
+    for (int q = 0; q < 6; ++q) xsl[q] = xsl[q + 1];
+    xsl[6] = t;
+    t = bsl[0];
+#pragma unroll
+    for (int q = 0; q < 4; ++q) bsl[q] = bsl[q + 1];
+    bsl[4] = t;
+  };
+  int j = k0 - 7;
+  for (; j + 1 < k1; j += 2) {
+    iter(std::integral_constant<int, 1>(), j, xA, bA);
+    rotate();
+    iter(std::integral_constant<int, 0>(), j + 1, xB, bB);
+    rotate();
+  }
+  if (j < k1) iter(std::integral_constant<int, 1>(), j, xA, bA);
 }
 
 }  // namespace mac3d

@@ -24,6 +24,7 @@ enum {
   KC_PROJECT = MAC3D_KC_PROJECT,
   KC_DYE = MAC3D_KC_DYE,
   KC_RESID64 = MAC3D_KC_REFINE64,
+  KC_SMOOTH0X2 = MAC3D_KC_SMOOTH2_L0,
   KC_COUNT = MAC3D_KC_COUNT
 };
 

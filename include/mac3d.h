@@ -82,13 +82,15 @@ extern "C" {
 #define MAC3D_MG_PROLONG 3  /* MGX[l] += P(MGX[l+1]) */
 #define MAC3D_MG_COARSE 4   /* MGX[L] <- exact solve of L(x) = MGB[L] */
 #define MAC3D_MG_ZERO 5     /* MGX[l] <- 0 */
+#define MAC3D_MG_SMOOTH_ZERO 6 /* count RBGS sweeps on MGB from a zero guess (MGX unread;
+                                  MG_RBGS_RED_ZERO, src/shaders/fluid.ts:605-641) */
 
 /* kernel classes timed by mac3d_profile (HIP events on the stream) */
 #define MAC3D_KC_FACE_TO_CENTRE 0
 #define MAC3D_KC_ADVECT 1
 #define MAC3D_KC_CASSON 2
 #define MAC3D_KC_DIVERGENCE 3
-#define MAC3D_KC_SMOOTH_L0 4       /* one RBGS colour pass on level 0 */
+#define MAC3D_KC_SMOOTH_L0 4       /* one RBGS sweep (both colours) on level 0 */
 #define MAC3D_KC_RESTRICT_L0 5     /* level-0 residual and restriction */
 #define MAC3D_KC_PROLONG_L0 6      /* prolongation onto level 0 */
 #define MAC3D_KC_MG_COARSE_LEVELS 7 /* all multigrid work on levels >= 1 */
@@ -96,7 +98,8 @@ extern "C" {
 #define MAC3D_KC_PROJECT 9
 #define MAC3D_KC_DYE 10            /* dye predictor + resolve */
 #define MAC3D_KC_REFINE64 11       /* mixed: FP64 residual/update */
-#define MAC3D_KC_COUNT 12
+#define MAC3D_KC_SMOOTH2_L0 12     /* two RBGS sweeps in one pass on level 0 (FP32) */
+#define MAC3D_KC_COUNT 13
 
 typedef struct mac3d_config {
   int32_t nx, ny, nz;   /* level-0 cells; each must divide by 2^levels */

@@ -205,3 +205,55 @@ def test_over_relaxed_smoother_fp64(level):
         s.mg("SMOOTH", level, 2)
         o.mg(oracle.MG_SMOOTH, level, 2)
         assert rel_l2(s.download("MGX", level), o.get(oracle.F_MGX, level)) <= 1e-11
+
+
+@pytest.mark.parametrize("dims,levels", [((32, 48, 32), 3), ((96, 40, 20), 2), ((64, 96, 64), 3)])
+@pytest.mark.parametrize("omega", [1.0, 1.2])
+def test_two_sweep_pass_equals_two_sweeps(dims, levels, omega):
+    """Level 0, FP32: a smoothing call of two sweeps runs as one temporally
+    blocked pass; it must equal two calls of one sweep bit for bit, from a
+    given guess and from a zero guess (grids with partial 64 x 16 tiles and
+    partial z-chunks included)."""
+    from mac3d import MacSolver
+    from mac3d.lib import config_for
+    from mac3d import scenarios
+    w = scenarios.scaled(scenarios.workload("tiny", 1), nx=dims[0], ny=dims[1], nz=dims[2],
+                         levels=levels, precision=0)
+    rng = np.random.default_rng(int(omega * 10) + dims[2])
+    with MacSolver(config_for(w, sor_omega=omega)) as s:
+        shp = s.level_shape(0)
+        x = rng.standard_normal(shp).astype(np.float32)
+        b = rng.standard_normal(shp).astype(np.float32)
+        res = {}
+        for name, ops in [("pass", [("SMOOTH", 2)]), ("sweeps", [("SMOOTH", 1), ("SMOOTH", 1)]),
+                          ("zpass", [("SMOOTH_ZERO", 2)]),
+                          ("zsweeps", [("SMOOTH_ZERO", 1), ("SMOOTH", 1)])]:
+            s.upload("MGX", x, 0)
+            s.upload("MGB", b, 0)
+            for op, n in ops:
+                s.mg(op, 0, n)
+            res[name] = s.download("MGX", 0, dtype=np.float32)
+        assert np.array_equal(res["pass"], res["sweeps"])
+        assert np.array_equal(res["zpass"], res["zsweeps"])
+        assert not np.array_equal(res["pass"], res["zpass"])
+
+
+def test_two_sweep_pass_against_oracle():
+    """The FP32 two-sweep pass (omega 1.2, the mixed mode's smoother) against
+    the oracle's two FP64 sweeps on the tiny grid."""
+    from mac3d import MacSolver
+    from mac3d.lib import config_for
+    from mac3d import scenarios
+    w = scenarios.scaled(scenarios.workload("tiny", 1), precision=0)
+    o = oracle.Oracle(w.nx, w.ny, w.nz, w.dx, levels=w.levels, omega=1.2)
+    rng = np.random.default_rng(5)
+    with MacSolver(config_for(w, sor_omega=1.2)) as s:
+        shp = s.level_shape(0)
+        x, b = rng.standard_normal(shp), rng.standard_normal(shp)
+        s.upload("MGX", x, 0)
+        s.upload("MGB", b, 0)
+        o.set(oracle.F_MGX, x, 0)
+        o.set(oracle.F_MGB, b, 0)
+        s.mg("SMOOTH", 0, 2)
+        o.mg(oracle.MG_SMOOTH, 0, 2)
+        assert rel_l2(s.download("MGX", 0), o.get(oracle.F_MGX, 0)) <= TOL32
