@@ -23,9 +23,13 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, os.path.join(ROOT, "drop-of-blood_amd"))
 
 HBM_PEAK_GBPS = 8000.0
-# algorithmic bytes per level-0 cell of one red-black sweep (both colours in
-# one kernel): read p and rhs, write p (FP32)
-SMOOTH_BYTES_PER_CELL = 12.0
+# algorithmic bytes per level-0 cell of one red-black sweep, as the
+# two-colour-pass algorithm moves them (read p and rhs, write p, per colour
+# pass: 2 x 12 B); the level-0 smoothing kernel runs two sweeps per launch
+# and itself moves 12 B per cell (read p and rhs once, write p once)
+SWEEP_BYTES_PER_CELL = 24.0
+SWEEPS_PER_LAUNCH = 2
+KERNEL_BYTES_PER_CELL = 12.0
 
 
 def log(msg):
@@ -122,20 +126,25 @@ def main():
         log(f"[bench] mixed {spsm:.2f} steps/s, {mixed['vcycles_per_step']} cycles/step")
 
     roof = None
-    if "SMOOTH_L0" in prof:
-        ms, n = prof["SMOOTH_L0"]
+    if "SMOOTH2_L0" in prof:
+        ms, n = prof["SMOOTH2_L0"]
         avg_s = ms / n / 1e3
-        bytes_launch = SMOOTH_BYTES_PER_CELL * w32.cells
+        bytes_launch = SWEEP_BYTES_PER_CELL * SWEEPS_PER_LAUNCH * w32.cells
         ach = bytes_launch / avg_s / 1e9
         roof = {"bound": "hbm", "achieved": round(ach, 1), "peak": HBM_PEAK_GBPS, "unit": "GB/s",
                 "frac": round(ach / HBM_PEAK_GBPS, 4), "traffic": None,
-                "kernel": "k_rbgs_sweep<float> (level-0 red-black sweep, both colours)",
+                "kernel": "k_rbgs_sweep2 (level 0, two red-black sweeps per launch)",
                 "avg_launch_us": round(avg_s * 1e6, 2), "launches": n,
-                "bytes_per_launch": bytes_launch}
-        traffic_file = os.path.join(ROOT, "profiles", "rbgs_l0_192x256x192_traffic.json")
+                "bytes_per_launch": bytes_launch,
+                "kernel_bytes_per_launch": KERNEL_BYTES_PER_CELL * w32.cells,
+                "kernel_gbps": round(KERNEL_BYTES_PER_CELL * w32.cells / avg_s / 1e9, 1)}
+        traffic_file = os.path.join(ROOT, "profiles", "rbgs2_l0_192x256x192_traffic.json")
         if os.path.exists(traffic_file):
             with open(traffic_file) as f:
-                roof["traffic"] = json.load(f).get("bytes_per_launch")
+                t = json.load(f)
+            roof["traffic"] = t.get("bytes_per_launch")
+            if roof["traffic"]:
+                roof["traffic_gbps"] = round(roof["traffic"] / avg_s / 1e9, 1)
     kernel_ms = {k: round(v[0] / v[1], 4) for k, v in prof.items()}
     if args.copy_probe:
         copy_probe(w32)

@@ -444,35 +444,37 @@ int sweep2_cz(const Dims &d) {
   return cz;
 }
 
-// Two red-black sweeps on level 0 in one pass (FP32 storage): mgx -> mgxa,
+// Two red-black sweeps on level l in one pass (FP32 storage): mgx -> mgxa,
 // then the two swap roles.  zero: the first sweep starts from a zero guess.
-int mg_sweep2(Ctx *ctx, bool zero) {
-  const Dims &d = ctx->d[0];
+int mg_sweep2(Ctx *ctx, int l, bool zero) {
+  const Dims &d = ctx->d[l];
   int cz = sweep2_cz(d);
   int ncx = (d.nx + TB_TX - 1) / TB_TX, ncy = (d.ny + TB_TY - 1) / TB_TY;
   dim3 grid(ncx * ncy * ((d.nz + cz - 1) / cz));
   {
-    ProfScope ps(ctx->prof, ctx->stream, KC_SMOOTH0X2);
+    ProfScope ps(ctx->prof, ctx->stream, l == 0 ? KC_SMOOTH0X2 : KC_MG);
     auto kern = zero ? k_rbgs_sweep2<true> : k_rbgs_sweep2<false>;
     hipLaunchKernelGGL(kern, grid, dim3(TB_T), 0, ctx->stream, d,
-                       (const float *)ctx->mgx[0].ptr, (float *)ctx->mgxa[0].ptr,
-                       (const float *)ctx->mgb[0].ptr, 1.f, 1.f, (float)ctx->omega,
-                       (const double *)ctx->scale, cz, ncx, ncy);
+                       (const float *)ctx->mgx[l].ptr, (float *)ctx->mgxa[l].ptr,
+                       (const float *)ctx->mgb[l].ptr, (float)level_h2(l),
+                       (float)(l == 0 ? 1.0 : top_coeff(l)), (float)ctx->omega,
+                       l == 0 ? (const double *)ctx->scale : nullptr, cz, ncx, ncy);
   }
   LAUNCHCHK();
-  swap_x(ctx, 0);
+  swap_x(ctx, l);
   return MAC3D_OK;
 }
 
-// sweeps red-black sweeps on level l; on level 0 with FP32 storage, pairs of
-// sweeps run as one two-sweep pass
+// sweeps red-black sweeps on level l.  With FP32 storage, pairs of sweeps
+// run as one two-sweep pass on level 0 and on coarser levels of at least
+// 2^20 cells (below that the pass's chunk overhead outweighs its saving).
 int mg_smooth(Ctx *ctx, int l, int sweeps, bool zero) {
   int rc;
   if (zero && sweeps == 0) return mg_fill_zero(ctx, ctx->mgx[l]);
   int s = 0;
-  if (l == 0 && ctx->mgx[0].type == TF32)
+  if (ctx->mgx[l].type == TF32 && (l == 0 || ctx->n[l] >= ((size_t)1 << 20)))
     for (; s + 2 <= sweeps; s += 2)
-      if ((rc = mg_sweep2(ctx, zero && s == 0))) return rc;
+      if ((rc = mg_sweep2(ctx, l, zero && s == 0))) return rc;
   for (; s < sweeps; ++s)
     if ((rc = mg_sweep(ctx, l, zero && s == 0))) return rc;
   return MAC3D_OK;
