@@ -60,6 +60,7 @@ struct Ctx {
   Dims d[kMaxLev + 1];
   size_t n[kMaxLev + 1];
   hipStream_t stream = nullptr;
+  hipEvent_t norms_ready = nullptr;  // mixed mode: the residual norms reached the host
   std::string err;
   // level-0 step fields
   Buf u, v, w, u2, v2, w2, p, dye, dye2, dpred, cvel, rhs;  // cvel: 4 per cell
@@ -633,9 +634,26 @@ void sync_p(Ctx *ctx) {
 
 // fold the residual partials, set the next correction's scale, and queue the
 // norms' copy to pinned host memory
+// z-chunks of the FP64 residual kernels: about three 512-thread blocks per CU,
+// a chunk's buffer ranges below 2^30 bytes
+struct R64Grid {
+  int cz, ncx, ncy, blocks;
+};
+R64Grid r64_grid(const Dims &d) {
+  R64Grid g;
+  g.ncx = (d.nx + R64_TX - 1) / R64_TX;
+  g.ncy = (d.ny + R64_TY - 1) / R64_TY;
+  int chunks = std::max(1, 768 / (g.ncx * g.ncy));
+  g.cz = (d.nz + chunks - 1) / chunks;
+  size_t pb = (size_t)d.nx * d.ny * 8;
+  while (g.cz > 1 && (size_t)(g.cz + 4) * pb >= ((size_t)1 << 30)) --g.cz;
+  g.blocks = g.ncx * g.ncy * ((d.nz + g.cz - 1) / g.cz);
+  return g;
+}
+
 int fold_norms(Ctx *ctx) {
   hipLaunchKernelGGL(k_fold3, dim3(1), dim3(256), 0, ctx->stream, (const double *)ctx->part,
-                     cell_blocks(ctx->d[0]), ctx->red, ctx->scale);
+                     (size_t)r64_grid(ctx->d[0]).blocks, ctx->red, ctx->scale);
   LAUNCHCHK();
   HIPCHK(hipMemcpyAsync(ctx->red_host, ctx->red, 3 * sizeof(double), hipMemcpyDeviceToHost,
                         ctx->stream));
@@ -653,11 +671,12 @@ int mixed_begin(Ctx *ctx, int order) {
   int *r = ctx->role;
   {
     ProfScope ps(ctx->prof, ctx->stream, KC_RESID64);
-    hipLaunchKernelGGL(k_extrap_residual64, cell_grid(d), kCellBlock, 0, ctx->stream, d,
-                       (const double *)ctx->pr[r[0]].ptr, (const double *)ctx->pr[r[1]].ptr,
-                       (const double *)ctx->pr[r[2]].ptr, c0, c1, c2,
-                       (const double *)ctx->rhs.ptr, (double *)ctx->pr[r[3]].ptr,
-                       (float *)ctx->r0.ptr, ctx->part);
+    R64Grid g = r64_grid(d);
+    hipLaunchKernelGGL(k_extrap_residual64_zm, dim3(g.blocks), dim3(R64_TX, R64_TY), 0,
+                       ctx->stream, d, (const double *)ctx->pr[r[0]].ptr,
+                       (const double *)ctx->pr[r[1]].ptr, (const double *)ctx->pr[r[2]].ptr, c0,
+                       c1, c2, (const double *)ctx->rhs.ptr, (double *)ctx->pr[r[3]].ptr,
+                       (float *)ctx->r0.ptr, ctx->part, g.cz, g.ncx, g.ncy);
   }
   LAUNCHCHK();
   int nr[4] = {r[3], r[0], r[1], r[2]};
@@ -676,10 +695,12 @@ int mixed_iter(Ctx *ctx) {
   int *r = ctx->role;
   {
     ProfScope ps(ctx->prof, ctx->stream, KC_RESID64);
-    hipLaunchKernelGGL(k_update_residual64, cell_grid(d), kCellBlock, 0, ctx->stream, d,
-                       (const double *)ctx->pr[r[0]].ptr, (const float *)ctx->mgx[0].ptr,
-                       (const double *)ctx->scale, (const double *)ctx->rhs.ptr,
-                       (double *)ctx->pr[r[3]].ptr, (float *)ctx->r0.ptr, ctx->part);
+    R64Grid g = r64_grid(d);
+    hipLaunchKernelGGL(k_update_residual64_zm, dim3(g.blocks), dim3(R64_TX, R64_TY), 0,
+                       ctx->stream, d, (const double *)ctx->pr[r[0]].ptr,
+                       (const float *)ctx->mgx[0].ptr, (const double *)ctx->scale,
+                       (const double *)ctx->rhs.ptr, (double *)ctx->pr[r[3]].ptr,
+                       (float *)ctx->r0.ptr, ctx->part, g.cz, g.ncx, g.ncy);
   }
   LAUNCHCHK();
   std::swap(r[0], r[3]);
@@ -821,7 +842,13 @@ int one_step(Ctx *ctx, double dt, int *cycles, int *iters, double *rel, int *con
   int it = 0;
   *conv = 0;
   for (;;) {
-    HIPCHK(hipStreamSynchronize(ctx->stream));
+    // the next launch waits on this test: poll for the norms rather than
+    // block in a stream synchronize, which may sleep before it wakes
+    HIPCHK(hipEventRecord(ctx->norms_ready, ctx->stream));
+    hipError_t q;
+    while ((q = hipEventQuery(ctx->norms_ready)) == hipErrorNotReady) {
+    }
+    if (q != hipSuccess) return fail(ctx, MAC3D_EHIP, std::string("norms: ") + hipGetErrorString(q));
     double mr = ctx->red_host[0], rr = ctx->red_host[1], bb = ctx->red_host[2];
     double r = bb > 0.0 ? std::sqrt(rr / bb) : 0.0;
     *rel = r;
@@ -898,6 +925,7 @@ int create_impl(Ctx *ctx, const mac3d_config *cfg) {
   ctx->t0 = ctx->prec == MAC3D_PREC_FP64 ? TF64 : TF32;
   HIPCHK(hipSetDevice(cfg->device));
   HIPCHK(hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking));
+  HIPCHK(hipEventCreateWithFlags(&ctx->norms_ready, hipEventDisableTiming));
   for (int l = 0; l <= ctx->L; ++l) {
     ctx->d[l] = Dims{cfg->nx >> l, cfg->ny >> l, cfg->nz >> l};
     ctx->n[l] = ctx->d[l].cells();
@@ -978,6 +1006,8 @@ void destroy_impl(Ctx *ctx) {
   if (ctx->blobs) (void)hipFree(ctx->blobs);
   ctx->blobs = nullptr;
   ctx->prof.destroy();
+  if (ctx->norms_ready) (void)hipEventDestroy(ctx->norms_ready);
+  ctx->norms_ready = nullptr;
   if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
   ctx->stream = nullptr;
 }
