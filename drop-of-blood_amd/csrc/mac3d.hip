@@ -432,13 +432,13 @@ int mg_fill_zero(Ctx *ctx, Buf &b) {
 }
 
 // z-chunk of the two-sweep level-0 kernel: about two 512-thread blocks per CU
-// (the LDS of two fills a CU), at least 8 planes, even (the kernel's
-// unrolled halves have fixed plane parities), and a chunk's buffer ranges
-// below 2^30 bytes
+// (the LDS of two fills a CU), at least 6 planes (each chunk pays seven
+// partial iterations at its start), even (the kernel's unrolled halves have
+// fixed plane parities), and a chunk's buffer ranges below 2^30 bytes
 int sweep2_cz(const Dims &d) {
   int cols = ((d.nx + TB_TX - 1) / TB_TX) * ((d.ny + TB_TY - 1) / TB_TY);
   int chunks = std::max(1, 512 / cols);
-  int cz = std::max((d.nz + chunks - 1) / chunks, std::min(d.nz, 8));
+  int cz = std::max((d.nz + chunks - 1) / chunks, std::min(d.nz, 6));
   cz = (cz + 1) & ~1;
   size_t pb = (size_t)d.nx * d.ny * sizeof(float);
   while (cz > 2 && (size_t)(cz + 16) * pb >= ((size_t)1 << 30)) cz -= 2;

@@ -65,15 +65,29 @@ __device__ inline void trace_back(const Dims &d,
 
 // traceBackUVW in texel space: the face point f (texel base + offset), RK2
 // displacement in texels v dt/h, clamps at [0.002 N - 1/2, 0.998 N - 1/2].
+// f is the +1/2 face of cell c along axis a (stride sa): unless the clamp
+// moves it, the first sample's weights are 1/2 along a and 0 on the other
+// axes, so it is mixv(cvel[c], cvel[c + sa], 1/2) -- the value of the
+// 8-texel sample (mixv(x, y, 0) = x exactly) from 2 loads.
 template <class T>
 __device__ inline void trace_back_tp(const Dims &d,
                                      const typename vec4_of<T>::type *__restrict__ cvel,
                                      T dth, const TPos<T> &f, const T lo[3], const T hi[3],
-                                     TPos<T> &out) {
+                                     size_t c, size_t sa, TPos<T> &out) {
   TPos<T> p0 = f;
   tpos_clamp(p0, lo, hi);
   T v0[3], vm[3];
-  sample_vec_tp<T>(cvel, d, p0, v0);
+  bool moved = false;
+#pragma unroll
+  for (int q = 0; q < 3; ++q) moved |= p0.b[q] != f.b[q] || p0.o[q] != f.o[q];
+  if (!moved) {
+    auto a0 = cvel[c], a1 = cvel[c + sa];
+    v0[0] = mixv(a0.x, a1.x, T(0.5));
+    v0[1] = mixv(a0.y, a1.y, T(0.5));
+    v0[2] = mixv(a0.z, a1.z, T(0.5));
+  } else {
+    sample_vec_tp<T>(cvel, d, p0, v0);
+  }
   TPos<T> m = f;
   for (int a = 0; a < 3; ++a) m.o[a] -= T(0.5) * (v0[a] * dth);
   tpos_clamp(m, lo, hi);
@@ -111,7 +125,8 @@ k_advect_mac(Dims d, T dth, T damp, const T *__restrict__ u, const T *__restrict
     f.o[0] = f.o[1] = f.o[2] = T(0);
     f.o[a] = T(0.5);
     TPos<T> b;
-    trace_back_tp<T>(d, cvel, dth, f, lo2, hi2, b);
+    const size_t sa = a == 0 ? 1 : (a == 1 ? (size_t)d.nx : (size_t)d.nx * d.ny);
+    trace_back_tp<T>(d, cvel, dth, f, lo2, hi2, c, sa, b);
     b.o[a] -= T(0.5);  // the -1/2 texel shift of sampleMacU/V/W
     tpos_clamp(b, lo1, hi1);
     res[a] = sample_tp(fld[a], d, b) * damp;
@@ -265,7 +280,14 @@ __device__ inline void dye_back_tp(const Dims &d, const typename vec4_of<T>::typ
   TPos<T> c;
   c.b[0] = i; c.b[1] = j; c.b[2] = k;
   c.o[0] = c.o[1] = c.o[2] = T(0);
-  sample_vec_tp<T>(cvel, d, c, v0);
+  // the first sample sits on the texel centre (weights 0: mixv(x, y, 0) = x
+  // exactly), so it is the texel's own value
+  {
+    auto a0 = cvel[idx3(d, i, j, k)];
+    v0[0] = a0.x;
+    v0[1] = a0.y;
+    v0[2] = a0.z;
+  }
   TPos<T> m = c;
   for (int a = 0; a < 3; ++a) m.o[a] -= T(0.5) * (v0[a] * dth);
   tpos_clamp(m, lo, hi);
