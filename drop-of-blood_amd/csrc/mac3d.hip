@@ -276,10 +276,15 @@ int launch_casson(Ctx *ctx, double dt) {
     ProfScope ps(ctx->prof, ctx->stream, KC_CASSON);
     with_step_type(ctx->t0, [&](auto tag) {
       using T = std::remove_pointer_t<decltype(tag)>;
-      hipLaunchKernelGGL(k_casson<T>, cell_grid(d), kCellBlock, 0, ctx->stream, d, P,
+      // z-marching kernel: 64 x 8 tiles, about three 512-thread blocks per CU
+      int ncx = (d.nx + CS_TX - 1) / CS_TX, ncy = (d.ny + CS_TY - 1) / CS_TY;
+      int chunks = std::max(1, 768 / (ncx * ncy));
+      int cz = (d.nz + chunks - 1) / chunks;
+      dim3 grid(ncx * ncy * ((d.nz + cz - 1) / cz));
+      hipLaunchKernelGGL(k_casson_zm<T>, grid, dim3(CS_TX, CS_TY), 0, ctx->stream, d, P,
                          (const T *)ctx->u.ptr, (const T *)ctx->v.ptr, (const T *)ctx->w.ptr,
                          (const T *)ctx->dye.ptr, (T *)ctx->u2.ptr, (T *)ctx->v2.ptr,
-                         (T *)ctx->w2.ptr);
+                         (T *)ctx->w2.ptr, cz, ncx, ncy);
     });
   }
   LAUNCHCHK();

@@ -158,47 +158,59 @@ k_residual_restrict_zm(Dims df, Dims dc, const Sf *__restrict__ x, const Sf *__r
   const size_t plane = (size_t)nx * ny;
   const int gx = x0 + tx, gy = y0 + ty;
   const bool inside = gx < nx && gy < ny;
-  long st_g[RR_LPT];
-  int st_r[RR_LPT], st_c[RR_LPT];
+  int st_o[RR_LPT], st_r[RR_LPT], st_c[RR_LPT];
 #pragma unroll
   for (int q = 0; q < RR_LPT; ++q) {
     int v = tid + q * RR_T;
     st_r[q] = v / RR_EX;
     st_c[q] = v % RR_EX;
     int sxg = x0 - 1 + st_c[q], syg = y0 - 1 + st_r[q];
-    st_g[q] = (v < RR_PL && sxg >= 0 && sxg < nx && syg >= 0 && syg < ny) ? (long)syg * nx + sxg : -1;
+    st_o[q] = (v < RR_PL && sxg >= 0 && sxg < nx && syg >= 0 && syg < ny)
+                  ? (syg * nx + sxg) * (int)sizeof(Sf) : MAC3D_OFF_NONE;
   }
-  C pre[RR_LPT];
+  // buffer ranges: x planes k0-1 .. k1, b planes k0 .. k1-1; planes outside
+  // (the z walls) and cells outside the grid load as 0
+  const int pb = (int)(plane * sizeof(Sf));
+  const int kx0 = max(k0 - 1, 0), kx1 = min(k1 + 1, nz);
+  const BufRsrc rx = buf_rsrc(x + (size_t)kx0 * plane, (uint32_t)(kx1 - kx0) * pb);
+  const BufRsrc rb = buf_rsrc(b + (size_t)k0 * plane, (uint32_t)(k1 - k0) * pb);
+  const int bo = inside ? (gy * nx + gx) * (int)sizeof(Sf) : MAC3D_OFF_NONE;
+  Sf pre[RR_LPT];
   auto fetch = [&](int k) {
 #pragma unroll
-    for (int q = 0; q < RR_LPT; ++q)
-      pre[q] = (st_g[q] >= 0 && k >= 0 && k < nz) ? (C)x[k * plane + st_g[q]] : C(0);
+    for (int q = 0; q < RR_LPT; ++q) pre[q] = buf_load<Sf>(rx, st_o[q] + (k - kx0) * pb);
   };
   auto stage = [&](int k) {
 #pragma unroll
     for (int q = 0; q < RR_LPT; ++q)
-      if (tid + q * RR_T < RR_PL) sx[(k + 3) % 3][st_r[q]][st_c[q]] = pre[q];
+      if (tid + q * RR_T < RR_PL) sx[(k + 3) % 3][st_r[q]][st_c[q]] = (C)pre[q];
   };
+  // diagonal of this column (the z terms are added per plane)
+  const C dgxy = ((gy < ny - 1) ? C(1) : topc) + C((gx > 0) + (gx < nx - 1) + (gy > 0));
   for (int k = k0 - 1; k <= k0; ++k) {
     fetch(k);
     stage(k);
   }
   fetch(k0 + 1);
+  Sf bnext = buf_load<Sf>(rb, bo);
   C rsum = C(0);
   for (int k = k0; k < k1; ++k) {
     stage(k + 1);
+    C bk = (C)bnext;
     __syncthreads();
     fetch(k + 2);
+    bnext = buf_load<Sf>(rb, bo + (k + 1 - k0) * pb);
     if (inside) {
+      // neighbours outside the grid are staged as 0: the sum runs over all six
+      // in the stencil's order
       const int r = ty + 1, c = tx + 1, s0 = k % 3;
-      C sn = C(0), dg = (gy < ny - 1) ? C(1) : topc;
-      if (gx > 0) { sn += sx[s0][r][c - 1]; dg += C(1); }
-      if (gx < nx - 1) { sn += sx[s0][r][c + 1]; dg += C(1); }
-      if (gy > 0) { sn += sx[s0][r - 1][c]; dg += C(1); }
-      if (gy < ny - 1) { sn += sx[s0][r + 1][c]; }
-      if (k > 0) { sn += sx[(k + 2) % 3][r][c]; dg += C(1); }
-      if (k < nz - 1) { sn += sx[(k + 1) % 3][r][c]; dg += C(1); }
-      rsum += (C)b[k * plane + (size_t)gy * nx + gx] * bscale - (sn - dg * sx[s0][r][c]) / h2;
+      C sn = sx[s0][r][c - 1] + sx[s0][r][c + 1];
+      sn += sx[s0][r - 1][c];
+      sn += sx[s0][r + 1][c];
+      sn += sx[(k + 2) % 3][r][c];
+      sn += sx[(k + 1) % 3][r][c];
+      C dg = dgxy + C((k > 0) + (k < nz - 1));
+      rsum += bk * bscale - (sn - dg * sx[s0][r][c]) / h2;
     }
     if (k & 1) {
       C pair = rsum + __shfl_xor(rsum, 1, 64);

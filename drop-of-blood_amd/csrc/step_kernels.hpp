@@ -168,6 +168,44 @@ __device__ inline T face_w(const Dims &d, const T *__restrict__ w, int i, int j,
   return w[idx3(d, i, j, k)];
 }
 
+// One cell of the Casson update and buoyancy (fluid.ts:451-486 with the
+// workload's SI Casson law): m the cell's three faces, x/y/z p/n the face
+// values one cell up/down each axis (wall ghosts applied), dc the cell's
+// dye, dup the dye one row up (the cell's own at the top row).
+template <class T>
+__device__ inline void casson_cell(const StepParams &P, T m0, T m1, T m2, T xp0, T xp1, T xp2,
+                                   T xn0, T xn1, T xn2, T yp0, T yp1, T yp2, T yn0, T yn1, T yn2,
+                                   T zp0, T zp1, T zp2, T zn0, T zn1, T zn2, T dc, T dup, T &o0,
+                                   T &o1, T &o2) {
+  const T inv2h = T(1.0 / (2.0 * P.dx));
+  const T dtoh2 = T(P.dt / (P.dx * P.dx));
+  const T buoy = T(-P.g * (P.rho_b - P.rho_w) / P.rho_w * P.dt);
+  T ss = T(0);
+  {
+    T a = xp0 - xn0, b = yp0 - yn0, e = zp0 - zn0;
+    ss += a * a + b * b + e * e;
+    a = xp1 - xn1; b = yp1 - yn1; e = zp1 - zn1;
+    ss += a * a + b * b + e * e;
+    a = xp2 - xn2; b = yp2 - yn2; e = zp2 - zn2;
+    ss += a * a + b * b + e * e;
+  }
+  T gam = inv2h * sqrt(ss + T(P.shear_eps2));
+  T cc = clampv(dc, T(0), T(1));
+  T sq_muinf = sqrt(T(P.mu_w) + cc * T(P.mu_inf - P.mu_w));
+  T tau = T(P.tau_y) * cc * cc;
+  T reg = T(P.gamma_reg);
+  T gr = gam + reg;
+  T yterm = sqrt(tau / (gr > reg ? gr : reg));
+  T mu_eff = (sq_muinf + yterm) * (sq_muinf + yterm);
+  T rho = T(P.rho_w) + cc * T(P.rho_b - P.rho_w);
+  T alpha = clampv(mu_eff / rho * dtoh2, T(0), T(P.alpha_max));
+  o0 = m0 + alpha * ((xp0 + xn0 + yp0 + yn0 + zp0 + zn0) - T(6) * m0);
+  o1 = m1 + alpha * ((xp1 + xn1 + yp1 + yn1 + zp1 + zn1) - T(6) * m1);
+  o2 = m2 + alpha * ((xp2 + xn2 + yp2 + yn2 + zp2 + zn2) - T(6) * m2);
+  T dyef = T(0.5) * (dc + dup);
+  o1 += buoy * dyef;
+}
+
 // FLUID_CASSON_FORCES (fluid.ts:377-511) parts (1) and (3): Casson
 // shear-thinning explicit viscous update and Boussinesq buoyancy.
 //   gamma  = |+-1 neighbour differences| / (2h)          (fluid.ts:451-454)
@@ -183,9 +221,6 @@ k_casson(Dims d, StepParams P, const T *__restrict__ u, const T *__restrict__ v,
          const T *__restrict__ w, const T *__restrict__ dye, T *__restrict__ uo,
          T *__restrict__ vo, T *__restrict__ wo) {
   MAC3D_CELL_PROLOGUE
-  const T inv2h = T(1.0 / (2.0 * P.dx));
-  const T dtoh2 = T(P.dt / (P.dx * P.dx));
-  const T buoy = T(-P.g * (P.rho_b - P.rho_w) / P.rho_w * P.dt);
   T m0 = u[c], m1 = v[c], m2 = w[c];
   T xp0 = face_u(d, u, i + 1, j, k), xp1 = face_v(d, v, i + 1, j, k), xp2 = face_w(d, w, i + 1, j, k);
   T xn0 = face_u(d, u, i - 1, j, k), xn1 = face_v(d, v, i - 1, j, k), xn2 = face_w(d, w, i - 1, j, k);
@@ -193,33 +228,126 @@ k_casson(Dims d, StepParams P, const T *__restrict__ u, const T *__restrict__ v,
   T yn0 = face_u(d, u, i, j - 1, k), yn1 = face_v(d, v, i, j - 1, k), yn2 = face_w(d, w, i, j - 1, k);
   T zp0 = face_u(d, u, i, j, k + 1), zp1 = face_v(d, v, i, j, k + 1), zp2 = face_w(d, w, i, j, k + 1);
   T zn0 = face_u(d, u, i, j, k - 1), zn1 = face_v(d, v, i, j, k - 1), zn2 = face_w(d, w, i, j, k - 1);
-  T ss = T(0);
-  {
-    T a = xp0 - xn0, b = yp0 - yn0, e = zp0 - zn0;
-    ss += a * a + b * b + e * e;
-    a = xp1 - xn1; b = yp1 - yn1; e = zp1 - zn1;
-    ss += a * a + b * b + e * e;
-    a = xp2 - xn2; b = yp2 - yn2; e = zp2 - zn2;
-    ss += a * a + b * b + e * e;
-  }
-  T gam = inv2h * sqrt(ss + T(P.shear_eps2));
-  T cc = clampv(dye[c], T(0), T(1));
-  T sq_muinf = sqrt(T(P.mu_w) + cc * T(P.mu_inf - P.mu_w));
-  T tau = T(P.tau_y) * cc * cc;
-  T reg = T(P.gamma_reg);
-  T gr = gam + reg;
-  T yterm = sqrt(tau / (gr > reg ? gr : reg));
-  T mu_eff = (sq_muinf + yterm) * (sq_muinf + yterm);
-  T rho = T(P.rho_w) + cc * T(P.rho_b - P.rho_w);
-  T alpha = clampv(mu_eff / rho * dtoh2, T(0), T(P.alpha_max));
-  T o0 = m0 + alpha * ((xp0 + xn0 + yp0 + yn0 + zp0 + zn0) - T(6) * m0);
-  T o1 = m1 + alpha * ((xp1 + xn1 + yp1 + yn1 + zp1 + zn1) - T(6) * m1);
-  T o2 = m2 + alpha * ((xp2 + xn2 + yp2 + yn2 + zp2 + zn2) - T(6) * m2);
-  T dyef = T(0.5) * (dye[c] + dye[(j + 1 < d.ny) ? c + sy : c]);
-  o1 += buoy * dyef;
+  T o0, o1, o2;
+  casson_cell<T>(P, m0, m1, m2, xp0, xp1, xp2, xn0, xn1, xn2, yp0, yp1, yp2, yn0, yn1, yn2, zp0, zp1,
+                 zp2, zn0, zn1, zn2, dye[c], dye[(j + 1 < d.ny) ? c + sy : c], o0, o1, o2);
   uo[c] = (i >= d.nx - 1) ? T(0) : o0;
   vo[c] = o1;
   wo[c] = (k >= d.nz - 1) ? T(0) : o2;
+}
+
+// The same Casson update marching in z: a 512-thread block owns a 64 x 8
+// column tile and a z-chunk [k0, k1).  Planes of u, v, w with a one-cell
+// ring are staged in a four-slot LDS ring (one barrier per plane) with the
+// wall ghosts of face_u/v/w already applied: a ring cell outside the grid
+// holds its mirror cell's value times the ghost sign (0 for the wall-normal
+// face, -1 for the no-slip tangential faces, +1 at the free-slip lid), and
+// the planes below 0 and above nz - 1 hold plane 0 / nz - 1 times (-1, -1,
+// 0).  So every load is in the grid and the stencil needs no branches; each
+// plane is read about 1.3 times instead of about 3.  Same cell arithmetic as
+// k_casson (casson_cell).
+#define CS_TX 64
+#define CS_TY 8
+#define CS_T (CS_TX * CS_TY)
+#define CS_EX (CS_TX + 2)
+#define CS_EY (CS_TY + 2)
+#define CS_PL (CS_EX * CS_EY)
+#define CS_LPT ((CS_PL + CS_T - 1) / CS_T)
+
+template <class T>
+__global__ void __launch_bounds__(CS_T)
+k_casson_zm(Dims d, StepParams P, const T *__restrict__ u, const T *__restrict__ v,
+            const T *__restrict__ w, const T *__restrict__ dye, T *__restrict__ uo,
+            T *__restrict__ vo, T *__restrict__ wo, int cz, int ncx, int ncy) {
+  __shared__ T su[4][CS_EY][CS_EX], sv[4][CS_EY][CS_EX], sw[4][CS_EY][CS_EX];
+  const int nx = d.nx, ny = d.ny, nz = d.nz;
+  const int lb = blockIdx.x;
+  const int x0 = (lb % ncx) * CS_TX, y0 = ((lb / ncx) % ncy) * CS_TY;
+  const int k0 = (lb / (ncx * ncy)) * cz, k1 = min(k0 + cz, nz);
+  const int tx = threadIdx.x, ty = threadIdx.y, tid = ty * CS_TX + tx;
+  const size_t plane = (size_t)nx * ny;
+  // staged positions: mirrored in-plane index and ghost signs per field
+  int so[CS_LPT], sr[CS_LPT], sc[CS_LPT];
+  T gu[CS_LPT], gv[CS_LPT], gw[CS_LPT];
+#pragma unroll
+  for (int q = 0; q < CS_LPT; ++q) {
+    int vq = tid + q * CS_T;
+    bool act = vq < CS_PL;
+    if (!act) vq = 0;
+    sr[q] = vq / CS_EX;
+    sc[q] = vq % CS_EX;
+    int gx = x0 - 1 + sc[q], gy = y0 - 1 + sr[q];
+    T a = T(1), b = T(1), e = T(1);
+    if (gx < 0 || gx >= nx) { a = T(0); b = T(-1); e = T(-1); }
+    else if (gy < 0) { a = T(-1); b = T(0); e = T(-1); }
+    int mx = gx < 0 ? 0 : (gx >= nx ? nx - 1 : gx), my = gy < 0 ? 0 : (gy >= ny ? ny - 1 : gy);
+    so[q] = my * nx + mx;
+    gu[q] = a;
+    gv[q] = b;
+    gw[q] = e;
+    if (!act) so[q] = -1;
+  }
+  T pu[CS_LPT], pv[CS_LPT], pw[CS_LPT];
+  auto fetch = [&](int k) {
+    int kc = k < 0 ? 0 : (k >= nz ? nz - 1 : k);
+    const size_t base = (size_t)kc * plane;
+#pragma unroll
+    for (int q = 0; q < CS_LPT; ++q) {
+      size_t o = base + (so[q] < 0 ? 0 : so[q]);
+      pu[q] = u[o];
+      pv[q] = v[o];
+      pw[q] = w[o];
+    }
+  };
+  auto stage = [&](int k) {
+    // ghost planes below and above the grid: (-u, -v, 0) of the edge plane
+    const bool zg = k < 0 || k >= nz;
+    const T zu = zg ? T(-1) : T(1), zw = zg ? T(0) : T(1);
+    const int sl = k & 3;
+#pragma unroll
+    for (int q = 0; q < CS_LPT; ++q)
+      if (so[q] >= 0) {
+        su[sl][sr[q]][sc[q]] = gu[q] * zu * pu[q];
+        sv[sl][sr[q]][sc[q]] = gv[q] * zu * pv[q];
+        sw[sl][sr[q]][sc[q]] = gw[q] * zw * pw[q];
+      }
+  };
+  const int gx = x0 + tx, gy = y0 + ty;
+  const bool inside = gx < nx && gy < ny;
+  const size_t cc0 = inside ? (size_t)gy * nx + gx : 0;
+  const size_t up = (gy + 1 < ny) ? (size_t)nx : 0;
+  fetch(k0 - 1);
+  stage(k0 - 1);
+  fetch(k0);
+  stage(k0);
+  fetch(k0 + 1);
+  T dn = dye[(size_t)k0 * plane + cc0], dnu = dye[(size_t)k0 * plane + cc0 + up];
+  const int r = ty + 1, c = tx + 1;
+  for (int k = k0; k < k1; ++k) {
+    stage(k + 1);
+    T dc = dn, dup = dnu;
+    __syncthreads();
+    fetch(k + 2);
+    if (k + 1 < k1) {
+      dn = dye[(size_t)(k + 1) * plane + cc0];
+      dnu = dye[(size_t)(k + 1) * plane + cc0 + up];
+    }
+    if (inside) {
+      const int s0 = k & 3, sm = (k - 1) & 3, sp = (k + 1) & 3;
+      T o0, o1, o2;
+      casson_cell<T>(P, su[s0][r][c], sv[s0][r][c], sw[s0][r][c],
+                     su[s0][r][c + 1], sv[s0][r][c + 1], sw[s0][r][c + 1],
+                     su[s0][r][c - 1], sv[s0][r][c - 1], sw[s0][r][c - 1],
+                     su[s0][r + 1][c], sv[s0][r + 1][c], sw[s0][r + 1][c],
+                     su[s0][r - 1][c], sv[s0][r - 1][c], sw[s0][r - 1][c],
+                     su[sp][r][c], sv[sp][r][c], sw[sp][r][c],
+                     su[sm][r][c], sv[sm][r][c], sw[sm][r][c], dc, dup, o0, o1, o2);
+      const size_t ci = (size_t)k * plane + cc0;
+      uo[ci] = (gx >= nx - 1) ? T(0) : o0;
+      vo[ci] = o1;
+      wo[ci] = (k >= nz - 1) ? T(0) : o2;
+    }
+  }
 }
 
 // FLUID_MAC_RHS (fluid.ts:517-555) with a flat lid (p_top = 0):
