@@ -61,6 +61,9 @@ struct Ctx {
   size_t n[kMaxLev + 1];
   hipStream_t stream = nullptr;
   hipEvent_t norms_ready = nullptr;  // mixed mode: the residual norms reached the host
+  // cvel holds the centre velocity of the current u, v, w (set by a step's
+  // last face-to-centre pass, cleared by anything else that may change them)
+  bool cvel_fresh = false;
   std::string err;
   // level-0 step fields
   Buf u, v, w, u2, v2, w2, p, dye, dye2, dpred, cvel, rhs;  // cvel: 4 per cell
@@ -145,6 +148,11 @@ int fail(Ctx *ctx, int code, const std::string &msg) {
 
 dim3 cell_grid(const Dims &d) { return dim3((d.nx + 63) / 64, (d.ny + 3) / 4, d.nz); }
 const dim3 kCellBlock(64, 4, 1);
+// the semi-Lagrangian gather kernels run 32 x 4 blocks: their 8-texel
+// samples reach one row and plane either side, and the narrower block keeps
+// more of them in one CU's L1 (measured fastest of 64x4, 32x8, 16x16, 32x4)
+dim3 gather_grid(const Dims &d) { return dim3((d.nx + 31) / 32, (d.ny + 3) / 4, d.nz); }
+const dim3 kGatherBlock(32, 4, 1);
 
 double top_coeff(int level) {
   double s = (double)(1 << level);
@@ -257,7 +265,7 @@ int launch_advect(Ctx *ctx, double dt) {
     ProfScope ps(ctx->prof, ctx->stream, KC_ADVECT);
     with_step_type(ctx->t0, [&](auto tag) {
       using T = std::remove_pointer_t<decltype(tag)>;
-      hipLaunchKernelGGL(k_advect_mac<T>, cell_grid(d), kCellBlock, 0, ctx->stream, d,
+      hipLaunchKernelGGL(k_advect_mac<T>, gather_grid(d), kGatherBlock, 0, ctx->stream, d,
                          (T)(dt / ctx->cfg.dx), (T)ctx->cfg.vel_damp, (const T *)ctx->u.ptr, (const T *)ctx->v.ptr,
                          (const T *)ctx->w.ptr, (const typename vec4_of<T>::type *)ctx->cvel.ptr,
                          (T *)ctx->u2.ptr, (T *)ctx->v2.ptr,
@@ -329,7 +337,7 @@ int launch_dye_predict(Ctx *ctx, double dt) {
   ProfScope ps(ctx->prof, ctx->stream, KC_DYE);
   with_step_type(ctx->t0, [&](auto tag) {
     using T = std::remove_pointer_t<decltype(tag)>;
-    hipLaunchKernelGGL(k_dye_predict<T>, cell_grid(d), kCellBlock, 0, ctx->stream, d,
+    hipLaunchKernelGGL(k_dye_predict<T>, gather_grid(d), kGatherBlock, 0, ctx->stream, d,
                        (T)(dt / ctx->cfg.dx),
                        (const typename vec4_of<T>::type *)ctx->cvel.ptr,
                        (const T *)ctx->dye.ptr, (T *)ctx->dpred.ptr);
@@ -344,7 +352,7 @@ int launch_dye_resolve(Ctx *ctx, double dt) {
     ProfScope ps(ctx->prof, ctx->stream, KC_DYE);
     with_step_type(ctx->t0, [&](auto tag) {
       using T = std::remove_pointer_t<decltype(tag)>;
-      hipLaunchKernelGGL(k_dye_resolve<T>, cell_grid(d), kCellBlock, 0, ctx->stream, d,
+      hipLaunchKernelGGL(k_dye_resolve<T>, gather_grid(d), kGatherBlock, 0, ctx->stream, d,
                          (T)(dt / ctx->cfg.dx), (T)ctx->cfg.maccormack, (const typename vec4_of<T>::type *)ctx->cvel.ptr,
                          (const T *)ctx->dye.ptr,
                          (const T *)ctx->dpred.ptr, (T *)ctx->dye2.ptr);
@@ -801,7 +809,8 @@ enum { SEQ_STEP = 1, SEQ_PRE = 2, SEQ_ITER = 3, SEQ_POST = 4 };
 
 int seq_pre(Ctx *ctx, double dt) {
   int rc;
-  if ((rc = launch_face_to_centre(ctx))) return rc;
+  // the previous step's last pass already formed cvel from this velocity
+  if (!ctx->cvel_fresh && (rc = launch_face_to_centre(ctx))) return rc;
   if ((rc = launch_advect(ctx, dt))) return rc;
   if ((rc = launch_casson(ctx, dt))) return rc;
   return launch_divergence(ctx);
@@ -829,8 +838,9 @@ int one_step(Ctx *ctx, double dt, int *cycles, int *iters, double *rel, int *con
       for (int c = 0; c < ctx->cfg.cycles; ++c)
         if ((r = vcycle(ctx, false))) return r;
       return seq_post(ctx, dt);
-    });
-    if (rc) return rc;
+    }, ctx->cvel_fresh ? 1 : 0);  // (graphs replay without running the lambda: the
+    if (rc) return rc;              // host sets the flag after each successful run)
+    ctx->cvel_fresh = true;
     *cycles += ctx->cfg.cycles;
     *iters = ctx->cfg.cycles;
     *rel = -1.0;
@@ -842,7 +852,7 @@ int one_step(Ctx *ctx, double dt, int *cycles, int *iters, double *rel, int *con
     int r;
     if ((r = seq_pre(ctx, dt))) return r;
     return mixed_begin(ctx, order);
-  }, order);
+  }, order | (ctx->cvel_fresh ? 4 : 0));
   if (rc) return rc;
   int it = 0;
   *conv = 0;
@@ -866,7 +876,9 @@ int one_step(Ctx *ctx, double dt, int *cycles, int *iters, double *rel, int *con
   *cycles += it;
   *iters = it;
   ctx->hist = std::min(ctx->hist + 1, 2);
-  return run_seq(ctx, SEQ_POST, dt, [&]() { return seq_post(ctx, dt); });
+  rc = run_seq(ctx, SEQ_POST, dt, [&]() { return seq_post(ctx, dt); });
+  if (!rc) ctx->cvel_fresh = true;
+  return rc;
 }
 
 Buf *field_buf(Ctx *ctx, int field, int level) {
@@ -1025,6 +1037,7 @@ int reset_scale(Ctx *ctx) {
 }
 
 int init_blobs_impl(Ctx *ctx, const mac3d_blob *blobs, int nblobs) {
+  ctx->cvel_fresh = false;
   ctx->hist = 0;
   if (nblobs < 0 || (nblobs > 0 && !blobs)) return fail(ctx, MAC3D_EINVAL, "bad blob list");
   for (int q = 0; q < nblobs; ++q) {
@@ -1123,6 +1136,7 @@ int save_state(Ctx *ctx, void *const *dst) {
 }
 
 int load_state(Ctx *ctx, void *const *src) {
+  ctx->cvel_fresh = false;
   ctx->hist = 0;
   for (int f = 0; f < 5; ++f)
     HIPCHK(hipMemcpyAsync(state_field(ctx, f), src[f], state_bytes(ctx, f),
@@ -1435,6 +1449,7 @@ int mac3d_destroy(void *h) {
 int mac3d_upload(void *h, int field, int level, const void *host, int dtype, size_t count) {
   if (!h) return MAC3D_EINVAL;
   Ctx *ctx = (Ctx *)h;
+  ctx->cvel_fresh = false;
   Buf *b = field_buf(ctx, field, level);
   if (!b || !host) return fail(ctx, MAC3D_EINVAL, "unknown field/level or null host pointer");
   if (dtype != MAC3D_F32 && dtype != MAC3D_F64) return fail(ctx, MAC3D_EINVAL, "unknown dtype");
@@ -1574,6 +1589,7 @@ int mac3d_adjoint(void *h, const mac3d_blob *blobs, int nblobs, double dt, int n
     return fail(ctx, MAC3D_EINVAL, "nsteps and checkpoint_every must be >= 1");
   if (dtype != MAC3D_F32 && dtype != MAC3D_F64) return fail(ctx, MAC3D_EINVAL, "unknown dtype");
   int rc = adjoint_impl(ctx, blobs, nblobs, dt, nsteps, checkpoint_every, target, dtype, out);
+  ctx->cvel_fresh = false;  // the reverse sweep reuses the work fields
   ctx->prof.collect();
   return rc;
 }
@@ -1581,6 +1597,7 @@ int mac3d_adjoint(void *h, const mac3d_blob *blobs, int nblobs, double dt, int n
 int mac3d_stage(void *h, int stage, double dt) {
   if (!h) return MAC3D_EINVAL;
   Ctx *ctx = (Ctx *)h;
+  ctx->cvel_fresh = false;
   if (!std::isfinite(dt) || dt < 0.0) return fail(ctx, MAC3D_EINVAL, "dt must be finite and >= 0");
   int rc;
   switch (stage) {
@@ -1603,6 +1620,7 @@ int mac3d_stage(void *h, int stage, double dt) {
 int mac3d_mg_op(void *h, int op, int level, int count) {
   if (!h) return MAC3D_EINVAL;
   Ctx *ctx = (Ctx *)h;
+  ctx->cvel_fresh = false;
   if (level < 0 || level > ctx->L) return fail(ctx, MAC3D_EINVAL, "level out of range");
   if (count < 0) return fail(ctx, MAC3D_EINVAL, "count must be >= 0");
   int rc;
