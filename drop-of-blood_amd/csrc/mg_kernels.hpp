@@ -68,34 +68,10 @@ k_restrict(Dims df, Dims dc, const Sf *__restrict__ rf, Sc *__restrict__ bc) {
   bc[idx3(dc, i, j, k)] = store_as<Sc, C>(s * C(0.125));
 }
 
-// MG_RESIDUAL followed by MG_RESTRICT (fluid.ts:744-800) in one pass: each
-// coarse cell sums the residuals of its 2x2x2 children, so the fine residual
-// never goes to memory.  Same arithmetic as the two kernels.
-template <class Sf, class Sc>
-__global__ void __launch_bounds__(256)
-k_residual_restrict(Dims df, Dims dc, const Sf *__restrict__ x, const Sf *__restrict__ b,
-                    Sc *__restrict__ bc, typename compute_of<Sf>::type h2,
-                    typename compute_of<Sf>::type topc, const double *__restrict__ scale) {
-  typedef typename compute_of<Sf>::type C;
-  const C bscale = scale ? (C)scale[0] : C(1);
-  int i = blockIdx.x * blockDim.x + threadIdx.x;
-  int j = blockIdx.y * blockDim.y + threadIdx.y;
-  int k = blockIdx.z;
-  if (i >= dc.nx || j >= dc.ny) return;
-  C s = C(0);
-#pragma unroll
-  for (int q = 0; q < 8; ++q) {
-    int fi = 2 * i + (q & 1), fj = 2 * j + ((q >> 1) & 1), fk = 2 * k + (q >> 2);
-    size_t c = idx3(df, fi, fj, fk);
-    C sn, dg;
-    mg_stencil<Sf, C>(x, df, topc, fi, fj, fk, c, sn, dg);
-    s += (C)b[c] * bscale - (sn - dg * (C)x[c]) / h2;
-  }
-  bc[idx3(dc, i, j, k)] = store_as<Sc, C>(s * C(0.125));
-}
-
-// The same residual-restriction with one thread per fine cell (coalesced
-// 7-point loads): a 64 x 2 x 2 block of fine cells sums to 32 coarse cells,
+// MG_RESIDUAL followed by MG_RESTRICT (fluid.ts:744-800) in one pass, so the
+// fine residual never goes to memory (same arithmetic as the two kernels),
+// one thread per fine cell (coalesced 7-point loads): a 64 x 2 x 2 block of
+// fine cells sums to 32 coarse cells,
 // pairs in x by a lane shuffle, the 2 x 2 (y, z) by LDS.
 template <class Sf, class Sc>
 __global__ void __launch_bounds__(256)
@@ -225,7 +201,22 @@ k_residual_restrict_zm(Dims df, Dims dc, const Sf *__restrict__ x, const Sf *__r
   }
 }
 
-// MG_PROLONG_ADD with one thread per 2 x 2 x 2 block of fine cells (the
+// Coarse correction value for MG_PROLONG_ADD (fluid.ts:806-855): coarse
+// indices clamped at the walls, 0 above the top row.
+template <class Sc>
+__device__ inline typename compute_of<Sc>::type
+coarse_at(const Sc *__restrict__ e, const Dims &dc, int i, int j, int k) {
+  typedef typename compute_of<Sc>::type C;
+  if (j >= dc.ny) return C(0);
+  i = clampi(i, 0, dc.nx - 1);
+  j = clampi(j, 0, dc.ny - 1);
+  k = clampi(k, 0, dc.nz - 1);
+  return (C)e[idx3(dc, i, j, k)];
+}
+
+// MG_PROLONG_ADD (fluid.ts:806-855): x_f += cell-centred trilinear
+// interpolation of the coarse correction.  Fine cell i sits at coarse
+// coordinate i/2 - 1/4.  One thread per 2 x 2 x 2 block of fine cells (the
 // children of one coarse cell), which share a 3 x 3 x 3 coarse
 // neighbourhood: child offset 0 along an axis takes 3/4 of the parent and
 // 1/4 of the parent's lower neighbour, offset 1 takes 3/4 and 1/4 of the
@@ -268,48 +259,6 @@ k_prolong_add_2x(Dims df, Dims dc, const Sc *__restrict__ ec, Sf *__restrict__ x
         xf[base + ox] = store_as<Sf, C>((C)xf[base + ox] + corr);
       }
     }
-}
-
-// MG_PROLONG_ADD (fluid.ts:806-855): x_f += cell-centred trilinear
-// interpolation of the coarse correction; coarse indices clamped at the
-// walls, 0 above the top row.  Fine cell i sits at coarse coordinate
-// i/2 - 1/4, so the weights are 3/4 (own parent) and 1/4 (next parent
-// towards the fine cell).
-template <class Sc>
-__device__ inline typename compute_of<Sc>::type
-coarse_at(const Sc *__restrict__ e, const Dims &dc, int i, int j, int k) {
-  typedef typename compute_of<Sc>::type C;
-  if (j >= dc.ny) return C(0);
-  i = clampi(i, 0, dc.nx - 1);
-  j = clampi(j, 0, dc.ny - 1);
-  k = clampi(k, 0, dc.nz - 1);
-  return (C)e[idx3(dc, i, j, k)];
-}
-
-template <class Sc, class Sf>
-__global__ void __launch_bounds__(256)
-k_prolong_add(Dims df, Dims dc, const Sc *__restrict__ ec, Sf *__restrict__ xf) {
-  typedef typename compute_of<Sf>::type C;
-  int i = blockIdx.x * blockDim.x + threadIdx.x;
-  int j = blockIdx.y * blockDim.y + threadIdx.y;
-  int k = blockIdx.z;
-  if (i >= df.nx || j >= df.ny) return;
-  // base = floor(i/2 - 1/4), frac = 3/4 for even i, 1/4 for odd i
-  int bi = (i & 1) ? (i >> 1) : (i >> 1) - 1;
-  int bj = (j & 1) ? (j >> 1) : (j >> 1) - 1;
-  int bk = (k & 1) ? (k >> 1) : (k >> 1) - 1;
-  C fx = (i & 1) ? C(0.25) : C(0.75);
-  C fy = (j & 1) ? C(0.25) : C(0.75);
-  C fz = (k & 1) ? C(0.25) : C(0.75);
-  C c000 = (C)coarse_at(ec, dc, bi, bj, bk), c100 = (C)coarse_at(ec, dc, bi + 1, bj, bk);
-  C c010 = (C)coarse_at(ec, dc, bi, bj + 1, bk), c110 = (C)coarse_at(ec, dc, bi + 1, bj + 1, bk);
-  C c001 = (C)coarse_at(ec, dc, bi, bj, bk + 1), c101 = (C)coarse_at(ec, dc, bi + 1, bj, bk + 1);
-  C c011 = (C)coarse_at(ec, dc, bi, bj + 1, bk + 1), c111 = (C)coarse_at(ec, dc, bi + 1, bj + 1, bk + 1);
-  C x00 = mixv(c000, c100, fx), x10 = mixv(c010, c110, fx);
-  C x01 = mixv(c001, c101, fx), x11 = mixv(c011, c111, fx);
-  C corr = mixv(mixv(x00, x10, fy), mixv(x01, x11, fy), fz);
-  size_t c = idx3(df, i, j, k);
-  xf[c] = store_as<Sf, C>((C)xf[c] + corr);
 }
 
 // ---------------------------------------------------------------------------

@@ -215,28 +215,8 @@ __device__ inline void casson_cell(const StepParams &P, T m0, T m1, T m2, T xp0,
 //                                                        (fluid.ts:464-465)
 //   v     += -g (rho_b - rho_w)/rho_w * dye_face * dt    (fluid.ts:477-486)
 // with SI Casson parameters blended by the dye fraction c.
-template <class T>
-__global__ void __launch_bounds__(256)
-k_casson(Dims d, StepParams P, const T *__restrict__ u, const T *__restrict__ v,
-         const T *__restrict__ w, const T *__restrict__ dye, T *__restrict__ uo,
-         T *__restrict__ vo, T *__restrict__ wo) {
-  MAC3D_CELL_PROLOGUE
-  T m0 = u[c], m1 = v[c], m2 = w[c];
-  T xp0 = face_u(d, u, i + 1, j, k), xp1 = face_v(d, v, i + 1, j, k), xp2 = face_w(d, w, i + 1, j, k);
-  T xn0 = face_u(d, u, i - 1, j, k), xn1 = face_v(d, v, i - 1, j, k), xn2 = face_w(d, w, i - 1, j, k);
-  T yp0 = face_u(d, u, i, j + 1, k), yp1 = face_v(d, v, i, j + 1, k), yp2 = face_w(d, w, i, j + 1, k);
-  T yn0 = face_u(d, u, i, j - 1, k), yn1 = face_v(d, v, i, j - 1, k), yn2 = face_w(d, w, i, j - 1, k);
-  T zp0 = face_u(d, u, i, j, k + 1), zp1 = face_v(d, v, i, j, k + 1), zp2 = face_w(d, w, i, j, k + 1);
-  T zn0 = face_u(d, u, i, j, k - 1), zn1 = face_v(d, v, i, j, k - 1), zn2 = face_w(d, w, i, j, k - 1);
-  T o0, o1, o2;
-  casson_cell<T>(P, m0, m1, m2, xp0, xp1, xp2, xn0, xn1, xn2, yp0, yp1, yp2, yn0, yn1, yn2, zp0, zp1,
-                 zp2, zn0, zn1, zn2, dye[c], dye[(j + 1 < d.ny) ? c + sy : c], o0, o1, o2);
-  uo[c] = (i >= d.nx - 1) ? T(0) : o0;
-  vo[c] = o1;
-  wo[c] = (k >= d.nz - 1) ? T(0) : o2;
-}
-
-// The same Casson update marching in z: a 512-thread block owns a 64 x 8
+//
+// Marching in z: a 512-thread block owns a 64 x 8
 // column tile and a z-chunk [k0, k1).  Planes of u, v, w with a one-cell
 // ring are staged in a four-slot LDS ring (one barrier per plane) with the
 // wall ghosts of face_u/v/w already applied: a ring cell outside the grid
@@ -244,8 +224,7 @@ k_casson(Dims d, StepParams P, const T *__restrict__ u, const T *__restrict__ v,
 // face, -1 for the no-slip tangential faces, +1 at the free-slip lid), and
 // the planes below 0 and above nz - 1 hold plane 0 / nz - 1 times (-1, -1,
 // 0).  So every load is in the grid and the stencil needs no branches; each
-// plane is read about 1.3 times instead of about 3.  Same cell arithmetic as
-// k_casson (casson_cell).
+// plane is read about 1.3 times instead of about 3 (by one thread per cell).
 #define CS_TX 64
 #define CS_TY 8
 #define CS_T (CS_TX * CS_TY)
