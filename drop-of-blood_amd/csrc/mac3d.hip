@@ -451,6 +451,8 @@ int mg_fill_zero(Ctx *ctx, Buf &b) {
 int sweep2_cz(const Dims &d) {
   int cols = ((d.nx + TB_TX - 1) / TB_TX) * ((d.ny + TB_TY - 1) / TB_TY);
   int chunks = std::max(1, 512 / cols);
+  // grids of up to 2^18 cells: 2-plane chunks (measured fastest on 48x64x48)
+  if ((size_t)d.nx * d.ny * d.nz <= ((size_t)1 << 18)) return 2;
   int cz = std::max((d.nz + chunks - 1) / chunks, std::min(d.nz, 6));
   cz = (cz + 1) & ~1;
   size_t pb = (size_t)d.nx * d.ny * sizeof(float);
@@ -481,12 +483,12 @@ int mg_sweep2(Ctx *ctx, int l, bool zero) {
 
 // sweeps red-black sweeps on level l.  With FP32 storage, pairs of sweeps
 // run as one two-sweep pass on level 0 and on coarser levels of at least
-// 2^20 cells (below that the pass's chunk overhead outweighs its saving).
+// 2^17 cells (below that four colour-pass launches are cheaper).
 int mg_smooth(Ctx *ctx, int l, int sweeps, bool zero) {
   int rc;
   if (zero && sweeps == 0) return mg_fill_zero(ctx, ctx->mgx[l]);
   int s = 0;
-  if (ctx->mgx[l].type == TF32 && (l == 0 || ctx->n[l] >= ((size_t)1 << 20)))
+  if (ctx->mgx[l].type == TF32 && (l == 0 || ctx->n[l] >= ((size_t)1 << 17)))
     for (; s + 2 <= sweeps; s += 2)
       if ((rc = mg_sweep2(ctx, l, zero && s == 0))) return rc;
   for (; s < sweeps; ++s)
@@ -539,7 +541,7 @@ int mg_residual_restrict(Ctx *ctx, int l) {
     with_type(ctx->mgb[l + 1].type, [&](auto ctag) {
       using Sc = std::remove_pointer_t<decltype(ctag)>;
       hipLaunchKernelGGL((k_residual_restrict_t<Sf, Sc>),
-                         dim3((df.nx + 63) / 64, dc.ny, dc.nz), dim3(64, 2, 2), 0, ctx->stream,
+                         dim3((df.nx + 31) / 32, dc.ny, dc.nz), dim3(32, 2, 2), 0, ctx->stream,
                          df, dc, (const Sf *)ctx->mgx[l].ptr, (const Sf *)ctx->mgb[l].ptr,
                          (Sc *)ctx->mgb[l + 1].ptr, (C)level_h2(l),
                          (C)(l == 0 ? 1.0 : top_coeff(l)),
@@ -572,8 +574,11 @@ int mg_prolong(Ctx *ctx, int l) {
     using Sc = std::remove_pointer_t<decltype(ctag)>;
     with_type(ctx->mgx[l].type, [&](auto ftag) {
       using Sf = std::remove_pointer_t<decltype(ftag)>;
-      hipLaunchKernelGGL((k_prolong_add_2x<Sc, Sf>), cell_grid(dc), kCellBlock, 0, ctx->stream, df,
-                         dc, (const Sc *)ctx->mgx[l + 1].ptr, (Sf *)ctx->mgx[l].ptr);
+      // 32 x 8 threads over the coarse grid (coarse rows of 96 fill three blocks)
+      hipLaunchKernelGGL((k_prolong_add_2x<Sc, Sf>),
+                         dim3((dc.nx + 31) / 32, (dc.ny + 7) / 8, dc.nz), dim3(32, 8), 0,
+                         ctx->stream, df, dc, (const Sc *)ctx->mgx[l + 1].ptr,
+                         (Sf *)ctx->mgx[l].ptr);
     });
   });
   LAUNCHCHK();
@@ -589,22 +594,28 @@ int mg_coarse(Ctx *ctx) {
   double h2 = level_h2(L);
   ProfScope ps(ctx->prof, ctx->stream, KC_COARSE);
   bool c64 = ctx->prec != MAC3D_PREC_FP32;
+  // eigenvector matrices in LDS when they fit in 48 KB (as FP64, the larger)
+  size_t qxz = ((size_t)d.nx * d.nx + (size_t)d.nz * d.nz), qyy = (size_t)d.ny * d.ny;
+  bool ql = std::max(qxz, qyy) * sizeof(double) <= 48 * 1024;
   with_type(ctx->mgx[L].type, [&](auto tag) {
     using S = std::remove_pointer_t<decltype(tag)>;
+    auto run = [&](auto ctag, auto qtag) {
+      using C = std::remove_pointer_t<decltype(ctag)>;
+      constexpr bool QL = decltype(qtag)::value;
+      size_t sh_xz = QL ? qxz * sizeof(C) : 0, sh_y = QL ? qyy * sizeof(C) : 0;
+      hipLaunchKernelGGL((k_coarse_xz_fwd<S, C, QL>), dim3(d.ny), dim3(256), sh_xz, ctx->stream,
+                         d, (const S *)ctx->mgb[L].ptr, qx, qz, ctx->cwork);
+      hipLaunchKernelGGL((k_coarse_y<C, QL>), dim3(d.nz), dim3(256), sh_y, ctx->stream, d, qy,
+                         lx, ly, lz, h2, ctx->cwork);
+      hipLaunchKernelGGL((k_coarse_xz_inv<S, C, QL>), dim3(d.ny), dim3(256), sh_xz, ctx->stream,
+                         d, (const double *)ctx->cwork, qx, qz, (S *)ctx->mgx[L].ptr);
+    };
     if (c64) {
-      hipLaunchKernelGGL((k_coarse_xz_fwd<S, double>), dim3(d.ny), dim3(256), 0, ctx->stream, d,
-                         (const S *)ctx->mgb[L].ptr, qx, qz, ctx->cwork);
-      hipLaunchKernelGGL((k_coarse_y<double>), dim3(d.nz), dim3(256), 0, ctx->stream, d, qy, lx,
-                         ly, lz, h2, ctx->cwork);
-      hipLaunchKernelGGL((k_coarse_xz_inv<S, double>), dim3(d.ny), dim3(256), 0, ctx->stream, d,
-                         (const double *)ctx->cwork, qx, qz, (S *)ctx->mgx[L].ptr);
+      if (ql) run((double *)nullptr, std::true_type());
+      else run((double *)nullptr, std::false_type());
     } else {
-      hipLaunchKernelGGL((k_coarse_xz_fwd<S, float>), dim3(d.ny), dim3(256), 0, ctx->stream, d,
-                         (const S *)ctx->mgb[L].ptr, qx, qz, ctx->cwork);
-      hipLaunchKernelGGL((k_coarse_y<float>), dim3(d.nz), dim3(256), 0, ctx->stream, d, qy, lx,
-                         ly, lz, h2, ctx->cwork);
-      hipLaunchKernelGGL((k_coarse_xz_inv<S, float>), dim3(d.ny), dim3(256), 0, ctx->stream, d,
-                         (const double *)ctx->cwork, qx, qz, (S *)ctx->mgx[L].ptr);
+      if (ql) run((float *)nullptr, std::true_type());
+      else run((float *)nullptr, std::false_type());
     }
   });
   LAUNCHCHK();

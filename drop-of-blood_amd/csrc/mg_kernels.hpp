@@ -82,7 +82,7 @@ k_residual_restrict_t(Dims df, Dims dc, const Sf *__restrict__ x, const Sf *__re
   __shared__ C sr[4][64];
   const C bscale = scale ? (C)scale[0] : C(1);
   const int tx = threadIdx.x, ty = threadIdx.y, tz = threadIdx.z;
-  const int i = blockIdx.x * 64 + tx, j = 2 * blockIdx.y + ty, k = 2 * blockIdx.z + tz;
+  const int i = blockIdx.x * blockDim.x + tx, j = 2 * blockIdx.y + ty, k = 2 * blockIdx.z + tz;
   C r = C(0);
   if (i < df.nx) {
     size_t c = idx3(df, i, j, k);
@@ -94,7 +94,7 @@ k_residual_restrict_t(Dims df, Dims dc, const Sf *__restrict__ x, const Sf *__re
   sr[ty * 2 + tz][tx] = r;
   __syncthreads();
   if (ty == 0 && tz == 0 && !(tx & 1)) {
-    int ci = (blockIdx.x * 64 + tx) >> 1;
+    int ci = (blockIdx.x * blockDim.x + tx) >> 1;
     if (ci < dc.nx) {
       C s = sr[0][tx] + sr[1][tx] + sr[2][tx] + sr[3][tx];
       bc[idx3(dc, ci, blockIdx.y, blockIdx.z)] = store_as<Sc, C>(s * C(0.125));
@@ -252,12 +252,15 @@ k_prolong_add_2x(Dims df, Dims dc, const Sc *__restrict__ ec, Sf *__restrict__ x
           zc[bq] = wl[oz] * e[0][bq][q] + wm * e[1][bq][q] + wu[oz] * e[2][bq][q];
         yz[q] = wl[oy] * zc[0] + wm * zc[1] + wu[oy] * zc[2];
       }
-      size_t base = idx3(df, 2 * I, 2 * J + oy, 2 * K + oz);
-#pragma unroll
-      for (int ox = 0; ox < 2; ++ox) {
-        C corr = wl[ox] * yz[0] + wm * yz[1] + wu[ox] * yz[2];
-        xf[base + ox] = store_as<Sf, C>((C)xf[base + ox] + corr);
-      }
+      // the two x-children as one aligned pair (2I is even)
+      struct alignas(2 * sizeof(Sf)) P2 { Sf a, b; };
+      P2 *px = (P2 *)(xf + idx3(df, 2 * I, 2 * J + oy, 2 * K + oz));
+      P2 f = *px;
+      C c0 = wl[0] * yz[0] + wm * yz[1] + wu[0] * yz[2];
+      C c1 = wl[1] * yz[0] + wm * yz[1] + wu[1] * yz[2];
+      f.a = store_as<Sf, C>((C)f.a + c0);
+      f.b = store_as<Sf, C>((C)f.b + c1);
+      *px = f;
     }
 }
 
@@ -275,13 +278,30 @@ k_prolong_add_2x(Dims df, Dims dc, const Sc *__restrict__ ec, Sf *__restrict__ x
 // LDS: two planes of at most COARSE_MAX_PLANE values each.
 #define COARSE_MAX_PLANE 2048
 
-template <class S, class C>
+// The 1-D eigenvector matrices are read nx (or ny, nz) times per output, so
+// with QL they are first copied to LDS (dynamic shared memory, converted to
+// the compute type C, as every use converts them): the dot products then
+// chain LDS reads rather than global loads.  The host passes QL when they
+// fit in 48 KB.
+template <bool QL, class C>
+__device__ inline const C *coarse_q(C *dst, const double *__restrict__ q, int n) {
+  if (!QL) return nullptr;
+  for (int t = threadIdx.x; t < n * n; t += blockDim.x) dst[t] = (C)q[t];
+  return dst;
+}
+
+template <class S, class C, bool QL>
 __global__ void __launch_bounds__(256)
 k_coarse_xz_fwd(Dims d, const S *__restrict__ b, const double *__restrict__ qx,
                 const double *__restrict__ qz, double *__restrict__ work) {
   __shared__ C s0[COARSE_MAX_PLANE], s1[COARSE_MAX_PLANE];
+  extern __shared__ unsigned char dyn_q[];
   int j = blockIdx.x;
   int nx = d.nx, nz = d.nz, np = nx * nz;
+  const C *lqx = coarse_q<QL, C>((C *)dyn_q, qx, nx);
+  const C *lqz = coarse_q<QL, C>((C *)dyn_q + nx * nx, qz, nz);
+  auto QX = [&](int r) { return QL ? lqx[r] : (C)qx[r]; };
+  auto QZ = [&](int r) { return QL ? lqz[r] : (C)qz[r]; };
   for (int t = threadIdx.x; t < np; t += 256) {
     int i = t % nx, k = t / nx;
     s0[t] = (C)b[idx3(d, i, j, k)];
@@ -291,7 +311,8 @@ k_coarse_xz_fwd(Dims d, const S *__restrict__ b, const double *__restrict__ qx,
   for (int t = threadIdx.x; t < np; t += 256) {
     int a = t % nx, k = t / nx;
     C acc = C(0);
-    for (int m = 0; m < nx; ++m) acc += (C)qx[m * nx + a] * s0[k * nx + m];
+#pragma unroll 4
+    for (int m = 0; m < nx; ++m) acc += QX(m * nx + a) * s0[k * nx + m];
     s1[t] = acc;
   }
   __syncthreads();
@@ -299,19 +320,23 @@ k_coarse_xz_fwd(Dims d, const S *__restrict__ b, const double *__restrict__ qx,
   for (int t = threadIdx.x; t < np; t += 256) {
     int a = t % nx, cc = t / nx;
     C acc = C(0);
-    for (int n = 0; n < nz; ++n) acc += (C)qz[n * nz + cc] * s1[n * nx + a];
+#pragma unroll 4
+    for (int n = 0; n < nz; ++n) acc += QZ(n * nz + cc) * s1[n * nx + a];
     work[idx3(d, a, j, cc)] = (double)acc;
   }
 }
 
-template <class C>
+template <class C, bool QL>
 __global__ void __launch_bounds__(256)
 k_coarse_y(Dims d, const double *__restrict__ qy, const double *__restrict__ lx,
            const double *__restrict__ ly, const double *__restrict__ lz, double h2,
            double *__restrict__ work) {
   __shared__ C s0[COARSE_MAX_PLANE], s1[COARSE_MAX_PLANE];
+  extern __shared__ unsigned char dyn_q[];
   int cz = blockIdx.x;
   int nx = d.nx, ny = d.ny, np = nx * ny;
+  const C *lqy = coarse_q<QL, C>((C *)dyn_q, qy, ny);
+  auto QY = [&](int r) { return QL ? lqy[r] : (C)qy[r]; };
   for (int t = threadIdx.x; t < np; t += 256) {
     int a = t % nx, m = t / nx;
     s0[t] = (C)work[idx3(d, a, m, cz)];
@@ -321,7 +346,8 @@ k_coarse_y(Dims d, const double *__restrict__ qy, const double *__restrict__ lx,
   for (int t = threadIdx.x; t < np; t += 256) {
     int a = t % nx, bb = t / nx;
     C acc = C(0);
-    for (int m = 0; m < ny; ++m) acc += (C)qy[m * ny + bb] * s0[m * nx + a];
+#pragma unroll 4
+    for (int m = 0; m < ny; ++m) acc += QY(m * ny + bb) * s0[m * nx + a];
     s1[t] = acc * (C)(-h2) / (C)(lx[a] + ly[bb] + lz[cz]);
   }
   __syncthreads();
@@ -329,18 +355,24 @@ k_coarse_y(Dims d, const double *__restrict__ qy, const double *__restrict__ lx,
   for (int t = threadIdx.x; t < np; t += 256) {
     int a = t % nx, m = t / nx;
     C acc = C(0);
-    for (int bb = 0; bb < ny; ++bb) acc += (C)qy[m * ny + bb] * s1[bb * nx + a];
+#pragma unroll 4
+    for (int bb = 0; bb < ny; ++bb) acc += QY(m * ny + bb) * s1[bb * nx + a];
     work[idx3(d, a, m, cz)] = (double)acc;
   }
 }
 
-template <class S, class C>
+template <class S, class C, bool QL>
 __global__ void __launch_bounds__(256)
 k_coarse_xz_inv(Dims d, const double *__restrict__ work, const double *__restrict__ qx,
                 const double *__restrict__ qz, S *__restrict__ x) {
   __shared__ C s0[COARSE_MAX_PLANE], s1[COARSE_MAX_PLANE];
+  extern __shared__ unsigned char dyn_q[];
   int j = blockIdx.x;
   int nx = d.nx, nz = d.nz, np = nx * nz;
+  const C *lqx = coarse_q<QL, C>((C *)dyn_q, qx, nx);
+  const C *lqz = coarse_q<QL, C>((C *)dyn_q + nx * nx, qz, nz);
+  auto QX = [&](int r) { return QL ? lqx[r] : (C)qx[r]; };
+  auto QZ = [&](int r) { return QL ? lqz[r] : (C)qz[r]; };
   for (int t = threadIdx.x; t < np; t += 256) {
     int a = t % nx, cc = t / nx;
     s0[t] = (C)work[idx3(d, a, j, cc)];
@@ -350,7 +382,8 @@ k_coarse_xz_inv(Dims d, const double *__restrict__ work, const double *__restric
   for (int t = threadIdx.x; t < np; t += 256) {
     int a = t % nx, n = t / nx;
     C acc = C(0);
-    for (int cc = 0; cc < nz; ++cc) acc += (C)qz[n * nz + cc] * s0[cc * nx + a];
+#pragma unroll 4
+    for (int cc = 0; cc < nz; ++cc) acc += QZ(n * nz + cc) * s0[cc * nx + a];
     s1[t] = acc;
   }
   __syncthreads();
@@ -358,7 +391,8 @@ k_coarse_xz_inv(Dims d, const double *__restrict__ work, const double *__restric
   for (int t = threadIdx.x; t < np; t += 256) {
     int m = t % nx, n = t / nx;
     C acc = C(0);
-    for (int a = 0; a < nx; ++a) acc += (C)qx[m * nx + a] * s1[n * nx + a];
+#pragma unroll 4
+    for (int a = 0; a < nx; ++a) acc += QX(m * nx + a) * s1[n * nx + a];
     x[idx3(d, m, j, n)] = store_as<S, C>(acc);
   }
 }

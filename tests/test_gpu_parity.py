@@ -259,11 +259,13 @@ def test_two_sweep_pass_against_oracle():
         assert rel_l2(s.download("MGX", 0), o.get(oracle.F_MGX, 0)) <= TOL32
 
 
+@pytest.mark.parametrize("level", [1, 2])
 @pytest.mark.parametrize("omega", [1.0, 1.2])
-def test_two_sweep_pass_level1_against_oracle(omega):
-    """Level 1 of the 192x256x192 grid (96x128x96, above the 2^20-cell
-    threshold) also smooths in two-sweep passes: against the oracle's two
-    FP64 sweeps (h^2 = 4, lid coefficient 4/3)."""
+def test_two_sweep_pass_coarse_levels_against_oracle(level, omega):
+    """Levels 1 and 2 of the 192x256x192 grid (96x128x96 and 48x64x48, at or
+    above 2^17 cells; level 2 in 2-plane chunks) also smooth in two-sweep
+    passes: against the oracle's two FP64 sweeps (h^2 = 4^l, lid coefficient
+    2 2^l / (2^l + 1))."""
     from mac3d import MacSolver
     from mac3d.lib import config_for
     from mac3d import scenarios
@@ -271,14 +273,14 @@ def test_two_sweep_pass_level1_against_oracle(omega):
     o = oracle.Oracle(w.nx, w.ny, w.nz, w.dx, levels=w.levels, omega=omega)
     rng = np.random.default_rng(17)
     with MacSolver(config_for(w, sor_omega=omega)) as s:
-        shp = s.level_shape(1)
-        assert np.prod(shp) >= 1 << 20
+        shp = s.level_shape(level)
+        assert np.prod(shp) >= 1 << 17
         x, b = rng.standard_normal(shp), rng.standard_normal(shp)
-        s.upload("MGX", x, 1)
-        s.upload("MGB", b, 1)
-        o.set(oracle.F_MGX, x, 1)
-        o.set(oracle.F_MGB, b, 1)
-        s.mg("SMOOTH", 1, 2)
-        o.mg(oracle.MG_SMOOTH, 1, 2)
-        assert rel_l2(s.download("MGX", 1), o.get(oracle.F_MGX, 1)) <= TOL32
+        s.upload("MGX", x, level)
+        s.upload("MGB", b, level)
+        o.set(oracle.F_MGX, x, level)
+        o.set(oracle.F_MGB, b, level)
+        s.mg("SMOOTH", level, 2)
+        o.mg(oracle.MG_SMOOTH, level, 2)
+        assert rel_l2(s.download("MGX", level), o.get(oracle.F_MGX, level)) <= TOL32
     o.close()
