@@ -290,8 +290,25 @@ __device__ inline const C *coarse_q(C *dst, const double *__restrict__ q, int n)
   return dst;
 }
 
+// sum of f(0) .. f(n-1) in four interleaved partial sums (a quarter of the
+// dependent chain of one running sum), combined pairwise
+template <class F>
+__device__ inline auto dot4(int n, F f) -> decltype(f(0)) {
+  typedef decltype(f(0)) C;
+  C a0 = C(0), a1 = C(0), a2 = C(0), a3 = C(0);
+  int m = 0;
+  for (; m + 4 <= n; m += 4) {
+    a0 += f(m);
+    a1 += f(m + 1);
+    a2 += f(m + 2);
+    a3 += f(m + 3);
+  }
+  for (; m < n; ++m) a0 += f(m);
+  return (a0 + a1) + (a2 + a3);
+}
+
 template <class S, class C, bool QL>
-__global__ void __launch_bounds__(256)
+__global__ void __launch_bounds__(1024)
 k_coarse_xz_fwd(Dims d, const S *__restrict__ b, const double *__restrict__ qx,
                 const double *__restrict__ qz, double *__restrict__ work) {
   __shared__ C s0[COARSE_MAX_PLANE], s1[COARSE_MAX_PLANE];
@@ -302,32 +319,26 @@ k_coarse_xz_fwd(Dims d, const S *__restrict__ b, const double *__restrict__ qx,
   const C *lqz = coarse_q<QL, C>((C *)dyn_q + nx * nx, qz, nz);
   auto QX = [&](int r) { return QL ? lqx[r] : (C)qx[r]; };
   auto QZ = [&](int r) { return QL ? lqz[r] : (C)qz[r]; };
-  for (int t = threadIdx.x; t < np; t += 256) {
+  for (int t = threadIdx.x; t < np; t += blockDim.x) {
     int i = t % nx, k = t / nx;
     s0[t] = (C)b[idx3(d, i, j, k)];
   }
   __syncthreads();
   // s1[k][a] = sum_m Qx[m][a] s0[k][m]
-  for (int t = threadIdx.x; t < np; t += 256) {
+  for (int t = threadIdx.x; t < np; t += blockDim.x) {
     int a = t % nx, k = t / nx;
-    C acc = C(0);
-#pragma unroll 4
-    for (int m = 0; m < nx; ++m) acc += QX(m * nx + a) * s0[k * nx + m];
-    s1[t] = acc;
+    s1[t] = dot4(nx, [&](int m) { return QX(m * nx + a) * s0[k * nx + m]; });
   }
   __syncthreads();
   // out[c][a] = sum_n Qz[n][c] s1[n][a]
-  for (int t = threadIdx.x; t < np; t += 256) {
+  for (int t = threadIdx.x; t < np; t += blockDim.x) {
     int a = t % nx, cc = t / nx;
-    C acc = C(0);
-#pragma unroll 4
-    for (int n = 0; n < nz; ++n) acc += QZ(n * nz + cc) * s1[n * nx + a];
-    work[idx3(d, a, j, cc)] = (double)acc;
+    work[idx3(d, a, j, cc)] = (double)dot4(nz, [&](int n) { return QZ(n * nz + cc) * s1[n * nx + a]; });
   }
 }
 
 template <class C, bool QL>
-__global__ void __launch_bounds__(256)
+__global__ void __launch_bounds__(1024)
 k_coarse_y(Dims d, const double *__restrict__ qy, const double *__restrict__ lx,
            const double *__restrict__ ly, const double *__restrict__ lz, double h2,
            double *__restrict__ work) {
@@ -337,32 +348,27 @@ k_coarse_y(Dims d, const double *__restrict__ qy, const double *__restrict__ lx,
   int nx = d.nx, ny = d.ny, np = nx * ny;
   const C *lqy = coarse_q<QL, C>((C *)dyn_q, qy, ny);
   auto QY = [&](int r) { return QL ? lqy[r] : (C)qy[r]; };
-  for (int t = threadIdx.x; t < np; t += 256) {
+  for (int t = threadIdx.x; t < np; t += blockDim.x) {
     int a = t % nx, m = t / nx;
     s0[t] = (C)work[idx3(d, a, m, cz)];
   }
   __syncthreads();
   // s1[bb][a] = (sum_m Qy[m][bb] s0[m][a]) / (lx_a + ly_bb + lz_cz) * (-h2)
-  for (int t = threadIdx.x; t < np; t += 256) {
+  for (int t = threadIdx.x; t < np; t += blockDim.x) {
     int a = t % nx, bb = t / nx;
-    C acc = C(0);
-#pragma unroll 4
-    for (int m = 0; m < ny; ++m) acc += QY(m * ny + bb) * s0[m * nx + a];
+    C acc = dot4(ny, [&](int m) { return QY(m * ny + bb) * s0[m * nx + a]; });
     s1[t] = acc * (C)(-h2) / (C)(lx[a] + ly[bb] + lz[cz]);
   }
   __syncthreads();
   // back: work[m][a] = sum_bb Qy[m][bb] s1[bb][a]
-  for (int t = threadIdx.x; t < np; t += 256) {
+  for (int t = threadIdx.x; t < np; t += blockDim.x) {
     int a = t % nx, m = t / nx;
-    C acc = C(0);
-#pragma unroll 4
-    for (int bb = 0; bb < ny; ++bb) acc += QY(m * ny + bb) * s1[bb * nx + a];
-    work[idx3(d, a, m, cz)] = (double)acc;
+    work[idx3(d, a, m, cz)] = (double)dot4(ny, [&](int bb) { return QY(m * ny + bb) * s1[bb * nx + a]; });
   }
 }
 
 template <class S, class C, bool QL>
-__global__ void __launch_bounds__(256)
+__global__ void __launch_bounds__(1024)
 k_coarse_xz_inv(Dims d, const double *__restrict__ work, const double *__restrict__ qx,
                 const double *__restrict__ qz, S *__restrict__ x) {
   __shared__ C s0[COARSE_MAX_PLANE], s1[COARSE_MAX_PLANE];
@@ -373,27 +379,21 @@ k_coarse_xz_inv(Dims d, const double *__restrict__ work, const double *__restric
   const C *lqz = coarse_q<QL, C>((C *)dyn_q + nx * nx, qz, nz);
   auto QX = [&](int r) { return QL ? lqx[r] : (C)qx[r]; };
   auto QZ = [&](int r) { return QL ? lqz[r] : (C)qz[r]; };
-  for (int t = threadIdx.x; t < np; t += 256) {
+  for (int t = threadIdx.x; t < np; t += blockDim.x) {
     int a = t % nx, cc = t / nx;
     s0[t] = (C)work[idx3(d, a, j, cc)];
   }
   __syncthreads();
   // s1[n][a] = sum_c Qz[n][c] s0[c][a]
-  for (int t = threadIdx.x; t < np; t += 256) {
+  for (int t = threadIdx.x; t < np; t += blockDim.x) {
     int a = t % nx, n = t / nx;
-    C acc = C(0);
-#pragma unroll 4
-    for (int cc = 0; cc < nz; ++cc) acc += QZ(n * nz + cc) * s0[cc * nx + a];
-    s1[t] = acc;
+    s1[t] = dot4(nz, [&](int cc) { return QZ(n * nz + cc) * s0[cc * nx + a]; });
   }
   __syncthreads();
   // x[n][m] = sum_a Qx[m][a] s1[n][a]
-  for (int t = threadIdx.x; t < np; t += 256) {
+  for (int t = threadIdx.x; t < np; t += blockDim.x) {
     int m = t % nx, n = t / nx;
-    C acc = C(0);
-#pragma unroll 4
-    for (int a = 0; a < nx; ++a) acc += QX(m * nx + a) * s1[n * nx + a];
-    x[idx3(d, m, j, n)] = store_as<S, C>(acc);
+    x[idx3(d, m, j, n)] = store_as<S, C>(dot4(nx, [&](int a) { return QX(m * nx + a) * s1[n * nx + a]; }));
   }
 }
 
@@ -898,11 +898,14 @@ k_rbgs_sweep2(Dims d, const float *__restrict__ xin, float *__restrict__ xout,
   // base of plane j - 1 + d (plane j + 6 reuses xsl[0]), bsl[d] the b slot
   // base of plane j + d
   int xsl[7], bsl[5];
-  auto iter = [&](auto PARc, int j, V2 (&xr)[2], V2 (&br)[2]) {
+  // CHK false: a steady-state iteration, in which every stage is in range and
+  // no plane is 0 or nz - 1 (the checks compile away)
+  auto iter = [&](auto PARc, auto CHKc, int j, V2 (&xr)[2], V2 (&br)[2]) {
     constexpr int par = decltype(PARc)::value;
-    auto edge = [&](int p) { return p == 0 || p == nz - 1; };
+    constexpr bool CHK = decltype(CHKc)::value;
+    auto edge = [&](int p) { return CHK && (p == 0 || p == nz - 1); };
     // group 1: B2(j), R1(j + 4)
-    if (j >= k0) {
+    if (!CHK || j >= k0) {
       int own = xsl[1] + XCH, oth = xsl[1];
       C vb = calc(0, par, edge(j), own, oth, xsl[0], xsl[2], ld(bsl[0] + BCH + sofs[0]));
       C vr = ld(oth + sofs[0]);
@@ -914,7 +917,7 @@ k_rbgs_sweep2(Dims d, const float *__restrict__ xin, float *__restrict__ xout,
         *(V2 *)(tout + (size_t)j * nx * ny) = o;
       }
     }
-    if (j + 4 >= r1lo && j + 4 < r1hi) {
+    if (!CHK || (j + 4 >= r1lo && j + 4 < r1hi)) {
       const bool bnd = edge(j + 4);
       int own = xsl[5], oth = xsl[5] + XCH, om = xsl[4] + XCH, op = xsl[6] + XCH;
       int bR = bsl[4], bK = bsl[4] + BCH;
@@ -939,7 +942,7 @@ k_rbgs_sweep2(Dims d, const float *__restrict__ xin, float *__restrict__ xout,
     fetch(rb, kb0, j + 6, bo, br);
     __syncthreads();
     // group 2: B1(j + 3)
-    if (j + 3 >= b1lo && j + 3 < b1hi) {
+    if (!CHK || (j + 3 >= b1lo && j + 3 < b1hi)) {
       constexpr int pp = par ^ 1;
       const bool bnd = edge(j + 3);
       int own = xsl[4] + XCH, oth = xsl[4], om = xsl[3], op = xsl[5];
@@ -951,7 +954,7 @@ k_rbgs_sweep2(Dims d, const float *__restrict__ xin, float *__restrict__ xout,
     }
     __syncthreads();
     // group 3: R2(j + 2), staging of x(j + 6)
-    if (j + 2 >= r2lo && j + 2 < r2hi) {
+    if (!CHK || (j + 2 >= r2lo && j + 2 < r2hi)) {
       const bool bnd = edge(j + 2);
       int own = xsl[3], oth = xsl[3] + XCH, om = xsl[2] + XCH, op = xsl[4] + XCH;
       int bR = bsl[2];
@@ -1014,14 +1017,34 @@ k_rbgs_sweep2(Dims d, const float *__restrict__ xin, float *__restrict__ xout,
     for (int q = 0; q < 4; ++q) bsl[q] = bsl[q + 1];
     bsl[4] = t;
   };
+  // Steady state: iterations j in [max(k0, 1), min(k1 - 1, nz - 5)) have
+  // every stage in range and no plane at 0 or nz - 1; they run unchecked, from
+  // the first odd j there (the loop pairs odd and even iterations).
+  const std::integral_constant<int, 1> odd;
+  const std::integral_constant<int, 0> even;
+  const std::true_type chk;
+  const std::false_type nochk;
+  const int js = max(k0, 1) | 1, je = min(k1 - 1, nz - 5);
   int j = k0 - 7;
-  for (; j + 1 < k1; j += 2) {
-    iter(std::integral_constant<int, 1>(), j, xA, bA);
+  for (; j + 1 < k1 && j < js; j += 2) {
+    iter(odd, chk, j, xA, bA);
     rotate();
-    iter(std::integral_constant<int, 0>(), j + 1, xB, bB);
+    iter(even, chk, j + 1, xB, bB);
     rotate();
   }
-  if (j < k1) iter(std::integral_constant<int, 1>(), j, xA, bA);
+  for (; j + 1 < je; j += 2) {
+    iter(odd, nochk, j, xA, bA);
+    rotate();
+    iter(even, nochk, j + 1, xB, bB);
+    rotate();
+  }
+  for (; j + 1 < k1; j += 2) {
+    iter(odd, chk, j, xA, bA);
+    rotate();
+    iter(even, chk, j + 1, xB, bB);
+    rotate();
+  }
+  if (j < k1) iter(odd, chk, j, xA, bA);
 }
 
 }  // namespace mac3d
