@@ -107,7 +107,9 @@ k_residual_restrict_t(Dims df, Dims dc, const Sf *__restrict__ x, const Sf *__re
 // LDS ring with a one-cell ring in x and y (each x value read about 1.3x
 // instead of ~3x by independent blocks), the 2 x 2 x 2 child sums are formed
 // in registers (z), a lane shuffle (x) and LDS (y).  Same arithmetic as
-// k_residual followed by k_restrict.
+// k_residual followed by k_restrict, except that the diagonal's terms are
+// summed in another order (the same value on level 0, where all are whole
+// numbers; within an ulp where the lid coefficient is not).
 #define RR_TX 64
 #define RR_TY 8
 #define RR_EX (RR_TX + 2)
