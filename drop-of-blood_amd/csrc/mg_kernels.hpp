@@ -834,6 +834,13 @@ k_rbgs_sweep2(Dims d, const float *__restrict__ xin, float *__restrict__ xout,
   for (int q = 0; q < 2; ++q) bits |= (unsigned)((y0 - 4 + (tid + q * TB_T) / TB_PW) & 1) << (28 + q);
   if (tid + TB_T < TB_PW * TB_EY) bits |= 1u << 30;
   if (tid + TB_T < TB_PW * TB_BY) bits |= 1u << 31;
+  // Second pairs of R1, B1 and R2 exist for the first 280, 168 and 100
+  // threads only: whole waves past them skip those updates (a scalar branch;
+  // their lanes would write back old values).
+  const int wb = __builtin_amdgcn_readfirstlane(tid & ~63) + TB_T;
+  const bool w2 = wb < TB_PW * TB_BY;
+  const bool w4 = wb < (TB_TX / 2 + 2) * (TB_TY + 4);
+  const bool w6 = wb < (TB_TX / 2 + 2) * (TB_TY + 2);
   const int tr = tid / (TB_TX / 2), tpc = tid % (TB_TX / 2);
   const int tgy = y0 + tr, tgx = x0 + 2 * tpc;
   const bool tin = tgx < nx && tgy < ny;
@@ -930,10 +937,11 @@ k_rbgs_sweep2(Dims d, const float *__restrict__ xin, float *__restrict__ xout,
         int rt = mask(4 * (1 + q) + 2 * par + 1);
         bred[q] = pick(rt, b1, b0);
         bblk[q] = pick(rt, b0, b1);
-        nv[q] = calc_w(1 + q, par, bnd, own, oth, om, op, bred[q]);
       }
-#pragma unroll
-      for (int q = 0; q < 2; ++q) st_(own + sofs[1 + q], nv[q]);
+      nv[0] = calc_w(1, par, bnd, own, oth, om, op, bred[0]);
+      if (w2) nv[1] = calc_w(2, par, bnd, own, oth, om, op, bred[1]);
+      st_(own + sofs[1], nv[0]);
+      if (w2) st_(own + sofs[2], nv[1]);
       st_(bR + sofs[1], bred[0]);
       st_(bK + sofs[1], bblk[0]);
       if (bits >> 31) {
@@ -950,9 +958,11 @@ k_rbgs_sweep2(Dims d, const float *__restrict__ xin, float *__restrict__ xout,
       int own = xsl[4] + XCH, oth = xsl[4], om = xsl[3], op = xsl[5];
       int bK = bsl[3] + BCH;
       C nv0 = calc_w(3, pp, bnd, own, oth, om, op, ld(bK + sofs[3]));
-      C nv1 = calc_w(4, pp, bnd, own, oth, om, op, ld(bK + sofs[4]));
       st_(own + sofs[3], nv0);
-      st_(own + sofs[4], nv1);
+      if (w4) {
+        C nv1 = calc_w(4, pp, bnd, own, oth, om, op, ld(bK + sofs[4]));
+        st_(own + sofs[4], nv1);
+      }
     }
     __syncthreads();
     // group 3: R2(j + 2), staging of x(j + 6)
@@ -961,9 +971,11 @@ k_rbgs_sweep2(Dims d, const float *__restrict__ xin, float *__restrict__ xout,
       int own = xsl[3], oth = xsl[3] + XCH, om = xsl[2] + XCH, op = xsl[4] + XCH;
       int bR = bsl[2];
       C nv0 = calc_w(5, par, bnd, own, oth, om, op, ld(bR + sofs[5]));
-      C nv1 = calc_w(6, par, bnd, own, oth, om, op, ld(bR + sofs[6]));
       st_(own + sofs[5], nv0);
-      st_(own + sofs[6], nv1);
+      if (w6) {
+        C nv1 = calc_w(6, par, bnd, own, oth, om, op, ld(bR + sofs[6]));
+        st_(own + sofs[6], nv1);
+      }
     }
     {
       int red = xsl[0], blk = xsl[0] + XCH;
