@@ -66,6 +66,9 @@ __device__ inline T sample_cell(const T *__restrict__ f, const Dims &d, T ux,
 template <class T> struct vec4_of;
 template <> struct vec4_of<float> { typedef float4 type; };
 template <> struct vec4_of<double> { typedef double4 type; };
+template <class T> struct vec2_of;
+template <> struct vec2_of<float> { typedef float2 type; };
+template <> struct vec2_of<double> { typedef double2 type; };
 
 template <class T>
 __device__ inline void sample_vec(const typename vec4_of<T>::type *__restrict__ f, const Dims &d,
@@ -234,6 +237,15 @@ template <class T> __device__ inline T wave_max(T v) {
     v = w > v ? w : v;
   }
   return v;
+}
+
+// Blocks are numbered on a 1-D grid and remapped so that each of the 8 XCDs
+// (blocks are dealt round-robin to XCDs) gets a contiguous run of tiles: the
+// halo planes and rows a tile shares with its neighbours then sit in the
+// same XCD's L2.
+__device__ inline int xcd_remap(int b, int nblocks) {
+  if (nblocks % 8) return b;
+  return (b % 8) * (nblocks / 8) + b / 8;
 }
 
 }  // namespace mac3d

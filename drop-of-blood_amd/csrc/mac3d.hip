@@ -515,23 +515,28 @@ int mg_residual_restrict(Ctx *ctx, int l) {
   const Dims &df = ctx->d[l], &dc = ctx->d[l + 1];
   ProfScope ps(ctx->prof, ctx->stream, l == 0 ? KC_RESID0 : KC_MG);
   if (l == 0 || (ctx->mgx[l].type != TF16 && ctx->n[l] >= ((size_t)1 << 20))) {
-    // z-marching kernel (level 0, and FP32/FP64 levels of 2^20 cells and up):
-    // even chunks, about 3 blocks per CU in one round
-    int ncx = (df.nx + RR_TX - 1) / RR_TX, ncy = (df.ny + RR_TY - 1) / RR_TY;
-    int chunks = std::max(1, 768 / (ncx * ncy));
+    // wave-column kernel (level 0, and FP32/FP64 levels of 2^20 cells and
+    // up): even chunks of at least 6 planes, about 24 waves per CU
+    int wcx = (df.nx + 63) / 64, cols = wcx * (df.ny / 2);
+    int chunks = std::max(1, 6144 / cols);
     int cz = (df.nz + chunks - 1) / chunks;
-    cz = std::max(2, (cz + 1) & ~1);
-    dim3 grid(ncx * ncy * ((df.nz + cz - 1) / cz));
+    cz = std::max(std::min(df.nz, 6), (cz + 1) & ~1);
+    // the chunk's buffer ranges and byte offsets (loads run up to four planes
+    // past the chunk) stay below 2^30
+    const size_t pb = (size_t)df.nx * df.ny * type_size(ctx->mgx[l].type);
+    while (cz > 2 && (size_t)(cz + 8) * pb >= ((size_t)1 << 30)) cz -= 2;
+    int nw = cols * ((df.nz + cz - 1) / cz);
+    dim3 grid((nw + RC_WAVES - 1) / RC_WAVES);
     with_step_type(ctx->mgx[l].type, [&](auto ftag) {
       using Sf = std::remove_pointer_t<decltype(ftag)>;
       using C = typename compute_of<Sf>::type;
       with_type(ctx->mgb[l + 1].type, [&](auto ctag) {
         using Sc = std::remove_pointer_t<decltype(ctag)>;
-        hipLaunchKernelGGL((k_residual_restrict_zm<Sf, Sc>), grid, dim3(RR_TX, RR_TY), 0,
+        hipLaunchKernelGGL((k_residual_restrict_col<Sf, Sc>), grid, dim3(64 * RC_WAVES), 0,
                            ctx->stream, df, dc, (const Sf *)ctx->mgx[l].ptr,
                            (const Sf *)ctx->mgb[l].ptr, (Sc *)ctx->mgb[l + 1].ptr,
                            (C)level_h2(l), (C)(l == 0 ? 1.0 : top_coeff(l)),
-                           l == 0 ? (const double *)ctx->scale : nullptr, cz, ncx, ncy);
+                           l == 0 ? (const double *)ctx->scale : nullptr, cz, wcx, nw);
       });
     });
     LAUNCHCHK();

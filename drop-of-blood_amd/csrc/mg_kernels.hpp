@@ -102,104 +102,107 @@ k_residual_restrict_t(Dims df, Dims dc, const Sf *__restrict__ x, const Sf *__re
   }
 }
 
-// Residual-restriction marching in z: a 512-thread block owns a 64 x 8 fine
-// column tile and an even z-chunk; planes of x are staged in a three-plane
-// LDS ring with a one-cell ring in x and y (each x value read about 1.3x
-// instead of ~3x by independent blocks), the 2 x 2 x 2 child sums are formed
-// in registers (z), a lane shuffle (x) and LDS (y).  Same arithmetic as
-// k_residual followed by k_restrict, except that the diagonal's terms are
-// summed in another order (the same value on level 0, where all are whole
-// numbers; within an ulp where the lid coefficient is not).
-#define RR_TX 64
-#define RR_TY 8
-#define RR_EX (RR_TX + 2)
-#define RR_EY (RR_TY + 2)
-#define RR_T (RR_TX * RR_TY)
-#define RR_PL (RR_EX * RR_EY)
-#define RR_LPT ((RR_PL + RR_T - 1) / RR_T)
+// Residual-restriction by wave columns: each wavefront owns 64 x 2 fine
+// cells (32 x-pairs by the two rows of one coarse row) and marches an even
+// z-chunk with no LDS and no barrier.  The z neighbours of its cells stay in
+// registers; x and y neighbours come from loads that hit the cache lines of
+// this and the adjacent waves; the two rows of a coarse cell meet in one lane
+// shuffle.  Loads run two planes ahead.  Same arithmetic as k_residual
+// followed by k_restrict (the children summed z, then x, then y), except that
+// the diagonal's terms are summed in another order (the same value on level
+// 0, where all are whole numbers; within an ulp where the lid coefficient is
+// not) and the division by h^2 is a product with its reciprocal (within an
+// ulp).
+#define RC_WAVES 4
 
 template <class Sf, class Sc>
-__global__ void __launch_bounds__(RR_T)
-k_residual_restrict_zm(Dims df, Dims dc, const Sf *__restrict__ x, const Sf *__restrict__ b,
-                       Sc *__restrict__ bc, typename compute_of<Sf>::type h2,
-                       typename compute_of<Sf>::type topc, const double *__restrict__ scale,
-                       int cz, int ncx, int ncy) {
+__global__ void __launch_bounds__(64 * RC_WAVES)
+k_residual_restrict_col(Dims df, Dims dc, const Sf *__restrict__ x, const Sf *__restrict__ b,
+                        Sc *__restrict__ bc, typename compute_of<Sf>::type h2,
+                        typename compute_of<Sf>::type topc, const double *__restrict__ scale,
+                        int cz, int ncx, int nwaves) {
   typedef typename compute_of<Sf>::type C;
-  __shared__ C sx[3][RR_EY][RR_EX];
-  __shared__ C sred[RR_TY][RR_TX];
-  const C bscale = scale ? (C)scale[0] : C(1);
-  const int nx = df.nx, ny = df.ny, nz = df.nz;
-  const int lb = blockIdx.x;
-  const int x0 = (lb % ncx) * RR_TX, y0 = ((lb / ncx) % ncy) * RR_TY;
-  const int k0 = (lb / (ncx * ncy)) * cz, k1 = min(k0 + cz, nz);
-  const int tx = threadIdx.x, ty = threadIdx.y, tid = ty * RR_TX + tx;
-  const size_t plane = (size_t)nx * ny;
-  const int gx = x0 + tx, gy = y0 + ty;
-  const bool inside = gx < nx && gy < ny;
-  int st_o[RR_LPT], st_r[RR_LPT], st_c[RR_LPT];
-#pragma unroll
-  for (int q = 0; q < RR_LPT; ++q) {
-    int v = tid + q * RR_T;
-    st_r[q] = v / RR_EX;
-    st_c[q] = v % RR_EX;
-    int sxg = x0 - 1 + st_c[q], syg = y0 - 1 + st_r[q];
-    st_o[q] = (v < RR_PL && sxg >= 0 && sxg < nx && syg >= 0 && syg < ny)
-                  ? (syg * nx + sxg) * (int)sizeof(Sf) : MAC3D_OFF_NONE;
-  }
-  // buffer ranges: x planes k0-1 .. k1, b planes k0 .. k1-1; planes outside
-  // (the z walls) and cells outside the grid load as 0
-  const int pb = (int)(plane * sizeof(Sf));
+  typedef typename vec2_of<Sf>::type V2;
+  const int wv = __builtin_amdgcn_readfirstlane(xcd_remap(blockIdx.x, gridDim.x) * RC_WAVES +
+                                                (int)(threadIdx.x >> 6));
+  if (wv >= nwaves) return;
+  const int lane = threadIdx.x & 63;
+  const int nx = df.nx, ny = df.ny, nz = df.nz, ncy = ny >> 1;
+  const int cx = wv % ncx, cy = (wv / ncx) % ncy, ch = wv / (ncx * ncy);
+  const int k0 = ch * cz, k1 = min(k0 + cz, nz);
+  const int gx = cx * 64 + 2 * (lane & 31), gy = 2 * cy + (lane >> 5);
+  const bool inside = gx < nx;
+  constexpr int es = (int)sizeof(Sf);
+  // in-plane byte offsets of the pair and of its x-, x+, y-, y+ neighbours
+  // (cells outside the grid load as 0)
+  const int oc = inside ? (gy * nx + gx) * es : MAC3D_OFF_NONE;
+  const int ol = inside && gx > 0 ? oc - es : MAC3D_OFF_NONE;
+  const int orr = inside && gx + 2 < nx ? oc + 2 * es : MAC3D_OFF_NONE;
+  const int od = inside && gy > 0 ? oc - nx * es : MAC3D_OFF_NONE;
+  const int ou = inside && gy + 1 < ny ? oc + nx * es : MAC3D_OFF_NONE;
+  // buffer ranges: x planes k0-1 .. k1, b planes k0 .. k1-1
+  const int pb = (int)((size_t)nx * ny * es);
   const int kx0 = max(k0 - 1, 0), kx1 = min(k1 + 1, nz);
-  const BufRsrc rx = buf_rsrc(x + (size_t)kx0 * plane, (uint32_t)(kx1 - kx0) * pb);
-  const BufRsrc rb = buf_rsrc(b + (size_t)k0 * plane, (uint32_t)(k1 - k0) * pb);
-  const int bo = inside ? (gy * nx + gx) * (int)sizeof(Sf) : MAC3D_OFF_NONE;
-  Sf pre[RR_LPT];
-  auto fetch = [&](int k) {
-#pragma unroll
-    for (int q = 0; q < RR_LPT; ++q) pre[q] = buf_load<Sf>(rx, st_o[q] + (k - kx0) * pb);
+  const BufRsrc rx = buf_rsrc(x + (size_t)kx0 * nx * ny, (uint32_t)(kx1 - kx0) * pb);
+  const BufRsrc rb = buf_rsrc(b + (size_t)k0 * nx * ny, (uint32_t)(k1 - k0) * pb);
+  const C bscale = scale ? (C)scale[0] : C(1);
+  const C rh2 = C(1) / h2;
+  // in-plane part of the diagonal of the pair's two cells
+  const C top = (gy < ny - 1) ? C(1) : topc;
+  const C dga = top + C((gx > 0) + (gx < nx - 1) + (gy > 0));
+  const C dgb = top + C((gx + 1 > 0) + (gx + 1 < nx - 1) + (gy > 0));
+  // loads for plane p: x pair of plane p+1; x-, x+, y-, y+ and b of plane p
+  struct Set { V2 cn, u, d, bb; Sf l, r; };
+  auto load_set = [&](int p, Set &S) {
+    const int zo = (p - kx0) * pb;
+    S.cn = buf_load<V2>(rx, oc + zo + pb);
+    S.l = buf_load<Sf>(rx, ol + zo);
+    S.r = buf_load<Sf>(rx, orr + zo);
+    S.d = buf_load<V2>(rx, od + zo);
+    S.u = buf_load<V2>(rx, ou + zo);
+    S.bb = buf_load<V2>(rb, oc + (p - k0) * pb);
   };
-  auto stage = [&](int k) {
-#pragma unroll
-    for (int q = 0; q < RR_LPT; ++q)
-      if (tid + q * RR_T < RR_PL) sx[(k + 3) % 3][st_r[q]][st_c[q]] = (C)pre[q];
+  V2 cm = buf_load<V2>(rx, oc + (k0 - 1 - kx0) * pb);  // plane k0-1 (0 below the grid)
+  V2 cc = buf_load<V2>(rx, oc + (k0 - kx0) * pb);
+  Set A, B;
+  load_set(k0, A);
+  load_set(k0 + 1, B);
+  C sa = C(0), sb = C(0);
+  auto plane = [&](int k, Set &S, auto ODDc) {
+    constexpr bool odd = decltype(ODDc)::value;
+    const C zt = C((k > 0) + (k < nz - 1));
+    const C xa = (C)cc.x, xb = (C)cc.y;
+    // the six neighbours in the stencil's order x-, x+, y-, y+, z-, z+
+    C na = (C)S.l + xb;
+    na += (C)S.d.x;
+    na += (C)S.u.x;
+    na += (C)cm.x;
+    na += (C)S.cn.x;
+    C nb = xa + (C)S.r;
+    nb += (C)S.d.y;
+    nb += (C)S.u.y;
+    nb += (C)cm.y;
+    nb += (C)S.cn.y;
+    const C ra = (C)S.bb.x * bscale - (na - (dga + zt) * xa) * rh2;
+    const C rbv = (C)S.bb.y * bscale - (nb - (dgb + zt) * xb) * rh2;
+    cm = cc;
+    cc = S.cn;
+    load_set(k + 2, S);
+    if (!odd) {
+      sa = ra;
+      sb = rbv;
+    } else {
+      sa += ra;
+      sb += rbv;
+      C s = sa + sb;
+      s += __shfl_xor(s, 32, 64);
+      if (lane < 32 && inside)
+        bc[idx3(dc, gx >> 1, cy, k >> 1)] = store_as<Sc, C>(s * C(0.125));
+    }
   };
-  // diagonal of this column (the z terms are added per plane)
-  const C dgxy = ((gy < ny - 1) ? C(1) : topc) + C((gx > 0) + (gx < nx - 1) + (gy > 0));
-  for (int k = k0 - 1; k <= k0; ++k) {
-    fetch(k);
-    stage(k);
-  }
-  fetch(k0 + 1);
-  Sf bnext = buf_load<Sf>(rb, bo);
-  C rsum = C(0);
-  for (int k = k0; k < k1; ++k) {
-    stage(k + 1);
-    C bk = (C)bnext;
-    __syncthreads();
-    fetch(k + 2);
-    bnext = buf_load<Sf>(rb, bo + (k + 1 - k0) * pb);
-    if (inside) {
-      // neighbours outside the grid are staged as 0: the sum runs over all six
-      // in the stencil's order
-      const int r = ty + 1, c = tx + 1, s0 = k % 3;
-      C sn = sx[s0][r][c - 1] + sx[s0][r][c + 1];
-      sn += sx[s0][r - 1][c];
-      sn += sx[s0][r + 1][c];
-      sn += sx[(k + 2) % 3][r][c];
-      sn += sx[(k + 1) % 3][r][c];
-      C dg = dgxy + C((k > 0) + (k < nz - 1));
-      rsum += bk * bscale - (sn - dg * sx[s0][r][c]) / h2;
-    }
-    if (k & 1) {
-      C pair = rsum + __shfl_xor(rsum, 1, 64);
-      sred[ty][tx] = pair;
-      rsum = C(0);
-    }
-    __syncthreads();
-    if ((k & 1) && !(ty & 1) && !(tx & 1) && inside) {
-      C s = sred[ty][tx] + sred[ty + 1][tx];
-      bc[idx3(dc, gx >> 1, gy >> 1, k >> 1)] = store_as<Sc, C>(s * C(0.125));
-    }
+  for (int k = k0; k < k1; k += 2) {
+    plane(k, A, std::false_type{});
+    plane(k + 1, B, std::true_type{});
   }
 }
 
@@ -476,21 +479,8 @@ k_rbgs_pass(Dims d, S *__restrict__ x, const S *__restrict__ b,
 #define SW_RP ((SW_TX + 2) / 2 * (SW_TY + 2))           // 594 red pairs
 #define SW_RPT ((SW_RP + SW_THREADS - 1) / SW_THREADS)  // 2
 
-template <class S> struct vec2_of;
-template <> struct vec2_of<float> { typedef float2 type; };
-template <> struct vec2_of<double> { typedef double2 type; };
-
 __device__ inline float fast_rcp(float x) { return __builtin_amdgcn_rcpf(x); }
 __device__ inline double fast_rcp(double x) { return 1.0 / x; }
-
-// Blocks are numbered on a 1-D grid and remapped so that each of the 8 XCDs
-// (blocks are dealt round-robin to XCDs) gets a contiguous run of tiles: the
-// halo planes and rows a tile shares with its neighbours then sit in the
-// same XCD's L2.
-__device__ inline int xcd_remap(int b, int nblocks) {
-  if (nblocks % 8) return b;
-  return (b % 8) * (nblocks / 8) + b / 8;
-}
 
 // Red-black update of one cell, shared by the level-0 sweep kernels so that
 // one sweep and the two-sweep pass round identically: x' = (neighbour sum -
