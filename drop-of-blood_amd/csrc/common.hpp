@@ -181,6 +181,24 @@ __device__ inline T sample_tp(const T *__restrict__ f, const Dims &d, const TPos
   return mixv(mixv(x00, x10, t.fy), mixv(x01, x11, t.fy), t.fz);
 }
 
+// sample_tp that also gives the least and greatest of its 8 texels: the
+// bounds of the MacCormack limiter around the same point (the limiter takes
+// the same clamped corners, in this fmin/fmax order)
+template <class T>
+__device__ inline T sample_tp_mm(const T *__restrict__ f, const Dims &d, const TPos<T> &p, T &mn,
+                                 T &mx) {
+  TriIdx<T> t = tpos_index(d, p);
+  T s0 = f[t.r00 + t.i0], s1 = f[t.r00 + t.i1], s2 = f[t.r10 + t.i0], s3 = f[t.r10 + t.i1];
+  T s4 = f[t.r01 + t.i0], s5 = f[t.r01 + t.i1], s6 = f[t.r11 + t.i0], s7 = f[t.r11 + t.i1];
+  mn = fmin(fmin(fmin(s0, s1), fmin(s2, s3)), fmin(fmin(s4, s5), fmin(s6, s7)));
+  mx = fmax(fmax(fmax(s0, s1), fmax(s2, s3)), fmax(fmax(s4, s5), fmax(s6, s7)));
+  T x00 = mixv(s0, s1, t.fx);
+  T x10 = mixv(s2, s3, t.fx);
+  T x01 = mixv(s4, s5, t.fx);
+  T x11 = mixv(s6, s7, t.fx);
+  return mixv(mixv(x00, x10, t.fy), mixv(x01, x11, t.fy), t.fz);
+}
+
 template <class T>
 __device__ inline void sample_vec_tp(const typename vec4_of<T>::type *__restrict__ f,
                                      const Dims &d, const TPos<T> &p, T out[3]) {

@@ -64,9 +64,13 @@ struct Ctx {
   // cvel holds the centre velocity of the current u, v, w (set by a step's
   // last face-to-centre pass, cleared by anything else that may change them)
   bool cvel_fresh = false;
+  // dmm holds the limiter bounds of the dye and centre velocity of the last
+  // dye predict, and nothing has changed either since (resolve may use them)
+  bool dmm_valid = false;
   std::string err;
   // level-0 step fields
   Buf u, v, w, u2, v2, w2, p, dye, dye2, dpred, cvel, rhs;  // cvel: 4 per cell
+  Buf dmm;  // limiter bounds of the dye predict: 2 per cell
   // multigrid: x, b, r per level (level 0 x/b alias p/rhs except in mixed);
   // sweeps run out of place between mgx and its alternate mgxa
   Buf mgx[kMaxLev + 1], mgxa[kMaxLev + 1], mgb[kMaxLev + 1], mgr[kMaxLev + 1];
@@ -232,6 +236,7 @@ void axis_eigen(int n, bool lid, double topc, std::vector<double> &Q, std::vecto
 // kernel launchers
 int launch_face_to_centre(Ctx *ctx) {
   const Dims &d = ctx->d[0];
+  ctx->dmm_valid = false;  // the limiter bounds were traced on the old cvel
   ProfScope ps(ctx->prof, ctx->stream, KC_F2C);
   with_step_type(ctx->t0, [&](auto tag) {
     using T = std::remove_pointer_t<decltype(tag)>;
@@ -340,9 +345,11 @@ int launch_dye_predict(Ctx *ctx, double dt) {
     hipLaunchKernelGGL(k_dye_predict<T>, gather_grid(d), kGatherBlock, 0, ctx->stream, d,
                        (T)(dt / ctx->cfg.dx),
                        (const typename vec4_of<T>::type *)ctx->cvel.ptr,
-                       (const T *)ctx->dye.ptr, (T *)ctx->dpred.ptr);
+                       (const T *)ctx->dye.ptr, (T *)ctx->dpred.ptr,
+                       (typename vec2_of<T>::type *)ctx->dmm.ptr);
   });
   LAUNCHCHK();
+  ctx->dmm_valid = true;
   return MAC3D_OK;
 }
 
@@ -352,13 +359,21 @@ int launch_dye_resolve(Ctx *ctx, double dt) {
     ProfScope ps(ctx->prof, ctx->stream, KC_DYE);
     with_step_type(ctx->t0, [&](auto tag) {
       using T = std::remove_pointer_t<decltype(tag)>;
-      hipLaunchKernelGGL(k_dye_resolve<T>, gather_grid(d), kGatherBlock, 0, ctx->stream, d,
-                         (T)(dt / ctx->cfg.dx), (T)ctx->cfg.maccormack, (const typename vec4_of<T>::type *)ctx->cvel.ptr,
-                         (const T *)ctx->dye.ptr,
-                         (const T *)ctx->dpred.ptr, (T *)ctx->dye2.ptr);
+      const T dth = (T)(dt / ctx->cfg.dx), s = (T)ctx->cfg.maccormack;
+      const auto *cv = (const typename vec4_of<T>::type *)ctx->cvel.ptr;
+      if (ctx->dmm_valid)
+        hipLaunchKernelGGL(k_dye_resolve_mm<T>, gather_grid(d), kGatherBlock, 0, ctx->stream, d,
+                           dth, s, cv, (const T *)ctx->dye.ptr, (const T *)ctx->dpred.ptr,
+                           (const typename vec2_of<T>::type *)ctx->dmm.ptr,
+                           (T *)ctx->dye2.ptr);
+      else
+        hipLaunchKernelGGL(k_dye_resolve<T>, gather_grid(d), kGatherBlock, 0, ctx->stream, d,
+                           dth, s, cv, (const T *)ctx->dye.ptr, (const T *)ctx->dpred.ptr,
+                           (T *)ctx->dye2.ptr);
     });
   }
   LAUNCHCHK();
+  ctx->dmm_valid = false;
   std::swap(ctx->dye, ctx->dye2);
   return MAC3D_OK;
 }
@@ -448,9 +463,14 @@ int mg_fill_zero(Ctx *ctx, Buf &b) {
 // (the LDS of two fills a CU), at least 6 planes (each chunk pays seven
 // partial iterations at its start), even (the kernel's unrolled halves have
 // fixed plane parities), and a chunk's buffer ranges below 2^30 bytes
-int sweep2_cz(const Dims &d) {
-  int cols = ((d.nx + TB_TX - 1) / TB_TX) * ((d.ny + TB_TY - 1) / TB_TY);
-  int chunks = std::max(1, 512 / cols);
+// Tile width of the two-sweep pass: 32 on grids of up to 2^18 cells (twice
+// the blocks of a 64-wide tile, measured faster on 48x64x48), else 64.
+static int sweep2_tx(const Dims &d) { return d.cells() <= ((size_t)1 << 18) ? 32 : 64; }
+#define SWEEP2_TY 16
+
+int sweep2_cz(const Dims &d, int tx) {
+  int cols = ((d.nx + tx - 1) / tx) * ((d.ny + SWEEP2_TY - 1) / SWEEP2_TY);
+  int chunks = std::max(1, (64 * 512 / tx) / cols);
   // grids of up to 2^18 cells: 2-plane chunks (measured fastest on 48x64x48)
   if ((size_t)d.nx * d.ny * d.nz <= ((size_t)1 << 18)) return 2;
   int cz = std::max((d.nz + chunks - 1) / chunks, std::min(d.nz, 6));
@@ -464,13 +484,17 @@ int sweep2_cz(const Dims &d) {
 // then the two swap roles.  zero: the first sweep starts from a zero guess.
 int mg_sweep2(Ctx *ctx, int l, bool zero) {
   const Dims &d = ctx->d[l];
-  int cz = sweep2_cz(d);
-  int ncx = (d.nx + TB_TX - 1) / TB_TX, ncy = (d.ny + TB_TY - 1) / TB_TY;
+  const int tx = sweep2_tx(d);
+  int cz = sweep2_cz(d, tx);
+  int ncx = (d.nx + tx - 1) / tx, ncy = (d.ny + SWEEP2_TY - 1) / SWEEP2_TY;
   dim3 grid(ncx * ncy * ((d.nz + cz - 1) / cz));
   {
     ProfScope ps(ctx->prof, ctx->stream, l == 0 ? KC_SMOOTH0X2 : KC_MG);
-    auto kern = zero ? k_rbgs_sweep2<true> : k_rbgs_sweep2<false>;
-    hipLaunchKernelGGL(kern, grid, dim3(TB_T), 0, ctx->stream, d,
+    auto kern = tx == 64 ? (zero ? k_rbgs_sweep2<true, 64, SWEEP2_TY>
+                                 : k_rbgs_sweep2<false, 64, SWEEP2_TY>)
+                         : (zero ? k_rbgs_sweep2<true, 32, SWEEP2_TY>
+                                 : k_rbgs_sweep2<false, 32, SWEEP2_TY>);
+    hipLaunchKernelGGL(kern, grid, dim3(tx / 2 * SWEEP2_TY), 0, ctx->stream, d,
                        (const float *)ctx->mgx[l].ptr, (float *)ctx->mgxa[l].ptr,
                        (const float *)ctx->mgb[l].ptr, (float)level_h2(l),
                        (float)(l == 0 ? 1.0 : top_coeff(l)), (float)ctx->omega,
@@ -972,6 +996,7 @@ int create_impl(Ctx *ctx, const mac3d_config *cfg) {
   for (Buf *b : f0)
     if ((rc = alloc(ctx, *b, t0, n0))) return rc;
   if ((rc = alloc(ctx, ctx->cvel, t0, 4 * n0))) return rc;
+  if ((rc = alloc(ctx, ctx->dmm, t0, 2 * n0))) return rc;
   bool mixed = ctx->prec == MAC3D_PREC_MIXED;
   ctx->omega = cfg->sor_omega > 0.0 ? cfg->sor_omega : (mixed ? 1.2 : 1.0);
   ctx->extrap = mixed ? (cfg->extrapolate < 0 ? 2 : std::min(cfg->extrapolate, 2)) : 0;
@@ -1056,6 +1081,7 @@ int reset_scale(Ctx *ctx) {
 
 int init_blobs_impl(Ctx *ctx, const mac3d_blob *blobs, int nblobs) {
   ctx->cvel_fresh = false;
+  ctx->dmm_valid = false;
   ctx->hist = 0;
   if (nblobs < 0 || (nblobs > 0 && !blobs)) return fail(ctx, MAC3D_EINVAL, "bad blob list");
   for (int q = 0; q < nblobs; ++q) {
@@ -1155,6 +1181,7 @@ int save_state(Ctx *ctx, void *const *dst) {
 
 int load_state(Ctx *ctx, void *const *src) {
   ctx->cvel_fresh = false;
+  ctx->dmm_valid = false;
   ctx->hist = 0;
   for (int f = 0; f < 5; ++f)
     HIPCHK(hipMemcpyAsync(state_field(ctx, f), src[f], state_bytes(ctx, f),
@@ -1468,6 +1495,7 @@ int mac3d_upload(void *h, int field, int level, const void *host, int dtype, siz
   if (!h) return MAC3D_EINVAL;
   Ctx *ctx = (Ctx *)h;
   ctx->cvel_fresh = false;
+  ctx->dmm_valid = false;
   Buf *b = field_buf(ctx, field, level);
   if (!b || !host) return fail(ctx, MAC3D_EINVAL, "unknown field/level or null host pointer");
   if (dtype != MAC3D_F32 && dtype != MAC3D_F64) return fail(ctx, MAC3D_EINVAL, "unknown dtype");
@@ -1608,6 +1636,7 @@ int mac3d_adjoint(void *h, const mac3d_blob *blobs, int nblobs, double dt, int n
   if (dtype != MAC3D_F32 && dtype != MAC3D_F64) return fail(ctx, MAC3D_EINVAL, "unknown dtype");
   int rc = adjoint_impl(ctx, blobs, nblobs, dt, nsteps, checkpoint_every, target, dtype, out);
   ctx->cvel_fresh = false;  // the reverse sweep reuses the work fields
+  ctx->dmm_valid = false;
   ctx->prof.collect();
   return rc;
 }

@@ -690,18 +690,20 @@ k_rbgs_sweep(Dims d, const S *__restrict__ xin, S *__restrict__ xout, const S *_
 
 
 // ---------------------------------------------------------------------------
-// Level-0 smoothing, two red-black sweeps in one pass (temporal blocking):
+// Two red-black sweeps in one pass (temporal blocking), FP32 levels:
 // R1, B1 (sweep 1) and R2, B2 (sweep 2) march through z together, so x and b
 // are read once and x written once for two sweeps (12 B per FP32 cell per
-// pass, against 24 for two passes of k_rbgs_sweep).  A 512-thread block owns
-// a 64 x 16 column tile and a z-chunk [k0, k1).  Each colour stage is
+// pass, against 24 for two passes of k_rbgs_sweep).  A block of TX / 2 * TY
+// threads (one tile pair each) owns a TX x TY column tile and a z-chunk
+// [k0, k1): 64 x 16, or 32 x 16 on small grids (twice the blocks).  Each
+// colour stage is
 // recomputed on a ring one cell wider than the next stage reads (R1 on the
 // tile plus three cells, B1 plus two, R2 plus one, B2 on the tile), so no
 // block waits on another and the result is exactly two sweeps, with the cell
 // arithmetic of k_rbgs_sweep (rb_cell).
 //
-// LDS holds planes of x in a 7-slot ring (tile plus four cells, 72 x 24) and
-// b in a 5-slot ring (tile plus three cells, 72 x 22), each plane split by
+// LDS holds planes of x in a 7-slot ring (tile plus four cells, 72 x 24 for
+// 64 x 16) and b in a 5-slot ring (tile plus three cells, 72 x 22), each plane split by
 // colour: [colour][row][pair], pair = (even, odd) x-pair.  A cell's six
 // neighbours are of the other colour: x-/x+ at pairs P-1+s and P+s of its row
 // (s: the cell's side in its pair), y and z at pair P, so every stage reads
@@ -720,25 +722,21 @@ k_rbgs_sweep(Dims d, const S *__restrict__ xin, S *__restrict__ xout, const S *_
 // ring and the grid.  ZERO: zero initial guess (x is not read, zeros are
 // staged).  scale (nullable): the right-hand side is b scale[0].  FP32 storage
 // only (an FP64 ring would not fit two blocks' LDS).
-#define TB_TX 64
-#define TB_TY 16
-#ifndef TB_T
-#define TB_T (TB_TX / 2 * TB_TY)   // 512 threads: one tile pair each
-#endif
-#define TB_PW ((TB_TX + 8) / 2)    // pairs per staged row: four-cell ring
-#define TB_EY (TB_TY + 8)
-#define TB_BY (TB_TY + 6)          // b rows: three-cell ring
-#define TB_CH (TB_EY * TB_PW)      // one colour of an x slot
-#define TB_BCH (TB_BY * TB_PW)     // one colour of a b slot
 #define TB_XR 7
 #define TB_BR 5
-#define TB_NQ ((TB_PW * TB_EY + TB_T - 1) / TB_T)  // pairs per thread in the staging and R1
 
-template <bool ZERO>
-__global__ void __launch_bounds__(TB_T)
+template <bool ZERO, int TX, int TY>
+__global__ void __launch_bounds__(TX / 2 * TY)
 k_rbgs_sweep2(Dims d, const float *__restrict__ xin, float *__restrict__ xout,
               const float *__restrict__ b, float h2, float topc, float omega,
               const double *__restrict__ scale, int cz, int ncx, int ncy) {
+  constexpr int TB_TX = TX, TB_TY = TY;
+  constexpr int TB_T = TX / 2 * TY;
+  constexpr int TB_PW = (TX + 8) / 2;   // pairs per staged row: four-cell ring
+  constexpr int TB_EY = TY + 8;
+  constexpr int TB_BY = TY + 6;         // b rows: three-cell ring
+  constexpr int TB_CH = TB_EY * TB_PW;  // one colour of an x slot
+  constexpr int TB_BCH = TB_BY * TB_PW; // one colour of a b slot
   typedef float C;
   typedef float2 V2;
   static_assert(TB_T == TB_TX / 2 * TB_TY, "one tile pair per thread");
@@ -824,9 +822,9 @@ k_rbgs_sweep2(Dims d, const float *__restrict__ xin, float *__restrict__ xout,
   for (int q = 0; q < 2; ++q) bits |= (unsigned)((y0 - 4 + (tid + q * TB_T) / TB_PW) & 1) << (28 + q);
   if (tid + TB_T < TB_PW * TB_EY) bits |= 1u << 30;
   if (tid + TB_T < TB_PW * TB_BY) bits |= 1u << 31;
-  // Second pairs of R1, B1 and R2 exist for the first 280, 168 and 100
-  // threads only: whole waves past them skip those updates (a scalar branch;
-  // their lanes would write back old values).
+  // Second pairs of R1, B1 and R2 exist for the first threads only (280, 168
+  // and 100 of 512 on a 64 x 16 tile): whole waves past them skip those
+  // updates (a scalar branch; their lanes would write back old values).
   const int wb = __builtin_amdgcn_readfirstlane(tid & ~63) + TB_T;
   const bool w2 = wb < TB_PW * TB_BY;
   const bool w4 = wb < (TB_TX / 2 + 2) * (TB_TY + 4);

@@ -414,16 +414,27 @@ __device__ inline void dye_bounds(const Dims &d, T lo[3], T hi[3]) {
   }
 }
 
+// CONC_PREDICT (advect.ts:28-72): the dye sampled at the backtrace point.  mm
+// (nullable) takes the least and greatest of the 8 dye texels of that sample
+// per cell, which are the bounds CONC_RESOLVE's limiter clamps to, so that
+// k_dye_resolve_mm need not trace back again.
 template <class T>
 __global__ void __launch_bounds__(256)
 k_dye_predict(Dims d, T dth, const typename vec4_of<T>::type *__restrict__ cvel,
-              const T *__restrict__ dye, T *__restrict__ pred) {
+              const T *__restrict__ dye, T *__restrict__ pred,
+              typename vec2_of<T>::type *__restrict__ mm) {
   MAC3D_CELL_PROLOGUE
   T lo[3], hi[3], v0[3];
   dye_bounds(d, lo, hi);
   TPos<T> back;
   dye_back_tp<T>(d, cvel, dth, i, j, k, lo, hi, v0, back);
-  pred[c] = sample_tp(dye, d, back);
+  if (mm) {
+    typename vec2_of<T>::type b;
+    pred[c] = sample_tp_mm(dye, d, back, b.x, b.y);
+    mm[c] = b;
+  } else {
+    pred[c] = sample_tp(dye, d, back);
+  }
 }
 
 // CONC_RESOLVE's MacCormack core (advect.ts:196-222): forward estimate from
@@ -457,6 +468,29 @@ k_dye_resolve(Dims d, T dth, T strength, const typename vec4_of<T>::type *__rest
   T mn = fmin(fmin(fmin(s0, s1), fmin(s2, s3)), fmin(fmin(s4, s5), fmin(s6, s7)));
   T mx = fmax(fmax(fmax(s0, s1), fmax(s2, s3)), fmax(fmax(s4, s5), fmax(s6, s7)));
   out[c] = clampv(corr, mn, mx);
+}
+
+// k_dye_resolve with the limiter's bounds from k_dye_predict (mm, formed from
+// the same dye and centre velocity): the same value with no second backtrace.
+template <class T>
+__global__ void __launch_bounds__(256)
+k_dye_resolve_mm(Dims d, T dth, T strength, const typename vec4_of<T>::type *__restrict__ cvel,
+                 const T *__restrict__ dye, const T *__restrict__ pred,
+                 const typename vec2_of<T>::type *__restrict__ mm, T *__restrict__ out) {
+  MAC3D_CELL_PROLOGUE
+  T lo[3], hi[3];
+  dye_bounds(d, lo, hi);
+  const auto a0 = cvel[c];
+  TPos<T> fw;
+  fw.b[0] = i; fw.b[1] = j; fw.b[2] = k;
+  fw.o[0] = a0.x * dth;
+  fw.o[1] = a0.y * dth;
+  fw.o[2] = a0.z * dth;
+  tpos_clamp(fw, lo, hi);
+  T est = sample_tp(pred, d, fw);
+  T corr = pred[c] + strength * (dye[c] - est);
+  const auto b = mm[c];
+  out[c] = clampv(corr, b.x, b.y);
 }
 
 // Seeded blob initial condition: dye at cell centres and -w0 on V faces,

@@ -71,6 +71,26 @@ def test_dye_stages(prec, tol):
         _cmp(s, o, ("DYE",), tol)
 
 
+@pytest.mark.parametrize("prec,tol", [(0, TOL32), (2, TOL64)])
+def test_dye_resolve_after_dye_change(prec, tol):
+    """The resolve stage after the dye changed since the predict (an upload
+    in between): the library drops the limiter bounds the predict formed and
+    traces back again, as the oracle's resolve does on the current dye."""
+    w, s, o = pair(precision=prec)
+    with s:
+        o.stage(oracle.ST_F2C)
+        s.stage("FACE_TO_CENTRE")
+        o.stage(oracle.ST_DYE_PRED, w.dt)
+        s.stage("DYE_PREDICT", w.dt)
+        dye = o.get(oracle.F_DYE)
+        dye = dye * 0.5 + np.roll(dye, 3, axis=2) * 0.5
+        o.set(oracle.F_DYE, dye)
+        s.upload("DYE", dye)
+        o.stage(oracle.ST_DYE_RES, w.dt)
+        s.stage("DYE_RESOLVE", w.dt)
+        _cmp(s, o, ("DYE",), tol)
+
+
 @pytest.mark.parametrize("prec,tol", [(0, 2e-6), (2, TOL64)])
 @pytest.mark.parametrize("level", [0, 1, 2])
 def test_mg_smooth_residual_restrict_prolong(level, prec, tol):
