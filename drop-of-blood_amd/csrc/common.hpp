@@ -237,6 +237,25 @@ template <> __device__ inline float2 buf_load<float2>(BufRsrc r, int off) {
 template <> __device__ inline double2 buf_load<double2>(BufRsrc r, int off) {
   return __builtin_bit_cast(double2, __builtin_amdgcn_raw_buffer_load_b128(r, off, 0, 0));
 }
+
+// An (even, odd) x-pair of S cells as float2 (FP16 widens exactly), and its
+// store from float2 (FP16 as store_as rounds it).
+typedef half_t half2v __attribute__((ext_vector_type(2)));
+template <class S> __device__ inline float2 buf_load_pair(BufRsrc r, int off);
+template <> __device__ inline float2 buf_load_pair<float>(BufRsrc r, int off) {
+  return buf_load<float2>(r, off);
+}
+template <> __device__ inline float2 buf_load_pair<half_t>(BufRsrc r, int off) {
+  half2v h = __builtin_bit_cast(half2v, __builtin_amdgcn_raw_buffer_load_b32(r, off, 0, 0));
+  return make_float2((float)h.x, (float)h.y);
+}
+__device__ inline void store_pair(float *p, float2 v) { *(float2 *)p = v; }
+__device__ inline void store_pair(half_t *p, float2 v) {
+  half2v h;
+  h.x = store_as<half_t, float>(v.x);
+  h.y = store_as<half_t, float>(v.y);
+  *(half2v *)p = h;
+}
 // in-plane byte offset of a staged position outside the grid: with every
 // range below 2^30 bytes, it stays negative (out of range) for any plane
 // offset the marching kernels add to it

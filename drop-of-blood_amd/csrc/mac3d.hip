@@ -30,6 +30,8 @@ namespace {
 
 enum { TF16 = 0, TF32 = 1, TF64 = 2 };
 const int kMaxLev = 8;
+// refinement iterations a mixed step may run before its first convergence test
+const int kMaxPreIters = 15;
 
 size_t type_size(int t) { return t == TF16 ? 2 : (t == TF32 ? 4 : 8); }
 
@@ -81,6 +83,10 @@ struct Ctx {
   Buf pr[4];
   int role[4] = {0, 1, 2, 3};
   int hist = 0;
+  // refinement iterations of the last few mixed steps (iter_n of them held):
+  // a step runs the fewest of them before its first convergence test
+  int iter_log[4] = {0, 0, 0, 0};
+  int iter_n = 0;
   double omega = 1.0;
   int extrap = 0;
   // right-hand-side scale of level 0: [0] = multiplier, [1] = its inverse
@@ -480,7 +486,7 @@ int sweep2_cz(const Dims &d, int tx) {
   return cz;
 }
 
-// Two red-black sweeps on level l in one pass (FP32 storage): mgx -> mgxa,
+// Two red-black sweeps on level l in one pass (FP32 or FP16 storage): mgx -> mgxa,
 // then the two swap roles.  zero: the first sweep starts from a zero guess.
 int mg_sweep2(Ctx *ctx, int l, bool zero) {
   const Dims &d = ctx->d[l];
@@ -490,29 +496,35 @@ int mg_sweep2(Ctx *ctx, int l, bool zero) {
   dim3 grid(ncx * ncy * ((d.nz + cz - 1) / cz));
   {
     ProfScope ps(ctx->prof, ctx->stream, l == 0 ? KC_SMOOTH0X2 : KC_MG);
-    auto kern = tx == 64 ? (zero ? k_rbgs_sweep2<true, 64, SWEEP2_TY>
-                                 : k_rbgs_sweep2<false, 64, SWEEP2_TY>)
-                         : (zero ? k_rbgs_sweep2<true, 32, SWEEP2_TY>
-                                 : k_rbgs_sweep2<false, 32, SWEEP2_TY>);
-    hipLaunchKernelGGL(kern, grid, dim3(tx / 2 * SWEEP2_TY), 0, ctx->stream, d,
-                       (const float *)ctx->mgx[l].ptr, (float *)ctx->mgxa[l].ptr,
-                       (const float *)ctx->mgb[l].ptr, (float)level_h2(l),
-                       (float)(l == 0 ? 1.0 : top_coeff(l)), (float)ctx->omega,
-                       l == 0 ? (const double *)ctx->scale : nullptr, cz, ncx, ncy);
+    auto launch = [&](auto stag) {
+      using S = std::remove_pointer_t<decltype(stag)>;
+      auto kern = tx == 64 ? (zero ? k_rbgs_sweep2<true, 64, SWEEP2_TY, S>
+                                   : k_rbgs_sweep2<false, 64, SWEEP2_TY, S>)
+                           : (zero ? k_rbgs_sweep2<true, 32, SWEEP2_TY, S>
+                                   : k_rbgs_sweep2<false, 32, SWEEP2_TY, S>);
+      hipLaunchKernelGGL(kern, grid, dim3(tx / 2 * SWEEP2_TY), 0, ctx->stream, d,
+                         (const S *)ctx->mgx[l].ptr, (S *)ctx->mgxa[l].ptr,
+                         (const S *)ctx->mgb[l].ptr, (float)level_h2(l),
+                         (float)(l == 0 ? 1.0 : top_coeff(l)), (float)ctx->omega,
+                         l == 0 ? (const double *)ctx->scale : nullptr, cz, ncx, ncy);
+    };
+    if (ctx->mgx[l].type == TF16) launch((half_t *)nullptr);
+    else launch((float *)nullptr);
   }
   LAUNCHCHK();
   swap_x(ctx, l);
   return MAC3D_OK;
 }
 
-// sweeps red-black sweeps on level l.  With FP32 storage, pairs of sweeps
-// run as one two-sweep pass on level 0 and on coarser levels of at least
-// 2^17 cells (below that four colour-pass launches are cheaper).
+// sweeps red-black sweeps on level l.  With FP32 or FP16 storage, pairs of
+// sweeps run as one two-sweep pass on level 0 and on coarser levels of at
+// least 2^17 cells (below that four colour-pass launches are cheaper).
 int mg_smooth(Ctx *ctx, int l, int sweeps, bool zero) {
   int rc;
   if (zero && sweeps == 0) return mg_fill_zero(ctx, ctx->mgx[l]);
   int s = 0;
-  if (ctx->mgx[l].type == TF32 && (l == 0 || ctx->n[l] >= ((size_t)1 << 17)))
+  if ((ctx->mgx[l].type == TF32 || ctx->mgx[l].type == TF16) &&
+      (l == 0 || ctx->n[l] >= ((size_t)1 << 17)))
     for (; s + 2 <= sweeps; s += 2)
       if ((rc = mg_sweep2(ctx, l, zero && s == 0))) return rc;
   for (; s < sweeps; ++s)
@@ -706,19 +718,20 @@ R64Grid r64_grid(const Dims &d) {
   return g;
 }
 
-int fold_norms(Ctx *ctx) {
+// The residual norms (max |r|, ||r||^2, ||b||^2) go to host slot `slot`.
+int fold_norms(Ctx *ctx, int slot) {
   hipLaunchKernelGGL(k_fold3, dim3(1), dim3(256), 0, ctx->stream, (const double *)ctx->part,
                      (size_t)r64_grid(ctx->d[0]).blocks, ctx->red, ctx->scale);
   LAUNCHCHK();
-  HIPCHK(hipMemcpyAsync(ctx->red_host, ctx->red, 3 * sizeof(double), hipMemcpyDeviceToHost,
-                        ctx->stream));
+  HIPCHK(hipMemcpyAsync(ctx->red_host + 3 * slot, ctx->red, 3 * sizeof(double),
+                        hipMemcpyDeviceToHost, ctx->stream));
   return MAC3D_OK;
 }
 
 // First residual of a step: the warm start extrapolated from the pressure
 // history (order min(extrap, hist)), formed into the spare buffer, which then
 // becomes current while the history shifts.  order 0 restarts from p.
-int mixed_begin(Ctx *ctx, int order) {
+int mixed_begin(Ctx *ctx, int order, int slot = 0) {
   const Dims &d = ctx->d[0];
   double c0 = 1.0, c1 = 0.0, c2 = 0.0;
   if (order >= 2) { c0 = 3.0; c1 = -3.0; c2 = 1.0; }
@@ -737,13 +750,13 @@ int mixed_begin(Ctx *ctx, int order) {
   int nr[4] = {r[3], r[0], r[1], r[2]};
   for (int q = 0; q < 4; ++q) r[q] = nr[q];
   sync_p(ctx);
-  return fold_norms(ctx);
+  return fold_norms(ctx, slot);
 }
 
 // One refinement iteration: correction V-cycle on r/max|r| from zero, then
 // p <- p + max|r| e formed into the spare buffer together with the new
 // residual; spare and current swap.
-int mixed_iter(Ctx *ctx) {
+int mixed_iter(Ctx *ctx, int slot = 0) {
   int rc;
   if ((rc = vcycle(ctx, true))) return rc;
   const Dims &d = ctx->d[0];
@@ -760,7 +773,7 @@ int mixed_iter(Ctx *ctx) {
   LAUNCHCHK();
   std::swap(r[0], r[3]);
   sync_p(ctx);
-  return fold_norms(ctx);
+  return fold_norms(ctx, slot);
 }
 
 int div_norms(Ctx *ctx, double *mx, double *l2) {
@@ -890,15 +903,34 @@ int one_step(Ctx *ctx, double dt, int *cycles, int *iters, double *rel, int *con
     return MAC3D_OK;
   }
   int order = std::min(ctx->extrap, ctx->hist);
+  // The first convergence test waits for the host; the iterations every
+  // recent step needed run before it, in the same sequence, so that a step
+  // stops for the host about once instead of once per iteration.  The norms
+  // after j of them go to host slot j, so the step still learns how many it
+  // needed (one that would have converged sooner refines a little further).
+  int pre_it = 0;
+  if (ctx->iter_n > 0) {
+    pre_it = ctx->iter_log[0];
+    for (int q = 1; q < ctx->iter_n; ++q) pre_it = std::min(pre_it, ctx->iter_log[q]);
+    pre_it = std::min(std::min(pre_it, ctx->cfg.ir_max_iters), kMaxPreIters);
+  }
   rc = run_seq(ctx, SEQ_PRE, dt, [&]() {
     int r;
     if ((r = seq_pre(ctx, dt))) return r;
-    return mixed_begin(ctx, order);
-  }, order | (ctx->cvel_fresh ? 4 : 0));
+    if ((r = mixed_begin(ctx, order, 0))) return r;
+    for (int q = 0; q < pre_it; ++q)
+      if ((r = mixed_iter(ctx, q + 1))) return r;
+    return MAC3D_OK;
+  }, order | (ctx->cvel_fresh ? 4 : 0) | (pre_it << 3));
   if (rc) return rc;
-  int it = 0;
+  auto converged = [&](const double *h, double *rr) {
+    double r = h[2] > 0.0 ? std::sqrt(h[1] / h[2]) : 0.0;
+    if (rr) *rr = r;
+    return r <= ctx->cfg.ir_tol || h[0] == 0.0;
+  };
+  int it = pre_it, needed = -1;
   *conv = 0;
-  for (;;) {
+  for (bool first = true;; first = false) {
     // the next launch waits on this test: poll for the norms rather than
     // block in a stream synchronize, which may sleep before it wakes
     HIPCHK(hipEventRecord(ctx->norms_ready, ctx->stream));
@@ -906,17 +938,26 @@ int one_step(Ctx *ctx, double dt, int *cycles, int *iters, double *rel, int *con
     while ((q = hipEventQuery(ctx->norms_ready)) == hipErrorNotReady) {
     }
     if (q != hipSuccess) return fail(ctx, MAC3D_EHIP, std::string("norms: ") + hipGetErrorString(q));
-    double mr = ctx->red_host[0], rr = ctx->red_host[1], bb = ctx->red_host[2];
-    double r = bb > 0.0 ? std::sqrt(rr / bb) : 0.0;
-    *rel = r;
-    if (r <= ctx->cfg.ir_tol || mr == 0.0) { *conv = 1; break; }
+    if (first) {
+      for (int j = 0; j <= pre_it && needed < 0; ++j)
+        if (converged(ctx->red_host + 3 * j, nullptr)) needed = j;
+      converged(ctx->red_host + 3 * pre_it, rel);
+      if (needed >= 0) { *conv = 1; break; }
+    } else if (converged(ctx->red_host, rel)) {
+      needed = it;
+      *conv = 1;
+      break;
+    }
     if (it >= ctx->cfg.ir_max_iters) break;
-    rc = run_seq(ctx, SEQ_ITER, 0.0, [&]() { return mixed_iter(ctx); });
+    rc = run_seq(ctx, SEQ_ITER, 0.0, [&]() { return mixed_iter(ctx, 0); });
     if (rc) return rc;
     ++it;
   }
   *cycles += it;
   *iters = it;
+  for (int q = 3; q > 0; --q) ctx->iter_log[q] = ctx->iter_log[q - 1];
+  ctx->iter_log[0] = needed >= 0 ? needed : it;
+  ctx->iter_n = std::min(ctx->iter_n + 1, 4);
   ctx->hist = std::min(ctx->hist + 1, 2);
   rc = run_seq(ctx, SEQ_POST, dt, [&]() { return seq_post(ctx, dt); });
   if (!rc) ctx->cvel_fresh = true;
@@ -1047,7 +1088,8 @@ int create_impl(Ctx *ctx, const mac3d_config *cfg) {
   if ((rc = dev_alloc(ctx, (void **)&ctx->part, 3 * ctx->part_n * sizeof(double)))) return rc;
   if ((rc = dev_alloc(ctx, (void **)&ctx->red, 4 * sizeof(double)))) return rc;
   if ((rc = dev_alloc(ctx, (void **)&ctx->scale, 2 * sizeof(double)))) return rc;
-  HIPCHK(hipHostMalloc(&ctx->red_host, 4 * sizeof(double), hipHostMallocDefault));
+  HIPCHK(hipHostMalloc(&ctx->red_host, 3 * (kMaxPreIters + 1) * sizeof(double),
+                       hipHostMallocDefault));
   if ((rc = dev_alloc(ctx, (void **)&ctx->stage, n0 * sizeof(double)))) return rc;
   hipLaunchKernelGGL(k_fill<double>, dim3(1), dim3(256), 0, ctx->stream, ctx->scale, (size_t)2, 1.0);
   LAUNCHCHK();
@@ -1083,6 +1125,7 @@ int init_blobs_impl(Ctx *ctx, const mac3d_blob *blobs, int nblobs) {
   ctx->cvel_fresh = false;
   ctx->dmm_valid = false;
   ctx->hist = 0;
+  ctx->iter_n = 0;
   if (nblobs < 0 || (nblobs > 0 && !blobs)) return fail(ctx, MAC3D_EINVAL, "bad blob list");
   for (int q = 0; q < nblobs; ++q) {
     const mac3d_blob &b = blobs[q];
@@ -1183,6 +1226,7 @@ int load_state(Ctx *ctx, void *const *src) {
   ctx->cvel_fresh = false;
   ctx->dmm_valid = false;
   ctx->hist = 0;
+  ctx->iter_n = 0;
   for (int f = 0; f < 5; ++f)
     HIPCHK(hipMemcpyAsync(state_field(ctx, f), src[f], state_bytes(ctx, f),
                           hipMemcpyDeviceToDevice, ctx->stream));
@@ -1502,7 +1546,10 @@ int mac3d_upload(void *h, int field, int level, const void *host, int dtype, siz
   int comp = (field >= MAC3D_FIELD_CU && field <= MAC3D_FIELD_CW) ? field - MAC3D_FIELD_CU : -1;
   if (count != (comp >= 0 ? ctx->n[0] : b->n))
     return fail(ctx, MAC3D_EINVAL, "element count does not match the field");
-  if (field == MAC3D_FIELD_P) ctx->hist = 0;  // an uploaded pressure restarts the history
+  if (field == MAC3D_FIELD_P) {  // an uploaded pressure restarts the history
+    ctx->hist = 0;
+    ctx->iter_n = 0;
+  }
   size_t hsz = dtype == MAC3D_F32 ? 4 : 8;
   HIPCHK(hipMemcpyAsync(ctx->stage, host, hsz * count, hipMemcpyHostToDevice, ctx->stream));
   if (comp >= 0) {

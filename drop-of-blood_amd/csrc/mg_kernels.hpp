@@ -720,15 +720,17 @@ k_rbgs_sweep(Dims d, const S *__restrict__ xin, S *__restrict__ xout, const S *_
 // past the chunk) load as zeros with no branch.  Each stage takes the pairs
 // covering its ring and updates its colour's cell when that cell lies in the
 // ring and the grid.  ZERO: zero initial guess (x is not read, zeros are
-// staged).  scale (nullable): the right-hand side is b scale[0].  FP32 storage
-// only (an FP64 ring would not fit two blocks' LDS).
+// staged).  scale (nullable): the right-hand side is b scale[0].  Storage S:
+// FP32 or FP16 (widened to FP32 as it is staged; the stages compute and keep
+// FP32 in LDS and only the written plane is rounded to FP16); an FP64 ring
+// would not fit two blocks' LDS.
 #define TB_XR 7
 #define TB_BR 5
 
-template <bool ZERO, int TX, int TY>
+template <bool ZERO, int TX, int TY, class S>
 __global__ void __launch_bounds__(TX / 2 * TY)
-k_rbgs_sweep2(Dims d, const float *__restrict__ xin, float *__restrict__ xout,
-              const float *__restrict__ b, float h2, float topc, float omega,
+k_rbgs_sweep2(Dims d, const S *__restrict__ xin, S *__restrict__ xout,
+              const S *__restrict__ b, float h2, float topc, float omega,
               const double *__restrict__ scale, int cz, int ncx, int ncy) {
   constexpr int TB_TX = TX, TB_TY = TY;
   constexpr int TB_T = TX / 2 * TY;
@@ -750,7 +752,7 @@ k_rbgs_sweep2(Dims d, const float *__restrict__ xin, float *__restrict__ xout,
   const int k1 = min(k0 + cz, nz);
   const int tid = threadIdx.x;
   const C bs = scale ? (C)scale[0] : C(1);
-  const int pb = (int)((size_t)nx * ny * sizeof(C));
+  const int pb = (int)((size_t)nx * ny * sizeof(S));
 
   // pairs of the x staging (tile plus four cells) and of R1 (rows of the
   // three-cell ring, the same columns): in-plane byte offsets
@@ -761,10 +763,10 @@ k_rbgs_sweep2(Dims d, const float *__restrict__ xin, float *__restrict__ xout,
     int row = v / TB_PW, pc = v % TB_PW;
     int gx = x0 - 4 + 2 * pc, gy = y0 - 4 + row;
     xo[q] = (v < TB_PW * TB_EY && gx >= 0 && gx < nx && gy >= 0 && gy < ny)
-                ? (gy * nx + gx) * (int)sizeof(C) : MAC3D_OFF_NONE;
+                ? (gy * nx + gx) * (int)sizeof(S) : MAC3D_OFF_NONE;
     gy += 1;
     bo[q] = (v < TB_PW * TB_BY && gx >= 0 && gx < nx && gy >= 0 && gy < ny)
-                ? (gy * nx + gx) * (int)sizeof(C) : MAC3D_OFF_NONE;
+                ? (gy * nx + gx) * (int)sizeof(S) : MAC3D_OFF_NONE;
   }
   const int kx0 = max(k0 - 4, 0), kx1 = min(k1 + 4, nz);
   const int kb0 = max(k0 - 3, 0), kb1 = min(k1 + 3, nz);
@@ -772,7 +774,7 @@ k_rbgs_sweep2(Dims d, const float *__restrict__ xin, float *__restrict__ xout,
   const BufRsrc rb = buf_rsrc(b + (size_t)kb0 * nx * ny, (uint32_t)(kb1 - kb0) * pb);
   auto fetch = [&](BufRsrc r, int kbase, int k, const int (&off)[2], V2 (&pre)[2]) {
 #pragma unroll
-    for (int q = 0; q < 2; ++q) pre[q] = buf_load<V2>(r, off[q] + (k - kbase) * pb);
+    for (int q = 0; q < 2; ++q) pre[q] = buf_load_pair<S>(r, off[q] + (k - kbase) * pb);
   };
 
   // ---- per-thread tables, fixed over the march ----------------------------
@@ -832,7 +834,7 @@ k_rbgs_sweep2(Dims d, const float *__restrict__ xin, float *__restrict__ xout,
   const int tr = tid / (TB_TX / 2), tpc = tid % (TB_TX / 2);
   const int tgy = y0 + tr, tgx = x0 + 2 * tpc;
   const bool tin = tgx < nx && tgy < ny;
-  float *const tout = xout + (size_t)tgy * nx + tgx;
+  S *const tout = xout + (size_t)tgy * nx + tgx;
 
   // ---- LDS addressing -------------------------------------------------------
   // Slot byte bases: x slot of colour X at XS + X * TB_CH * 4, b slot of
@@ -911,7 +913,7 @@ k_rbgs_sweep2(Dims d, const float *__restrict__ xin, float *__restrict__ xout,
         V2 o;
         o.x = pick(rt, vr, vb);
         o.y = pick(rt, vb, vr);
-        *(V2 *)(tout + (size_t)j * nx * ny) = o;
+        store_pair(tout + (size_t)j * nx * ny, o);
       }
     }
     if (!CHK || (j + 4 >= r1lo && j + 4 < r1hi)) {

@@ -275,6 +275,36 @@ def test_over_relaxed_smoother_fp64(level):
         assert rel_l2(s.download("MGX", level), o.get(oracle.F_MGX, level)) <= 1e-11
 
 
+@pytest.mark.parametrize("op", ["SMOOTH", "SMOOTH_ZERO"])
+def test_two_sweep_pass_fp16_level_against_oracle(op):
+    """Mixed mode, 192x256x192: level 2 (48x64x48, FP16 storage) smooths a
+    pair of sweeps in one two-sweep pass, FP32 in between and rounded to FP16
+    once on the store.  Against the oracle's two over-relaxed sweeps from the
+    same FP16 values: within 1e-3 relative (FP16 rounds to 2^-11)."""
+    from mac3d import MacSolver
+    from mac3d.lib import config_for
+    from mac3d import scenarios
+    w = scenarios.scaled(scenarios.workload("ref", 1), precision=1)
+    o = oracle.Oracle(w.nx, w.ny, w.nz, w.dx, levels=w.levels, omega=1.2)
+    rng = np.random.default_rng(5)
+    level = 2
+    with MacSolver(config_for(w)) as s:
+        shp = s.level_shape(level)
+        x = rng.standard_normal(shp).astype(np.float16).astype(np.float64)
+        b = rng.standard_normal(shp).astype(np.float16).astype(np.float64)
+        s.upload("MGX", x, level)
+        s.upload("MGB", b, level)
+        o.set(oracle.F_MGX, x, level)
+        o.set(oracle.F_MGB, b, level)
+        s.mg(op, level, 2)
+        if op == "SMOOTH_ZERO":
+            o.set(oracle.F_MGX, np.zeros(shp), level)
+        o.mg(oracle.MG_SMOOTH, level, 2)
+        r = rel_l2(s.download("MGX", level), o.get(oracle.F_MGX, level))
+        assert r <= 1e-3, r
+    o.close()
+
+
 @pytest.mark.parametrize("dims,levels", [((32, 48, 32), 3), ((96, 40, 20), 2), ((64, 96, 64), 3)])
 @pytest.mark.parametrize("omega", [1.0, 1.2])
 def test_two_sweep_pass_equals_two_sweeps(dims, levels, omega):
