@@ -101,8 +101,11 @@ __device__ inline void trace_back_tp(const Dims &d,
 // traced back from its own face and sampled with the -1/2 texel shift of
 // sampleMacU/V/W (fluid.ts:154-167), then multiplied by the damping factor;
 // the two solid wall faces stay 0.  sc = dt / domain size per axis.
+// (FP32: at least 6 waves per SIMD, more gathers in flight, measured 248
+// against 264 us at 192x256x192; 8 waves forces too few registers and was
+// slower.  FP64 needs its registers.)
 template <class T>
-__global__ void __launch_bounds__(256)
+__global__ void __launch_bounds__(256, sizeof(T) == 4 ? 6 : 1)
 k_advect_mac(Dims d, T dth, T damp, const T *__restrict__ u, const T *__restrict__ v,
              const T *__restrict__ w, const typename vec4_of<T>::type *__restrict__ cvel,
              T *__restrict__ uo, T *__restrict__ vo, T *__restrict__ wo) {
@@ -233,8 +236,11 @@ __device__ inline void casson_cell(const StepParams &P, T m0, T m1, T m2, T xp0,
 #define CS_PL (CS_EX * CS_EY)
 #define CS_LPT ((CS_PL + CS_T - 1) / CS_T)
 
+// (FP32: at least 6 waves per SIMD, three blocks per CU, measured 65-69
+// against 84 us at 192x256x192; 8 waves forces too few registers and was
+// slower.  FP64 needs its registers.)
 template <class T>
-__global__ void __launch_bounds__(CS_T)
+__global__ void __launch_bounds__(CS_T, sizeof(T) == 4 ? 6 : 1)
 k_casson_zm(Dims d, StepParams P, const T *__restrict__ u, const T *__restrict__ v,
             const T *__restrict__ w, const T *__restrict__ dye, T *__restrict__ uo,
             T *__restrict__ vo, T *__restrict__ wo, int cz, int ncx, int ncy) {
