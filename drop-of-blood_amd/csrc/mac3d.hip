@@ -496,20 +496,28 @@ int mg_sweep2(Ctx *ctx, int l, bool zero) {
   dim3 grid(ncx * ncy * ((d.nz + cz - 1) / cz));
   {
     ProfScope ps(ctx->prof, ctx->stream, l == 0 ? KC_SMOOTH0X2 : KC_MG);
-    auto launch = [&](auto stag) {
+    auto launch = [&](auto stag, auto w1tag) {
       using S = std::remove_pointer_t<decltype(stag)>;
-      auto kern = tx == 64 ? (zero ? k_rbgs_sweep2<true, 64, SWEEP2_TY, S>
-                                   : k_rbgs_sweep2<false, 64, SWEEP2_TY, S>)
-                           : (zero ? k_rbgs_sweep2<true, 32, SWEEP2_TY, S>
-                                   : k_rbgs_sweep2<false, 32, SWEEP2_TY, S>);
+      constexpr bool W1 = decltype(w1tag)::value;
+      auto kern = tx == 64 ? (zero ? k_rbgs_sweep2<true, 64, SWEEP2_TY, S, W1>
+                                   : k_rbgs_sweep2<false, 64, SWEEP2_TY, S, W1>)
+                           : (zero ? k_rbgs_sweep2<true, 32, SWEEP2_TY, S, W1>
+                                   : k_rbgs_sweep2<false, 32, SWEEP2_TY, S, W1>);
       hipLaunchKernelGGL(kern, grid, dim3(tx / 2 * SWEEP2_TY), 0, ctx->stream, d,
                          (const S *)ctx->mgx[l].ptr, (S *)ctx->mgxa[l].ptr,
                          (const S *)ctx->mgb[l].ptr, (float)level_h2(l),
                          (float)(l == 0 ? 1.0 : top_coeff(l)), (float)ctx->omega,
                          l == 0 ? (const double *)ctx->scale : nullptr, cz, ncx, ncy);
     };
-    if (ctx->mgx[l].type == TF16) launch((half_t *)nullptr);
-    else launch((float *)nullptr);
+    // omega exactly 1 in the kernel's FP32: the variant without the blend
+    const bool w1 = (float)ctx->omega == 1.0f;
+    if (ctx->mgx[l].type == TF16) {
+      if (w1) launch((half_t *)nullptr, std::true_type{});
+      else launch((half_t *)nullptr, std::false_type{});
+    } else {
+      if (w1) launch((float *)nullptr, std::true_type{});
+      else launch((float *)nullptr, std::false_type{});
+    }
   }
   LAUNCHCHK();
   swap_x(ctx, l);

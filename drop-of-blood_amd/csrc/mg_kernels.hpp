@@ -499,6 +499,16 @@ __device__ inline C rb_cell_r(C xm, C xp, C ym, C yp, C zm, C zp, C bv, C old, C
   if (omega != C(1)) v = old + omega * (v - old);
   return v;
 }
+// the same with omega known to be 1 at compile time (no blend to select from)
+template <class C>
+__device__ inline C rb_cell_r1(C xm, C xp, C ym, C yp, C zm, C zp, C bv, C h2, C rdg) {
+  C sn = xm + xp;
+  sn += ym;
+  sn += yp;
+  sn += zm;
+  sn += zp;
+  return (sn - h2 * bv) * rdg;
+}
 template <class C>
 __device__ inline C rb_cell(C xm, C xp, C ym, C yp, C zm, C zp, C bv, C old, C h2, C dg,
                             C omega) {
@@ -727,7 +737,8 @@ k_rbgs_sweep(Dims d, const S *__restrict__ xin, S *__restrict__ xout, const S *_
 #define TB_XR 7
 #define TB_BR 5
 
-template <bool ZERO, int TX, int TY, class S>
+// W1: omega is 1 (the plain Gauss-Seidel sweep), so no over-relaxation blend.
+template <bool ZERO, int TX, int TY, class S, bool W1>
 __global__ void __launch_bounds__(TX / 2 * TY)
 k_rbgs_sweep2(Dims d, const S *__restrict__ xin, S *__restrict__ xout,
               const S *__restrict__ b, float h2, float topc, float omega,
@@ -850,9 +861,12 @@ k_rbgs_sweep2(Dims d, const S *__restrict__ xin, S *__restrict__ xout,
   // (x- pair offset at odd planes: the cell changes side, 2 ofs - 4 - even)
   auto calc = [&](int st, int par, bool bnd, int own, int oth, int om, int op, C bv) -> C {
     int o = sofs[st], xm = oth + (par ? 2 * o - 4 - sxm[st] : sxm[st]);
+    const C rdg = bnd ? srb[par][st] : srd[par][st];
+    if (W1)
+      return rb_cell_r1<C>(ld(xm), ld(xm + 4), ld(oth + o - 4 * TB_PW), ld(oth + o + 4 * TB_PW),
+                           ld(om + o), ld(op + o), bv, h2, rdg);
     return rb_cell_r<C>(ld(xm), ld(xm + 4), ld(oth + o - 4 * TB_PW), ld(oth + o + 4 * TB_PW),
-                        ld(om + o), ld(op + o), bv, ld(own + o), h2,
-                        bnd ? srb[par][st] : srd[par][st], omega);
+                        ld(om + o), ld(op + o), bv, ld(own + o), h2, rdg, omega);
   };
   // all-ones where bit k of bits is set (a sign-extending bit extract), and
   // a select by such a mask (bitfield insert): no predicate registers
@@ -868,9 +882,12 @@ k_rbgs_sweep2(Dims d, const S *__restrict__ xin, S *__restrict__ xout,
     int o = sofs[st];
     C old = ld(own + o);
     int xm = oth + (par ? 2 * o - 4 - sxm[st] : sxm[st]);
-    C nv = rb_cell_r<C>(ld(xm), ld(xm + 4), ld(oth + o - 4 * TB_PW), ld(oth + o + 4 * TB_PW),
-                        ld(om + o), ld(op + o), bv, old, h2, bnd ? srb[par][st] : srd[par][st],
-                        omega);
+    const C rdg = bnd ? srb[par][st] : srd[par][st];
+    C nv = W1 ? rb_cell_r1<C>(ld(xm), ld(xm + 4), ld(oth + o - 4 * TB_PW),
+                              ld(oth + o + 4 * TB_PW), ld(om + o), ld(op + o), bv, h2, rdg)
+              : rb_cell_r<C>(ld(xm), ld(xm + 4), ld(oth + o - 4 * TB_PW),
+                             ld(oth + o + 4 * TB_PW), ld(om + o), ld(op + o), bv, old, h2, rdg,
+                             omega);
     return pick(mask(4 * st + 2 * par), nv, old);
   };
 
