@@ -62,7 +62,7 @@ __device__ inline void tri_grad(const T *__restrict__ f, const Dims &d, const Tr
 // d sample / d position of the three packed centre-velocity components:
 // g[a][b] = d v_a / d position_b
 template <class T>
-__device__ inline void tri_grad4(const typename vec4_of<T>::type *__restrict__ f, const Dims &d,
+__device__ inline void tri_grad4(const typename cvec_of<T>::type *__restrict__ f, const Dims &d,
                                  const Tri<T> &t, T g[3][3]) {
   auto c000 = f[t.r00 + t.i0], c100 = f[t.r00 + t.i1], c010 = f[t.r10 + t.i0], c110 = f[t.r10 + t.i1];
   auto c001 = f[t.r01 + t.i0], c101 = f[t.r01 + t.i1], c011 = f[t.r11 + t.i0], c111 = f[t.r11 + t.i1];
@@ -162,7 +162,7 @@ k_adj_div(Dims d, const T *__restrict__ lu1, const T *__restrict__ lv1, const T 
 template <class T>
 __global__ void __launch_bounds__(256)
 k_adj_dye_resolve(Dims d, T sc0, T sc1, T sc2, T strength,
-                  const typename vec4_of<T>::type *__restrict__ cvel, const T *__restrict__ dye,
+                  const typename cvec_of<T>::type *__restrict__ cvel, const T *__restrict__ dye,
                   const T *__restrict__ pred, const T *__restrict__ ldye_out,
                   T *__restrict__ ldye, T *__restrict__ lpred, T *__restrict__ lcu,
                   T *__restrict__ lcv, T *__restrict__ lcw) {
@@ -221,7 +221,7 @@ k_adj_dye_resolve(Dims d, T sc0, T sc1, T sc2, T strength,
 // lo/hi: the clamp of the trace points; shift: the sample-point shift per
 // axis (MAC faces) with its own clamp sl/sh.
 template <class T>
-__device__ inline void adj_trace(const Dims &d, const typename vec4_of<T>::type *__restrict__ cvel,
+__device__ inline void adj_trace(const Dims &d, const typename cvec_of<T>::type *__restrict__ cvel,
                                  const T sc[3], const T f[3], bool clamp_p0, T lo, T hi,
                                  const T gback[3], T *__restrict__ lcu, T *__restrict__ lcv,
                                  T *__restrict__ lcw) {
@@ -257,7 +257,7 @@ __device__ inline void adj_trace(const Dims &d, const typename vec4_of<T>::type 
 
 template <class T>
 __global__ void __launch_bounds__(256)
-k_adj_dye_predict(Dims d, T sc0, T sc1, T sc2, const typename vec4_of<T>::type *__restrict__ cvel,
+k_adj_dye_predict(Dims d, T sc0, T sc1, T sc2, const typename cvec_of<T>::type *__restrict__ cvel,
                   const T *__restrict__ dye, const T *__restrict__ lpred, T *__restrict__ ldye,
                   T *__restrict__ lcu, T *__restrict__ lcv, T *__restrict__ lcw) {
   MAC3D_CELL_PROLOGUE
@@ -282,7 +282,7 @@ template <class T>
 __global__ void __launch_bounds__(256)
 k_adj_advect(Dims d, T sc0, T sc1, T sc2, T damp, const T *__restrict__ u,
              const T *__restrict__ v, const T *__restrict__ w,
-             const typename vec4_of<T>::type *__restrict__ cvel, const T *__restrict__ lout_u,
+             const typename cvec_of<T>::type *__restrict__ cvel, const T *__restrict__ lout_u,
              const T *__restrict__ lout_v, const T *__restrict__ lout_w, T *__restrict__ lu,
              T *__restrict__ lv, T *__restrict__ lw, T *__restrict__ lcu, T *__restrict__ lcv,
              T *__restrict__ lcw) {

@@ -61,17 +61,18 @@ __device__ inline T sample_cell(const T *__restrict__ f, const Dims &d, T ux,
   return mixv(mixv(x00, x10, fy), mixv(x01, x11, fy), fz);
 }
 
-// Cell-centred velocity is held as one 4-vector per cell (x, y, z, unused) so
-// a trilinear sample of all three components is eight vector loads.
-template <class T> struct vec4_of;
-template <> struct vec4_of<float> { typedef float4 type; };
-template <> struct vec4_of<double> { typedef double4 type; };
+// Cell-centred velocity is held as one 3-vector per cell (x, y, z; 12 B in
+// FP32, one dwordx3 load) so a trilinear sample of all three components is
+// eight vector loads.
+template <class T> struct cvec_of;
+template <> struct cvec_of<float> { typedef float3 type; };
+template <> struct cvec_of<double> { typedef double3 type; };
 template <class T> struct vec2_of;
 template <> struct vec2_of<float> { typedef float2 type; };
 template <> struct vec2_of<double> { typedef double2 type; };
 
 template <class T>
-__device__ inline void sample_vec(const typename vec4_of<T>::type *__restrict__ f, const Dims &d,
+__device__ inline void sample_vec(const typename cvec_of<T>::type *__restrict__ f, const Dims &d,
                                   T ux, T uy, T uz, T out[3]) {
   T tx = ux * (T)d.nx - T(0.5), ty = uy * (T)d.ny - T(0.5), tz = uz * (T)d.nz - T(0.5);
   T bx = floor(tx), by = floor(ty), bz = floor(tz);
@@ -200,7 +201,7 @@ __device__ inline T sample_tp_mm(const T *__restrict__ f, const Dims &d, const T
 }
 
 template <class T>
-__device__ inline void sample_vec_tp(const typename vec4_of<T>::type *__restrict__ f,
+__device__ inline void sample_vec_tp(const typename cvec_of<T>::type *__restrict__ f,
                                      const Dims &d, const TPos<T> &p, T out[3]) {
   TriIdx<T> t = tpos_index(d, p);
   auto a000 = f[t.r00 + t.i0], a100 = f[t.r00 + t.i1], a010 = f[t.r10 + t.i0], a110 = f[t.r10 + t.i1];

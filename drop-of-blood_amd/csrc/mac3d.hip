@@ -71,7 +71,7 @@ struct Ctx {
   bool dmm_valid = false;
   std::string err;
   // level-0 step fields
-  Buf u, v, w, u2, v2, w2, p, dye, dye2, dpred, cvel, rhs;  // cvel: 4 per cell
+  Buf u, v, w, u2, v2, w2, p, dye, dye2, dpred, cvel, rhs;  // cvel: 3 per cell
   Buf dmm;  // limiter bounds of the dye predict: 2 per cell
   // multigrid: x, b, r per level (level 0 x/b alias p/rhs except in mixed);
   // sweeps run out of place between mgx and its alternate mgxa
@@ -248,7 +248,7 @@ int launch_face_to_centre(Ctx *ctx) {
     using T = std::remove_pointer_t<decltype(tag)>;
     hipLaunchKernelGGL(k_face_to_centre<T>, cell_grid(d), kCellBlock, 0, ctx->stream, d,
                        (const T *)ctx->u.ptr, (const T *)ctx->v.ptr, (const T *)ctx->w.ptr,
-                       (typename vec4_of<T>::type *)ctx->cvel.ptr);
+                       (typename cvec_of<T>::type *)ctx->cvel.ptr);
   });
   LAUNCHCHK();
   return MAC3D_OK;
@@ -278,7 +278,7 @@ int launch_advect(Ctx *ctx, double dt) {
       using T = std::remove_pointer_t<decltype(tag)>;
       hipLaunchKernelGGL(k_advect_mac<T>, gather_grid(d), kGatherBlock, 0, ctx->stream, d,
                          (T)(dt / ctx->cfg.dx), (T)ctx->cfg.vel_damp, (const T *)ctx->u.ptr, (const T *)ctx->v.ptr,
-                         (const T *)ctx->w.ptr, (const typename vec4_of<T>::type *)ctx->cvel.ptr,
+                         (const T *)ctx->w.ptr, (const typename cvec_of<T>::type *)ctx->cvel.ptr,
                          (T *)ctx->u2.ptr, (T *)ctx->v2.ptr,
                          (T *)ctx->w2.ptr);
     });
@@ -350,7 +350,7 @@ int launch_dye_predict(Ctx *ctx, double dt) {
     using T = std::remove_pointer_t<decltype(tag)>;
     hipLaunchKernelGGL(k_dye_predict<T>, gather_grid(d), kGatherBlock, 0, ctx->stream, d,
                        (T)(dt / ctx->cfg.dx),
-                       (const typename vec4_of<T>::type *)ctx->cvel.ptr,
+                       (const typename cvec_of<T>::type *)ctx->cvel.ptr,
                        (const T *)ctx->dye.ptr, (T *)ctx->dpred.ptr,
                        (typename vec2_of<T>::type *)ctx->dmm.ptr);
   });
@@ -366,7 +366,7 @@ int launch_dye_resolve(Ctx *ctx, double dt) {
     with_step_type(ctx->t0, [&](auto tag) {
       using T = std::remove_pointer_t<decltype(tag)>;
       const T dth = (T)(dt / ctx->cfg.dx), s = (T)ctx->cfg.maccormack;
-      const auto *cv = (const typename vec4_of<T>::type *)ctx->cvel.ptr;
+      const auto *cv = (const typename cvec_of<T>::type *)ctx->cvel.ptr;
       if (ctx->dmm_valid)
         hipLaunchKernelGGL(k_dye_resolve_mm<T>, gather_grid(d), kGatherBlock, 0, ctx->stream, d,
                            dth, s, cv, (const T *)ctx->dye.ptr, (const T *)ctx->dpred.ptr,
@@ -1044,7 +1044,7 @@ int create_impl(Ctx *ctx, const mac3d_config *cfg) {
                &ctx->dye2, &ctx->dpred};
   for (Buf *b : f0)
     if ((rc = alloc(ctx, *b, t0, n0))) return rc;
-  if ((rc = alloc(ctx, ctx->cvel, t0, 4 * n0))) return rc;
+  if ((rc = alloc(ctx, ctx->cvel, t0, 3 * n0))) return rc;
   if ((rc = alloc(ctx, ctx->dmm, t0, 2 * n0))) return rc;
   bool mixed = ctx->prec == MAC3D_PREC_MIXED;
   ctx->omega = cfg->sor_omega > 0.0 ? cfg->sor_omega : (mixed ? 1.2 : 1.0);
@@ -1178,7 +1178,7 @@ int adj_alloc(Ctx *ctx) {
   if (A.ready) return MAC3D_OK;
   int rc;
   size_t n0 = ctx->n[0], bytes = type_size(ctx->t0) * n0;
-  if ((rc = dev_alloc(ctx, &A.c1, 4 * bytes))) return rc;
+  if ((rc = dev_alloc(ctx, &A.c1, 3 * bytes))) return rc;
   void **arrs[] = {&A.vela[0], &A.vela[1], &A.vela[2],
                    &A.lvel[0], &A.lvel[1], &A.lvel[2], &A.lvelc[0], &A.lvelc[1], &A.lvelc[2],
                    &A.lvela[0], &A.lvela[1], &A.lvela[2], &A.lnew[0], &A.lnew[1], &A.lnew[2],
@@ -1280,7 +1280,7 @@ int adj_step(Ctx *ctx, double dt, void *const *src) {
   // forward recompute of the stage inputs
   if ((rc = load_state(ctx, src))) return rc;
   if ((rc = launch_face_to_centre(ctx))) return rc;
-  HIPCHK(hipMemcpyAsync(A.c1, ctx->cvel.ptr, 4 * tb, hipMemcpyDeviceToDevice, ctx->stream));
+  HIPCHK(hipMemcpyAsync(A.c1, ctx->cvel.ptr, 3 * tb, hipMemcpyDeviceToDevice, ctx->stream));
   if ((rc = launch_advect(ctx, dt))) return rc;
   void *va[3] = {ctx->u.ptr, ctx->v.ptr, ctx->w.ptr};
   for (int a = 0; a < 3; ++a)
@@ -1320,12 +1320,12 @@ int adj_step(Ctx *ctx, double dt, void *const *src) {
     T s0 = (T)(dt / dom[0]), s1 = (T)(dt / dom[1]), s2 = (T)(dt / dom[2]);
     const T *dyen = (const T *)src[SV_DYE];
     hipLaunchKernelGGL(k_adj_dye_resolve<T>, cell_grid(d), kCellBlock, 0, ctx->stream, d, s0, s1,
-                       s2, (T)ctx->cfg.maccormack, (const typename vec4_of<T>::type *)ctx->cvel.ptr,
+                       s2, (T)ctx->cfg.maccormack, (const typename cvec_of<T>::type *)ctx->cvel.ptr,
                        dyen, (const T *)ctx->dpred.ptr,
                        (const T *)A.ldye, (T *)A.ldyen, (T *)A.lpred, (T *)A.lc[0],
                        (T *)A.lc[1], (T *)A.lc[2]);
     hipLaunchKernelGGL(k_adj_dye_predict<T>, cell_grid(d), kCellBlock, 0, ctx->stream, d, s0, s1,
-                       s2, (const typename vec4_of<T>::type *)ctx->cvel.ptr,
+                       s2, (const typename cvec_of<T>::type *)ctx->cvel.ptr,
                        dyen, (const T *)A.lpred, (T *)A.ldyen, (T *)A.lc[0], (T *)A.lc[1],
                        (T *)A.lc[2]);
     hipLaunchKernelGGL(k_adj_face_to_centre<T>, cell_grid(d), kCellBlock, 0, ctx->stream, d,
@@ -1372,7 +1372,7 @@ int adj_step(Ctx *ctx, double dt, void *const *src) {
     T s0 = (T)(dt / dom[0]), s1 = (T)(dt / dom[1]), s2 = (T)(dt / dom[2]);
     hipLaunchKernelGGL(k_adj_advect<T>, cell_grid(d), kCellBlock, 0, ctx->stream, d, s0, s1, s2,
                        (T)ctx->cfg.vel_damp, (const T *)src[SV_U], (const T *)src[SV_V],
-                       (const T *)src[SV_W], (const typename vec4_of<T>::type *)A.c1,
+                       (const T *)src[SV_W], (const typename cvec_of<T>::type *)A.c1,
                        (const T *)A.lvela[0], (const T *)A.lvela[1],
                        (const T *)A.lvela[2], (T *)A.lnew[0], (T *)A.lnew[1], (T *)A.lnew[2],
                        (T *)A.lc[0], (T *)A.lc[1], (T *)A.lc[2]);
@@ -1566,10 +1566,10 @@ int mac3d_upload(void *h, int field, int level, const void *host, int dtype, siz
       dim3 g((unsigned)((count + 255) / 256));
       if (dtype == MAC3D_F32)
         hipLaunchKernelGGL((k_convert_strided<float, S>), g, dim3(256), 0, ctx->stream,
-                           (const float *)ctx->stage, (S *)b->ptr, count, 1, 0, 4, comp);
+                           (const float *)ctx->stage, (S *)b->ptr, count, 1, 0, 3, comp);
       else
         hipLaunchKernelGGL((k_convert_strided<double, S>), g, dim3(256), 0, ctx->stream,
-                           (const double *)ctx->stage, (S *)b->ptr, count, 1, 0, 4, comp);
+                           (const double *)ctx->stage, (S *)b->ptr, count, 1, 0, 3, comp);
     });
     LAUNCHCHK();
     HIPCHK(hipStreamSynchronize(ctx->stream));
@@ -1606,10 +1606,10 @@ int mac3d_download(void *h, int field, int level, void *host, int dtype, size_t 
       dim3 g((unsigned)((count + 255) / 256));
       if (dtype == MAC3D_F32)
         hipLaunchKernelGGL((k_convert_strided<S, float>), g, dim3(256), 0, ctx->stream,
-                           (const S *)b->ptr, (float *)ctx->stage, count, 4, comp, 1, 0);
+                           (const S *)b->ptr, (float *)ctx->stage, count, 3, comp, 1, 0);
       else
         hipLaunchKernelGGL((k_convert_strided<S, double>), g, dim3(256), 0, ctx->stream,
-                           (const S *)b->ptr, (double *)ctx->stage, count, 4, comp, 1, 0);
+                           (const S *)b->ptr, (double *)ctx->stage, count, 3, comp, 1, 0);
     });
   } else
   with_type(b->type, [&](auto tag) {

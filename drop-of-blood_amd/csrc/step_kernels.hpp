@@ -29,7 +29,7 @@ struct StepParams {
 template <class T>
 __global__ void __launch_bounds__(256)
 k_face_to_centre(Dims d, const T *__restrict__ u, const T *__restrict__ v,
-                 const T *__restrict__ w, typename vec4_of<T>::type *__restrict__ cvel) {
+                 const T *__restrict__ w, typename cvec_of<T>::type *__restrict__ cvel) {
   MAC3D_CELL_PROLOGUE
   T uR = (i >= d.nx - 1) ? T(0) : u[c];
   T uL = (i == 0) ? T(0) : u[c - sx];
@@ -37,11 +37,10 @@ k_face_to_centre(Dims d, const T *__restrict__ u, const T *__restrict__ v,
   T vD = (j == 0) ? T(0) : v[c - sy];
   T wF = (k >= d.nz - 1) ? T(0) : w[c];
   T wB = (k == 0) ? T(0) : w[c - sz];
-  typename vec4_of<T>::type o;
+  typename cvec_of<T>::type o;
   o.x = T(0.5) * (uL + uR);
   o.y = T(0.5) * (vD + vU);
   o.z = T(0.5) * (wB + wF);
-  o.w = T(0);
   cvel[c] = o;
 }
 
@@ -49,7 +48,7 @@ k_face_to_centre(Dims d, const T *__restrict__ u, const T *__restrict__ v,
 // velocity in normalised coordinates, clamped to [0.002, 0.998].
 template <class T>
 __device__ inline void trace_back(const Dims &d,
-                                  const typename vec4_of<T>::type *__restrict__ cvel,
+                                  const typename cvec_of<T>::type *__restrict__ cvel,
                                   const T sc[3], const T f[3], T out[3]) {
   const T lo = T(0.002), hi = T(0.998);
   T v0[3], vm[3];
@@ -71,7 +70,7 @@ __device__ inline void trace_back(const Dims &d,
 // 8-texel sample (mixv(x, y, 0) = x exactly) from 2 loads.
 template <class T>
 __device__ inline void trace_back_tp(const Dims &d,
-                                     const typename vec4_of<T>::type *__restrict__ cvel,
+                                     const typename cvec_of<T>::type *__restrict__ cvel,
                                      T dth, const TPos<T> &f, const T lo[3], const T hi[3],
                                      size_t c, size_t sa, TPos<T> &out) {
   TPos<T> p0 = f;
@@ -107,7 +106,7 @@ __device__ inline void trace_back_tp(const Dims &d,
 template <class T>
 __global__ void __launch_bounds__(256, sizeof(T) == 4 ? 6 : 1)
 k_advect_mac(Dims d, T dth, T damp, const T *__restrict__ u, const T *__restrict__ v,
-             const T *__restrict__ w, const typename vec4_of<T>::type *__restrict__ cvel,
+             const T *__restrict__ w, const typename cvec_of<T>::type *__restrict__ cvel,
              T *__restrict__ uo, T *__restrict__ vo, T *__restrict__ wo) {
   MAC3D_CELL_PROLOGUE
   const T N[3] = {(T)d.nx, (T)d.ny, (T)d.nz};
@@ -370,7 +369,7 @@ k_project(Dims d, T *__restrict__ u, T *__restrict__ v, T *__restrict__ w,
 // CONC_PREDICT (advect.ts:28-72): RK2 semi-Lagrangian predictor of the dye
 // on the centred velocity, normalised coordinates clamped to [0.001, 0.999].
 template <class T>
-__device__ inline void dye_back(const Dims &d, const typename vec4_of<T>::type *__restrict__ cvel,
+__device__ inline void dye_back(const Dims &d, const typename cvec_of<T>::type *__restrict__ cvel,
                                 const T sc[3], const T uvw[3], T v0[3], T back[3]) {
   const T lo = T(0.001), hi = T(0.999);
   sample_vec<T>(cvel, d, uvw[0], uvw[1], uvw[2], v0);
@@ -387,7 +386,7 @@ __device__ inline void dye_back(const Dims &d, const typename vec4_of<T>::type *
 // dye_back in texel space from the cell centre (no clamp on the first
 // sample), clamps at [0.001 N - 1/2, 0.999 N - 1/2]
 template <class T>
-__device__ inline void dye_back_tp(const Dims &d, const typename vec4_of<T>::type *__restrict__ cvel,
+__device__ inline void dye_back_tp(const Dims &d, const typename cvec_of<T>::type *__restrict__ cvel,
                                    T dth, int i, int j, int k, const T lo[3], const T hi[3],
                                    T v0[3], TPos<T> &back) {
   TPos<T> c;
@@ -426,7 +425,7 @@ __device__ inline void dye_bounds(const Dims &d, T lo[3], T hi[3]) {
 // k_dye_resolve_mm need not trace back again.
 template <class T>
 __global__ void __launch_bounds__(256)
-k_dye_predict(Dims d, T dth, const typename vec4_of<T>::type *__restrict__ cvel,
+k_dye_predict(Dims d, T dth, const typename cvec_of<T>::type *__restrict__ cvel,
               const T *__restrict__ dye, T *__restrict__ pred,
               typename vec2_of<T>::type *__restrict__ mm) {
   MAC3D_CELL_PROLOGUE
@@ -448,7 +447,7 @@ k_dye_predict(Dims d, T dth, const typename vec4_of<T>::type *__restrict__ cvel,
 // clamped to the min/max of the 8 source texels around the backtrace point.
 template <class T>
 __global__ void __launch_bounds__(256)
-k_dye_resolve(Dims d, T dth, T strength, const typename vec4_of<T>::type *__restrict__ cvel,
+k_dye_resolve(Dims d, T dth, T strength, const typename cvec_of<T>::type *__restrict__ cvel,
               const T *__restrict__ dye, const T *__restrict__ pred, T *__restrict__ out) {
   MAC3D_CELL_PROLOGUE
   T lo[3], hi[3], v0[3];
@@ -480,7 +479,7 @@ k_dye_resolve(Dims d, T dth, T strength, const typename vec4_of<T>::type *__rest
 // the same dye and centre velocity): the same value with no second backtrace.
 template <class T>
 __global__ void __launch_bounds__(256)
-k_dye_resolve_mm(Dims d, T dth, T strength, const typename vec4_of<T>::type *__restrict__ cvel,
+k_dye_resolve_mm(Dims d, T dth, T strength, const typename cvec_of<T>::type *__restrict__ cvel,
                  const T *__restrict__ dye, const T *__restrict__ pred,
                  const typename vec2_of<T>::type *__restrict__ mm, T *__restrict__ out) {
   MAC3D_CELL_PROLOGUE
