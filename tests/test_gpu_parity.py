@@ -149,6 +149,25 @@ def test_vcycle(prec, tol):
         _cmp(s, o, ("P",), tol)
 
 
+@pytest.mark.parametrize("prec,tol", [(0, TOL32), (2, 1e-10)])
+@pytest.mark.parametrize("nu1,nu2,levels", [(1, 1, 3), (3, 2, 3), (2, 3, 2), (2, 2, 4)])
+def test_vcycle_schedules(nu1, nu2, levels, prec, tol):
+    """V-cycles with odd sweep counts (a two-sweep pass and a single sweep on
+    one level) and with 2 and 4 levels, against the oracle's cycle with the
+    same schedule (32x48x32)."""
+    w, s, o = pair(precision=prec, nu1=nu1, nu2=nu2, levels=levels)
+    with s:
+        o.stage(oracle.ST_F2C); o.stage(oracle.ST_ADVECT, w.dt); o.stage(oracle.ST_CASSON, w.dt)
+        o.stage(oracle.ST_DIV)
+        copy_state(o, s, ("U", "V", "W", "P"))
+        s.stage("DIVERGENCE")
+        for _ in range(2):
+            o.stage(oracle.ST_VCYCLE)
+            s.stage("VCYCLE")
+        _cmp(s, o, ("P",), tol)
+    o.close()
+
+
 def _full_size_rhs(precision):
     w, s, o = pair("ref", precision=precision, warm_steps=0)
     o.stage(oracle.ST_F2C); o.stage(oracle.ST_ADVECT, w.dt); o.stage(oracle.ST_CASSON, w.dt)
