@@ -627,7 +627,8 @@ int mg_prolong(Ctx *ctx, int l) {
       using Sf = std::remove_pointer_t<decltype(ftag)>;
       // 32 x 8 threads over the coarse grid (coarse rows of 96 fill three blocks)
       hipLaunchKernelGGL((k_prolong_add_2x<Sc, Sf>),
-                         dim3((dc.nx + 31) / 32, (dc.ny + 7) / 8, dc.nz), dim3(32, 8), 0,
+                         dim3((dc.nx + PR_BX - 1) / PR_BX, (dc.ny + PR_BY - 1) / PR_BY, dc.nz),
+                         dim3(PR_BX, PR_BY), 0,
                          ctx->stream, df, dc, (const Sc *)ctx->mgx[l + 1].ptr,
                          (Sf *)ctx->mgx[l].ptr);
     });

@@ -226,14 +226,25 @@ coarse_at(const Sc *__restrict__ e, const Dims &dc, int i, int j, int k) {
 // neighbourhood: child offset 0 along an axis takes 3/4 of the parent and
 // 1/4 of the parent's lower neighbour, offset 1 takes 3/4 and 1/4 of the
 // upper neighbour (cell-centred weights of fluid.ts:806-855; clamped at the
-// walls, 0 above the top row).
+// walls, 0 above the top row).  A 32 x 8 block of coarse cells first stages
+// its neighbourhood (34 x 10 x 3 values) in LDS, so each thread reads its 27
+// from LDS rather than issuing 27 gathers.
+#define PR_BX 32
+#define PR_BY 8
 template <class Sc, class Sf>
-__global__ void __launch_bounds__(256)
+__global__ void __launch_bounds__(PR_BX * PR_BY)
 k_prolong_add_2x(Dims df, Dims dc, const Sc *__restrict__ ec, Sf *__restrict__ xf) {
   typedef typename compute_of<Sf>::type C;
-  int I = blockIdx.x * blockDim.x + threadIdx.x;
-  int J = blockIdx.y * blockDim.y + threadIdx.y;
-  int K = blockIdx.z;
+  __shared__ C se[3][PR_BY + 2][PR_BX + 2];
+  const int tx = threadIdx.x, ty = threadIdx.y, tid = ty * PR_BX + tx;
+  const int I0 = blockIdx.x * PR_BX, J0 = blockIdx.y * PR_BY, K = blockIdx.z;
+  for (int t = tid; t < 3 * (PR_BY + 2) * (PR_BX + 2); t += PR_BX * PR_BY) {
+    const int q = t % (PR_BX + 2), r = (t / (PR_BX + 2)) % (PR_BY + 2);
+    const int a = t / ((PR_BX + 2) * (PR_BY + 2));
+    se[a][r][q] = (C)coarse_at(ec, dc, I0 + q - 1, J0 + r - 1, K + a - 1);
+  }
+  __syncthreads();
+  const int I = I0 + tx, J = J0 + ty;
   if (I >= dc.nx || J >= dc.ny) return;
   C e[3][3][3];  // [dz][dy][dx], offsets -1..1
 #pragma unroll
@@ -241,7 +252,7 @@ k_prolong_add_2x(Dims df, Dims dc, const Sc *__restrict__ ec, Sf *__restrict__ x
 #pragma unroll
     for (int bq = 0; bq < 3; ++bq)
 #pragma unroll
-      for (int q = 0; q < 3; ++q) e[a][bq][q] = (C)coarse_at(ec, dc, I + q - 1, J + bq - 1, K + a - 1);
+      for (int q = 0; q < 3; ++q) e[a][bq][q] = se[a][ty + bq][tx + q];
   // weights along one axis for child offset o: (lower, mid, upper)
   const C wl[2] = {C(0.25), C(0)}, wm = C(0.75), wu[2] = {C(0), C(0.25)};
 #pragma unroll
