@@ -147,15 +147,17 @@ __device__ inline void tpos_clamp(TPos<T> &p, const T lo[3], const T hi[3]) {
   }
 }
 
-template <class T> struct TriIdx {
-  size_t r00, r10, r01, r11;
+template <class T, class I = size_t> struct TriIdx {
+  I r00, r10, r01, r11;
   int i0, i1;
   T fx, fy, fz;
 };
 
-template <class T>
-__device__ inline TriIdx<T> tpos_index(const Dims &d, const TPos<T> &p) {
-  TriIdx<T> t;
+// corner rows and weights of a trilinear sample at p; I: the index type (int
+// when the field is addressed with 32-bit element indices)
+template <class T, class I = size_t>
+__device__ inline TriIdx<T, I> tpos_index(const Dims &d, const TPos<T> &p) {
+  TriIdx<T, I> t;
   T fl0 = floor(p.o[0]), fl1 = floor(p.o[1]), fl2 = floor(p.o[2]);
   int ib = p.b[0] + (int)fl0, jb = p.b[1] + (int)fl1, kb = p.b[2] + (int)fl2;
   t.fx = p.o[0] - fl0;
@@ -165,32 +167,44 @@ __device__ inline TriIdx<T> tpos_index(const Dims &d, const TPos<T> &p) {
   t.i1 = clampi(ib + 1, 0, d.nx - 1);
   int j0 = clampi(jb, 0, d.ny - 1), j1 = clampi(jb + 1, 0, d.ny - 1);
   int k0 = clampi(kb, 0, d.nz - 1), k1 = clampi(kb + 1, 0, d.nz - 1);
-  t.r00 = ((size_t)k0 * d.ny + j0) * d.nx;
-  t.r10 = ((size_t)k0 * d.ny + j1) * d.nx;
-  t.r01 = ((size_t)k1 * d.ny + j0) * d.nx;
-  t.r11 = ((size_t)k1 * d.ny + j1) * d.nx;
+  t.r00 = ((I)k0 * d.ny + j0) * d.nx;
+  t.r10 = ((I)k0 * d.ny + j1) * d.nx;
+  t.r01 = ((I)k1 * d.ny + j0) * d.nx;
+  t.r11 = ((I)k1 * d.ny + j1) * d.nx;
   return t;
 }
 
+// Element readers for the samplers: PtrRead takes 64-bit element indices
+// into a plain pointer; BufRead (below) takes 32-bit ones into a buffer
+// resource, so a gather needs no 64-bit index or address arithmetic.
+template <class V> struct PtrRead {
+  typedef size_t index;
+  const V *__restrict__ p;
+  __device__ V operator()(size_t i) const { return p[i]; }
+};
+
+template <class T, class R>
+__device__ inline T sample_tp_r(const R &f, const Dims &d, const TPos<T> &p) {
+  auto t = tpos_index<T, typename R::index>(d, p);
+  T x00 = mixv(f(t.r00 + t.i0), f(t.r00 + t.i1), t.fx);
+  T x10 = mixv(f(t.r10 + t.i0), f(t.r10 + t.i1), t.fx);
+  T x01 = mixv(f(t.r01 + t.i0), f(t.r01 + t.i1), t.fx);
+  T x11 = mixv(f(t.r11 + t.i0), f(t.r11 + t.i1), t.fx);
+  return mixv(mixv(x00, x10, t.fy), mixv(x01, x11, t.fy), t.fz);
+}
 template <class T>
 __device__ inline T sample_tp(const T *__restrict__ f, const Dims &d, const TPos<T> &p) {
-  TriIdx<T> t = tpos_index(d, p);
-  T x00 = mixv(f[t.r00 + t.i0], f[t.r00 + t.i1], t.fx);
-  T x10 = mixv(f[t.r10 + t.i0], f[t.r10 + t.i1], t.fx);
-  T x01 = mixv(f[t.r01 + t.i0], f[t.r01 + t.i1], t.fx);
-  T x11 = mixv(f[t.r11 + t.i0], f[t.r11 + t.i1], t.fx);
-  return mixv(mixv(x00, x10, t.fy), mixv(x01, x11, t.fy), t.fz);
+  return sample_tp_r<T>(PtrRead<T>{f}, d, p);
 }
 
 // sample_tp that also gives the least and greatest of its 8 texels: the
 // bounds of the MacCormack limiter around the same point (the limiter takes
 // the same clamped corners, in this fmin/fmax order)
-template <class T>
-__device__ inline T sample_tp_mm(const T *__restrict__ f, const Dims &d, const TPos<T> &p, T &mn,
-                                 T &mx) {
-  TriIdx<T> t = tpos_index(d, p);
-  T s0 = f[t.r00 + t.i0], s1 = f[t.r00 + t.i1], s2 = f[t.r10 + t.i0], s3 = f[t.r10 + t.i1];
-  T s4 = f[t.r01 + t.i0], s5 = f[t.r01 + t.i1], s6 = f[t.r11 + t.i0], s7 = f[t.r11 + t.i1];
+template <class T, class R>
+__device__ inline T sample_tp_mm_r(const R &f, const Dims &d, const TPos<T> &p, T &mn, T &mx) {
+  auto t = tpos_index<T, typename R::index>(d, p);
+  T s0 = f(t.r00 + t.i0), s1 = f(t.r00 + t.i1), s2 = f(t.r10 + t.i0), s3 = f(t.r10 + t.i1);
+  T s4 = f(t.r01 + t.i0), s5 = f(t.r01 + t.i1), s6 = f(t.r11 + t.i0), s7 = f(t.r11 + t.i1);
   mn = fmin(fmin(fmin(s0, s1), fmin(s2, s3)), fmin(fmin(s4, s5), fmin(s6, s7)));
   mx = fmax(fmax(fmax(s0, s1), fmax(s2, s3)), fmax(fmax(s4, s5), fmax(s6, s7)));
   T x00 = mixv(s0, s1, t.fx);
@@ -200,12 +214,11 @@ __device__ inline T sample_tp_mm(const T *__restrict__ f, const Dims &d, const T
   return mixv(mixv(x00, x10, t.fy), mixv(x01, x11, t.fy), t.fz);
 }
 
-template <class T>
-__device__ inline void sample_vec_tp(const typename cvec_of<T>::type *__restrict__ f,
-                                     const Dims &d, const TPos<T> &p, T out[3]) {
-  TriIdx<T> t = tpos_index(d, p);
-  auto a000 = f[t.r00 + t.i0], a100 = f[t.r00 + t.i1], a010 = f[t.r10 + t.i0], a110 = f[t.r10 + t.i1];
-  auto a001 = f[t.r01 + t.i0], a101 = f[t.r01 + t.i1], a011 = f[t.r11 + t.i0], a111 = f[t.r11 + t.i1];
+template <class T, class R>
+__device__ inline void sample_vec_tp_r(const R &f, const Dims &d, const TPos<T> &p, T out[3]) {
+  auto t = tpos_index<T, typename R::index>(d, p);
+  auto a000 = f(t.r00 + t.i0), a100 = f(t.r00 + t.i1), a010 = f(t.r10 + t.i0), a110 = f(t.r10 + t.i1);
+  auto a001 = f(t.r01 + t.i0), a101 = f(t.r01 + t.i1), a011 = f(t.r11 + t.i0), a111 = f(t.r11 + t.i1);
 #define MAC3D_TRI(C)                                                                    \
   mixv(mixv(mixv(a000.C, a100.C, t.fx), mixv(a010.C, a110.C, t.fx), t.fy),              \
        mixv(mixv(a001.C, a101.C, t.fx), mixv(a011.C, a111.C, t.fx), t.fy), t.fz)
@@ -213,6 +226,11 @@ __device__ inline void sample_vec_tp(const typename cvec_of<T>::type *__restrict
   out[1] = MAC3D_TRI(y);
   out[2] = MAC3D_TRI(z);
 #undef MAC3D_TRI
+}
+template <class T>
+__device__ inline void sample_vec_tp(const typename cvec_of<T>::type *__restrict__ f,
+                                     const Dims &d, const TPos<T> &p, T out[3]) {
+  sample_vec_tp_r<T>(PtrRead<typename cvec_of<T>::type>{f}, d, p, out);
 }
 
 // Raw buffer resource over `bytes` bytes from `base` (stride 0; word 3 is the
@@ -237,6 +255,29 @@ template <> __device__ inline float2 buf_load<float2>(BufRsrc r, int off) {
 }
 template <> __device__ inline double2 buf_load<double2>(BufRsrc r, int off) {
   return __builtin_bit_cast(double2, __builtin_amdgcn_raw_buffer_load_b128(r, off, 0, 0));
+}
+
+template <> __device__ inline float3 buf_load<float3>(BufRsrc r, int off) {
+  auto v = __builtin_amdgcn_raw_buffer_load_b96(r, off, 0, 0);
+  return make_float3(__int_as_float(v[0]), __int_as_float(v[1]), __int_as_float(v[2]));
+}
+template <> __device__ inline double3 buf_load<double3>(BufRsrc r, int off) {
+  return make_double3(buf_load<double>(r, off), buf_load<double>(r, off + 8),
+                      buf_load<double>(r, off + 16));
+}
+template <class V> struct BufRead {
+  typedef int index;
+  BufRsrc r;
+  __device__ V operator()(int i) const { return buf_load<V>(r, i * (int)sizeof(V)); }
+};
+// reader of n elements at p: a buffer resource when BUF (the host checks
+// that the field is under 2^31 bytes), else the plain pointer
+template <bool BUF, class V> struct Reader { typedef PtrRead<V> type; };
+template <class V> struct Reader<true, V> { typedef BufRead<V> type; };
+template <bool BUF, class V>
+__device__ inline typename Reader<BUF, V>::type make_reader(const V *p, size_t n) {
+  if constexpr (BUF) return BufRead<V>{buf_rsrc(p, (uint32_t)(n * sizeof(V)))};
+  else return PtrRead<V>{p};
 }
 
 // An (even, odd) x-pair of S cells as float2 (FP16 widens exactly), and its

@@ -270,13 +270,21 @@ void swap_vel(Ctx *ctx) {
   std::swap(ctx->w, ctx->w2);
 }
 
+// The gather kernels read their fields through buffer resources with 32-bit
+// element indices when the largest (the packed centre velocity) is under
+// 2^31 bytes, else through plain pointers.
+template <class T> bool gather_buf(const Dims &d) {
+  return 3 * d.cells() * sizeof(T) < ((size_t)1 << 31);
+}
+
 int launch_advect(Ctx *ctx, double dt) {
   const Dims &d = ctx->d[0];
   {
     ProfScope ps(ctx->prof, ctx->stream, KC_ADVECT);
     with_step_type(ctx->t0, [&](auto tag) {
       using T = std::remove_pointer_t<decltype(tag)>;
-      hipLaunchKernelGGL(k_advect_mac<T>, gather_grid(d), kGatherBlock, 0, ctx->stream, d,
+      auto kern = gather_buf<T>(d) ? k_advect_mac<T, true> : k_advect_mac<T, false>;
+      hipLaunchKernelGGL(kern, gather_grid(d), kGatherBlock, 0, ctx->stream, d,
                          (T)(dt / ctx->cfg.dx), (T)ctx->cfg.vel_damp, (const T *)ctx->u.ptr, (const T *)ctx->v.ptr,
                          (const T *)ctx->w.ptr, (const typename cvec_of<T>::type *)ctx->cvel.ptr,
                          (T *)ctx->u2.ptr, (T *)ctx->v2.ptr,
@@ -348,7 +356,8 @@ int launch_dye_predict(Ctx *ctx, double dt) {
   ProfScope ps(ctx->prof, ctx->stream, KC_DYE);
   with_step_type(ctx->t0, [&](auto tag) {
     using T = std::remove_pointer_t<decltype(tag)>;
-    hipLaunchKernelGGL(k_dye_predict<T>, gather_grid(d), kGatherBlock, 0, ctx->stream, d,
+    auto kern = gather_buf<T>(d) ? k_dye_predict<T, true> : k_dye_predict<T, false>;
+    hipLaunchKernelGGL(kern, gather_grid(d), kGatherBlock, 0, ctx->stream, d,
                        (T)(dt / ctx->cfg.dx),
                        (const typename cvec_of<T>::type *)ctx->cvel.ptr,
                        (const T *)ctx->dye.ptr, (T *)ctx->dpred.ptr,
@@ -367,15 +376,16 @@ int launch_dye_resolve(Ctx *ctx, double dt) {
       using T = std::remove_pointer_t<decltype(tag)>;
       const T dth = (T)(dt / ctx->cfg.dx), s = (T)ctx->cfg.maccormack;
       const auto *cv = (const typename cvec_of<T>::type *)ctx->cvel.ptr;
+      const bool buf = gather_buf<T>(d);
+      auto kmm = buf ? k_dye_resolve_mm<T, true> : k_dye_resolve_mm<T, false>;
+      auto kfull = buf ? k_dye_resolve<T, true> : k_dye_resolve<T, false>;
       if (ctx->dmm_valid)
-        hipLaunchKernelGGL(k_dye_resolve_mm<T>, gather_grid(d), kGatherBlock, 0, ctx->stream, d,
-                           dth, s, cv, (const T *)ctx->dye.ptr, (const T *)ctx->dpred.ptr,
-                           (const typename vec2_of<T>::type *)ctx->dmm.ptr,
-                           (T *)ctx->dye2.ptr);
+        hipLaunchKernelGGL(kmm, gather_grid(d), kGatherBlock, 0, ctx->stream, d, dth, s, cv,
+                           (const T *)ctx->dye.ptr, (const T *)ctx->dpred.ptr,
+                           (const typename vec2_of<T>::type *)ctx->dmm.ptr, (T *)ctx->dye2.ptr);
       else
-        hipLaunchKernelGGL(k_dye_resolve<T>, gather_grid(d), kGatherBlock, 0, ctx->stream, d,
-                           dth, s, cv, (const T *)ctx->dye.ptr, (const T *)ctx->dpred.ptr,
-                           (T *)ctx->dye2.ptr);
+        hipLaunchKernelGGL(kfull, gather_grid(d), kGatherBlock, 0, ctx->stream, d, dth, s, cv,
+                           (const T *)ctx->dye.ptr, (const T *)ctx->dpred.ptr, (T *)ctx->dye2.ptr);
     });
   }
   LAUNCHCHK();

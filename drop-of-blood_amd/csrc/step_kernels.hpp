@@ -68,11 +68,10 @@ __device__ inline void trace_back(const Dims &d,
 // moves it, the first sample's weights are 1/2 along a and 0 on the other
 // axes, so it is mixv(cvel[c], cvel[c + sa], 1/2) -- the value of the
 // 8-texel sample (mixv(x, y, 0) = x exactly) from 2 loads.
-template <class T>
-__device__ inline void trace_back_tp(const Dims &d,
-                                     const typename cvec_of<T>::type *__restrict__ cvel,
-                                     T dth, const TPos<T> &f, const T lo[3], const T hi[3],
-                                     size_t c, size_t sa, TPos<T> &out) {
+template <class T, class CR>
+__device__ inline void trace_back_tp(const Dims &d, const CR &cvel, T dth, const TPos<T> &f,
+                                     const T lo[3], const T hi[3], typename CR::index c,
+                                     typename CR::index sa, TPos<T> &out) {
   TPos<T> p0 = f;
   tpos_clamp(p0, lo, hi);
   T v0[3], vm[3];
@@ -80,17 +79,17 @@ __device__ inline void trace_back_tp(const Dims &d,
 #pragma unroll
   for (int q = 0; q < 3; ++q) moved |= p0.b[q] != f.b[q] || p0.o[q] != f.o[q];
   if (!moved) {
-    auto a0 = cvel[c], a1 = cvel[c + sa];
+    auto a0 = cvel(c), a1 = cvel(c + sa);
     v0[0] = mixv(a0.x, a1.x, T(0.5));
     v0[1] = mixv(a0.y, a1.y, T(0.5));
     v0[2] = mixv(a0.z, a1.z, T(0.5));
   } else {
-    sample_vec_tp<T>(cvel, d, p0, v0);
+    sample_vec_tp_r<T>(cvel, d, p0, v0);
   }
   TPos<T> m = f;
   for (int a = 0; a < 3; ++a) m.o[a] -= T(0.5) * (v0[a] * dth);
   tpos_clamp(m, lo, hi);
-  sample_vec_tp<T>(cvel, d, m, vm);
+  sample_vec_tp_r<T>(cvel, d, m, vm);
   out = f;
   for (int a = 0; a < 3; ++a) out.o[a] -= vm[a] * dth;
   tpos_clamp(out, lo, hi);
@@ -102,13 +101,19 @@ __device__ inline void trace_back_tp(const Dims &d,
 // the two solid wall faces stay 0.  sc = dt / domain size per axis.
 // (FP32: at least 6 waves per SIMD, more gathers in flight, measured 248
 // against 264 us at 192x256x192; 8 waves forces too few registers and was
-// slower.  FP64 needs its registers.)
-template <class T>
+// slower.  FP64 needs its registers.)  BUF: fields read through buffer
+// resources with 32-bit element indices (fields under 2^31 bytes).
+template <class T, bool BUF>
 __global__ void __launch_bounds__(256, sizeof(T) == 4 ? 6 : 1)
 k_advect_mac(Dims d, T dth, T damp, const T *__restrict__ u, const T *__restrict__ v,
              const T *__restrict__ w, const typename cvec_of<T>::type *__restrict__ cvel,
              T *__restrict__ uo, T *__restrict__ vo, T *__restrict__ wo) {
   MAC3D_CELL_PROLOGUE
+  const size_t n = d.cells();
+  const auto cv = make_reader<BUF>(cvel, n);
+  typedef typename decltype(cv)::index I;
+  const decltype(make_reader<BUF>(u, n)) rf[3] = {make_reader<BUF>(u, n), make_reader<BUF>(v, n),
+                                                  make_reader<BUF>(w, n)};
   const T N[3] = {(T)d.nx, (T)d.ny, (T)d.nz};
   T lo2[3], hi2[3], lo1[3], hi1[3];
   for (int a = 0; a < 3; ++a) {
@@ -117,7 +122,6 @@ k_advect_mac(Dims d, T dth, T damp, const T *__restrict__ u, const T *__restrict
     lo1[a] = T(0.001) * N[a] - T(0.5);
     hi1[a] = T(0.999) * N[a] - T(0.5);
   }
-  const T *fld[3] = {u, v, w};
   T res[3];
 #pragma unroll
   for (int a = 0; a < 3; ++a) {
@@ -127,11 +131,11 @@ k_advect_mac(Dims d, T dth, T damp, const T *__restrict__ u, const T *__restrict
     f.o[0] = f.o[1] = f.o[2] = T(0);
     f.o[a] = T(0.5);
     TPos<T> b;
-    const size_t sa = a == 0 ? 1 : (a == 1 ? (size_t)d.nx : (size_t)d.nx * d.ny);
-    trace_back_tp<T>(d, cvel, dth, f, lo2, hi2, c, sa, b);
+    const I sa = a == 0 ? 1 : (a == 1 ? (I)d.nx : (I)d.nx * d.ny);
+    trace_back_tp<T>(d, cv, dth, f, lo2, hi2, (I)c, sa, b);
     b.o[a] -= T(0.5);  // the -1/2 texel shift of sampleMacU/V/W
     tpos_clamp(b, lo1, hi1);
-    res[a] = sample_tp(fld[a], d, b) * damp;
+    res[a] = sample_tp_r<T>(rf[a], d, b) * damp;
   }
   uo[c] = (i >= d.nx - 1) ? T(0) : res[0];
   vo[c] = res[1];
@@ -385,17 +389,16 @@ __device__ inline void dye_back(const Dims &d, const typename cvec_of<T>::type *
 
 // dye_back in texel space from the cell centre (no clamp on the first
 // sample), clamps at [0.001 N - 1/2, 0.999 N - 1/2]
-template <class T>
-__device__ inline void dye_back_tp(const Dims &d, const typename cvec_of<T>::type *__restrict__ cvel,
-                                   T dth, int i, int j, int k, const T lo[3], const T hi[3],
-                                   T v0[3], TPos<T> &back) {
+template <class T, class CR>
+__device__ inline void dye_back_tp(const Dims &d, const CR &cvel, T dth, int i, int j, int k,
+                                   const T lo[3], const T hi[3], T v0[3], TPos<T> &back) {
   TPos<T> c;
   c.b[0] = i; c.b[1] = j; c.b[2] = k;
   c.o[0] = c.o[1] = c.o[2] = T(0);
   // the first sample sits on the texel centre (weights 0: mixv(x, y, 0) = x
   // exactly), so it is the texel's own value
   {
-    auto a0 = cvel[idx3(d, i, j, k)];
+    auto a0 = cvel((typename CR::index)idx3(d, i, j, k));
     v0[0] = a0.x;
     v0[1] = a0.y;
     v0[2] = a0.z;
@@ -404,7 +407,7 @@ __device__ inline void dye_back_tp(const Dims &d, const typename cvec_of<T>::typ
   for (int a = 0; a < 3; ++a) m.o[a] -= T(0.5) * (v0[a] * dth);
   tpos_clamp(m, lo, hi);
   T vm[3];
-  sample_vec_tp<T>(cvel, d, m, vm);
+  sample_vec_tp_r<T>(cvel, d, m, vm);
   back = c;
   for (int a = 0; a < 3; ++a) back.o[a] -= vm[a] * dth;
   tpos_clamp(back, lo, hi);
@@ -423,42 +426,46 @@ __device__ inline void dye_bounds(const Dims &d, T lo[3], T hi[3]) {
 // (nullable) takes the least and greatest of the 8 dye texels of that sample
 // per cell, which are the bounds CONC_RESOLVE's limiter clamps to, so that
 // k_dye_resolve_mm need not trace back again.
-template <class T>
+template <class T, bool BUF>
 __global__ void __launch_bounds__(256)
 k_dye_predict(Dims d, T dth, const typename cvec_of<T>::type *__restrict__ cvel,
               const T *__restrict__ dye, T *__restrict__ pred,
               typename vec2_of<T>::type *__restrict__ mm) {
   MAC3D_CELL_PROLOGUE
+  const size_t n = d.cells();
+  const auto cv = make_reader<BUF>(cvel, n);
+  const auto rd = make_reader<BUF>(dye, n);
   T lo[3], hi[3], v0[3];
   dye_bounds(d, lo, hi);
   TPos<T> back;
-  dye_back_tp<T>(d, cvel, dth, i, j, k, lo, hi, v0, back);
+  dye_back_tp<T>(d, cv, dth, i, j, k, lo, hi, v0, back);
   if (mm) {
     typename vec2_of<T>::type b;
-    pred[c] = sample_tp_mm(dye, d, back, b.x, b.y);
+    pred[c] = sample_tp_mm_r<T>(rd, d, back, b.x, b.y);
     mm[c] = b;
   } else {
-    pred[c] = sample_tp(dye, d, back);
+    pred[c] = sample_tp_r<T>(rd, d, back);
   }
 }
 
 // CONC_RESOLVE's MacCormack core (advect.ts:196-222): forward estimate from
 // the predictor at uvw + v0 dt, corrected value pred + s (orig - estimate),
 // clamped to the min/max of the 8 source texels around the backtrace point.
-template <class T>
+template <class T, bool BUF>
 __global__ void __launch_bounds__(256)
 k_dye_resolve(Dims d, T dth, T strength, const typename cvec_of<T>::type *__restrict__ cvel,
               const T *__restrict__ dye, const T *__restrict__ pred, T *__restrict__ out) {
   MAC3D_CELL_PROLOGUE
+  const size_t n = d.cells();
   T lo[3], hi[3], v0[3];
   dye_bounds(d, lo, hi);
   TPos<T> back;
-  dye_back_tp<T>(d, cvel, dth, i, j, k, lo, hi, v0, back);
+  dye_back_tp<T>(d, make_reader<BUF>(cvel, n), dth, i, j, k, lo, hi, v0, back);
   TPos<T> fw;
   fw.b[0] = i; fw.b[1] = j; fw.b[2] = k;
   for (int a = 0; a < 3; ++a) fw.o[a] = v0[a] * dth;
   tpos_clamp(fw, lo, hi);
-  T est = sample_tp(pred, d, fw);
+  T est = sample_tp_r<T>(make_reader<BUF>(pred, n), d, fw);
   T corr = pred[c] + strength * (dye[c] - est);
   int bi = back.b[0] + (int)floor(back.o[0]);
   int bj = back.b[1] + (int)floor(back.o[1]);
@@ -477,7 +484,7 @@ k_dye_resolve(Dims d, T dth, T strength, const typename cvec_of<T>::type *__rest
 
 // k_dye_resolve with the limiter's bounds from k_dye_predict (mm, formed from
 // the same dye and centre velocity): the same value with no second backtrace.
-template <class T>
+template <class T, bool BUF>
 __global__ void __launch_bounds__(256)
 k_dye_resolve_mm(Dims d, T dth, T strength, const typename cvec_of<T>::type *__restrict__ cvel,
                  const T *__restrict__ dye, const T *__restrict__ pred,
@@ -492,7 +499,7 @@ k_dye_resolve_mm(Dims d, T dth, T strength, const typename cvec_of<T>::type *__r
   fw.o[1] = a0.y * dth;
   fw.o[2] = a0.z * dth;
   tpos_clamp(fw, lo, hi);
-  T est = sample_tp(pred, d, fw);
+  T est = sample_tp_r<T>(make_reader<BUF>(pred, d.cells()), d, fw);
   T corr = pred[c] + strength * (dye[c] - est);
   const auto b = mm[c];
   out[c] = clampv(corr, b.x, b.y);
