@@ -724,12 +724,14 @@ k_rbgs_sweep(Dims d, const S *__restrict__ xin, S *__restrict__ xout, const S *_
 // arithmetic of k_rbgs_sweep (rb_cell).
 //
 // LDS holds planes of x in a 7-slot ring (tile plus four cells, 72 x 24 for
-// 64 x 16) and b in a 5-slot ring (tile plus three cells, 72 x 22), each plane split by
-// colour: [colour][row][pair], pair = (even, odd) x-pair.  A cell's six
-// neighbours are of the other colour: x-/x+ at pairs P-1+s and P+s of its row
-// (s: the cell's side in its pair), y and z at pair P, so every stage reads
-// LDS at stride one.  Iteration j of the march runs three groups with a
-// barrier after each:
+// 64 x 16) and b in a 5-slot ring (tile plus three cells, 72 x 22), each row
+// split by colour: [row][colour][pair], pair = (even, odd) x-pair.  A cell's
+// six neighbours are of the other colour: in x the other cell of its pair P
+// and a cell of pair P-1 or P+1 (by the cell's side), in y and z the cells at
+// pair P of the rows and planes either side, so every stage reads LDS at
+// stride one.  The z neighbours of a stage's first pair stay in registers
+// from one plane to the next.  Iteration j of the march runs three groups
+// with a barrier after each:
 //   B2(j) (reads slots j-1..j+1, writes plane j of xout) with R1(j+4) (reads
 //     slots j+3..j+5, writes red of j+4 and b(j+4) into its b slot);
 //   B1(j+3);
@@ -750,7 +752,7 @@ k_rbgs_sweep(Dims d, const S *__restrict__ xin, S *__restrict__ xout, const S *_
 
 // W1: omega is 1 (the plain Gauss-Seidel sweep), so no over-relaxation blend.
 template <bool ZERO, int TX, int TY, class S, bool W1>
-__global__ void __launch_bounds__(TX / 2 * TY)
+__global__ void __launch_bounds__(TX / 2 * TY, TX == 64 ? 4 : 3)
 k_rbgs_sweep2(Dims d, const S *__restrict__ xin, S *__restrict__ xout,
               const S *__restrict__ b, float h2, float topc, float omega,
               const double *__restrict__ scale, int cz, int ncx, int ncy) {
@@ -759,13 +761,13 @@ k_rbgs_sweep2(Dims d, const S *__restrict__ xin, S *__restrict__ xout,
   constexpr int TB_PW = (TX + 8) / 2;   // pairs per staged row: four-cell ring
   constexpr int TB_EY = TY + 8;
   constexpr int TB_BY = TY + 6;         // b rows: three-cell ring
-  constexpr int TB_CH = TB_EY * TB_PW;  // one colour of an x slot
-  constexpr int TB_BCH = TB_BY * TB_PW; // one colour of a b slot
+  constexpr int CS = TB_PW * 4;         // byte offset of a row's black pairs
+  constexpr int RS = 2 * CS;            // bytes per staged row, both colours
   typedef float C;
   typedef float2 V2;
   static_assert(TB_T == TB_TX / 2 * TB_TY, "one tile pair per thread");
   static_assert(TB_PW * TB_EY <= 2 * TB_T && TB_PW * TB_BY <= 2 * TB_T, "two pairs per thread");
-  __shared__ C lds[TB_XR * 2 * TB_CH + TB_BR * 2 * TB_BCH];
+  __shared__ C lds[(TB_XR * TB_EY + TB_BR * TB_BY) * 2 * TB_PW];
   char *const L = (char *)lds;
   const int nx = d.nx, ny = d.ny, nz = d.nz;
   const int lb = xcd_remap(blockIdx.x, gridDim.x);
@@ -777,13 +779,15 @@ k_rbgs_sweep2(Dims d, const S *__restrict__ xin, S *__restrict__ xout,
   const int pb = (int)((size_t)nx * ny * sizeof(S));
 
   // pairs of the x staging (tile plus four cells) and of R1 (rows of the
-  // three-cell ring, the same columns): in-plane byte offsets
-  int xo[2], bo[2];
+  // three-cell ring, the same columns): in-plane byte offsets, and the
+  // staged pair's byte offset in a slot (red cell; black CS further)
+  int xo[2], bo[2], so[2];
 #pragma unroll
   for (int q = 0; q < 2; ++q) {
     int v = tid + q * TB_T;
     int row = v / TB_PW, pc = v % TB_PW;
     int gx = x0 - 4 + 2 * pc, gy = y0 - 4 + row;
+    so[q] = row * RS + 4 * pc;
     xo[q] = (v < TB_PW * TB_EY && gx >= 0 && gx < nx && gy >= 0 && gy < ny)
                 ? (gy * nx + gx) * (int)sizeof(S) : MAC3D_OFF_NONE;
     gy += 1;
@@ -801,28 +805,31 @@ k_rbgs_sweep2(Dims d, const S *__restrict__ xin, S *__restrict__ xout,
 
   // ---- per-thread tables, fixed over the march ----------------------------
   // Stage slots: 0 = B2 (the thread's tile pair), 1-2 = R1, 3-4 = B1, 5-6 =
-  // R2 (pairs tid and tid + TB_T of the stage's ring).  Per slot: byte
-  // offset of the pair in a colour array and of its x- neighbour at even
-  // planes; per slot and plane parity: 1/diag on planes with both z
-  // neighbours in the grid (srd) and on planes 0 and nz - 1 (srb), and bits
-  // 4 st + 2 par (cell valid) and 4 st + 2 par + 1 (cell
-  // on the right of its pair).  Bits 28 + q: staged pair q has red on the
-  // right at even planes; bit 30: staged pair 1 exists; bit 31: R1 pair 1
-  // exists.
-  int sofs[7], sxm[7];
+  // R2 (pairs tid and tid + TB_T of the stage's ring).  Per slot: ob, the
+  // byte offset in a slot of the pair's red cell one row up (the pair's own
+  // row starts RS further), and per plane parity fo, the byte offset of the
+  // cell's x neighbour in the next pair over (pair - 1 for a cell on the
+  // left, + 1 on the right); 1/diag on planes with both z neighbours in the
+  // grid (srd) and on planes 0 and nz - 1 (srb); bits 4 st + 2 par (cell
+  // valid) and 4 st + 2 par + 1 (cell on the right of its pair).  Bits 28 +
+  // q: staged pair q has red on the right at even planes; bit 30: staged
+  // pair 1 exists; bit 31: R1 pair 1 exists.
+  int ob[7], fo[2][7];
   C srd[2][7], srb[2][7];
   unsigned bits = 0;
   auto setup = [&](int st, int W, int X, int v, int NP) {
     int PAD = (W + 1) & ~1, PC = TB_TX / 2 + PAD;
-    bool act = v < NP;
-    int row = v / PC, pc = v % PC;
+    // lanes past the stage's pairs take pair 0's addresses (they store
+    // nothing, and their reads stay inside the ring)
+    const bool act = v < NP;
+    const int row = act ? v / PC : 0, pc = act ? v % PC : 0;
     int gy = y0 - W + row, gxl = x0 - PAD + 2 * pc;
-    int ip = (row + 4 - W) * TB_PW + pc + (4 - PAD) / 2;
-    sofs[st] = 4 * ip;
+    int xr = row + 4 - W, pcol = pc + (4 - PAD) / 2;  // staged row and pair column
+    ob[st] = (xr - 1) * RS + 4 * pcol;
 #pragma unroll
     for (int par = 0; par < 2; ++par) {
       int sd = (gy + par + X) & 1, gx = gxl + sd;
-      if (par == 0) sxm[st] = 4 * (ip - 1 + sd);
+      fo[par][st] = xr * RS + (1 - X) * CS + 4 * (pcol + (sd ? 1 : -1));
       bool ok = act && gx >= x0 - W && gx < x0 + TB_TX + W && gx >= 0 && gx < nx && gy >= 0 &&
                 gy < ny;
       // rb_diag with both z neighbours in the grid (srd) and with one (srb,
@@ -848,7 +855,7 @@ k_rbgs_sweep2(Dims d, const S *__restrict__ xin, S *__restrict__ xout,
   if (tid + TB_T < TB_PW * TB_BY) bits |= 1u << 31;
   // Second pairs of R1, B1 and R2 exist for the first threads only (280, 168
   // and 100 of 512 on a 64 x 16 tile): whole waves past them skip those
-  // updates (a scalar branch; their lanes would write back old values).
+  // updates (a scalar branch; their lanes write nothing).
   const int wb = __builtin_amdgcn_readfirstlane(tid & ~63) + TB_T;
   const bool w2 = wb < TB_PW * TB_BY;
   const bool w4 = wb < (TB_TX / 2 + 2) * (TB_TY + 4);
@@ -859,60 +866,51 @@ k_rbgs_sweep2(Dims d, const S *__restrict__ xin, S *__restrict__ xout,
   S *const tout = xout + (size_t)tgy * nx + tgx;
 
   // ---- LDS addressing -------------------------------------------------------
-  // Slot byte bases: x slot of colour X at XS + X * TB_CH * 4, b slot of
-  // colour X at BS + X * TB_BCH * 4 (the b base one row low: b rows start
-  // one below the x slot's).
-  constexpr int XSL = 2 * TB_CH * 4, BSL = 2 * TB_BCH * 4, XCH = TB_CH * 4, BCH = TB_BCH * 4;
-  constexpr int BOFF = TB_XR * XSL - TB_PW * 4;
+  // A slot holds one plane, row by row: TB_PW red pairs, then TB_PW black
+  // ones (row-interleaved), so the cell of a stage pair, the other colour's
+  // cell of the pair and that colour's cells one row down and up lie at fixed
+  // offsets from one base (ob), within the reach of the LDS instructions'
+  // immediate offsets.  x slots come first, then b slots, whose rows start
+  // one row lower than the x slot's (so ob addresses b too).
+  constexpr int XSL = TB_EY * RS, BSL = TB_BY * RS, BOFF = TB_XR * XSL;
   auto ld = [&](int a) { return *(const C *)(L + a); };
   auto st_ = [&](int a, C v) { *(C *)(L + a) = v; };
-  // new value of the cell of stage slot st at plane parity par, interior
-  // plane: own colour at own, other colour at oth (same plane), om, op
-  // (planes below and above)
-  // (x- pair offset at odd planes: the cell changes side, 2 ofs - 4 - even)
-  auto calc = [&](int st, int par, bool bnd, int own, int oth, int om, int op, C bv) -> C {
-    int o = sofs[st], xm = oth + (par ? 2 * o - 4 - sxm[st] : sxm[st]);
-    const C rdg = bnd ? srb[par][st] : srd[par][st];
-    if (W1)
-      return rb_cell_r1<C>(ld(xm), ld(xm + 4), ld(oth + o - 4 * TB_PW), ld(oth + o + 4 * TB_PW),
-                           ld(om + o), ld(op + o), bv, h2, rdg);
-    return rb_cell_r<C>(ld(xm), ld(xm + 4), ld(oth + o - 4 * TB_PW), ld(oth + o + 4 * TB_PW),
-                        ld(om + o), ld(op + o), bv, ld(own + o), h2, rdg, omega);
-  };
   // all-ones where bit k of bits is set (a sign-extending bit extract), and
-  // a select by such a mask (bitfield insert): no predicate registers
+  // a select by such a mask: no predicate registers
   auto mask = [&](int k) { return ((int)(bits << (31 - k))) >> 31; };
   auto pick = [](int m, C a, C b) {  // m ? a : b
     return __int_as_float((__float_as_int(a) & m) | (__float_as_int(b) & ~m));
   };
-  // Write of a stage cell: cells outside the ring or the grid write back the
-  // old value.  Lanes past a stage's pairs are not clamped: their pair index
-  // runs past the ring's rows into rows no stage of the same group writes,
-  // so the write-back is a no-op there too.
-  auto calc_w = [&](int st, int par, bool bnd, int own, int oth, int om, int op, C bv) {
-    int o = sofs[st];
-    C old = ld(own + o);
-    int xm = oth + (par ? 2 * o - 4 - sxm[st] : sxm[st]);
+  // the other colour's cell of stage slot st's pair in the slot at byte Sb
+  // (X: the stage's colour)
+  auto oc_at = [&](int st, int Sb, int X) { return ld(Sb + ob[st] + RS + (1 - X) * CS); };
+  // new value of the cell (colour X) of stage slot st at plane parity par in
+  // slot Sb, given the other colour's cell of its pair (sm) and its z
+  // neighbours; the x neighbours add in either order (the sum is the same)
+  auto upd = [&](int st, int par, bool bnd, int Sb, int X, C sm, C zm, C zp, C bv) -> C {
+    const int a = Sb + ob[st], oc = (1 - X) * CS;
+    const C ym = ld(a + oc), yp = ld(a + 2 * RS + oc), fx = ld(Sb + fo[par][st]);
     const C rdg = bnd ? srb[par][st] : srd[par][st];
-    C nv = W1 ? rb_cell_r1<C>(ld(xm), ld(xm + 4), ld(oth + o - 4 * TB_PW),
-                              ld(oth + o + 4 * TB_PW), ld(om + o), ld(op + o), bv, h2, rdg)
-              : rb_cell_r<C>(ld(xm), ld(xm + 4), ld(oth + o - 4 * TB_PW),
-                             ld(oth + o + 4 * TB_PW), ld(om + o), ld(op + o), bv, old, h2, rdg,
-                             omega);
-    return pick(mask(4 * st + 2 * par), nv, old);
+    if (W1) return rb_cell_r1<C>(sm, fx, ym, yp, zm, zp, bv, h2, rdg);
+    return rb_cell_r<C>(sm, fx, ym, yp, zm, zp, bv, ld(a + RS + X * CS), h2, rdg, omega);
+  };
+  // store of a stage cell: only cells in the stage's ring and the grid
+  // (lanes past the stage's pairs write nothing)
+  auto put = [&](int st, int par, int Sb, int X, C v) {
+    if ((bits >> (4 * st + 2 * par)) & 1u) st_(Sb + ob[st] + RS + X * CS, v);
   };
 
   // the prologue's staging (plane p at x ring position (p - (k0 - 4)) mod 7)
   const int px = k0 - 4;
   auto stage_x_g = [&](int p, const V2 (&pre)[2]) {
-    C *red = lds + ((p - px) % TB_XR) * (2 * TB_CH), *blk = red + TB_CH;
+    const int Sb = ((p - px) % TB_XR) * XSL;
 #pragma unroll
     for (int q = 0; q < 2; ++q) {
       int v = tid + q * TB_T;
       if (v < TB_PW * TB_EY) {
         bool rs = (y0 - 4 + v / TB_PW + p) & 1;
-        red[v] = rs ? pre[q].y : pre[q].x;
-        blk[v] = rs ? pre[q].x : pre[q].y;
+        st_(Sb + so[q], rs ? pre[q].y : pre[q].x);
+        st_(Sb + so[q] + CS, rs ? pre[q].x : pre[q].y);
       }
     }
   };
@@ -925,30 +923,38 @@ k_rbgs_sweep2(Dims d, const S *__restrict__ xin, S *__restrict__ xout,
   // base of plane j - 1 + d (plane j + 6 reuses xsl[0]), bsl[d] the b slot
   // base of plane j + d
   int xsl[7], bsl[5];
+  // z carries of the first pair of each stage (B2, R1, B1, R2): the other
+  // colour's cell of the pair on the stage's plane (cm) and on the plane
+  // above (cs) are that cell's z- neighbour and pair cell one iteration later
+  // (no stage writes them in between), so unchecked iterations read only the
+  // plane above from LDS
+  C cm[4], cs[4];
   // CHK false: a steady-state iteration, in which every stage is in range and
-  // no plane is 0 or nz - 1 (the checks compile away)
+  // no plane is 0 or nz - 1 (the checks compile away), and the stages ran in
+  // the iteration before (the carries are set)
   auto iter = [&](auto PARc, auto CHKc, int j, V2 (&xr)[2], V2 (&br)[2]) {
     constexpr int par = decltype(PARc)::value;
     constexpr bool CHK = decltype(CHKc)::value;
     auto edge = [&](int p) { return CHK && (p == 0 || p == nz - 1); };
     // group 1: B2(j), R1(j + 4)
     if (!CHK || j >= k0) {
-      int own = xsl[1] + XCH, oth = xsl[1];
-      C vb = calc(0, par, edge(j), own, oth, xsl[0], xsl[2], ld(bsl[0] + BCH + sofs[0]));
-      C vr = ld(oth + sofs[0]);
+      const C zm = CHK ? oc_at(0, xsl[0], 1) : cm[0];
+      const C sm = CHK ? oc_at(0, xsl[1], 1) : cs[0];
+      const C zp = oc_at(0, xsl[2], 1);
+      const C vb = upd(0, par, edge(j), xsl[1], 1, sm, zm, zp, ld(bsl[0] + ob[0] + CS));
+      cm[0] = sm;
+      cs[0] = zp;
       if (tin) {
         int rt = mask(2 * par + 1);
         V2 o;
-        o.x = pick(rt, vr, vb);
-        o.y = pick(rt, vb, vr);
+        o.x = pick(rt, sm, vb);
+        o.y = pick(rt, vb, sm);
         store_pair(tout + (size_t)j * nx * ny, o);
       }
     }
     if (!CHK || (j + 4 >= r1lo && j + 4 < r1hi)) {
       const bool bnd = edge(j + 4);
-      int own = xsl[5], oth = xsl[5] + XCH, om = xsl[4] + XCH, op = xsl[6] + XCH;
-      int bR = bsl[4], bK = bsl[4] + BCH;
-      C nv[2], bred[2], bblk[2];
+      C bred[2], bblk[2];
 #pragma unroll
       for (int q = 0; q < 2; ++q) {
         C b0 = br[q].x * bs, b1 = br[q].y * bs;
@@ -956,15 +962,23 @@ k_rbgs_sweep2(Dims d, const S *__restrict__ xin, S *__restrict__ xout,
         bred[q] = pick(rt, b1, b0);
         bblk[q] = pick(rt, b0, b1);
       }
-      nv[0] = calc_w(1, par, bnd, own, oth, om, op, bred[0]);
-      if (w2) nv[1] = calc_w(2, par, bnd, own, oth, om, op, bred[1]);
-      st_(own + sofs[1], nv[0]);
-      if (w2) st_(own + sofs[2], nv[1]);
-      st_(bR + sofs[1], bred[0]);
-      st_(bK + sofs[1], bblk[0]);
+      const C zm = CHK ? oc_at(1, xsl[4], 0) : cm[1];
+      const C sm = CHK ? oc_at(1, xsl[5], 0) : cs[1];
+      const C zp = oc_at(1, xsl[6], 0);
+      const C nv0 = upd(1, par, bnd, xsl[5], 0, sm, zm, zp, bred[0]);
+      cm[1] = sm;
+      cs[1] = zp;
+      C nv1 = C(0);
+      if (w2)
+        nv1 = upd(2, par, bnd, xsl[5], 0, oc_at(2, xsl[5], 0), oc_at(2, xsl[4], 0),
+                  oc_at(2, xsl[6], 0), bred[1]);
+      put(1, par, xsl[5], 0, nv0);
+      if (w2) put(2, par, xsl[5], 0, nv1);
+      st_(bsl[4] + ob[1], bred[0]);
+      st_(bsl[4] + ob[1] + CS, bblk[0]);
       if (bits >> 31) {
-        st_(bR + sofs[2], bred[1]);
-        st_(bK + sofs[2], bblk[1]);
+        st_(bsl[4] + ob[2], bred[1]);
+        st_(bsl[4] + ob[2] + CS, bblk[1]);
       }
     }
     fetch(rb, kb0, j + 6, bo, br);
@@ -973,37 +987,44 @@ k_rbgs_sweep2(Dims d, const S *__restrict__ xin, S *__restrict__ xout,
     if (!CHK || (j + 3 >= b1lo && j + 3 < b1hi)) {
       constexpr int pp = par ^ 1;
       const bool bnd = edge(j + 3);
-      int own = xsl[4] + XCH, oth = xsl[4], om = xsl[3], op = xsl[5];
-      int bK = bsl[3] + BCH;
-      C nv0 = calc_w(3, pp, bnd, own, oth, om, op, ld(bK + sofs[3]));
-      st_(own + sofs[3], nv0);
-      if (w4) {
-        C nv1 = calc_w(4, pp, bnd, own, oth, om, op, ld(bK + sofs[4]));
-        st_(own + sofs[4], nv1);
-      }
+      const C zm = CHK ? oc_at(3, xsl[3], 1) : cm[2];
+      const C sm = CHK ? oc_at(3, xsl[4], 1) : cs[2];
+      const C zp = oc_at(3, xsl[5], 1);
+      const C nv0 = upd(3, pp, bnd, xsl[4], 1, sm, zm, zp, ld(bsl[3] + ob[3] + CS));
+      cm[2] = sm;
+      cs[2] = zp;
+      C nv1 = C(0);
+      if (w4)
+        nv1 = upd(4, pp, bnd, xsl[4], 1, oc_at(4, xsl[4], 1), oc_at(4, xsl[3], 1),
+                  oc_at(4, xsl[5], 1), ld(bsl[3] + ob[4] + CS));
+      put(3, pp, xsl[4], 1, nv0);
+      if (w4) put(4, pp, xsl[4], 1, nv1);
     }
     __syncthreads();
     // group 3: R2(j + 2), staging of x(j + 6)
     if (!CHK || (j + 2 >= r2lo && j + 2 < r2hi)) {
       const bool bnd = edge(j + 2);
-      int own = xsl[3], oth = xsl[3] + XCH, om = xsl[2] + XCH, op = xsl[4] + XCH;
-      int bR = bsl[2];
-      C nv0 = calc_w(5, par, bnd, own, oth, om, op, ld(bR + sofs[5]));
-      st_(own + sofs[5], nv0);
-      if (w6) {
-        C nv1 = calc_w(6, par, bnd, own, oth, om, op, ld(bR + sofs[6]));
-        st_(own + sofs[6], nv1);
-      }
+      const C zm = CHK ? oc_at(5, xsl[2], 0) : cm[3];
+      const C sm = CHK ? oc_at(5, xsl[3], 0) : cs[3];
+      const C zp = oc_at(5, xsl[4], 0);
+      const C nv0 = upd(5, par, bnd, xsl[3], 0, sm, zm, zp, ld(bsl[2] + ob[5]));
+      cm[3] = sm;
+      cs[3] = zp;
+      C nv1 = C(0);
+      if (w6)
+        nv1 = upd(6, par, bnd, xsl[3], 0, oc_at(6, xsl[3], 0), oc_at(6, xsl[2], 0),
+                  oc_at(6, xsl[4], 0), ld(bsl[2] + ob[6]));
+      put(5, par, xsl[3], 0, nv0);
+      if (w6) put(6, par, xsl[3], 0, nv1);
     }
     {
-      int red = xsl[0], blk = xsl[0] + XCH;
+      const int Sb = xsl[0];
 #pragma unroll
       for (int q = 0; q < 2; ++q) {
         if (q == 0 || ((bits >> 30) & 1)) {
           int rs = mask(28 + q) ^ (par ? -1 : 0);
-          int a = 4 * (tid + q * TB_T);
-          st_(red + a, pick(rs, xr[q].y, xr[q].x));
-          st_(blk + a, pick(rs, xr[q].x, xr[q].y));
+          st_(Sb + so[q], pick(rs, xr[q].y, xr[q].x));
+          st_(Sb + so[q] + CS, pick(rs, xr[q].x, xr[q].y));
         }
       }
     }
