@@ -179,12 +179,54 @@ __device__ inline TriIdx<T, I> tpos_index(const Dims &d, const TPos<T> &p) {
 // resource, so a gather needs no 64-bit index or address arithmetic.
 template <class V> struct PtrRead {
   typedef size_t index;
+  static constexpr bool bytes = false;  // offsets count elements
+  static constexpr int unit = 1;
   const V *__restrict__ p;
   __device__ V operator()(size_t i) const { return p[i]; }
+  __device__ V ld(size_t i) const { return p[i]; }
+  // offset of cell (i, j, k)
+  __device__ size_t cell(const Dims &d, int i, int j, int k) const { return idx3(d, i, j, k); }
 };
+
+// Corner byte offsets of a trilinear sample at p for a field of ES-byte
+// elements read through a buffer resource: o + {0, dx} + {0, dy} + {0, dz}.
+// Every sample point is clamped inside [-1/2, N - 1/2) texels on each axis,
+// so its lower corner index lies in [-1, N - 1] and clamp-to-edge only merges
+// the two corners at -1 and at N - 1 (d = 0 there).  The offsets come from
+// 24-bit multiplies (full rate; 32-bit ones are quarter rate), which need a
+// plane of the field under 2^24 bytes (the host checks).
+template <class T> struct TriOff {
+  int o, dx, dy, dz;
+  T fx, fy, fz;
+};
+template <class T, int ES>
+__device__ inline TriOff<T> tpos_off(const Dims &d, const TPos<T> &p) {
+  TriOff<T> t;
+  const T fl0 = floor(p.o[0]), fl1 = floor(p.o[1]), fl2 = floor(p.o[2]);
+  const int ib = p.b[0] + (int)fl0, jb = p.b[1] + (int)fl1, kb = p.b[2] + (int)fl2;
+  t.fx = p.o[0] - fl0;
+  t.fy = p.o[1] - fl1;
+  t.fz = p.o[2] - fl2;
+  const int sy = d.nx * ES, sz = d.nx * d.ny * ES;
+  t.dx = (unsigned)ib < (unsigned)(d.nx - 1) ? ES : 0;
+  t.dy = (unsigned)jb < (unsigned)(d.ny - 1) ? sy : 0;
+  t.dz = (unsigned)kb < (unsigned)(d.nz - 1) ? sz : 0;
+  t.o = __umul24((unsigned)max(kb, 0), (unsigned)sz) + __umul24((unsigned)max(jb, 0), (unsigned)sy) +
+        __umul24((unsigned)max(ib, 0), (unsigned)ES);
+  return t;
+}
 
 template <class T, class R>
 __device__ inline T sample_tp_r(const R &f, const Dims &d, const TPos<T> &p) {
+  if constexpr (R::bytes) {
+    const auto t = tpos_off<T, R::unit>(d, p);
+    const int o1 = t.o + t.dy, o2 = t.o + t.dz, o3 = o2 + t.dy;
+    T x00 = mixv(f.ld(t.o), f.ld(t.o + t.dx), t.fx);
+    T x10 = mixv(f.ld(o1), f.ld(o1 + t.dx), t.fx);
+    T x01 = mixv(f.ld(o2), f.ld(o2 + t.dx), t.fx);
+    T x11 = mixv(f.ld(o3), f.ld(o3 + t.dx), t.fx);
+    return mixv(mixv(x00, x10, t.fy), mixv(x01, x11, t.fy), t.fz);
+  }
   auto t = tpos_index<T, typename R::index>(d, p);
   T x00 = mixv(f(t.r00 + t.i0), f(t.r00 + t.i1), t.fx);
   T x10 = mixv(f(t.r10 + t.i0), f(t.r10 + t.i1), t.fx);
@@ -202,26 +244,48 @@ __device__ inline T sample_tp(const T *__restrict__ f, const Dims &d, const TPos
 // the same clamped corners, in this fmin/fmax order)
 template <class T, class R>
 __device__ inline T sample_tp_mm_r(const R &f, const Dims &d, const TPos<T> &p, T &mn, T &mx) {
-  auto t = tpos_index<T, typename R::index>(d, p);
-  T s0 = f(t.r00 + t.i0), s1 = f(t.r00 + t.i1), s2 = f(t.r10 + t.i0), s3 = f(t.r10 + t.i1);
-  T s4 = f(t.r01 + t.i0), s5 = f(t.r01 + t.i1), s6 = f(t.r11 + t.i0), s7 = f(t.r11 + t.i1);
+  T s0, s1, s2, s3, s4, s5, s6, s7, fx, fy, fz;
+  if constexpr (R::bytes) {
+    const auto t = tpos_off<T, R::unit>(d, p);
+    const int o1 = t.o + t.dy, o2 = t.o + t.dz, o3 = o2 + t.dy;
+    s0 = f.ld(t.o); s1 = f.ld(t.o + t.dx); s2 = f.ld(o1); s3 = f.ld(o1 + t.dx);
+    s4 = f.ld(o2); s5 = f.ld(o2 + t.dx); s6 = f.ld(o3); s7 = f.ld(o3 + t.dx);
+    fx = t.fx; fy = t.fy; fz = t.fz;
+  } else {
+    auto t = tpos_index<T, typename R::index>(d, p);
+    s0 = f(t.r00 + t.i0); s1 = f(t.r00 + t.i1); s2 = f(t.r10 + t.i0); s3 = f(t.r10 + t.i1);
+    s4 = f(t.r01 + t.i0); s5 = f(t.r01 + t.i1); s6 = f(t.r11 + t.i0); s7 = f(t.r11 + t.i1);
+    fx = t.fx; fy = t.fy; fz = t.fz;
+  }
   mn = fmin(fmin(fmin(s0, s1), fmin(s2, s3)), fmin(fmin(s4, s5), fmin(s6, s7)));
   mx = fmax(fmax(fmax(s0, s1), fmax(s2, s3)), fmax(fmax(s4, s5), fmax(s6, s7)));
-  T x00 = mixv(s0, s1, t.fx);
-  T x10 = mixv(s2, s3, t.fx);
-  T x01 = mixv(s4, s5, t.fx);
-  T x11 = mixv(s6, s7, t.fx);
-  return mixv(mixv(x00, x10, t.fy), mixv(x01, x11, t.fy), t.fz);
+  T x00 = mixv(s0, s1, fx);
+  T x10 = mixv(s2, s3, fx);
+  T x01 = mixv(s4, s5, fx);
+  T x11 = mixv(s6, s7, fx);
+  return mixv(mixv(x00, x10, fy), mixv(x01, x11, fy), fz);
 }
 
 template <class T, class R>
 __device__ inline void sample_vec_tp_r(const R &f, const Dims &d, const TPos<T> &p, T out[3]) {
-  auto t = tpos_index<T, typename R::index>(d, p);
-  auto a000 = f(t.r00 + t.i0), a100 = f(t.r00 + t.i1), a010 = f(t.r10 + t.i0), a110 = f(t.r10 + t.i1);
-  auto a001 = f(t.r01 + t.i0), a101 = f(t.r01 + t.i1), a011 = f(t.r11 + t.i0), a111 = f(t.r11 + t.i1);
+  typedef typename cvec_of<T>::type V3;
+  V3 a000, a100, a010, a110, a001, a101, a011, a111;
+  T fx, fy, fz;
+  if constexpr (R::bytes) {
+    const auto t = tpos_off<T, R::unit>(d, p);
+    const int o1 = t.o + t.dy, o2 = t.o + t.dz, o3 = o2 + t.dy;
+    a000 = f.ld(t.o); a100 = f.ld(t.o + t.dx); a010 = f.ld(o1); a110 = f.ld(o1 + t.dx);
+    a001 = f.ld(o2); a101 = f.ld(o2 + t.dx); a011 = f.ld(o3); a111 = f.ld(o3 + t.dx);
+    fx = t.fx; fy = t.fy; fz = t.fz;
+  } else {
+    auto t = tpos_index<T, typename R::index>(d, p);
+    a000 = f(t.r00 + t.i0); a100 = f(t.r00 + t.i1); a010 = f(t.r10 + t.i0); a110 = f(t.r10 + t.i1);
+    a001 = f(t.r01 + t.i0); a101 = f(t.r01 + t.i1); a011 = f(t.r11 + t.i0); a111 = f(t.r11 + t.i1);
+    fx = t.fx; fy = t.fy; fz = t.fz;
+  }
 #define MAC3D_TRI(C)                                                                    \
-  mixv(mixv(mixv(a000.C, a100.C, t.fx), mixv(a010.C, a110.C, t.fx), t.fy),              \
-       mixv(mixv(a001.C, a101.C, t.fx), mixv(a011.C, a111.C, t.fx), t.fy), t.fz)
+  mixv(mixv(mixv(a000.C, a100.C, fx), mixv(a010.C, a110.C, fx), fy),                    \
+       mixv(mixv(a001.C, a101.C, fx), mixv(a011.C, a111.C, fx), fy), fz)
   out[0] = MAC3D_TRI(x);
   out[1] = MAC3D_TRI(y);
   out[2] = MAC3D_TRI(z);
@@ -267,11 +331,20 @@ template <> __device__ inline double3 buf_load<double3>(BufRsrc r, int off) {
 }
 template <class V> struct BufRead {
   typedef int index;
+  static constexpr bool bytes = true;  // offsets count bytes
+  static constexpr int unit = (int)sizeof(V);
   BufRsrc r;
   __device__ V operator()(int i) const { return buf_load<V>(r, i * (int)sizeof(V)); }
+  __device__ V ld(int off) const { return buf_load<V>(r, off); }
+  // byte offset of cell (i, j, k) (24-bit multiplies: a plane is under 2^24 bytes)
+  __device__ int cell(const Dims &d, int i, int j, int k) const {
+    return __umul24((unsigned)k, (unsigned)(d.nx * d.ny * unit)) +
+           __umul24((unsigned)j, (unsigned)(d.nx * unit)) + __umul24((unsigned)i, (unsigned)unit);
+  }
 };
 // reader of n elements at p: a buffer resource when BUF (the host checks
-// that the field is under 2^31 bytes), else the plain pointer
+// that the field is under 2^31 bytes and a plane under 2^24), else the plain
+// pointer
 template <bool BUF, class V> struct Reader { typedef PtrRead<V> type; };
 template <class V> struct Reader<true, V> { typedef BufRead<V> type; };
 template <bool BUF, class V>

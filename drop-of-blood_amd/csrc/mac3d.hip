@@ -271,10 +271,12 @@ void swap_vel(Ctx *ctx) {
 }
 
 // The gather kernels read their fields through buffer resources with 32-bit
-// element indices when the largest (the packed centre velocity) is under
-// 2^31 bytes, else through plain pointers.
+// byte offsets (formed by 24-bit multiplies) when the largest field (the
+// packed centre velocity) is under 2^31 bytes and a plane of it under 2^24,
+// else through plain pointers.
 template <class T> bool gather_buf(const Dims &d) {
-  return 3 * d.cells() * sizeof(T) < ((size_t)1 << 31);
+  return 3 * d.cells() * sizeof(T) < ((size_t)1 << 31) &&
+         3 * (size_t)d.nx * d.ny * sizeof(T) < ((size_t)1 << 24);
 }
 
 int launch_advect(Ctx *ctx, double dt) {

@@ -64,7 +64,8 @@ __device__ inline void trace_back(const Dims &d,
 
 // traceBackUVW in texel space: the face point f (texel base + offset), RK2
 // displacement in texels v dt/h, clamps at [0.002 N - 1/2, 0.998 N - 1/2].
-// f is the +1/2 face of cell c along axis a (stride sa): unless the clamp
+// f is the +1/2 face of cell c along axis a (c and the axis stride sa in the
+// reader's offset unit): unless the clamp
 // moves it, the first sample's weights are 1/2 along a and 0 on the other
 // axes, so it is mixv(cvel[c], cvel[c + sa], 1/2) -- the value of the
 // 8-texel sample (mixv(x, y, 0) = x exactly) from 2 loads.
@@ -79,7 +80,7 @@ __device__ inline void trace_back_tp(const Dims &d, const CR &cvel, T dth, const
 #pragma unroll
   for (int q = 0; q < 3; ++q) moved |= p0.b[q] != f.b[q] || p0.o[q] != f.o[q];
   if (!moved) {
-    auto a0 = cvel(c), a1 = cvel(c + sa);
+    auto a0 = cvel.ld(c), a1 = cvel.ld(c + sa);
     v0[0] = mixv(a0.x, a1.x, T(0.5));
     v0[1] = mixv(a0.y, a1.y, T(0.5));
     v0[2] = mixv(a0.z, a1.z, T(0.5));
@@ -114,6 +115,7 @@ k_advect_mac(Dims d, T dth, T damp, const T *__restrict__ u, const T *__restrict
   typedef typename decltype(cv)::index I;
   const decltype(make_reader<BUF>(u, n)) rf[3] = {make_reader<BUF>(u, n), make_reader<BUF>(v, n),
                                                   make_reader<BUF>(w, n)};
+  const I cu = cv.cell(d, i, j, k);  // cell c in the reader's offset unit
   const T N[3] = {(T)d.nx, (T)d.ny, (T)d.nz};
   T lo2[3], hi2[3], lo1[3], hi1[3];
   for (int a = 0; a < 3; ++a) {
@@ -131,8 +133,8 @@ k_advect_mac(Dims d, T dth, T damp, const T *__restrict__ u, const T *__restrict
     f.o[0] = f.o[1] = f.o[2] = T(0);
     f.o[a] = T(0.5);
     TPos<T> b;
-    const I sa = a == 0 ? 1 : (a == 1 ? (I)d.nx : (I)d.nx * d.ny);
-    trace_back_tp<T>(d, cv, dth, f, lo2, hi2, (I)c, sa, b);
+    const I sa = (a == 0 ? 1 : (a == 1 ? (I)d.nx : (I)d.nx * d.ny)) * decltype(cv)::unit;
+    trace_back_tp<T>(d, cv, dth, f, lo2, hi2, cu, sa, b);
     b.o[a] -= T(0.5);  // the -1/2 texel shift of sampleMacU/V/W
     tpos_clamp(b, lo1, hi1);
     res[a] = sample_tp_r<T>(rf[a], d, b) * damp;
@@ -398,7 +400,7 @@ __device__ inline void dye_back_tp(const Dims &d, const CR &cvel, T dth, int i, 
   // the first sample sits on the texel centre (weights 0: mixv(x, y, 0) = x
   // exactly), so it is the texel's own value
   {
-    auto a0 = cvel((typename CR::index)idx3(d, i, j, k));
+    auto a0 = cvel.ld(cvel.cell(d, i, j, k));
     v0[0] = a0.x;
     v0[1] = a0.y;
     v0[2] = a0.z;
