@@ -103,7 +103,8 @@ __device__ inline void trace_back_tp(const Dims &d, const CR &cvel, T dth, const
 // (FP32: at least 6 waves per SIMD, more gathers in flight, measured 248
 // against 264 us at 192x256x192; 8 waves forces too few registers and was
 // slower.  FP64 needs its registers.)  BUF: fields read through buffer
-// resources with 32-bit element indices (fields under 2^31 bytes).
+// resources with 32-bit byte offsets (fields under 2^31 bytes, planes under
+// 2^24).
 template <class T, bool BUF>
 __global__ void __launch_bounds__(256, sizeof(T) == 4 ? 6 : 1)
 k_advect_mac(Dims d, T dth, T damp, const T *__restrict__ u, const T *__restrict__ v,
@@ -251,7 +252,7 @@ k_casson_zm(Dims d, StepParams P, const T *__restrict__ u, const T *__restrict__
             T *__restrict__ vo, T *__restrict__ wo, int cz, int ncx, int ncy) {
   __shared__ T su[4][CS_EY][CS_EX], sv[4][CS_EY][CS_EX], sw[4][CS_EY][CS_EX];
   const int nx = d.nx, ny = d.ny, nz = d.nz;
-  const int lb = blockIdx.x;
+  const int lb = xcd_remap(blockIdx.x, gridDim.x);  // one z-chunk of tiles per XCD
   const int x0 = (lb % ncx) * CS_TX, y0 = ((lb / ncx) % ncy) * CS_TY;
   const int k0 = (lb / (ncx * ncy)) * cz, k1 = min(k0 + cz, nz);
   const int tx = threadIdx.x, ty = threadIdx.y, tid = ty * CS_TX + tx;
