@@ -164,13 +164,15 @@ __device__ inline void residual64_zm(const Dims &d, const Src &src, int kbase, i
   }
 }
 
-// block -> (tile, chunk) and the chunk's buffer ranges (planes k0-1 .. k1)
+// block -> (tile, chunk) and the chunk's buffer ranges (planes k0-1 .. k1);
+// blocks in XCD-contiguous order (one z-chunk of tiles per XCD), so a
+// tile's ring lines are its neighbours' lines in the same L2
 struct R64Chunk {
   int x0, y0, k0, k1, kb, n;
 };
 __device__ inline R64Chunk r64_chunk(const Dims &d, int cz, int ncx, int ncy) {
   R64Chunk c;
-  int lb = blockIdx.x;
+  int lb = xcd_remap(blockIdx.x, gridDim.x);
   c.x0 = (lb % ncx) * R64_TX;
   c.y0 = ((lb / ncx) % ncy) * R64_TY;
   c.k0 = (lb / (ncx * ncy)) * cz;
