@@ -477,15 +477,21 @@ int mg_fill_zero(Ctx *ctx, Buf &b) {
   return MAC3D_OK;
 }
 
-// z-chunk of the two-sweep level-0 kernel: about two 512-thread blocks per CU
-// (the LDS of two fills a CU), at least 6 planes (each chunk pays seven
-// partial iterations at its start), even (the kernel's unrolled halves have
-// fixed plane parities), and a chunk's buffer ranges below 2^30 bytes
 // Tile width of the two-sweep pass: 32 on grids of up to 2^18 cells (twice
-// the blocks of a 64-wide tile, measured faster on 48x64x48), else 64.
-static int sweep2_tx(const Dims &d) { return d.cells() <= ((size_t)1 << 18) ? 32 : 64; }
+// the blocks of a 64-wide tile, measured faster on 48x64x48); 48 where 64-wide
+// tiles would overhang the grid and 48-wide ones fit it (96 columns: 16.5
+// against 17.5 us on 96x128x96); else 64.
+static int sweep2_tx(const Dims &d) {
+  if (d.cells() <= ((size_t)1 << 18)) return 32;
+  if (d.nx % 64 != 0 && d.nx % 48 == 0) return 48;
+  return 64;
+}
 #define SWEEP2_TY 16
 
+// z-chunk of the two-sweep pass: about two blocks per CU (the LDS of two
+// fills a CU), at least 6 planes (each chunk pays seven partial iterations at
+// its start), even (the kernel's unrolled halves have fixed plane parities),
+// and a chunk's buffer ranges below 2^30 bytes
 int sweep2_cz(const Dims &d, int tx) {
   int cols = ((d.nx + tx - 1) / tx) * ((d.ny + SWEEP2_TY - 1) / SWEEP2_TY);
   int chunks = std::max(1, (64 * 512 / tx) / cols);
@@ -511,10 +517,12 @@ int mg_sweep2(Ctx *ctx, int l, bool zero) {
     auto launch = [&](auto stag, auto w1tag) {
       using S = std::remove_pointer_t<decltype(stag)>;
       constexpr bool W1 = decltype(w1tag)::value;
-      auto kern = tx == 64 ? (zero ? k_rbgs_sweep2<true, 64, SWEEP2_TY, S, W1>
-                                   : k_rbgs_sweep2<false, 64, SWEEP2_TY, S, W1>)
-                           : (zero ? k_rbgs_sweep2<true, 32, SWEEP2_TY, S, W1>
-                                   : k_rbgs_sweep2<false, 32, SWEEP2_TY, S, W1>);
+      auto kern = tx == 64   ? (zero ? k_rbgs_sweep2<true, 64, SWEEP2_TY, S, W1>
+                                     : k_rbgs_sweep2<false, 64, SWEEP2_TY, S, W1>)
+                  : tx == 48 ? (zero ? k_rbgs_sweep2<true, 48, SWEEP2_TY, S, W1>
+                                     : k_rbgs_sweep2<false, 48, SWEEP2_TY, S, W1>)
+                             : (zero ? k_rbgs_sweep2<true, 32, SWEEP2_TY, S, W1>
+                                     : k_rbgs_sweep2<false, 32, SWEEP2_TY, S, W1>);
       hipLaunchKernelGGL(kern, grid, dim3(tx / 2 * SWEEP2_TY), 0, ctx->stream, d,
                          (const S *)ctx->mgx[l].ptr, (S *)ctx->mgxa[l].ptr,
                          (const S *)ctx->mgb[l].ptr, (float)level_h2(l),

@@ -716,9 +716,8 @@ k_rbgs_sweep(Dims d, const S *__restrict__ xin, S *__restrict__ xout, const S *_
 // are read once and x written once for two sweeps (12 B per FP32 cell per
 // pass, against 24 for two passes of k_rbgs_sweep).  A block of TX / 2 * TY
 // threads (one tile pair each) owns a TX x TY column tile and a z-chunk
-// [k0, k1): 64 x 16, or 32 x 16 on small grids (twice the blocks).  Each
-// colour stage is
-// recomputed on a ring one cell wider than the next stage reads (R1 on the
+// [k0, k1): 64 x 16, 48 x 16 on grids 48-wide tiles fit better, or 32 x 16 on
+// small grids (twice the blocks).  Each colour stage is recomputed on a ring one cell wider than the next stage reads (R1 on the
 // tile plus three cells, B1 plus two, R2 plus one, B2 on the tile), so no
 // block waits on another and the result is exactly two sweeps, with the cell
 // arithmetic of k_rbgs_sweep (rb_cell).
