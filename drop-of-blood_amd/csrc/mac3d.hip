@@ -22,6 +22,7 @@
 #include "mixed_kernels.hpp"
 #include "step_kernels.hpp"
 #include "adjoint_kernels.hpp"
+#include "wave_kernels.hpp"
 #include "profile.hpp"
 
 using namespace mac3d;
@@ -125,8 +126,17 @@ struct Ctx {
   size_t part_n = 0;
   double *red = nullptr;     // device fold output (3 doubles)
   double *red_host = nullptr; // pinned
-  // staging for upload/download
+  // free-surface coupling (off until mac3d_wave_enable): wave state (4 per
+  // texel), complex FFT work (2 per texel), the force pass's film, the lid
+  // pressure (nx * nz), the drop slots and the FFT twiddles on the device
+  bool wave_on = false;
+  WaveParams wp{};
+  Buf wave, wz[2], wfilm, ptop;
+  WaveDrop *drops = nullptr;
+  float2 *wtw = nullptr;
+  // staging for upload/download (stage_n doubles)
   double *stage = nullptr;
+  size_t stage_n = 0;
   mac3d::BlobDev *blobs = nullptr;
   int blobs_cap = 0;
   Profiler prof;
@@ -321,6 +331,64 @@ int launch_casson(Ctx *ctx, double dt) {
   return MAC3D_OK;
 }
 
+// the lid's Dirichlet pressure for the divergence and projection: the wave
+// coupling's, or none (the flat lid, p_top = 0)
+const float *lid_pressure(const Ctx *ctx) {
+  return ctx->wave_on ? (const float *)ctx->ptop.ptr : nullptr;
+}
+
+WaveParams wave_params_dt(const Ctx *ctx, double dt) {
+  WaveParams p = ctx->wp;
+  p.dt = (float)std::min(std::max(dt, 0.0), 0.025);  // viewportAndDt.z clamp (wave.ts:83)
+  return p;
+}
+
+// One wave step (BloodDropEngine.ts:1886-1914): force pass, forward FFT of
+// the rows then the columns, spectral propagation, inverse FFT of the
+// columns then the rows, finalize into the wave state.
+int launch_wave_step(Ctx *ctx, double dt) {
+  const WaveParams P = wave_params_dt(ctx, dt);
+  const int n = P.n;
+  int logn = 0;
+  while ((1 << logn) < n) ++logn;
+  const dim3 g2((n + 15) / 16, (n + 15) / 16), b2(16, 16);
+  ProfScope ps(ctx->prof, ctx->stream, KC_WAVE);
+  float2 *z0 = (float2 *)ctx->wz[0].ptr, *z1 = (float2 *)ctx->wz[1].ptr;
+  with_step_type(ctx->t0, [&](auto tag) {
+    using T = std::remove_pointer_t<decltype(tag)>;
+    hipLaunchKernelGGL(k_wave_force<T>, g2, b2, 0, ctx->stream, P, (const WaveDrop *)ctx->drops,
+                       (const float4 *)ctx->wave.ptr, ctx->d[0], (const T *)ctx->v.ptr, z0,
+                       (float *)ctx->wfilm.ptr);
+  });
+  const dim3 bf(std::max(64, std::min(512, n / 2)));
+  const size_t sh = (size_t)n * sizeof(float2);
+  hipLaunchKernelGGL(k_wave_fft, dim3(n), bf, sh, ctx->stream, z0, n, logn, 1, 0,
+                     (const float2 *)ctx->wtw);
+  hipLaunchKernelGGL(k_wave_fft, dim3(n), bf, sh, ctx->stream, z0, n, logn, 0, 0,
+                     (const float2 *)ctx->wtw);
+  hipLaunchKernelGGL(k_wave_propagate, g2, b2, 0, ctx->stream, P, (const float2 *)z0, z1);
+  hipLaunchKernelGGL(k_wave_fft, dim3(n), bf, sh, ctx->stream, z1, n, logn, 0, 1,
+                     (const float2 *)ctx->wtw);
+  hipLaunchKernelGGL(k_wave_fft, dim3(n), bf, sh, ctx->stream, z1, n, logn, 1, 1,
+                     (const float2 *)ctx->wtw);
+  hipLaunchKernelGGL(k_wave_finalize, g2, b2, 0, ctx->stream, P, (const float2 *)z1,
+                     (const float *)ctx->wfilm.ptr, (float4 *)ctx->wave.ptr);
+  LAUNCHCHK();
+  return MAC3D_OK;
+}
+
+// evalTopDirichletPressure over the lid plane into ptop
+int launch_top_pressure(Ctx *ctx, double dt) {
+  const WaveParams P = wave_params_dt(ctx, dt);
+  const Dims &d = ctx->d[0];
+  ProfScope ps(ctx->prof, ctx->stream, KC_WAVE);
+  hipLaunchKernelGGL(k_top_pressure, dim3((d.nx + 15) / 16, (d.nz + 15) / 16), dim3(16, 16), 0,
+                     ctx->stream, P, (const WaveDrop *)ctx->drops, (const float4 *)ctx->wave.ptr,
+                     d.nx, d.nz, (float *)ctx->ptop.ptr);
+  LAUNCHCHK();
+  return MAC3D_OK;
+}
+
 int launch_divergence(Ctx *ctx) {
   const Dims &d = ctx->d[0];
   ProfScope ps(ctx->prof, ctx->stream, KC_DIV);
@@ -330,7 +398,7 @@ int launch_divergence(Ctx *ctx) {
       using R = std::remove_pointer_t<decltype(rtag)>;
       hipLaunchKernelGGL((k_divergence<T, R>), cell_grid(d), kCellBlock, 0, ctx->stream, d,
                          (const T *)ctx->u.ptr, (const T *)ctx->v.ptr, (const T *)ctx->w.ptr,
-                         (R *)ctx->rhs.ptr);
+                         (R *)ctx->rhs.ptr, lid_pressure(ctx));
     });
   });
   LAUNCHCHK();
@@ -346,7 +414,7 @@ int launch_project(Ctx *ctx) {
       using Pt = std::remove_pointer_t<decltype(ptag)>;
       hipLaunchKernelGGL((k_project<T, Pt>), cell_grid(d), kCellBlock, 0, ctx->stream, d,
                          (T *)ctx->u.ptr, (T *)ctx->v.ptr, (T *)ctx->w.ptr,
-                         (const Pt *)ctx->p.ptr);
+                         (const Pt *)ctx->p.ptr, lid_pressure(ctx));
     });
   });
   LAUNCHCHK();
@@ -893,6 +961,10 @@ enum { SEQ_STEP = 1, SEQ_PRE = 2, SEQ_ITER = 3, SEQ_POST = 4 };
 
 int seq_pre(Ctx *ctx, double dt) {
   int rc;
+  if (ctx->wave_on) {  // the wave pass reads this step's starting velocity
+    if ((rc = launch_wave_step(ctx, dt))) return rc;
+    if ((rc = launch_top_pressure(ctx, dt))) return rc;
+  }
   // the previous step's last pass already formed cvel from this velocity
   if (!ctx->cvel_fresh && (rc = launch_face_to_centre(ctx))) return rc;
   if ((rc = launch_advect(ctx, dt))) return rc;
@@ -1010,6 +1082,8 @@ Buf *field_buf(Ctx *ctx, int field, int level) {
     case MAC3D_FIELD_MGX: return &ctx->mgx[level];
     case MAC3D_FIELD_MGB: return &ctx->mgb[level];
     case MAC3D_FIELD_MGR: return &ctx->mgr[level];
+    case MAC3D_FIELD_WAVE: return l0 && ctx->wave_on ? &ctx->wave : nullptr;
+    case MAC3D_FIELD_PTOP: return l0 && ctx->wave_on ? &ctx->ptop : nullptr;
   }
   return nullptr;
 }
@@ -1120,6 +1194,7 @@ int create_impl(Ctx *ctx, const mac3d_config *cfg) {
   HIPCHK(hipHostMalloc(&ctx->red_host, 3 * (kMaxPreIters + 1) * sizeof(double),
                        hipHostMallocDefault));
   if ((rc = dev_alloc(ctx, (void **)&ctx->stage, n0 * sizeof(double)))) return rc;
+  ctx->stage_n = n0;
   hipLaunchKernelGGL(k_fill<double>, dim3(1), dim3(256), 0, ctx->stream, ctx->scale, (size_t)2, 1.0);
   LAUNCHCHK();
   HIPCHK(hipStreamSynchronize(ctx->stream));
@@ -1649,6 +1724,99 @@ int mac3d_download(void *h, int field, int level, void *host, int dtype, size_t 
   return MAC3D_OK;
 }
 
+int mac3d_wave_default_params(mac3d_wave_params *p) {
+  if (!p) return MAC3D_EINVAL;
+  std::memset(p, 0, sizeof(*p));
+  p->n = 512;               // WAVE_GRID_SIZE (common.ts:1)
+  p->half_extent = 1.60;    // WAVE_HALF_EXTENT (common.ts:2)
+  p->half_xz = 0.0;         // nx dx / 2
+  p->depth = 0.0;           // ny dx
+  p->surface_y = 0.30;      // WATER_SURFACE_Y
+  p->marangoni = 0.0;
+  return MAC3D_OK;
+}
+
+int mac3d_wave_enable(void *h, const mac3d_wave_params *p) {
+  if (!h) return MAC3D_EINVAL;
+  Ctx *ctx = (Ctx *)h;
+  drop_graphs(ctx);  // the step's launch sequence changes
+  if (!p) {
+    ctx->wave_on = false;
+    return MAC3D_OK;
+  }
+  const int n = p->n;
+  if (n < 16 || n > 1024 || (n & (n - 1)))
+    return fail(ctx, MAC3D_EINVAL, "wave n must be a power of two in [16, 1024]");
+  const double ext = p->half_extent;
+  const double hx = p->half_xz > 0.0 ? p->half_xz : 0.5 * ctx->cfg.nx * ctx->cfg.dx;
+  const double depth = p->depth > 0.0 ? p->depth : ctx->cfg.ny * ctx->cfg.dx;
+  if (!(ext > 0.0) || !std::isfinite(ext) || !std::isfinite(hx) || !std::isfinite(depth) ||
+      !std::isfinite(p->surface_y) || !std::isfinite(p->marangoni) || p->half_xz < 0.0 ||
+      p->depth < 0.0)
+    return fail(ctx, MAC3D_EINVAL, "wave lengths must be finite, half_extent > 0");
+  int rc;
+  const size_t nn = (size_t)n * n;
+  if (!ctx->wave.ptr || ctx->wp.n != n) {  // (earlier buffers stay owned until destroy)
+    if ((rc = alloc(ctx, ctx->wave, TF32, 4 * nn))) return rc;
+    for (int q = 0; q < 2; ++q)
+      if ((rc = alloc(ctx, ctx->wz[q], TF32, 2 * nn))) return rc;
+    if ((rc = alloc(ctx, ctx->wfilm, TF32, nn))) return rc;
+    if ((rc = dev_alloc(ctx, (void **)&ctx->wtw, (size_t)(n / 2) * sizeof(float2)))) return rc;
+    std::vector<float2> tw(n / 2);
+    for (int m = 0; m < n / 2; ++m) {
+      const double a = 2.0 * M_PI * m / n;
+      tw[m] = make_float2((float)std::cos(a), (float)-std::sin(a));
+    }
+    HIPCHK(hipMemcpyAsync(ctx->wtw, tw.data(), tw.size() * sizeof(float2), hipMemcpyHostToDevice,
+                          ctx->stream));
+    if (4 * nn > ctx->stage_n) {
+      if ((rc = dev_alloc(ctx, (void **)&ctx->stage, 4 * nn * sizeof(double)))) return rc;
+      ctx->stage_n = 4 * nn;
+    }
+  }
+  if (!ctx->ptop.ptr && (rc = alloc(ctx, ctx->ptop, TF32, (size_t)ctx->cfg.nx * ctx->cfg.nz)))
+    return rc;
+  if (!ctx->drops) {
+    if ((rc = dev_alloc(ctx, (void **)&ctx->drops, WV_MAX_DROPS * sizeof(WaveDrop)))) return rc;
+    HIPCHK(hipMemsetAsync(ctx->drops, 0, WV_MAX_DROPS * sizeof(WaveDrop), ctx->stream));
+  }
+  // a flat, film-free surface (WAVE_CLEAR, wave.ts:15-30) and no lid pressure
+  HIPCHK(hipMemsetAsync(ctx->wave.ptr, 0, 4 * nn * sizeof(float), ctx->stream));
+  HIPCHK(hipMemsetAsync(ctx->ptop.ptr, 0, ctx->ptop.n * sizeof(float), ctx->stream));
+  HIPCHK(hipStreamSynchronize(ctx->stream));
+  ctx->wp.n = n;
+  ctx->wp.ext = (float)ext;
+  ctx->wp.half_xz = (float)hx;
+  ctx->wp.depth = (float)depth;
+  ctx->wp.surface_y = (float)p->surface_y;
+  ctx->wp.marangoni = (float)p->marangoni;
+  ctx->wp.dt = 0.f;
+  ctx->wave_on = true;
+  return MAC3D_OK;
+}
+
+int mac3d_set_drops(void *h, const mac3d_drop *drops, int ndrops) {
+  if (!h) return MAC3D_EINVAL;
+  Ctx *ctx = (Ctx *)h;
+  if (ndrops < 0 || ndrops > MAC3D_MAX_DROPS || (ndrops > 0 && !drops))
+    return fail(ctx, MAC3D_EINVAL, "need 0..MAC3D_MAX_DROPS drops");
+  if (!ctx->wave_on) return fail(ctx, MAC3D_ESTATE, "the wave coupling is not enabled");
+  WaveDrop hd[WV_MAX_DROPS];
+  std::memset(hd, 0, sizeof(hd));
+  for (int q = 0; q < ndrops; ++q) {
+    const mac3d_drop &a = drops[q];
+    const double v[8] = {a.x, a.y, a.z, a.radius, a.vel_y, a.osc, a.drain, a.seed};
+    for (double x : v)
+      if (!std::isfinite(x)) return fail(ctx, MAC3D_EINVAL, "drop values must be finite");
+    hd[q] = WaveDrop{(float)a.x, (float)a.y, (float)a.z, (float)a.radius,
+                     (float)a.vel_y, (float)a.osc, (float)a.drain, (float)a.seed};
+  }
+  // (the captured step sequences read the slots from device memory)
+  HIPCHK(hipMemcpyAsync(ctx->drops, hd, sizeof(hd), hipMemcpyHostToDevice, ctx->stream));
+  HIPCHK(hipStreamSynchronize(ctx->stream));
+  return MAC3D_OK;
+}
+
 int mac3d_init_blobs(void *h, const mac3d_blob *blobs, int nblobs) {
   if (!h) return MAC3D_EINVAL;
   return init_blobs_impl((Ctx *)h, blobs, nblobs);
@@ -1704,6 +1872,8 @@ int mac3d_adjoint(void *h, const mac3d_blob *blobs, int nblobs, double dt, int n
   if (!h) return MAC3D_EINVAL;
   Ctx *ctx = (Ctx *)h;
   if (!out || !target) return fail(ctx, MAC3D_EINVAL, "null target or result");
+  if (ctx->wave_on)
+    return fail(ctx, MAC3D_ESTATE, "the adjoint covers the flat lid only: disable the wave");
   if (nblobs < 1 || nblobs > MAC3D_MAX_BLOBS || !blobs)
     return fail(ctx, MAC3D_EINVAL, "need 1..MAC3D_MAX_BLOBS blobs");
   if (!std::isfinite(dt) || !(dt > 0.0)) return fail(ctx, MAC3D_EINVAL, "dt must be finite and > 0");
@@ -1732,6 +1902,11 @@ int mac3d_stage(void *h, int stage, double dt) {
     case MAC3D_STAGE_PROJECT: rc = launch_project(ctx); break;
     case MAC3D_STAGE_DYE_PREDICT: rc = launch_dye_predict(ctx, dt); break;
     case MAC3D_STAGE_DYE_RESOLVE: rc = launch_dye_resolve(ctx, dt); break;
+    case MAC3D_STAGE_WAVE:
+    case MAC3D_STAGE_TOP_PRESSURE:
+      if (!ctx->wave_on) return fail(ctx, MAC3D_ESTATE, "the wave coupling is not enabled");
+      rc = stage == MAC3D_STAGE_WAVE ? launch_wave_step(ctx, dt) : launch_top_pressure(ctx, dt);
+      break;
     default: return fail(ctx, MAC3D_EINVAL, "unknown stage");
   }
   if (rc) return rc;

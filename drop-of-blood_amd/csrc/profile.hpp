@@ -25,6 +25,7 @@ enum {
   KC_DYE = MAC3D_KC_DYE,
   KC_RESID64 = MAC3D_KC_REFINE64,
   KC_SMOOTH0X2 = MAC3D_KC_SMOOTH2_L0,
+  KC_WAVE = MAC3D_KC_WAVE,
   KC_COUNT = MAC3D_KC_COUNT
 };
 

@@ -341,12 +341,14 @@ k_casson_zm(Dims d, StepParams P, const T *__restrict__ u, const T *__restrict__
   }
 }
 
-// FLUID_MAC_RHS (fluid.ts:517-555) with a flat lid (p_top = 0):
-// D = (uR - uL) + (vT - vB) + (wF - wB), solid wall faces 0, top face free.
+// FLUID_MAC_RHS (fluid.ts:517-555): D = (uR - uL) + (vT - vB) + (wF - wB),
+// solid wall faces 0, top face free; on the top row the lid pressure moves
+// to the right-hand side, D -= p_top (fluid.ts:548-551).  ptop (nx * nz,
+// nullable: the flat lid, p_top = 0) holds evalTopDirichletPressure.
 template <class T, class R>
 __global__ void __launch_bounds__(256)
 k_divergence(Dims d, const T *__restrict__ u, const T *__restrict__ v,
-             const T *__restrict__ w, R *__restrict__ out) {
+             const T *__restrict__ w, R *__restrict__ out, const float *__restrict__ ptop) {
   MAC3D_CELL_PROLOGUE
   typedef typename compute_of<R>::type C;
   C uR = (i >= d.nx - 1) ? C(0) : (C)u[c];
@@ -355,20 +357,23 @@ k_divergence(Dims d, const T *__restrict__ u, const T *__restrict__ v,
   C vB = (j == 0) ? C(0) : (C)v[c - sy];
   C wF = (k >= d.nz - 1) ? C(0) : (C)w[c];
   C wB = (k == 0) ? C(0) : (C)w[c - sz];
-  out[c] = (R)((uR - uL) + (vT - vB) + (wF - wB));
+  C rhs = (uR - uL) + (vT - vB) + (wF - wB);
+  if (ptop && j == d.ny - 1) rhs = rhs - (C)ptop[(size_t)k * d.nx + i];
+  out[c] = (R)rhs;
 }
 
-// FLUID_MAC_PROJECT (fluid.ts:863-911) with a flat lid: face -= p(+1) - p,
-// solid wall faces 0, top face uses the zero ghost above the top row.
+// FLUID_MAC_PROJECT (fluid.ts:863-911): face -= p(+1) - p, solid wall faces
+// 0, the top face against the lid's ghost pressure p_top (ptop, nullable:
+// the flat lid's 0; fluid.ts:893-898).
 template <class T, class Pt>
 __global__ void __launch_bounds__(256)
 k_project(Dims d, T *__restrict__ u, T *__restrict__ v, T *__restrict__ w,
-          const Pt *__restrict__ p) {
+          const Pt *__restrict__ p, const float *__restrict__ ptop) {
   MAC3D_CELL_PROLOGUE
   typedef typename compute_of<Pt>::type C;
   C pc = (C)p[c];
   if (i < d.nx - 1) u[c] = (T)((C)u[c] - ((C)p[c + sx] - pc)); else u[c] = T(0);
-  C pu = (j < d.ny - 1) ? (C)p[c + sy] : C(0);
+  C pu = (j < d.ny - 1) ? (C)p[c + sy] : (ptop ? (C)ptop[(size_t)k * d.nx + i] : C(0));
   v[c] = (T)((C)v[c] - (pu - pc));
   if (k < d.nz - 1) w[c] = (T)((C)w[c] - ((C)p[c + sz] - pc)); else w[c] = T(0);
 }

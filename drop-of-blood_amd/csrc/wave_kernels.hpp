@@ -1,0 +1,317 @@
+// Free-surface wave solver and the top Dirichlet pressure it feeds: the
+// reference's WAVE_SPATIAL_FORCE / radix-2 FFT / WAVE_SPECTRAL_PROPAGATE /
+// WAVE_FINALIZE passes (src/shaders/wave.ts:56-388) and
+// evalTopDirichletPressure (src/shaders/fluid.ts:16-78), which the MAC
+// divergence (fluid.ts:548-551) and projection (fluid.ts:896-898) read on
+// the lid plane.
+//
+// The wave lives on an n x n texel grid over [-E, E]^2 (E = 1.60 m, n = 512 in
+// the reference), texel (ix, iy) at uv = (i + 1/2) / n, x fastest.  State per
+// texel: (eta, d eta/dt, 9-point Laplacian of eta, surfactant film), FP32
+// (the reference stores it as rgba16float; the spectra are FP32 in both).
+// One wave step: force pass -> forward 2-D FFT of eta + i deta/dt -> spectral
+// propagation -> inverse 2-D FFT -> finalize.  The FFTs are unitary
+// (1/sqrt(n) per 1-D transform), as the reference's.
+#pragma once
+#include "common.hpp"
+
+namespace mac3d {
+
+#define WV_MAX_DROPS 8
+
+// one drop slot of SimUniforms.drops (common.ts:26-43): posAndRadius and
+// velOscAndDrain
+struct WaveDrop {
+  float x, y, z, r;
+  float vy, osc, drain, seed;
+};
+
+struct WaveParams {
+  int n;            // texels per side (power of two)
+  float ext;        // wave domain half-width E (m)
+  float half_xz;    // tank half-width (domainBounds.x)
+  float depth;      // tank depth (domainBounds.z - domainBounds.y)
+  float surface_y;  // still surface height (surfaceParams.x)
+  float marangoni;  // surfaceParams.y
+  float dt;         // substep dt clamped to [0, 0.025] (viewportAndDt.z)
+};
+
+__device__ inline float wv_smoothstep(float e0, float e1, float x) {
+  float t = clampv((x - e0) / (e1 - e0), 0.f, 1.f);
+  return t * t * (3.f - 2.f * t);
+}
+
+// texel of the wave state, indices clamped to the grid (loadWaveClamped)
+__device__ inline float4 wv_load(const float4 *__restrict__ w, int n, int x, int y) {
+  x = clampi(x, 0, n - 1);
+  y = clampi(y, 0, n - 1);
+  return w[(size_t)y * n + x];
+}
+
+// bilinear clamp-to-edge sample of the wave state at uv (textureSampleLevel
+// with the linear sampler, BloodDropEngine.ts:497-505)
+__device__ inline float4 wv_sample(const float4 *__restrict__ w, int n, float u, float v) {
+  float tx = u * n - 0.5f, ty = v * n - 0.5f;
+  float fx0 = floorf(tx), fy0 = floorf(ty);
+  int x0 = (int)fx0, y0 = (int)fy0;
+  float fx = tx - fx0, fy = ty - fy0;
+  float4 a = wv_load(w, n, x0, y0), b = wv_load(w, n, x0 + 1, y0);
+  float4 c = wv_load(w, n, x0, y0 + 1), e = wv_load(w, n, x0 + 1, y0 + 1);
+  auto mx = [](float4 p, float4 q, float t) {
+    return make_float4(mixv(p.x, q.x, t), mixv(p.y, q.y, t), mixv(p.z, q.z, t), mixv(p.w, q.w, t));
+  };
+  return mx(mx(a, b, fx), mx(c, e, fx), fy);
+}
+
+// trilinear clamp-to-edge sample of a MAC face field stored per cell at
+// normalised uvw (the velMac texture read of the force pass)
+template <class T>
+__device__ inline float wv_sample_mac(const T *__restrict__ f, const Dims &d, float u, float v,
+                                      float w) {
+  float t[3] = {u * d.nx - 0.5f, v * d.ny - 0.5f, w * d.nz - 0.5f};
+  const int N[3] = {d.nx, d.ny, d.nz};
+  int i0[3], i1[3];
+  float fr[3];
+#pragma unroll
+  for (int a = 0; a < 3; ++a) {
+    float fl = floorf(t[a]);
+    fr[a] = t[a] - fl;
+    i0[a] = clampi((int)fl, 0, N[a] - 1);
+    i1[a] = clampi((int)fl + 1, 0, N[a] - 1);
+  }
+  auto at = [&](int i, int j, int k) { return (float)f[idx3(d, i, j, k)]; };
+  float x00 = mixv(at(i0[0], i0[1], i0[2]), at(i1[0], i0[1], i0[2]), fr[0]);
+  float x10 = mixv(at(i0[0], i1[1], i0[2]), at(i1[0], i1[1], i0[2]), fr[0]);
+  float x01 = mixv(at(i0[0], i0[1], i1[2]), at(i1[0], i0[1], i1[2]), fr[0]);
+  float x11 = mixv(at(i0[0], i1[1], i1[2]), at(i1[0], i1[1], i1[2]), fr[0]);
+  return mixv(mixv(x00, x10, fr[1]), mixv(x01, x11, fr[1]), fr[2]);
+}
+
+// WAVE_SPATIAL_FORCE (wave.ts:56-178): drop impulses (zero-integral polar
+// Ricker craters and capillary contact impulses), the surfactant film and its
+// Marangoni pull, the MAC grid's upward top-layer velocity, the radial
+// absorber.  Writes z = eta + i deta/dt for the FFT and the updated film.
+template <class T>
+__global__ void __launch_bounds__(256)
+k_wave_force(WaveParams P, const WaveDrop *__restrict__ drops, const float4 *__restrict__ wave,
+             Dims d, const T *__restrict__ vmac, float2 *__restrict__ z,
+             float *__restrict__ film_out) {
+  const int n = P.n;
+  const int ix = blockIdx.x * 16 + threadIdx.x, iy = blockIdx.y * 16 + threadIdx.y;
+  if (ix >= n || iy >= n) return;
+  const float4 c = wv_load(wave, n, ix, iy);
+  float eta = c.x, vel = c.y, film = c.w;
+  const float dt = P.dt;
+  const float time_scale = dt / 0.0166667f;
+  const float lap_scale = ((float)n / 256.f) * ((float)n / 256.f);
+  const float raw_lap = 0.20f * (wv_load(wave, n, ix, iy + 1).w + wv_load(wave, n, ix, iy - 1).w +
+                                 wv_load(wave, n, ix + 1, iy).w + wv_load(wave, n, ix - 1, iy).w) +
+                        0.05f * (wv_load(wave, n, ix + 1, iy + 1).w + wv_load(wave, n, ix - 1, iy + 1).w +
+                                 wv_load(wave, n, ix + 1, iy - 1).w + wv_load(wave, n, ix - 1, iy - 1).w) -
+                        film;
+  const float lap_film = raw_lap * lap_scale;
+  const float wx = ((ix + 0.5f) / n) * (2.f * P.ext) - P.ext;
+  const float wz = ((iy + 0.5f) / n) * (2.f * P.ext) - P.ext;
+
+  float force_drops = 0.f, active = 0.f, max_drain = 0.f;
+  for (int q = 0; q < WV_MAX_DROPS; ++q) {
+    const WaveDrop s = drops[q];
+    if (!(s.seed > 0.f && s.drain > 0.0005f)) continue;
+    max_drain = fmaxf(max_drain, s.drain);
+    const float rd = fmaxf(s.r, 0.018f);
+    const float speed = clampv(-s.vy, 0.06f, 0.65f);
+    const float ex = wx - s.x, ez = wz - s.z;
+    const float r = sqrtf(ex * ex + ez * ez);
+    const float r_foot = clampv(rd * 1.05f, 0.024f, 0.082f);
+    const float rn = r / r_foot;
+    const float emerg = (s.y - P.surface_y) / fmaxf(rd, 0.018f);
+    const float excav = wv_smoothstep(-0.78f, -0.04f, emerg);
+    const float bell = sinf(clampv((1.05f - emerg) / 1.83f, 0.f, 1.f) * 3.14159265f) * excav;
+    const float shape = expf(-rn * rn) * (1.f - rn * rn);
+    const float dyn = (6.8f * bell * (rd / 0.056f)) * (0.72f + 2.65f * speed);
+    const float r_cap = clampv(rd * 0.44f, 0.013f, 0.030f);
+    const float rnc = r / r_cap;
+    const float cap_win = wv_smoothstep(-0.35f, 0.28f, emerg) * (1.f - wv_smoothstep(0.68f, 1.06f, emerg));
+    const float cap_shape = expf(-rnc * rnc) * (1.f - rnc * rnc);
+    const float cap = (4.6f * cap_win * (0.60f + 0.40f * (rd / 0.056f))) * (0.65f + 2.20f * speed);
+    force_drops = force_drops - dyn * shape - cap * cap_shape;
+    active = fmaxf(active, excav * expf(-0.5f * rn * rn));
+    const float r_stain = fmaxf(0.018f, rd * 0.60f);
+    const float stain = expf(-(r * r) / (r_stain * r_stain)) * clampv(s.drain * 0.42f, 0.f, 1.f);
+    film = fmaxf(film, stain * 0.65f);
+  }
+
+  const float m = P.marangoni;
+  const float eff_lap = mixv(fmaxf(lap_film, -0.0015f), lap_film, active);
+  const float force_mar = m * 0.28f * eff_lap;
+  film = clampv(film + m * 0.065f * raw_lap * time_scale, 0.f, 1.f);
+  film = film * powf(0.88f, time_scale);
+
+  float force_sub = 0.f;
+  const float hx = P.half_xz;
+  const float gate = clampv(active * 1.4f + wv_smoothstep(0.f, 0.06f, max_drain), 0.f, 1.f);
+  const float rw = sqrtf(wx * wx + wz * wz);
+  if (gate > 0.001f && rw < hx * 0.94f) {
+    const float tu = (wx + hx) / (2.f * hx), tw = (wz + hx) / (2.f * hx);
+    const float vsub = wv_sample_mac<T>(vmac, d, tu, 0.988f, tw);
+    const float taper = 1.f - wv_smoothstep(hx * 0.62f, hx * 0.92f, rw);
+    force_sub = clampv(fmaxf(vsub, 0.f), 0.f, 0.48f) * 0.42f * taper * gate;
+  }
+  vel = vel + (force_drops + force_mar + force_sub) * dt;
+
+  const float absorb = wv_smoothstep(0.90f, 1.54f, rw);
+  const float rdamp = expf(-dt * (24.f * absorb * absorb));
+  const float border = 1.f - wv_smoothstep(1.22f, 1.56f, rw);
+  eta = clampv(eta * rdamp * border, -0.14f, 0.14f);
+  vel = clampv(vel * rdamp * border, -1.8f, 1.8f);
+  film = film * border;
+  const size_t t = (size_t)iy * n + ix;
+  z[t] = make_float2(eta, vel);
+  film_out[t] = film;
+}
+
+// One 1-D unitary complex FFT of length n per line, radix-2 in LDS (n/2
+// threads, one butterfly each per stage), input in bit-reversed order as the
+// reference's createRadix2Fft1DShader (wave.ts:180-252).  rows: lines are
+// rows (x fastest, contiguous); else columns.  tw[m] = exp(-2 pi i m / n),
+// conjugated for the inverse.
+__global__ void __launch_bounds__(512)
+k_wave_fft(float2 *__restrict__ z, int n, int logn, int rows, int inverse,
+           const float2 *__restrict__ tw) {
+  extern __shared__ float2 sm[];
+  const int line = blockIdx.x, tid = threadIdx.x, h = n >> 1;
+  const size_t base = rows ? (size_t)line * n : (size_t)line;
+  const size_t step = rows ? 1 : (size_t)n;
+  for (int i = tid; i < n; i += blockDim.x) {
+    const int r = (int)(__brev((unsigned)i) >> (32 - logn));
+    sm[i] = z[base + (size_t)r * step];
+  }
+  __syncthreads();
+  for (int s = 0; s < logn; ++s) {
+    const int half = 1 << s;
+    for (int b = tid; b < h; b += blockDim.x) {
+      const int k = b & (half - 1);
+      const int i0 = ((b >> s) << (s + 1)) + k, i1 = i0 + half;
+      float2 w = tw[k << (logn - 1 - s)];
+      if (inverse) w.y = -w.y;
+      const float2 a = sm[i0], q = sm[i1];
+      const float2 t = make_float2(q.x * w.x - q.y * w.y, q.x * w.y + q.y * w.x);
+      sm[i0] = make_float2(a.x + t.x, a.y + t.y);
+      sm[i1] = make_float2(a.x - t.x, a.y - t.y);
+    }
+    __syncthreads();
+  }
+  const float sc = rsqrtf((float)n);
+  for (int i = tid; i < n; i += blockDim.x) {
+    const float2 v = sm[i];
+    z[base + (size_t)i * step] = make_float2(v.x * sc, v.y * sc);
+  }
+}
+
+// WAVE_SPECTRAL_PROPAGATE (wave.ts:254-331): Hermitian unpacking of the two
+// real fields packed as Z = eta + i w, the gravity-capillary dispersion
+// omega(k) = sqrt((g k + s k^3) tanh(k H)) over one dt with viscous damping and
+// a high-k roll-off, DC and the Nyquist row and column set to 0, repacked.
+__global__ void __launch_bounds__(256)
+k_wave_propagate(WaveParams P, const float2 *__restrict__ zin, float2 *__restrict__ zout) {
+  const int n = P.n;
+  const int ux = blockIdx.x * 16 + threadIdx.x, uy = blockIdx.y * 16 + threadIdx.y;
+  if (ux >= n || uy >= n) return;
+  const int hn = n >> 1;
+  const size_t t = (size_t)uy * n + ux;
+  if ((ux == 0 && uy == 0) || ux == hn || uy == hn) {
+    zout[t] = make_float2(0.f, 0.f);
+    return;
+  }
+  const int nxn = (n - ux) & (n - 1), nyn = (n - uy) & (n - 1);
+  const float2 fp = zin[t], fm = zin[(size_t)nyn * n + nxn];
+  const float2 eh = make_float2(0.5f * (fp.x + fm.x), 0.5f * (fp.y - fm.y));
+  const float2 vh = make_float2(0.5f * (fp.y + fm.y), 0.5f * (-fp.x + fm.x));
+  const float mx = (float)(ux > hn ? ux - n : ux), mz = (float)(uy > hn ? uy - n : uy);
+  const float two_pi_l = 6.28318530718f / (2.f * P.ext);
+  const float kx = mx * two_pi_l, kz = mz * two_pi_l;
+  const float k2 = kx * kx + kz * kz;
+  const float k = sqrtf(k2);
+  const float dt = P.dt;
+  const float th = tanhf(fminf(k * P.depth, 12.f));
+  const float omega = sqrtf(fmaxf((0.84f * k + 0.0024f * k * k2) * th, 1e-8f));
+  const float visc = 0.36f + 0.00018f * k2;
+  const float roll = 1.f - wv_smoothstep(335.f, 465.f, k);
+  const float damp = expf(-visc * dt) * roll;
+  const float ph = omega * dt;
+  const float cp = cosf(ph), sp = sinf(ph);
+  const float2 en = make_float2(damp * (eh.x * cp + vh.x * (sp / omega)),
+                                damp * (eh.y * cp + vh.y * (sp / omega)));
+  const float2 vn = make_float2(damp * (-eh.x * (omega * sp) + vh.x * cp),
+                                damp * (-eh.y * (omega * sp) + vh.y * cp));
+  zout[t] = make_float2(en.x - vn.y, en.y + vn.x);
+}
+
+// WAVE_FINALIZE (wave.ts:333-388): propagated (eta, deta/dt), the film of the
+// force pass, the resolution-scaled 9-point Laplacian of eta, the outer
+// border fade and clamps.
+__global__ void __launch_bounds__(256)
+k_wave_finalize(WaveParams P, const float2 *__restrict__ z, const float *__restrict__ film,
+                float4 *__restrict__ wave) {
+  const int n = P.n;
+  const int ix = blockIdx.x * 16 + threadIdx.x, iy = blockIdx.y * 16 + threadIdx.y;
+  if (ix >= n || iy >= n) return;
+  auto eta = [&](int x, int y) {
+    x = clampi(x, 0, n - 1);
+    y = clampi(y, 0, n - 1);
+    return z[(size_t)y * n + x].x;
+  };
+  const size_t t = (size_t)iy * n + ix;
+  const float2 pr = z[t];
+  const float lap_scale = ((float)n / 256.f) * ((float)n / 256.f);
+  const float lap = (0.20f * (eta(ix, iy + 1) + eta(ix, iy - 1) + eta(ix + 1, iy) + eta(ix - 1, iy)) +
+                     0.05f * (eta(ix + 1, iy + 1) + eta(ix - 1, iy + 1) + eta(ix + 1, iy - 1) +
+                              eta(ix - 1, iy - 1)) -
+                     pr.x) *
+                    lap_scale;
+  const float wx = ((ix + 0.5f) / n) * (2.f * P.ext) - P.ext;
+  const float wz = ((iy + 0.5f) / n) * (2.f * P.ext) - P.ext;
+  const float border = 1.f - wv_smoothstep(1.22f, 1.56f, sqrtf(wx * wx + wz * wz));
+  wave[t] = make_float4(clampv(pr.x * border, -0.14f, 0.14f), clampv(pr.y * border, -1.8f, 1.8f),
+                        lap * border, film[t] * border);
+}
+
+// evalTopDirichletPressure (fluid.ts:16-78) on the lid plane, one value per
+// (ix, iz): hydrostatic and capillary pressure of the wave under the cell,
+// the drops' wetting-neck overpressure (with the m = 4, 6, 8 azimuthal
+// modulation), times dt / h and a wall taper.  ptop[iz * nx + ix].
+__global__ void __launch_bounds__(256)
+k_top_pressure(WaveParams P, const WaveDrop *__restrict__ drops, const float4 *__restrict__ wave,
+               int nx, int nz, float *__restrict__ ptop) {
+  const int ix = blockIdx.x * 16 + threadIdx.x, iz = blockIdx.y * 16 + threadIdx.y;
+  if (ix >= nx || iz >= nz) return;
+  const float hx = P.half_xz;
+  const float fx = (ix + 0.5f) / nx, fz = (iz + 0.5f) / nz;
+  const float wx = mixv(-hx, hx, fx), wz = mixv(-hx, hx, fz);
+  const float dt = P.dt;
+  const float cell_h = (2.f * hx) / nx;
+  const float dt_h = dt / cell_h;
+  const float su = clampv((wx + P.ext) / (2.f * P.ext), 0.002f, 0.998f);
+  const float sv = clampv((wz + P.ext) / (2.f * P.ext), 0.002f, 0.998f);
+  const float4 ws = wv_sample(wave, P.n, su, sv);
+  float pk = 2.45f * (0.155f * ws.x + 0.00095f * (-18.4f * ws.z));
+  for (int q = 0; q < WV_MAX_DROPS; ++q) {
+    const WaveDrop s = drops[q];
+    if (!(s.seed > 0.f && s.drain > 0.0005f)) continue;
+    const float ex = wx - s.x, ez = wz - s.z;
+    const float r = sqrtf(ex * ex + ez * ez);
+    const float th = atan2f(ez, ex);
+    const float azim = 1.f + 0.022f * cosf(4.f * th + s.seed * 1.73f) +
+                       0.016f * cosf(6.f * th - s.seed * 2.31f) + 0.010f * sinf(8.f * th + s.seed * 0.89f);
+    const float slug = wv_smoothstep(0.32f, 0.92f, s.drain);
+    const float r_or = clampv(s.r * 1.12f + mixv(0.018f, 0.032f, slug), 0.030f, 0.090f) * azim;
+    const float rn = r / fmaxf(r_or, 1e-4f);
+    const float speed = clampv(-s.vy, 0.08f, 0.65f);
+    const float neck = (0.28f + 1.05f * rn * rn) * expf(-0.92f * rn * rn) * (1.f - 0.38f * rn * rn);
+    pk = pk + s.drain * (0.014f + 0.048f * speed) * neck;
+  }
+  const float taper = 1.f - wv_smoothstep(hx * 0.72f, hx * 0.96f, fmaxf(fabsf(wx), fabsf(wz)));
+  ptop[(size_t)iz * nx + ix] = pk * dt_h * taper;
+}
+
+}  // namespace mac3d

@@ -13,13 +13,13 @@ LIB_PATH = os.path.join(PKG, "lib", "libmac3d.so")
 F32, F64 = 0, 1
 PREC = {"fp32": 0, "mixed": 1, "fp64": 2}
 FIELD = dict(U=0, V=1, W=2, P=3, DYE=4, CU=5, CV=6, CW=7, RHS=8, MGX=9, MGB=10, MGR=11,
-             DYE_PRED=12)
+             DYE_PRED=12, WAVE=13, PTOP=14)
 STAGE = dict(FACE_TO_CENTRE=0, ADVECT=1, CASSON=2, DIVERGENCE=3, VCYCLE=4, PROJECT=5,
-             DYE_PREDICT=6, DYE_RESOLVE=7)
+             DYE_PREDICT=6, DYE_RESOLVE=7, WAVE=8, TOP_PRESSURE=9)
 MG = dict(SMOOTH=0, RESIDUAL=1, RESTRICT=2, PROLONG=3, COARSE=4, ZERO=5, SMOOTH_ZERO=6)
 KC = dict(FACE_TO_CENTRE=0, ADVECT=1, CASSON=2, DIVERGENCE=3, SMOOTH_L0=4, RESTRICT_L0=5,
           PROLONG_L0=6, MG_COARSE_LEVELS=7, COARSE_SOLVE=8, PROJECT=9, DYE=10, REFINE64=11,
-          SMOOTH2_L0=12)
+          SMOOTH2_L0=12, WAVE=13)
 STATUS = {0: "OK", 1: "EINVAL", 2: "EHIP", 3: "ENOMEM", 4: "ESTATE"}
 
 
@@ -34,6 +34,20 @@ class Config(C.Structure):
 
 class Blob(C.Structure):
     _fields_ = [(n, C.c_double) for n in ("cx", "cy", "cz", "radius", "w0")]
+
+
+class WaveParams(C.Structure):
+    _fields_ = [("n", C.c_int32), ("reserved", C.c_int32)] + \
+               [(n, C.c_double) for n in ("half_extent", "half_xz", "depth", "surface_y",
+                                           "marangoni")]
+
+
+class Drop(C.Structure):
+    _fields_ = [(n, C.c_double) for n in ("x", "y", "z", "radius", "vel_y", "osc", "drain",
+                                           "seed")]
+
+
+MAX_DROPS = 8
 
 
 class Stats(C.Structure):
@@ -66,7 +80,8 @@ _lib = None
 EXPORTS = ("mac3d_default_config", "mac3d_create", "mac3d_destroy", "mac3d_upload",
            "mac3d_download", "mac3d_init_blobs", "mac3d_step", "mac3d_stage", "mac3d_mg_op",
            "mac3d_div_norms", "mac3d_profile", "mac3d_profile_read", "mac3d_adjoint",
-           "mac3d_last_error", "mac3d_version")
+           "mac3d_last_error", "mac3d_version", "mac3d_wave_default_params",
+           "mac3d_wave_enable", "mac3d_set_drops")
 
 
 def load_library(path: Optional[str] = None):
@@ -93,6 +108,9 @@ def load_library(path: Optional[str] = None):
     L.mac3d_profile_read.argtypes = [vp, C.c_int, C.POINTER(C.c_double), C.POINTER(C.c_int64)]
     L.mac3d_adjoint.argtypes = [vp, C.POINTER(Blob), C.c_int, C.c_double, C.c_int, C.c_int, vp,
                                 C.c_int, C.POINTER(Grad)]
+    L.mac3d_wave_default_params.argtypes = [C.POINTER(WaveParams)]
+    L.mac3d_wave_enable.argtypes = [vp, C.POINTER(WaveParams)]
+    L.mac3d_set_drops.argtypes = [vp, C.POINTER(Drop), C.c_int]
     L.mac3d_last_error.argtypes = [vp]
     L.mac3d_last_error.restype = C.c_char_p
     L.mac3d_version.restype = C.c_char_p
@@ -159,9 +177,17 @@ class MacSolver:
         nz, ny, nx = self.shape
         return (nz >> level, ny >> level, nx >> level)
 
+    def field_shape(self, fid: int, level: int = 0):
+        if fid == FIELD["WAVE"]:
+            n = self.wave_n
+            return (n, n, 4)
+        if fid == FIELD["PTOP"]:
+            return (self.cfg.nz, self.cfg.nx)
+        return self.level_shape(level)
+
     def download(self, field, level: int = 0, dtype=np.float64) -> np.ndarray:
         fid = FIELD[field] if isinstance(field, str) else field
-        out = np.empty(self.level_shape(level), dtype=dtype)
+        out = np.empty(self.field_shape(fid, level), dtype=dtype)
         dt = F64 if out.dtype == np.float64 else F32
         self._check(self.L.mac3d_download(self.h, fid, level, out.ctypes.data, dt, out.size))
         return out
@@ -194,6 +220,31 @@ class MacSolver:
 
     @property
     def dye(self): return self.download("DYE")
+
+    # -- free-surface coupling
+    wave_n = 0
+
+    def wave_enable(self, enable: bool = True, **kw):
+        """Enable the free-surface wave coupling (flat surface, no drops) with
+        mac3d_wave_params fields as keywords (n, half_extent, half_xz, depth,
+        surface_y, marangoni), or disable it."""
+        if not enable:
+            self._check(self.L.mac3d_wave_enable(self.h, None))
+            self.wave_n = 0
+            return
+        p = WaveParams()
+        self._check(self.L.mac3d_wave_default_params(C.byref(p)))
+        for k, v in kw.items():
+            if not hasattr(p, k):
+                raise AttributeError(k)
+            setattr(p, k, v)
+        self._check(self.L.mac3d_wave_enable(self.h, C.byref(p)))
+        self.wave_n = p.n
+
+    def set_drops(self, drops: Sequence):
+        """drops: up to 8 sequences (x, y, z, radius, vel_y, osc, drain, seed)"""
+        arr = (Drop * max(1, len(drops)))(*[Drop(*map(float, d)) for d in drops])
+        self._check(self.L.mac3d_set_drops(self.h, arr, len(drops)))
 
     # -- stepping
     def init_blobs(self, blobs: Sequence):

@@ -62,7 +62,12 @@ extern "C" {
 #define MAC3D_FIELD_MGB 10 /* multigrid right-hand side of a level */
 #define MAC3D_FIELD_MGR 11 /* multigrid residual of a level */
 #define MAC3D_FIELD_DYE_PRED 12 /* dye semi-Lagrangian predictor */
-#define MAC3D_FIELD_COUNT 13
+#define MAC3D_FIELD_WAVE 13 /* free-surface wave state, n*n texels x 4 (eta (m), d eta/dt
+                               (m/s), 9-point Laplacian of eta, surfactant film), texel
+                               (ix, iy) at 4*(iy*n + ix); needs mac3d_wave_enable */
+#define MAC3D_FIELD_PTOP 14 /* top Dirichlet pressure of the lid (dimensionless like P),
+                               nx*nz values at iz*nx + ix; needs mac3d_wave_enable */
+#define MAC3D_FIELD_COUNT 15
 
 /* single stages of the step, for kernel-level parity tests */
 #define MAC3D_STAGE_FACE_TO_CENTRE 0 /* CU,CV,CW <- U,V,W */
@@ -73,7 +78,9 @@ extern "C" {
 #define MAC3D_STAGE_PROJECT 5        /* U,V,W <- U,V,W - G(P) */
 #define MAC3D_STAGE_DYE_PREDICT 6    /* DYE_PRED <- semi-Lagrangian(DYE) */
 #define MAC3D_STAGE_DYE_RESOLVE 7    /* DYE <- MacCormack + limiter */
-#define MAC3D_STAGE_COUNT 8
+#define MAC3D_STAGE_WAVE 8           /* WAVE <- one free-surface wave step (needs V) */
+#define MAC3D_STAGE_TOP_PRESSURE 9   /* PTOP <- top Dirichlet pressure from WAVE and drops */
+#define MAC3D_STAGE_COUNT 10
 
 /* single multigrid operators on one level, for parity tests */
 #define MAC3D_MG_SMOOTH 0   /* count RBGS sweeps on (MGX, MGB) */
@@ -99,7 +106,8 @@ extern "C" {
 #define MAC3D_KC_DYE 10            /* dye predictor + resolve */
 #define MAC3D_KC_REFINE64 11       /* mixed: FP64 residual/update */
 #define MAC3D_KC_SMOOTH2_L0 12     /* two RBGS sweeps in one pass on level 0 (FP32) */
-#define MAC3D_KC_COUNT 13
+#define MAC3D_KC_WAVE 13           /* free-surface wave step and top pressure */
+#define MAC3D_KC_COUNT 14
 
 typedef struct mac3d_config {
   int32_t nx, ny, nz;   /* level-0 cells; each must divide by 2^levels */
@@ -193,6 +201,46 @@ int mac3d_step(void *ctx, double dt, int nsteps, mac3d_stats *stats);
  * Leaves the context at an unspecified state of the run. */
 int mac3d_adjoint(void *ctx, const mac3d_blob *blobs, int nblobs, double dt, int nsteps,
                   int checkpoint_every, const void *target, int dtype, mac3d_grad *out);
+
+/* Free-surface coupling.  The wave solver of the
+ * reference scene -- a gravity-capillary spectral propagator on an n x n grid
+ * with drop impulses, surfactant film, Marangoni pull and the MAC top layer's
+ * upward velocity as forcing (src/shaders/wave.ts:56-388) -- and the top
+ * Dirichlet pressure it feeds into the MAC divergence and projection
+ * (evalTopDirichletPressure, src/shaders/fluid.ts:16-78, used at
+ * fluid.ts:548-551 and 896-898).  Enabled, every step first advances the
+ * wave one step and evaluates the lid pressure, in the reference's order
+ * (BloodDropEngine.ts:1886-1914, before the MAC advection).  The wave computes in FP32 in every mode.
+ * Lengths in metres; the tank spans [-half_xz, half_xz] in x and z. */
+typedef struct mac3d_wave_params {
+  int32_t n;          /* wave texels per side, power of two in [16, 1024] (512) */
+  int32_t reserved;   /* 0 */
+  double half_extent; /* half-width of the wave domain (WAVE_HALF_EXTENT, 1.60) */
+  double half_xz;     /* tank half-width (domainBounds.x; 0: nx*dx/2) */
+  double depth;       /* tank depth (domainBounds.z - domainBounds.y; 0: ny*dx) */
+  double surface_y;   /* still-surface height (surfaceParams.x, 0.30) */
+  double marangoni;   /* Marangoni strength (surfaceParams.y, 0) */
+} mac3d_wave_params;
+
+/* one drop slot (SimUniforms.drops[d], common.ts:26-43, packed by
+ * uniformLayout.ts:161-187): active when seed > 0 and drain > 0.0005 */
+typedef struct mac3d_drop {
+  double x, y, z, radius; /* posAndRadius (m) */
+  double vel_y, osc;      /* velOscAndDrain.x (m/s), .y */
+  double drain, seed;     /* velOscAndDrain.z, .w */
+} mac3d_drop;
+#define MAC3D_MAX_DROPS 8
+
+/* no reference counterpart (the reference fills these uniforms itself) */
+int mac3d_wave_default_params(mac3d_wave_params *p);
+/* replaces: the wave textures and pipelines of the BloodDropEngine
+ * constructor (BloodDropEngine.ts:545-589) and WAVE_CLEAR (wave.ts:15-30):
+ * allocates the wave and resets it to a flat surface; p == NULL disables
+ * the coupling (the lid pressure is 0 again) */
+int mac3d_wave_enable(void *ctx, const mac3d_wave_params *p);
+/* replaces: the drop slots of SimUniformPacker.pack (uniformLayout.ts:161-187);
+ * ndrops in [0, 8], the other slots inactive; ESTATE unless enabled */
+int mac3d_set_drops(void *ctx, const mac3d_drop *drops, int ndrops);
 
 /* one stage of the step (MAC3D_STAGE_*), blocking */
 int mac3d_stage(void *ctx, int stage, double dt);

@@ -27,9 +27,10 @@
 #define MAXLEV 8
 
 enum { F_U, F_V, F_W, F_P, F_DYE, F_CU, F_CV, F_CW, F_RHS, F_MGX, F_MGB,
-       F_MGR, F_DYE_PRED, F_COUNT };
+       F_MGR, F_DYE_PRED, F_WAVE, F_PTOP, F_COUNT };
 enum { ST_F2C, ST_ADVECT, ST_CASSON, ST_DIV, ST_VCYCLE, ST_PROJECT,
-       ST_DYE_PRED, ST_DYE_RES };
+       ST_DYE_PRED, ST_DYE_RES, ST_WAVE, ST_TOP_PRESSURE };
+#define MAX_DROPS 8
 enum { MG_SMOOTH, MG_RESIDUAL, MG_RESTRICT, MG_PROLONG, MG_COARSE, MG_ZERO };
 
 typedef struct { int nx, ny, nz; } dims3;
@@ -42,6 +43,12 @@ struct orc_state {
   double *u2, *v2, *w2;
   double *mgx[MAXLEV + 1], *mgb[MAXLEV + 1], *mgr[MAXLEV + 1];
   double *cg[4]; /* coarse-solve CG vectors */
+  /* wave coupling: state (4 per texel), complex work (2 per texel), film,
+   * lid pressure (nx * nz) */
+  int wave_on;
+  orc_wave_params wp;
+  double *wave, *wz, *wz2, *wfilm, *ptop;
+  orc_drop drops[MAX_DROPS];
 };
 
 #define IDX(D, i, j, k) ((((long)(k)) * (D).ny + (j)) * (long)(D).nx + (i))
@@ -269,8 +276,9 @@ static void casson_buoyancy(orc_state *s, double dt) {
 }
 
 /* ---- FLUID_MAC_RHS (fluid.ts:517-555): D = (uR-uL)+(vT-vB)+(wF-wB),
- * solid wall faces 0, top face free.  WORKLOAD: flat lid, p_top = 0
- * (evalTopDirichletPressure fluid.ts:16-78 needs the wave solver). */
+ * solid wall faces 0, top face free; on the top row D -= p_top
+ * (fluid.ts:548-551), the wave coupling's lid pressure, else the flat lid's
+ * p_top = 0. */
 static void divergence(const orc_state *s, const double *u, const double *v,
                        const double *w, double *out) {
   dims3 d = s->d[0];
@@ -285,7 +293,9 @@ static void divergence(const orc_state *s, const double *u, const double *v,
         double vB = (j == 0) ? 0.0 : v[IDX(d, i, j - 1, k)];
         double wF = (k >= d.nz - 1) ? 0.0 : w[c];
         double wB = (k == 0) ? 0.0 : w[IDX(d, i, j, k - 1)];
-        out[c] = (uR - uL) + (vT - vB) + (wF - wB);
+        double rhs = (uR - uL) + (vT - vB) + (wF - wB);
+        if (s->wave_on && j == d.ny - 1) rhs -= s->ptop[(long)k * d.nx + i];
+        out[c] = rhs;
       }
 }
 
@@ -515,8 +525,9 @@ static double rel_residual(orc_state *s) {
 }
 
 /* ---- FLUID_MAC_PROJECT (fluid.ts:863-911): face -= p(+1) - p, wall faces
- * 0, top face uses the lid ghost (p_top = 0 for the flat lid).  WORKLOAD: the
- * reference's +-2 clamp (lattice units) is not part of the workload. */
+ * 0, top face against the lid's ghost p_top (fluid.ts:893-898; the wave
+ * coupling's, else 0 for the flat lid).  WORKLOAD: the reference's +-2 clamp
+ * (lattice units) is not part of the workload. */
 static void project(orc_state *s) {
   dims3 d = s->d[0];
 #pragma omp parallel for
@@ -526,7 +537,8 @@ static void project(orc_state *s) {
         long c = IDX(d, i, j, k);
         double pc = s->pr[c];
         if (i < d.nx - 1) s->u[c] -= s->pr[IDX(d, i + 1, j, k)] - pc; else s->u[c] = 0.0;
-        if (j < d.ny - 1) s->v[c] -= s->pr[IDX(d, i, j + 1, k)] - pc; else s->v[c] -= 0.0 - pc;
+        double ptop = s->wave_on ? s->ptop[(long)k * d.nx + i] : 0.0;
+        if (j < d.ny - 1) s->v[c] -= s->pr[IDX(d, i, j + 1, k)] - pc; else s->v[c] -= ptop - pc;
         if (k < d.nz - 1) s->w[c] -= s->pr[IDX(d, i, j, k + 1)] - pc; else s->w[c] = 0.0;
       }
 }
@@ -604,6 +616,271 @@ static void dye_resolve(orc_state *s, double dt) {
   memcpy(s->dye, out, sizeof(double) * s->n[0]);
 }
 
+/* ==== free-surface wave coupling (wave.ts:15-388, fluid.ts:16-78) ==== */
+
+static double sstep(double e0, double e1, double x) { return orc_smoothstep(e0, e1, x); }
+static const double PI_D = 3.14159265358979323846;
+
+/* loadWaveClamped (wave.ts:63-67): texel (x, y) channel ch, indices clamped */
+static double wv_load(const double *w, int n, int x, int y, int ch) {
+  x = clampi(x, 0, n - 1);
+  y = clampi(y, 0, n - 1);
+  return w[4 * ((long)y * n + x) + ch];
+}
+
+/* textureSampleLevel of the wave state with the linear clamp-to-edge sampler
+ * (BloodDropEngine.ts:497-505): bilinear in texel space t = uv n - 1/2 */
+static double wv_sample(const double *w, int n, double u, double v, int ch) {
+  double tx = u * n - 0.5, ty = v * n - 0.5;
+  double fx0 = floor(tx), fy0 = floor(ty);
+  int x0 = (int)fx0, y0 = (int)fy0;
+  double fx = tx - fx0, fy = ty - fy0;
+  double a = mixd(wv_load(w, n, x0, y0, ch), wv_load(w, n, x0 + 1, y0, ch), fx);
+  double b = mixd(wv_load(w, n, x0, y0 + 1, ch), wv_load(w, n, x0 + 1, y0 + 1, ch), fx);
+  return mixd(a, b, fy);
+}
+
+/* WAVE_SPATIAL_FORCE (wave.ts:56-178) into wz (eta + i deta/dt) and wfilm */
+static void wave_force(orc_state *s, double dt) {
+  const int n = s->wp.n;
+  const double E = s->wp.half_extent, hx = s->wp.half_xz;
+  const double time_scale = dt / 0.0166667;
+  const double lap_scale = ((double)n / 256.0) * ((double)n / 256.0);
+#pragma omp parallel for
+  for (int iy = 0; iy < n; ++iy)
+    for (int ix = 0; ix < n; ++ix) {
+      const double *W = s->wave;
+      double eta = wv_load(W, n, ix, iy, 0), vel = wv_load(W, n, ix, iy, 1);
+      double film = wv_load(W, n, ix, iy, 3);
+      double raw_lap = 0.20 * (wv_load(W, n, ix, iy + 1, 3) + wv_load(W, n, ix, iy - 1, 3) +
+                               wv_load(W, n, ix + 1, iy, 3) + wv_load(W, n, ix - 1, iy, 3)) +
+                       0.05 * (wv_load(W, n, ix + 1, iy + 1, 3) + wv_load(W, n, ix - 1, iy + 1, 3) +
+                               wv_load(W, n, ix + 1, iy - 1, 3) + wv_load(W, n, ix - 1, iy - 1, 3)) -
+                       film;
+      double lap_film = raw_lap * lap_scale;
+      /* waveUVToWorldXZ (common.ts:69-71) at the texel centre */
+      double wx = ((ix + 0.5) / n) * (2.0 * E) - E, wz = ((iy + 0.5) / n) * (2.0 * E) - E;
+      double force_drops = 0.0, active = 0.0, max_drain = 0.0;
+      for (int q = 0; q < MAX_DROPS; ++q) {
+        const orc_drop *dr = &s->drops[q];
+        if (!(dr->seed > 0.0 && dr->drain > 0.0005)) continue;
+        max_drain = fmax(max_drain, dr->drain);
+        double rd = fmax(dr->radius, 0.018);
+        double speed = clampd(-dr->vel_y, 0.06, 0.65);
+        double r = hypot(wx - dr->x, wz - dr->z);
+        double rn = r / clampd(rd * 1.05, 0.024, 0.082);
+        double emerg = (dr->y - s->wp.surface_y) / fmax(rd, 0.018);
+        double excav = sstep(-0.78, -0.04, emerg);
+        double bell = sin(clampd((1.05 - emerg) / 1.83, 0.0, 1.0) * 3.14159265) * excav;
+        double shape = exp(-rn * rn) * (1.0 - rn * rn);
+        double dyn = (6.8 * bell * (rd / 0.056)) * (0.72 + 2.65 * speed);
+        double rnc = r / clampd(rd * 0.44, 0.013, 0.030);
+        double cap_win = sstep(-0.35, 0.28, emerg) * (1.0 - sstep(0.68, 1.06, emerg));
+        double cap_shape = exp(-rnc * rnc) * (1.0 - rnc * rnc);
+        double cap = (4.6 * cap_win * (0.60 + 0.40 * (rd / 0.056))) * (0.65 + 2.20 * speed);
+        force_drops = force_drops - dyn * shape - cap * cap_shape;
+        active = fmax(active, excav * exp(-0.5 * rn * rn));
+        double r_stain = fmax(0.018, rd * 0.60);
+        double stain = exp(-(r * r) / (r_stain * r_stain)) * clampd(dr->drain * 0.42, 0.0, 1.0);
+        film = fmax(film, stain * 0.65);
+      }
+      double m = s->wp.marangoni;
+      double eff_lap = mixd(fmax(lap_film, -0.0015), lap_film, active);
+      double force_mar = m * 0.28 * eff_lap;
+      film = clampd(film + m * 0.065 * raw_lap * time_scale, 0.0, 1.0);
+      film = film * pow(0.88, time_scale);
+      double force_sub = 0.0;
+      double gate = clampd(active * 1.4 + sstep(0.0, 0.06, max_drain), 0.0, 1.0);
+      double rw = hypot(wx, wz);
+      if (gate > 0.001 && rw < hx * 0.94) {
+        /* velMac .y at uvw (tankUV.x, 0.988, tankUV.y), trilinear clamp-to-edge */
+        double vsub = sample_cell(s->v, s->d[0], (wx + hx) / (2.0 * hx), 0.988, (wz + hx) / (2.0 * hx));
+        double taper = 1.0 - sstep(hx * 0.62, hx * 0.92, rw);
+        force_sub = clampd(fmax(vsub, 0.0), 0.0, 0.48) * 0.42 * taper * gate;
+      }
+      vel = vel + (force_drops + force_mar + force_sub) * dt;
+      double absorb = sstep(0.90, 1.54, rw);
+      double rdamp = exp(-dt * (24.0 * absorb * absorb));
+      double border = 1.0 - sstep(1.22, 1.56, rw);
+      eta = clampd(eta * rdamp * border, -0.14, 0.14);
+      vel = clampd(vel * rdamp * border, -1.8, 1.8);
+      long t = (long)iy * n + ix;
+      s->wz[2 * t] = eta;
+      s->wz[2 * t + 1] = vel;
+      s->wfilm[t] = film * border;
+    }
+}
+
+/* Unitary radix-2 FFT (the reference's createRadix2Fft1DShader, wave.ts:180-
+ * 252, is a decimation-in-time radix-2 with 1/sqrt(n) on every 1-D
+ * transform): bit-reversed load, log2 n butterfly stages with exact
+ * twiddles exp(sign 2 pi i k / span), natural-order result. */
+void orc_fft(double *z, int n, int stride, int sign) {
+  double *t = (double *)malloc(sizeof(double) * 2 * n);
+  int logn = 0;
+  while ((1 << logn) < n) ++logn;
+  for (int i = 0; i < n; ++i) {
+    int r = 0;
+    for (int b = 0; b < logn; ++b) r |= ((i >> b) & 1) << (logn - 1 - b);
+    t[2 * i] = z[2 * (long)r * stride];
+    t[2 * i + 1] = z[2 * (long)r * stride + 1];
+  }
+  for (int s = 0; s < logn; ++s) {
+    int half = 1 << s, span = half << 1;
+    for (int blk = 0; blk < n; blk += span)
+      for (int k = 0; k < half; ++k) {
+        double ang = sign * 2.0 * PI_D * k / span;
+        double c = cos(ang), sn = sin(ang);
+        int i0 = blk + k, i1 = i0 + half;
+        double br = t[2 * i1] * c - t[2 * i1 + 1] * sn, bi = t[2 * i1] * sn + t[2 * i1 + 1] * c;
+        double ar = t[2 * i0], ai = t[2 * i0 + 1];
+        t[2 * i0] = ar + br;
+        t[2 * i0 + 1] = ai + bi;
+        t[2 * i1] = ar - br;
+        t[2 * i1 + 1] = ai - bi;
+      }
+  }
+  double sc = 1.0 / sqrt((double)n);
+  for (int i = 0; i < n; ++i) {
+    z[2 * (long)i * stride] = t[2 * i] * sc;
+    z[2 * (long)i * stride + 1] = t[2 * i + 1] * sc;
+  }
+  free(t);
+}
+
+/* WAVE_SPECTRAL_PROPAGATE (wave.ts:254-331) from wz into wz2 */
+static void wave_propagate(orc_state *s, double dt) {
+  const int n = s->wp.n, hn = n / 2;
+  const double L = 2.0 * s->wp.half_extent, H = s->wp.depth;
+#pragma omp parallel for
+  for (int uy = 0; uy < n; ++uy)
+    for (int ux = 0; ux < n; ++ux) {
+      long t = (long)uy * n + ux;
+      if ((ux == 0 && uy == 0) || ux == hn || uy == hn) {
+        s->wz2[2 * t] = s->wz2[2 * t + 1] = 0.0;
+        continue;
+      }
+      long tn = (long)((n - uy) & (n - 1)) * n + ((n - ux) & (n - 1));
+      double pr = s->wz[2 * t], pi = s->wz[2 * t + 1], mr = s->wz[2 * tn], mi = s->wz[2 * tn + 1];
+      double er = 0.5 * (pr + mr), ei = 0.5 * (pi - mi);   /* eta_hat */
+      double vr = 0.5 * (pi + mi), vi = 0.5 * (-pr + mr);  /* w_hat */
+      double kx = (ux > hn ? ux - n : ux) * (2.0 * PI_D / L);
+      double kz = (uy > hn ? uy - n : uy) * (2.0 * PI_D / L);
+      double k2 = kx * kx + kz * kz, k = sqrt(k2);
+      double th = tanh(fmin(k * H, 12.0));
+      double omega = sqrt(fmax((0.84 * k + 0.0024 * k * k2) * th, 1e-8));
+      double damp = exp(-(0.36 + 0.00018 * k2) * dt) * (1.0 - sstep(335.0, 465.0, k));
+      double c = cos(omega * dt), sn = sin(omega * dt);
+      double enr = damp * (er * c + vr * (sn / omega)), eni = damp * (ei * c + vi * (sn / omega));
+      double vnr = damp * (-er * (omega * sn) + vr * c), vni = damp * (-ei * (omega * sn) + vi * c);
+      s->wz2[2 * t] = enr - vni;
+      s->wz2[2 * t + 1] = eni + vnr;
+    }
+}
+
+/* WAVE_FINALIZE (wave.ts:333-388) from wz2 and wfilm into the wave state */
+static void wave_finalize(orc_state *s) {
+  const int n = s->wp.n;
+  const double E = s->wp.half_extent;
+  const double lap_scale = ((double)n / 256.0) * ((double)n / 256.0);
+#define ETA(x, y) s->wz2[2 * ((long)clampi((y), 0, n - 1) * n + clampi((x), 0, n - 1))]
+#pragma omp parallel for
+  for (int iy = 0; iy < n; ++iy)
+    for (int ix = 0; ix < n; ++ix) {
+      long t = (long)iy * n + ix;
+      double px = s->wz2[2 * t], py = s->wz2[2 * t + 1];
+      double lap = (0.20 * (ETA(ix, iy + 1) + ETA(ix, iy - 1) + ETA(ix + 1, iy) + ETA(ix - 1, iy)) +
+                    0.05 * (ETA(ix + 1, iy + 1) + ETA(ix - 1, iy + 1) + ETA(ix + 1, iy - 1) +
+                            ETA(ix - 1, iy - 1)) - px) * lap_scale;
+      double wx = ((ix + 0.5) / n) * (2.0 * E) - E, wz = ((iy + 0.5) / n) * (2.0 * E) - E;
+      double border = 1.0 - sstep(1.22, 1.56, hypot(wx, wz));
+      s->wave[4 * t] = clampd(px * border, -0.14, 0.14);
+      s->wave[4 * t + 1] = clampd(py * border, -1.8, 1.8);
+      s->wave[4 * t + 2] = lap * border;
+      s->wave[4 * t + 3] = s->wfilm[t] * border;
+    }
+#undef ETA
+}
+
+/* one wave step in the pass order of BloodDropEngine.ts:1886-1914 */
+static void wave_step(orc_state *s, double dt) {
+  const int n = s->wp.n;
+  dt = clampd(dt, 0.0, 0.025); /* viewportAndDt.z clamp (wave.ts:83) */
+  wave_force(s, dt);
+#pragma omp parallel for
+  for (int y = 0; y < n; ++y) orc_fft(s->wz + 2 * (long)y * n, n, 1, -1);
+#pragma omp parallel for
+  for (int x = 0; x < n; ++x) orc_fft(s->wz + 2 * (long)x, n, n, -1);
+  wave_propagate(s, dt);
+#pragma omp parallel for
+  for (int x = 0; x < n; ++x) orc_fft(s->wz2 + 2 * (long)x, n, n, +1);
+#pragma omp parallel for
+  for (int y = 0; y < n; ++y) orc_fft(s->wz2 + 2 * (long)y * n, n, 1, +1);
+  wave_finalize(s);
+}
+
+/* evalTopDirichletPressure (fluid.ts:16-78) over the lid plane */
+static void top_pressure(orc_state *s, double dt) {
+  dims3 d = s->d[0];
+  const double hx = s->wp.half_xz, E = s->wp.half_extent;
+  dt = clampd(dt, 0.0, 0.025);
+  const double dt_h = dt / ((2.0 * hx) / d.nx);
+#pragma omp parallel for
+  for (int iz = 0; iz < d.nz; ++iz)
+    for (int ix = 0; ix < d.nx; ++ix) {
+      double wx = mixd(-hx, hx, (ix + 0.5) / d.nx), wz = mixd(-hx, hx, (iz + 0.5) / d.nz);
+      double su = clampd((wx + E) / (2.0 * E), 0.002, 0.998);
+      double sv = clampd((wz + E) / (2.0 * E), 0.002, 0.998);
+      double eta = wv_sample(s->wave, s->wp.n, su, sv, 0);
+      double lap = wv_sample(s->wave, s->wp.n, su, sv, 2);
+      double pk = 2.45 * (0.155 * eta + 0.00095 * (-18.4 * lap));
+      for (int q = 0; q < MAX_DROPS; ++q) {
+        const orc_drop *dr = &s->drops[q];
+        if (!(dr->seed > 0.0 && dr->drain > 0.0005)) continue;
+        double ex = wx - dr->x, ez = wz - dr->z;
+        double r = hypot(ex, ez), th = atan2(ez, ex);
+        double azim = 1.0 + 0.022 * cos(4.0 * th + dr->seed * 1.73) +
+                      0.016 * cos(6.0 * th - dr->seed * 2.31) + 0.010 * sin(8.0 * th + dr->seed * 0.89);
+        double slug = sstep(0.32, 0.92, dr->drain);
+        double r_or = clampd(dr->radius * 1.12 + mixd(0.018, 0.032, slug), 0.030, 0.090) * azim;
+        double rn = r / fmax(r_or, 1e-4);
+        double speed = clampd(-dr->vel_y, 0.08, 0.65);
+        double neck = (0.28 + 1.05 * rn * rn) * exp(-0.92 * rn * rn) * (1.0 - 0.38 * rn * rn);
+        pk = pk + dr->drain * (0.014 + 0.048 * speed) * neck;
+      }
+      double taper = 1.0 - sstep(hx * 0.72, hx * 0.96, fmax(fabs(wx), fabs(wz)));
+      s->ptop[(long)iz * d.nx + ix] = pk * dt_h * taper;
+    }
+}
+
+void orc_wave_enable(orc_state *s, const orc_wave_params *p) {
+  if (!p) {
+    s->wave_on = 0;
+    return;
+  }
+  if (!s->wave || s->wp.n != p->n) {
+    free(s->wave); free(s->wz); free(s->wz2); free(s->wfilm);
+    long nn = (long)p->n * p->n;
+    s->wave = (double *)calloc(4 * nn, sizeof(double));
+    s->wz = (double *)calloc(2 * nn, sizeof(double));
+    s->wz2 = (double *)calloc(2 * nn, sizeof(double));
+    s->wfilm = (double *)calloc(nn, sizeof(double));
+  }
+  if (!s->ptop) s->ptop = (double *)calloc((long)s->d[0].nx * s->d[0].nz, sizeof(double));
+  s->wp = *p;
+  if (!(s->wp.half_xz > 0.0)) s->wp.half_xz = 0.5 * s->p.nx * s->p.dx;
+  if (!(s->wp.depth > 0.0)) s->wp.depth = s->p.ny * s->p.dx;
+  memset(s->wave, 0, sizeof(double) * 4 * (long)p->n * p->n);
+  memset(s->ptop, 0, sizeof(double) * (long)s->d[0].nx * s->d[0].nz);
+  memset(s->drops, 0, sizeof(s->drops));
+  s->wave_on = 1;
+}
+
+void orc_set_drops(orc_state *s, const orc_drop *dr, int n) {
+  memset(s->drops, 0, sizeof(s->drops));
+  for (int q = 0; q < n && q < MAX_DROPS; ++q) s->drops[q] = dr[q];
+}
+
 /* ---- state ---- */
 orc_state *orc_create(const orc_params *p) {
   if (p->levels < 1 || p->levels > MAXLEV) return NULL;
@@ -638,6 +915,7 @@ void orc_destroy(orc_state *s) {
   for (unsigned a = 0; a < sizeof(f0) / sizeof(f0[0]); ++a) free(f0[a]);
   for (int l = 1; l <= s->p.levels; ++l) { free(s->mgx[l]); free(s->mgb[l]); free(s->mgr[l]); }
   for (int a = 0; a < 4; ++a) free(s->cg[a]);
+  free(s->wave); free(s->wz); free(s->wz2); free(s->wfilm); free(s->ptop);
   free(s);
 }
 
@@ -657,12 +935,16 @@ double *orc_field(orc_state *s, int field, int level) {
     case F_MGB: return s->mgb[level];
     case F_MGR: return s->mgr[level];
     case F_DYE_PRED: return s->dpred;
+    case F_WAVE: return s->wave_on && level == 0 ? s->wave : NULL;
+    case F_PTOP: return s->wave_on && level == 0 ? s->ptop : NULL;
   }
   return NULL;
 }
 
 long orc_field_len(orc_state *s, int field, int level) {
   if (field == F_MGX || field == F_MGB || field == F_MGR) return s->n[level];
+  if (field == F_WAVE) return 4L * s->wp.n * s->wp.n;
+  if (field == F_PTOP) return (long)s->d[0].nx * s->d[0].nz;
   return s->n[0];
 }
 
@@ -706,6 +988,8 @@ void orc_stage(orc_state *s, int stage, double dt) {
     case ST_PROJECT: project(s); break;
     case ST_DYE_PRED: dye_predict(s, dt); break;
     case ST_DYE_RES: dye_resolve(s, dt); break;
+    case ST_WAVE: if (s->wave_on) wave_step(s, dt); break;
+    case ST_TOP_PRESSURE: if (s->wave_on) top_pressure(s, dt); break;
   }
 }
 
@@ -729,15 +1013,20 @@ void orc_mg_op(orc_state *s, int op, int level, int count) {
   }
 }
 
-/* The step order of encodePhysicsStep (BloodDropEngine.ts:1916-2069) and
- * pass B's dye transport (:2096-2108): advect, Casson + buoyancy, RHS,
- * V-cycles, project, face->centre, dye.  The face->centre reconstruction
+/* The step order of encodePhysicsStep (BloodDropEngine.ts:1886-2069) and
+ * pass B's dye transport (:2096-2108): with the wave coupling, the wave step
+ * and the lid pressure first (:1886-1914); then advect, Casson + buoyancy,
+ * RHS, V-cycles, project, face->centre, dye.  The face->centre reconstruction
  * before advection (:1922-1925) equals the one after the previous step's
  * projection (:2067-2069), so it is computed once per step here. */
 int orc_step(orc_state *s, double dt, int nsteps, double *rel_res) {
   int cycles = 0;
   double rr = 0.0;
   for (int n = 0; n < nsteps; ++n) {
+    if (s->wave_on) {
+      wave_step(s, dt);
+      top_pressure(s, dt);
+    }
     face_to_centre(s);
     advect_mac(s, dt);
     casson_buoyancy(s, dt);

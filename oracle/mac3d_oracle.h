@@ -31,6 +31,17 @@ typedef struct orc_blob {
   double cx, cy, cz, radius, w0;
 } orc_blob;
 
+/* free-surface wave coupling (lengths in metres; 0 picks nx dx / 2 and
+ * ny dx for half_xz and depth) and one drop slot, as in include/mac3d.h */
+typedef struct orc_wave_params {
+  int n;
+  double half_extent, half_xz, depth, surface_y, marangoni;
+} orc_wave_params;
+
+typedef struct orc_drop {
+  double x, y, z, radius, vel_y, osc, drain, seed;
+} orc_drop;
+
 /* state of one simulation held by the oracle */
 typedef struct orc_state orc_state;
 
@@ -46,6 +57,12 @@ void orc_stage(orc_state *s, int stage, double dt);
 void orc_mg_op(orc_state *s, int op, int level, int count);
 void orc_div_norms(orc_state *s, double *max_div, double *l2_div);
 int orc_set_threads(int n);
+/* enable (p != NULL: flat surface, no drops) or disable the wave coupling */
+void orc_wave_enable(orc_state *s, const orc_wave_params *p);
+void orc_set_drops(orc_state *s, const orc_drop *d, int n);
+/* unitary in-place complex FFT of length n (a power of two) over interleaved
+ * (re, im) doubles, sign -1 forward / +1 inverse: exposed for its pinning */
+void orc_fft(double *z, int n, int stride, int sign);
 
 /* helpers exposed for the pinning tests */
 double orc_sample_cell(const double *f, int nx, int ny, int nz, double ux,

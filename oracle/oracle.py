@@ -15,8 +15,10 @@ import numpy as np
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB = os.path.join(HERE, "liborc.so")
 
-F_U, F_V, F_W, F_P, F_DYE, F_CU, F_CV, F_CW, F_RHS, F_MGX, F_MGB, F_MGR, F_DYE_PRED = range(13)
-ST_F2C, ST_ADVECT, ST_CASSON, ST_DIV, ST_VCYCLE, ST_PROJECT, ST_DYE_PRED, ST_DYE_RES = range(8)
+(F_U, F_V, F_W, F_P, F_DYE, F_CU, F_CV, F_CW, F_RHS, F_MGX, F_MGB, F_MGR, F_DYE_PRED, F_WAVE,
+ F_PTOP) = range(15)
+(ST_F2C, ST_ADVECT, ST_CASSON, ST_DIV, ST_VCYCLE, ST_PROJECT, ST_DYE_PRED, ST_DYE_RES, ST_WAVE,
+ ST_TOP_PRESSURE) = range(10)
 MG_SMOOTH, MG_RESIDUAL, MG_RESTRICT, MG_PROLONG, MG_COARSE, MG_ZERO = range(6)
 
 
@@ -30,6 +32,15 @@ class OrcParams(C.Structure):
 
 class OrcBlob(C.Structure):
     _fields_ = [(n, C.c_double) for n in ("cx", "cy", "cz", "radius", "w0")]
+
+
+class OrcWaveParams(C.Structure):
+    _fields_ = [("n", C.c_int)] + [(n, C.c_double) for n in (
+        "half_extent", "half_xz", "depth", "surface_y", "marangoni")]
+
+
+class OrcDrop(C.Structure):
+    _fields_ = [(n, C.c_double) for n in ("x", "y", "z", "radius", "vel_y", "osc", "drain", "seed")]
 
 
 def build():
@@ -58,6 +69,9 @@ def lib():
         L.orc_stage.argtypes = [C.c_void_p, C.c_int, C.c_double]
         L.orc_mg_op.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_int]
         L.orc_div_norms.argtypes = [C.c_void_p, C.POINTER(C.c_double), C.POINTER(C.c_double)]
+        L.orc_wave_enable.argtypes = [C.c_void_p, C.POINTER(OrcWaveParams)]
+        L.orc_set_drops.argtypes = [C.c_void_p, C.POINTER(OrcDrop), C.c_int]
+        L.orc_fft.argtypes = [C.POINTER(C.c_double), C.c_int, C.c_int, C.c_int]
         L.orc_set_threads.restype = C.c_int
         L.orc_set_threads.argtypes = [C.c_int]
         L.orc_sample_cell.restype = C.c_double
@@ -113,7 +127,17 @@ class Oracle:
         """numpy view (nz, ny, nx) of an oracle field; writes go through."""
         n = lib().orc_field_len(self.h, field, level)
         ptr = lib().orc_field(self.h, field, level)
-        shape = self.level_shape(level) if field in (F_MGX, F_MGB, F_MGR) else self.shape
+        if not ptr:
+            raise ValueError("oracle: field not available")
+        if field in (F_MGX, F_MGB, F_MGR):
+            shape = self.level_shape(level)
+        elif field == F_WAVE:
+            w = int(round((n // 4) ** 0.5))
+            shape = (w, w, 4)
+        elif field == F_PTOP:
+            shape = (self.shape[0], self.shape[2])
+        else:
+            shape = self.shape
         return np.ctypeslib.as_array(ptr, shape=(n,)).reshape(shape)
 
     def get(self, field, level=0):
@@ -142,6 +166,29 @@ class Oracle:
         a, b = C.c_double(), C.c_double()
         lib().orc_div_norms(self.h, C.byref(a), C.byref(b))
         return a.value, b.value
+
+    def wave_enable(self, n=512, half_extent=1.60, half_xz=0.0, depth=0.0, surface_y=0.30,
+                    marangoni=0.0, off=False):
+        """enable (flat surface, no drops) or, off=True, disable the wave coupling"""
+        if off:
+            lib().orc_wave_enable(self.h, None)
+            return
+        self.wave_params = OrcWaveParams(n=n, half_extent=half_extent, half_xz=half_xz,
+                                         depth=depth, surface_y=surface_y, marangoni=marangoni)
+        lib().orc_wave_enable(self.h, C.byref(self.wave_params))
+
+    def set_drops(self, drops):
+        """drops: sequences (x, y, z, radius, vel_y, osc, drain, seed)"""
+        arr = (OrcDrop * max(1, len(drops)))(*[OrcDrop(*map(float, d)) for d in drops])
+        lib().orc_set_drops(self.h, arr, len(drops))
+
+
+def fft(z, sign=-1):
+    """the oracle's unitary radix-2 FFT of a complex vector (length a power of two)"""
+    z = np.asarray(z, dtype=np.complex128)
+    buf = np.ascontiguousarray(np.stack([z.real, z.imag], axis=-1).ravel())
+    lib().orc_fft(buf.ctypes.data_as(C.POINTER(C.c_double)), z.size, 1, int(sign))
+    return buf[0::2] + 1j * buf[1::2]
 
 
 def sample_cell(f, ux, uy, uz):

@@ -1,0 +1,152 @@
+"""GPU parity of the free-surface coupling against the FP64 oracle.
+
+The wave passes (src/shaders/wave.ts:56-388) compute in FP32 on the device,
+as the reference's spectra do; the oracle restates them in FP64, so every
+comparison here is at FP32 tolerance, written in each test.  The set-up
+puts two active drops (one with the capillary contact window open), a
+surfactant film with Marangoni pull, a non-flat surface and an upward MAC
+top layer under the surface, so every forcing term of the force pass and
+both terms of the lid pressure (fluid.ts:16-78) are exercised.
+"""
+import numpy as np
+import pytest
+
+import oracle
+from conftest import rel_l2
+from gpu_helpers import pair
+from mac3d.lib import Mac3dError
+
+pytestmark = pytest.mark.gpu
+
+TOLW = 2e-5  # FP32 FFTs and transcendental functions against FP64
+GEOM = dict(half_extent=0.06, half_xz=0.016, depth=0.048, surface_y=0.0, marangoni=0.8)
+DROPS = [(0.004, -0.003, -0.002, 0.020, -0.4, 0.0, 0.55, 1.3),
+         (-0.010, 0.010, 0.006, 0.018, -0.2, 0.0, 0.20, 2.1)]
+
+
+def wave_state(n, seed=5):
+    """a smooth random surface (eta, d eta/dt, Laplacian slot, film)"""
+    rng = np.random.default_rng(seed)
+    k = np.fft.fftfreq(n)
+    filt = np.exp(-(k[:, None] ** 2 + k[None, :] ** 2) * 400.0)
+    W = np.empty((n, n, 4))
+    for c, a in enumerate((0.01, 0.05, 0.002, 0.3)):
+        f = np.fft.ifft2(np.fft.fft2(rng.standard_normal((n, n))) * filt).real
+        W[..., c] = a * f / np.abs(f).max()
+    W[..., 3] = np.abs(W[..., 3])
+    return W
+
+
+def coupled(n=64, prec=0, warm_steps=2):
+    w, s, o = pair(precision=prec, warm_steps=warm_steps)
+    s.wave_enable(n=n, **GEOM)
+    o.wave_enable(n=n, **GEOM)
+    s.set_drops(DROPS)
+    o.set_drops(DROPS)
+    W = wave_state(n)
+    s.upload("WAVE", W)
+    o.view(oracle.F_WAVE)[...] = W
+    # an upward top layer under the surface feeds the force pass
+    V = o.get(oracle.F_V)
+    V[:, -3:, :] += 0.2
+    o.set(oracle.F_V, V)
+    s.upload("V", V)
+    return w, s, o
+
+
+def _cmp_wave(s, o, tol):
+    """eta, d eta/dt and film to rel L2 <= tol; the 9-point Laplacian is a
+    difference of near-equal FP32 values, so its error is bounded by the
+    stencil's weight (2.4 (n/256)^2) times tol times the size of eta"""
+    a, b = s.download("WAVE"), o.get(oracle.F_WAVE)
+    n = a.shape[0]
+    for c in (0, 1, 3):
+        e = rel_l2(a[..., c], b[..., c])
+        assert e <= tol, f"wave channel {c}: rel L2 {e:.3e} > {tol:.1e}"
+    bound = 2.4 * (n / 256.0) ** 2 * tol * np.linalg.norm(b[..., 0])
+    e = np.linalg.norm(a[..., 2] - b[..., 2])
+    assert e <= bound, f"wave Laplacian: L2 error {e:.3e} > {bound:.3e}"
+
+
+@pytest.mark.parametrize("n", [64, 512])
+def test_wave_step_against_oracle(n):
+    """Three wave steps (force, FFT rows/columns, propagation, inverse FFT,
+    finalize) from the same surface, drops and MAC top layer."""
+    w, s, o = coupled(n)
+    with s:
+        for _ in range(3):
+            s.stage("WAVE", w.dt)
+            o.stage(oracle.ST_WAVE, w.dt)
+        _cmp_wave(s, o, TOLW)
+    o.close()
+
+
+def test_top_pressure_against_oracle():
+    """evalTopDirichletPressure on the lid: the wave's hydrostatic and
+    capillary terms and both drops' neck overpressure."""
+    w, s, o = coupled(64)
+    with s:
+        s.stage("WAVE", w.dt)
+        o.stage(oracle.ST_WAVE, w.dt)
+        s.stage("TOP_PRESSURE", w.dt)
+        o.stage(oracle.ST_TOP_PRESSURE, w.dt)
+        a, b = s.download("PTOP"), o.get(oracle.F_PTOP)
+        assert np.abs(b).max() > 0
+        assert rel_l2(a, b) <= TOLW
+    o.close()
+
+
+@pytest.mark.parametrize("prec,tol", [(0, 2e-5), (2, 2e-5)])
+def test_coupled_steps_against_oracle(prec, tol):
+    """Five full steps with the coupling on: the wave step and lid pressure
+    first, then the MAC step with p_top in its right-hand side and top-face
+    projection.  FP64 mode too: its wave is FP32 (bound TOLW-like)."""
+    w, s, o = coupled(64, prec=prec)
+    with s:
+        s.step(w.dt, 5)
+        o.step(w.dt, 5)
+        for f in ("U", "V", "W", "P", "DYE"):
+            e = rel_l2(s.download(f), o.get(getattr(oracle, "F_" + f)))
+            assert e <= tol, f"{f}: rel L2 {e:.3e} > {tol:.1e}"
+        _cmp_wave(s, o, TOLW)
+        assert rel_l2(s.download("PTOP"), o.get(oracle.F_PTOP)) <= TOLW
+    o.close()
+
+
+@pytest.mark.parametrize("prec", [0, 1])
+def test_flat_calm_surface_is_the_flat_lid(prec):
+    """With a flat surface and no drops the lid pressure is exactly 0 and the
+    coupled step equals the flat-lid step bit for bit."""
+    w, s1, o = pair(precision=prec, warm_steps=2)
+    o.close()
+    w, s2, o = pair(precision=prec, warm_steps=2)
+    with s1, s2:
+        s2.wave_enable(n=32, **GEOM)
+        s1.step(w.dt, 3)
+        s2.step(w.dt, 3)
+        assert np.all(s2.download("PTOP") == 0.0)
+        for f in ("U", "V", "W", "P", "DYE"):
+            assert np.array_equal(s1.download(f), s2.download(f)), f
+    o.close()
+
+
+def test_wave_calls_need_the_coupling_and_valid_arguments():
+    """ESTATE before mac3d_wave_enable; EINVAL for a grid that is not a
+    power of two in [16, 1024] or more than 8 drops; disabling restores the
+    flat lid (the wave fields are gone)."""
+    w, s, o = pair(warm_steps=0)
+    o.close()
+    with s:
+        with pytest.raises(Mac3dError, match="ESTATE"):
+            s.stage("WAVE", w.dt)
+        with pytest.raises(Mac3dError, match="ESTATE"):
+            s.set_drops(DROPS)
+        for n in (8, 100, 2048):
+            with pytest.raises(Mac3dError, match="EINVAL"):
+                s.wave_enable(n=n)
+        s.wave_enable(n=16)
+        with pytest.raises(Mac3dError, match="EINVAL"):
+            s.set_drops(DROPS * 5)
+        s.wave_enable(False)
+        with pytest.raises(Mac3dError, match="EINVAL"):
+            s.download("PTOP")

@@ -1,0 +1,205 @@
+"""Pins of the oracle's free-surface wave coupling (CPU only).
+
+The oracle restates the reference's wave passes (src/shaders/wave.ts:56-388)
+and evalTopDirichletPressure (src/shaders/fluid.ts:16-78).  The reference
+holds no numeric test of them, so each part is pinned here against what the
+mathematics fixes: the FFT against numpy's; the whole wave step, on a domain
+small enough that the radial absorber and border fade never act, against
+the closed-form evolution of one Fourier mode under the gravity-capillary
+dispersion relation; the drop forcing against the zero integral of its polar
+Ricker wavelet and the closed form of its film deposit; the lid pressure
+against its hydrostatic closed form and the zero lateral dipole of its
+azimuthal modulation; the divergence and projection against p_top moved
+between the right-hand side and the ghost.
+"""
+import math
+
+import numpy as np
+import pytest
+
+import oracle
+
+
+def make(nx=16, ny=16, nz=16, levels=2, dx=1e-3, **kw):
+    return oracle.Oracle(nx, ny, nz, dx, levels=levels, **kw)
+
+
+# ---------------------------------------------------------------- FFT
+@pytest.mark.parametrize("n", [16, 64, 512])
+def test_fft_is_numpys_unitary_fft(n):
+    """orc_fft = unitary DFT (1/sqrt(n) per transform, wave.ts:243-247) with
+    sign -1 forward and +1 inverse; a wrong twiddle sign, a bit-reversal
+    slip or a lost scale fails it."""
+    rng = np.random.default_rng(n)
+    z = rng.standard_normal(n) + 1j * rng.standard_normal(n)
+    f = oracle.fft(z, -1)
+    assert np.allclose(f, np.fft.fft(z) / math.sqrt(n), atol=1e-12 * n)
+    assert np.allclose(oracle.fft(z, +1), np.fft.ifft(z) * math.sqrt(n), atol=1e-12 * n)
+    assert np.allclose(oracle.fft(f, +1), z, atol=1e-12 * n)
+
+
+# ---------------------------------------------------------------- propagation
+def omega_of(k, depth):
+    """gravity-capillary dispersion with the reference's coefficients
+    (g_eff = 0.84, sigma/rho = 0.0024, wave.ts:296-298)"""
+    return math.sqrt(max((0.84 * k + 0.0024 * k ** 3) * math.tanh(min(k * depth, 12.0)), 1e-8))
+
+
+@pytest.mark.parametrize("m,eta0,w0", [(3, 0.01, 0.0), (5, 0.0, 0.02), (11, 0.004, -0.01)])
+def test_wave_step_evolves_one_mode_in_closed_form(m, eta0, w0):
+    """On a 1 m domain (every texel within 0.71 m of the centre, so the
+    absorber (r > 0.90) and border fade (r > 1.22) are 1) with no drops, film
+    or Marangoni, one step maps eta = a cos(kx), w = b cos(kx) to
+    eta' = D (a cos wdt + b sin wdt / w) cos(kx), w' = D (-a w sin wdt + b cos wdt) cos(kx),
+    D = exp(-(0.36 + 0.00018 k^2) dt) (below the high-k roll-off): the FFT
+    scale, the Hermitian unpacking and repacking, the k = 2 pi m / L map and
+    the phase rotation all enter."""
+    n, ext, depth, dt = 64, 0.5, 1.28, 0.01
+    o = make()
+    o.wave_enable(n=n, half_extent=ext, depth=depth)
+    L = 2 * ext
+    x = (np.arange(n) + 0.5) / n * L - ext
+    k = 2 * math.pi * m / L
+    mode = np.broadcast_to(np.cos(k * (x + ext)), (n, n))  # varies along x (the fast axis)
+    W = o.view(oracle.F_WAVE)
+    W[..., 0] = eta0 * mode
+    W[..., 1] = w0 * mode
+    o.stage(oracle.ST_WAVE, dt)
+    om = omega_of(k, depth)
+    D = math.exp(-(0.36 + 0.00018 * k * k) * dt)
+    eta1 = D * (eta0 * math.cos(om * dt) + w0 * math.sin(om * dt) / om) * mode
+    w1 = D * (-eta0 * om * math.sin(om * dt) + w0 * math.cos(om * dt)) * mode
+    got = o.get(oracle.F_WAVE)
+    assert np.allclose(got[..., 0], eta1, atol=1e-12)
+    assert np.allclose(got[..., 1], w1, atol=1e-12)
+    # 9-point Laplacian of the mode in the interior: 0.6 (cos kh - 1) eta (n/256)^2
+    h = L / n
+    lap = 0.6 * (math.cos(k * h) - 1.0) * eta1 * (n / 256.0) ** 2
+    assert np.allclose(got[2:-2, 2:-2, 2], lap[2:-2, 2:-2], atol=1e-12)
+    assert np.all(got[..., 3] == 0.0)
+
+
+def test_wave_step_removes_the_mean_and_the_nyquist_mode():
+    """DC (volume conservation) and the Nyquist row/column are zeroed in the
+    spectrum (wave.ts:275-279): a constant and a checkerboard vanish."""
+    n = 32
+    o = make()
+    o.wave_enable(n=n, half_extent=0.5)
+    W = o.view(oracle.F_WAVE)
+    ix = np.arange(n)
+    W[..., 0] = 0.01 + 0.003 * (-1.0) ** ix[None, :]
+    o.stage(oracle.ST_WAVE, 0.01)
+    assert np.abs(o.get(oracle.F_WAVE)[..., 0]).max() < 1e-15
+
+
+# ---------------------------------------------------------------- forcing
+def test_drop_kick_and_film_in_closed_form():
+    """One drop with emergence -0.5 radii (crater gate open, capillary
+    window shut) on a flat surface: the force pass kicks d eta/dt by
+    -dt (6.8 bell (r/0.056)(0.72 + 2.65 s)) exp(-q^2)(1 - q^2), a polar Ricker
+    wavelet of zero plane integral (wave.ts:102-125), and deposits the film
+    0.65 exp(-r^2/rs^2) clamp(0.42 drain), decayed by 0.88^(dt/0.0167)
+    (wave.ts:127-143).  The step's result is that kick carried by the
+    spectral propagation, written with numpy's FFT over the two real fields
+    (the packed form of the reference is pinned by the one-mode test)."""
+    n, ext, dt = 128, 0.4, 0.004
+    o = make()
+    o.wave_enable(n=n, half_extent=ext, depth=1.28)
+    rd, drain, vy = 0.04, 0.5, -0.3
+    e = -0.5
+    o.set_drops([(0.0, 0.30 + e * rd, 0.0, rd, vy, 0.0, drain, 1.0)])
+    o.stage(oracle.ST_WAVE, dt)
+    W = o.get(oracle.F_WAVE)
+    x = (np.arange(n) + 0.5) / n * 2 * ext - ext
+    X, Z = np.meshgrid(x, x)
+    r2 = X ** 2 + Z ** 2
+    rs = max(0.018, 0.6 * rd)
+    film = 0.65 * np.exp(-r2 / rs ** 2) * min(max(0.42 * drain, 0.0), 1.0) * 0.88 ** (dt / 0.0166667)
+    assert np.allclose(W[..., 3], film, atol=1e-13)
+
+    excav = oracle.smoothstep(-0.78, -0.04, e)
+    bell = math.sin(min(max((1.05 - e) / 1.83, 0.0), 1.0) * 3.14159265) * excav
+    speed = min(max(-vy, 0.06), 0.65)
+    q2 = r2 / min(max(rd * 1.05, 0.024), 0.082) ** 2
+    shape = np.exp(-q2) * (1 - q2)
+    assert abs(shape.sum()) < 1e-6 * np.abs(shape).sum()  # zero-integral wavelet
+    kick = -dt * (6.8 * bell * (rd / 0.056)) * (0.72 + 2.65 * speed) * shape
+    L = 2 * ext
+    m = np.fft.fftfreq(n, 1.0 / n)
+    KX, KZ = np.meshgrid(2 * np.pi * m / L, 2 * np.pi * m / L)
+    k = np.sqrt(KX ** 2 + KZ ** 2)
+    om = np.sqrt(np.maximum((0.84 * k + 0.0024 * k ** 3) * np.tanh(np.minimum(k * 1.28, 12.0)), 1e-8))
+    D = np.exp(-(0.36 + 0.00018 * k ** 2) * dt) * (1.0 - np.vectorize(oracle.smoothstep)(335.0, 465.0, k))
+    keep = np.ones((n, n))
+    keep[0, 0] = 0.0
+    keep[n // 2, :] = keep[:, n // 2] = 0.0
+    V = np.fft.fft2(kick)
+    eta1 = np.fft.ifft2(keep * D * np.sin(om * dt) / om * V).real
+    w1 = np.fft.ifft2(keep * D * np.cos(om * dt) * V).real
+    assert np.abs(eta1).max() > 1e-6
+    assert np.abs(W[..., 0] - eta1).max() <= 1e-10 * np.abs(eta1).max()
+    assert np.abs(W[..., 1] - w1).max() <= 1e-10 * np.abs(w1).max()
+
+
+# ---------------------------------------------------------------- lid pressure
+def test_top_pressure_hydrostatic_closed_form():
+    """A uniform surface eta = c (Laplacian 0), no drops: p_top = 2.45 *
+    0.155 c dt/h inside the wall taper, where h = 2 half_xz / nx
+    (fluid.ts:36-47, 73-76); the taper rises over 0.72..0.96 half_xz."""
+    nx = nz = 16
+    o = make(nx=nx, nz=nz)
+    o.wave_enable(n=32, half_xz=0.48, depth=1.28)
+    W = o.view(oracle.F_WAVE)
+    W[..., 0] = 0.02
+    dt = 0.01
+    o.stage(oracle.ST_TOP_PRESSURE, dt)
+    p = o.get(oracle.F_PTOP)
+    h = 2 * 0.48 / nx
+    want = 2.45 * 0.155 * 0.02 * dt / h
+    c = (np.arange(nx) + 0.5) / nx * 0.96 - 0.48
+    inner = np.abs(c) < 0.72 * 0.48
+    assert np.allclose(p[np.ix_(inner, inner)], want, rtol=1e-13)
+    assert np.all(p[0, :] < want) and np.all(p[:, 0] < want)  # tapered at the walls
+
+
+def test_top_pressure_drop_term_has_no_lateral_dipole():
+    """The drop's neck overpressure carries the m = 4, 6, 8 azimuthal
+    modulation (fluid.ts:57-63), which has no m = 1 part: the pressure field
+    around a drop at the tank centre has zero first moment (no lateral push)."""
+    nx = nz = 64
+    o = make(nx=nx, nz=nz, ny=16)
+    o.wave_enable(n=32, half_xz=0.16, depth=1.0)
+    o.set_drops([(0.0, 0.25, 0.0, 0.03, -0.3, 0.0, 0.6, 0.77)])
+    o.stage(oracle.ST_TOP_PRESSURE, 0.01)
+    p = o.get(oracle.F_PTOP)
+    c = (np.arange(nx) + 0.5) / nx * 0.32 - 0.16
+    X, Z = np.meshgrid(c, c)
+    assert p.max() > 0
+    mx = (p * X).sum() / np.abs(p).sum()
+    mz = (p * Z).sum() / np.abs(p).sum()
+    assert abs(mx) < 2e-4 and abs(mz) < 2e-4
+
+
+def test_divergence_and_projection_carry_the_lid_pressure():
+    """p_top moves to the top row's right-hand side (fluid.ts:548-551) and is
+    the ghost of the top face's gradient (fluid.ts:896-898): with p = 0,
+    D(u) on the top row drops by p_top and V on the top face drops by p_top."""
+    o = make(nx=16, ny=16, nz=16)
+    rng = np.random.default_rng(3)
+    for f in (oracle.F_U, oracle.F_V, oracle.F_W):
+        o.set(f, rng.standard_normal(o.shape) * 0.01)
+    o.stage(oracle.ST_DIV)
+    d0 = o.get(oracle.F_RHS)
+    o.wave_enable(n=16, half_xz=0.008, depth=0.016)
+    pt = rng.standard_normal((16, 16)) * 1e-3
+    o.view(oracle.F_PTOP)[...] = pt
+    o.stage(oracle.ST_DIV)
+    d1 = o.get(oracle.F_RHS)
+    assert np.array_equal(d1[:, :-1, :], d0[:, :-1, :])
+    assert np.allclose(d1[:, -1, :], d0[:, -1, :] - pt, atol=1e-15)
+    v0 = o.get(oracle.F_V)
+    o.set(oracle.F_P, np.zeros(o.shape))
+    o.stage(oracle.ST_PROJECT)
+    v1 = o.get(oracle.F_V)
+    assert np.allclose(v1[:, -1, :], v0[:, -1, :] - pt, atol=1e-15)
+    assert np.array_equal(v1[:, :-1, :], v0[:, :-1, :])
