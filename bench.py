@@ -7,7 +7,9 @@ sweeps and 8 cycles per step, plain FP32.  `value` is FP32 steps/s with all
 fields resident in HBM.  The same line carries the mixed-precision run
 (FP64 refinement to ||r||/||b|| <= 1e-8, FP16 level>=2 storage) and its
 speed-up, the smoother's roofline figures (HIP events on the solver's
-stream) and the FP64 CPU oracle timed on this host as the CPU baseline.
+stream), the FP32 step with the free-surface wave coupling on (a 512 x 512
+wave grid, two active drops) and the FP64 CPU oracle timed on this host as
+the CPU baseline.
 
 python bench.py [--gpus 1] [--steps K] [--warmup W]
 """
@@ -30,18 +32,33 @@ HBM_PEAK_GBPS = 8000.0
 SWEEP_BYTES_PER_CELL = 24.0
 SWEEPS_PER_LAUNCH = 2
 KERNEL_BYTES_PER_CELL = 12.0
+# algorithmic bytes per wave texel of one wave step: force pass (read the
+# state 16, write eta + i deta/dt 8 and the film 4), four FFT passes (8 in,
+# 8 out each), propagation (8 in, 8 out), finalize (8 + 4 in, 16 out)
+WAVE_BYTES_PER_TEXEL = 28.0 + 4 * 16.0 + 16.0 + 28.0
+WAVE_N = 512
+# two drops entering the surface at the tank centre (x, y, z, radius, vel_y,
+# osc, drain, seed; metres and m/s, surface at y = 0.30)
+WAVE_DROPS = [(0.006, 0.295, -0.004, 0.020, -0.4, 0.0, 0.55, 1.3),
+              (-0.012, 0.305, 0.010, 0.018, -0.2, 0.0, 0.20, 2.1)]
 
 
 def log(msg):
     print(msg, file=sys.stderr, flush=True)
 
 
-def run_gpu(workload, steps, warmup, profile_steps=3):
+PROFILE_STEPS = 3  # steps timed kernel by kernel (HIP events) after the timed run
+
+
+def run_gpu(workload, steps, warmup, profile_steps=PROFILE_STEPS, wave=False):
     from mac3d import MacSolver
     from mac3d.lib import config_for
     cfg = config_for(workload)
     with MacSolver(cfg) as s:
         s.init_blobs(workload.blobs)
+        if wave:
+            s.wave_enable(n=WAVE_N, marangoni=0.8)
+            s.set_drops(WAVE_DROPS)
         if warmup:
             s.step_nostats(workload.dt, warmup)
         t0 = time.perf_counter()
@@ -95,6 +112,7 @@ def main():
     ap.add_argument("--seed", type=int, default=1)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-mixed", action="store_true")
+    ap.add_argument("--no-wave", action="store_true")
     ap.add_argument("--copy-probe", action="store_true",
                     help="after the run, 5 device copies of one FP32 field (a plain copy "
                          "kernel of known size, to calibrate HBM counters)")
@@ -124,6 +142,21 @@ def main():
                  "max_div": stm["max_div"], "rel_res": stm["rel_res"],
                  "converged": bool(stm["converged"])}
         log(f"[bench] mixed {spsm:.2f} steps/s, {mixed['vcycles_per_step']} cycles/step")
+
+    wave = None
+    if not args.no_wave:
+        log(f"[bench] fp32 with the wave coupling ({WAVE_N}^2 wave grid)")
+        stw, wallw, profw = run_gpu(w32, args.steps, args.warmup, wave=True)
+        spsw = args.steps / wallw
+        ms, n = profw.get("WAVE", (0.0, 0))
+        wave_us = 1e3 * ms / PROFILE_STEPS if n else None
+        wbytes = WAVE_BYTES_PER_TEXEL * WAVE_N * WAVE_N
+        wave = {"steps_per_s": round(spsw, 3), "n": WAVE_N,
+                "wave_us_per_step": round(wave_us, 2) if wave_us else None,
+                "wave_bytes_per_step": wbytes,
+                "wave_gbps": round(wbytes / (wave_us * 1e-6) / 1e9, 1) if wave_us else None,
+                "max_div": stw["max_div"]}
+        log(f"[bench] wave-coupled fp32 {spsw:.2f} steps/s, wave step {wave['wave_us_per_step']} us")
 
     roof = None
     if "SMOOTH2_L0" in prof:
@@ -175,6 +208,7 @@ def main():
                    "dx": w32.dx, "dt": w32.dt, "seed": args.seed, "parallelism": "single GPU"},
         "max_div": st32["max_div"],
         "mixed": mixed,
+        "wave": wave,
         "roofline": roof,
         "kernel_avg_ms": kernel_ms,
         "cpu_baseline": cpu,

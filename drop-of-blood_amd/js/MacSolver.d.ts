@@ -9,12 +9,20 @@ export interface MacConfig {
   maccormack?: number; ir_tol?: number; extrapolate?: -1 | 0 | 1 | 2; sor_omega?: number;
 }
 export interface Blob { cx: number; cy: number; cz: number; radius: number; w0: number; }
+export interface WaveParams {
+  n?: number; halfExtent?: number; halfXZ?: number; depth?: number;
+  surfaceY?: number; marangoni?: number;
+}
+export interface Drop {
+  x: number; y: number; z: number; radius: number; velY: number; osc?: number;
+  drain: number; seed: number;
+}
 export interface StepStats {
   steps: number; vcycles: number; irIters: number; converged: number;
   maxDiv: number; l2Div: number; maxDivRate: number; relRes: number; ms: number;
 }
 export declare const FIELD: Readonly<{ U: 0; V: 1; W: 2; P: 3; DYE: 4; CU: 5; CV: 6; CW: 7;
-  RHS: 8; MGX: 9; MGB: 10; MGR: 11; DYE_PRED: 12 }>;
+  RHS: 8; MGX: 9; MGB: 10; MGR: 11; DYE_PRED: 12; WAVE: 13; PTOP: 14 }>;
 export declare const PRECISION: Readonly<{ fp32: 0; mixed: 1; fp64: 2 }>;
 export declare function version(): string;
 export declare class MacSolver {
@@ -25,6 +33,8 @@ export declare class MacSolver {
   readonly p: Float32Array; readonly dye: Float32Array;
   destroyed: boolean;
   initBlobs(blobs: Blob[]): void;
+  waveEnable(params?: WaveParams | null): void;
+  setDrops(drops: Drop[]): void;
   step(dt: number, n?: number): StepStats;
   sync(): this;
   write(name: keyof typeof FIELD, data: Float32Array | Float64Array): void;

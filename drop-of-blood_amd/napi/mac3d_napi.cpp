@@ -187,6 +187,64 @@ napi_value InitBlobs(napi_env env, napi_callback_info info) {
   return nullptr;
 }
 
+// waveEnable(h, params | null): mac3d_wave_enable; missing fields keep the
+// defaults of mac3d_wave_default_params
+napi_value WaveEnable(napi_env env, napi_callback_info info) {
+  napi_value argv[2];
+  if (!get_args(env, info, 2, argv)) return nullptr;
+  void *ctx = get_handle(env, argv[0]);
+  if (!ctx) return nullptr;
+  napi_valuetype t;
+  NAPI_CALL(env, napi_typeof(env, argv[1], &t));
+  int rc;
+  if (t == napi_null || t == napi_undefined) {
+    rc = mac3d_wave_enable(ctx, nullptr);
+  } else {
+    mac3d_wave_params p;
+    mac3d_wave_default_params(&p);
+    double n = p.n;
+    prop_num(env, argv[1], "n", &n);
+    p.n = (int32_t)n;
+    prop_num(env, argv[1], "halfExtent", &p.half_extent);
+    prop_num(env, argv[1], "halfXZ", &p.half_xz);
+    prop_num(env, argv[1], "depth", &p.depth);
+    prop_num(env, argv[1], "surfaceY", &p.surface_y);
+    prop_num(env, argv[1], "marangoni", &p.marangoni);
+    rc = mac3d_wave_enable(ctx, &p);
+  }
+  if (rc) return throw_status(env, ctx, rc);
+  return nullptr;
+}
+
+// setDrops(h, [{x, y, z, radius, velY, osc, drain, seed}, ...])
+napi_value SetDrops(napi_env env, napi_callback_info info) {
+  napi_value argv[2];
+  if (!get_args(env, info, 2, argv)) return nullptr;
+  void *ctx = get_handle(env, argv[0]);
+  if (!ctx) return nullptr;
+  uint32_t n = 0;
+  NAPI_CALL(env, napi_get_array_length(env, argv[1], &n));
+  std::vector<mac3d_drop> drops(n ? n : 1);
+  for (uint32_t q = 0; q < n; ++q) {
+    napi_value o;
+    NAPI_CALL(env, napi_get_element(env, argv[1], q, &o));
+    mac3d_drop &d = drops[q];
+    d.x = d.y = d.z = d.radius = d.vel_y = d.drain = d.seed = NAN;
+    d.osc = 0.0;
+    prop_num(env, o, "x", &d.x);
+    prop_num(env, o, "y", &d.y);
+    prop_num(env, o, "z", &d.z);
+    prop_num(env, o, "radius", &d.radius);
+    prop_num(env, o, "velY", &d.vel_y);
+    prop_num(env, o, "osc", &d.osc);
+    prop_num(env, o, "drain", &d.drain);
+    prop_num(env, o, "seed", &d.seed);
+  }
+  int rc = mac3d_set_drops(ctx, drops.data(), (int)n);
+  if (rc) return throw_status(env, ctx, rc);
+  return nullptr;
+}
+
 napi_value Step(napi_env env, napi_callback_info info) {
   napi_value argv[3];
   if (!get_args(env, info, 3, argv)) return nullptr;
@@ -306,6 +364,8 @@ napi_value Init(napi_env env, napi_value exports) {
       {"upload", nullptr, Upload, nullptr, nullptr, nullptr, napi_default, nullptr},
       {"download", nullptr, Download, nullptr, nullptr, nullptr, napi_default, nullptr},
       {"initBlobs", nullptr, InitBlobs, nullptr, nullptr, nullptr, napi_default, nullptr},
+      {"waveEnable", nullptr, WaveEnable, nullptr, nullptr, nullptr, napi_default, nullptr},
+      {"setDrops", nullptr, SetDrops, nullptr, nullptr, nullptr, napi_default, nullptr},
       {"step", nullptr, Step, nullptr, nullptr, nullptr, napi_default, nullptr},
       {"stage", nullptr, Stage, nullptr, nullptr, nullptr, napi_default, nullptr},
       {"divNorms", nullptr, DivNorms, nullptr, nullptr, nullptr, napi_default, nullptr},

@@ -26,6 +26,25 @@ console.log(JSON.stringify(out));
 // adjoint through the TypeScript host: gradient of the tiny run's dye
 // against a zero target must be finite and match the C ABI's (checked by the
 // pytest driver)
+// the free-surface coupling through the same host: checksums of the fields
+// and the wave state after the steps (compared with the C ABI's by pytest)
+if (process.argv[3] === 'wave') {
+  const s3 = new MacSolver({ nx: w.nx, ny: w.ny, nz: w.nz, dx: w.dx, levels: w.levels,
+    cycles: w.cycles, precision: 0 });
+  s3.initBlobs(w.blobs);
+  s3.waveEnable({ n: 64, halfExtent: 0.06, halfXZ: 0.016, depth: 0.048, surfaceY: 0.0,
+    marangoni: 0.8 });
+  s3.setDrops([{ x: 0.004, y: -0.003, z: -0.002, radius: 0.02, velY: -0.4, drain: 0.55, seed: 1.3 },
+    { x: -0.01, y: 0.01, z: 0.006, radius: 0.018, velY: -0.2, drain: 0.2, seed: 2.1 }]);
+  s3.step(w.dt, steps);
+  s3.sync();
+  const wave = s3.read('WAVE', new Float64Array(64 * 64 * 4));
+  const ptop = s3.read('PTOP', new Float64Array(w.nx * w.nz));
+  console.log(JSON.stringify({ wave: { v: sum(s3.v), p: sum(s3.p), dye: sum(s3.dye),
+    wave: sum(wave), ptop: sum(ptop) } }));
+  s3.destroy();
+}
+
 if (process.argv[3] === 'backward') {
   const s2 = new MacSolver({ nx: w.nx, ny: w.ny, nz: w.nz, dx: w.dx, levels: w.levels,
     cycles: w.cycles, precision: 2 });

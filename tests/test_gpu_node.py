@@ -50,3 +50,28 @@ def test_node_backward_matches_c_abi():
     for p, n in enumerate(["cx", "cy", "cz", "radius", "w0"]):
         assert abs(js["dBlobs"][0][n] - g["d_blob"][0][p]) <= 1e-9 * abs(g["d_blob"][0][p]) + 1e-12
     assert abs(js["dRhoB"] - g["d_rho_b"]) <= 1e-9 * abs(g["d_rho_b"]) + 1e-15
+
+
+def test_node_wave_coupling_matches_c_abi():
+    """waveEnable / setDrops through the N-API host give the C ABI's fields
+    and wave state after the same steps."""
+    from mac3d import MacSolver
+    from mac3d.lib import config_for
+    steps = 3
+    r = subprocess.run(["node", os.path.join(ROOT, "tests", "js", "step_tiny.js"), str(steps),
+                        "wave"], capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stderr
+    js = json.loads(r.stdout.strip().splitlines()[-1])["wave"]
+    w = scenarios.workload("tiny", 1)
+    with MacSolver(config_for(w)) as s:
+        s.init_blobs(w.blobs)
+        s.wave_enable(n=64, half_extent=0.06, half_xz=0.016, depth=0.048, surface_y=0.0,
+                      marangoni=0.8)
+        s.set_drops([(0.004, -0.003, -0.002, 0.02, -0.4, 0.0, 0.55, 1.3),
+                     (-0.01, 0.01, 0.006, 0.018, -0.2, 0.0, 0.2, 2.1)])
+        s.step(w.dt, steps)
+        got = {"v": s.download("V"), "p": s.download("P"), "dye": s.download("DYE"),
+               "wave": s.download("WAVE"), "ptop": s.download("PTOP")}
+    assert js["wave"] > 0 and js["ptop"] > 0
+    for k, a in got.items():
+        assert js[k] == pytest.approx(float(np.sqrt(np.sum(a * a))), rel=1e-12), k
