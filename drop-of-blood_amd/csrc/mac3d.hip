@@ -127,11 +127,11 @@ struct Ctx {
   double *red = nullptr;     // device fold output (3 doubles)
   double *red_host = nullptr; // pinned
   // free-surface coupling (off until mac3d_wave_enable): wave state (4 per
-  // texel), complex FFT work (2 per texel), the force pass's film, the lid
+  // texel), the complex spectrum (2 per texel), the force pass's film, the lid
   // pressure (nx * nz), the drop slots and the FFT twiddles on the device
   bool wave_on = false;
   WaveParams wp{};
-  Buf wave, wz[2], wfilm, ptop;
+  Buf wave, wz, wfilm, ptop;
   WaveDrop *drops = nullptr;
   float2 *wtw = nullptr;
   // staging for upload/download (stage_n doubles)
@@ -343,9 +343,10 @@ WaveParams wave_params_dt(const Ctx *ctx, double dt) {
   return p;
 }
 
-// One wave step (BloodDropEngine.ts:1886-1914): force pass, forward FFT of
-// the rows then the columns, spectral propagation, inverse FFT of the
-// columns then the rows, finalize into the wave state.
+// One wave step (BloodDropEngine.ts:1886-1914): the force pass with the
+// forward FFT of each row, the forward column FFT with the spectral
+// propagation and the inverse column FFT, the inverse FFT of each row, and
+// the finalize into the wave state (the reference's seven passes in four).
 int launch_wave_step(Ctx *ctx, double dt) {
   const WaveParams P = wave_params_dt(ctx, dt);
   const int n = P.n;
@@ -353,25 +354,22 @@ int launch_wave_step(Ctx *ctx, double dt) {
   while ((1 << logn) < n) ++logn;
   const dim3 g2((n + 15) / 16, (n + 15) / 16), b2(16, 16);
   ProfScope ps(ctx->prof, ctx->stream, KC_WAVE);
-  float2 *z0 = (float2 *)ctx->wz[0].ptr, *z1 = (float2 *)ctx->wz[1].ptr;
+  float2 *z = (float2 *)ctx->wz.ptr;
+  const float2 *tw = (const float2 *)ctx->wtw;
+  const dim3 bf(std::max(64, std::min(512, n / 2)));
+  const size_t sh = (size_t)n * sizeof(float2), shtw = (size_t)(n / 2) * sizeof(float2);
   with_step_type(ctx->t0, [&](auto tag) {
     using T = std::remove_pointer_t<decltype(tag)>;
-    hipLaunchKernelGGL(k_wave_force<T>, g2, b2, 0, ctx->stream, P, (const WaveDrop *)ctx->drops,
-                       (const float4 *)ctx->wave.ptr, ctx->d[0], (const T *)ctx->v.ptr, z0,
-                       (float *)ctx->wfilm.ptr);
+    // (one thread per texel for the force pass; the FFT stages use half)
+    hipLaunchKernelGGL(k_wave_force_rows<T>, dim3(n), dim3(std::max(64, std::min(512, n))),
+                       sh + shtw, ctx->stream, P,
+                       (const WaveDrop *)ctx->drops, (const float4 *)ctx->wave.ptr, ctx->d[0],
+                       (const T *)ctx->v.ptr, z, (float *)ctx->wfilm.ptr, logn, tw);
   });
-  const dim3 bf(std::max(64, std::min(512, n / 2)));
-  const size_t sh = (size_t)n * sizeof(float2);
-  hipLaunchKernelGGL(k_wave_fft, dim3(n), bf, sh, ctx->stream, z0, n, logn, 1, 0,
-                     (const float2 *)ctx->wtw);
-  hipLaunchKernelGGL(k_wave_fft, dim3(n), bf, sh, ctx->stream, z0, n, logn, 0, 0,
-                     (const float2 *)ctx->wtw);
-  hipLaunchKernelGGL(k_wave_propagate, g2, b2, 0, ctx->stream, P, (const float2 *)z0, z1);
-  hipLaunchKernelGGL(k_wave_fft, dim3(n), bf, sh, ctx->stream, z1, n, logn, 0, 1,
-                     (const float2 *)ctx->wtw);
-  hipLaunchKernelGGL(k_wave_fft, dim3(n), bf, sh, ctx->stream, z1, n, logn, 1, 1,
-                     (const float2 *)ctx->wtw);
-  hipLaunchKernelGGL(k_wave_finalize, g2, b2, 0, ctx->stream, P, (const float2 *)z1,
+  hipLaunchKernelGGL(k_wave_colpair, dim3(n / 2 + 1), bf, 2 * sh + shtw, ctx->stream, P, z, logn,
+                     tw);
+  hipLaunchKernelGGL(k_wave_fft, dim3(n), bf, sh + shtw, ctx->stream, z, n, logn, 1, 1, tw);
+  hipLaunchKernelGGL(k_wave_finalize, g2, b2, 0, ctx->stream, P, (const float2 *)z,
                      (const float *)ctx->wfilm.ptr, (float4 *)ctx->wave.ptr);
   LAUNCHCHK();
   return MAC3D_OK;
@@ -1758,8 +1756,7 @@ int mac3d_wave_enable(void *h, const mac3d_wave_params *p) {
   const size_t nn = (size_t)n * n;
   if (!ctx->wave.ptr || ctx->wp.n != n) {  // (earlier buffers stay owned until destroy)
     if ((rc = alloc(ctx, ctx->wave, TF32, 4 * nn))) return rc;
-    for (int q = 0; q < 2; ++q)
-      if ((rc = alloc(ctx, ctx->wz[q], TF32, 2 * nn))) return rc;
+    if ((rc = alloc(ctx, ctx->wz, TF32, 2 * (size_t)n * wv_pitch(n)))) return rc;
     if ((rc = alloc(ctx, ctx->wfilm, TF32, nn))) return rc;
     if ((rc = dev_alloc(ctx, (void **)&ctx->wtw, (size_t)(n / 2) * sizeof(float2)))) return rc;
     std::vector<float2> tw(n / 2);

@@ -10,8 +10,10 @@
 // texel: (eta, d eta/dt, 9-point Laplacian of eta, surfactant film), FP32
 // (the reference stores it as rgba16float; the spectra are FP32 in both).
 // One wave step: force pass -> forward 2-D FFT of eta + i deta/dt -> spectral
-// propagation -> inverse 2-D FFT -> finalize.  The FFTs are unitary
-// (1/sqrt(n) per 1-D transform), as the reference's.
+// propagation -> inverse 2-D FFT -> finalize, in four launches (force + row
+// FFT; column FFT + propagation + inverse column FFT; inverse row FFT;
+// finalize).  The FFTs are unitary (1/sqrt(n) per 1-D transform), as the
+// reference's.
 #pragma once
 #include "common.hpp"
 
@@ -87,18 +89,17 @@ __device__ inline float wv_sample_mac(const T *__restrict__ f, const Dims &d, fl
   return mixv(mixv(x00, x10, fr[1]), mixv(x01, x11, fr[1]), fr[2]);
 }
 
-// WAVE_SPATIAL_FORCE (wave.ts:56-178): drop impulses (zero-integral polar
-// Ricker craters and capillary contact impulses), the surfactant film and its
-// Marangoni pull, the MAC grid's upward top-layer velocity, the radial
-// absorber.  Writes z = eta + i deta/dt for the FFT and the updated film.
+// WAVE_SPATIAL_FORCE (wave.ts:56-178) at texel (ix, iy): drop impulses
+// (zero-integral polar Ricker craters and capillary contact impulses), the
+// surfactant film and its Marangoni pull, the MAC grid's upward top-layer
+// velocity, the radial absorber.  Gives z = eta + i deta/dt for the FFT and
+// the updated film.
 template <class T>
-__global__ void __launch_bounds__(256)
-k_wave_force(WaveParams P, const WaveDrop *__restrict__ drops, const float4 *__restrict__ wave,
-             Dims d, const T *__restrict__ vmac, float2 *__restrict__ z,
-             float *__restrict__ film_out) {
+__device__ inline void wave_force_texel(const WaveParams &P, const WaveDrop *__restrict__ drops,
+                                        const float4 *__restrict__ wave, const Dims &d,
+                                        const T *__restrict__ vmac, int ix, int iy, float2 &zv,
+                                        float &film_out) {
   const int n = P.n;
-  const int ix = blockIdx.x * 16 + threadIdx.x, iy = blockIdx.y * 16 + threadIdx.y;
-  if (ix >= n || iy >= n) return;
   const float4 c = wv_load(wave, n, ix, iy);
   float eta = c.x, vel = c.y, film = c.w;
   const float dt = P.dt;
@@ -164,10 +165,79 @@ k_wave_force(WaveParams P, const WaveDrop *__restrict__ drops, const float4 *__r
   const float border = 1.f - wv_smoothstep(1.22f, 1.56f, rw);
   eta = clampv(eta * rdamp * border, -0.14f, 0.14f);
   vel = clampv(vel * rdamp * border, -1.8f, 1.8f);
-  film = film * border;
-  const size_t t = (size_t)iy * n + ix;
-  z[t] = make_float2(eta, vel);
-  film_out[t] = film;
+  zv = make_float2(eta, vel);
+  film_out = film * border;
+}
+
+// log2(n) radix-2 butterfly stages in LDS over a line of n complex values
+// held in bit-reversed order (decimation in time, as the reference's
+// createRadix2Fft1DShader, wave.ts:180-252), natural order out, unscaled;
+// tw[m] = exp(-2 pi i m / n) (in LDS: a stage waits on no memory load),
+// conjugated for the inverse.  Every thread of the (1-D) block calls it: it
+// holds a barrier per stage.
+// (sb, nullable: a second line transformed in the same stages, so the two
+// share the stages' barriers)
+__device__ inline void wv_butterfly(float2 *sm, int i0, int i1, float2 w) {
+  const float2 a = sm[i0], q = sm[i1];
+  const float2 t = make_float2(q.x * w.x - q.y * w.y, q.x * w.y + q.y * w.x);
+  sm[i0] = make_float2(a.x + t.x, a.y + t.y);
+  sm[i1] = make_float2(a.x - t.x, a.y - t.y);
+}
+__device__ inline void wv_fft_stages(float2 *sm, int n, int logn, const float2 *tw,
+                                     bool inverse, float2 *sb = nullptr) {
+  const int h = n >> 1;
+  for (int s = 0; s < logn; ++s) {
+    const int half = 1 << s;
+    for (int b = threadIdx.x; b < h; b += blockDim.x) {
+      const int k = b & (half - 1);
+      const int i0 = ((b >> s) << (s + 1)) + k, i1 = i0 + half;
+      float2 w = tw[k << (logn - 1 - s)];
+      if (inverse) w.y = -w.y;
+      wv_butterfly(sm, i0, i1, w);
+      if (sb) wv_butterfly(sb, i0, i1, w);
+    }
+    __syncthreads();
+  }
+}
+
+__device__ inline int wv_brev(int i, int logn) { return (int)(__brev((unsigned)i) >> (32 - logn)); }
+
+// Row pitch of the spectrum in complex values: n + 32, so that successive
+// rows start 256 B apart modulo 4 KB and a column's elements spread over the
+// L2 channels instead of all falling in one (a 4 KB row stride does that).
+#define WV_PAD 32
+__host__ __device__ inline size_t wv_pitch(int n) { return (size_t)n + WV_PAD; }
+
+// the n/2 twiddles into LDS (the caller's barrier publishes them)
+__device__ inline void wv_load_twiddles(float2 *tws, int n, const float2 *__restrict__ tw) {
+  for (int m = threadIdx.x; m < (n >> 1); m += blockDim.x) tws[m] = tw[m];
+}
+
+// The force pass fused with the forward FFT of its row: one block per row
+// iy evaluates the row's texels into LDS (bit-reversed), transforms it and
+// writes the row's spectrum (1/sqrt(n) scaled) and the film.
+template <class T>
+__global__ void __launch_bounds__(512)
+k_wave_force_rows(WaveParams P, const WaveDrop *__restrict__ drops,
+                  const float4 *__restrict__ wave, Dims d, const T *__restrict__ vmac,
+                  float2 *__restrict__ z, float *__restrict__ film, int logn,
+                  const float2 *__restrict__ tw) {
+  extern __shared__ float2 sm[];  // the row, then the twiddles
+  const int n = P.n, iy = blockIdx.x;
+  float2 *tws = sm + n;
+  wv_load_twiddles(tws, n, tw);
+  for (int ix = threadIdx.x; ix < n; ix += blockDim.x) {
+    float2 zv;
+    float f;
+    wave_force_texel<T>(P, drops, wave, d, vmac, ix, iy, zv, f);
+    film[(size_t)iy * n + ix] = f;
+    sm[wv_brev(ix, logn)] = zv;
+  }
+  __syncthreads();
+  wv_fft_stages(sm, n, logn, tws, false);
+  const float sc = rsqrtf((float)n);
+  for (int i = threadIdx.x; i < n; i += blockDim.x)
+    z[(size_t)iy * wv_pitch(n) + i] = make_float2(sm[i].x * sc, sm[i].y * sc);
 }
 
 // One 1-D unitary complex FFT of length n per line, radix-2 in LDS (n/2
@@ -178,29 +248,15 @@ k_wave_force(WaveParams P, const WaveDrop *__restrict__ drops, const float4 *__r
 __global__ void __launch_bounds__(512)
 k_wave_fft(float2 *__restrict__ z, int n, int logn, int rows, int inverse,
            const float2 *__restrict__ tw) {
-  extern __shared__ float2 sm[];
-  const int line = blockIdx.x, tid = threadIdx.x, h = n >> 1;
-  const size_t base = rows ? (size_t)line * n : (size_t)line;
-  const size_t step = rows ? 1 : (size_t)n;
-  for (int i = tid; i < n; i += blockDim.x) {
-    const int r = (int)(__brev((unsigned)i) >> (32 - logn));
-    sm[i] = z[base + (size_t)r * step];
-  }
+  extern __shared__ float2 sm[];  // the line, then the twiddles
+  const int line = blockIdx.x, tid = threadIdx.x;
+  const size_t base = rows ? (size_t)line * wv_pitch(n) : (size_t)line;
+  const size_t step = rows ? 1 : wv_pitch(n);
+  float2 *tws = sm + n;
+  wv_load_twiddles(tws, n, tw);
+  for (int i = tid; i < n; i += blockDim.x) sm[i] = z[base + (size_t)wv_brev(i, logn) * step];
   __syncthreads();
-  for (int s = 0; s < logn; ++s) {
-    const int half = 1 << s;
-    for (int b = tid; b < h; b += blockDim.x) {
-      const int k = b & (half - 1);
-      const int i0 = ((b >> s) << (s + 1)) + k, i1 = i0 + half;
-      float2 w = tw[k << (logn - 1 - s)];
-      if (inverse) w.y = -w.y;
-      const float2 a = sm[i0], q = sm[i1];
-      const float2 t = make_float2(q.x * w.x - q.y * w.y, q.x * w.y + q.y * w.x);
-      sm[i0] = make_float2(a.x + t.x, a.y + t.y);
-      sm[i1] = make_float2(a.x - t.x, a.y - t.y);
-    }
-    __syncthreads();
-  }
+  wv_fft_stages(sm, n, logn, tws, inverse != 0);
   const float sc = rsqrtf((float)n);
   for (int i = tid; i < n; i += blockDim.x) {
     const float2 v = sm[i];
@@ -208,23 +264,15 @@ k_wave_fft(float2 *__restrict__ z, int n, int logn, int rows, int inverse,
   }
 }
 
-// WAVE_SPECTRAL_PROPAGATE (wave.ts:254-331): Hermitian unpacking of the two
-// real fields packed as Z = eta + i w, the gravity-capillary dispersion
-// omega(k) = sqrt((g k + s k^3) tanh(k H)) over one dt with viscous damping and
-// a high-k roll-off, DC and the Nyquist row and column set to 0, repacked.
-__global__ void __launch_bounds__(256)
-k_wave_propagate(WaveParams P, const float2 *__restrict__ zin, float2 *__restrict__ zout) {
-  const int n = P.n;
-  const int ux = blockIdx.x * 16 + threadIdx.x, uy = blockIdx.y * 16 + threadIdx.y;
-  if (ux >= n || uy >= n) return;
-  const int hn = n >> 1;
-  const size_t t = (size_t)uy * n + ux;
-  if ((ux == 0 && uy == 0) || ux == hn || uy == hn) {
-    zout[t] = make_float2(0.f, 0.f);
-    return;
-  }
-  const int nxn = (n - ux) & (n - 1), nyn = (n - uy) & (n - 1);
-  const float2 fp = zin[t], fm = zin[(size_t)nyn * n + nxn];
+// WAVE_SPECTRAL_PROPAGATE (wave.ts:254-331) of spectral texel (ux, uy), fp
+// its value and fm the value at the mirror texel (-ux, -uy): Hermitian
+// unpacking of the two real fields packed as Z = eta + i w, the
+// gravity-capillary dispersion omega(k) = sqrt((g k + s k^3) tanh(k H)) over
+// one dt with viscous damping and a high-k roll-off, DC and the Nyquist row
+// and column set to 0, repacked.
+__device__ inline float2 wv_propagate(const WaveParams &P, int ux, int uy, float2 fp, float2 fm) {
+  const int n = P.n, hn = n >> 1;
+  if ((ux == 0 && uy == 0) || ux == hn || uy == hn) return make_float2(0.f, 0.f);
   const float2 eh = make_float2(0.5f * (fp.x + fm.x), 0.5f * (fp.y - fm.y));
   const float2 vh = make_float2(0.5f * (fp.y + fm.y), 0.5f * (-fp.x + fm.x));
   const float mx = (float)(ux > hn ? ux - n : ux), mz = (float)(uy > hn ? uy - n : uy);
@@ -244,7 +292,62 @@ k_wave_propagate(WaveParams P, const float2 *__restrict__ zin, float2 *__restric
                                 damp * (eh.y * cp + vh.y * (sp / omega)));
   const float2 vn = make_float2(damp * (-eh.x * (omega * sp) + vh.x * cp),
                                 damp * (-eh.y * (omega * sp) + vh.y * cp));
-  zout[t] = make_float2(en.x - vn.y, en.y + vn.x);
+  return make_float2(en.x - vn.y, en.y + vn.x);
+}
+
+// Forward column FFT, propagation and inverse column FFT in one pass: block
+// a (0 <= a <= n/2) holds column a and its mirror column n - a in LDS (the
+// propagation pairs texel (a, y) with (n - a, n - y)); columns 0 and n/2 are
+// their own mirrors.  Each thread propagates one mirror pair in place (it
+// reads both values before writing either), so no two threads touch one
+// texel.
+__global__ void __launch_bounds__(512)
+k_wave_colpair(WaveParams P, float2 *__restrict__ z, int logn, const float2 *__restrict__ tw) {
+  extern __shared__ float2 sm[];  // column a, column n - a, the twiddles
+  const int n = P.n, hn = n >> 1;
+  const int a = blockIdx.x, b = (n - a) & (n - 1);
+  const bool self = a == b;
+  float2 *ca = sm, *cb = self ? sm : sm + n, *tws = sm + 2 * n;
+  wv_load_twiddles(tws, n, tw);
+  for (int i = threadIdx.x; i < n; i += blockDim.x) {
+    const size_t r = (size_t)wv_brev(i, logn) * wv_pitch(n);
+    ca[i] = z[r + a];
+    if (!self) cb[i] = z[r + b];
+  }
+  __syncthreads();
+  wv_fft_stages(ca, n, logn, tws, false, self ? nullptr : cb);
+  const float sc = rsqrtf((float)n);
+  const int npairs = self ? hn + 1 : n;
+  for (int uy = threadIdx.x; uy < npairs; uy += blockDim.x) {
+    const int my = (n - uy) & (n - 1);
+    const float2 fp = make_float2(ca[uy].x * sc, ca[uy].y * sc);
+    const float2 fm = make_float2(cb[my].x * sc, cb[my].y * sc);
+    const float2 op = wv_propagate(P, a, uy, fp, fm);
+    const float2 om = wv_propagate(P, b, my, fm, fp);
+    ca[uy] = op;
+    cb[my] = om;
+  }
+  __syncthreads();
+  // natural -> bit-reversed order for the inverse transform (pairwise swaps)
+  for (int i = threadIdx.x; i < n; i += blockDim.x) {
+    const int r = wv_brev(i, logn);
+    if (i < r) {
+      float2 t = ca[i];
+      ca[i] = ca[r];
+      ca[r] = t;
+      if (!self) {
+        t = cb[i];
+        cb[i] = cb[r];
+        cb[r] = t;
+      }
+    }
+  }
+  __syncthreads();
+  wv_fft_stages(ca, n, logn, tws, true, self ? nullptr : cb);
+  for (int i = threadIdx.x; i < n; i += blockDim.x) {
+    z[(size_t)i * wv_pitch(n) + a] = make_float2(ca[i].x * sc, ca[i].y * sc);
+    if (!self) z[(size_t)i * wv_pitch(n) + b] = make_float2(cb[i].x * sc, cb[i].y * sc);
+  }
 }
 
 // WAVE_FINALIZE (wave.ts:333-388): propagated (eta, deta/dt), the film of the
@@ -259,10 +362,10 @@ k_wave_finalize(WaveParams P, const float2 *__restrict__ z, const float *__restr
   auto eta = [&](int x, int y) {
     x = clampi(x, 0, n - 1);
     y = clampi(y, 0, n - 1);
-    return z[(size_t)y * n + x].x;
+    return z[(size_t)y * wv_pitch(n) + x].x;
   };
   const size_t t = (size_t)iy * n + ix;
-  const float2 pr = z[t];
+  const float2 pr = z[(size_t)iy * wv_pitch(n) + ix];
   const float lap_scale = ((float)n / 256.f) * ((float)n / 256.f);
   const float lap = (0.20f * (eta(ix, iy + 1) + eta(ix, iy - 1) + eta(ix + 1, iy) + eta(ix - 1, iy)) +
                      0.05f * (eta(ix + 1, iy + 1) + eta(ix - 1, iy + 1) + eta(ix + 1, iy - 1) +
