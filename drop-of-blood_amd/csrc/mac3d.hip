@@ -1814,6 +1814,33 @@ int mac3d_set_drops(void *h, const mac3d_drop *drops, int ndrops) {
   return MAC3D_OK;
 }
 
+int mac3d_set_uniforms(void *h, const float *u, size_t count) {
+  if (!h) return MAC3D_EINVAL;
+  Ctx *ctx = (Ctx *)h;
+  if (!u || count != 128) return fail(ctx, MAC3D_EINVAL, "need the 128-float SimUniforms block");
+  if (!ctx->wave_on) return fail(ctx, MAC3D_ESTATE, "the wave coupling is not enabled");
+  for (size_t q = 40; q < 128; ++q)
+    if (!std::isfinite(u[q])) return fail(ctx, MAC3D_EINVAL, "non-finite uniform value");
+  const float half_xz = u[52], depth = u[54] - u[53];
+  if (!(half_xz > 0.f) || !(depth > 0.f))
+    return fail(ctx, MAC3D_EINVAL, "domainBounds must give half_xz > 0 and yMax > yMin");
+  mac3d_drop d[MAC3D_MAX_DROPS];
+  for (int q = 0; q < MAC3D_MAX_DROPS; ++q) {
+    const float *b = u + 64 + 8 * q;
+    d[q] = mac3d_drop{b[0], b[1], b[2], b[3], b[4], b[5], b[6], b[7]};
+  }
+  int rc = mac3d_set_drops(h, d, MAC3D_MAX_DROPS);
+  if (rc) return rc;
+  if (ctx->wp.surface_y != u[40] || ctx->wp.marangoni != u[41] || ctx->wp.half_xz != half_xz ||
+      ctx->wp.depth != depth)
+    drop_graphs(ctx);  // (the captured launches hold the parameters by value)
+  ctx->wp.surface_y = u[40];
+  ctx->wp.marangoni = u[41];
+  ctx->wp.half_xz = half_xz;
+  ctx->wp.depth = depth;
+  return MAC3D_OK;
+}
+
 int mac3d_init_blobs(void *h, const mac3d_blob *blobs, int nblobs) {
   if (!h) return MAC3D_EINVAL;
   return init_blobs_impl((Ctx *)h, blobs, nblobs);

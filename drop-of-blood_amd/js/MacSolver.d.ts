@@ -35,6 +35,7 @@ export declare class MacSolver {
   initBlobs(blobs: Blob[]): void;
   waveEnable(params?: WaveParams | null): void;
   setDrops(drops: Drop[]): void;
+  setUniforms(block: Float32Array): void;
   step(dt: number, n?: number): StepStats;
   sync(): this;
   write(name: keyof typeof FIELD, data: Float32Array | Float64Array): void;

@@ -80,6 +80,15 @@ class MacSolver {
   }
 
   /**
+   * The engine's packed SimUniforms block (SimUniformPacker.pack's
+   * Float32Array(128)): surface height, Marangoni, domain bounds, drop slots.
+   * @param {Float32Array} block
+   */
+  setUniforms(block) {
+    addon.setUniforms(this._live(), block);
+  }
+
+  /**
    * Advance n steps of fixed dt (seconds); blocks until the GPU is done.
    * @param {number} dt @param {number} [n]
    * @returns {StepStats}

@@ -81,7 +81,7 @@ EXPORTS = ("mac3d_default_config", "mac3d_create", "mac3d_destroy", "mac3d_uploa
            "mac3d_download", "mac3d_init_blobs", "mac3d_step", "mac3d_stage", "mac3d_mg_op",
            "mac3d_div_norms", "mac3d_profile", "mac3d_profile_read", "mac3d_adjoint",
            "mac3d_last_error", "mac3d_version", "mac3d_wave_default_params",
-           "mac3d_wave_enable", "mac3d_set_drops")
+           "mac3d_wave_enable", "mac3d_set_drops", "mac3d_set_uniforms")
 
 
 def load_library(path: Optional[str] = None):
@@ -111,6 +111,7 @@ def load_library(path: Optional[str] = None):
     L.mac3d_wave_default_params.argtypes = [C.POINTER(WaveParams)]
     L.mac3d_wave_enable.argtypes = [vp, C.POINTER(WaveParams)]
     L.mac3d_set_drops.argtypes = [vp, C.POINTER(Drop), C.c_int]
+    L.mac3d_set_uniforms.argtypes = [vp, vp, C.c_size_t]
     L.mac3d_last_error.argtypes = [vp]
     L.mac3d_last_error.restype = C.c_char_p
     L.mac3d_version.restype = C.c_char_p
@@ -245,6 +246,12 @@ class MacSolver:
         """drops: up to 8 sequences (x, y, z, radius, vel_y, osc, drain, seed)"""
         arr = (Drop * max(1, len(drops)))(*[Drop(*map(float, d)) for d in drops])
         self._check(self.L.mac3d_set_drops(self.h, arr, len(drops)))
+
+    def set_uniforms(self, block):
+        """the reference's packed 128-float SimUniforms block (surface height,
+        Marangoni, domain bounds and the 8 drop slots are read)"""
+        b = np.ascontiguousarray(block, dtype=np.float32)
+        self._check(self.L.mac3d_set_uniforms(self.h, b.ctypes.data, b.size))
 
     # -- stepping
     def init_blobs(self, blobs: Sequence):

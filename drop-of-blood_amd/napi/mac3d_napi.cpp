@@ -245,6 +245,26 @@ napi_value SetDrops(napi_env env, napi_callback_info info) {
   return nullptr;
 }
 
+// setUniforms(h, Float32Array(128)): the packed SimUniforms block
+napi_value SetUniforms(napi_env env, napi_callback_info info) {
+  napi_value argv[2];
+  if (!get_args(env, info, 2, argv)) return nullptr;
+  void *ctx = get_handle(env, argv[0]);
+  if (!ctx) return nullptr;
+  napi_typedarray_type tt;
+  size_t len = 0, off = 0;
+  void *data = nullptr;
+  napi_value ab;
+  NAPI_CALL(env, napi_get_typedarray_info(env, argv[1], &tt, &len, &data, &ab, &off));
+  if (tt != napi_float32_array) {
+    napi_throw_type_error(env, nullptr, "expected a Float32Array");
+    return nullptr;
+  }
+  int rc = mac3d_set_uniforms(ctx, (const float *)data, len);
+  if (rc) return throw_status(env, ctx, rc);
+  return nullptr;
+}
+
 napi_value Step(napi_env env, napi_callback_info info) {
   napi_value argv[3];
   if (!get_args(env, info, 3, argv)) return nullptr;
@@ -366,6 +386,7 @@ napi_value Init(napi_env env, napi_value exports) {
       {"initBlobs", nullptr, InitBlobs, nullptr, nullptr, nullptr, napi_default, nullptr},
       {"waveEnable", nullptr, WaveEnable, nullptr, nullptr, nullptr, napi_default, nullptr},
       {"setDrops", nullptr, SetDrops, nullptr, nullptr, nullptr, napi_default, nullptr},
+      {"setUniforms", nullptr, SetUniforms, nullptr, nullptr, nullptr, napi_default, nullptr},
       {"step", nullptr, Step, nullptr, nullptr, nullptr, napi_default, nullptr},
       {"stage", nullptr, Stage, nullptr, nullptr, nullptr, napi_default, nullptr},
       {"divNorms", nullptr, DivNorms, nullptr, nullptr, nullptr, napi_default, nullptr},

@@ -241,6 +241,13 @@ int mac3d_wave_enable(void *ctx, const mac3d_wave_params *p);
 /* replaces: the drop slots of SimUniformPacker.pack (uniformLayout.ts:161-187);
  * ndrops in [0, 8], the other slots inactive; ESTATE unless enabled */
 int mac3d_set_drops(void *ctx, const mac3d_drop *drops, int ndrops);
+/* replaces: queue.writeBuffer of the packed 512-B SimUniforms block
+ * (BloodDropEngine.ts:2602; layout common.ts:26-43, packed by
+ * uniformLayout.ts:56-190) for what the coupling reads from it: count = 128
+ * floats; surfaceParams.x/.y [40, 41] (surface height, Marangoni),
+ * domainBounds [52..55] (half_xz = .x, depth = .z - .y) and the 8 drop slots
+ * [64..127].  dt stays the step's argument.  ESTATE unless enabled. */
+int mac3d_set_uniforms(void *ctx, const float *block, size_t count);
 
 /* one stage of the step (MAC3D_STAGE_*), blocking */
 int mac3d_stage(void *ctx, int stage, double dt);

@@ -150,3 +150,36 @@ def test_wave_calls_need_the_coupling_and_valid_arguments():
         s.wave_enable(False)
         with pytest.raises(Mac3dError, match="EINVAL"):
             s.download("PTOP")
+
+
+def test_packed_uniform_block_sets_the_coupling():
+    """mac3d_set_uniforms reads surfaceParams, domainBounds and the 8 drop
+    slots from the reference's packed 128-float block (common.ts:26-43,
+    uniformLayout.ts:56-190) and gives the same wave step as the same values
+    set one by one."""
+    w, s1, o = pair(warm_steps=0)
+    o.close()
+    w, s2, o = pair(warm_steps=0)
+    o.close()
+    blk = np.zeros(128, dtype=np.float32)
+    blk[40], blk[41] = 0.0, 0.8                       # surfaceY, marangoni
+    blk[52], blk[53], blk[54], blk[55] = 0.016, -0.032, 0.016, 1.1  # domainBounds
+    for q, d in enumerate(DROPS):
+        blk[64 + 8 * q: 72 + 8 * q] = d
+    depth = float(np.float32(blk[54]) - np.float32(blk[53]))
+    with s1, s2:
+        s1.wave_enable(n=64, half_extent=0.06)
+        s1.set_uniforms(blk)
+        s2.wave_enable(n=64, half_extent=0.06, half_xz=float(blk[52]), depth=depth,
+                       surface_y=0.0, marangoni=float(blk[41]))
+        s2.set_drops([tuple(float(x) for x in np.float32(d)) for d in DROPS])
+        W = wave_state(64)
+        s1.upload("WAVE", W)
+        s2.upload("WAVE", W)
+        for s in (s1, s2):
+            s.stage("WAVE", w.dt)
+            s.stage("TOP_PRESSURE", w.dt)
+        assert np.array_equal(s1.download("WAVE"), s2.download("WAVE"))
+        assert np.array_equal(s1.download("PTOP"), s2.download("PTOP"))
+        with pytest.raises(Mac3dError, match="EINVAL"):
+            s1.set_uniforms(blk[:64])
