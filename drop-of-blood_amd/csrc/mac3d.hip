@@ -130,6 +130,7 @@ struct Ctx {
   // texel), the complex spectrum (2 per texel), the force pass's film, the lid
   // pressure (nx * nz), the drop slots and the FFT twiddles on the device
   bool wave_on = false;
+  bool wave_scene = false;  // also the advection's scene terms
   WaveParams wp{};
   Buf wave, wz, wfilm, ptop;
   WaveDrop *drops = nullptr;
@@ -289,18 +290,28 @@ template <class T> bool gather_buf(const Dims &d) {
          3 * (size_t)d.nx * d.ny * sizeof(T) < ((size_t)1 << 24);
 }
 
+// the wave coupling's parameters with the step's dt
+WaveParams wave_params_dt(const Ctx *ctx, double dt) {
+  WaveParams p = ctx->wp;
+  p.dt = (float)std::min(std::max(dt, 0.0), 0.025);  // viewportAndDt.z clamp (wave.ts:83)
+  return p;
+}
+
 int launch_advect(Ctx *ctx, double dt) {
   const Dims &d = ctx->d[0];
   {
     ProfScope ps(ctx->prof, ctx->stream, KC_ADVECT);
     with_step_type(ctx->t0, [&](auto tag) {
       using T = std::remove_pointer_t<decltype(tag)>;
-      auto kern = gather_buf<T>(d) ? k_advect_mac<T, true> : k_advect_mac<T, false>;
+      const bool buf = gather_buf<T>(d), scene = ctx->wave_on && ctx->wave_scene;
+      auto kern = scene ? (buf ? k_advect_mac<T, true, true> : k_advect_mac<T, false, true>)
+                        : (buf ? k_advect_mac<T, true, false> : k_advect_mac<T, false, false>);
       hipLaunchKernelGGL(kern, gather_grid(d), kGatherBlock, 0, ctx->stream, d,
                          (T)(dt / ctx->cfg.dx), (T)ctx->cfg.vel_damp, (const T *)ctx->u.ptr, (const T *)ctx->v.ptr,
                          (const T *)ctx->w.ptr, (const typename cvec_of<T>::type *)ctx->cvel.ptr,
                          (T *)ctx->u2.ptr, (T *)ctx->v2.ptr,
-                         (T *)ctx->w2.ptr);
+                         (T *)ctx->w2.ptr, wave_params_dt(ctx, dt), (const WaveDrop *)ctx->drops,
+                         (const float4 *)ctx->wave.ptr);
     });
   }
   LAUNCHCHK();
@@ -335,12 +346,6 @@ int launch_casson(Ctx *ctx, double dt) {
 // coupling's, or none (the flat lid, p_top = 0)
 const float *lid_pressure(const Ctx *ctx) {
   return ctx->wave_on ? (const float *)ctx->ptop.ptr : nullptr;
-}
-
-WaveParams wave_params_dt(const Ctx *ctx, double dt) {
-  WaveParams p = ctx->wp;
-  p.dt = (float)std::min(std::max(dt, 0.0), 0.025);  // viewportAndDt.z clamp (wave.ts:83)
-  return p;
 }
 
 // One wave step (BloodDropEngine.ts:1886-1914): the force pass with the
@@ -1788,6 +1793,7 @@ int mac3d_wave_enable(void *h, const mac3d_wave_params *p) {
   ctx->wp.surface_y = (float)p->surface_y;
   ctx->wp.marangoni = (float)p->marangoni;
   ctx->wp.dt = 0.f;
+  ctx->wave_scene = p->scene != 0;
   ctx->wave_on = true;
   return MAC3D_OK;
 }

@@ -417,4 +417,118 @@ k_top_pressure(WaveParams P, const WaveDrop *__restrict__ drops, const float4 *_
   ptop[(size_t)iz * nx + ix] = pk * dt_h * taper;
 }
 
+
+// The advection's scene terms (FLUID_ADVECT_MAC, fluid.ts:198-321), applied to
+// cell (i, j, k)'s advected and damped face values before the wall faces are
+// zeroed: the drops' wetting-neck momentum (the vertical slug/stem draft on
+// the y-face and the radial flare on the x- and z-faces, each drop's
+// m = 4, 6, 8 azimuthal modulation), the surfactant film's Marangoni pull and
+// the kinematic top condition from the wave (top three rows), and the soft
+// circular wall and floor sponges.  World positions: the tank [-hx, hx] in x
+// and z, [ytop - depth, ytop] in y with ytop the surface height (uvwToWorld,
+// common.ts:45-64, with domainBounds.z = surfaceParams.x as in the scene).
+template <class T>
+__device__ inline void advect_scene_terms(const Dims &d, int i, int j, int k, const WaveParams &P,
+                                          const WaveDrop *__restrict__ drops,
+                                          const float4 *__restrict__ wave, T &vx, T &vy, T &vz) {
+  const T hx = (T)P.half_xz, ytop = (T)P.surface_y, depth = (T)P.depth, y0 = ytop - depth;
+  const T nx = (T)d.nx, ny = (T)d.ny, nz = (T)d.nz;
+  auto wx_of = [&](T u) { return -hx + u * (T(2) * hx); };
+  auto wy_of = [&](T v) { return y0 + v * depth; };
+  const T dt = (T)P.dt;
+  const T cx = wx_of((i + T(0.5)) / nx), cy = wy_of((j + T(0.5)) / ny), cz = wx_of((k + T(0.5)) / nz);
+  const T fy_y = wy_of((j + T(1)) / ny);              // y-face (uvwY)
+  const T xf_x = wx_of((i + T(1)) / nx), zf_z = wx_of((k + T(1)) / nz);  // x-, z-faces
+  const T surf = (T)P.surface_y;
+  auto ss = [](T e0, T e1, T x) {
+    T t = clampv((x - e0) / (e1 - e0), T(0), T(1));
+    return t * t * (T(3) - T(2) * t);
+  };
+  auto azim_of = [](T th, T seed) {
+    return T(1) + T(0.036) * cos(T(4) * th + seed * T(1.73)) +
+           T(0.026) * cos(T(6) * th - seed * T(2.31)) + T(0.018) * sin(T(8) * th + seed * T(0.89));
+  };
+  auto radial_of = [&](T rr, T slug) {
+    const T sl = (T(0.14) + T(1) * fmin(rr * rr, T(1.20))) * (T(1) - ss(T(0.78), T(1.15), rr));
+    const T st = (T(0.68) + T(0.32) * fmin(rr * rr, T(1))) * (T(1) - ss(T(0.74), T(1.10), rr));
+    return mixv(st, sl, slug);
+  };
+  T active = T(0);
+  for (int q = 0; q < WV_MAX_DROPS; ++q) {
+    const WaveDrop s = drops[q];
+    if (!(s.seed > 0.f && s.drain > 0.0005f)) continue;
+    const T dx0 = (T)s.x, dz0 = (T)s.z, seed = (T)s.seed, drain = (T)s.drain;
+    const T ex = cx - dx0, ez = cz - dz0;  // (the y-face shares the cell's x and z)
+    const T r = sqrt(ex * ex + ez * ez);
+    const T th = atan2(ez, ex);
+    const T slug = ss(T(0.14), T(0.76), drain);
+    const T rbase = clampv((T)s.r * mixv(T(0.80), T(1.08), slug) + mixv(T(0.012), T(0.030), slug),
+                           T(0.018), T(0.088));
+    const T rr = r / fmax(rbase * azim_of(th, seed), T(1e-4));
+    const T radial = radial_of(rr, slug);
+    const T vmask = ss(surf - T(0.058), surf - T(0.014), fy_y);
+    const T shear = mixv(T(1), T(0.36), ss(surf - T(0.024), surf - T(0.002), fy_y));
+    const T mask = radial * vmask;
+    active = fmax(active, radial);
+    if (mask > T(0.001)) {
+      const T lobe = T(1) + T(0.036) * cos(T(4) * th + seed * T(1.73)) +
+                     T(0.026) * cos(T(6) * th - seed * T(2.31));
+      const T impact = clampv(-(T)s.vy, T(0.10), T(0.65));
+      const T slug_drive = clampv(pow(fmax(drain, T(0)), T(0.66)) * T(0.84), T(0), T(1.24));
+      const T stem_drive = clampv(T(0.045) + drain * T(0.22), T(0), T(0.20)) * ss(T(0), T(0.05), drain);
+      const T flux = mixv(stem_drive, slug_drive, slug);
+      const T target = -clampv(T(0.14) + T(0.64) * impact, T(0.15), T(0.39)) * flux * lobe * shear;
+      const T blend = clampv(dt * mixv(T(14), T(22), slug) * mask, T(0), T(1));
+      vy = mixv(vy, target, blend);
+    }
+    const T vmask_xz = ss(surf - T(0.058), surf - T(0.014), cy);
+    if (vmask_xz > T(0.001)) {
+      const T flare = mixv(T(0.104) * (T(0.35) + T(0.65) * slug), T(-0.020),
+                           ss(surf - T(0.066), surf - T(0.012), cy)) * clampv(drain, T(0), T(1.3));
+      // x-face at (xf_x, cz), z-face at (cx, zf_z)
+      const T fxs[2] = {xf_x - dx0, cx - dx0}, fzs[2] = {ez, zf_z - dz0};
+#pragma unroll
+      for (int f = 0; f < 2; ++f) {
+        const T ax = fxs[f], az = fzs[f];
+        const T rf = sqrt(ax * ax + az * az);
+        if (rf > T(1e-4) && rf < rbase * T(1.34)) {
+          const T rrf = rf / fmax(rbase * azim_of(atan2(az, ax), seed), T(1e-4));
+          if (rrf < T(1.22)) {
+            const T mf = radial_of(rrf, slug) * vmask_xz;
+            const T bf = clampv(dt * mixv(T(14), T(22), slug) * mf, T(0), T(1));
+            if (f == 0) vx = mixv(vx, (ax / rf) * flare, bf * T(0.65));
+            else vz = mixv(vz, (az / rf) * flare, bf * T(0.65));
+          }
+        }
+      }
+    }
+  }
+  if (j >= d.ny - 3) {
+    const T fade = T(j - (d.ny - 3) + 1) / T(3);
+    const float eps = 0.015f, E = P.ext;
+    auto uvw = [&](float x) { return clampv((x + E) / (2.f * E), 0.002f, 0.998f); };
+    const float4 wc = wv_sample(wave, P.n, uvw((float)cx), uvw((float)cz));
+    const float wxp = wv_sample(wave, P.n, uvw((float)cx + eps), uvw((float)cz)).w;
+    const float wxn = wv_sample(wave, P.n, uvw((float)cx - eps), uvw((float)cz)).w;
+    const float wzp = wv_sample(wave, P.n, uvw((float)cx), uvw((float)cz + eps)).w;
+    const float wzn = wv_sample(wave, P.n, uvw((float)cx), uvw((float)cz - eps)).w;
+    const T cmc = T(1) - ss(T(0.45), T(0.88), (T)wc.w);
+    const T gx = (T)(wxp - wxn) / (T(2) * (T)eps), gz = (T)(wzp - wzn) / (T(2) * (T)eps);
+    const T m = (T)P.marangoni;
+    vx = vx + (-m * T(0.026) * cmc * gx * fade) * dt;
+    vz = vz + (-m * T(0.026) * cmc * gz * fade) * dt;
+    if (j >= d.ny - 2 && active < T(0.02)) {
+      const T calm = T(1) - ss(T(0.020), T(0.14), fabs((T)wc.y));
+      const T layer = (j == d.ny - 1) ? T(0.72) : T(0.26);
+      vy = mixv(vy, clampv((T)wc.y * T(0.35), T(-0.045), T(0.045)), calm * layer);
+    }
+  }
+  const T rc = sqrt(cx * cx + cz * cz);
+  const T fade = (T(1) - ss(hx * T(0.68), hx * T(0.94), rc)) * ss(y0 + T(0.004), y0 + T(0.052), cy);
+  const T sponge = mixv(T(0.84), T(1), fade);
+  vx = vx * sponge;
+  vy = vy * sponge;
+  vz = vz * sponge;
+}
+
 }  // namespace mac3d

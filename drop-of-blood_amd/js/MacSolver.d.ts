@@ -11,7 +11,7 @@ export interface MacConfig {
 export interface Blob { cx: number; cy: number; cz: number; radius: number; w0: number; }
 export interface WaveParams {
   n?: number; halfExtent?: number; halfXZ?: number; depth?: number;
-  surfaceY?: number; marangoni?: number;
+  surfaceY?: number; marangoni?: number; scene?: 0 | 1;
 }
 export interface Drop {
   x: number; y: number; z: number; radius: number; velY: number; osc?: number;

@@ -64,7 +64,7 @@ class MacSolver {
    * Free-surface coupling: the wave solver and the lid pressure it feeds
    * (wave.ts:56-388, fluid.ts:16-78).  null disables it.
    * @param {{n?:number, halfExtent?:number, halfXZ?:number, depth?:number,
-   *          surfaceY?:number, marangoni?:number} | null} params
+   *          surfaceY?:number, marangoni?:number, scene?:number} | null} params
    */
   waveEnable(params) {
     addon.waveEnable(this._live(), params === undefined ? {} : params);

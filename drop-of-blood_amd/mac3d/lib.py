@@ -37,7 +37,7 @@ class Blob(C.Structure):
 
 
 class WaveParams(C.Structure):
-    _fields_ = [("n", C.c_int32), ("reserved", C.c_int32)] + \
+    _fields_ = [("n", C.c_int32), ("scene", C.c_int32)] + \
                [(n, C.c_double) for n in ("half_extent", "half_xz", "depth", "surface_y",
                                            "marangoni")]
 
@@ -227,8 +227,8 @@ class MacSolver:
 
     def wave_enable(self, enable: bool = True, **kw):
         """Enable the free-surface wave coupling (flat surface, no drops) with
-        mac3d_wave_params fields as keywords (n, half_extent, half_xz, depth,
-        surface_y, marangoni), or disable it."""
+        mac3d_wave_params fields as keywords (n, scene, half_extent, half_xz,
+        depth, surface_y, marangoni), or disable it."""
         if not enable:
             self._check(self.L.mac3d_wave_enable(self.h, None))
             self.wave_n = 0

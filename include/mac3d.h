@@ -214,7 +214,10 @@ int mac3d_adjoint(void *ctx, const mac3d_blob *blobs, int nblobs, double dt, int
  * Lengths in metres; the tank spans [-half_xz, half_xz] in x and z. */
 typedef struct mac3d_wave_params {
   int32_t n;          /* wave texels per side, power of two in [16, 1024] (512) */
-  int32_t reserved;   /* 0 */
+  int32_t scene;      /* 1: the advection also takes the reference scene's terms
+                         (drop wetting-neck momentum, Marangoni pull and kinematic
+                         top from the wave, wall/floor sponges; fluid.ts:198-321),
+                         the tank top at surface_y; 0 (default): not */
   double half_extent; /* half-width of the wave domain (WAVE_HALF_EXTENT, 1.60) */
   double half_xz;     /* tank half-width (domainBounds.x; 0: nx*dx/2) */
   double depth;       /* tank depth (domainBounds.z - domainBounds.y; 0: ny*dx) */

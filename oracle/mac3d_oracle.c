@@ -141,11 +141,15 @@ static void trace_back(const orc_state *s, double dt, const double uvw[3],
   for (int a = 0; a < 3; ++a) out[a] = clampd(uvw[a] - (vm[a] * dt) / dom[a], 0.002, 0.998);
 }
 
+static void advect_scene(const orc_state *s, int i, int j, int k, double dt, double *vx,
+                         double *vy, double *vz);
+
 /* ---- FLUID_ADVECT_MAC (fluid.ts:144-192, 323-325): semi-Lagrangian RK2
  * advection of each staggered component, sampled with the -1/2 texel shift
  * of sampleMacU/V/W (fluid.ts:154-167), times velDamp.  WORKLOAD: the scene
  * forcing blocks (drop necks :198-285, Marangoni/kinematic top :287-310,
- * sponges :312-321) are not part of the workload. */
+ * sponges :312-321) apply only with the wave coupling's scene flag
+ * (advect_scene). */
 static void advect_mac(orc_state *s, double dt) {
   dims3 d = s->d[0];
   double damp = s->p.vel_damp;
@@ -167,6 +171,7 @@ static void advect_mac(orc_state *s, double dt) {
         trace_back(s, dt, fZ, b);
         double vz = sample_cell(s->w, d, clampd(b[0], 0.001, 0.999), clampd(b[1], 0.001, 0.999),
                                 clampd(b[2] - 0.5 / d.nz, 0.001, 0.999)) * damp;
+        if (s->wave_on && s->wp.scene) advect_scene(s, i, j, k, dt, &vx, &vy, &vz);
         if (i >= d.nx - 1) vx = 0.0;
         if (k >= d.nz - 1) vz = 0.0;
         s->u2[c] = vx; s->v2[c] = vy; s->w2[c] = vz;
@@ -851,6 +856,91 @@ static void top_pressure(orc_state *s, double dt) {
       double taper = 1.0 - sstep(hx * 0.72, hx * 0.96, fmax(fabs(wx), fabs(wz)));
       s->ptop[(long)iz * d.nx + ix] = pk * dt_h * taper;
     }
+}
+
+
+/* The advection's scene terms (fluid.ts:198-321) on cell (i, j, k)'s damped
+ * face values: the drops' wetting-neck momentum (:198-285: slug/stem draft on
+ * the y-face, radial flare on the x- and z-faces, each with the m = 4, 6, 8
+ * azimuthal modulation), the film's Marangoni pull and the kinematic top from
+ * the wave (:287-310), the circular wall and floor sponges (:312-321).  World
+ * positions by uvwToWorld (common.ts:45-64) over the tank [-hx, hx] x
+ * [ytop - depth, ytop] x [-hx, hx], ytop the surface height. */
+static double azim3(double th, double seed) {
+  return 1.0 + 0.036 * cos(4.0 * th + seed * 1.73) + 0.026 * cos(6.0 * th - seed * 2.31) +
+         0.018 * sin(8.0 * th + seed * 0.89);
+}
+static double neck_radial(double rr, double slug) {
+  double sl = (0.14 + 1.00 * fmin(rr * rr, 1.20)) * (1.0 - sstep(0.78, 1.15, rr));
+  double st = (0.68 + 0.32 * fmin(rr * rr, 1.0)) * (1.0 - sstep(0.74, 1.10, rr));
+  return mixd(st, sl, slug);
+}
+static void advect_scene(const orc_state *s, int i, int j, int k, double dt, double *vx,
+                         double *vy, double *vz) {
+  dims3 d = s->d[0];
+  const double hx = s->wp.half_xz, ytop = s->wp.surface_y, y0 = ytop - s->wp.depth;
+  dt = clampd(dt, 0.0, 0.025);
+  double cx = -hx + (i + 0.5) / d.nx * 2.0 * hx, cz = -hx + (k + 0.5) / d.nz * 2.0 * hx;
+  double cy = y0 + (j + 0.5) / d.ny * s->wp.depth, fy = y0 + (j + 1.0) / d.ny * s->wp.depth;
+  double xf = -hx + (i + 1.0) / d.nx * 2.0 * hx, zf = -hx + (k + 1.0) / d.nz * 2.0 * hx;
+  double surf = s->wp.surface_y, active = 0.0;
+  for (int q = 0; q < MAX_DROPS; ++q) {
+    const orc_drop *dr = &s->drops[q];
+    if (!(dr->seed > 0.0 && dr->drain > 0.0005)) continue;
+    double ex = cx - dr->x, ez = cz - dr->z, r = hypot(ex, ez), th = atan2(ez, ex);
+    double slug = sstep(0.14, 0.76, dr->drain);
+    double rbase = clampd(dr->radius * mixd(0.80, 1.08, slug) + mixd(0.012, 0.030, slug), 0.018, 0.088);
+    double radial = neck_radial(r / fmax(rbase * azim3(th, dr->seed), 1e-4), slug);
+    double mask = radial * sstep(surf - 0.058, surf - 0.014, fy);
+    double shear = mixd(1.0, 0.36, sstep(surf - 0.024, surf - 0.002, fy));
+    active = fmax(active, radial);
+    if (mask > 0.001) {
+      double lobe = 1.0 + 0.036 * cos(4.0 * th + dr->seed * 1.73) + 0.026 * cos(6.0 * th - dr->seed * 2.31);
+      double impact = clampd(-dr->vel_y, 0.10, 0.65);
+      double slug_drive = clampd(pow(fmax(dr->drain, 0.0), 0.66) * 0.84, 0.0, 1.24);
+      double stem_drive = clampd(0.045 + dr->drain * 0.22, 0.0, 0.20) * sstep(0.0, 0.05, dr->drain);
+      double target = -clampd(0.14 + 0.64 * impact, 0.15, 0.39) * mixd(stem_drive, slug_drive, slug) *
+                      lobe * shear;
+      *vy = mixd(*vy, target, clampd(dt * mixd(14.0, 22.0, slug) * mask, 0.0, 1.0));
+    }
+    double vmxz = sstep(surf - 0.058, surf - 0.014, cy);
+    if (vmxz > 0.001) {
+      double flare = mixd(0.104 * (0.35 + 0.65 * slug), -0.020, sstep(surf - 0.066, surf - 0.012, cy)) *
+                     clampd(dr->drain, 0.0, 1.3);
+      double fxs[2] = {xf - dr->x, cx - dr->x}, fzs[2] = {ez, zf - dr->z};
+      for (int f = 0; f < 2; ++f) {
+        double rf = hypot(fxs[f], fzs[f]);
+        if (!(rf > 1e-4 && rf < rbase * 1.34)) continue;
+        double rrf = rf / fmax(rbase * azim3(atan2(fzs[f], fxs[f]), dr->seed), 1e-4);
+        if (!(rrf < 1.22)) continue;
+        double bf = clampd(dt * mixd(14.0, 22.0, slug) * neck_radial(rrf, slug) * vmxz, 0.0, 1.0);
+        if (f == 0) *vx = mixd(*vx, (fxs[f] / rf) * flare, bf * 0.65);
+        else *vz = mixd(*vz, (fzs[f] / rf) * flare, bf * 0.65);
+      }
+    }
+  }
+  if (j >= d.ny - 3) {
+    double fade = (double)(j - (d.ny - 3) + 1) / 3.0, eps = 0.015, E = s->wp.half_extent;
+    int n = s->wp.n;
+#define UV(x) clampd(((x) + E) / (2.0 * E), 0.002, 0.998)
+    double film = wv_sample(s->wave, n, UV(cx), UV(cz), 3), wvel = wv_sample(s->wave, n, UV(cx), UV(cz), 1);
+    double gx = (wv_sample(s->wave, n, UV(cx + eps), UV(cz), 3) - wv_sample(s->wave, n, UV(cx - eps), UV(cz), 3)) / (2.0 * eps);
+    double gz = (wv_sample(s->wave, n, UV(cx), UV(cz + eps), 3) - wv_sample(s->wave, n, UV(cx), UV(cz - eps), 3)) / (2.0 * eps);
+#undef UV
+    double cmc = 1.0 - sstep(0.45, 0.88, film);
+    *vx += -s->wp.marangoni * 0.026 * cmc * gx * fade * dt;
+    *vz += -s->wp.marangoni * 0.026 * cmc * gz * fade * dt;
+    if (j >= d.ny - 2 && active < 0.02) {
+      double calm = 1.0 - sstep(0.020, 0.14, fabs(wvel));
+      double layer = (j == d.ny - 1) ? 0.72 : 0.26;
+      *vy = mixd(*vy, clampd(wvel * 0.35, -0.045, 0.045), calm * layer);
+    }
+  }
+  double wf = (1.0 - sstep(hx * 0.68, hx * 0.94, hypot(cx, cz))) * sstep(y0 + 0.004, y0 + 0.052, cy);
+  double sponge = mixd(0.84, 1.0, wf);
+  *vx *= sponge;
+  *vy *= sponge;
+  *vz *= sponge;
 }
 
 void orc_wave_enable(orc_state *s, const orc_wave_params *p) {

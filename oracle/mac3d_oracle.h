@@ -34,7 +34,7 @@ typedef struct orc_blob {
 /* free-surface wave coupling (lengths in metres; 0 picks nx dx / 2 and
  * ny dx for half_xz and depth) and one drop slot, as in include/mac3d.h */
 typedef struct orc_wave_params {
-  int n;
+  int n, scene; /* scene: also the advection's scene terms */
   double half_extent, half_xz, depth, surface_y, marangoni;
 } orc_wave_params;
 

@@ -35,7 +35,7 @@ class OrcBlob(C.Structure):
 
 
 class OrcWaveParams(C.Structure):
-    _fields_ = [("n", C.c_int)] + [(n, C.c_double) for n in (
+    _fields_ = [("n", C.c_int), ("scene", C.c_int)] + [(n, C.c_double) for n in (
         "half_extent", "half_xz", "depth", "surface_y", "marangoni")]
 
 
@@ -168,13 +168,14 @@ class Oracle:
         return a.value, b.value
 
     def wave_enable(self, n=512, half_extent=1.60, half_xz=0.0, depth=0.0, surface_y=0.30,
-                    marangoni=0.0, off=False):
+                    marangoni=0.0, scene=0, off=False):
         """enable (flat surface, no drops) or, off=True, disable the wave coupling"""
         if off:
             lib().orc_wave_enable(self.h, None)
             return
-        self.wave_params = OrcWaveParams(n=n, half_extent=half_extent, half_xz=half_xz,
-                                         depth=depth, surface_y=surface_y, marangoni=marangoni)
+        self.wave_params = OrcWaveParams(n=n, scene=int(scene), half_extent=half_extent,
+                                         half_xz=half_xz, depth=depth, surface_y=surface_y,
+                                         marangoni=marangoni)
         lib().orc_wave_enable(self.h, C.byref(self.wave_params))
 
     def set_drops(self, drops):

@@ -37,10 +37,10 @@ def wave_state(n, seed=5):
     return W
 
 
-def coupled(n=64, prec=0, warm_steps=2):
+def coupled(n=64, prec=0, warm_steps=2, scene=0):
     w, s, o = pair(precision=prec, warm_steps=warm_steps)
-    s.wave_enable(n=n, **GEOM)
-    o.wave_enable(n=n, **GEOM)
+    s.wave_enable(n=n, scene=scene, **GEOM)
+    o.wave_enable(n=n, scene=scene, **GEOM)
     s.set_drops(DROPS)
     o.set_drops(DROPS)
     W = wave_state(n)
@@ -183,3 +183,37 @@ def test_packed_uniform_block_sets_the_coupling():
         assert np.array_equal(s1.download("PTOP"), s2.download("PTOP"))
         with pytest.raises(Mac3dError, match="EINVAL"):
             s1.set_uniforms(blk[:64])
+
+
+def test_scene_advection_against_oracle():
+    """The advection with the scene terms (drop necks, Marangoni pull and
+    kinematic top from the wave, sponges; fluid.ts:198-321) after one wave
+    step, against the oracle's."""
+    w, s, o = coupled(64, scene=1)
+    with s:
+        s.stage("WAVE", w.dt)
+        o.stage(oracle.ST_WAVE, w.dt)
+        s.stage("FACE_TO_CENTRE")
+        o.stage(oracle.ST_F2C)
+        before = s.download("V")
+        s.stage("ADVECT", w.dt)
+        o.stage(oracle.ST_ADVECT, w.dt)
+        assert not np.allclose(s.download("V"), before)
+        for f in ("U", "V", "W"):
+            e = rel_l2(s.download(f), o.get(getattr(oracle, "F_" + f)))
+            assert e <= TOLW, f"{f}: rel L2 {e:.3e}"
+    o.close()
+
+
+def test_scene_coupled_steps_against_oracle():
+    """Five full FP32 steps with the wave, the lid pressure and the
+    advection's scene terms."""
+    w, s, o = coupled(64, scene=1)
+    with s:
+        s.step(w.dt, 5)
+        o.step(w.dt, 5)
+        for f in ("U", "V", "W", "P", "DYE"):
+            e = rel_l2(s.download(f), o.get(getattr(oracle, "F_" + f)))
+            assert e <= 2e-5, f"{f}: rel L2 {e:.3e}"
+        _cmp_wave(s, o, TOLW)
+    o.close()

@@ -203,3 +203,123 @@ def test_divergence_and_projection_carry_the_lid_pressure():
     v1 = o.get(oracle.F_V)
     assert np.allclose(v1[:, -1, :], v0[:, -1, :] - pt, atol=1e-15)
     assert np.array_equal(v1[:, :-1, :], v0[:, :-1, :])
+
+
+# ---------------------------------------------------------------- advection scene terms
+def scene_pair(drops=(), wave=None, **geom):
+    """two oracles on one random velocity: coupling with the advection's scene
+    terms (fluid.ts:198-321) and without them"""
+    rng = np.random.default_rng(11)
+    out = []
+    for scene in (1, 0):
+        o = make(nx=16, ny=24, nz=16)
+        for f in (oracle.F_U, oracle.F_V, oracle.F_W):
+            o.set(f, rng.standard_normal(o.shape) * 0.01 if scene else out[0][1][f])
+        g = dict(n=32, half_extent=0.05, half_xz=0.008, depth=0.024, surface_y=0.0, marangoni=0.6)
+        g.update(geom)
+        o.wave_enable(scene=scene, **g)
+        o.set_drops(list(drops))
+        if wave is not None:
+            o.view(oracle.F_WAVE)[...] = wave
+        out.append((o, {f: o.get(f) for f in (oracle.F_U, oracle.F_V, oracle.F_W)}))
+    return out[0][0], out[1][0]
+
+
+def sponge_of(o):
+    """the sponge multiplier mix(0.84, 1, wallFade) per cell (fluid.ts:312-321)"""
+    nz, ny, nx = o.shape
+    hx, y0 = 0.008, 0.0 - 0.024
+    x = -hx + (np.arange(nx) + 0.5) / nx * 2 * hx
+    z = -hx + (np.arange(nz) + 0.5) / nz * 2 * hx
+    y = y0 + (np.arange(ny) + 0.5) / ny * 0.024
+    Z, Y, X = np.meshgrid(z, y, x, indexing="ij")
+    ss = np.vectorize(oracle.smoothstep)
+    fade = (1 - ss(hx * 0.68, hx * 0.94, np.sqrt(X ** 2 + Z ** 2))) * ss(y0 + 0.004, y0 + 0.052, Y)
+    return 0.84 + (1.0 - 0.84) * fade
+
+
+def test_scene_sponge_closed_form():
+    """No drops and a calm surface: below the top three rows the scene terms
+    only multiply each advected face value by the sponge factor."""
+    a, b = scene_pair()
+    dt = 0.002
+    a.stage(oracle.ST_F2C)
+    b.stage(oracle.ST_F2C)
+    a.stage(oracle.ST_ADVECT, dt)
+    b.stage(oracle.ST_ADVECT, dt)
+    sp = sponge_of(a)
+    for f in (oracle.F_U, oracle.F_V, oracle.F_W):
+        ga, gb = a.get(f)[:, :-3, :], b.get(f)[:, :-3, :]
+        assert np.allclose(ga, gb * sp[:, :-3, :], rtol=1e-14, atol=1e-300)
+
+
+def test_scene_kinematic_top_and_marangoni_closed_form():
+    """A surface moving at d eta/dt = g everywhere with a film rising
+    linearly in x (film = f0 + s x): on the top three rows the x-faces gain
+    -m 0.026 (1 - smoothstep(0.45, 0.88, film)) s fade dt (the film gradient by
+    centred differences of the bilinear samples is exact for a linear film),
+    and the top two rows' y-faces blend towards clamp(0.35 g) by
+    (1 - smoothstep(0.02, 0.14, |g|)) times 0.26 / 0.72 (fluid.ts:287-310)."""
+    n, E = 32, 0.05
+    x = (np.arange(n) + 0.5) / n * 2 * E - E
+    W = np.zeros((n, n, 4))
+    g, f0, s = 0.05, 0.4, 3.0
+    W[..., 1] = g
+    W[..., 3] = f0 + s * x[None, :]
+    a, b = scene_pair(wave=W)
+    dt = 0.002
+    for o in (a, b):
+        o.stage(oracle.ST_F2C)
+        o.stage(oracle.ST_ADVECT, dt)
+    sp = sponge_of(a)
+    nz, ny, nx = a.shape
+    hx = 0.008
+    cx = -hx + (np.arange(nx) + 0.5) / nx * 2 * hx
+    ss = oracle.smoothstep
+    ua, ub = a.get(oracle.F_U), b.get(oracle.F_U)
+    va, vb = a.get(oracle.F_V), b.get(oracle.F_V)
+    for j in range(ny - 3, ny):
+        fade = (j - (ny - 3) + 1) / 3.0
+        cmc = np.array([1 - ss(0.45, 0.88, f0 + s * xc) for xc in cx])
+        want = (ub[:, j, :] + (-0.6 * 0.026 * cmc * s * fade) * dt) * sp[:, j, :]
+        assert np.allclose(ua[:, j, :-1], want[:, :-1], rtol=1e-12, atol=1e-15)
+    calm = 1 - ss(0.020, 0.14, abs(g))
+    tgt = min(max(g * 0.35, -0.045), 0.045)
+    for j, lw in ((ny - 2, 0.26), (ny - 1, 0.72)):
+        want = (vb[:, j, :] * (1 - calm * lw) + tgt * calm * lw) * sp[:, j, :]
+        assert np.allclose(va[:, j, :], want, rtol=1e-12, atol=1e-15)
+
+
+def test_scene_neck_draft_on_the_drop_axis():
+    """The y-face on a drop's axis (r = 0): the neck profile's centre value
+    mix(0.68, 0.14, slug), the vertical mask at the face height, the lobe at
+    theta = 0 and the flux drive give the draft target and blend of
+    fluid.ts:213-245 in closed form."""
+    hx, nx = 0.008, 16
+    # a drop whose axis is the centre of cell (8, *, 8): cell centres at -hx + (i + 1/2) h
+    h = 2 * hx / nx
+    x0 = -hx + 8.5 * h
+    rd, vy, drain, seed = 0.02, -0.3, 0.4, 0.9
+    a, b = scene_pair(drops=[(x0, 0.01, x0, rd, vy, 0.0, drain, seed)])
+    dt = 0.002
+    for o in (a, b):
+        o.stage(oracle.ST_F2C)
+        o.stage(oracle.ST_ADVECT, dt)
+    sp = sponge_of(a)
+    ss = oracle.smoothstep
+    nz, ny, _ = a.shape
+    slug = ss(0.14, 0.76, drain)
+    radial = 0.68 + (0.14 - 0.68) * slug
+    lobe = 1 + 0.036 * math.cos(seed * 1.73) + 0.026 * math.cos(-seed * 2.31)
+    flux = (min(max(0.045 + drain * 0.22, 0), 0.2) * ss(0, 0.05, drain)) * (1 - slug) + \
+        min(max(max(drain, 0) ** 0.66 * 0.84, 0), 1.24) * slug
+    impact = min(max(-vy, 0.10), 0.65)
+    va, vb = a.get(oracle.F_V), b.get(oracle.F_V)
+    for j in range(ny - 6, ny - 3):  # below the kinematic top rows
+        fy = -0.024 + (j + 1.0) / ny * 0.024
+        mask = radial * ss(-0.058, -0.014, fy)
+        shear = 1 + (0.36 - 1) * ss(-0.024, -0.002, fy)
+        target = -min(max(0.14 + 0.64 * impact, 0.15), 0.39) * flux * lobe * shear
+        blend = min(max(dt * (14 + 8 * slug) * mask, 0), 1)
+        want = (vb[8, j, 8] * (1 - blend) + target * blend) * sp[8, j, 8]
+        assert va[8, j, 8] == pytest.approx(want, rel=1e-12, abs=1e-15)
