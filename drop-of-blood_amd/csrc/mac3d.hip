@@ -133,7 +133,8 @@ struct Ctx {
   bool wave_scene = false;  // also the advection's scene terms
   WaveParams wp{};
   Buf wave, wz, wfilm, ptop;
-  WaveDrop *drops = nullptr;
+  Buf curl;  // scene: curl of the post-advection centre velocity, 4 per cell
+  WaveDrop *drops = nullptr;  // WV_MAX_DROPS slots, then the SceneClock
   float2 *wtw = nullptr;
   // staging for upload/download (stage_n doubles)
   double *stage = nullptr;
@@ -319,9 +320,53 @@ int launch_advect(Ctx *ctx, double dt) {
   return MAC3D_OK;
 }
 
+const SceneClock *scene_clock(const Ctx *ctx) {
+  return (const SceneClock *)(ctx->drops + WV_MAX_DROPS);
+}
+
+// The scene's Casson terms (BloodDropEngine.ts:1922-1934): the centre velocity
+// of the advected faces, its curl, then, after the viscous update, the
+// confinement and unfurling forces in place.
+int launch_curl(Ctx *ctx) {
+  const Dims &d = ctx->d[0];
+  int rc;
+  if ((rc = launch_face_to_centre(ctx))) return rc;
+  {
+    ProfScope ps(ctx->prof, ctx->stream, KC_CASSON);
+    with_step_type(ctx->t0, [&](auto tag) {
+      using T = std::remove_pointer_t<decltype(tag)>;
+      hipLaunchKernelGGL(k_curl<T>, cell_grid(d), kCellBlock, 0, ctx->stream, d,
+                         (const typename cvec_of<T>::type *)ctx->cvel.ptr,
+                         (typename q4_of<T>::type *)ctx->curl.ptr);
+    });
+  }
+  LAUNCHCHK();
+  return MAC3D_OK;
+}
+
+int launch_casson_scene(Ctx *ctx, double dt) {
+  const Dims &d = ctx->d[0];
+  {
+    ProfScope ps(ctx->prof, ctx->stream, KC_CASSON);
+    with_step_type(ctx->t0, [&](auto tag) {
+      using T = std::remove_pointer_t<decltype(tag)>;
+      hipLaunchKernelGGL(k_casson_scene<T>, cell_grid(d), kCellBlock, 0, ctx->stream, d,
+                         wave_params_dt(ctx, dt), scene_clock(ctx),
+                         (const typename q4_of<T>::type *)ctx->curl.ptr,
+                         (const T *)ctx->dye.ptr, (T *)ctx->u.ptr, (T *)ctx->v.ptr,
+                         (T *)ctx->w.ptr);
+    });
+  }
+  LAUNCHCHK();
+  return MAC3D_OK;
+}
+
 int launch_casson(Ctx *ctx, double dt) {
   const Dims &d = ctx->d[0];
   StepParams P = step_params(ctx, dt);
+  const bool scene = ctx->wave_on && ctx->wave_scene;
+  int rc;
+  if (scene && (rc = launch_curl(ctx))) return rc;
   {
     ProfScope ps(ctx->prof, ctx->stream, KC_CASSON);
     with_step_type(ctx->t0, [&](auto tag) {
@@ -339,7 +384,7 @@ int launch_casson(Ctx *ctx, double dt) {
   }
   LAUNCHCHK();
   swap_vel(ctx);
-  return MAC3D_OK;
+  return scene ? launch_casson_scene(ctx, dt) : MAC3D_OK;
 }
 
 // the lid's Dirichlet pressure for the divergence and projection: the wave
@@ -1087,6 +1132,7 @@ Buf *field_buf(Ctx *ctx, int field, int level) {
     case MAC3D_FIELD_MGR: return &ctx->mgr[level];
     case MAC3D_FIELD_WAVE: return l0 && ctx->wave_on ? &ctx->wave : nullptr;
     case MAC3D_FIELD_PTOP: return l0 && ctx->wave_on ? &ctx->ptop : nullptr;
+    case MAC3D_FIELD_CURL: return l0 && ctx->curl.ptr ? &ctx->curl : nullptr;
   }
   return nullptr;
 }
@@ -1736,6 +1782,9 @@ int mac3d_wave_default_params(mac3d_wave_params *p) {
   p->depth = 0.0;           // ny dx
   p->surface_y = 0.30;      // WATER_SURFACE_Y
   p->marangoni = 0.0;
+  p->vort_conf = 1.02;      // vorticityConfinement (uniformLayout.ts:32)
+  p->sim_time = 0.0;
+  p->schlieren = 0.032;     // schlierenStrength (uniformLayout.ts:44)
   return MAC3D_OK;
 }
 
@@ -1755,7 +1804,8 @@ int mac3d_wave_enable(void *h, const mac3d_wave_params *p) {
   const double depth = p->depth > 0.0 ? p->depth : ctx->cfg.ny * ctx->cfg.dx;
   if (!(ext > 0.0) || !std::isfinite(ext) || !std::isfinite(hx) || !std::isfinite(depth) ||
       !std::isfinite(p->surface_y) || !std::isfinite(p->marangoni) || p->half_xz < 0.0 ||
-      p->depth < 0.0)
+      p->depth < 0.0 || !std::isfinite(p->vort_conf) || !std::isfinite(p->sim_time) ||
+      !std::isfinite(p->schlieren))
     return fail(ctx, MAC3D_EINVAL, "wave lengths must be finite, half_extent > 0");
   int rc;
   const size_t nn = (size_t)n * n;
@@ -1778,11 +1828,22 @@ int mac3d_wave_enable(void *h, const mac3d_wave_params *p) {
   }
   if (!ctx->ptop.ptr && (rc = alloc(ctx, ctx->ptop, TF32, (size_t)ctx->cfg.nx * ctx->cfg.nz)))
     return rc;
-  if (!ctx->drops) {
-    if ((rc = dev_alloc(ctx, (void **)&ctx->drops, WV_MAX_DROPS * sizeof(WaveDrop)))) return rc;
-    HIPCHK(hipMemsetAsync(ctx->drops, 0, WV_MAX_DROPS * sizeof(WaveDrop), ctx->stream));
+  const size_t dsz = WV_MAX_DROPS * sizeof(WaveDrop) + sizeof(SceneClock);
+  if (!ctx->drops && (rc = dev_alloc(ctx, (void **)&ctx->drops, dsz))) return rc;
+  HIPCHK(hipMemsetAsync(ctx->drops, 0, WV_MAX_DROPS * sizeof(WaveDrop), ctx->stream));
+  if (p->scene && !ctx->curl.ptr) {
+    const size_t n4 = 4 * ctx->n[0];
+    if ((rc = alloc(ctx, ctx->curl, ctx->t0, n4))) return rc;  // (zeroed)
+    if (n4 > ctx->stage_n) {
+      if ((rc = dev_alloc(ctx, (void **)&ctx->stage, n4 * sizeof(double)))) return rc;
+      ctx->stage_n = n4;
+    }
   }
-  // a flat, film-free surface (WAVE_CLEAR, wave.ts:15-30) and no lid pressure
+  const SceneClock clk{(float)p->sim_time, (float)p->schlieren, (float)p->vort_conf, 0.f};
+  HIPCHK(hipMemcpyAsync((void *)scene_clock(ctx), &clk, sizeof(clk), hipMemcpyHostToDevice,
+                        ctx->stream));
+  // a flat, film-free surface (WAVE_CLEAR, wave.ts:15-30), no drops and no lid
+  // pressure
   HIPCHK(hipMemsetAsync(ctx->wave.ptr, 0, 4 * nn * sizeof(float), ctx->stream));
   HIPCHK(hipMemsetAsync(ctx->ptop.ptr, 0, ctx->ptop.n * sizeof(float), ctx->stream));
   HIPCHK(hipStreamSynchronize(ctx->stream));
@@ -1825,7 +1886,7 @@ int mac3d_set_uniforms(void *h, const float *u, size_t count) {
   Ctx *ctx = (Ctx *)h;
   if (!u || count != 128) return fail(ctx, MAC3D_EINVAL, "need the 128-float SimUniforms block");
   if (!ctx->wave_on) return fail(ctx, MAC3D_ESTATE, "the wave coupling is not enabled");
-  for (size_t q = 40; q < 128; ++q)
+  for (size_t q = 35; q < 128; ++q)
     if (!std::isfinite(u[q])) return fail(ctx, MAC3D_EINVAL, "non-finite uniform value");
   const float half_xz = u[52], depth = u[54] - u[53];
   if (!(half_xz > 0.f) || !(depth > 0.f))
@@ -1837,6 +1898,11 @@ int mac3d_set_uniforms(void *h, const float *u, size_t count) {
   }
   int rc = mac3d_set_drops(h, d, MAC3D_MAX_DROPS);
   if (rc) return rc;
+  // simTime, schlierenStr, vorticityConf (device memory: no graph rebuild)
+  const SceneClock clk{u[35], u[60], u[44], 0.f};
+  HIPCHK(hipMemcpyAsync((void *)scene_clock(ctx), &clk, sizeof(clk), hipMemcpyHostToDevice,
+                        ctx->stream));
+  HIPCHK(hipStreamSynchronize(ctx->stream));
   if (ctx->wp.surface_y != u[40] || ctx->wp.marangoni != u[41] || ctx->wp.half_xz != half_xz ||
       ctx->wp.depth != depth)
     drop_graphs(ctx);  // (the captured launches hold the parameters by value)

@@ -531,4 +531,124 @@ __device__ inline void advect_scene_terms(const Dims &d, int i, int j, int k, co
   vz = vz * sponge;
 }
 
+// the scene's per-frame uniforms that the Casson pass reads from device memory
+// (so a captured step keeps them current): simTime (cameraPosAndTime.w),
+// schlierenStr (particleParams.x), vorticityConf (fluidParams.x)
+struct SceneClock {
+  float sim_time, schlieren, vort_conf, pad;
+};
+
+template <class T> struct q4_of;
+template <> struct q4_of<float> { typedef float4 type; };
+template <> struct q4_of<double> { typedef double4 type; };
+
+// FLUID_CURL (src/shaders/fluid.ts:331-369): the curl of the cell-centred
+// velocity by central differences of the neighbouring cells (grid units: no
+// 1/h), neighbour indices clamped to the grid, and its length in .w.
+template <class T>
+__global__ void __launch_bounds__(256)
+k_curl(Dims d, const typename cvec_of<T>::type *__restrict__ cvel,
+       typename q4_of<T>::type *__restrict__ curl) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  const int j = blockIdx.y * blockDim.y + threadIdx.y;
+  const int k = blockIdx.z;
+  if (i >= d.nx || j >= d.ny) return;
+  auto at = [&](int a, int b, int e) { return cvel[idx3(d, a, b, e)]; };
+  const auto vR = at(min(i + 1, d.nx - 1), j, k), vL = at(max(i - 1, 0), j, k);
+  const auto vU = at(i, min(j + 1, d.ny - 1), k), vD = at(i, max(j - 1, 0), k);
+  const auto vF = at(i, j, min(k + 1, d.nz - 1)), vB = at(i, j, max(k - 1, 0));
+  typename q4_of<T>::type o;
+  o.x = T(0.5) * ((vU.z - vD.z) - (vF.y - vB.y));
+  o.y = T(0.5) * ((vF.x - vB.x) - (vR.z - vL.z));
+  o.z = T(0.5) * ((vR.y - vL.y) - (vU.x - vD.x));
+  o.w = sqrt(o.x * o.x + o.y * o.y + o.z * o.z);
+  curl[idx3(d, i, j, k)] = o;
+}
+
+// evalVortConf (fluid.ts:394-423): the confinement acceleration at cell
+// (i, j, k): zero within two cells of the side walls and the floor and three
+// of the lid, and where |omega| <= 0.18; else
+// scale * gate(|omega|) * (N x omega), N the unit central-difference gradient
+// of |omega| (zero when that gradient is below 1e-5).
+template <class T>
+__device__ inline void vort_conf_at(const Dims &d, const typename q4_of<T>::type *__restrict__ curl,
+                                    int i, int j, int k, T scale, T f[3]) {
+  f[0] = f[1] = f[2] = T(0);
+  if (i <= 1 || i >= d.nx - 2 || j <= 1 || j >= d.ny - 3 || k <= 1 || k >= d.nz - 2) return;
+  const auto om = curl[idx3(d, i, j, k)];
+  if (om.w <= T(0.18)) return;
+  const T t = clampv((om.w - T(0.18)) / T(0.95 - 0.18), T(0), T(1));
+  const T gate = t * t * (T(3) - T(2) * t) / (T(1) + T(0.28) * om.w);
+  const T ex = T(0.5) * (curl[idx3(d, i + 1, j, k)].w - curl[idx3(d, i - 1, j, k)].w);
+  const T ey = T(0.5) * (curl[idx3(d, i, j + 1, k)].w - curl[idx3(d, i, j - 1, k)].w);
+  const T ez = T(0.5) * (curl[idx3(d, i, j, k + 1)].w - curl[idx3(d, i, j, k - 1)].w);
+  const T el = sqrt(ex * ex + ey * ey + ez * ez);
+  if (el <= T(1e-5)) return;
+  const T g = scale * gate / el;
+  f[0] = g * (ey * om.z - ez * om.y);
+  f[1] = g * (ez * om.x - ex * om.z);
+  f[2] = g * (ex * om.y - ey * om.x);
+}
+
+// The Casson pass's scene terms (FLUID_CASSON_FORCES, fluid.ts:467-509),
+// added in place to the faces of cell (i, j, k) after the viscous update and
+// buoyancy: the vorticity confinement averaged onto each face with its
+// +1 neighbour (:467-472), the deep-column solenoidal unfurling forcing
+// (:488-503; gate from the dye, a single species here, so the lysed channel
+// is 0), the +-1.8 clamp and the wall faces zeroed (:505-507).  World
+// positions as in advect_scene_terms; dt clamped to [0, 0.025].
+template <class T>
+__global__ void __launch_bounds__(256)
+k_casson_scene(Dims d, WaveParams P, const SceneClock *__restrict__ clk,
+               const typename q4_of<T>::type *__restrict__ curl, const T *__restrict__ dye,
+               T *__restrict__ u, T *__restrict__ v, T *__restrict__ w) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  const int j = blockIdx.y * blockDim.y + threadIdx.y;
+  const int k = blockIdx.z;
+  if (i >= d.nx || j >= d.ny) return;
+  const size_t c = idx3(d, i, j, k);
+  const T dt = (T)P.dt, scale = (T)clk->vort_conf;
+  T f0[3], fx[3], fy[3], fz[3];
+  vort_conf_at<T>(d, curl, i, j, k, scale, f0);
+  vort_conf_at<T>(d, curl, min(i + 1, d.nx - 1), j, k, scale, fx);
+  vort_conf_at<T>(d, curl, i, min(j + 1, d.ny - 1), k, scale, fy);
+  vort_conf_at<T>(d, curl, i, j, min(k + 1, d.nz - 1), scale, fz);
+  T mx = u[c] + T(0.5) * (f0[0] + fx[0]) * dt;
+  T my = v[c] + T(0.5) * (f0[1] + fy[1]) * dt;
+  T mz = w[c] + T(0.5) * (f0[2] + fz[2]) * dt;
+  const T hx = (T)P.half_xz, depth = (T)P.depth, y0 = (T)P.surface_y - depth;
+  const T px = -hx + (i + T(0.5)) / (T)d.nx * (T(2) * hx);
+  const T py = y0 + (j + T(0.5)) / (T)d.ny * depth;
+  const T pz = -hx + (k + T(0.5)) / (T)d.nz * (T(2) * hx);
+  const T below = (T)P.surface_y - py;
+  auto ss = [](T e0, T e1, T x) {
+    T t = clampv((x - e0) / (e1 - e0), T(0), T(1));
+    return t * t * (T(3) - T(2) * t);
+  };
+  const T crbc = clampv(dye[c], T(0), T(1.5));
+  const T gate = ss(T(0.11), T(0.36), below) * ss(T(0.0008), T(0.015), crbc);
+  if (gate > T(0.001)) {
+    const T ts = (T)clk->sim_time * T(0.36) + (T)clk->schlieren * T(1.7);
+    const T q1 = T(6.4) * px + T(4.0) * py + T(5.2) * pz + ts;
+    const T q2 = T(-5.6) * px + T(4.8) * py + T(7.2) * pz - ts * T(0.77);
+    const T q3 = T(9.0) * px - T(6.0) * py - T(6.4) * pz + ts * T(1.13);
+    const T c1 = cos(q1), c2 = cos(q2), s3 = T(0.42) * sin(q3);
+    const T sx = T(-6.25) * c1 + T(-6.30) * c2 + T(6.40) * s3;
+    const T sy = T(2.85) * c1 + T(-1.26) * c2 + T(-0.96) * s3;
+    const T sz = T(5.50) * c1 + T(-4.06) * c2 + T(9.90) * s3;
+    const T a = T(0.0028) * gate * dt;
+    mx += sx * a;
+    my += sy * ss(T(0.16), T(0.42), below) * a;
+    mz += sz * a;
+  }
+  mx = clampv(mx, T(-1.8), T(1.8));
+  my = clampv(my, T(-1.8), T(1.8));
+  mz = clampv(mz, T(-1.8), T(1.8));
+  if (i >= d.nx - 1) mx = T(0);
+  if (k >= d.nz - 1) mz = T(0);
+  u[c] = mx;
+  v[c] = my;
+  w[c] = mz;
+}
+
 }  // namespace mac3d

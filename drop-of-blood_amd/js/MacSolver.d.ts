@@ -12,6 +12,7 @@ export interface Blob { cx: number; cy: number; cz: number; radius: number; w0: 
 export interface WaveParams {
   n?: number; halfExtent?: number; halfXZ?: number; depth?: number;
   surfaceY?: number; marangoni?: number; scene?: 0 | 1;
+  vortConf?: number; simTime?: number; schlieren?: number;
 }
 export interface Drop {
   x: number; y: number; z: number; radius: number; velY: number; osc?: number;
@@ -22,7 +23,7 @@ export interface StepStats {
   maxDiv: number; l2Div: number; maxDivRate: number; relRes: number; ms: number;
 }
 export declare const FIELD: Readonly<{ U: 0; V: 1; W: 2; P: 3; DYE: 4; CU: 5; CV: 6; CW: 7;
-  RHS: 8; MGX: 9; MGB: 10; MGR: 11; DYE_PRED: 12; WAVE: 13; PTOP: 14 }>;
+  RHS: 8; MGX: 9; MGB: 10; MGR: 11; DYE_PRED: 12; WAVE: 13; PTOP: 14; CURL: 15 }>;
 export declare const PRECISION: Readonly<{ fp32: 0; mixed: 1; fp64: 2 }>;
 export declare function version(): string;
 export declare class MacSolver {

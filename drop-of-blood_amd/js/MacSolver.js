@@ -19,7 +19,7 @@ const path = require('path');
 const addon = require(path.join(__dirname, '..', 'lib', 'mac3d.node'));
 
 const FIELD = Object.freeze({ U: 0, V: 1, W: 2, P: 3, DYE: 4, CU: 5, CV: 6, CW: 7, RHS: 8,
-  MGX: 9, MGB: 10, MGR: 11, DYE_PRED: 12, WAVE: 13, PTOP: 14 });
+  MGX: 9, MGB: 10, MGR: 11, DYE_PRED: 12, WAVE: 13, PTOP: 14, CURL: 15 });
 const PRECISION = Object.freeze({ fp32: 0, mixed: 1, fp64: 2 });
 
 /**
@@ -64,7 +64,8 @@ class MacSolver {
    * Free-surface coupling: the wave solver and the lid pressure it feeds
    * (wave.ts:56-388, fluid.ts:16-78).  null disables it.
    * @param {{n?:number, halfExtent?:number, halfXZ?:number, depth?:number,
-   *          surfaceY?:number, marangoni?:number, scene?:number} | null} params
+   *          surfaceY?:number, marangoni?:number, scene?:number, vortConf?:number,
+   *          simTime?:number, schlieren?:number} | null} params
    */
   waveEnable(params) {
     addon.waveEnable(this._live(), params === undefined ? {} : params);

@@ -13,7 +13,7 @@ LIB_PATH = os.path.join(PKG, "lib", "libmac3d.so")
 F32, F64 = 0, 1
 PREC = {"fp32": 0, "mixed": 1, "fp64": 2}
 FIELD = dict(U=0, V=1, W=2, P=3, DYE=4, CU=5, CV=6, CW=7, RHS=8, MGX=9, MGB=10, MGR=11,
-             DYE_PRED=12, WAVE=13, PTOP=14)
+             DYE_PRED=12, WAVE=13, PTOP=14, CURL=15)
 STAGE = dict(FACE_TO_CENTRE=0, ADVECT=1, CASSON=2, DIVERGENCE=3, VCYCLE=4, PROJECT=5,
              DYE_PREDICT=6, DYE_RESOLVE=7, WAVE=8, TOP_PRESSURE=9)
 MG = dict(SMOOTH=0, RESIDUAL=1, RESTRICT=2, PROLONG=3, COARSE=4, ZERO=5, SMOOTH_ZERO=6)
@@ -39,7 +39,7 @@ class Blob(C.Structure):
 class WaveParams(C.Structure):
     _fields_ = [("n", C.c_int32), ("scene", C.c_int32)] + \
                [(n, C.c_double) for n in ("half_extent", "half_xz", "depth", "surface_y",
-                                           "marangoni")]
+                                           "marangoni", "vort_conf", "sim_time", "schlieren")]
 
 
 class Drop(C.Structure):
@@ -184,6 +184,8 @@ class MacSolver:
             return (n, n, 4)
         if fid == FIELD["PTOP"]:
             return (self.cfg.nz, self.cfg.nx)
+        if fid == FIELD["CURL"]:
+            return self.level_shape(0) + (4,)
         return self.level_shape(level)
 
     def download(self, field, level: int = 0, dtype=np.float64) -> np.ndarray:
@@ -228,7 +230,8 @@ class MacSolver:
     def wave_enable(self, enable: bool = True, **kw):
         """Enable the free-surface wave coupling (flat surface, no drops) with
         mac3d_wave_params fields as keywords (n, scene, half_extent, half_xz,
-        depth, surface_y, marangoni), or disable it."""
+        depth, surface_y, marangoni, vort_conf, sim_time, schlieren), or
+        disable it."""
         if not enable:
             self._check(self.L.mac3d_wave_enable(self.h, None))
             self.wave_n = 0

@@ -210,6 +210,9 @@ napi_value WaveEnable(napi_env env, napi_callback_info info) {
     prop_num(env, argv[1], "depth", &p.depth);
     prop_num(env, argv[1], "surfaceY", &p.surface_y);
     prop_num(env, argv[1], "marangoni", &p.marangoni);
+    prop_num(env, argv[1], "vortConf", &p.vort_conf);
+    prop_num(env, argv[1], "simTime", &p.sim_time);
+    prop_num(env, argv[1], "schlieren", &p.schlieren);
     double scene = 0.0;
     prop_num(env, argv[1], "scene", &scene);
     p.scene = scene != 0.0;

@@ -67,7 +67,10 @@ extern "C" {
                                (ix, iy) at 4*(iy*n + ix); needs mac3d_wave_enable */
 #define MAC3D_FIELD_PTOP 14 /* top Dirichlet pressure of the lid (dimensionless like P),
                                nx*nz values at iz*nx + ix; needs mac3d_wave_enable */
-#define MAC3D_FIELD_COUNT 15
+#define MAC3D_FIELD_CURL 15 /* scene only: curl of the post-advection centre velocity,
+                               4 per cell (omega x, y, z, |omega|; grid units, 1/s
+                               times h), FLUID_CURL (fluid.ts:331-369) */
+#define MAC3D_FIELD_COUNT 16
 
 /* single stages of the step, for kernel-level parity tests */
 #define MAC3D_STAGE_FACE_TO_CENTRE 0 /* CU,CV,CW <- U,V,W */
@@ -214,15 +217,24 @@ int mac3d_adjoint(void *ctx, const mac3d_blob *blobs, int nblobs, double dt, int
  * Lengths in metres; the tank spans [-half_xz, half_xz] in x and z. */
 typedef struct mac3d_wave_params {
   int32_t n;          /* wave texels per side, power of two in [16, 1024] (512) */
-  int32_t scene;      /* 1: the advection also takes the reference scene's terms
-                         (drop wetting-neck momentum, Marangoni pull and kinematic
-                         top from the wave, wall/floor sponges; fluid.ts:198-321),
-                         the tank top at surface_y; 0 (default): not */
+  int32_t scene;      /* 1: the step also takes the reference scene's terms: in
+                         the advection the drop wetting-neck momentum, Marangoni
+                         pull and kinematic top from the wave, wall/floor sponges
+                         (fluid.ts:198-321); in the Casson pass the vorticity
+                         confinement from the curl of the advected velocity, the
+                         solenoidal unfurling forcing and the +-1.8 m/s clamp
+                         (fluid.ts:331-509); the tank top at surface_y.
+                         0 (default): not */
   double half_extent; /* half-width of the wave domain (WAVE_HALF_EXTENT, 1.60) */
   double half_xz;     /* tank half-width (domainBounds.x; 0: nx*dx/2) */
   double depth;       /* tank depth (domainBounds.z - domainBounds.y; 0: ny*dx) */
   double surface_y;   /* still-surface height (surfaceParams.x, 0.30) */
   double marangoni;   /* Marangoni strength (surfaceParams.y, 0) */
+  double vort_conf;   /* scene: vorticity confinement scale (fluidParams.x, 1.02) */
+  double sim_time;    /* scene: simulation time of the unfurling phase
+                         (cameraPosAndTime.w, 0 s) */
+  double schlieren;   /* scene: schlierenStr, the unfurling phase offset
+                         (particleParams.x, 0.032) */
 } mac3d_wave_params;
 
 /* one drop slot (SimUniforms.drops[d], common.ts:26-43, packed by
@@ -247,9 +259,11 @@ int mac3d_set_drops(void *ctx, const mac3d_drop *drops, int ndrops);
 /* replaces: queue.writeBuffer of the packed 512-B SimUniforms block
  * (BloodDropEngine.ts:2602; layout common.ts:26-43, packed by
  * uniformLayout.ts:56-190) for what the coupling reads from it: count = 128
- * floats; surfaceParams.x/.y [40, 41] (surface height, Marangoni),
- * domainBounds [52..55] (half_xz = .x, depth = .z - .y) and the 8 drop slots
- * [64..127].  dt stays the step's argument.  ESTATE unless enabled. */
+ * floats; cameraPosAndTime.w [35] (simTime), surfaceParams.x/.y [40, 41]
+ * (surface height, Marangoni), fluidParams.x [44] (vorticity confinement),
+ * domainBounds [52..55] (half_xz = .x, depth = .z - .y), particleParams.x
+ * [60] (schlierenStr) and the 8 drop slots [64..127].  dt stays the step's
+ * argument.  ESTATE unless enabled. */
 int mac3d_set_uniforms(void *ctx, const float *block, size_t count);
 
 /* one stage of the step (MAC3D_STAGE_*), blocking */

@@ -27,7 +27,7 @@
 #define MAXLEV 8
 
 enum { F_U, F_V, F_W, F_P, F_DYE, F_CU, F_CV, F_CW, F_RHS, F_MGX, F_MGB,
-       F_MGR, F_DYE_PRED, F_WAVE, F_PTOP, F_COUNT };
+       F_MGR, F_DYE_PRED, F_WAVE, F_PTOP, F_CURL, F_COUNT };
 enum { ST_F2C, ST_ADVECT, ST_CASSON, ST_DIV, ST_VCYCLE, ST_PROJECT,
        ST_DYE_PRED, ST_DYE_RES, ST_WAVE, ST_TOP_PRESSURE };
 #define MAX_DROPS 8
@@ -49,6 +49,7 @@ struct orc_state {
   orc_wave_params wp;
   double *wave, *wz, *wz2, *wfilm, *ptop;
   orc_drop drops[MAX_DROPS];
+  double *curl; /* scene: curl of the advected centre velocity, 4 per cell */
 };
 
 #define IDX(D, i, j, k) ((((long)(k)) * (D).ny + (j)) * (long)(D).nx + (i))
@@ -943,6 +944,107 @@ static void advect_scene(const orc_state *s, int i, int j, int k, double dt, dou
   *vz *= sponge;
 }
 
+/* ---- FLUID_CURL (fluid.ts:331-369): omega = 0.5 (central differences of
+ * the cell-centred velocity), neighbour indices clamped to the grid (grid
+ * units, no 1/h), |omega| in the fourth slot. */
+static void curl_pass(orc_state *s) {
+  dims3 d = s->d[0];
+#pragma omp parallel for
+  for (int k = 0; k < d.nz; ++k)
+    for (int j = 0; j < d.ny; ++j)
+      for (int i = 0; i < d.nx; ++i) {
+        long R = IDX(d, clampi(i + 1, 0, d.nx - 1), j, k), L = IDX(d, clampi(i - 1, 0, d.nx - 1), j, k);
+        long U = IDX(d, i, clampi(j + 1, 0, d.ny - 1), k), D = IDX(d, i, clampi(j - 1, 0, d.ny - 1), k);
+        long F = IDX(d, i, j, clampi(k + 1, 0, d.nz - 1)), B = IDX(d, i, j, clampi(k - 1, 0, d.nz - 1));
+        double *o = s->curl + 4 * IDX(d, i, j, k);
+        o[0] = 0.5 * ((s->cw[U] - s->cw[D]) - (s->cv[F] - s->cv[B]));
+        o[1] = 0.5 * ((s->cu[F] - s->cu[B]) - (s->cw[R] - s->cw[L]));
+        o[2] = 0.5 * ((s->cv[R] - s->cv[L]) - (s->cu[U] - s->cu[D]));
+        o[3] = sqrt(o[0] * o[0] + o[1] * o[1] + o[2] * o[2]);
+      }
+}
+
+/* evalVortConf (fluid.ts:394-423): 0 within two cells of the side walls and
+ * floor and three of the lid, or where |omega| <= 0.18; else
+ * scale * smoothstep(0.18, 0.95, |omega|) / (1 + 0.28 |omega|) * (N x omega),
+ * N the normalised central-difference gradient of |omega| (0 below 1e-5). */
+static void vort_conf(const orc_state *s, int i, int j, int k, double scale, double f[3]) {
+  dims3 d = s->d[0];
+  f[0] = f[1] = f[2] = 0.0;
+  if (i <= 1 || i >= d.nx - 2 || j <= 1 || j >= d.ny - 3 || k <= 1 || k >= d.nz - 2) return;
+  const double *om = s->curl + 4 * IDX(d, i, j, k);
+  if (om[3] <= 0.18) return;
+  double gate = orc_smoothstep(0.18, 0.95, om[3]) / (1.0 + 0.28 * om[3]);
+  double e[3] = {0.5 * (s->curl[4 * IDX(d, i + 1, j, k) + 3] - s->curl[4 * IDX(d, i - 1, j, k) + 3]),
+                 0.5 * (s->curl[4 * IDX(d, i, j + 1, k) + 3] - s->curl[4 * IDX(d, i, j - 1, k) + 3]),
+                 0.5 * (s->curl[4 * IDX(d, i, j, k + 1) + 3] - s->curl[4 * IDX(d, i, j, k - 1) + 3])};
+  double el = sqrt(e[0] * e[0] + e[1] * e[1] + e[2] * e[2]);
+  if (el <= 1e-5) return;
+  for (int a = 0; a < 3; ++a) e[a] /= el;
+  f[0] = scale * gate * (e[1] * om[2] - e[2] * om[1]);
+  f[1] = scale * gate * (e[2] * om[0] - e[0] * om[2]);
+  f[2] = scale * gate * (e[0] * om[1] - e[1] * om[0]);
+}
+
+/* ---- FLUID_CASSON_FORCES scene terms (fluid.ts:467-509), after the
+ * viscous update and buoyancy: the confinement force averaged onto each face
+ * with the +1 neighbour cell's (:467-472), the deep-column solenoidal
+ * unfurling forcing at the cell centre (:488-503; the lysed-Hb channel of its
+ * gate is 0 with one dye species), the +-1.8 clamp and the wall faces
+ * zeroed (:505-507).  Positions as in advect_scene; dt in [0, 0.025]. */
+static void casson_scene(orc_state *s, double dt) {
+  dims3 d = s->d[0];
+  const double hx = s->wp.half_xz, ytop = s->wp.surface_y, y0 = ytop - s->wp.depth;
+  const double scale = s->wp.vort_conf;
+  const double ts = s->wp.sim_time * 0.36 + s->wp.schlieren * 1.7;
+  dt = clampd(dt, 0.0, 0.025);
+#pragma omp parallel for
+  for (int k = 0; k < d.nz; ++k)
+    for (int j = 0; j < d.ny; ++j)
+      for (int i = 0; i < d.nx; ++i) {
+        long c = IDX(d, i, j, k);
+        double f0[3], fx[3], fy[3], fz[3];
+        vort_conf(s, i, j, k, scale, f0);
+        vort_conf(s, clampi(i + 1, 0, d.nx - 1), j, k, scale, fx);
+        vort_conf(s, i, clampi(j + 1, 0, d.ny - 1), k, scale, fy);
+        vort_conf(s, i, j, clampi(k + 1, 0, d.nz - 1), scale, fz);
+        double m[3] = {s->u[c] + 0.5 * (f0[0] + fx[0]) * dt, s->v[c] + 0.5 * (f0[1] + fy[1]) * dt,
+                       s->w[c] + 0.5 * (f0[2] + fz[2]) * dt};
+        double px = -hx + (i + 0.5) / d.nx * 2.0 * hx, pz = -hx + (k + 0.5) / d.nz * 2.0 * hx;
+        double py = y0 + (j + 0.5) / d.ny * s->wp.depth;
+        double below = ytop - py;
+        double gate = sstep(0.11, 0.36, below) * sstep(0.0008, 0.015, clampd(s->dye[c], 0.0, 1.5));
+        if (gate > 0.001) {
+          double q1 = 6.4 * px + 4.0 * py + 5.2 * pz + ts;
+          double q2 = -5.6 * px + 4.8 * py + 7.2 * pz - ts * 0.77;
+          double q3 = 9.0 * px - 6.0 * py - 6.4 * pz + ts * 1.13;
+          double k1[3] = {-6.25, 2.85, 5.50}, k2[3] = {-6.30, -1.26, -4.06}, k3[3] = {6.40, -0.96, 9.90};
+          double a = 0.0028 * gate * dt;
+          for (int e = 0; e < 3; ++e) {
+            double sf = k1[e] * cos(q1) + k2[e] * cos(q2) + k3[e] * (0.42 * sin(q3));
+            if (e == 1) sf *= sstep(0.16, 0.42, below);
+            m[e] += sf * a;
+          }
+        }
+        for (int e = 0; e < 3; ++e) m[e] = clampd(m[e], -1.8, 1.8);
+        if (i >= d.nx - 1) m[0] = 0.0;
+        if (k >= d.nz - 1) m[2] = 0.0;
+        s->u[c] = m[0]; s->v[c] = m[1]; s->w[c] = m[2];
+      }
+}
+
+/* the Casson pass: with the scene, the centre velocity of the advected faces
+ * and its curl first (BloodDropEngine.ts:1922-1930), the scene terms last */
+static void casson_stage(orc_state *s, double dt) {
+  int scene = s->wave_on && s->wp.scene;
+  if (scene) {
+    face_to_centre(s);
+    curl_pass(s);
+  }
+  casson_buoyancy(s, dt);
+  if (scene) casson_scene(s, dt);
+}
+
 void orc_wave_enable(orc_state *s, const orc_wave_params *p) {
   if (!p) {
     s->wave_on = 0;
@@ -957,6 +1059,7 @@ void orc_wave_enable(orc_state *s, const orc_wave_params *p) {
     s->wfilm = (double *)calloc(nn, sizeof(double));
   }
   if (!s->ptop) s->ptop = (double *)calloc((long)s->d[0].nx * s->d[0].nz, sizeof(double));
+  if (p->scene && !s->curl) s->curl = (double *)calloc(4 * s->n[0], sizeof(double));
   s->wp = *p;
   if (!(s->wp.half_xz > 0.0)) s->wp.half_xz = 0.5 * s->p.nx * s->p.dx;
   if (!(s->wp.depth > 0.0)) s->wp.depth = s->p.ny * s->p.dx;
@@ -1005,7 +1108,7 @@ void orc_destroy(orc_state *s) {
   for (unsigned a = 0; a < sizeof(f0) / sizeof(f0[0]); ++a) free(f0[a]);
   for (int l = 1; l <= s->p.levels; ++l) { free(s->mgx[l]); free(s->mgb[l]); free(s->mgr[l]); }
   for (int a = 0; a < 4; ++a) free(s->cg[a]);
-  free(s->wave); free(s->wz); free(s->wz2); free(s->wfilm); free(s->ptop);
+  free(s->wave); free(s->wz); free(s->wz2); free(s->wfilm); free(s->ptop); free(s->curl);
   free(s);
 }
 
@@ -1027,6 +1130,7 @@ double *orc_field(orc_state *s, int field, int level) {
     case F_DYE_PRED: return s->dpred;
     case F_WAVE: return s->wave_on && level == 0 ? s->wave : NULL;
     case F_PTOP: return s->wave_on && level == 0 ? s->ptop : NULL;
+    case F_CURL: return level == 0 ? s->curl : NULL;
   }
   return NULL;
 }
@@ -1035,6 +1139,7 @@ long orc_field_len(orc_state *s, int field, int level) {
   if (field == F_MGX || field == F_MGB || field == F_MGR) return s->n[level];
   if (field == F_WAVE) return 4L * s->wp.n * s->wp.n;
   if (field == F_PTOP) return (long)s->d[0].nx * s->d[0].nz;
+  if (field == F_CURL) return 4L * s->n[0];
   return s->n[0];
 }
 
@@ -1072,7 +1177,7 @@ void orc_stage(orc_state *s, int stage, double dt) {
   switch (stage) {
     case ST_F2C: face_to_centre(s); break;
     case ST_ADVECT: advect_mac(s, dt); break;
-    case ST_CASSON: casson_buoyancy(s, dt); break;
+    case ST_CASSON: casson_stage(s, dt); break;
     case ST_DIV: divergence(s, s->u, s->v, s->w, s->rhs); break;
     case ST_VCYCLE: vcycle(s); break;
     case ST_PROJECT: project(s); break;
@@ -1106,9 +1211,10 @@ void orc_mg_op(orc_state *s, int op, int level, int count) {
 /* The step order of encodePhysicsStep (BloodDropEngine.ts:1886-2069) and
  * pass B's dye transport (:2096-2108): with the wave coupling, the wave step
  * and the lid pressure first (:1886-1914); then advect, Casson + buoyancy,
- * RHS, V-cycles, project, face->centre, dye.  The face->centre reconstruction
- * before advection (:1922-1925) equals the one after the previous step's
- * projection (:2067-2069), so it is computed once per step here. */
+ * RHS, V-cycles, project, face->centre, dye.  The advection reads the
+ * centre velocity of the previous step's projected faces (:2067-2069), formed
+ * here at the top of each step; the post-advection reconstruction
+ * (:1922-1925) feeds only the curl, inside casson_stage with the scene. */
 int orc_step(orc_state *s, double dt, int nsteps, double *rel_res) {
   int cycles = 0;
   double rr = 0.0;
@@ -1119,7 +1225,7 @@ int orc_step(orc_state *s, double dt, int nsteps, double *rel_res) {
     }
     face_to_centre(s);
     advect_mac(s, dt);
-    casson_buoyancy(s, dt);
+    casson_stage(s, dt);
     divergence(s, s->u, s->v, s->w, s->rhs);
     if (s->p.solve_to_tol) {
       rr = rel_residual(s);

@@ -37,8 +37,9 @@ typedef struct orc_blob {
 /* free-surface wave coupling (lengths in metres; 0 picks nx dx / 2 and
  * ny dx for half_xz and depth) and one drop slot, as in include/mac3d.h */
 typedef struct orc_wave_params {
-  int n, scene; /* scene: also the advection's scene terms */
+  int n, scene; /* scene: also the advection's and Casson pass's scene terms */
   double half_extent, half_xz, depth, surface_y, marangoni;
+  double vort_conf, sim_time, schlieren; /* Casson scene terms */
 } orc_wave_params;
 
 typedef struct orc_drop {

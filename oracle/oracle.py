@@ -16,7 +16,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 LIB = os.path.join(HERE, "liborc.so")
 
 (F_U, F_V, F_W, F_P, F_DYE, F_CU, F_CV, F_CW, F_RHS, F_MGX, F_MGB, F_MGR, F_DYE_PRED, F_WAVE,
- F_PTOP) = range(15)
+ F_PTOP, F_CURL) = range(16)
 (ST_F2C, ST_ADVECT, ST_CASSON, ST_DIV, ST_VCYCLE, ST_PROJECT, ST_DYE_PRED, ST_DYE_RES, ST_WAVE,
  ST_TOP_PRESSURE) = range(10)
 MG_SMOOTH, MG_RESIDUAL, MG_RESTRICT, MG_PROLONG, MG_COARSE, MG_ZERO = range(6)
@@ -36,7 +36,8 @@ class OrcBlob(C.Structure):
 
 class OrcWaveParams(C.Structure):
     _fields_ = [("n", C.c_int), ("scene", C.c_int)] + [(n, C.c_double) for n in (
-        "half_extent", "half_xz", "depth", "surface_y", "marangoni")]
+        "half_extent", "half_xz", "depth", "surface_y", "marangoni", "vort_conf", "sim_time",
+        "schlieren")]
 
 
 class OrcDrop(C.Structure):
@@ -136,6 +137,8 @@ class Oracle:
             shape = (w, w, 4)
         elif field == F_PTOP:
             shape = (self.shape[0], self.shape[2])
+        elif field == F_CURL:
+            shape = tuple(self.shape) + (4,)
         else:
             shape = self.shape
         return np.ctypeslib.as_array(ptr, shape=(n,)).reshape(shape)
@@ -168,14 +171,16 @@ class Oracle:
         return a.value, b.value
 
     def wave_enable(self, n=512, half_extent=1.60, half_xz=0.0, depth=0.0, surface_y=0.30,
-                    marangoni=0.0, scene=0, off=False):
+                    marangoni=0.0, scene=0, vort_conf=1.02, sim_time=0.0, schlieren=0.032,
+                    off=False):
         """enable (flat surface, no drops) or, off=True, disable the wave coupling"""
         if off:
             lib().orc_wave_enable(self.h, None)
             return
         self.wave_params = OrcWaveParams(n=n, scene=int(scene), half_extent=half_extent,
                                          half_xz=half_xz, depth=depth, surface_y=surface_y,
-                                         marangoni=marangoni)
+                                         marangoni=marangoni, vort_conf=vort_conf,
+                                         sim_time=sim_time, schlieren=schlieren)
         lib().orc_wave_enable(self.h, C.byref(self.wave_params))
 
     def set_drops(self, drops):

@@ -217,3 +217,90 @@ def test_scene_coupled_steps_against_oracle():
             assert e <= 2e-5, f"{f}: rel L2 {e:.3e}"
         _cmp_wave(s, o, TOLW)
     o.close()
+
+
+DEEP = dict(half_extent=0.5, half_xz=0.2, depth=0.8, surface_y=0.0, marangoni=0.3,
+            vort_conf=1.4, sim_time=0.9, schlieren=0.05)
+
+
+def deep_scene(scene=1, prec=0):
+    """A deep tank (the unfurling gate opens 0.11 m under the surface) with a
+    swirling velocity whose curl is 0.25-1.26 in grid units on 98% of the
+    cells (so the confinement acts nearly everywhere) and random dye."""
+    w, s, o = pair(precision=prec, warm_steps=0)
+    nz, ny, nx = o.shape
+    k, j, i = np.meshgrid(np.arange(nz), np.arange(ny), np.arange(nx), indexing="ij")
+    fields = {"U": 0.9 * np.sin(2 * np.pi * j / 9.0) * np.cos(2 * np.pi * k / 11.0),
+              "V": 0.9 * np.sin(2 * np.pi * k / 7.0 + 0.3) * np.cos(2 * np.pi * i / 10.0),
+              "W": 0.9 * np.sin(2 * np.pi * i / 8.0 + 0.5) * np.cos(2 * np.pi * j / 12.0),
+              "DYE": np.clip(np.random.default_rng(3).random(o.shape) * 1.2 - 0.1, 0, 1.2)}
+    for f, a in fields.items():
+        o.set(getattr(oracle, "F_" + f), a)
+        s.upload(f, a)
+    s.wave_enable(n=64, scene=scene, **DEEP)
+    o.wave_enable(n=64, scene=scene, **DEEP)
+    return w, s, o
+
+
+def test_scene_casson_against_oracle():
+    """The Casson pass with the scene terms (fluid.ts:331-509): the curl of
+    the advected centre velocity, then the vorticity confinement, unfurling
+    forcing and clamp after the viscous update.  The scene's own increment
+    (scene on minus scene off, about 1% of the velocity) is compared by
+    itself, so FP32 rounding of the velocity does not hide it."""
+    dt = 0.02
+    got = {}
+    for scene in (1, 0):
+        w, s, o = deep_scene(scene)
+        with s:
+            s.stage("CASSON", dt)
+            o.stage(oracle.ST_CASSON, dt)
+            got[scene] = {f: (s.download(f), o.get(getattr(oracle, "F_" + f)))
+                          for f in ("U", "V", "W")}
+            if scene:
+                c_dev, c_orc = s.download("CURL"), o.get(oracle.F_CURL)
+                assert rel_l2(c_dev, c_orc) <= 2e-6, "curl"
+        o.close()
+    for f in ("U", "V", "W"):
+        (d1, o1), (d0, o0) = got[1][f], got[0][f]
+        assert rel_l2(d1, o1) <= TOLW, f
+        inc_d, inc_o = d1 - d0, o1 - o0
+        assert np.linalg.norm(inc_o) > 3e-3 * np.linalg.norm(o0), f
+        e = rel_l2(inc_d, inc_o)
+        assert e <= 1e-4, f"{f} scene increment: rel L2 {e:.3e}"
+
+
+def test_scene_deep_steps_against_oracle():
+    """Three full FP32 steps of the deep swirling set-up with every scene
+    term on (wave, lid pressure, advection and Casson scene terms)."""
+    w, s, o = deep_scene(1)
+    with s:
+        s.step(w.dt, 3)
+        o.step(w.dt, 3)
+        for f in ("U", "V", "W", "P", "DYE"):
+            e = rel_l2(s.download(f), o.get(getattr(oracle, "F_" + f)))
+            assert e <= 2e-5, f"{f}: rel L2 {e:.3e}"
+    o.close()
+
+
+def test_scene_clock_from_the_uniform_block():
+    """simTime [35], fluidParams.x [44] and particleParams.x [60] of the
+    packed block set the Casson scene terms as the same values given to
+    mac3d_wave_enable do (device memory: a captured step sees them)."""
+    w, s1, o = deep_scene(1)
+    o.close()
+    w, s2, o = deep_scene(1)
+    o.close()
+    blk = np.zeros(128, dtype=np.float32)
+    blk[35], blk[44], blk[60] = 2.5, 0.7, 0.11
+    blk[40], blk[41] = DEEP["surface_y"], DEEP["marangoni"]
+    blk[52], blk[53], blk[54] = DEEP["half_xz"], -DEEP["depth"], 0.0
+    with s1, s2:
+        s1.set_uniforms(blk)
+        s2.wave_enable(n=64, scene=1, **dict(DEEP, sim_time=float(blk[35]),
+                                             vort_conf=float(blk[44]),
+                                             schlieren=float(blk[60])))
+        for s in (s1, s2):
+            s.stage("CASSON", 0.02)
+        for f in ("U", "V", "W"):
+            assert np.array_equal(s1.download(f), s2.download(f)), f

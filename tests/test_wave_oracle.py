@@ -323,3 +323,145 @@ def test_scene_neck_draft_on_the_drop_axis():
         blend = min(max(dt * (14 + 8 * slug) * mask, 0), 1)
         want = (vb[8, j, 8] * (1 - blend) + target * blend) * sp[8, j, 8]
         assert va[8, j, 8] == pytest.approx(want, rel=1e-12, abs=1e-15)
+
+
+# ---------------------------------------------------------------- Casson scene terms
+def casson_scene_oracle(scene=1, nx=16, ny=24, nz=16, geom=None, **clock):
+    """an oracle with the wave coupling on a deep tank (the unfurling gate is
+    open below 0.36 m under the surface), zero dye and velocity"""
+    o = make(nx=nx, ny=ny, nz=nz)
+    g = dict(n=32, half_extent=0.5, half_xz=0.1, depth=0.6, surface_y=0.0, marangoni=0.0)
+    g.update(geom or {})
+    o.wave_enable(scene=scene, **g, **clock)
+    for f in (oracle.F_U, oracle.F_V, oracle.F_W, oracle.F_DYE):
+        o.set(f, np.zeros(o.shape))
+    return o
+
+
+def test_curl_of_a_rigid_rotation_is_twice_the_rate():
+    """Faces sampled from the rigid rotation v = Omega x r (grid units) give
+    centre velocities on the same rotation away from the walls, whose
+    central-difference curl is exactly 2 Omega with |omega| = 2 |Omega|
+    (FLUID_CURL, fluid.ts:331-369)."""
+    o = casson_scene_oracle()
+    nz, ny, nx = o.shape
+    Om = np.array([0.03, -0.02, 0.05])
+    c0 = np.array([nx / 2.0, ny / 2.0, nz / 2.0])
+    k, j, i = np.meshgrid(np.arange(nz), np.arange(ny), np.arange(nx), indexing="ij")
+
+    def rot(x, y, z):
+        r = np.stack([x - c0[0], y - c0[1], z - c0[2]], axis=-1)
+        return np.cross(Om, r)
+    o.set(oracle.F_U, rot(i + 1.0, j + 0.5, k + 0.5)[..., 0])  # face positions
+    o.set(oracle.F_V, rot(i + 0.5, j + 1.0, k + 0.5)[..., 1])
+    o.set(oracle.F_W, rot(i + 0.5, j + 0.5, k + 1.0)[..., 2])
+    o.stage(oracle.ST_CASSON, 0.002)
+    cu = o.get(oracle.F_CURL)[2:-2, 2:-2, 2:-2]
+    assert np.allclose(cu[..., :3], 2 * Om, rtol=0, atol=1e-13)
+    assert np.allclose(cu[..., 3], 2 * np.linalg.norm(Om), rtol=0, atol=1e-13)
+
+
+def test_curl_clamps_its_neighbours_at_the_walls():
+    """v_y = a x (grid units) on every y-face: omega = (0, 0, a) inside and
+    a / 2 on the two x-wall columns, where the clamped neighbour turns the
+    central difference one-sided (loadVelC, fluid.ts:338-343)."""
+    o = casson_scene_oracle()
+    nz, ny, nx = o.shape
+    a = 0.07
+    o.set(oracle.F_V, np.broadcast_to(a * (np.arange(nx) + 0.5), o.shape).copy())
+    o.stage(oracle.ST_CASSON, 0.002)
+    cu = o.get(oracle.F_CURL)[:, 1:, :]  # (the floor row's centre sees the solid face)
+    want = np.full(nx, a)
+    want[0] = want[-1] = a / 2
+    assert np.allclose(cu[..., 2], want, rtol=0, atol=1e-14)
+    assert np.allclose(cu[..., :2], 0.0, atol=1e-14)
+
+
+def test_vorticity_confinement_closed_form():
+    """v_y = a xc^2 - 1.5 with xc the cell's x offset from the middle:
+    omega = (0, 0, 2 a xc), N = (sign xc, 0, 0) and the confinement pushes
+    each y-face by -dt scale gate(|omega|) |omega| with
+    gate(m) = smoothstep(0.18, 0.95, m) / (1 + 0.28 m) (evalVortConf,
+    fluid.ts:394-423), averaged with the cell above (:470), so the rows
+    next to the inactive ones take half; nothing on the x- and z-faces.  The scene's other terms are off (no dye, no clamp reached)."""
+    dt, scale, a = 0.004, 1.3, 0.05
+    got = []
+    for scene in (1, 0):
+        o = casson_scene_oracle(scene=scene, vort_conf=scale)
+        nz, ny, nx = o.shape
+        xc = np.arange(nx) + 0.5 - nx / 2
+        o.set(oracle.F_V, np.broadcast_to(a * xc ** 2 - 1.5, o.shape).copy())
+        o.stage(oracle.ST_CASSON, dt)
+        got.append([o.get(f) for f in (oracle.F_U, oracle.F_V, oracle.F_W)])
+    du, dv, dw = (s - n for s, n in zip(*got))
+    assert np.all(du == 0) and np.all(dw == 0)
+    m = 2 * a * np.abs(xc)
+    f = np.array([-scale * oracle.smoothstep(0.18, 0.95, x) / (1 + 0.28 * x) * x
+                  if 2 <= i <= nx - 3 else 0.0 for i, x in enumerate(m)])
+    assert np.count_nonzero(f) >= 8
+    zin = slice(2, nz - 2)
+    assert np.allclose(dv[zin, 2:ny - 4, :], dt * f, rtol=1e-12, atol=1e-16)
+    for j in (1, ny - 4):  # one of the two cells of the face is inactive
+        assert np.allclose(dv[zin, j, :], 0.5 * dt * f, rtol=1e-12, atol=1e-16)
+    assert np.all(dv[:, 0, :] == 0) and np.all(dv[:, ny - 3:, :] == 0)
+    assert np.all(dv[:2] == 0) and np.all(dv[nz - 2:] == 0)
+
+
+def _unfurl_increment(**clock):
+    o = casson_scene_oracle(nx=32, ny=48, nz=32, geom=dict(half_xz=0.2), **clock)
+    o.set(oracle.F_DYE, np.ones(o.shape))
+    ref = casson_scene_oracle(scene=0, nx=32, ny=48, nz=32, geom=dict(half_xz=0.2))
+    ref.set(oracle.F_DYE, np.ones(ref.shape))
+    for s in (o, ref):
+        s.stage(oracle.ST_CASSON, 0.01)
+    return [o.get(f) - ref.get(f) for f in (oracle.F_U, oracle.F_V, oracle.F_W)]
+
+
+def test_unfurling_forcing_is_solenoidal():
+    """Deep in the column (gate and vertical modulation 1) the unfurling
+    forcing is the sum of three transverse plane waves (k_i . d_i = 0,
+    fluid.ts:488-503), so its central-difference divergence at the cell
+    centres vanishes to O((h k)^2) against each term's size; a wrong sign or
+    component in any polarisation leaves an O(1) divergence."""
+    du, dv, dw = _unfurl_increment(sim_time=0.7)
+    nz, ny, nx = du.shape
+    hx, hy = 0.4 / nx, 0.6 / ny
+    # centres 0.43 m or more below the surface, two cells from the x/z walls
+    # (whose last faces are zeroed)
+    jt = max(j for j in range(ny) if 0.6 - (j + 0.5) * hy >= 0.43)
+    K, J, I = slice(2, nz - 2), slice(1, jt), slice(2, nx - 2)
+
+    def cdiff(f, axis, h):
+        return (np.roll(f, -1, axis) - np.roll(f, 1, axis)) / (2 * h)
+    terms = [cdiff(du, 2, hx)[K, J, I], cdiff(dv, 1, hy)[K, J, I], cdiff(dw, 0, hx)[K, J, I]]
+    div = sum(terms)
+    size = np.sqrt(sum(np.mean(t ** 2) for t in terms))
+    assert size > 0
+    assert np.sqrt(np.mean(div ** 2)) <= 0.01 * size
+
+
+def test_unfurling_phase_takes_time_and_schlieren_together():
+    """The forcing's phase is 0.36 simTime + 1.7 schlierenStr (fluid.ts:492):
+    a schlieren offset s equals a time shift of 1.7 s / 0.36."""
+    a = _unfurl_increment(sim_time=0.5, schlieren=0.2)
+    b = _unfurl_increment(sim_time=0.5 + 1.7 * 0.2 / 0.36, schlieren=0.0)
+    c = _unfurl_increment(sim_time=0.5, schlieren=0.0)
+    for x, y, z in zip(a, b, c):
+        assert np.allclose(x, y, rtol=1e-9, atol=1e-15)
+        assert not np.allclose(x, z, rtol=1e-3, atol=0)
+
+
+def test_casson_scene_clamps_and_keeps_walls():
+    """Faces beyond 1.8 m/s come out at +-1.8 (fluid.ts:505) and the wall
+    faces at 0 (:506-507); without the scene the clamp is not applied."""
+    for scene in (1, 0):
+        o = casson_scene_oracle(scene=scene)
+        o.set(oracle.F_U, np.full(o.shape, 2.5))
+        o.set(oracle.F_V, np.full(o.shape, -3.0))
+        o.stage(oracle.ST_CASSON, 0.002)
+        u, v = o.get(oracle.F_U), o.get(oracle.F_V)
+        assert np.all(u[:, :, -1] == 0)
+        if scene:
+            assert u[:, :, :-1].max() == 1.8 and v.min() == -1.8
+        else:
+            assert u.max() > 1.8 and v.min() < -1.8
