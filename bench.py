@@ -8,8 +8,9 @@ fields resident in HBM.  The same line carries the mixed-precision run
 (FP64 refinement to ||r||/||b|| <= 1e-8, FP16 level>=2 storage) and its
 speed-up, the smoother's roofline figures (HIP events on the solver's
 stream), the FP32 step with the free-surface wave coupling on (a 512 x 512
-wave grid, two active drops) and the FP64 CPU oracle timed on this host as
-the CPU baseline.
+wave grid, two active drops), the same with the reference scene's advection
+and Casson terms, and the FP64 CPU oracle timed on this host as the CPU
+baseline.
 
 python bench.py [--gpus 1] [--steps K] [--warmup W]
 """
@@ -50,14 +51,14 @@ def log(msg):
 PROFILE_STEPS = 3  # steps timed kernel by kernel (HIP events) after the timed run
 
 
-def run_gpu(workload, steps, warmup, profile_steps=PROFILE_STEPS, wave=False):
+def run_gpu(workload, steps, warmup, profile_steps=PROFILE_STEPS, wave=False, scene=False):
     from mac3d import MacSolver
     from mac3d.lib import config_for
     cfg = config_for(workload)
     with MacSolver(cfg) as s:
         s.init_blobs(workload.blobs)
         if wave:
-            s.wave_enable(n=WAVE_N, marangoni=0.8)
+            s.wave_enable(n=WAVE_N, marangoni=0.8, scene=int(scene))
             s.set_drops(WAVE_DROPS)
         if warmup:
             s.step_nostats(workload.dt, warmup)
@@ -157,6 +158,20 @@ def main():
                 "wave_gbps": round(wbytes / (wave_us * 1e-6) / 1e9, 1) if wave_us else None,
                 "max_div": stw["max_div"]}
         log(f"[bench] wave-coupled fp32 {spsw:.2f} steps/s, wave step {wave['wave_us_per_step']} us")
+        # the same with the reference scene's terms in the advection and the
+        # Casson pass (curl pass, vorticity confinement, unfurling forcing)
+        sts, walls, profs = run_gpu(w32, args.steps, args.warmup, wave=True, scene=True)
+
+        def per_step_us(prof, name):
+            ms, n = prof.get(name, (0.0, 0))
+            return round(1e3 * ms / PROFILE_STEPS, 2) if n else None
+        wave["scene"] = {"steps_per_s": round(args.steps / walls, 3),
+                         "advect_us_per_step": per_step_us(profs, "ADVECT"),
+                         "casson_us_per_step": per_step_us(profs, "CASSON"),
+                         "advect_us_per_step_no_scene": per_step_us(profw, "ADVECT"),
+                         "casson_us_per_step_no_scene": per_step_us(profw, "CASSON"),
+                         "max_div": sts["max_div"]}
+        log(f"[bench] scene-term fp32 {wave['scene']['steps_per_s']:.2f} steps/s")
 
     roof = None
     if "SMOOTH2_L0" in prof:

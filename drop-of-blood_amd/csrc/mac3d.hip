@@ -298,6 +298,17 @@ WaveParams wave_params_dt(const Ctx *ctx, double dt) {
   return p;
 }
 
+// the drop slots' allocation: WV_MAX_DROPS WaveDrop, the SceneClock, then
+// WV_MAX_DROPS DropPhase
+const size_t kDropBytes =
+    WV_MAX_DROPS * sizeof(WaveDrop) + sizeof(SceneClock) + WV_MAX_DROPS * sizeof(DropPhase);
+const SceneClock *scene_clock(const Ctx *ctx) {
+  return (const SceneClock *)(ctx->drops + WV_MAX_DROPS);
+}
+const DropPhase *drop_phases(const Ctx *ctx) {
+  return (const DropPhase *)(scene_clock(ctx) + 1);
+}
+
 int launch_advect(Ctx *ctx, double dt) {
   const Dims &d = ctx->d[0];
   {
@@ -310,18 +321,14 @@ int launch_advect(Ctx *ctx, double dt) {
       hipLaunchKernelGGL(kern, gather_grid(d), kGatherBlock, 0, ctx->stream, d,
                          (T)(dt / ctx->cfg.dx), (T)ctx->cfg.vel_damp, (const T *)ctx->u.ptr, (const T *)ctx->v.ptr,
                          (const T *)ctx->w.ptr, (const typename cvec_of<T>::type *)ctx->cvel.ptr,
-                         (T *)ctx->u2.ptr, (T *)ctx->v2.ptr,
-                         (T *)ctx->w2.ptr, wave_params_dt(ctx, dt), (const WaveDrop *)ctx->drops,
+                         (T *)ctx->u2.ptr, (T *)ctx->v2.ptr, (T *)ctx->w2.ptr,
+                         wave_params_dt(ctx, dt), (const WaveDrop *)ctx->drops, drop_phases(ctx),
                          (const float4 *)ctx->wave.ptr);
     });
   }
   LAUNCHCHK();
   swap_vel(ctx);
   return MAC3D_OK;
-}
-
-const SceneClock *scene_clock(const Ctx *ctx) {
-  return (const SceneClock *)(ctx->drops + WV_MAX_DROPS);
 }
 
 // The scene's Casson terms (BloodDropEngine.ts:1922-1934): the centre velocity
@@ -1828,8 +1835,7 @@ int mac3d_wave_enable(void *h, const mac3d_wave_params *p) {
   }
   if (!ctx->ptop.ptr && (rc = alloc(ctx, ctx->ptop, TF32, (size_t)ctx->cfg.nx * ctx->cfg.nz)))
     return rc;
-  const size_t dsz = WV_MAX_DROPS * sizeof(WaveDrop) + sizeof(SceneClock);
-  if (!ctx->drops && (rc = dev_alloc(ctx, (void **)&ctx->drops, dsz))) return rc;
+  if (!ctx->drops && (rc = dev_alloc(ctx, (void **)&ctx->drops, kDropBytes))) return rc;
   HIPCHK(hipMemsetAsync(ctx->drops, 0, WV_MAX_DROPS * sizeof(WaveDrop), ctx->stream));
   if (p->scene && !ctx->curl.ptr) {
     const size_t n4 = 4 * ctx->n[0];
@@ -1866,7 +1872,9 @@ int mac3d_set_drops(void *h, const mac3d_drop *drops, int ndrops) {
     return fail(ctx, MAC3D_EINVAL, "need 0..MAC3D_MAX_DROPS drops");
   if (!ctx->wave_on) return fail(ctx, MAC3D_ESTATE, "the wave coupling is not enabled");
   WaveDrop hd[WV_MAX_DROPS];
+  DropPhase hp[WV_MAX_DROPS];
   std::memset(hd, 0, sizeof(hd));
+  std::memset(hp, 0, sizeof(hp));
   for (int q = 0; q < ndrops; ++q) {
     const mac3d_drop &a = drops[q];
     const double v[8] = {a.x, a.y, a.z, a.radius, a.vel_y, a.osc, a.drain, a.seed};
@@ -1874,9 +1882,17 @@ int mac3d_set_drops(void *h, const mac3d_drop *drops, int ndrops) {
       if (!std::isfinite(x)) return fail(ctx, MAC3D_EINVAL, "drop values must be finite");
     hd[q] = WaveDrop{(float)a.x, (float)a.y, (float)a.z, (float)a.radius,
                      (float)a.vel_y, (float)a.osc, (float)a.drain, (float)a.seed};
+    const double sd = hd[q].seed, dr = hd[q].drain;
+    hp[q] = DropPhase{(float)std::cos(1.73 * sd), (float)std::sin(1.73 * sd),
+                      (float)std::cos(-2.31 * sd), (float)std::sin(-2.31 * sd),
+                      (float)std::cos(0.89 * sd), (float)std::sin(0.89 * sd),
+                      (float)std::min(std::max(std::pow(std::max(dr, 0.0), 0.66) * 0.84, 0.0), 1.24),
+                      0.f};
   }
   // (the captured step sequences read the slots from device memory)
   HIPCHK(hipMemcpyAsync(ctx->drops, hd, sizeof(hd), hipMemcpyHostToDevice, ctx->stream));
+  HIPCHK(hipMemcpyAsync((void *)drop_phases(ctx), hp, sizeof(hp), hipMemcpyHostToDevice,
+                        ctx->stream));
   HIPCHK(hipStreamSynchronize(ctx->stream));
   return MAC3D_OK;
 }

@@ -105,15 +105,15 @@ __device__ inline void trace_back_tp(const Dims &d, const CR &cvel, T dth, const
 // against 264 us at 192x256x192; 8 waves forces too few registers and was
 // slower.  FP64 needs its registers.)  BUF: fields read through buffer
 // resources with 32-bit byte offsets (fields under 2^31 bytes, planes under
-// 2^24).
-// SCENE: the free-surface coupling's scene terms (advect_scene_terms) from
-// the drop slots and the wave state.
+// 2^24).  SCENE: the coupling's scene terms (advect_scene_terms) from the
+// drop slots and the wave state.
 template <class T, bool BUF, bool SCENE>
 __global__ void __launch_bounds__(256, sizeof(T) == 4 ? 6 : 1)
 k_advect_mac(Dims d, T dth, T damp, const T *__restrict__ u, const T *__restrict__ v,
              const T *__restrict__ w, const typename cvec_of<T>::type *__restrict__ cvel,
              T *__restrict__ uo, T *__restrict__ vo, T *__restrict__ wo, WaveParams wp,
-             const WaveDrop *__restrict__ drops, const float4 *__restrict__ wave) {
+             const WaveDrop *__restrict__ drops, const DropPhase *__restrict__ phs,
+             const float4 *__restrict__ wave) {
   MAC3D_CELL_PROLOGUE
   const size_t n = d.cells();
   const auto cv = make_reader<BUF>(cvel, n);
@@ -144,7 +144,8 @@ k_advect_mac(Dims d, T dth, T damp, const T *__restrict__ u, const T *__restrict
     tpos_clamp(b, lo1, hi1);
     res[a] = sample_tp_r<T>(rf[a], d, b) * damp;
   }
-  if constexpr (SCENE) advect_scene_terms<T>(d, i, j, k, wp, drops, wave, res[0], res[1], res[2]);
+  if constexpr (SCENE)
+    advect_scene_terms<T>(d, i, j, k, wp, drops, phs, wave, res[0], res[1], res[2]);
   uo[c] = (i >= d.nx - 1) ? T(0) : res[0];
   vo[c] = res[1];
   wo[c] = (k >= d.nz - 1) ? T(0) : res[2];

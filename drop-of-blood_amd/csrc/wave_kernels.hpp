@@ -28,6 +28,14 @@ struct WaveDrop {
   float vy, osc, drain, seed;
 };
 
+// per-drop constants of the advection's neck terms, formed on the host when
+// the drops are set: (cos, sin) of the azimuthal phases 1.73 seed,
+// -2.31 seed and 0.89 seed, and the slug drive
+// clamp(max(drain, 0)^0.66 * 0.84, 0, 1.24) (fluid.ts:211-240)
+struct DropPhase {
+  float c1, s1, c2, s2, c3, s3, slug_drive, pad;
+};
+
 struct WaveParams {
   int n;            // texels per side (power of two)
   float ext;        // wave domain half-width E (m)
@@ -427,9 +435,13 @@ k_top_pressure(WaveParams P, const WaveDrop *__restrict__ drops, const float4 *_
 // circular wall and floor sponges.  World positions: the tank [-hx, hx] in x
 // and z, [ytop - depth, ytop] in y with ytop the surface height (uvwToWorld,
 // common.ts:45-64, with domainBounds.z = surfaceParams.x as in the scene).
+// The azimuthal terms cos(m theta + phase) are formed from the unit direction
+// (cos theta, sin theta) by complex powers and the drop's phase factors, which
+// equals the reference's atan2-and-cosine form without per-cell trigonometry.
 template <class T>
 __device__ inline void advect_scene_terms(const Dims &d, int i, int j, int k, const WaveParams &P,
                                           const WaveDrop *__restrict__ drops,
+                                          const DropPhase *__restrict__ phs,
                                           const float4 *__restrict__ wave, T &vx, T &vy, T &vz) {
   const T hx = (T)P.half_xz, ytop = (T)P.surface_y, depth = (T)P.depth, y0 = ytop - depth;
   const T nx = (T)d.nx, ny = (T)d.ny, nz = (T)d.nz;
@@ -444,9 +456,27 @@ __device__ inline void advect_scene_terms(const Dims &d, int i, int j, int k, co
     T t = clampv((x - e0) / (e1 - e0), T(0), T(1));
     return t * t * (T(3) - T(2) * t);
   };
-  auto azim_of = [](T th, T seed) {
-    return T(1) + T(0.036) * cos(T(4) * th + seed * T(1.73)) +
-           T(0.026) * cos(T(6) * th - seed * T(2.31)) + T(0.018) * sin(T(8) * th + seed * T(0.89));
+  // 1 + 0.036 cos(4 th + 1.73 seed) + 0.026 cos(6 th - 2.31 seed) [the lobe],
+  // plus 0.018 sin(8 th + 0.89 seed) [the full azimuthal factor], from the
+  // unit direction (c, s) of th
+  auto azim_of = [](T c, T s, const DropPhase &ph, T &lobe) {
+    const T c2 = c * c - s * s, s2 = T(2) * c * s;
+    const T c4 = c2 * c2 - s2 * s2, s4 = T(2) * c2 * s2;
+    const T c6 = c4 * c2 - s4 * s2, s6 = s4 * c2 + c4 * s2;
+    const T c8 = c4 * c4 - s4 * s4, s8 = T(2) * c4 * s4;
+    lobe = T(1) + T(0.036) * (c4 * (T)ph.c1 - s4 * (T)ph.s1) +
+           T(0.026) * (c6 * (T)ph.c2 - s6 * (T)ph.s2);
+    return lobe + T(0.018) * (s8 * (T)ph.c3 + c8 * (T)ph.s3);
+  };
+  // (cos, sin) of atan2(b, a); atan2(0, 0) = 0
+  auto unit_of = [](T a, T b, T r, T &c, T &s) {
+    if (r > T(0)) {
+      c = a / r;
+      s = b / r;
+    } else {
+      c = T(1);
+      s = T(0);
+    }
   };
   auto radial_of = [&](T rr, T slug) {
     const T sl = (T(0.14) + T(1) * fmin(rr * rr, T(1.20))) * (T(1) - ss(T(0.78), T(1.15), rr));
@@ -454,27 +484,31 @@ __device__ inline void advect_scene_terms(const Dims &d, int i, int j, int k, co
     return mixv(st, sl, slug);
   };
   T active = T(0);
-  for (int q = 0; q < WV_MAX_DROPS; ++q) {
+  // below surf - 0.058 both neck masks are 0, and `active` is read only on the
+  // top two rows: skip the drops there
+  const bool necks = fy_y > surf - T(0.058) || j >= d.ny - 2;
+  for (int q = 0; necks && q < WV_MAX_DROPS; ++q) {
     const WaveDrop s = drops[q];
     if (!(s.seed > 0.f && s.drain > 0.0005f)) continue;
-    const T dx0 = (T)s.x, dz0 = (T)s.z, seed = (T)s.seed, drain = (T)s.drain;
+    const DropPhase ph = phs[q];
+    const T dx0 = (T)s.x, dz0 = (T)s.z, drain = (T)s.drain;
     const T ex = cx - dx0, ez = cz - dz0;  // (the y-face shares the cell's x and z)
     const T r = sqrt(ex * ex + ez * ez);
-    const T th = atan2(ez, ex);
+    T cth, sth, lobe;
+    unit_of(ex, ez, r, cth, sth);
+    const T azim = azim_of(cth, sth, ph, lobe);
     const T slug = ss(T(0.14), T(0.76), drain);
     const T rbase = clampv((T)s.r * mixv(T(0.80), T(1.08), slug) + mixv(T(0.012), T(0.030), slug),
                            T(0.018), T(0.088));
-    const T rr = r / fmax(rbase * azim_of(th, seed), T(1e-4));
+    const T rr = r / fmax(rbase * azim, T(1e-4));
     const T radial = radial_of(rr, slug);
     const T vmask = ss(surf - T(0.058), surf - T(0.014), fy_y);
     const T shear = mixv(T(1), T(0.36), ss(surf - T(0.024), surf - T(0.002), fy_y));
     const T mask = radial * vmask;
     active = fmax(active, radial);
     if (mask > T(0.001)) {
-      const T lobe = T(1) + T(0.036) * cos(T(4) * th + seed * T(1.73)) +
-                     T(0.026) * cos(T(6) * th - seed * T(2.31));
       const T impact = clampv(-(T)s.vy, T(0.10), T(0.65));
-      const T slug_drive = clampv(pow(fmax(drain, T(0)), T(0.66)) * T(0.84), T(0), T(1.24));
+      const T slug_drive = (T)ph.slug_drive;
       const T stem_drive = clampv(T(0.045) + drain * T(0.22), T(0), T(0.20)) * ss(T(0), T(0.05), drain);
       const T flux = mixv(stem_drive, slug_drive, slug);
       const T target = -clampv(T(0.14) + T(0.64) * impact, T(0.15), T(0.39)) * flux * lobe * shear;
@@ -492,7 +526,9 @@ __device__ inline void advect_scene_terms(const Dims &d, int i, int j, int k, co
         const T ax = fxs[f], az = fzs[f];
         const T rf = sqrt(ax * ax + az * az);
         if (rf > T(1e-4) && rf < rbase * T(1.34)) {
-          const T rrf = rf / fmax(rbase * azim_of(atan2(az, ax), seed), T(1e-4));
+          T cf, sf, lobe_f;
+          unit_of(ax, az, rf, cf, sf);
+          const T rrf = rf / fmax(rbase * azim_of(cf, sf, ph, lobe_f), T(1e-4));
           if (rrf < T(1.22)) {
             const T mf = radial_of(rrf, slug) * vmask_xz;
             const T bf = clampv(dt * mixv(T(14), T(22), slug) * mf, T(0), T(1));
