@@ -8,9 +8,9 @@
  * Every function follows one WGSL kernel of the reference, cited at its
  * definition in mac3d_oracle.c, with the documented changes the MI355X
  * workload makes (SI Casson/Boussinesq, no-slip walls, exact coarse solve).
- * Parity unpinned: advect_scene's radial flare off a drop's axis (its
- * azimuthally modulated profile has no closed form or invariant the tests
- * can use; only the device parity checks it).
+ * Parity unpinned: advect_scene's azimuthal modulation between the lines
+ * through a drop's axis along x and z (its value there is pinned in closed
+ * form; in between only the device parity checks it).
  * Arrays are cell-major with x fastest: idx = (k*ny + j)*nx + i.
  */
 #ifndef MAC3D_ORACLE_H

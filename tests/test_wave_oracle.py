@@ -325,6 +325,59 @@ def test_scene_neck_draft_on_the_drop_axis():
         assert va[8, j, 8] == pytest.approx(want, rel=1e-12, abs=1e-15)
 
 
+def test_scene_flare_along_the_drop_axis():
+    """The x-faces on the line z = z_d through a drop's axis (theta = 0 or pi)
+    and the z-faces on the line x = x_d (theta = +-pi/2, where cos(6 theta + a)
+    = -cos a): the radial flare blends each face towards sign(offset) times
+    the flare speed by 0.65 clamp(dt mix(14, 22, slug)
+    radial(r / (r_base azim(theta))) vmask) (fluid.ts:247-285), in closed form
+    at every distance the neck reaches."""
+    hx, nx = 0.008, 16
+    h = 2 * hx / nx
+    x0 = -hx + 8.5 * h
+    rd, vy, drain, seed = 0.02, -0.3, 0.4, 0.9
+    a, b = scene_pair(drops=[(x0, 0.01, x0, rd, vy, 0.0, drain, seed)])
+    dt = 0.002
+    for o in (a, b):
+        o.stage(oracle.ST_F2C)
+        o.stage(oracle.ST_ADVECT, dt)
+    sp = sponge_of(a)
+    ss = oracle.smoothstep
+    nz, ny, _ = a.shape
+    slug = ss(0.14, 0.76, drain)
+    rbase = min(max(rd * (0.80 + 0.28 * slug) + 0.012 + 0.018 * slug, 0.018), 0.088)
+    azim_x = 1 + 0.036 * math.cos(1.73 * seed) + 0.026 * math.cos(-2.31 * seed) + \
+        0.018 * math.sin(0.89 * seed)                 # theta = 0, pi
+    azim_z = 1 + 0.036 * math.cos(1.73 * seed) - 0.026 * math.cos(-2.31 * seed) + \
+        0.018 * math.sin(0.89 * seed)                 # theta = +-pi/2
+
+    def radial(rr):
+        sl = (0.14 + min(rr * rr, 1.20)) * (1 - ss(0.78, 1.15, rr))
+        st = (0.68 + 0.32 * min(rr * rr, 1.0)) * (1 - ss(0.74, 1.10, rr))
+        return st + (sl - st) * slug
+    lines = ((oracle.F_U, azim_x, lambda j, q: (8, j, q)),   # x-face of cell (q, j, 8)
+             (oracle.F_W, azim_z, lambda j, q: (q, j, 8)))   # z-face of cell (8, j, q)
+    for f, azim, at in lines:
+        fa, fb = a.get(f), b.get(f)
+        moved = 0
+        for j in range(ny - 3):  # below the Marangoni and kinematic-top rows
+            cy = -0.024 + (j + 0.5) / ny * 0.024
+            vmask = ss(-0.058, -0.014, cy)
+            f0 = 0.104 * (0.35 + 0.65 * slug)
+            flare = (f0 + (-0.020 - f0) * ss(-0.066, -0.012, cy)) * min(max(drain, 0), 1.3)
+            for q in range(nx - 1):
+                off = (q - 7.5) * h  # face offset from the axis along the line
+                rr = abs(off) / max(rbase * azim, 1e-4)
+                want = fb[at(j, q)]
+                if vmask > 0.001 and abs(off) < rbase * 1.34 and rr < 1.22:
+                    bf = min(max(dt * (14 + 8 * slug) * radial(rr) * vmask, 0), 1)
+                    want = want * (1 - 0.65 * bf) + 0.65 * bf * math.copysign(1.0, off) * flare
+                    moved += bf > 0
+                got = fa[at(j, q)]
+                assert got == pytest.approx(want * sp[at(j, q)], rel=1e-12, abs=1e-15), (f, j, q)
+        assert moved > 50
+
+
 # ---------------------------------------------------------------- Casson scene terms
 def casson_scene_oracle(scene=1, nx=16, ny=24, nz=16, geom=None, **clock):
     """an oracle with the wave coupling on a deep tank (the unfurling gate is
