@@ -258,9 +258,12 @@ int launch_face_to_centre(Ctx *ctx) {
   ProfScope ps(ctx->prof, ctx->stream, KC_F2C);
   with_step_type(ctx->t0, [&](auto tag) {
     using T = std::remove_pointer_t<decltype(tag)>;
-    hipLaunchKernelGGL(k_face_to_centre<T>, cell_grid(d), kCellBlock, 0, ctx->stream, d,
+    auto kern = ctx->wave_on && ctx->wave_scene ? k_face_to_centre<T, true>
+                                                : k_face_to_centre<T, false>;
+    hipLaunchKernelGGL(kern, cell_grid(d), kCellBlock, 0, ctx->stream, d,
                        (const T *)ctx->u.ptr, (const T *)ctx->v.ptr, (const T *)ctx->w.ptr,
-                       (typename cvec_of<T>::type *)ctx->cvel.ptr);
+                       (typename cvec_of<T>::type *)ctx->cvel.ptr, ctx->wp,
+                       (const WaveDrop *)ctx->drops);
   });
   LAUNCHCHK();
   return MAC3D_OK;

@@ -26,11 +26,13 @@ struct StepParams {
   (void)sx; (void)sy; (void)sz;
 
 // FLUID_MAC_TO_CENTER (src/shaders/fluid.ts:917-943): centre velocity is the
-// mean of the two faces of each axis, solid wall faces read as 0.
-template <class T>
+// mean of the two faces of each axis, solid wall faces read as 0.  SCENE: the
+// top-row stagnation gate (fluid.ts:944-959) from the drop slots.
+template <class T, bool SCENE>
 __global__ void __launch_bounds__(256)
 k_face_to_centre(Dims d, const T *__restrict__ u, const T *__restrict__ v,
-                 const T *__restrict__ w, typename cvec_of<T>::type *__restrict__ cvel) {
+                 const T *__restrict__ w, typename cvec_of<T>::type *__restrict__ cvel,
+                 WaveParams wp, const WaveDrop *__restrict__ drops) {
   MAC3D_CELL_PROLOGUE
   T uR = (i >= d.nx - 1) ? T(0) : u[c];
   T uL = (i == 0) ? T(0) : u[c - sx];
@@ -42,6 +44,8 @@ k_face_to_centre(Dims d, const T *__restrict__ u, const T *__restrict__ v,
   o.x = T(0.5) * (uL + uR);
   o.y = T(0.5) * (vD + vU);
   o.z = T(0.5) * (wB + wF);
+  if constexpr (SCENE)
+    if (j >= d.ny - 2 && o.y < T(0)) o.y *= top_stagnation_gate<T>(d, i, j, k, wp, drops);
   cvel[c] = o;
 }
 

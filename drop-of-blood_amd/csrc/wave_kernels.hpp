@@ -567,6 +567,35 @@ __device__ inline void advect_scene_terms(const Dims &d, int i, int j, int k, co
   vz = vz * sponge;
 }
 
+// The top-row stagnation gate of FLUID_MAC_TO_CENTER (fluid.ts:944-959): the
+// factor on a downward centre velocity in the top two rows,
+// mix(mix(0.45, 1, smoothstep(0.002, 0.018, surface - y)), 1, activeNeck),
+// activeNeck the largest of each draining drop's (seed > 0, drain > 0.008)
+// horizontal footprint times smoothstep(0.008, 0.065, drain).
+template <class T>
+__device__ inline T top_stagnation_gate(const Dims &d, int i, int j, int k, const WaveParams &P,
+                                        const WaveDrop *__restrict__ drops) {
+  const T hx = (T)P.half_xz, depth = (T)P.depth, surf = (T)P.surface_y, y0 = surf - depth;
+  const T px = -hx + (i + T(0.5)) / (T)d.nx * (T(2) * hx);
+  const T py = y0 + (j + T(0.5)) / (T)d.ny * depth;
+  const T pz = -hx + (k + T(0.5)) / (T)d.nz * (T(2) * hx);
+  auto ss = [](T e0, T e1, T x) {
+    T t = clampv((x - e0) / (e1 - e0), T(0), T(1));
+    return t * t * (T(3) - T(2) * t);
+  };
+  T act = T(0);
+  for (int q = 0; q < WV_MAX_DROPS; ++q) {
+    const WaveDrop s = drops[q];
+    if (!(s.seed > 0.f && s.drain > 0.008f)) continue;
+    const T ra = clampv((T)s.r * T(1.15) + T(0.032), T(0.046), T(0.092));
+    const T ex = px - (T)s.x, ez = pz - (T)s.z;
+    const T rd = sqrt(ex * ex + ez * ez);
+    act = fmax(act, (T(1) - ss(ra * T(0.78), ra * T(1.20), rd)) * ss(T(0.008), T(0.065), (T)s.drain));
+  }
+  const T stag = mixv(T(0.45), T(1), ss(T(0.002), T(0.018), fmax(T(0), surf - py)));
+  return mixv(stag, T(1), act);
+}
+
 // the scene's per-frame uniforms that the Casson pass reads from device memory
 // (so a captured step keeps them current): simTime (cameraPosAndTime.w),
 // schlierenStr (particleParams.x), vorticityConf (fluidParams.x)

@@ -102,9 +102,11 @@ double orc_sample_cell(const double *f, int nx, int ny, int nz, double ux,
   return sample_cell(f, d, ux, uy, uz);
 }
 
-/* ---- face -> centre: FLUID_MAC_TO_CENTER (fluid.ts:917-943); the
- * scene-specific top-row stagnation gate (fluid.ts:944-959) is not part of
- * the workload. */
+static double top_stagnation_gate(const orc_state *s, int i, int j, int k);
+
+/* ---- face -> centre: FLUID_MAC_TO_CENTER (fluid.ts:917-943); with the wave
+ * coupling's scene flag, the top-row stagnation gate (fluid.ts:944-959,
+ * top_stagnation_gate) on downward centre velocities of the top two rows. */
 static void face_to_centre(orc_state *s) {
   dims3 d = s->d[0];
 #pragma omp parallel for
@@ -121,6 +123,8 @@ static void face_to_centre(orc_state *s) {
         s->cu[c] = 0.5 * (uL + uR);
         s->cv[c] = 0.5 * (vD + vU);
         s->cw[c] = 0.5 * (wB + wF);
+        if (s->wave_on && s->wp.scene && j >= d.ny - 2 && s->cv[c] < 0.0)
+          s->cv[c] *= top_stagnation_gate(s, i, j, k);
       }
 }
 
@@ -942,6 +946,28 @@ static void advect_scene(const orc_state *s, int i, int j, int k, double dt, dou
   *vx *= sponge;
   *vy *= sponge;
   *vz *= sponge;
+}
+
+/* the stagnation factor of FLUID_MAC_TO_CENTER (fluid.ts:944-959):
+ * mix(stagGate, 1, activeNeck), stagGate = mix(0.45, 1, smoothstep(0.002,
+ * 0.018, surface - y)), activeNeck the largest over drops with seed > 0 and
+ * drain > 0.008 of (1 - smoothstep(0.78 ra, 1.20 ra, r_xz)) smoothstep(0.008,
+ * 0.065, drain), ra = clamp(1.15 radius + 0.032, 0.046, 0.092). */
+static double top_stagnation_gate(const orc_state *s, int i, int j, int k) {
+  dims3 d = s->d[0];
+  const double hx = s->wp.half_xz, ytop = s->wp.surface_y, y0 = ytop - s->wp.depth;
+  double px = -hx + (i + 0.5) / d.nx * 2.0 * hx, pz = -hx + (k + 0.5) / d.nz * 2.0 * hx;
+  double py = y0 + (j + 0.5) / d.ny * s->wp.depth;
+  double act = 0.0;
+  for (int q = 0; q < MAX_DROPS; ++q) {
+    const orc_drop *dr = &s->drops[q];
+    if (!(dr->seed > 0.0 && dr->drain > 0.008)) continue;
+    double ra = clampd(dr->radius * 1.15 + 0.032, 0.046, 0.092);
+    double rd = hypot(px - dr->x, pz - dr->z);
+    act = fmax(act, (1.0 - sstep(ra * 0.78, ra * 1.20, rd)) * sstep(0.008, 0.065, dr->drain));
+  }
+  double stag = mixd(0.45, 1.0, sstep(0.002, 0.018, fmax(0.0, ytop - py)));
+  return mixd(stag, 1.0, act);
 }
 
 /* ---- FLUID_CURL (fluid.ts:331-369): omega = 0.5 (central differences of

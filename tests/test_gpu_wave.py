@@ -304,3 +304,26 @@ def test_scene_clock_from_the_uniform_block():
             s.stage("CASSON", 0.02)
         for f in ("U", "V", "W"):
             assert np.array_equal(s1.download(f), s2.download(f)), f
+
+
+def test_scene_face_to_centre_against_oracle():
+    """The centre velocity with the scene's top-row stagnation gate
+    (fluid.ts:944-959) on the deep swirling set-up with the two drops: equal
+    to the oracle's, held back only on downward top-row centres."""
+    w, s, o = deep_scene(1)
+    with s:
+        s.set_drops(DROPS)
+        o.set_drops(DROPS)
+        V = o.get(oracle.F_V)
+        s.stage("FACE_TO_CENTRE")
+        o.stage(oracle.ST_F2C)
+        for f in ("CU", "CV", "CW"):
+            e = rel_l2(s.download(f), o.get(getattr(oracle, "F_" + f)))
+            assert e <= 1e-6, f"{f}: rel L2 {e:.3e}"
+        cv = s.download("CV")
+        plain = 0.5 * (V[:, -2] + V[:, -1])
+        down = plain < 0
+        assert np.all(np.abs(cv[:, -1][~down] - plain[~down]) <= 1e-6)
+        assert np.all(np.abs(cv[:, -1][down]) < np.abs(plain[down]) + 1e-6)
+        assert np.abs(cv[:, -1] - plain).max() > 0.05
+    o.close()

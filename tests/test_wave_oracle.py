@@ -225,6 +225,17 @@ def scene_pair(drops=(), wave=None, **geom):
     return out[0][0], out[1][0]
 
 
+def advect_pair(a, b, dt):
+    """face->centre and advection on both; the no-scene oracle traces with the
+    scene's centre velocity (its top-row stagnation gate, fluid.ts:944-959),
+    so the two differ only by the advection's own scene terms"""
+    for o in (a, b):
+        o.stage(oracle.ST_F2C)
+    b.set(oracle.F_CV, a.get(oracle.F_CV))
+    for o in (a, b):
+        o.stage(oracle.ST_ADVECT, dt)
+
+
 def sponge_of(o):
     """the sponge multiplier mix(0.84, 1, wallFade) per cell (fluid.ts:312-321)"""
     nz, ny, nx = o.shape
@@ -243,10 +254,7 @@ def test_scene_sponge_closed_form():
     only multiply each advected face value by the sponge factor."""
     a, b = scene_pair()
     dt = 0.002
-    a.stage(oracle.ST_F2C)
-    b.stage(oracle.ST_F2C)
-    a.stage(oracle.ST_ADVECT, dt)
-    b.stage(oracle.ST_ADVECT, dt)
+    advect_pair(a, b, dt)
     sp = sponge_of(a)
     for f in (oracle.F_U, oracle.F_V, oracle.F_W):
         ga, gb = a.get(f)[:, :-3, :], b.get(f)[:, :-3, :]
@@ -268,9 +276,7 @@ def test_scene_kinematic_top_and_marangoni_closed_form():
     W[..., 3] = f0 + s * x[None, :]
     a, b = scene_pair(wave=W)
     dt = 0.002
-    for o in (a, b):
-        o.stage(oracle.ST_F2C)
-        o.stage(oracle.ST_ADVECT, dt)
+    advect_pair(a, b, dt)
     sp = sponge_of(a)
     nz, ny, nx = a.shape
     hx = 0.008
@@ -302,9 +308,7 @@ def test_scene_neck_draft_on_the_drop_axis():
     rd, vy, drain, seed = 0.02, -0.3, 0.4, 0.9
     a, b = scene_pair(drops=[(x0, 0.01, x0, rd, vy, 0.0, drain, seed)])
     dt = 0.002
-    for o in (a, b):
-        o.stage(oracle.ST_F2C)
-        o.stage(oracle.ST_ADVECT, dt)
+    advect_pair(a, b, dt)
     sp = sponge_of(a)
     ss = oracle.smoothstep
     nz, ny, _ = a.shape
@@ -338,9 +342,7 @@ def test_scene_flare_along_the_drop_axis():
     rd, vy, drain, seed = 0.02, -0.3, 0.4, 0.9
     a, b = scene_pair(drops=[(x0, 0.01, x0, rd, vy, 0.0, drain, seed)])
     dt = 0.002
-    for o in (a, b):
-        o.stage(oracle.ST_F2C)
-        o.stage(oracle.ST_ADVECT, dt)
+    advect_pair(a, b, dt)
     sp = sponge_of(a)
     ss = oracle.smoothstep
     nz, ny, _ = a.shape
@@ -518,3 +520,40 @@ def test_casson_scene_clamps_and_keeps_walls():
             assert u[:, :, :-1].max() == 1.8 and v.min() == -1.8
         else:
             assert u.max() > 1.8 and v.min() < -1.8
+
+
+def test_scene_top_row_stagnation_gate():
+    """FLUID_MAC_TO_CENTER's top-row gate (fluid.ts:944-959): a downward centre
+    velocity in the top two rows is scaled by mix(0.45, 1, smoothstep(0.002,
+    0.018, depth below the surface)) (0.50 and 0.95 for 10 mm rows), except
+    inside a draining drop's footprint (within 0.78 ra of its axis, drain past
+    0.065: factor 1); upward centres and lower rows keep 0.5 (vD + vU)."""
+    ss = oracle.smoothstep
+    o = casson_scene_oracle(geom=dict(depth=0.24, half_xz=0.1))
+    nz, ny, nx = o.shape
+    V = np.full(o.shape, -0.1)
+    V[:, :, :4] = 0.1  # upward in the first four columns
+    o.set(oracle.F_V, V)
+    hx, h = 0.1, 0.2 / nx
+    xd = -hx + 12.5 * h  # a drop over cell column (12, 12)
+    rd = 0.01
+    o.set_drops([(xd, 0.0, xd, rd, -0.3, 0.0, 0.1, 1.2)])
+    o.stage(oracle.ST_F2C)
+    cv = o.get(oracle.F_CV)
+    ra = min(max(rd * 1.15 + 0.032, 0.046), 0.092)
+    x = -hx + (np.arange(nx) + 0.5) * h
+    for j in range(1, ny):
+        depth_below = -(-0.24 + (j + 0.5) * 0.01)
+        stag = 0.45 + 0.55 * ss(0.002, 0.018, depth_below)
+        for k in range(nz):
+            for i in range(nx):
+                want = V[k, j, i]
+                if j >= ny - 2 and want < 0:
+                    r = math.hypot(x[i] - xd, x[k] - xd)
+                    act = (1 - ss(0.78 * ra, 1.20 * ra, r)) * ss(0.008, 0.065, 0.1)
+                    want *= stag + (1 - stag) * act
+                assert cv[k, j, i] == pytest.approx(want, rel=1e-14, abs=1e-16), (k, j, i)
+    top = cv[:, ny - 1, 4:]
+    # (negative values: the least held back is the minimum)
+    assert np.isclose(top.min(), -0.1)
+    assert np.isclose(top.max(), -0.1 * (0.45 + 0.55 * ss(0.002, 0.018, 0.005)))
