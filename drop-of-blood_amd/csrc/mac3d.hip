@@ -23,6 +23,7 @@
 #include "step_kernels.hpp"
 #include "adjoint_kernels.hpp"
 #include "wave_kernels.hpp"
+#include "parcel_kernels.hpp"
 #include "profile.hpp"
 
 using namespace mac3d;
@@ -134,6 +135,11 @@ struct Ctx {
   WaveParams wp{};
   Buf wave, wz, wfilm, ptop;
   Buf curl;  // scene: curl of the post-advection centre velocity, 4 per cell
+  SceneClock clk_host{0.f, 0.032f, 1.02f, 0.088f};  // host copy of the device clock
+  // Lagrangian parcels and their two splat counters (intact, lysed; n[0] each)
+  Parcel *parcels = nullptr;
+  int np = 0, np_cap = 0;
+  unsigned *splat = nullptr;
   WaveDrop *drops = nullptr;  // WV_MAX_DROPS slots, then the SceneClock
   float2 *wtw = nullptr;
   // staging for upload/download (stage_n doubles)
@@ -305,6 +311,31 @@ WaveParams wave_params_dt(const Ctx *ctx, double dt) {
 // WV_MAX_DROPS DropPhase
 const size_t kDropBytes =
     WV_MAX_DROPS * sizeof(WaveDrop) + sizeof(SceneClock) + WV_MAX_DROPS * sizeof(DropPhase);
+// One parcel step (BloodDropEngine.ts:2073-2093): both splat counters
+// cleared, then every parcel advanced and splatted.
+int launch_parcels(Ctx *ctx, double dt) {
+  const Dims &d = ctx->d[0];
+  if (ctx->t0 != TF32) return fail(ctx, MAC3D_EINVAL, "parcels need the FP32 or mixed step");
+  const size_t n2 = 2 * ctx->n[0];
+  ProfScope ps(ctx->prof, ctx->stream, KC_PARCELS);
+  hipLaunchKernelGGL(k_zero_u32, dim3((unsigned)((n2 + 255) / 256)), dim3(256), 0, ctx->stream,
+                     ctx->splat, n2);
+  LAUNCHCHK();
+  if (ctx->np == 0) return MAC3D_OK;
+  ParcelScene S;
+  S.dt = (float)std::min(std::max(dt, 0.0), 0.025);
+  S.half_xz = ctx->wp.half_xz;
+  S.y_max = ctx->wp.surface_y;
+  S.y_min = ctx->wp.surface_y - ctx->wp.depth;
+  S.sim_time = ctx->clk_host.sim_time;
+  S.k_lysis = ctx->clk_host.hemolysis;
+  hipLaunchKernelGGL(k_parcel_advect_splat, dim3((unsigned)((ctx->np + 255) / 256)), dim3(256), 0,
+                     ctx->stream, d, S, ctx->np, ctx->parcels, (const float3 *)ctx->cvel.ptr,
+                     (const float *)ctx->dye.ptr, ctx->splat, ctx->splat + ctx->n[0]);
+  LAUNCHCHK();
+  return MAC3D_OK;
+}
+
 const SceneClock *scene_clock(const Ctx *ctx) {
   return (const SceneClock *)(ctx->drops + WV_MAX_DROPS);
 }
@@ -1795,6 +1826,7 @@ int mac3d_wave_default_params(mac3d_wave_params *p) {
   p->vort_conf = 1.02;      // vorticityConfinement (uniformLayout.ts:32)
   p->sim_time = 0.0;
   p->schlieren = 0.032;     // schlierenStrength (uniformLayout.ts:44)
+  p->hemolysis = 0.088;     // hemolysisRate (uniformLayout.ts:43)
   return MAC3D_OK;
 }
 
@@ -1815,7 +1847,7 @@ int mac3d_wave_enable(void *h, const mac3d_wave_params *p) {
   if (!(ext > 0.0) || !std::isfinite(ext) || !std::isfinite(hx) || !std::isfinite(depth) ||
       !std::isfinite(p->surface_y) || !std::isfinite(p->marangoni) || p->half_xz < 0.0 ||
       p->depth < 0.0 || !std::isfinite(p->vort_conf) || !std::isfinite(p->sim_time) ||
-      !std::isfinite(p->schlieren))
+      !std::isfinite(p->schlieren) || !std::isfinite(p->hemolysis))
     return fail(ctx, MAC3D_EINVAL, "wave lengths must be finite, half_extent > 0");
   int rc;
   const size_t nn = (size_t)n * n;
@@ -1848,7 +1880,9 @@ int mac3d_wave_enable(void *h, const mac3d_wave_params *p) {
       ctx->stage_n = n4;
     }
   }
-  const SceneClock clk{(float)p->sim_time, (float)p->schlieren, (float)p->vort_conf, 0.f};
+  const SceneClock clk{(float)p->sim_time, (float)p->schlieren, (float)p->vort_conf,
+                       (float)p->hemolysis};
+  ctx->clk_host = clk;
   HIPCHK(hipMemcpyAsync((void *)scene_clock(ctx), &clk, sizeof(clk), hipMemcpyHostToDevice,
                         ctx->stream));
   // a flat, film-free surface (WAVE_CLEAR, wave.ts:15-30), no drops and no lid
@@ -1918,7 +1952,8 @@ int mac3d_set_uniforms(void *h, const float *u, size_t count) {
   int rc = mac3d_set_drops(h, d, MAC3D_MAX_DROPS);
   if (rc) return rc;
   // simTime, schlierenStr, vorticityConf (device memory: no graph rebuild)
-  const SceneClock clk{u[35], u[60], u[44], 0.f};
+  const SceneClock clk{u[35], u[60], u[44], u[59]};
+  ctx->clk_host = clk;
   HIPCHK(hipMemcpyAsync((void *)scene_clock(ctx), &clk, sizeof(clk), hipMemcpyHostToDevice,
                         ctx->stream));
   HIPCHK(hipStreamSynchronize(ctx->stream));
@@ -1929,6 +1964,50 @@ int mac3d_set_uniforms(void *h, const float *u, size_t count) {
   ctx->wp.marangoni = u[41];
   ctx->wp.half_xz = half_xz;
   ctx->wp.depth = depth;
+  return MAC3D_OK;
+}
+
+int mac3d_parcels_upload(void *h, const float *parcels, int32_t n) {
+  if (!h) return MAC3D_EINVAL;
+  Ctx *ctx = (Ctx *)h;
+  if (n < 0 || (n > 0 && !parcels)) return fail(ctx, MAC3D_EINVAL, "need n >= 0 parcels");
+  if (!ctx->wave_on) return fail(ctx, MAC3D_ESTATE, "the wave coupling is not enabled");
+  int rc;
+  if (n > ctx->np_cap) {  // (an earlier buffer stays owned until destroy)
+    if ((rc = dev_alloc(ctx, (void **)&ctx->parcels, (size_t)n * sizeof(Parcel)))) return rc;
+    ctx->np_cap = n;
+  }
+  if (!ctx->splat && (rc = dev_alloc(ctx, (void **)&ctx->splat, 2 * ctx->n[0] * sizeof(unsigned))))
+    return rc;
+  if (n)
+    HIPCHK(hipMemcpyAsync(ctx->parcels, parcels, (size_t)n * sizeof(Parcel), hipMemcpyHostToDevice,
+                          ctx->stream));
+  HIPCHK(hipStreamSynchronize(ctx->stream));
+  ctx->np = n;
+  return MAC3D_OK;
+}
+
+int mac3d_parcels_download(void *h, float *parcels, int32_t n) {
+  if (!h) return MAC3D_EINVAL;
+  Ctx *ctx = (Ctx *)h;
+  if (n < 0 || n > ctx->np || (n > 0 && !parcels))
+    return fail(ctx, MAC3D_EINVAL, "n must be within the uploaded parcels");
+  if (n)
+    HIPCHK(hipMemcpyAsync(parcels, ctx->parcels, (size_t)n * sizeof(Parcel), hipMemcpyDeviceToHost,
+                          ctx->stream));
+  HIPCHK(hipStreamSynchronize(ctx->stream));
+  return MAC3D_OK;
+}
+
+int mac3d_splat_download(void *h, int which, uint32_t *out, size_t count) {
+  if (!h) return MAC3D_EINVAL;
+  Ctx *ctx = (Ctx *)h;
+  if ((which != 0 && which != 1) || !out || count != ctx->n[0])
+    return fail(ctx, MAC3D_EINVAL, "which is 0 or 1 and count the cells");
+  if (!ctx->splat) return fail(ctx, MAC3D_ESTATE, "no parcels uploaded");
+  HIPCHK(hipMemcpyAsync(out, ctx->splat + which * ctx->n[0], count * sizeof(unsigned),
+                        hipMemcpyDeviceToHost, ctx->stream));
+  HIPCHK(hipStreamSynchronize(ctx->stream));
   return MAC3D_OK;
 }
 
@@ -2005,9 +2084,16 @@ int mac3d_adjoint(void *h, const mac3d_blob *blobs, int nblobs, double dt, int n
 int mac3d_stage(void *h, int stage, double dt) {
   if (!h) return MAC3D_EINVAL;
   Ctx *ctx = (Ctx *)h;
-  ctx->cvel_fresh = false;
   if (!std::isfinite(dt) || dt < 0.0) return fail(ctx, MAC3D_EINVAL, "dt must be finite and >= 0");
   int rc;
+  if (stage == MAC3D_STAGE_PARCELS) {  // (reads the velocity, changes none of it)
+    if (!ctx->wave_on || !ctx->splat) return fail(ctx, MAC3D_ESTATE, "needs the coupling and parcels");
+    if ((rc = launch_parcels(ctx, dt))) return rc;
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    ctx->prof.collect();
+    return MAC3D_OK;
+  }
+  ctx->cvel_fresh = false;
   switch (stage) {
     case MAC3D_STAGE_FACE_TO_CENTRE: rc = launch_face_to_centre(ctx); break;
     case MAC3D_STAGE_ADVECT: rc = launch_advect(ctx, dt); break;

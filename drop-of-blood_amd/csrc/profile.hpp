@@ -26,6 +26,7 @@ enum {
   KC_RESID64 = MAC3D_KC_REFINE64,
   KC_SMOOTH0X2 = MAC3D_KC_SMOOTH2_L0,
   KC_WAVE = MAC3D_KC_WAVE,
+  KC_PARCELS = MAC3D_KC_PARCELS,
   KC_COUNT = MAC3D_KC_COUNT
 };
 

@@ -598,9 +598,10 @@ __device__ inline T top_stagnation_gate(const Dims &d, int i, int j, int k, cons
 
 // the scene's per-frame uniforms that the Casson pass reads from device memory
 // (so a captured step keeps them current): simTime (cameraPosAndTime.w),
-// schlierenStr (particleParams.x), vorticityConf (fluidParams.x)
+// schlierenStr (particleParams.x), vorticityConf (fluidParams.x), and the
+// parcels' hemolysisRate (lensAndRheology.w)
 struct SceneClock {
-  float sim_time, schlieren, vort_conf, pad;
+  float sim_time, schlieren, vort_conf, hemolysis;
 };
 
 template <class T> struct q4_of;

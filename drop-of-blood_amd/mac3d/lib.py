@@ -15,11 +15,11 @@ PREC = {"fp32": 0, "mixed": 1, "fp64": 2}
 FIELD = dict(U=0, V=1, W=2, P=3, DYE=4, CU=5, CV=6, CW=7, RHS=8, MGX=9, MGB=10, MGR=11,
              DYE_PRED=12, WAVE=13, PTOP=14, CURL=15)
 STAGE = dict(FACE_TO_CENTRE=0, ADVECT=1, CASSON=2, DIVERGENCE=3, VCYCLE=4, PROJECT=5,
-             DYE_PREDICT=6, DYE_RESOLVE=7, WAVE=8, TOP_PRESSURE=9)
+             DYE_PREDICT=6, DYE_RESOLVE=7, WAVE=8, TOP_PRESSURE=9, PARCELS=10)
 MG = dict(SMOOTH=0, RESIDUAL=1, RESTRICT=2, PROLONG=3, COARSE=4, ZERO=5, SMOOTH_ZERO=6)
 KC = dict(FACE_TO_CENTRE=0, ADVECT=1, CASSON=2, DIVERGENCE=3, SMOOTH_L0=4, RESTRICT_L0=5,
           PROLONG_L0=6, MG_COARSE_LEVELS=7, COARSE_SOLVE=8, PROJECT=9, DYE=10, REFINE64=11,
-          SMOOTH2_L0=12, WAVE=13)
+          SMOOTH2_L0=12, WAVE=13, PARCELS=14)
 STATUS = {0: "OK", 1: "EINVAL", 2: "EHIP", 3: "ENOMEM", 4: "ESTATE"}
 
 
@@ -39,7 +39,8 @@ class Blob(C.Structure):
 class WaveParams(C.Structure):
     _fields_ = [("n", C.c_int32), ("scene", C.c_int32)] + \
                [(n, C.c_double) for n in ("half_extent", "half_xz", "depth", "surface_y",
-                                           "marangoni", "vort_conf", "sim_time", "schlieren")]
+                                           "marangoni", "vort_conf", "sim_time", "schlieren",
+                                           "hemolysis")]
 
 
 class Drop(C.Structure):
@@ -81,7 +82,8 @@ EXPORTS = ("mac3d_default_config", "mac3d_create", "mac3d_destroy", "mac3d_uploa
            "mac3d_download", "mac3d_init_blobs", "mac3d_step", "mac3d_stage", "mac3d_mg_op",
            "mac3d_div_norms", "mac3d_profile", "mac3d_profile_read", "mac3d_adjoint",
            "mac3d_last_error", "mac3d_version", "mac3d_wave_default_params",
-           "mac3d_wave_enable", "mac3d_set_drops", "mac3d_set_uniforms")
+           "mac3d_wave_enable", "mac3d_set_drops", "mac3d_set_uniforms", "mac3d_parcels_upload",
+           "mac3d_parcels_download", "mac3d_splat_download")
 
 
 def load_library(path: Optional[str] = None):
@@ -112,6 +114,9 @@ def load_library(path: Optional[str] = None):
     L.mac3d_wave_enable.argtypes = [vp, C.POINTER(WaveParams)]
     L.mac3d_set_drops.argtypes = [vp, C.POINTER(Drop), C.c_int]
     L.mac3d_set_uniforms.argtypes = [vp, vp, C.c_size_t]
+    L.mac3d_parcels_upload.argtypes = [vp, vp, C.c_int32]
+    L.mac3d_parcels_download.argtypes = [vp, vp, C.c_int32]
+    L.mac3d_splat_download.argtypes = [vp, C.c_int, vp, C.c_size_t]
     L.mac3d_last_error.argtypes = [vp]
     L.mac3d_last_error.restype = C.c_char_p
     L.mac3d_version.restype = C.c_char_p
@@ -140,6 +145,8 @@ class MacSolver:
     Mirrors the TypeScript MacSolver (js/MacSolver.js); arrays are numpy
     (nz, ny, nx) with x fastest, as the reference's textures are laid out.
     """
+
+    n_parcels = 0
 
     def __init__(self, config: Optional[Config] = None, **kw):
         self.L = load_library()
@@ -255,6 +262,26 @@ class MacSolver:
         Marangoni, domain bounds and the 8 drop slots are read)"""
         b = np.ascontiguousarray(block, dtype=np.float32)
         self._check(self.L.mac3d_set_uniforms(self.h, b.ctypes.data, b.size))
+
+    def parcels_upload(self, parcels):
+        """Lagrangian parcels as an (n, 8) array: x, y, z, mass, stretch x, y,
+        z, age ticks + lysis (float32 on the device)"""
+        p = np.ascontiguousarray(parcels, dtype=np.float32).reshape(-1, 8)
+        self._check(self.L.mac3d_parcels_upload(self.h, p.ctypes.data, p.shape[0]))
+        self.n_parcels = p.shape[0]
+
+    def parcels_download(self, n=None) -> np.ndarray:
+        n = self.n_parcels if n is None else n
+        out = np.empty((n, 8), dtype=np.float32)
+        self._check(self.L.mac3d_parcels_download(self.h, out.ctypes.data, n))
+        return out
+
+    def splat_download(self, which) -> np.ndarray:
+        """the splat counters (nz, ny, nx) uint32: which 0 or "INTACT", 1 or "LYSED"."""
+        w = {"INTACT": 0, "LYSED": 1}.get(which, which)
+        out = np.empty(self.level_shape(0), dtype=np.uint32)
+        self._check(self.L.mac3d_splat_download(self.h, int(w), out.ctypes.data, out.size))
+        return out
 
     # -- stepping
     def init_blobs(self, blobs: Sequence):

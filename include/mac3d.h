@@ -83,7 +83,9 @@ extern "C" {
 #define MAC3D_STAGE_DYE_RESOLVE 7    /* DYE <- MacCormack + limiter */
 #define MAC3D_STAGE_WAVE 8           /* WAVE <- one free-surface wave step (needs V) */
 #define MAC3D_STAGE_TOP_PRESSURE 9   /* PTOP <- top Dirichlet pressure from WAVE and drops */
-#define MAC3D_STAGE_COUNT 10
+#define MAC3D_STAGE_PARCELS 10       /* parcels advanced through CU,CV,CW; splat counters <- their
+                                        mass (cleared first); needs the coupling and parcels */
+#define MAC3D_STAGE_COUNT 11
 
 /* single multigrid operators on one level, for parity tests */
 #define MAC3D_MG_SMOOTH 0   /* count RBGS sweeps on (MGX, MGB) */
@@ -110,7 +112,8 @@ extern "C" {
 #define MAC3D_KC_REFINE64 11       /* mixed: FP64 residual/update */
 #define MAC3D_KC_SMOOTH2_L0 12     /* two RBGS sweeps in one pass on level 0 (FP32) */
 #define MAC3D_KC_WAVE 13           /* free-surface wave step and top pressure */
-#define MAC3D_KC_COUNT 14
+#define MAC3D_KC_PARCELS 14        /* parcel step and splat (MAC3D_STAGE_PARCELS) */
+#define MAC3D_KC_COUNT 15
 
 typedef struct mac3d_config {
   int32_t nx, ny, nz;   /* level-0 cells; each must divide by 2^levels */
@@ -235,6 +238,7 @@ typedef struct mac3d_wave_params {
                          (cameraPosAndTime.w, 0 s) */
   double schlieren;   /* scene: schlierenStr, the unfurling phase offset
                          (particleParams.x, 0.032) */
+  double hemolysis;   /* parcels' osmotic lysis rate (lensAndRheology.w, 0.088) */
 } mac3d_wave_params;
 
 /* one drop slot (SimUniforms.drops[d], common.ts:26-43, packed by
@@ -265,6 +269,22 @@ int mac3d_set_drops(void *ctx, const mac3d_drop *drops, int ndrops);
  * [60] (schlierenStr) and the 8 drop slots [64..127].  dt stays the step's
  * argument.  ESTATE unless enabled. */
 int mac3d_set_uniforms(void *ctx, const float *block, size_t count);
+
+/* Lagrangian blood parcels (RbcParcel, particles.ts:16-20): 8 floats each,
+ * position (m), intact mass (0: inactive), stretch vector (m), age ticks +
+ * lysis fraction.  MAC3D_STAGE_PARCELS advances them one step through the
+ * centre velocity (PARTICLE_ADVECT_SPLAT, particles.ts:189-463) and splats
+ * their mass into two 32-bit counters per cell, 65536 per unit mass (intact,
+ * lysed; splatIntactBuffer/splatLysedBuffer).  The world box is the
+ * coupling's: [-half_xz, half_xz] x [surface_y - depth, surface_y] x
+ * [-half_xz, half_xz].
+ * replaces: the parcel storage buffer's writeBuffer (BloodDropEngine.ts:1716-1720);
+ * n parcels (n >= 0); ESTATE unless the coupling is enabled */
+int mac3d_parcels_upload(void *ctx, const float *parcels, int32_t n);
+/* the parcels after MAC3D_STAGE_PARCELS (n <= the uploaded count) */
+int mac3d_parcels_download(void *ctx, float *parcels, int32_t n);
+/* the splat counters: which 0 intact, 1 lysed; count = cells */
+int mac3d_splat_download(void *ctx, int which, uint32_t *out, size_t count);
 
 /* one stage of the step (MAC3D_STAGE_*), blocking */
 int mac3d_stage(void *ctx, int stage, double dt);

@@ -29,7 +29,7 @@
 enum { F_U, F_V, F_W, F_P, F_DYE, F_CU, F_CV, F_CW, F_RHS, F_MGX, F_MGB,
        F_MGR, F_DYE_PRED, F_WAVE, F_PTOP, F_CURL, F_COUNT };
 enum { ST_F2C, ST_ADVECT, ST_CASSON, ST_DIV, ST_VCYCLE, ST_PROJECT,
-       ST_DYE_PRED, ST_DYE_RES, ST_WAVE, ST_TOP_PRESSURE };
+       ST_DYE_PRED, ST_DYE_RES, ST_WAVE, ST_TOP_PRESSURE, ST_PARCELS };
 #define MAX_DROPS 8
 enum { MG_SMOOTH, MG_RESIDUAL, MG_RESTRICT, MG_PROLONG, MG_COARSE, MG_ZERO };
 
@@ -50,6 +50,11 @@ struct orc_state {
   double *wave, *wz, *wz2, *wfilm, *ptop;
   orc_drop drops[MAX_DROPS];
   double *curl; /* scene: curl of the advected centre velocity, 4 per cell */
+  /* Lagrangian parcels (8 per parcel: position, mass, stretch, age + lysis)
+   * and their two splat counters (intact, lysed; n[0] each) */
+  double *parcels;
+  int np, np_cap;
+  uint32_t *splat[2];
 };
 
 #define IDX(D, i, j, k) ((((long)(k)) * (D).ny + (j)) * (long)(D).nx + (i))
@@ -1071,6 +1076,227 @@ static void casson_stage(orc_state *s, double dt) {
   if (scene) casson_scene(s, dt);
 }
 
+/* ---- PARTICLE_ADVECT_SPLAT (particles.ts:189-463) ----
+ * World box [-hx, hx] x [y_min, y_max] x [-hx, hx], y_max the surface height
+ * and y_min = y_max - depth (worldToUVW, common.ts:56-58).  The two per-parcel
+ * hash ranks fract(g 0.7548776662) and fract(g 0.6180339887 + 0.17) are formed
+ * in single precision as the reference's f32 does (for g ~ 4e5 the FP64 value
+ * differs by up to 0.02); the rest is FP64. */
+typedef struct { double hx, ymin, ymax, t, kl, dt; } parcel_scene;
+
+static void pc_uvw(const parcel_scene *S, const double p[3], double u[3]) {
+  u[0] = (p[0] + S->hx) / (2.0 * S->hx);
+  u[1] = (p[1] - S->ymin) / (S->ymax - S->ymin);
+  u[2] = (p[2] + S->hx) / (2.0 * S->hx);
+}
+
+/* sampleVelWorld (particles.ts:201-236) */
+static void pc_velocity(const orc_state *s, const parcel_scene *S, const double p[3],
+                        double mature, double hold, double v[3]) {
+  dims3 d = s->d[0];
+  double below = S->ymax - p[1], u[3];
+  pc_uvw(S, p, u);
+  for (int a = 0; a < 3; ++a) u[a] = clampd(u[a], 0.002, 0.998);
+  double gate = mixd(0.38, 1.0, sstep(0.002, 0.055, below));
+  double mono = mixd(0.020, 0.046, sstep(0.04, 0.42, below)) * gate;
+  double stack = mixd(0.032, 0.062, sstep(0.04, 0.48, below));
+  double tail = hold * (1.0 - sstep(0.003, 0.034, fmax(0.0, below)));
+  double settle = mixd(mono, stack, mature) * mixd(1.0, 0.035, tail);
+  double sub[3] = {0.0, 0.0, 0.0};
+  double deep = sstep(0.035, 0.24, below);
+  if (deep > 0.001) {
+    double tp = S->t * 0.42;
+    double a1 = 8.4 * p[0] + 5.2 * p[1] + 6.6 * p[2] + tp;
+    double a2 = -7.2 * p[0] + 6.0 * p[1] + 9.4 * p[2] - tp * 0.81;
+    double a3 = 11.5 * p[0] - 7.5 * p[1] - 8.5 * p[2] + tp * 1.13;
+    double k1[3] = {-8.16, 3.84, 7.36}, k2[3] = {-8.20, -1.38, -5.40}, k3[3] = {8.30, -1.10, 12.20};
+    for (int a = 0; a < 3; ++a)
+      sub[a] = k1[a] * cos(a1) + k2[a] * cos(a2) + k3[a] * (0.45 * sin(a3));
+    sub[1] *= sstep(0.06, 0.24, below);
+    for (int a = 0; a < 3; ++a) sub[a] *= 0.00028 * deep;
+  }
+  v[0] = sample_cell(s->cu, d, u[0], u[1], u[2]) + sub[0];
+  v[1] = sample_cell(s->cv, d, u[0], u[1], u[2]) * mixd(1.0, 0.045, tail) - settle + sub[1];
+  v[2] = sample_cell(s->cw, d, u[0], u[1], u[2]) + sub[2];
+}
+
+/* splatTrilinearIntact/Lysed (particles.ts:238-301): round-half-even of
+ * ((mass 65536) w_x) w_y w_z added to the eight clamped cells */
+static void pc_splat(orc_state *s, int which, const double u[3], double mass) {
+  dims3 d = s->d[0];
+  int n[3] = {d.nx, d.ny, d.nz}, b0[3], b1[3];
+  double f[3];
+  for (int a = 0; a < 3; ++a) {
+    double c = u[a] * n[a] - 0.5, b = floor(c);
+    f[a] = clampd(c - b, 0.0, 1.0);
+    b0[a] = clampi((int)b, 0, n[a] - 1);
+    b1[a] = clampi((int)b + 1, 0, n[a] - 1);
+  }
+  double sc = mass * 65536.0;
+  for (int c = 0; c < 8; ++c) {
+    int a = c & 1, b = (c >> 1) & 1, e = c >> 2;
+    double w = sc * (a ? f[0] : 1.0 - f[0]) * (b ? f[1] : 1.0 - f[1]) * (e ? f[2] : 1.0 - f[2]);
+    uint32_t q = (uint32_t)nearbyint(w);
+    if (q > 0) s->splat[which][IDX(d, a ? b1[0] : b0[0], b ? b1[1] : b0[1], e ? b1[2] : b0[2])] += q;
+  }
+}
+
+static void parcel_step(orc_state *s, int g, const parcel_scene *S) {
+  double *P = s->parcels + 8L * g;
+  double mass = P[3];
+  if (mass <= 1e-6) return;
+  const double dt = S->dt, hx = S->hx, ymin = S->ymin;
+  double pos[3] = {P[0], P[1], P[2]}, st[3] = {P[4], P[5], P[6]};
+  double raw = fmax(0.0, P[7]), ticks = floor(raw);
+  double lysis = clampd(raw - ticks, 0.0, 0.98), age = ticks / 240.0;
+  double mature = sstep(8.5, 18.0, age);
+  float tr = (float)g * 0.7548776662f;
+  double tail_rank = (double)(tr - floorf(tr));
+  double hold = pow(sstep(0.36, 0.995, tail_rank), 1.65) * (1.0 - sstep(5.2, 10.2, age));
+  double v1[3], v2[3], v3[3], va[3], q[3];
+  pc_velocity(s, S, pos, mature, hold, v1);
+  double sp2 = v1[0] * v1[0] + v1[1] * v1[1] + v1[2] * v1[2];
+  if (dt > 0.0) {
+    ticks = fmin(8191.0, ticks + fmax(1.0, nearbyint(dt * 240.0)));
+    for (int a = 0; a < 3; ++a) q[a] = pos[a] + 0.5 * v1[a] * dt;
+    pc_velocity(s, S, q, mature, hold, v2);
+    for (int a = 0; a < 3; ++a) va[a] = v2[a];
+    if (sp2 > 0.0025) {
+      for (int a = 0; a < 3; ++a) q[a] = pos[a] + 0.75 * v2[a] * dt;
+      pc_velocity(s, S, q, mature, hold, v3);
+      for (int a = 0; a < 3; ++a) va[a] = (2.0 / 9.0) * v1[a] + (3.0 / 9.0) * v2[a] + (4.0 / 9.0) * v3[a];
+    }
+    double slip = sstep(ymin + 0.005, ymin + 0.042, pos[1]);
+    double rxz = hypot(pos[0], pos[2]), o[2] = {0.0, 0.0};
+    if (rxz > 1e-4) {
+      double k = 0.016 * (1.0 - slip) * sstep(0.01, 0.22, rxz);
+      o[0] = pos[0] / fmax(rxz, 1e-4) * k;
+      o[1] = pos[2] / fmax(rxz, 1e-4) * k;
+    }
+    va[0] = va[0] * mixd(0.45, 1.0, slip) + o[0];
+    va[1] = va[1] * mixd(0.15, 1.0, slip);
+    va[2] = va[2] * mixd(0.45, 1.0, slip) + o[1];
+    for (int a = 0; a < 3; ++a) pos[a] += va[a] * dt;
+    pos[1] = clampd(pos[1], ymin + 0.0055, S->ymax - 0.0018);
+    if (sp2 > 2.5e-7 && slip > 0.15) {
+      double sl = fmax(sqrt(st[0] * st[0] + st[1] * st[1] + st[2] * st[2]), 1e-5), vp[3], vm[3];
+      double sd[3] = {st[0] / sl, st[1] / sl, st[2] / sl}, pr = 0.0065;
+      for (int a = 0; a < 3; ++a) q[a] = pos[a] + sd[a] * pr;
+      pc_velocity(s, S, q, mature, hold, vp);
+      for (int a = 0; a < 3; ++a) q[a] = pos[a] - sd[a] * pr;
+      pc_velocity(s, S, q, mature, hold, vm);
+      for (int a = 0; a < 3; ++a) st[a] += (vp[a] - vm[a]) * (sl / (2.0 * pr)) * dt;
+      double vmag = sqrt(sp2);
+      if (vmag > 0.001 && sl < 0.006)
+        for (int a = 0; a < 3; ++a) st[a] += (v1[a] / vmag) * (0.0022 * dt);
+      double maxlen = mixd(0.0270, 0.0420, hold * (1.0 - sstep(0.005, 0.065, fmax(0.0, S->ymax - pos[1]))));
+      double len = sqrt(st[0] * st[0] + st[1] * st[1] + st[2] * st[2]);
+      double nl = clampd(len, 0.0026, maxlen);
+      for (int a = 0; a < 3; ++a) st[a] = st[a] / fmax(len, 1e-5) * nl;
+    } else if (slip <= 0.15) {
+      st[1] *= exp(-dt * 5.0);
+    }
+  }
+  double u[3], uc[3];
+  pc_uvw(S, pos, u);
+  double rh = hypot(pos[0], pos[2]);
+  mass *= mixd(1.0, 1.0 - sstep(hx * 0.68, hx * 0.94, rh), clampd(dt * 6.0, 0.0, 1.0));
+  if (mass <= 1e-5 || rh >= hx * 0.95 || u[0] <= 0.002 || u[0] >= 0.998 || u[2] <= 0.002 ||
+      u[2] >= 0.998 || u[1] >= 0.998) {
+    P[3] = 0.0; /* retired where it stood at the start of the step */
+    return;
+  }
+  for (int a = 0; a < 3; ++a) uc[a] = clampd(u[a], 0.002, 0.998);
+  if (dt > 0.0 && pos[1] < ymin + 0.026) {
+    double w = 1.0 - sstep(ymin + 0.005, ymin + 0.026, pos[1]);
+    double dm = mass * clampd(w * dt * 1.65, 0.0, 0.095);
+    if (dm > 1e-7) {
+      mass = fmax(0.0, mass - dm);
+      if (dm * 0.68 > 1e-7) pc_splat(s, 1, uc, dm * 0.68);
+    }
+    if (mass <= 1e-5) {
+      P[3] = 0.0;
+      return;
+    }
+  }
+  float crf = (float)g * 0.6180339887f + 0.17f;
+  double cell_res = (double)(crf - floorf(crf));
+  double cn = sample_cell(s->dye, s->d[0], uc[0], uc[1], uc[2]);
+  double dilute = 1.0 - sstep(0.012, 0.22, cn);
+  double disp = sstep(1.1, 7.8, age + dilute * 2.2 + lysis * 2.5);
+  double late = sstep(7.2, 16.0, age) * dilute;
+  double min_intact = 0.13400 * mixd(mixd(0.46, 0.64, cell_res),
+                                     mixd(mixd(0.07, 0.20, cell_res), mixd(0.014, 0.052, cell_res), late), disp);
+  double headroom = mass - min_intact;
+  double shelter = sstep(ymin + 0.024, ymin + 0.062, pos[1]);
+  if (dt > 0.0 && headroom > 1e-7 && shelter > 0.02) {
+    double shear = mixd(0.22, 1.0, sstep(0.0005, 0.0120, sp2));
+    double water = (1.0 - 0.72 * sstep(0.016, 0.30, cn)) * shelter * shear;
+    lysis = clampd(lysis + S->kl * (1.85 * water) * dt, 0.0, 0.98);
+    double frag = sstep(0.04, 0.48, lysis) * water;
+    double dm = fmin(headroom, mass * clampd(frag * S->kl * 1.95 * dt, 0.0, 0.16));
+    if (dm > 1e-7) {
+      mass = fmax(0.0, mass - dm);
+      pc_splat(s, 1, uc, dm);
+    }
+  }
+  P[0] = pos[0]; P[1] = pos[1]; P[2] = pos[2]; P[3] = mass;
+  P[4] = st[0]; P[5] = st[1]; P[6] = st[2]; P[7] = ticks + clampd(lysis, 0.0, 0.98);
+  double sl = sqrt(st[0] * st[0] + st[1] * st[1] + st[2] * st[2]);
+  if (sl < 0.0038) {
+    if (mass > 1e-6) pc_splat(s, 0, uc, mass);
+    return;
+  }
+  double sxz = hypot(st[0], st[2]), ht[3] = {1.0, 0.0, 0.0};
+  if (sxz > 1e-4) {
+    ht[0] = -st[2] / fmax(sxz, 1e-5);
+    ht[2] = st[0] / fmax(sxz, 1e-5);
+  }
+  double sc[3] = {st[1] * ht[2] - st[2] * ht[1], st[2] * ht[0] - st[0] * ht[2], st[0] * ht[1] - st[1] * ht[0]};
+  double scl = sqrt(sc[0] * sc[0] + sc[1] * sc[1] + sc[2] * sc[2]), cc[3];
+  for (int a = 0; a < 3; ++a) cc[a] = scl > 1e-4 ? sc[a] / fmax(scl, 1e-5) : ht[a];
+  double hm = sstep(0.35, 0.85, fabs(st[1]) / fmax(sl, 1e-5)), nn[3];
+  for (int a = 0; a < 3; ++a) nn[a] = mixd(cc[a], ht[a], hm);
+  double nl = sqrt(nn[0] * nn[0] + nn[1] * nn[1] + nn[2] * nn[2]);
+  double span = clampd(sl * 0.62, 0.0032, 0.0085);
+  double al[4] = {-0.42, -0.14, 0.14, 0.42}, bl[4] = {0.0, 1.0, -1.0, 0.0}, wt[4] = {0.21, 0.29, 0.29, 0.21};
+  for (int k = 0; k < 4; ++k) {
+    double pq[3], uq[3];
+    for (int a = 0; a < 3; ++a) pq[a] = pos[a] + al[k] * st[a] + bl[k] * nn[a] / nl * span;
+    pc_uvw(S, pq, uq);
+    for (int a = 0; a < 3; ++a) uq[a] = clampd(uq[a], 0.002, 0.998);
+    if (wt[k] * mass > 1e-6) pc_splat(s, 0, uq, wt[k] * mass);
+  }
+}
+
+/* the parcel stage: both counters cleared (BloodDropEngine.ts:2073-2074),
+ * then every parcel stepped and splatted (:2089-2093) */
+static void parcels_stage(orc_state *s, double dt) {
+  if (!s->wave_on || !s->parcels) return;
+  parcel_scene S = {s->wp.half_xz, s->wp.surface_y - s->wp.depth, s->wp.surface_y, s->wp.sim_time,
+                    s->wp.hemolysis, clampd(dt, 0.0, 0.025)};
+  memset(s->splat[0], 0, sizeof(uint32_t) * s->n[0]);
+  memset(s->splat[1], 0, sizeof(uint32_t) * s->n[0]);
+  for (int g = 0; g < s->np; ++g) parcel_step(s, g, &S);
+}
+
+void orc_parcels_set(orc_state *s, const double *p, int n) {
+  if (n > s->np_cap) {
+    free(s->parcels);
+    s->parcels = (double *)malloc(sizeof(double) * 8 * (size_t)n);
+    s->np_cap = n;
+  }
+  if (!s->splat[0]) {
+    s->splat[0] = (uint32_t *)calloc(s->n[0], sizeof(uint32_t));
+    s->splat[1] = (uint32_t *)calloc(s->n[0], sizeof(uint32_t));
+  }
+  memcpy(s->parcels, p, sizeof(double) * 8 * (size_t)n);
+  s->np = n;
+}
+
+double *orc_parcels(orc_state *s) { return s->parcels; }
+uint32_t *orc_splat(orc_state *s, int which) { return (which == 0 || which == 1) ? s->splat[which] : NULL; }
+
 void orc_wave_enable(orc_state *s, const orc_wave_params *p) {
   if (!p) {
     s->wave_on = 0;
@@ -1135,6 +1361,7 @@ void orc_destroy(orc_state *s) {
   for (int l = 1; l <= s->p.levels; ++l) { free(s->mgx[l]); free(s->mgb[l]); free(s->mgr[l]); }
   for (int a = 0; a < 4; ++a) free(s->cg[a]);
   free(s->wave); free(s->wz); free(s->wz2); free(s->wfilm); free(s->ptop); free(s->curl);
+  free(s->parcels); free(s->splat[0]); free(s->splat[1]);
   free(s);
 }
 
@@ -1211,6 +1438,7 @@ void orc_stage(orc_state *s, int stage, double dt) {
     case ST_DYE_RES: dye_resolve(s, dt); break;
     case ST_WAVE: if (s->wave_on) wave_step(s, dt); break;
     case ST_TOP_PRESSURE: if (s->wave_on) top_pressure(s, dt); break;
+    case ST_PARCELS: parcels_stage(s, dt); break;
   }
 }
 

@@ -16,6 +16,8 @@
 #ifndef MAC3D_ORACLE_H
 #define MAC3D_ORACLE_H
 
+#include <stdint.h>
+
 #ifdef __cplusplus
 extern "C" {
 #endif
@@ -40,6 +42,7 @@ typedef struct orc_wave_params {
   int n, scene; /* scene: also the advection's and Casson pass's scene terms */
   double half_extent, half_xz, depth, surface_y, marangoni;
   double vort_conf, sim_time, schlieren; /* Casson scene terms */
+  double hemolysis;                      /* parcels' lysis rate */
 } orc_wave_params;
 
 typedef struct orc_drop {
@@ -64,6 +67,12 @@ int orc_set_threads(int n);
 /* enable (p != NULL: flat surface, no drops) or disable the wave coupling */
 void orc_wave_enable(orc_state *s, const orc_wave_params *p);
 void orc_set_drops(orc_state *s, const orc_drop *d, int n);
+/* Lagrangian parcels: n x 8 doubles (x, y, z, mass, stretch x, y, z,
+ * age ticks + lysis); orc_parcels returns them after ST_PARCELS, orc_splat
+ * the intact (0) and lysed (1) counters, n cells each */
+void orc_parcels_set(orc_state *s, const double *p, int n);
+double *orc_parcels(orc_state *s);
+uint32_t *orc_splat(orc_state *s, int which);
 /* unitary in-place complex FFT of length n (a power of two) over interleaved
  * (re, im) doubles, sign -1 forward / +1 inverse: exposed for its pinning */
 void orc_fft(double *z, int n, int stride, int sign);

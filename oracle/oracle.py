@@ -18,7 +18,7 @@ LIB = os.path.join(HERE, "liborc.so")
 (F_U, F_V, F_W, F_P, F_DYE, F_CU, F_CV, F_CW, F_RHS, F_MGX, F_MGB, F_MGR, F_DYE_PRED, F_WAVE,
  F_PTOP, F_CURL) = range(16)
 (ST_F2C, ST_ADVECT, ST_CASSON, ST_DIV, ST_VCYCLE, ST_PROJECT, ST_DYE_PRED, ST_DYE_RES, ST_WAVE,
- ST_TOP_PRESSURE) = range(10)
+ ST_TOP_PRESSURE, ST_PARCELS) = range(11)
 MG_SMOOTH, MG_RESIDUAL, MG_RESTRICT, MG_PROLONG, MG_COARSE, MG_ZERO = range(6)
 
 
@@ -37,7 +37,7 @@ class OrcBlob(C.Structure):
 class OrcWaveParams(C.Structure):
     _fields_ = [("n", C.c_int), ("scene", C.c_int)] + [(n, C.c_double) for n in (
         "half_extent", "half_xz", "depth", "surface_y", "marangoni", "vort_conf", "sim_time",
-        "schlieren")]
+        "schlieren", "hemolysis")]
 
 
 class OrcDrop(C.Structure):
@@ -71,6 +71,11 @@ def lib():
         L.orc_mg_op.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_int]
         L.orc_div_norms.argtypes = [C.c_void_p, C.POINTER(C.c_double), C.POINTER(C.c_double)]
         L.orc_wave_enable.argtypes = [C.c_void_p, C.POINTER(OrcWaveParams)]
+        L.orc_parcels_set.argtypes = [C.c_void_p, C.POINTER(C.c_double), C.c_int]
+        L.orc_parcels.argtypes = [C.c_void_p]
+        L.orc_parcels.restype = C.POINTER(C.c_double)
+        L.orc_splat.argtypes = [C.c_void_p, C.c_int]
+        L.orc_splat.restype = C.POINTER(C.c_uint32)
         L.orc_set_drops.argtypes = [C.c_void_p, C.POINTER(OrcDrop), C.c_int]
         L.orc_fft.argtypes = [C.POINTER(C.c_double), C.c_int, C.c_int, C.c_int]
         L.orc_set_threads.restype = C.c_int
@@ -95,6 +100,8 @@ DEFAULTS = dict(rho_w=1000.0, mu_w=1e-3, rho_b=1060.0, tau_y=0.005, mu_inf=0.003
 
 class Oracle:
     """One FP64 simulation on the CPU."""
+
+    n_parcels = 0
 
     def __init__(self, nx, ny, nz, dx, levels=3, nu1=2, nu2=2, cycles=8, solve_to_tol=False,
                  max_cycles=50, **kw):
@@ -172,7 +179,7 @@ class Oracle:
 
     def wave_enable(self, n=512, half_extent=1.60, half_xz=0.0, depth=0.0, surface_y=0.30,
                     marangoni=0.0, scene=0, vort_conf=1.02, sim_time=0.0, schlieren=0.032,
-                    off=False):
+                    hemolysis=0.088, off=False):
         """enable (flat surface, no drops) or, off=True, disable the wave coupling"""
         if off:
             lib().orc_wave_enable(self.h, None)
@@ -180,8 +187,26 @@ class Oracle:
         self.wave_params = OrcWaveParams(n=n, scene=int(scene), half_extent=half_extent,
                                          half_xz=half_xz, depth=depth, surface_y=surface_y,
                                          marangoni=marangoni, vort_conf=vort_conf,
-                                         sim_time=sim_time, schlieren=schlieren)
+                                         sim_time=sim_time, schlieren=schlieren,
+                                         hemolysis=hemolysis)
         lib().orc_wave_enable(self.h, C.byref(self.wave_params))
+
+    def parcels_set(self, parcels):
+        """Lagrangian parcels, an (n, 8) array (x, y, z, mass, stretch x, y, z,
+        age ticks + lysis)"""
+        p = np.ascontiguousarray(parcels, dtype=np.float64).reshape(-1, 8)
+        lib().orc_parcels_set(self.h, p.ctypes.data_as(C.POINTER(C.c_double)), p.shape[0])
+        self.n_parcels = p.shape[0]
+
+    def parcels(self):
+        ptr = lib().orc_parcels(self.h)
+        return np.ctypeslib.as_array(ptr, shape=(self.n_parcels * 8,)).reshape(-1, 8).copy()
+
+    def splat(self, which):
+        """the intact (0) or lysed (1) splat counters, (nz, ny, nx) uint32"""
+        ptr = lib().orc_splat(self.h, int(which))
+        n = int(np.prod(self.shape))
+        return np.ctypeslib.as_array(ptr, shape=(n,)).reshape(self.shape).copy()
 
     def set_drops(self, drops):
         """drops: sequences (x, y, z, radius, vel_y, osc, drain, seed)"""
