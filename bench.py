@@ -80,6 +80,54 @@ def run_gpu(workload, steps, warmup, profile_steps=PROFILE_STEPS, wave=False, sc
         return st, wall, prof
 
 
+PARCELS = 393216  # the reference's persistent parcel count (PARTICLE_COUNT)
+# parcel state read and written once per step (two float4 each way)
+PARCEL_BYTES = 64.0
+
+
+def run_parcels(workload, steps, order):
+    """The parcel stage on the scene-term set-up: PARCELS parcels in a cloud
+    under the two drops, advanced and splatted `steps` times through the
+    flow after two MAC steps; HIP-event time of the stage per call.
+    order "cell": parcels stored in the order of their cells (neighbours in
+    memory sample neighbouring cells, as parcels emitted together do);
+    "random": uniformly shuffled (every lane of a wave gathers elsewhere)."""
+    import numpy as np
+    from mac3d import MacSolver
+    from mac3d.lib import config_for
+    with MacSolver(config_for(workload)) as s:
+        s.init_blobs(workload.blobs)
+        s.wave_enable(n=WAVE_N, marangoni=0.8, scene=1)
+        s.set_drops(WAVE_DROPS)
+        s.step_nostats(workload.dt, 2)
+        hx, top = 0.5 * workload.nx * workload.dx, 0.30
+        depth = workload.ny * workload.dx
+        rng = np.random.default_rng(5)
+        P = np.zeros((PARCELS, 8), dtype=np.float32)
+        r, th = rng.uniform(0, 0.5 * hx, PARCELS), rng.uniform(0, 2 * np.pi, PARCELS)
+        P[:, 0], P[:, 2] = r * np.cos(th), r * np.sin(th)
+        P[:, 1] = rng.uniform(top - 0.8 * depth, top - 0.004, PARCELS)
+        P[:, 3] = 0.134
+        d = rng.standard_normal((PARCELS, 3))
+        P[:, 4:7] = d / np.linalg.norm(d, axis=1, keepdims=True) * \
+            np.where(rng.random(PARCELS) < 0.4, 0.010, 0.002)[:, None]
+        P[:, 7] = rng.integers(0, 3000, PARCELS)
+        if order == "cell":
+            ix = ((P[:, 0] + hx) / workload.dx).astype(np.int64)
+            iy = ((P[:, 1] - (top - depth)) / workload.dx).astype(np.int64)
+            iz = ((P[:, 2] + hx) / workload.dx).astype(np.int64)
+            P = P[np.argsort((iz * workload.ny + iy) * workload.nx + ix, kind="stable")]
+        s.parcels_upload(P)
+        s.stage("PARCELS", workload.dt)
+        s.profile(True)
+        for _ in range(steps):
+            s.stage("PARCELS", workload.dt)
+        ms, n = s.profile_read("PARCELS")
+        s.profile(False)
+        live = int((s.parcels_download()[:, 3] > 0).sum())
+        return ms / max(n, 1), live
+
+
 def copy_probe(workload, times=5):
     """k_convert<float, float> over one field: reads and writes 4 B per cell."""
     from mac3d import MacSolver
@@ -172,6 +220,14 @@ def main():
                          "casson_us_per_step_no_scene": per_step_us(profw, "CASSON"),
                          "max_div": sts["max_div"]}
         log(f"[bench] scene-term fp32 {wave['scene']['steps_per_s']:.2f} steps/s")
+        wave["parcels"] = {"count": PARCELS}
+        for order in ("cell", "random"):
+            ms_p, live = run_parcels(w32, max(5, min(args.steps, 20)), order)
+            wave["parcels"][order] = {
+                "live_after": live, "us_per_step": round(1e3 * ms_p, 2),
+                "parcels_per_s": round(PARCELS / (ms_p * 1e-3), 1),
+                "state_gbps": round(PARCEL_BYTES * PARCELS / (ms_p * 1e-3) / 1e9, 1)}
+            log(f"[bench] parcels {PARCELS} ({order} order): {1e3 * ms_p:.1f} us per step")
 
     roof = None
     if "SMOOTH2_L0" in prof:
