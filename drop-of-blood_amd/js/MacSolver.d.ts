@@ -12,7 +12,7 @@ export interface Blob { cx: number; cy: number; cz: number; radius: number; w0: 
 export interface WaveParams {
   n?: number; halfExtent?: number; halfXZ?: number; depth?: number;
   surfaceY?: number; marangoni?: number; scene?: 0 | 1;
-  vortConf?: number; simTime?: number; schlieren?: number;
+  vortConf?: number; simTime?: number; schlieren?: number; hemolysis?: number;
 }
 export interface Drop {
   x: number; y: number; z: number; radius: number; velY: number; osc?: number;
@@ -37,6 +37,10 @@ export declare class MacSolver {
   waveEnable(params?: WaveParams | null): void;
   setDrops(drops: Drop[]): void;
   setUniforms(block: Float32Array): void;
+  parcelsUpload(parcels: Float32Array): void;
+  parcelsStep(dt: number): void;
+  parcelsDownload(out: Float32Array): Float32Array;
+  splatDownload(which: 0 | 1, out: Uint32Array): Uint32Array;
   step(dt: number, n?: number): StepStats;
   sync(): this;
   write(name: keyof typeof FIELD, data: Float32Array | Float64Array): void;

@@ -90,6 +90,33 @@ class MacSolver {
   }
 
   /**
+   * Lagrangian parcels (RbcParcel, 8 floats each: x, y, z, mass, stretch x, y, z,
+   * age ticks + lysis); parcelsStep(dt) advances and splats them.
+   * @param {Float32Array} parcels
+   */
+  parcelsUpload(parcels) {
+    addon.parcelsUpload(this._live(), parcels);
+  }
+
+  /** One parcel step and splat (MAC3D_STAGE_PARCELS) @param {number} dt */
+  parcelsStep(dt) {
+    if (!Number.isFinite(dt) || dt < 0) throw new RangeError('dt must be finite and >= 0');
+    addon.stage(this._live(), 10, dt);
+  }
+
+  /** @param {Float32Array} out 8 floats per parcel to read back */
+  parcelsDownload(out) {
+    addon.parcelsDownload(this._live(), out);
+    return out;
+  }
+
+  /** @param {0|1} which intact or lysed @param {Uint32Array} out one per cell */
+  splatDownload(which, out) {
+    addon.splatDownload(this._live(), which, out);
+    return out;
+  }
+
+  /**
    * Advance n steps of fixed dt (seconds); blocks until the GPU is done.
    * @param {number} dt @param {number} [n]
    * @returns {StepStats}

@@ -213,6 +213,7 @@ napi_value WaveEnable(napi_env env, napi_callback_info info) {
     prop_num(env, argv[1], "vortConf", &p.vort_conf);
     prop_num(env, argv[1], "simTime", &p.sim_time);
     prop_num(env, argv[1], "schlieren", &p.schlieren);
+    prop_num(env, argv[1], "hemolysis", &p.hemolysis);
     double scene = 0.0;
     prop_num(env, argv[1], "scene", &scene);
     p.scene = scene != 0.0;
@@ -267,6 +268,70 @@ napi_value SetUniforms(napi_env env, napi_callback_info info) {
     return nullptr;
   }
   int rc = mac3d_set_uniforms(ctx, (const float *)data, len);
+  if (rc) return throw_status(env, ctx, rc);
+  return nullptr;
+}
+
+// the typed array argv[i] of type want: its data and element count
+static bool typed_arg(napi_env env, napi_value v, napi_typedarray_type want, const char *what,
+                      void **data, size_t *len) {
+  napi_typedarray_type tt;
+  size_t off = 0;
+  napi_value ab;
+  if (napi_get_typedarray_info(env, v, &tt, len, data, &ab, &off) != napi_ok || tt != want) {
+    napi_throw_type_error(env, nullptr, what);
+    return false;
+  }
+  return true;
+}
+
+// parcelsUpload(h, Float32Array(8 n)) / parcelsDownload(h, Float32Array(8 n)):
+// the parcel storage buffer (RbcParcel, 8 floats each)
+napi_value ParcelsUpload(napi_env env, napi_callback_info info) {
+  napi_value argv[2];
+  if (!get_args(env, info, 2, argv)) return nullptr;
+  void *ctx = get_handle(env, argv[0]);
+  if (!ctx) return nullptr;
+  void *data = nullptr;
+  size_t len = 0;
+  if (!typed_arg(env, argv[1], napi_float32_array, "expected a Float32Array", &data, &len))
+    return nullptr;
+  if (len % 8) {
+    napi_throw_range_error(env, nullptr, "parcels are 8 floats each");
+    return nullptr;
+  }
+  int rc = mac3d_parcels_upload(ctx, (const float *)data, (int32_t)(len / 8));
+  if (rc) return throw_status(env, ctx, rc);
+  return nullptr;
+}
+
+napi_value ParcelsDownload(napi_env env, napi_callback_info info) {
+  napi_value argv[2];
+  if (!get_args(env, info, 2, argv)) return nullptr;
+  void *ctx = get_handle(env, argv[0]);
+  if (!ctx) return nullptr;
+  void *data = nullptr;
+  size_t len = 0;
+  if (!typed_arg(env, argv[1], napi_float32_array, "expected a Float32Array", &data, &len))
+    return nullptr;
+  int rc = mac3d_parcels_download(ctx, (float *)data, (int32_t)(len / 8));
+  if (rc) return throw_status(env, ctx, rc);
+  return nullptr;
+}
+
+// splatDownload(h, which, Uint32Array(cells)): 0 intact, 1 lysed
+napi_value SplatDownload(napi_env env, napi_callback_info info) {
+  napi_value argv[3];
+  if (!get_args(env, info, 3, argv)) return nullptr;
+  void *ctx = get_handle(env, argv[0]);
+  if (!ctx) return nullptr;
+  int32_t which = 0;
+  NAPI_CALL(env, napi_get_value_int32(env, argv[1], &which));
+  void *data = nullptr;
+  size_t len = 0;
+  if (!typed_arg(env, argv[2], napi_uint32_array, "expected a Uint32Array", &data, &len))
+    return nullptr;
+  int rc = mac3d_splat_download(ctx, which, (uint32_t *)data, len);
   if (rc) return throw_status(env, ctx, rc);
   return nullptr;
 }
@@ -393,6 +458,9 @@ napi_value Init(napi_env env, napi_value exports) {
       {"waveEnable", nullptr, WaveEnable, nullptr, nullptr, nullptr, napi_default, nullptr},
       {"setDrops", nullptr, SetDrops, nullptr, nullptr, nullptr, napi_default, nullptr},
       {"setUniforms", nullptr, SetUniforms, nullptr, nullptr, nullptr, napi_default, nullptr},
+      {"parcelsUpload", nullptr, ParcelsUpload, nullptr, nullptr, nullptr, napi_default, nullptr},
+      {"parcelsDownload", nullptr, ParcelsDownload, nullptr, nullptr, nullptr, napi_default, nullptr},
+      {"splatDownload", nullptr, SplatDownload, nullptr, nullptr, nullptr, napi_default, nullptr},
       {"step", nullptr, Step, nullptr, nullptr, nullptr, napi_default, nullptr},
       {"stage", nullptr, Stage, nullptr, nullptr, nullptr, napi_default, nullptr},
       {"divNorms", nullptr, DivNorms, nullptr, nullptr, nullptr, napi_default, nullptr},

@@ -45,6 +45,31 @@ if (process.argv[3] === 'wave') {
   s3.destroy();
 }
 
+// the parcel stage through the same host: 512 parcels laid out by formula
+// after the coupled steps; checksums of the parcels and both splat counters
+if (process.argv[3] === 'parcels') {
+  const s4 = new MacSolver({ nx: w.nx, ny: w.ny, nz: w.nz, dx: w.dx, levels: w.levels,
+    cycles: w.cycles, precision: 0 });
+  s4.initBlobs(w.blobs);
+  s4.waveEnable({ n: 64, halfExtent: 0.06, halfXZ: 0.016, depth: 0.048, surfaceY: 0.0 });
+  s4.step(w.dt, steps);
+  const n = 512;
+  const P = new Float32Array(8 * n);
+  for (let q = 0; q < n; q++) {
+    P.set([-0.008 + 0.016 * ((q * 37) % 64) / 64, -0.04 + 0.035 * ((q * 11) % 128) / 128,
+      -0.008 + 0.016 * ((q * 23) % 64) / 64, 0.1, q % 3 ? 0.002 : 0.006, q % 3 ? 0 : 0.004, 0,
+      (q * 7) % 3000], 8 * q);
+  }
+  s4.parcelsUpload(P);
+  s4.parcelsStep(w.dt);
+  const out = s4.parcelsDownload(new Float32Array(8 * n));
+  const cells = w.nx * w.ny * w.nz;
+  const a = s4.splatDownload(0, new Uint32Array(cells)), b = s4.splatDownload(1, new Uint32Array(cells));
+  console.log(JSON.stringify({ parcels: { p: sum(out), intact: a.reduce((x, y) => x + y, 0),
+    lysed: b.reduce((x, y) => x + y, 0) } }));
+  s4.destroy();
+}
+
 if (process.argv[3] === 'backward') {
   const s2 = new MacSolver({ nx: w.nx, ny: w.ny, nz: w.nz, dx: w.dx, levels: w.levels,
     cycles: w.cycles, precision: 2 });

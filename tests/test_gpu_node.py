@@ -75,3 +75,37 @@ def test_node_wave_coupling_matches_c_abi():
     assert js["wave"] > 0 and js["ptop"] > 0
     for k, a in got.items():
         assert js[k] == pytest.approx(float(np.sqrt(np.sum(a * a))), rel=1e-12), k
+
+
+def test_node_parcel_stage_matches_c_abi():
+    """parcelsUpload / parcelsStep / splatDownload through the N-API host
+    give the C ABI's parcels and counters bit for bit (512 parcels laid out by
+    the same formula after the same coupled steps)."""
+    from mac3d import MacSolver
+    from mac3d.lib import config_for
+    steps = 2
+    r = subprocess.run(["node", os.path.join(ROOT, "tests", "js", "step_tiny.js"), str(steps),
+                        "parcels"], capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stderr
+    js = json.loads(r.stdout.strip().splitlines()[-1])["parcels"]
+    w = scenarios.workload("tiny", 1)
+    q = np.arange(512)
+    P = np.zeros((512, 8))
+    P[:, 0] = -0.008 + 0.016 * ((q * 37) % 64) / 64
+    P[:, 1] = -0.04 + 0.035 * ((q * 11) % 128) / 128
+    P[:, 2] = -0.008 + 0.016 * ((q * 23) % 64) / 64
+    P[:, 3] = 0.1
+    P[:, 4] = np.where(q % 3, 0.002, 0.006)
+    P[:, 5] = np.where(q % 3, 0.0, 0.004)
+    P[:, 7] = (q * 7) % 3000
+    with MacSolver(config_for(w)) as s:
+        s.init_blobs(w.blobs)
+        s.wave_enable(n=64, half_extent=0.06, half_xz=0.016, depth=0.048, surface_y=0.0)
+        s.step(w.dt, steps)
+        s.parcels_upload(P)
+        s.stage("PARCELS", w.dt)
+        out = s.parcels_download().astype(np.float64)
+        intact, lysed = int(s.splat_download(0).sum()), int(s.splat_download(1).sum())
+    assert intact > 0
+    assert js["intact"] == intact and js["lysed"] == lysed
+    assert js["p"] == pytest.approx(float(np.sqrt(np.sum(out * out))), rel=1e-12)
