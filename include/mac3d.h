@@ -279,7 +279,9 @@ int mac3d_set_uniforms(void *ctx, const float *block, size_t count);
  * coupling's: [-half_xz, half_xz] x [surface_y - depth, surface_y] x
  * [-half_xz, half_xz].
  * replaces: the parcel storage buffer's writeBuffer (BloodDropEngine.ts:1716-1720);
- * n parcels (n >= 0); ESTATE unless the coupling is enabled */
+ * n parcels (n >= 0); ESTATE unless the coupling is enabled.  The stage runs in
+ * the FP32 and mixed modes (EINVAL in FP64: the parcels sample the FP32
+ * centre velocity) */
 int mac3d_parcels_upload(void *ctx, const float *parcels, int32_t n);
 /* the parcels after MAC3D_STAGE_PARCELS (n <= the uploaded count) */
 int mac3d_parcels_download(void *ctx, float *parcels, int32_t n);
