@@ -5,7 +5,10 @@
 // FP32 as the reference stores them (RbcParcel, particles.ts:16-20).
 //
 // The splat adds integer weights with atomics, so the counters do not depend
-// on the order in which parcels land.  Each weight is
+// on the order in which parcels land.  Device-scope atomics bound the stage;
+// two x-neighbour cells in one aligned 64-bit word take one 64-bit atomic,
+// which equals two 32-bit adds while no counter reaches 2^32 (32,768 units of
+// mass in one cell).  Each weight is
 // round(mass 65536 w) with w a product of trilinear factors, round-half-even
 // as WGSL's round.
 #pragma once
@@ -94,15 +97,27 @@ __device__ inline void pc_splat(const Dims &d, unsigned *__restrict__ acc, float
   const int y0 = clampi(iy, 0, d.ny - 1), y1 = clampi(iy + 1, 0, d.ny - 1);
   const int z0 = clampi(iz, 0, d.nz - 1), z1 = clampi(iz + 1, 0, d.nz - 1);
   const float s = mass * kSplatScale;
-  const float wx[2] = {gx, fx}, wy[2] = {gy, fy}, wz[2] = {gz, fz};
-  const int xs[2] = {x0, x1}, ys[2] = {y0, y1}, zs[2] = {z0, z1};
+  const float wy[2] = {gy, fy}, wz[2] = {gz, fz};
+  const int ys[2] = {y0, y1}, zs[2] = {z0, z1};
 #pragma unroll
-  for (int c = 0; c < 8; ++c) {
-    const int a = c & 1, b = (c >> 1) & 1, e = c >> 2;
+  for (int c = 0; c < 4; ++c) {
+    const int b = c & 1, e = c >> 1;
     // ((s wx) wy) wz, each product rounded (no fused multiply-add)
-    const float w = __fmul_rn(__fmul_rn(__fmul_rn(s, wx[a]), wy[b]), wz[e]);
-    const unsigned q = (unsigned)rintf(w);
-    if (q > 0u) atomicAdd(acc + idx3(d, xs[a], ys[b], zs[e]), q);
+    const unsigned q0 = (unsigned)rintf(__fmul_rn(__fmul_rn(__fmul_rn(s, gx), wy[b]), wz[e]));
+    const unsigned q1 = (unsigned)rintf(__fmul_rn(__fmul_rn(__fmul_rn(s, fx), wy[b]), wz[e]));
+    unsigned *row = acc + idx3(d, 0, ys[b], zs[e]);
+    if (x1 == x0 + 1 && !(x0 & 1)) {
+      // both cells in one aligned 64-bit word (rows start even: nx is even):
+      // one atomic adds both counts (no carry while a counter stays below 2^32)
+      if (q0 | q1)
+        atomicAdd((unsigned long long *)(row + x0),
+                  ((unsigned long long)q1 << 32) | (unsigned long long)q0);
+    } else if (x1 == x0) {  // clamped at a wall: both weights on one cell
+      if (q0 + q1) atomicAdd(row + x0, q0 + q1);
+    } else {
+      if (q0) atomicAdd(row + x0, q0);
+      if (q1) atomicAdd(row + x1, q1);
+    }
   }
 }
 
