@@ -10,7 +10,10 @@
  * workload makes (SI Casson/Boussinesq, no-slip walls, exact coarse solve).
  * Parity unpinned: advect_scene's azimuthal modulation between the lines
  * through a drop's axis along x and z (its value there is pinned in closed
- * form; in between only the device parity checks it).
+ * form; in between only the device parity checks it); parcel_step's
+ * meniscus-hold, stretch and hemolysis heuristics beyond what the splat,
+ * settling and mass-conservation pins fix (only the device parity checks
+ * them).
  * Arrays are cell-major with x fastest: idx = (k*ny + j)*nx + i.
  */
 #ifndef MAC3D_ORACLE_H
