@@ -327,3 +327,29 @@ def test_scene_face_to_centre_against_oracle():
         assert np.all(np.abs(cv[:, -1][down]) < np.abs(plain[down]) + 1e-6)
         assert np.abs(cv[:, -1] - plain).max() > 0.05
     o.close()
+
+
+def test_scene_graph_replays_see_the_uniform_block():
+    """The scene step replayed from its captured graph against the same
+    steps launched eagerly (profiling switches graphs off), with the packed
+    block changing simTime, vorticity confinement, hemolysis and the drop
+    slots between steps: these live in device memory, so every replay sees
+    them and matches the eager step bit for bit."""
+    w, s, o = deep_scene(1)
+    o.close()
+    w, e, o = deep_scene(1)
+    o.close()
+    blk = np.zeros(128, dtype=np.float32)
+    blk[40], blk[41] = DEEP["surface_y"], DEEP["marangoni"]
+    blk[52], blk[53], blk[54] = DEEP["half_xz"], -DEEP["depth"], 0.0
+    with s, e:
+        e.profile(True)
+        for step in range(4):
+            blk[35], blk[44], blk[59] = 0.3 * step, 1.0 + 0.2 * step, 0.05 * step
+            d = DROPS[step % 2]
+            blk[64:72] = (d[0], d[1], d[2], d[3], d[4], d[5], d[6] * (1 + step), d[7])
+            for x in (s, e):
+                x.set_uniforms(blk)
+                x.step(w.dt, 1)
+            for f in ("U", "V", "W", "P", "DYE"):
+                assert np.array_equal(s.download(f), e.download(f)), (step, f)
