@@ -318,8 +318,8 @@ int launch_parcels(Ctx *ctx, double dt) {
   if (ctx->t0 != TF32) return fail(ctx, MAC3D_EINVAL, "parcels need the FP32 or mixed step");
   const size_t n2 = 2 * ctx->n[0];
   ProfScope ps(ctx->prof, ctx->stream, KC_PARCELS);
-  hipLaunchKernelGGL(k_zero_u32, dim3((unsigned)((n2 + 255) / 256)), dim3(256), 0, ctx->stream,
-                     ctx->splat, n2);
+  hipLaunchKernelGGL(k_zero_u32, dim3((unsigned)((n2 / 4 + 3 + 255) / 256)), dim3(256), 0,
+                     ctx->stream, ctx->splat, n2);  // (hipMalloc: 256-byte aligned)
   LAUNCHCHK();
   if (ctx->np == 0) return MAC3D_OK;
   ParcelScene S;

@@ -290,9 +290,13 @@ k_parcel_advect_splat(Dims d, ParcelScene S, int np, Parcel *__restrict__ parcel
 }
 
 // fills n counters with 0 (a kernel, not a memset, so a captured graph may hold it)
+// with 16-byte stores (p 16-byte aligned), the n mod 4 tail one by one
 __global__ void __launch_bounds__(256) k_zero_u32(unsigned *__restrict__ p, size_t n) {
-  const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < n) p[i] = 0u;
+  const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x, n4 = n / 4;
+  if (i < n4)
+    reinterpret_cast<uint4 *>(p)[i] = make_uint4(0u, 0u, 0u, 0u);
+  else if (i < n4 + (n & 3))
+    p[4 * n4 + (i - n4)] = 0u;
 }
 
 }  // namespace mac3d

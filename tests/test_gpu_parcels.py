@@ -58,11 +58,12 @@ def _cmp_splat(a, b, l1_rel, tot_rel):
 
 
 def test_splat_only_against_oracle():
-    """dt = 0: no motion, only the splat of 20000 parcels; each weight is at
-    most one count off (FP32 rounding), so the L1 difference is below
-    2e-5 of the total."""
+    """dt = 0: no motion, only the splat of 20000 parcels (twice: each call
+    clears the counters first); each weight is at most one count off (FP32
+    rounding), so the L1 difference is below 2e-5 of the total."""
     w, s, o, P = run_both(20000, 0.0)
     with s:
+        s.stage("PARCELS", 0.0)  # again: the counters are cleared first, not added to
         assert np.array_equal(s.parcels_download().astype(np.float64), P)
         _cmp_splat(s.splat_download(0), o.splat(0), 2e-5, 2e-6)
         assert s.splat_download(1).sum() == 0 and o.splat(1).sum() == 0
