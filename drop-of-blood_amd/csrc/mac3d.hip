@@ -717,13 +717,24 @@ int mg_smooth(Ctx *ctx, int l, int sweeps, bool zero) {
   return MAC3D_OK;
 }
 
+// The plain FP32 step forms the level-0 residual in FP64 arithmetic (FP32
+// storage): the accuracy the cycles can reach is set by the finest level's
+// residual, and an FP32 one stalls the smoothest mode at ~eps/lambda_min.
+bool fine_residual_fp64(const Ctx *ctx, int l) {
+  return l == 0 && ctx->prec == MAC3D_PREC_FP32 && ctx->mgx[0].type == TF32;
+}
+
 int mg_residual(Ctx *ctx, int l) {
   const Dims &d = ctx->d[l];
   ProfScope ps(ctx->prof, ctx->stream, l == 0 ? KC_RESID0 : KC_MG);
+  const bool acc64 = fine_residual_fp64(ctx, l);
   with_type(ctx->mgx[l].type, [&](auto tag) {
     using S = std::remove_pointer_t<decltype(tag)>;
     using C = typename compute_of<S>::type;
-    hipLaunchKernelGGL(k_residual<S>, cell_grid(d), kCellBlock, 0, ctx->stream, d,
+    auto kern = k_residual<S>;
+    if constexpr (std::is_same<S, float>::value)
+      if (acc64) kern = k_residual<S, double>;
+    hipLaunchKernelGGL(kern, cell_grid(d), kCellBlock, 0, ctx->stream, d,
                        (const S *)ctx->mgx[l].ptr, (const S *)ctx->mgb[l].ptr,
                        (S *)ctx->mgr[l].ptr, (C)level_h2(l),
                        (C)(l == 0 ? 1.0 : top_coeff(l)), l == 0 ? (const double *)ctx->scale : nullptr);
@@ -748,12 +759,16 @@ int mg_residual_restrict(Ctx *ctx, int l) {
     while (cz > 2 && (size_t)(cz + 8) * pb >= ((size_t)1 << 30)) cz -= 2;
     int nw = cols * ((df.nz + cz - 1) / cz);
     dim3 grid((nw + RC_WAVES - 1) / RC_WAVES);
+    const bool acc64 = fine_residual_fp64(ctx, l);
     with_step_type(ctx->mgx[l].type, [&](auto ftag) {
       using Sf = std::remove_pointer_t<decltype(ftag)>;
       using C = typename compute_of<Sf>::type;
       with_type(ctx->mgb[l + 1].type, [&](auto ctag) {
         using Sc = std::remove_pointer_t<decltype(ctag)>;
-        hipLaunchKernelGGL((k_residual_restrict_col<Sf, Sc>), grid, dim3(64 * RC_WAVES), 0,
+        auto kern = k_residual_restrict_col<Sf, Sc>;
+        if constexpr (std::is_same<Sf, float>::value && std::is_same<Sc, float>::value)
+          if (acc64) kern = k_residual_restrict_col<Sf, Sc, double>;
+        hipLaunchKernelGGL(kern, grid, dim3(64 * RC_WAVES), 0,
                            ctx->stream, df, dc, (const Sf *)ctx->mgx[l].ptr,
                            (const Sf *)ctx->mgb[l].ptr, (Sc *)ctx->mgb[l + 1].ptr,
                            (C)level_h2(l), (C)(l == 0 ? 1.0 : top_coeff(l)),

@@ -33,12 +33,14 @@ __device__ inline void mg_stencil(const S *__restrict__ x, const Dims &d, C topc
 
 // MG_RESIDUAL (fluid.ts:744-770): r = b scale - (sum - diag x)/h^2, with
 // scale = scale[0] (nullable: 1)
-template <class S>
+// (Acc: arithmetic type, FP64 on level 0 of the plain FP32 step as in
+// k_residual_restrict_col)
+template <class S, class Acc = typename compute_of<S>::type>
 __global__ void __launch_bounds__(256)
 k_residual(Dims d, const S *__restrict__ x, const S *__restrict__ b, S *__restrict__ r,
            typename compute_of<S>::type h2, typename compute_of<S>::type topc,
            const double *__restrict__ scale) {
-  typedef typename compute_of<S>::type C;
+  typedef Acc C;
   const C bscale = scale ? (C)scale[0] : C(1);
   int i = blockIdx.x * blockDim.x + threadIdx.x;
   int j = blockIdx.y * blockDim.y + threadIdx.y;
@@ -46,8 +48,8 @@ k_residual(Dims d, const S *__restrict__ x, const S *__restrict__ b, S *__restri
   if (i >= d.nx || j >= d.ny) return;
   size_t c = idx3(d, i, j, k);
   C sn, dg;
-  mg_stencil<S, C>(x, d, topc, i, j, k, c, sn, dg);
-  r[c] = store_as<S, C>((C)b[c] * bscale - (sn - dg * (C)x[c]) / h2);
+  mg_stencil<S, C>(x, d, (C)topc, i, j, k, c, sn, dg);
+  r[c] = store_as<S, C>((C)b[c] * bscale - (sn - dg * (C)x[c]) / (C)h2);
 }
 
 // MG_RESTRICT (fluid.ts:775-800): coarse b = 1/8 sum of the 2x2x2 children
@@ -113,15 +115,24 @@ k_residual_restrict_t(Dims df, Dims dc, const Sf *__restrict__ x, const Sf *__re
 // 0, where all are whole numbers; within an ulp where the lid coefficient is
 // not) and the division by h^2 is a product with its reciprocal (within an
 // ulp).
+//
+// Acc is the arithmetic type of the residual and the restriction sum.  With
+// FP32 storage on level 0 of the plain FP32 step it is FP64: the stored
+// values convert exactly, the neighbour sum of a smooth field and diag x
+// cancel to ~1/N^2 of their size, and an FP32 residual would carry rounding
+// of ~eps |x| that the coarse levels amplify by 1/lambda_min in the smoothest
+// mode (a relative pressure error of ~1e-2 at 256 x 384 x 256).  In FP64 the
+// residual of the stored iterate is exact to rounding of its own size, so the
+// cycles converge to the FP32-representable solution.
 #define RC_WAVES 4
 
-template <class Sf, class Sc>
+template <class Sf, class Sc, class Acc = typename compute_of<Sf>::type>
 __global__ void __launch_bounds__(64 * RC_WAVES)
 k_residual_restrict_col(Dims df, Dims dc, const Sf *__restrict__ x, const Sf *__restrict__ b,
                         Sc *__restrict__ bc, typename compute_of<Sf>::type h2,
                         typename compute_of<Sf>::type topc, const double *__restrict__ scale,
                         int cz, int ncx, int nwaves) {
-  typedef typename compute_of<Sf>::type C;
+  typedef Acc C;
   typedef typename vec2_of<Sf>::type V2;
   const int wv = __builtin_amdgcn_readfirstlane(xcd_remap(blockIdx.x, gridDim.x) * RC_WAVES +
                                                 (int)(threadIdx.x >> 6));
@@ -146,9 +157,9 @@ k_residual_restrict_col(Dims df, Dims dc, const Sf *__restrict__ x, const Sf *__
   const BufRsrc rx = buf_rsrc(x + (size_t)kx0 * nx * ny, (uint32_t)(kx1 - kx0) * pb);
   const BufRsrc rb = buf_rsrc(b + (size_t)k0 * nx * ny, (uint32_t)(k1 - k0) * pb);
   const C bscale = scale ? (C)scale[0] : C(1);
-  const C rh2 = C(1) / h2;
+  const C rh2 = C(1) / (C)h2;
   // in-plane part of the diagonal of the pair's two cells
-  const C top = (gy < ny - 1) ? C(1) : topc;
+  const C top = (gy < ny - 1) ? C(1) : (C)topc;
   const C dga = top + C((gx > 0) + (gx < nx - 1) + (gy > 0));
   const C dgb = top + C((gx + 1 > 0) + (gx + 1 < nx - 1) + (gy > 0));
   // loads for plane p: x pair of plane p+1; x-, x+, y-, y+ and b of plane p
