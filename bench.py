@@ -26,10 +26,11 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, os.path.join(ROOT, "drop-of-blood_amd"))
 
 HBM_PEAK_GBPS = 8000.0
-# algorithmic bytes per level-0 cell of one red-black sweep, as the
-# two-colour-pass algorithm moves them (read p and rhs, write p, per colour
-# pass: 2 x 12 B); the level-0 smoothing kernel runs two sweeps per launch
-# and itself moves 12 B per cell (read p and rhs once, write p once)
+# bytes per level-0 cell: the level-0 smoothing kernel runs two red-black
+# sweeps per launch and must move 12 B per cell (read p and rhs once, write p
+# once), the roofline's algorithmic bytes; the two-colour-pass algorithm
+# moves 24 B per cell per sweep (read p and rhs, write p, per colour pass),
+# reported as a sweep-equivalent rate
 SWEEP_BYTES_PER_CELL = 24.0
 SWEEPS_PER_LAUNCH = 2
 KERNEL_BYTES_PER_CELL = 12.0
@@ -139,18 +140,32 @@ def copy_probe(workload, times=5):
             s.download_into("U", out)
 
 
-def cpu_baseline(workload, steps=1):
+def cpu_model():
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def cpu_baseline(workload, steps, threads):
+    """Seconds per step of the FP64 oracle on `threads` host threads, timed
+    over `steps` steps after the first (the first touches the pages)."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle
-    threads = oracle.set_threads(int(os.environ.get("OMP_NUM_THREADS", "0") or 0))
+    used = oracle.set_threads(threads)
     o = oracle.Oracle(workload.nx, workload.ny, workload.nz, workload.dx, levels=workload.levels,
                       nu1=workload.nu1, nu2=workload.nu2, cycles=workload.cycles)
     o.init_blobs(workload.blobs)
+    o.step(workload.dt, 1)
     t0 = time.perf_counter()
     o.step(workload.dt, steps)
     dt = time.perf_counter() - t0
     o.close()
-    return steps / dt, threads, dt
+    return dt / steps, used
 
 
 def main():
@@ -233,15 +248,20 @@ def main():
     if "SMOOTH2_L0" in prof:
         ms, n = prof["SMOOTH2_L0"]
         avg_s = ms / n / 1e3
-        bytes_launch = SWEEP_BYTES_PER_CELL * SWEEPS_PER_LAUNCH * w32.cells
-        ach = bytes_launch / avg_s / 1e9
+        # achieved: the bytes the fused two-sweep launch must move (read x
+        # and b once, write x once: 12 B per cell), so frac is a fraction of
+        # the HBM peak; the sweep-equivalent rate charges the two-colour-pass
+        # algorithm's 24 B per cell per sweep and can exceed the peak
+        kbytes = KERNEL_BYTES_PER_CELL * w32.cells
+        ach = kbytes / avg_s / 1e9
+        sweep_bytes = SWEEP_BYTES_PER_CELL * SWEEPS_PER_LAUNCH * w32.cells
         roof = {"bound": "hbm", "achieved": round(ach, 1), "peak": HBM_PEAK_GBPS, "unit": "GB/s",
                 "frac": round(ach / HBM_PEAK_GBPS, 4), "traffic": None,
                 "kernel": "k_rbgs_sweep2 (level 0, two red-black sweeps per launch)",
                 "avg_launch_us": round(avg_s * 1e6, 2), "launches": n,
-                "bytes_per_launch": bytes_launch,
-                "kernel_bytes_per_launch": KERNEL_BYTES_PER_CELL * w32.cells,
-                "kernel_gbps": round(KERNEL_BYTES_PER_CELL * w32.cells / avg_s / 1e9, 1)}
+                "bytes_per_launch": kbytes,
+                "sweep_equivalent_bytes_per_launch": sweep_bytes,
+                "sweep_equivalent_gbps": round(sweep_bytes / avg_s / 1e9, 1)}
         traffic_file = os.path.join(ROOT, "profiles", "rbgs2_l0_192x256x192_traffic.json")
         if os.path.exists(traffic_file):
             with open(traffic_file) as f:
@@ -249,17 +269,27 @@ def main():
             roof["traffic"] = t.get("bytes_per_launch")
             if roof["traffic"]:
                 roof["traffic_gbps"] = round(roof["traffic"] / avg_s / 1e9, 1)
+                roof["traffic_frac"] = round(roof["traffic"] / avg_s / 1e9 / HBM_PEAK_GBPS, 4)
     kernel_ms = {k: round(v[0] / v[1], 4) for k, v in prof.items()}
     if args.copy_probe:
         copy_probe(w32)
 
     cpu = None
     if not args.no_cpu:
-        log("[bench] cpu baseline (FP64 oracle, 1 step at full size)")
-        cps, threads, secs = cpu_baseline(w32, 1)
-        cpu = {"value": round(cps, 5), "unit": "steps/s", "cores": threads, "kind": "port",
-               "sample": f"1 step (8 V-cycles) of the 192x256x192 workload, FP64 C oracle, "
-                         f"OpenMP {threads} threads, {secs:.1f} s"}
+        # the box's CPU share is 16 threads (OMP_NUM_THREADS there)
+        threads = int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or min(16, os.cpu_count() or 1)
+        log(f"[bench] cpu baseline: FP64 oracle, 2 steps on {threads} threads, 1 on 1 thread")
+        s_all, used = cpu_baseline(w32, 2, threads)
+        s_one, _ = cpu_baseline(w32, 1, 1)
+        cpu = {"value": round(1.0 / s_all, 5), "unit": "steps/s", "cores": used, "kind": "port",
+               "cpu_model": cpu_model(), "host_threads_visible": os.cpu_count(),
+               "seconds_per_step_all_threads": round(s_all, 3),
+               "seconds_per_step_single_thread": round(s_one, 3),
+               "single_thread_steps_per_s": round(1.0 / s_one, 5),
+               "sample": f"the 192x256x192 FP32 workload's step (8 V-cycles) in the FP64 C "
+                         f"oracle after one untimed step: 2 steps on {used} OpenMP threads "
+                         f"({2 * s_all:.1f} s) and 1 step on 1 thread ({s_one:.1f} s)"}
+        log(f"[bench] cpu {s_all:.2f} s/step on {used} threads, {s_one:.2f} s/step on 1")
 
     line = {
         "metric": "MAC steps/sec @192x256x192 (mixed vs FP32, div<=1e-6); HBM GB/s % peak",

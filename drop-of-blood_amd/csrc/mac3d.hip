@@ -1959,6 +1959,13 @@ int mac3d_set_uniforms(void *h, const float *u, size_t count) {
   const float half_xz = u[52], depth = u[54] - u[53];
   if (!(half_xz > 0.f) || !(depth > 0.f))
     return fail(ctx, MAC3D_EINVAL, "domainBounds must give half_xz > 0 and yMax > yMin");
+  // the coupling's tank top is the still surface (the reference scene packs
+  // surfaceParams.x = domainBounds.z, uniformLayout.ts:97,150-153); a block
+  // with the two apart would shift the world-to-grid mapping of the scene
+  // terms and the parcels
+  if (u[40] != u[54])
+    return fail(ctx, MAC3D_EINVAL, "surfaceParams.x (surface height) must equal domainBounds.z "
+                                   "(tank top)");
   mac3d_drop d[MAC3D_MAX_DROPS];
   for (int q = 0; q < MAC3D_MAX_DROPS; ++q) {
     const float *b = u + 64 + 8 * q;
@@ -1987,6 +1994,19 @@ int mac3d_parcels_upload(void *h, const float *parcels, int32_t n) {
   Ctx *ctx = (Ctx *)h;
   if (n < 0 || (n > 0 && !parcels)) return fail(ctx, MAC3D_EINVAL, "need n >= 0 parcels");
   if (!ctx->wave_on) return fail(ctx, MAC3D_ESTATE, "the wave coupling is not enabled");
+  // the splat counters hold 65536 per unit mass in 32 bits: a cell overflows
+  // at 65536 units, so the parcels' total mass (which a step never raises)
+  // must stay below that
+  double total = 0.0;
+  for (int32_t q = 0; q < n; ++q) {
+    const float m = parcels[8 * q + 3];
+    if (!std::isfinite(m) || m < 0.f)
+      return fail(ctx, MAC3D_EINVAL, "parcel masses must be finite and >= 0");
+    total += m;
+  }
+  if (total >= 65536.0)
+    return fail(ctx, MAC3D_EINVAL, "total parcel mass must stay below 65536 (32-bit splat "
+                                   "counters at 65536 per unit mass)");
   int rc;
   if (n > ctx->np_cap) {  // (an earlier buffer stays owned until destroy)
     if ((rc = dev_alloc(ctx, (void **)&ctx->parcels, (size_t)n * sizeof(Parcel)))) return rc;
@@ -2103,6 +2123,12 @@ int mac3d_stage(void *h, int stage, double dt) {
   int rc;
   if (stage == MAC3D_STAGE_PARCELS) {  // (reads the velocity, changes none of it)
     if (!ctx->wave_on || !ctx->splat) return fail(ctx, MAC3D_ESTATE, "needs the coupling and parcels");
+    // the parcels sample the centre velocity of the current faces: form it
+    // first when the faces changed since the last face->centre pass
+    if (!ctx->cvel_fresh) {
+      if ((rc = launch_face_to_centre(ctx))) return rc;
+      ctx->cvel_fresh = true;
+    }
     if ((rc = launch_parcels(ctx, dt))) return rc;
     HIPCHK(hipStreamSynchronize(ctx->stream));
     ctx->prof.collect();
@@ -2128,6 +2154,7 @@ int mac3d_stage(void *h, int stage, double dt) {
   if (rc) return rc;
   HIPCHK(hipStreamSynchronize(ctx->stream));
   ctx->prof.collect();
+  if (stage == MAC3D_STAGE_FACE_TO_CENTRE) ctx->cvel_fresh = true;
   return MAC3D_OK;
 }
 

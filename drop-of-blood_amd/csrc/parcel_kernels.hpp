@@ -7,8 +7,11 @@
 // The splat adds integer weights with atomics, so the counters do not depend
 // on the order in which parcels land.  Device-scope atomics bound the stage;
 // two x-neighbour cells in one aligned 64-bit word take one 64-bit atomic,
-// which equals two 32-bit adds while no counter reaches 2^32 (32,768 units of
-// mass in one cell).  Each weight is
+// which equals two 32-bit adds while no counter reaches 2^32 (65,536 units of
+// mass in one cell).  Past that the low counter would carry into its x + 1
+// neighbour, where the reference's 32-bit atomics wrap; mac3d_parcels_upload
+// keeps the parcels' total mass below 65,536 units, so no counter gets there
+// (the reference's 393,216 parcels of 0.134 hold 52,691).  Each weight is
 // round(mass 65536 w) with w a product of trilinear factors, round-half-even
 // as WGSL's round.
 #pragma once

@@ -5,8 +5,9 @@ The device computes in FP32, as the reference's parcels do; positions and
 masses are compared at FP32 tolerance.  The splat counters are integer sums
 of round(65536 m w): a weight can round the other way where the FP32 and
 FP64 products straddle a half, and a parcel near one of the step's branch
-thresholds can take the other branch, so the counters are compared by their
-totals and their L1 difference, with the bounds written in each test."""
+thresholds could take the other branch, so the counters are compared by their
+totals and their L1 difference, with the bounds written in each test; the
+parcels themselves are bounded one by one."""
 import numpy as np
 import pytest
 
@@ -73,9 +74,13 @@ def test_splat_only_against_oracle():
 def test_parcel_step_against_oracle():
     """One step (dt 0.5 ms) of 20000 parcels through the workload's flow:
     RK3/RK2 advection with settling and unfurling, stretch, floor bed,
-    hemolysis and both splats.  Positions within 1e-6 m and masses within
-    1e-4 relative for 99.5% of the parcels; counters within 1e-3 (L1) and
-    1e-4 (totals)."""
+    hemolysis and both splats.  Every parcel: position within 1e-6 m, mass
+    within 1e-5 relative, stretch within 1e-4 relative (measured maxima over
+    the 20000: 1.9e-7 m, 5.3e-7, 1.4e-5; none of these parcels lies close
+    enough to a branch threshold of the step -- RK3 above speed^2 0.0025, the
+    stretch update, the ribbon splat above stretch 3.8 mm, the hemolysis
+    headroom -- for FP32 rounding to change its branch); age and active flag
+    equal; counters within 1e-3 (L1) and 1e-4 (totals)."""
     w, s, o, P = run_both(20000, 0.0005)
     with s:
         a, b = s.parcels_download().astype(np.float64), o.parcels()
@@ -86,9 +91,9 @@ def test_parcel_step_against_oracle():
         dm = np.abs(a[:, 3] - b[:, 3]) / np.maximum(b[:, 3], 1e-6)
         dst = np.linalg.norm(a[:, 4:7] - b[:, 4:7], axis=1) / np.maximum(
             np.linalg.norm(b[:, 4:7], axis=1), 1e-6)
-        for name, e, tol in (("position", dpos, 1e-6), ("mass", dm, 1e-4), ("stretch", dst, 1e-4)):
-            frac = np.mean(e > tol)
-            assert frac <= 0.005, f"{name}: {frac:.2%} of parcels beyond {tol}"
+        for name, e, tol in (("position", dpos, 1e-6), ("mass", dm, 1e-5), ("stretch", dst, 1e-4)):
+            worst = int(np.argmax(e))
+            assert e[worst] <= tol, f"{name}: parcel {worst} off by {e[worst]:.3e} > {tol}"
         assert np.all(np.abs(np.floor(a[:, 7]) - np.floor(b[:, 7])) == 0)
         _cmp_splat(s.splat_download(0), o.splat(0), 1e-3, 1e-4)
         _cmp_splat(s.splat_download(1), o.splat(1), 1e-3, 1e-4)
@@ -111,3 +116,44 @@ def test_parcel_calls_need_the_coupling():
         assert s.splat_download(0).sum() == 0
         with pytest.raises(Mac3dError, match="EINVAL"):
             s.parcels_download(5)
+
+
+def test_parcels_after_a_velocity_upload():
+    """The parcel stage samples the centre velocity of the current faces:
+    after an upload of U the library forms it first, so the stage equals an
+    explicit face->centre pass followed by the stage, bit for bit."""
+    res = []
+    for explicit in (False, True):
+        w, s, o = pair(warm_steps=2)
+        o.close()
+        with s:
+            s.wave_enable(n=16, **BOX)
+            s.stage("FACE_TO_CENTRE")
+            s.parcels_upload(cloud(5000, seed=3))
+            U = s.download("U")
+            s.upload("U", U * 1.5 + 0.01)
+            if explicit:
+                s.stage("FACE_TO_CENTRE")
+            s.stage("PARCELS", 0.0005)
+            res.append((s.parcels_download(), s.splat_download(0), s.splat_download(1)))
+    for x, y in zip(*res):
+        assert np.array_equal(x, y)
+
+
+def test_parcel_upload_rejects_counter_overflow():
+    """The splat counters hold 65536 per unit mass in 32 bits, so parcels
+    whose total mass reaches 65536 units are refused (EINVAL), as are
+    negative or non-finite masses."""
+    w, s, o = pair(warm_steps=0)
+    o.close()
+    with s:
+        s.wave_enable(n=16, **BOX)
+        P = cloud(4)
+        P[:, 3] = 16384.0
+        with pytest.raises(Mac3dError, match="EINVAL"):
+            s.parcels_upload(P)
+        P[:, 3] = 16383.0
+        s.parcels_upload(P)
+        P[0, 3] = -1.0
+        with pytest.raises(Mac3dError, match="EINVAL"):
+            s.parcels_upload(P)

@@ -163,7 +163,7 @@ def test_packed_uniform_block_sets_the_coupling():
     o.close()
     blk = np.zeros(128, dtype=np.float32)
     blk[40], blk[41] = 0.0, 0.8                       # surfaceY, marangoni
-    blk[52], blk[53], blk[54], blk[55] = 0.016, -0.032, 0.016, 1.1  # domainBounds
+    blk[52], blk[53], blk[54], blk[55] = 0.016, -0.048, 0.0, 1.1  # domainBounds (top = surface)
     for q, d in enumerate(DROPS):
         blk[64 + 8 * q: 72 + 8 * q] = d
     depth = float(np.float32(blk[54]) - np.float32(blk[53]))
@@ -183,6 +183,11 @@ def test_packed_uniform_block_sets_the_coupling():
         assert np.array_equal(s1.download("PTOP"), s2.download("PTOP"))
         with pytest.raises(Mac3dError, match="EINVAL"):
             s1.set_uniforms(blk[:64])
+        # the tank top (domainBounds.z) must be the surface height
+        bad = blk.copy()
+        bad[54] = 0.004
+        with pytest.raises(Mac3dError, match="EINVAL"):
+            s1.set_uniforms(bad)
 
 
 def test_scene_advection_against_oracle():

@@ -159,3 +159,55 @@ def test_hemolysis_and_bed_conserve_mass():
     bed_max = (P[near, 3] * 0.095).sum()
     assert moved - lysed <= 0.32 * bed_max + 8 * n / S
     assert abs(intact - Q[:, 3].sum()) <= 8 * n / S
+
+
+def _settle_young(y, surface=0.0):
+    """settling speed of a young parcel with no meniscus hold at height y
+    (particles.ts:201-215): mono(b) gate(b), b the depth below the surface"""
+    ss = oracle.smoothstep
+    b = surface - y
+    return (0.020 + 0.026 * ss(0.04, 0.42, b)) * (0.38 + 0.62 * ss(0.002, 0.055, b))
+
+
+@pytest.mark.parametrize("fast", [True, False])
+def test_advection_in_a_linear_flow_is_ralston_rk3(fast):
+    """A parcel in an affine centre-velocity field c(p) = A + B p (trilinear
+    sampling reproduces it exactly inside the tank), 2 cm under the surface
+    (no unfurling, no floor or wall terms), young and with no meniscus hold:
+    its velocity is v(p) = c(p) - settle(y) e_y.  Above speed^2 0.0025 the
+    step is Ralston's RK3, p + dt (2 k1 + 3 k2 + 4 k3) / 9 with k2 at
+    p + dt k1 / 2 and k3 at p + 3 dt k2 / 4; below it the RK2 midpoint
+    p + dt k2 (particles.ts:310-336).  Both against the closed form to
+    rounding, and each far from the other rule."""
+    o = tank(nx=16, ny=48, nz=16, depth=0.1)
+    nz, ny, nx = o.shape
+    hx, depth = 0.008, 0.1
+    x = -hx + (np.arange(nx) + 0.5) * (2 * hx / nx)
+    y = -depth + (np.arange(ny) + 0.5) * (depth / ny)
+    z = -hx + (np.arange(nz) + 0.5) * (2 * hx / nz)
+    Z, Y, X = np.meshgrid(z, y, x, indexing="ij")
+    A = np.array([0.07, 0.02, 0.05]) if fast else np.array([0.012, 0.004, -0.008])
+    B = np.array([[0.0, 0.8, 0.5], [1.5, 0.0, -0.7], [0.4, 0.9, -1.0]])
+    P3 = np.stack([X, Y, Z], axis=-1)
+    c = A + P3 @ B.T
+    for a, f in enumerate((oracle.F_CU, oracle.F_CV, oracle.F_CW)):
+        o.set(f, c[..., a])
+
+    def v(p):
+        w = A + B @ p
+        w[1] -= _settle_young(p[1])
+        return w
+    p0 = np.array([0.0011, -0.020, -0.0007])
+    o.parcels_set([one(p0, mass=0.1)])
+    dt = 0.01
+    o.stage(oracle.ST_PARCELS, dt)
+    got = o.parcels()[0][:3]
+    k1 = v(p0)
+    k2 = v(p0 + 0.5 * dt * k1)
+    k3 = v(p0 + 0.75 * dt * k2)
+    rk3 = p0 + dt * (2 * k1 + 3 * k2 + 4 * k3) / 9.0
+    rk2 = p0 + dt * k2
+    assert (k1 @ k1 > 0.0025) == fast
+    want, other = (rk3, rk2) if fast else (rk2, rk3)
+    assert np.abs(got - want).max() < 1e-15
+    assert np.abs(other - want).max() > 1e-9

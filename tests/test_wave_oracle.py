@@ -557,3 +557,61 @@ def test_scene_top_row_stagnation_gate():
     # (negative values: the least held back is the minimum)
     assert np.isclose(top.min(), -0.1)
     assert np.isclose(top.max(), -0.1 * (0.45 + 0.55 * ss(0.002, 0.018, 0.005)))
+
+
+def test_scene_neck_draft_on_the_diagonals():
+    """The neck draft of the y-faces on the two diagonals through a drop's
+    axis, theta = pi/4 (5 pi/4) and -pi/4 (3 pi/4), where the azimuthal
+    modulation takes closed values: cos(4 theta + a) = -cos a,
+    cos(6 theta - b) = -+sin b, sin(8 theta + c) = sin c, so
+      azim  = 1 - 0.036 cos(1.73 s) -+ 0.026 sin(2.31 s) + 0.018 sin(0.89 s)
+      lobe  = 1 - 0.036 cos(1.73 s) -+ 0.026 sin(2.31 s)
+    (- on the main diagonal, + on the other).  With r = sqrt(2) |q| h the
+    draft target, mask and blend of fluid.ts:198-245 follow as on the axis.
+    A wrong multiple of theta in any of the three modes, or a swapped sine
+    and cosine, changes azim or lobe on one of the diagonals."""
+    hx, nx = 0.008, 16
+    h = 2 * hx / nx
+    x0 = -hx + 8.5 * h
+    rd, vy, drain, seed = 0.02, -0.3, 0.4, 0.9
+    a, b = scene_pair(drops=[(x0, 0.01, x0, rd, vy, 0.0, drain, seed)])
+    dt = 0.002
+    advect_pair(a, b, dt)
+    sp = sponge_of(a)
+    ss = oracle.smoothstep
+    nz, ny, _ = a.shape
+    slug = ss(0.14, 0.76, drain)
+    rbase = min(max(rd * (0.80 + 0.28 * slug) + 0.012 + 0.018 * slug, 0.018), 0.088)
+    flux = (min(max(0.045 + drain * 0.22, 0), 0.2) * ss(0, 0.05, drain)) * (1 - slug) + \
+        min(max(max(drain, 0) ** 0.66 * 0.84, 0), 1.24) * slug
+    impact = min(max(-vy, 0.10), 0.65)
+
+    def radial(rr):
+        sl = (0.14 + min(rr * rr, 1.20)) * (1 - ss(0.78, 1.15, rr))
+        st = (0.68 + 0.32 * min(rr * rr, 1.0)) * (1 - ss(0.74, 1.10, rr))
+        return st + (sl - st) * slug
+    va, vb = a.get(oracle.F_V), b.get(oracle.F_V)
+    checked = 0
+    for sgn, cell in ((-1.0, lambda q: (8 + q, 8 + q)), (1.0, lambda q: (8 - q, 8 + q))):
+        azim = 1 - 0.036 * math.cos(1.73 * seed) + sgn * 0.026 * math.sin(2.31 * seed) + \
+            0.018 * math.sin(0.89 * seed)
+        lobe = 1 - 0.036 * math.cos(1.73 * seed) + sgn * 0.026 * math.sin(2.31 * seed)
+        for q in range(-7, 8):
+            if q == 0:
+                continue
+            k, i = cell(q)
+            r = math.sqrt(2.0) * abs(q) * h
+            rad = radial(r / max(rbase * azim, 1e-4))
+            for j in range(ny - 6, ny - 3):
+                fy = -0.024 + (j + 1.0) / ny * 0.024
+                mask = rad * ss(-0.058, -0.014, fy)
+                want = vb[k, j, i]
+                if mask > 0.001:
+                    shear = 1 + (0.36 - 1) * ss(-0.024, -0.002, fy)
+                    target = -min(max(0.14 + 0.64 * impact, 0.15), 0.39) * flux * lobe * shear
+                    blend = min(max(dt * (14 + 8 * slug) * mask, 0), 1)
+                    want = want * (1 - blend) + target * blend
+                    checked += 1
+                assert va[k, j, i] == pytest.approx(want * sp[k, j, i], rel=1e-12, abs=1e-15), \
+                    (sgn, q, j)
+    assert checked > 40
