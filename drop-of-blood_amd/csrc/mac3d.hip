@@ -1686,6 +1686,42 @@ int adjoint_impl(Ctx *ctx, const mac3d_blob *blobs, int nb, double dt, int nstep
 
 thread_local std::string g_create_err;
 
+// ---- checkpoints: the state a step evolves, in one caller-owned byte buffer
+// (a header, then the sections in a fixed order; layout in mac3d.h)
+struct CkptHeader {
+  char magic[8];
+  int32_t abi, nx, ny, nz, levels, precision, elem;
+  int32_t wave_n, np, hist, iter_n;
+  int32_t iter_log[4];
+  uint64_t bytes;
+};
+static_assert(sizeof(CkptHeader) == 80, "checkpoint header layout");
+const char kCkptMagic[8] = {'M', 'A', 'C', '3', 'D', 'C', 'K', '1'};
+
+struct CkptSec {
+  void *dev;
+  size_t bytes;
+};
+// the device ranges of a checkpoint's sections, in order
+std::vector<CkptSec> ckpt_sections(Ctx *ctx, int wave_n, int np) {
+  const size_t n0 = ctx->n[0], es = type_size(ctx->t0);
+  std::vector<CkptSec> v = {{ctx->u.ptr, es * n0}, {ctx->v.ptr, es * n0}, {ctx->w.ptr, es * n0},
+                            {ctx->dye.ptr, es * n0}};
+  if (ctx->prec == MAC3D_PREC_MIXED)  // the current pressure and the two before it, FP64
+    for (int q = 0; q < 3; ++q) v.push_back({ctx->pr[ctx->role[q]].ptr, 8 * n0});
+  else
+    v.push_back({ctx->p.ptr, es * n0});
+  if (wave_n > 0) v.push_back({ctx->wave.ptr, 4 * sizeof(float) * (size_t)wave_n * wave_n});
+  if (np > 0) v.push_back({(void *)ctx->parcels, (size_t)np * sizeof(Parcel)});
+  return v;
+}
+size_t ckpt_bytes(Ctx *ctx, int wave_n, int np) {
+  size_t b = sizeof(CkptHeader);
+  for (const CkptSec &c : ckpt_sections(ctx, wave_n, np)) b += c.bytes;
+  return b;
+}
+int ckpt_wave_n(const Ctx *ctx) { return ctx->wave_on ? ctx->wp.n : 0; }
+
 }  // namespace
 
 // ===========================================================================
@@ -2093,6 +2129,94 @@ int mac3d_step(void *h, double dt, int nsteps, mac3d_stats *stats) {
     stats->l2_div = l2;
     stats->max_div_rate = mx / ctx->cfg.dx;
   }
+  return MAC3D_OK;
+}
+
+int mac3d_checkpoint_size(void *h, size_t *bytes) {
+  if (!h) return MAC3D_EINVAL;
+  Ctx *ctx = (Ctx *)h;
+  if (!bytes) return fail(ctx, MAC3D_EINVAL, "null size pointer");
+  *bytes = ckpt_bytes(ctx, ckpt_wave_n(ctx), ctx->np);
+  return MAC3D_OK;
+}
+
+int mac3d_checkpoint_save(void *h, void *host, size_t bytes) {
+  if (!h) return MAC3D_EINVAL;
+  Ctx *ctx = (Ctx *)h;
+  const int wn = ckpt_wave_n(ctx), np = ctx->np;
+  const size_t need = ckpt_bytes(ctx, wn, np);
+  if (!host || bytes < need)
+    return fail(ctx, MAC3D_EINVAL, "checkpoint buffer is null or smaller than mac3d_checkpoint_size");
+  CkptHeader hd;
+  std::memset(&hd, 0, sizeof(hd));
+  std::memcpy(hd.magic, kCkptMagic, 8);
+  hd.abi = MAC3D_ABI_VERSION;
+  hd.nx = ctx->cfg.nx;
+  hd.ny = ctx->cfg.ny;
+  hd.nz = ctx->cfg.nz;
+  hd.levels = ctx->L;
+  hd.precision = ctx->prec;
+  hd.elem = (int32_t)type_size(ctx->t0);
+  hd.wave_n = wn;
+  hd.np = np;
+  hd.hist = ctx->hist;
+  hd.iter_n = ctx->iter_n;
+  for (int q = 0; q < 4; ++q) hd.iter_log[q] = ctx->iter_log[q];
+  hd.bytes = need;
+  std::memcpy(host, &hd, sizeof(hd));
+  char *dst = (char *)host + sizeof(hd);
+  for (const CkptSec &c : ckpt_sections(ctx, wn, np)) {
+    HIPCHK(hipMemcpyAsync(dst, c.dev, c.bytes, hipMemcpyDeviceToHost, ctx->stream));
+    dst += c.bytes;
+  }
+  HIPCHK(hipStreamSynchronize(ctx->stream));
+  return MAC3D_OK;
+}
+
+int mac3d_checkpoint_load(void *h, const void *host, size_t bytes) {
+  if (!h) return MAC3D_EINVAL;
+  Ctx *ctx = (Ctx *)h;
+  CkptHeader hd;
+  if (!host || bytes < sizeof(hd)) return fail(ctx, MAC3D_EINVAL, "checkpoint buffer too small");
+  std::memcpy(&hd, host, sizeof(hd));
+  if (std::memcmp(hd.magic, kCkptMagic, 8) != 0 || hd.abi != MAC3D_ABI_VERSION)
+    return fail(ctx, MAC3D_EINVAL, "not a mac3d checkpoint of this ABI version");
+  if (hd.nx != ctx->cfg.nx || hd.ny != ctx->cfg.ny || hd.nz != ctx->cfg.nz || hd.levels != ctx->L ||
+      hd.precision != ctx->prec || hd.elem != (int32_t)type_size(ctx->t0))
+    return fail(ctx, MAC3D_EINVAL, "checkpoint grid, levels or precision differ from the context's");
+  if (hd.wave_n != ckpt_wave_n(ctx))
+    return fail(ctx, MAC3D_EINVAL, "checkpoint wave grid differs from the context's coupling "
+                                   "(enable it with the same n first, or disable it)");
+  if (hd.np < 0 || (hd.np > 0 && !ctx->wave_on))
+    return fail(ctx, MAC3D_EINVAL, "checkpoint holds parcels: enable the wave coupling first");
+  if (hd.hist < 0 || hd.hist > 2 || hd.iter_n < 0 || hd.iter_n > 4)
+    return fail(ctx, MAC3D_EINVAL, "checkpoint header is corrupt");
+  for (int q = 0; q < 4; ++q)
+    if (hd.iter_log[q] < 0) return fail(ctx, MAC3D_EINVAL, "checkpoint header is corrupt");
+  const size_t need = ckpt_bytes(ctx, hd.wave_n, hd.np);
+  if (hd.bytes != need || bytes < need)
+    return fail(ctx, MAC3D_EINVAL, "checkpoint size does not match its header");
+  int rc;
+  if (hd.np > ctx->np_cap) {  // (an earlier buffer stays owned until destroy)
+    if ((rc = dev_alloc(ctx, (void **)&ctx->parcels, (size_t)hd.np * sizeof(Parcel)))) return rc;
+    ctx->np_cap = hd.np;
+  }
+  if (hd.np > 0 && !ctx->splat &&
+      (rc = dev_alloc(ctx, (void **)&ctx->splat, 2 * ctx->n[0] * sizeof(unsigned))))
+    return rc;
+  const char *src = (const char *)host + sizeof(hd);
+  for (const CkptSec &c : ckpt_sections(ctx, hd.wave_n, hd.np)) {
+    HIPCHK(hipMemcpyAsync(c.dev, src, c.bytes, hipMemcpyHostToDevice, ctx->stream));
+    src += c.bytes;
+  }
+  HIPCHK(hipStreamSynchronize(ctx->stream));
+  ctx->np = hd.np;
+  ctx->hist = hd.hist;
+  ctx->iter_n = hd.iter_n;
+  for (int q = 0; q < 4; ++q) ctx->iter_log[q] = hd.iter_log[q];
+  ctx->cvel_fresh = false;
+  ctx->dmm_valid = false;
+  sync_p(ctx);
   return MAC3D_OK;
 }
 

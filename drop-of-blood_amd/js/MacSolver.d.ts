@@ -49,5 +49,7 @@ export declare class MacSolver {
            run: { blobs: Blob[]; dt: number; steps: number; checkpointEvery?: number }):
     { loss: number; dRhoB: number; dBlobs: Blob[] };
   divNorms(): { maxDiv: number; l2Div: number };
+  saveCheckpoint(): Uint8Array;
+  loadCheckpoint(bytes: Uint8Array | ArrayBuffer): void;
   destroy(): void;
 }

@@ -175,6 +175,21 @@ class MacSolver {
     return addon.divNorms(this._live());
   }
 
+  /**
+   * The evolving state (fields, pressure history, wave, parcels) as bytes the
+   * caller keeps; loadCheckpoint resumes from them (the reference resumes by
+   * replaying from a reset, BloodDropEngine.ts:2142-2179).
+   * @returns {Uint8Array}
+   */
+  saveCheckpoint() {
+    return new Uint8Array(addon.checkpointSave(this._live()));
+  }
+
+  /** @param {Uint8Array|ArrayBuffer} bytes a checkpoint of a solver of the same config */
+  loadCheckpoint(bytes) {
+    addon.checkpointLoad(this._live(), bytes);
+  }
+
   /** Idempotent, like the reference engine's destroy() guard. */
   destroy() {
     if (this.destroyed) return;

@@ -83,7 +83,8 @@ EXPORTS = ("mac3d_default_config", "mac3d_create", "mac3d_destroy", "mac3d_uploa
            "mac3d_div_norms", "mac3d_profile", "mac3d_profile_read", "mac3d_adjoint",
            "mac3d_last_error", "mac3d_version", "mac3d_wave_default_params",
            "mac3d_wave_enable", "mac3d_set_drops", "mac3d_set_uniforms", "mac3d_parcels_upload",
-           "mac3d_parcels_download", "mac3d_splat_download")
+           "mac3d_parcels_download", "mac3d_splat_download", "mac3d_checkpoint_size",
+           "mac3d_checkpoint_save", "mac3d_checkpoint_load")
 
 
 def load_library(path: Optional[str] = None):
@@ -117,6 +118,9 @@ def load_library(path: Optional[str] = None):
     L.mac3d_parcels_upload.argtypes = [vp, vp, C.c_int32]
     L.mac3d_parcels_download.argtypes = [vp, vp, C.c_int32]
     L.mac3d_splat_download.argtypes = [vp, C.c_int, vp, C.c_size_t]
+    L.mac3d_checkpoint_size.argtypes = [vp, C.POINTER(C.c_size_t)]
+    L.mac3d_checkpoint_save.argtypes = [vp, vp, C.c_size_t]
+    L.mac3d_checkpoint_load.argtypes = [vp, vp, C.c_size_t]
     L.mac3d_last_error.argtypes = [vp]
     L.mac3d_last_error.restype = C.c_char_p
     L.mac3d_version.restype = C.c_char_p
@@ -308,6 +312,23 @@ class MacSolver:
         self._check(self.L.mac3d_adjoint(self.h, arr, len(blobs), dt, steps, checkpoint_every,
                                          t.ctypes.data, F64, C.byref(g)))
         return g.as_dict()
+
+    # -- checkpoints
+    def checkpoint_save(self) -> np.ndarray:
+        """The evolving state (fields, pressure history, wave, parcels) as a
+        byte array (mac3d_checkpoint_save)."""
+        n = C.c_size_t()
+        self._check(self.L.mac3d_checkpoint_size(self.h, C.byref(n)))
+        buf = np.empty(n.value, dtype=np.uint8)
+        self._check(self.L.mac3d_checkpoint_save(self.h, buf.ctypes.data, buf.size))
+        return buf
+
+    def checkpoint_load(self, buf) -> None:
+        b = np.ascontiguousarray(buf, dtype=np.uint8)
+        self._check(self.L.mac3d_checkpoint_load(self.h, b.ctypes.data, b.size))
+        # the parcel count is the header's int32 after magic (8 bytes) and 8
+        # int32 fields (ABI, nx, ny, nz, levels, precision, element size, wave n)
+        self.n_parcels = int(np.frombuffer(b[40:44].tobytes(), dtype="<i4")[0])
 
     def stage(self, stage, dt: float = 0.0):
         sid = STAGE[stage] if isinstance(stage, str) else stage

@@ -441,6 +441,55 @@ napi_value Adjoint(napi_env env, napi_callback_info info) {
   return o;
 }
 
+// checkpointSave(h) -> ArrayBuffer holding mac3d_checkpoint_save's bytes
+napi_value CheckpointSave(napi_env env, napi_callback_info info) {
+  napi_value argv[1];
+  if (!get_args(env, info, 1, argv)) return nullptr;
+  void *ctx = get_handle(env, argv[0]);
+  if (!ctx) return nullptr;
+  size_t n = 0;
+  int rc = mac3d_checkpoint_size(ctx, &n);
+  if (rc) return throw_status(env, ctx, rc);
+  void *data = nullptr;
+  napi_value ab;
+  NAPI_CALL(env, napi_create_arraybuffer(env, n, &data, &ab));
+  rc = mac3d_checkpoint_save(ctx, data, n);
+  if (rc) return throw_status(env, ctx, rc);
+  return ab;
+}
+
+// checkpointLoad(h, Uint8Array | ArrayBuffer)
+napi_value CheckpointLoad(napi_env env, napi_callback_info info) {
+  napi_value argv[2];
+  if (!get_args(env, info, 2, argv)) return nullptr;
+  void *ctx = get_handle(env, argv[0]);
+  if (!ctx) return nullptr;
+  void *data = nullptr;
+  size_t n = 0;
+  bool is_ab = false, is_ta = false;
+  napi_is_arraybuffer(env, argv[1], &is_ab);
+  napi_is_typedarray(env, argv[1], &is_ta);
+  if (is_ab) {
+    NAPI_CALL(env, napi_get_arraybuffer_info(env, argv[1], &data, &n));
+  } else if (is_ta) {
+    napi_typedarray_type t;
+    size_t len = 0, off = 0;
+    napi_value ab;
+    NAPI_CALL(env, napi_get_typedarray_info(env, argv[1], &t, &len, &data, &ab, &off));
+    if (t != napi_uint8_array) {
+      napi_throw_type_error(env, nullptr, "checkpoint must be a Uint8Array or an ArrayBuffer");
+      return nullptr;
+    }
+    n = len;
+  } else {
+    napi_throw_type_error(env, nullptr, "checkpoint must be a Uint8Array or an ArrayBuffer");
+    return nullptr;
+  }
+  int rc = mac3d_checkpoint_load(ctx, data, n);
+  if (rc) return throw_status(env, ctx, rc);
+  return nullptr;
+}
+
 napi_value Version(napi_env env, napi_callback_info) {
   napi_value s;
   NAPI_CALL(env, napi_create_string_utf8(env, mac3d_version(), NAPI_AUTO_LENGTH, &s));
@@ -465,6 +514,8 @@ napi_value Init(napi_env env, napi_value exports) {
       {"stage", nullptr, Stage, nullptr, nullptr, nullptr, napi_default, nullptr},
       {"divNorms", nullptr, DivNorms, nullptr, nullptr, nullptr, napi_default, nullptr},
       {"adjoint", nullptr, Adjoint, nullptr, nullptr, nullptr, napi_default, nullptr},
+      {"checkpointSave", nullptr, CheckpointSave, nullptr, nullptr, nullptr, napi_default, nullptr},
+      {"checkpointLoad", nullptr, CheckpointLoad, nullptr, nullptr, nullptr, napi_default, nullptr},
       {"version", nullptr, Version, nullptr, nullptr, nullptr, napi_default, nullptr},
   };
   napi_define_properties(env, exports, sizeof(props) / sizeof(props[0]), props);

@@ -198,6 +198,29 @@ int mac3d_init_blobs(void *ctx, const mac3d_blob *blobs, int nblobs);
  * dt must be finite and > 0 (EINVAL otherwise, state untouched). */
 int mac3d_step(void *ctx, double dt, int nsteps, mac3d_stats *stats);
 
+/* Checkpoints of the evolving state, in a byte buffer the caller owns.
+ * replaces: the reference resumes by deterministic replay, resetSimulation +
+ * seekToSimTime (BloodDropEngine.ts:1682-1763, 2142-2179); these let a host
+ * keep the state itself and resume from it instead of replaying.
+ * Layout: an 80-byte header (magic "MAC3DCK1", ABI version, nx, ny, nz, levels,
+ * precision, element size, wave n (0: coupling off), parcel count, and the
+ * mixed mode's pressure-history count and refinement-iteration log), then
+ * U, V, W, DYE (element size each, n cells), the pressure (mixed: the FP64
+ * pressures of the current and the two previous steps, which the warm start
+ * extrapolates from; else P), the wave state (4 floats per texel) and the
+ * parcels (8 floats each) when present.  A step after a load gives bit for
+ * bit what the same step gives after the save.  Not in the checkpoint: the
+ * per-step inputs (dt, the drop slots, the coupling's parameters and scene
+ * clock), which the host sets each step as the reference packs them per
+ * substep, and the splat counters, which each parcel stage clears. */
+/* bytes a checkpoint of the context's current state takes */
+int mac3d_checkpoint_size(void *ctx, size_t *bytes);
+/* writes the checkpoint into host[0 .. size); EINVAL if bytes < size */
+int mac3d_checkpoint_save(void *ctx, void *host, size_t bytes);
+/* restores a checkpoint saved from a context of the same grid, levels and
+ * precision, with the same wave n (enable the coupling first) */
+int mac3d_checkpoint_load(void *ctx, const void *host, size_t bytes);
+
 /* Adjoint of nsteps steps from the blob initial condition (no reference
  * counterpart: the reference has no differentiation).  Runs the forward
  * pass keeping the state every checkpoint_every steps, computes

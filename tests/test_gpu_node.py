@@ -109,3 +109,28 @@ def test_node_parcel_stage_matches_c_abi():
     assert intact > 0
     assert js["intact"] == intact and js["lysed"] == lysed
     assert js["p"] == pytest.approx(float(np.sqrt(np.sum(out * out))), rel=1e-12)
+
+
+@pytest.mark.skipif(shutil.which("node") is None, reason="node not installed")
+@pytest.mark.parametrize("mode", ["fp32", "mixed"])
+def test_node_checkpoint_round_trip(mode, tmp_path):
+    """saveCheckpoint -> step -> loadCheckpoint -> step through the N-API
+    host repeats the steps bit for bit; the same bytes loaded into a C ABI
+    solver give the same fields after the same steps."""
+    from mac3d import MacSolver
+    from mac3d.lib import config_for
+    r = subprocess.run(["node", os.path.join(ROOT, "tests", "js", "checkpoint_tiny.js"),
+                        str(tmp_path), mode], capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stderr
+    out = json.loads(r.stdout.strip().splitlines()[-1])
+    assert out["identical"] is True
+    assert out["badLoad"] is not None and "status 1" in out["badLoad"]  # EINVAL: truncated
+    ck = np.fromfile(str(tmp_path / "ck.bin"), dtype=np.uint8)
+    js_fields = np.fromfile(str(tmp_path / "fields.bin"), dtype=np.float32)
+    w = scenarios.scaled(scenarios.workload("tiny", 1), precision=1 if mode == "mixed" else 0)
+    with MacSolver(config_for(w)) as s:
+        s.checkpoint_load(ck)
+        s.step(w.dt, 2)
+        mine = np.concatenate([s.download(f, dtype=np.float32).ravel()
+                               for f in ("U", "V", "W", "P", "DYE")])
+    assert np.array_equal(mine, js_fields)
