@@ -11,6 +11,7 @@
 #include <cstdio>
 #include <cstring>
 #include <map>
+#include <mutex>
 #include <new>
 #include <string>
 #include <type_traits>
@@ -1684,7 +1685,22 @@ int adjoint_impl(Ctx *ctx, const mac3d_blob *blobs, int nb, double dt, int nstep
   return MAC3D_OK;
 }
 
-thread_local std::string g_create_err;
+thread_local std::string g_create_err;  // last error of a call without a context
+
+// ---- scratch of mac3d_div_norms_noalloc: the per-block (max, sum of squares)
+// pairs of its divergence pass, 131072 pairs (2 MiB) per GPU, held until the
+// process ends.  The host refers to it only through the address
+// hipGetSymbolAddress returns for the current device (kept per device).
+constexpr size_t kNoAllocMaxParts = 131072;
+static __device__ __align__(256) double scratch[2 * kNoAllocMaxParts];
+constexpr int kNoAllocMaxDevices = 64;
+std::mutex g_scratch_mu;
+double *g_scratch_dev[kNoAllocMaxDevices] = {};
+
+int noalloc_fail(int code, const std::string &msg) {
+  g_create_err = msg;
+  return code;
+}
 
 // ---- checkpoints: the state a step evolves, in one caller-owned byte buffer
 // (a header, then the sections in a fixed order; layout in mac3d.h)
@@ -2341,6 +2357,52 @@ int mac3d_profile_read(void *h, int kclass, double *ms_total, int64_t *launches)
   ctx->prof.collect();
   *ms_total = ctx->prof.ms[kclass];
   *launches = ctx->prof.count[kclass];
+  return MAC3D_OK;
+}
+
+int mac3d_div_norms_noalloc(const void *u, const void *v, const void *w, int dtype, int nx,
+                            int ny, int nz, double *out, void *stream) {
+  if (!u || !v || !w || !out) return noalloc_fail(MAC3D_EINVAL, "null field or output pointer");
+  if (dtype != MAC3D_F32 && dtype != MAC3D_F64)
+    return noalloc_fail(MAC3D_EINVAL, "dtype must be MAC3D_F32 or MAC3D_F64");
+  if (nx < 1 || ny < 1 || nz < 1) return noalloc_fail(MAC3D_EINVAL, "need nx, ny, nz >= 1");
+  const Dims d{nx, ny, nz};
+  const dim3 g = cell_grid(d);
+  const size_t parts = (size_t)g.x * g.y * g.z;
+  if (g.y > 65535 || g.z > 65535 || parts > kNoAllocMaxParts)
+    return noalloc_fail(MAC3D_EINVAL,
+                        "grid too large for the static scratch: ceil(nx/64) ceil(ny/4) nz = " +
+                            std::to_string(parts) + " partial sums, at most " +
+                            std::to_string(kNoAllocMaxParts) + " (and ceil(ny/4), nz <= 65535)");
+  int dev = -1;
+  hipError_t e = hipGetDevice(&dev);
+  if (e != hipSuccess) return noalloc_fail(MAC3D_EHIP, std::string("hipGetDevice: ") + hipGetErrorString(e));
+  if (dev < 0 || dev >= kNoAllocMaxDevices) return noalloc_fail(MAC3D_EINVAL, "device ordinal out of range");
+  double *scr = nullptr;
+  {
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (!g_scratch_dev[dev]) {  // once per device, with that device current
+      void *p = nullptr;
+      e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+      if (e != hipSuccess || !p)
+        return noalloc_fail(MAC3D_EHIP, std::string("hipGetSymbolAddress(scratch): ") +
+                                            hipGetErrorString(e));
+      g_scratch_dev[dev] = (double *)p;
+    }
+    scr = g_scratch_dev[dev];
+  }
+  // every block writes its pair, so nothing in the scratch needs clearing
+  hipStream_t st = (hipStream_t)stream;
+  if (dtype == MAC3D_F32)
+    hipLaunchKernelGGL(k_div_partials<float>, g, kCellBlock, 0, st, d, (const float *)u,
+                       (const float *)v, (const float *)w, scr);
+  else
+    hipLaunchKernelGGL(k_div_partials<double>, g, kCellBlock, 0, st, d, (const double *)u,
+                       (const double *)v, (const double *)w, scr);
+  hipLaunchKernelGGL(k_fold_div_norms, dim3(1), dim3(256), 0, st, (const double *)scr, parts,
+                     (double)d.cells(), out);
+  e = hipGetLastError();
+  if (e != hipSuccess) return noalloc_fail(MAC3D_EHIP, std::string("kernel launch: ") + hipGetErrorString(e));
   return MAC3D_OK;
 }
 

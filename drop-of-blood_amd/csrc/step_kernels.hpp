@@ -615,6 +615,31 @@ k_fold_partials(const double *__restrict__ part, size_t n, double *__restrict__ 
   }
 }
 
+// k_fold_partials for the context-free divergence norms: out[0] = max,
+// out[1] = sqrt(sum / ncells), the RMS (same summation order)
+__global__ void __launch_bounds__(256)
+k_fold_div_norms(const double *__restrict__ part, size_t n, double ncells,
+                 double *__restrict__ out) {
+  double mx = 0.0, ss = 0.0;
+  for (size_t q = threadIdx.x; q < n; q += 256) {
+    double a = part[2 * q];
+    mx = a > mx ? a : mx;
+    ss += part[2 * q + 1];
+  }
+  mx = wave_max(mx);
+  ss = wave_sum(ss);
+  __shared__ double smx[4], sss[4];
+  int t = threadIdx.x;
+  if ((t & 63) == 0) { smx[t >> 6] = mx; sss[t >> 6] = ss; }
+  __syncthreads();
+  if (t == 0) {
+    double a = smx[0], b = sss[0];
+    for (int q = 1; q < 4; ++q) { a = smx[q] > a ? smx[q] : a; b += sss[q]; }
+    out[0] = a;
+    out[1] = sqrt(b / ncells);
+  }
+}
+
 // dtype conversion between a staging buffer and a field
 template <class A, class B>
 __global__ void __launch_bounds__(256) k_convert(const A *__restrict__ src, B *__restrict__ dst, size_t n) {

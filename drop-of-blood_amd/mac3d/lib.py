@@ -84,7 +84,7 @@ EXPORTS = ("mac3d_default_config", "mac3d_create", "mac3d_destroy", "mac3d_uploa
            "mac3d_last_error", "mac3d_version", "mac3d_wave_default_params",
            "mac3d_wave_enable", "mac3d_set_drops", "mac3d_set_uniforms", "mac3d_parcels_upload",
            "mac3d_parcels_download", "mac3d_splat_download", "mac3d_checkpoint_size",
-           "mac3d_checkpoint_save", "mac3d_checkpoint_load")
+           "mac3d_checkpoint_save", "mac3d_checkpoint_load", "mac3d_div_norms_noalloc")
 
 
 def load_library(path: Optional[str] = None):
@@ -121,6 +121,7 @@ def load_library(path: Optional[str] = None):
     L.mac3d_checkpoint_size.argtypes = [vp, C.POINTER(C.c_size_t)]
     L.mac3d_checkpoint_save.argtypes = [vp, vp, C.c_size_t]
     L.mac3d_checkpoint_load.argtypes = [vp, vp, C.c_size_t]
+    L.mac3d_div_norms_noalloc.argtypes = [vp, vp, vp, C.c_int, C.c_int, C.c_int, C.c_int, vp, vp]
     L.mac3d_last_error.argtypes = [vp]
     L.mac3d_last_error.restype = C.c_char_p
     L.mac3d_version.restype = C.c_char_p

@@ -318,6 +318,22 @@ int mac3d_mg_op(void *ctx, int op, int level, int count);
 /* max|D|, RMS(D) of the current velocity, FP64 reduction, blocking */
 int mac3d_div_norms(void *ctx, double *max_div, double *l2_div);
 
+/* No counterpart in the reference.  For callers that cannot allocate device
+ * memory -- code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc: mac3d_div_norms on caller-owned device arrays, with no
+ * context and no allocation.  u, v, w: device arrays of an nx*ny*nz face
+ * velocity in the layout of MAC3D_FIELD_U/V/W, dtype MAC3D_F32 or MAC3D_F64;
+ * out: device array of 2 doubles that receives max |D(u)| and RMS(D) (m/s),
+ * the same reduction as mac3d_div_norms; stream: a hipStream_t (NULL: the null
+ * stream).  Only enqueues work: the caller synchronises before reading out.
+ * The per-block partial sums live in a static device array of 131072
+ * (max, sum) pairs (2 MiB) on each GPU, kept until the process ends: grids
+ * with ceil(nx/64) ceil(ny/4) nz > 131072 are refused with EINVAL.  Calls that
+ * overlap on one GPU share that array and race; serialise them (one stream, or
+ * events).  Errors: mac3d_last_error(NULL). */
+int mac3d_div_norms_noalloc(const void *u, const void *v, const void *w, int dtype, int nx,
+                            int ny, int nz, double *out, void *stream);
+
 /* kernel timing (no reference counterpart; the reference reads pass-level
  * timestamp queries, BloodDropEngine.ts:411-439,2400-2456).  enable != 0
  * clears the totals and brackets every launch with HIP events on the
