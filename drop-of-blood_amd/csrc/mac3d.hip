@@ -133,6 +133,10 @@ struct Ctx {
   // pressure (nx * nz), the drop slots and the FFT twiddles on the device
   bool wave_on = false;
   bool wave_scene = false;  // also the advection's scene terms
+  // the reference's own physics (lattice Casson, tanh buoyancy, clamp-to-edge
+  // neighbours, the projection clamp, the dye predictor's branches)
+  bool ref_physics = false;
+  double casson_scale = 1.0, buoyancy = 0.48;
   WaveParams wp{};
   Buf wave, wz, wfilm, ptop;
   Buf curl;  // scene: curl of the post-advection centre velocity, 4 per cell
@@ -283,8 +287,15 @@ StepParams step_params(const Ctx *ctx, double dt) {
   P.tau_y = c.tau_y; P.mu_inf = c.mu_inf; P.g = c.g; P.gamma_reg = c.gamma_reg;
   P.shear_eps2 = c.shear_eps2; P.alpha_max = c.alpha_max; P.vel_damp = c.vel_damp;
   P.maccormack = c.maccormack;
+  P.casson_scale = ctx->casson_scale;
+  P.buoyancy = ctx->buoyancy;
+  P.surface_y = ctx->wp.surface_y;
+  P.depth = ctx->wp.depth;
   return P;
 }
+
+// the coupling is on with the reference's own physics
+bool ref_physics(const Ctx *ctx) { return ctx->wave_on && ctx->ref_physics; }
 
 void swap_vel(Ctx *ctx) {
   std::swap(ctx->u, ctx->u2);
@@ -418,7 +429,8 @@ int launch_casson(Ctx *ctx, double dt) {
       int chunks = std::max(1, 768 / (ncx * ncy));
       int cz = (d.nz + chunks - 1) / chunks;
       dim3 grid(ncx * ncy * ((d.nz + cz - 1) / cz));
-      hipLaunchKernelGGL(k_casson_zm<T>, grid, dim3(CS_TX, CS_TY), 0, ctx->stream, d, P,
+      auto kern = ref_physics(ctx) ? k_casson_zm<T, true> : k_casson_zm<T, false>;
+      hipLaunchKernelGGL(kern, grid, dim3(CS_TX, CS_TY), 0, ctx->stream, d, P,
                          (const T *)ctx->u.ptr, (const T *)ctx->v.ptr, (const T *)ctx->w.ptr,
                          (const T *)ctx->dye.ptr, (T *)ctx->u2.ptr, (T *)ctx->v2.ptr,
                          (T *)ctx->w2.ptr, cz, ncx, ncy);
@@ -502,7 +514,8 @@ int launch_project(Ctx *ctx) {
     using T = std::remove_pointer_t<decltype(tag)>;
     with_step_type(ctx->p.type, [&](auto ptag) {
       using Pt = std::remove_pointer_t<decltype(ptag)>;
-      hipLaunchKernelGGL((k_project<T, Pt>), cell_grid(d), kCellBlock, 0, ctx->stream, d,
+      auto kern = ref_physics(ctx) ? k_project<T, Pt, true> : k_project<T, Pt, false>;
+      hipLaunchKernelGGL(kern, cell_grid(d), kCellBlock, 0, ctx->stream, d,
                          (T *)ctx->u.ptr, (T *)ctx->v.ptr, (T *)ctx->w.ptr,
                          (const Pt *)ctx->p.ptr, lid_pressure(ctx));
     });
@@ -516,15 +529,19 @@ int launch_dye_predict(Ctx *ctx, double dt) {
   ProfScope ps(ctx->prof, ctx->stream, KC_DYE);
   with_step_type(ctx->t0, [&](auto tag) {
     using T = std::remove_pointer_t<decltype(tag)>;
-    auto kern = gather_buf<T>(d) ? k_dye_predict<T, true> : k_dye_predict<T, false>;
+    const bool buf = gather_buf<T>(d), ref = ref_physics(ctx);
+    auto kern = ref ? (buf ? k_dye_predict<T, true, true> : k_dye_predict<T, false, true>)
+                    : (buf ? k_dye_predict<T, true, false> : k_dye_predict<T, false, false>);
+    // (the reference's branches sample elsewhere than the resolve's trace:
+    // no limiter bounds from the predict there)
     hipLaunchKernelGGL(kern, gather_grid(d), kGatherBlock, 0, ctx->stream, d,
                        (T)(dt / ctx->cfg.dx),
                        (const typename cvec_of<T>::type *)ctx->cvel.ptr,
                        (const T *)ctx->dye.ptr, (T *)ctx->dpred.ptr,
-                       (typename vec2_of<T>::type *)ctx->dmm.ptr);
+                       ref ? nullptr : (typename vec2_of<T>::type *)ctx->dmm.ptr);
   });
   LAUNCHCHK();
-  ctx->dmm_valid = true;
+  ctx->dmm_valid = !ref_physics(ctx);
   return MAC3D_OK;
 }
 
@@ -538,7 +555,9 @@ int launch_dye_resolve(Ctx *ctx, double dt) {
       const auto *cv = (const typename cvec_of<T>::type *)ctx->cvel.ptr;
       const bool buf = gather_buf<T>(d);
       auto kmm = buf ? k_dye_resolve_mm<T, true> : k_dye_resolve_mm<T, false>;
-      auto kfull = buf ? k_dye_resolve<T, true> : k_dye_resolve<T, false>;
+      auto kfull = ref_physics(ctx)
+                       ? (buf ? k_dye_resolve<T, true, true> : k_dye_resolve<T, false, true>)
+                       : (buf ? k_dye_resolve<T, true, false> : k_dye_resolve<T, false, false>);
       if (ctx->dmm_valid)
         hipLaunchKernelGGL(kmm, gather_grid(d), kGatherBlock, 0, ctx->stream, d, dth, s, cv,
                            (const T *)ctx->dye.ptr, (const T *)ctx->dpred.ptr,
@@ -1894,6 +1913,9 @@ int mac3d_wave_default_params(mac3d_wave_params *p) {
   p->sim_time = 0.0;
   p->schlieren = 0.032;     // schlierenStrength (uniformLayout.ts:44)
   p->hemolysis = 0.088;     // hemolysisRate (uniformLayout.ts:43)
+  p->physics = 0;
+  p->casson_scale = 1.0;    // cassonYield (uniformLayout.ts:42)
+  p->buoyancy = 0.48;       // buoyancyStrength (uniformLayout.ts:33)
   return MAC3D_OK;
 }
 
@@ -1914,8 +1936,11 @@ int mac3d_wave_enable(void *h, const mac3d_wave_params *p) {
   if (!(ext > 0.0) || !std::isfinite(ext) || !std::isfinite(hx) || !std::isfinite(depth) ||
       !std::isfinite(p->surface_y) || !std::isfinite(p->marangoni) || p->half_xz < 0.0 ||
       p->depth < 0.0 || !std::isfinite(p->vort_conf) || !std::isfinite(p->sim_time) ||
-      !std::isfinite(p->schlieren) || !std::isfinite(p->hemolysis))
+      !std::isfinite(p->schlieren) || !std::isfinite(p->hemolysis) ||
+      !std::isfinite(p->casson_scale) || !std::isfinite(p->buoyancy))
     return fail(ctx, MAC3D_EINVAL, "wave lengths must be finite, half_extent > 0");
+  if (p->physics != 0 && p->physics != 1)
+    return fail(ctx, MAC3D_EINVAL, "physics must be 0 (the workload's) or 1 (the reference's)");
   int rc;
   const size_t nn = (size_t)n * n;
   if (!ctx->wave.ptr || ctx->wp.n != n) {  // (earlier buffers stay owned until destroy)
@@ -1965,6 +1990,9 @@ int mac3d_wave_enable(void *h, const mac3d_wave_params *p) {
   ctx->wp.marangoni = (float)p->marangoni;
   ctx->wp.dt = 0.f;
   ctx->wave_scene = p->scene != 0;
+  ctx->ref_physics = p->physics == 1;
+  ctx->casson_scale = p->casson_scale;
+  ctx->buoyancy = p->buoyancy;
   ctx->wave_on = true;
   return MAC3D_OK;
 }
@@ -2031,9 +2059,21 @@ int mac3d_set_uniforms(void *h, const float *u, size_t count) {
   HIPCHK(hipMemcpyAsync((void *)scene_clock(ctx), &clk, sizeof(clk), hipMemcpyHostToDevice,
                         ctx->stream));
   HIPCHK(hipStreamSynchronize(ctx->stream));
-  if (ctx->wp.surface_y != u[40] || ctx->wp.marangoni != u[41] || ctx->wp.half_xz != half_xz ||
-      ctx->wp.depth != depth)
-    drop_graphs(ctx);  // (the captured launches hold the parameters by value)
+  bool changed = ctx->wp.surface_y != u[40] || ctx->wp.marangoni != u[41] ||
+                 ctx->wp.half_xz != half_xz || ctx->wp.depth != depth;
+  if (ctx->ref_physics) {
+    // the reference physics' own parameters: buoyancyStrength [45],
+    // velocityDamping [46], concSharpness [47] (MacCormack 0.45 sharpness,
+    // advect.ts:197), cassonYield [58]
+    const double vd = u[46], mc = 0.45 * (double)u[47];
+    changed = changed || ctx->buoyancy != u[45] || ctx->casson_scale != u[58] ||
+              ctx->cfg.vel_damp != vd || ctx->cfg.maccormack != mc;
+    ctx->buoyancy = u[45];
+    ctx->casson_scale = u[58];
+    ctx->cfg.vel_damp = vd;
+    ctx->cfg.maccormack = mc;
+  }
+  if (changed) drop_graphs(ctx);  // (the captured launches hold the parameters by value)
   ctx->wp.surface_y = u[40];
   ctx->wp.marangoni = u[41];
   ctx->wp.half_xz = half_xz;
@@ -2411,6 +2451,6 @@ const char *mac3d_last_error(void *h) {
   return ((Ctx *)h)->err.c_str();
 }
 
-const char *mac3d_version(void) { return "mac3d 0.1.0 (gfx950, abi 1)"; }
+const char *mac3d_version(void) { return "mac3d 0.2.0 (gfx950, abi 2)"; }
 
 }  // extern "C"

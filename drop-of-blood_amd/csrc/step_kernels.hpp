@@ -14,7 +14,17 @@ struct StepParams {
   double dt, dx;
   double rho_w, mu_w, rho_b, tau_y, mu_inf, g;
   double gamma_reg, shear_eps2, alpha_max, vel_damp, maccormack;
+  // the coupling's reference physics: Casson scale (lensAndRheology.z),
+  // buoyancy strength (fluidParams.y), the world height of the lid (the
+  // still surface) and the tank depth
+  double casson_scale, buoyancy, surface_y, depth;
 };
+
+template <class T>
+__device__ inline T smoothstep_v(T e0, T e1, T x) {
+  T t = clampv((x - e0) / (e1 - e0), T(0), T(1));
+  return t * t * (T(3) - T(2) * t);
+}
 
 #define MAC3D_CELL_PROLOGUE                                  \
   int i = blockIdx.x * blockDim.x + threadIdx.x;             \
@@ -225,6 +235,49 @@ __device__ inline void casson_cell(const StepParams &P, T m0, T m1, T m2, T xp0,
   o1 += buoy * dyef;
 }
 
+// One cell of FLUID_CASSON_FORCES parts (1) and (3) with the reference's own
+// physics (the coupling's physics flag): the lattice Casson law
+//   gamma = 50 sqrt(|du|^2 + |dv|^2 + |dw|^2 + 1e-6)        (fluid.ts:451-454)
+//   alpha = clamp((sqrt(1e-4 (1 + 1.35 c)) + sqrt(0.0036 scale c^2 /
+//           max(gamma + 0.4, 0.4)))^2, 1e-4, 0.0044)      (fluid.ts:458-462)
+// with c = clamp(dye, 0, 1.5) and no dt (a per-step rate), then the
+// tanh-saturated buoyancy of the face concentration rho = 1.24 c_face, gated
+// by the face's height under the surface (fluid.ts:477-486); the neighbours
+// come clamped to the edge (fluid.ts:387-392).  j: the cell's row.
+template <class T>
+__device__ inline void casson_cell_ref(const StepParams &P, const Dims &d, int j, T m0, T m1, T m2,
+                                       T xp0, T xp1, T xp2, T xn0, T xn1, T xn2, T yp0, T yp1,
+                                       T yp2, T yn0, T yn1, T yn2, T zp0, T zp1, T zp2, T zn0,
+                                       T zn1, T zn2, T dc, T dup, T &o0, T &o1, T &o2) {
+  T ss = T(0);
+  {
+    T a = xp0 - xn0, b = yp0 - yn0, e = zp0 - zn0;
+    ss += a * a + b * b + e * e;
+    a = xp1 - xn1; b = yp1 - yn1; e = zp1 - zn1;
+    ss += a * a + b * b + e * e;
+    a = xp2 - xn2; b = yp2 - yn2; e = zp2 - zn2;
+    ss += a * a + b * b + e * e;
+  }
+  const T gam = T(50) * sqrt(ss + T(1e-6));
+  const T cr = clampv(dc, T(0), T(1.5));
+  const T sq_muinf = sqrt(T(0.00010) * (T(1) + T(1.35) * cr));
+  const T tau = T(0.0036) * T(P.casson_scale) * cr * cr;
+  const T gr = gam + T(0.40);
+  const T yterm = sqrt(tau / (gr > T(0.40) ? gr : T(0.40)));
+  const T alpha = clampv((sq_muinf + yterm) * (sq_muinf + yterm), T(0.00010), T(0.0044));
+  o0 = m0 + alpha * ((xp0 + xn0 + yp0 + yn0 + zp0 + zn0) - T(6) * m0);
+  o1 = m1 + alpha * ((xp1 + xn1 + yp1 + yn1 + zp1 + zn1) - T(6) * m1);
+  o2 = m2 + alpha * ((xp2 + xn2 + yp2 + yn2 + zp2 + zn2) - T(6) * m2);
+  const T rho = T(1.24) * (T(0.5) * (dc + dup));
+  if (rho > T(0.0008)) {
+    const T yf = T(P.surface_y - P.depth + (j + 1.0) / d.ny * P.depth);
+    const T below = T(P.surface_y) - yf;
+    const T gate = smoothstep_v<T>(T(0.002), T(0.024), below > T(0) ? below : T(0));
+    const T dtc = T(P.dt < 0.0 ? 0.0 : (P.dt > 0.025 ? 0.025 : P.dt));
+    o1 += -T(P.buoyancy) * T(0.345) * (T(1.24) * tanh(rho / T(1.24))) * gate * dtc;
+  }
+}
+
 // FLUID_CASSON_FORCES (fluid.ts:377-511) parts (1) and (3): Casson
 // shear-thinning explicit viscous update and Boussinesq buoyancy.
 //   gamma  = |+-1 neighbour differences| / (2h)          (fluid.ts:451-454)
@@ -255,7 +308,9 @@ __device__ inline void casson_cell(const StepParams &P, T m0, T m1, T m2, T xp0,
 // (FP32: at least 6 waves per SIMD, three blocks per CU, measured 65-69
 // against 84 us at 192x256x192; 8 waves forces too few registers and was
 // slower.  FP64 needs its registers.)
-template <class T>
+// REF: the reference's own physics (casson_cell_ref): every ghost is the
+// clamped edge value (sign +1), as loadMac clamps its indices.
+template <class T, bool REF>
 __global__ void __launch_bounds__(CS_T, sizeof(T) == 4 ? 6 : 1)
 k_casson_zm(Dims d, StepParams P, const T *__restrict__ u, const T *__restrict__ v,
             const T *__restrict__ w, const T *__restrict__ dye, T *__restrict__ uo,
@@ -279,8 +334,10 @@ k_casson_zm(Dims d, StepParams P, const T *__restrict__ u, const T *__restrict__
     sc[q] = vq % CS_EX;
     int gx = x0 - 1 + sc[q], gy = y0 - 1 + sr[q];
     T a = T(1), b = T(1), e = T(1);
-    if (gx < 0 || gx >= nx) { a = T(0); b = T(-1); e = T(-1); }
-    else if (gy < 0) { a = T(-1); b = T(0); e = T(-1); }
+    if (!REF) {
+      if (gx < 0 || gx >= nx) { a = T(0); b = T(-1); e = T(-1); }
+      else if (gy < 0) { a = T(-1); b = T(0); e = T(-1); }
+    }
     int mx = gx < 0 ? 0 : (gx >= nx ? nx - 1 : gx), my = gy < 0 ? 0 : (gy >= ny ? ny - 1 : gy);
     so[q] = my * nx + mx;
     gu[q] = a;
@@ -302,7 +359,7 @@ k_casson_zm(Dims d, StepParams P, const T *__restrict__ u, const T *__restrict__
   };
   auto stage = [&](int k) {
     // ghost planes below and above the grid: (-u, -v, 0) of the edge plane
-    const bool zg = k < 0 || k >= nz;
+    const bool zg = !REF && (k < 0 || k >= nz);
     const T zu = zg ? T(-1) : T(1), zw = zg ? T(0) : T(1);
     const int sl = k & 3;
 #pragma unroll
@@ -336,6 +393,15 @@ k_casson_zm(Dims d, StepParams P, const T *__restrict__ u, const T *__restrict__
     if (inside) {
       const int s0 = k & 3, sm = (k - 1) & 3, sp = (k + 1) & 3;
       T o0, o1, o2;
+      if (REF)
+        casson_cell_ref<T>(P, d, gy, su[s0][r][c], sv[s0][r][c], sw[s0][r][c],
+                           su[s0][r][c + 1], sv[s0][r][c + 1], sw[s0][r][c + 1],
+                           su[s0][r][c - 1], sv[s0][r][c - 1], sw[s0][r][c - 1],
+                           su[s0][r + 1][c], sv[s0][r + 1][c], sw[s0][r + 1][c],
+                           su[s0][r - 1][c], sv[s0][r - 1][c], sw[s0][r - 1][c],
+                           su[sp][r][c], sv[sp][r][c], sw[sp][r][c],
+                           su[sm][r][c], sv[sm][r][c], sw[sm][r][c], dc, dup, o0, o1, o2);
+      else
       casson_cell<T>(P, su[s0][r][c], sv[s0][r][c], sw[s0][r][c],
                      su[s0][r][c + 1], sv[s0][r][c + 1], sw[s0][r][c + 1],
                      su[s0][r][c - 1], sv[s0][r][c - 1], sw[s0][r][c - 1],
@@ -375,17 +441,26 @@ k_divergence(Dims d, const T *__restrict__ u, const T *__restrict__ v,
 // FLUID_MAC_PROJECT (fluid.ts:863-911): face -= p(+1) - p, solid wall faces
 // 0, the top face against the lid's ghost pressure p_top (ptop, nullable:
 // the flat lid's 0; fluid.ts:893-898).
-template <class T, class Pt>
+// CLAMP2: the reference physics' +-2 clamp of the projected faces (fluid.ts:908).
+template <class T, class Pt, bool CLAMP2>
 __global__ void __launch_bounds__(256)
 k_project(Dims d, T *__restrict__ u, T *__restrict__ v, T *__restrict__ w,
           const Pt *__restrict__ p, const float *__restrict__ ptop) {
   MAC3D_CELL_PROLOGUE
   typedef typename compute_of<Pt>::type C;
   C pc = (C)p[c];
-  if (i < d.nx - 1) u[c] = (T)((C)u[c] - ((C)p[c + sx] - pc)); else u[c] = T(0);
+  C uu = (i < d.nx - 1) ? (C)u[c] - ((C)p[c + sx] - pc) : C(0);
   C pu = (j < d.ny - 1) ? (C)p[c + sy] : (ptop ? (C)ptop[(size_t)k * d.nx + i] : C(0));
-  v[c] = (T)((C)v[c] - (pu - pc));
-  if (k < d.nz - 1) w[c] = (T)((C)w[c] - ((C)p[c + sz] - pc)); else w[c] = T(0);
+  C vv = (C)v[c] - (pu - pc);
+  C ww = (k < d.nz - 1) ? (C)w[c] - ((C)p[c + sz] - pc) : C(0);
+  if (CLAMP2) {
+    uu = clampv(uu, C(-2), C(2));
+    vv = clampv(vv, C(-2), C(2));
+    ww = clampv(ww, C(-2), C(2));
+  }
+  u[c] = (T)uu;
+  v[c] = (T)vv;
+  w[c] = (T)ww;
 }
 
 // CONC_PREDICT (advect.ts:28-72): RK2 semi-Lagrangian predictor of the dye
@@ -444,7 +519,17 @@ __device__ inline void dye_bounds(const Dims &d, T lo[3], T hi[3]) {
 // (nullable) takes the least and greatest of the 8 dye texels of that sample
 // per cell, which are the bounds CONC_RESOLVE's limiter clamps to, so that
 // k_dye_resolve_mm need not trace back again.
-template <class T, bool BUF>
+// the cells CONC_PREDICT and CONC_RESOLVE hold at zero: the lateral walls and
+// the top row (advect.ts:42-45, 100-103)
+__device__ inline bool conc_wall(const Dims &d, int i, int j, int k) {
+  return i == 0 || i >= d.nx - 1 || k == 0 || k >= d.nz - 1 || j >= d.ny - 1;
+}
+
+// REF: the reference's predictor branches (the coupling's physics flag): 0 at
+// conc_wall cells; the cell's own dye below speed^2 2.5e-5 (advect.ts:54-58);
+// a one-stage trace t - v0 dt/h below 0.012 (advect.ts:60-62); else RK2.
+// (mm is null with REF: the resolve traces again.)
+template <class T, bool BUF, bool REF>
 __global__ void __launch_bounds__(256)
 k_dye_predict(Dims d, T dth, const typename cvec_of<T>::type *__restrict__ cvel,
               const T *__restrict__ dye, T *__restrict__ pred,
@@ -455,6 +540,28 @@ k_dye_predict(Dims d, T dth, const typename cvec_of<T>::type *__restrict__ cvel,
   const auto rd = make_reader<BUF>(dye, n);
   T lo[3], hi[3], v0[3];
   dye_bounds(d, lo, hi);
+  if (REF) {
+    if (conc_wall(d, i, j, k)) {
+      pred[c] = T(0);
+      return;
+    }
+    const auto a0 = cv.ld(cv.cell(d, i, j, k));
+    const T sp2 = a0.x * a0.x + a0.y * a0.y + a0.z * a0.z;
+    if (sp2 < T(2.5e-5)) {
+      pred[c] = dye[c];
+      return;
+    }
+    if (sp2 < T(0.012)) {
+      TPos<T> b1;
+      b1.b[0] = i; b1.b[1] = j; b1.b[2] = k;
+      b1.o[0] = -a0.x * dth;
+      b1.o[1] = -a0.y * dth;
+      b1.o[2] = -a0.z * dth;
+      tpos_clamp(b1, lo, hi);
+      pred[c] = sample_tp_r<T>(rd, d, b1);
+      return;
+    }
+  }
   TPos<T> back;
   dye_back_tp<T>(d, cv, dth, i, j, k, lo, hi, v0, back);
   if (mm) {
@@ -469,11 +576,16 @@ k_dye_predict(Dims d, T dth, const typename cvec_of<T>::type *__restrict__ cvel,
 // CONC_RESOLVE's MacCormack core (advect.ts:196-222): forward estimate from
 // the predictor at uvw + v0 dt, corrected value pred + s (orig - estimate),
 // clamped to the min/max of the 8 source texels around the backtrace point.
-template <class T, bool BUF>
+// REF: conc_wall cells resolve to 0 (advect.ts:100-103).
+template <class T, bool BUF, bool REF>
 __global__ void __launch_bounds__(256)
 k_dye_resolve(Dims d, T dth, T strength, const typename cvec_of<T>::type *__restrict__ cvel,
               const T *__restrict__ dye, const T *__restrict__ pred, T *__restrict__ out) {
   MAC3D_CELL_PROLOGUE
+  if (REF && conc_wall(d, i, j, k)) {
+    out[c] = T(0);
+    return;
+  }
   const size_t n = d.cells();
   T lo[3], hi[3], v0[3];
   dye_bounds(d, lo, hi);
@@ -529,11 +641,6 @@ struct BlobDev {
   double cx, cy, cz, radius, w0;
 };
 
-template <class T>
-__device__ inline T smoothstep_v(T e0, T e1, T x) {
-  T t = clampv((x - e0) / (e1 - e0), T(0), T(1));
-  return t * t * (T(3) - T(2) * t);
-}
 
 template <class T>
 __global__ void __launch_bounds__(256)

@@ -13,6 +13,7 @@ export interface WaveParams {
   n?: number; halfExtent?: number; halfXZ?: number; depth?: number;
   surfaceY?: number; marangoni?: number; scene?: 0 | 1;
   vortConf?: number; simTime?: number; schlieren?: number; hemolysis?: number;
+  physics?: 0 | 1; cassonScale?: number; buoyancy?: number;
 }
 export interface Drop {
   x: number; y: number; z: number; radius: number; velY: number; osc?: number;

@@ -40,7 +40,9 @@ class WaveParams(C.Structure):
     _fields_ = [("n", C.c_int32), ("scene", C.c_int32)] + \
                [(n, C.c_double) for n in ("half_extent", "half_xz", "depth", "surface_y",
                                            "marangoni", "vort_conf", "sim_time", "schlieren",
-                                           "hemolysis")]
+                                           "hemolysis")] + \
+               [("physics", C.c_int32), ("reserved", C.c_int32), ("casson_scale", C.c_double),
+                ("buoyancy", C.c_double)]
 
 
 class Drop(C.Structure):

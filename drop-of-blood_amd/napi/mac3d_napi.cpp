@@ -214,9 +214,13 @@ napi_value WaveEnable(napi_env env, napi_callback_info info) {
     prop_num(env, argv[1], "simTime", &p.sim_time);
     prop_num(env, argv[1], "schlieren", &p.schlieren);
     prop_num(env, argv[1], "hemolysis", &p.hemolysis);
-    double scene = 0.0;
+    prop_num(env, argv[1], "cassonScale", &p.casson_scale);
+    prop_num(env, argv[1], "buoyancy", &p.buoyancy);
+    double scene = 0.0, physics = 0.0;
     prop_num(env, argv[1], "scene", &scene);
+    prop_num(env, argv[1], "physics", &physics);
     p.scene = scene != 0.0;
+    p.physics = (int32_t)physics;
     rc = mac3d_wave_enable(ctx, &p);
   }
   if (rc) return throw_status(env, ctx, rc);

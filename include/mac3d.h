@@ -30,7 +30,7 @@
 extern "C" {
 #endif
 
-#define MAC3D_ABI_VERSION 1
+#define MAC3D_ABI_VERSION 2
 
 /* status codes */
 #define MAC3D_OK 0
@@ -262,6 +262,22 @@ typedef struct mac3d_wave_params {
   double schlieren;   /* scene: schlierenStr, the unfurling phase offset
                          (particleParams.x, 0.032) */
   double hemolysis;   /* parcels' osmotic lysis rate (lensAndRheology.w, 0.088) */
+  int32_t physics;    /* 1: the reference's own lattice physics in place of the
+                         workload's SI laws: the lattice Casson law with its
+                         50 s^-1 strain scale and [1e-4, 0.0044] clamp, the
+                         tanh-saturated buoyancy gated under the surface, the
+                         viscous stencil's neighbours clamped to the edge
+                         (fluid.ts:387-392, 451-486), the projection's +-2
+                         clamp (fluid.ts:908) and the dye predictor's static /
+                         one-stage branches with the walls and top row held at
+                         0 (advect.ts:42-67, 100-103).  0 (default): not.
+                         With mac3d_set_uniforms the block's buoyancy [45],
+                         velocityDamping [46], concSharpness [47] and
+                         cassonYield [58] then set these and the config's
+                         vel_damp and maccormack (0.45 concSharpness) */
+  int32_t reserved;
+  double casson_scale;/* physics 1: Casson yield scale (lensAndRheology.z, 1.0) */
+  double buoyancy;    /* physics 1: buoyancy strength (fluidParams.y, 0.48) */
 } mac3d_wave_params;
 
 /* one drop slot (SimUniforms.drops[d], common.ts:26-43, packed by

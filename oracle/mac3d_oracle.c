@@ -108,6 +108,7 @@ double orc_sample_cell(const double *f, int nx, int ny, int nz, double ux,
 }
 
 static double top_stagnation_gate(const orc_state *s, int i, int j, int k);
+static double sstep(double e0, double e1, double x);
 
 /* ---- face -> centre: FLUID_MAC_TO_CENTER (fluid.ts:917-943); with the wave
  * coupling's scene flag, the top-row stagnation gate (fluid.ts:944-959,
@@ -280,6 +281,77 @@ static void casson_buoyancy(orc_state *s, double dt) {
         int jp = j + 1 < d.ny ? j + 1 : d.ny - 1;
         double dye_face = 0.5 * (s->dye[c] + s->dye[IDX(d, i, jp, k)]);
         out[1] += buoy * dye_face;
+        if (i >= d.nx - 1) out[0] = 0.0;
+        if (k >= d.nz - 1) out[2] = 0.0;
+        s->u2[c] = out[0]; s->v2[c] = out[1]; s->w2[c] = out[2];
+      }
+  double *t;
+  t = s->u; s->u = s->u2; s->u2 = t;
+  t = s->v; s->v = s->v2; s->v2 = t;
+  t = s->w; s->w = s->w2; s->w2 = t;
+}
+
+/* ---- FLUID_CASSON_FORCES (fluid.ts:377-511), parts (1) and (3), with the
+ * reference's own physics (the coupling's physics flag):
+ *   neighbours clamped to the edge (loadMac, fluid.ts:387-392);
+ *   gamma = 50 sqrt(|du|^2 + |dv|^2 + |dw|^2 + 1e-6), du = mac(i+1) - mac(i-1)
+ *     (all three components) etc. (fluid.ts:451-454);
+ *   c = clamp(intact concentration, 0, 1.5) (fluid.ts:447-448; the dye here);
+ *   alpha = clamp((sqrt(1e-4 (1 + 1.35 c)) + sqrt(0.0036 scale c^2 /
+ *     max(gamma + 0.4, 0.4)))^2, 1e-4, 0.0044) (fluid.ts:458-462);
+ *   mac += alpha (sum of 6 neighbours - 6 mac) (fluid.ts:464-465);
+ *   y-faces: rho' = 1.24 c_face (+ 0.26 lysed, 0 here) sampled at the face;
+ *     if rho' > 0.0008: v += -buoyancy 0.345 1.24 tanh(rho'/1.24)
+ *     smoothstep(0.002, 0.024, max(0, surface - y_face)) dt (fluid.ts:477-486);
+ *   wall faces zeroed (fluid.ts:506-507).
+ * The scene's terms (2), (4) and the clamp follow in casson_scene. */
+static double mac_clamped(const double *f, dims3 d, int i, int j, int k) {
+  return f[IDX(d, clampi(i, 0, d.nx - 1), clampi(j, 0, d.ny - 1), clampi(k, 0, d.nz - 1))];
+}
+static void casson_ref(orc_state *s, double dt) {
+  dims3 d = s->d[0];
+  const double scale = s->wp.casson_scale, buoy = s->wp.buoyancy;
+  const double ytop = s->wp.surface_y, y0 = ytop - s->wp.depth;
+  dt = clampd(dt, 0.0, 0.025);
+#pragma omp parallel for
+  for (int k = 0; k < d.nz; ++k)
+    for (int j = 0; j < d.ny; ++j)
+      for (int i = 0; i < d.nx; ++i) {
+        long c = IDX(d, i, j, k);
+        const double *F[3] = {s->u, s->v, s->w};
+        double m[3], xp[3], xn[3], yp[3], yn[3], zp[3], zn[3];
+        for (int a = 0; a < 3; ++a) {
+          m[a] = F[a][c];
+          xp[a] = mac_clamped(F[a], d, i + 1, j, k);
+          xn[a] = mac_clamped(F[a], d, i - 1, j, k);
+          yp[a] = mac_clamped(F[a], d, i, j + 1, k);
+          yn[a] = mac_clamped(F[a], d, i, j - 1, k);
+          zp[a] = mac_clamped(F[a], d, i, j, k + 1);
+          zn[a] = mac_clamped(F[a], d, i, j, k - 1);
+        }
+        double ss = 0.0;
+        for (int a = 0; a < 3; ++a) {
+          double du = xp[a] - xn[a], dv = yp[a] - yn[a], dw = zp[a] - zn[a];
+          ss += du * du + dv * dv + dw * dw;
+        }
+        double gam = 50.0 * sqrt(ss + 1e-6);
+        double cr = clampd(s->dye[c], 0.0, 1.5);
+        double sq_muinf = sqrt(0.00010 * (1.0 + 1.35 * cr));
+        double tau = 0.0036 * scale * cr * cr;
+        double yterm = sqrt(tau / fmax(gam + 0.40, 0.40));
+        double alpha = clampd((sq_muinf + yterm) * (sq_muinf + yterm), 0.00010, 0.0044);
+        double out[3];
+        for (int a = 0; a < 3; ++a) {
+          double lap = (xp[a] + xn[a] + yp[a] + yn[a] + zp[a] + zn[a]) - 6.0 * m[a];
+          out[a] = m[a] + alpha * lap;
+        }
+        int jp = j + 1 < d.ny ? j + 1 : d.ny - 1;
+        double rho = 1.24 * (0.5 * (s->dye[c] + s->dye[IDX(d, i, jp, k)]));
+        if (rho > 0.0008) {
+          double yf = y0 + (j + 1.0) / d.ny * s->wp.depth;
+          double gate = sstep(0.002, 0.024, fmax(0.0, ytop - yf));
+          out[1] += -buoy * 0.345 * (1.24 * tanh(rho / 1.24)) * gate * dt;
+        }
         if (i >= d.nx - 1) out[0] = 0.0;
         if (k >= d.nz - 1) out[2] = 0.0;
         s->u2[c] = out[0]; s->v2[c] = out[1]; s->w2[c] = out[2];
@@ -541,10 +613,12 @@ static double rel_residual(orc_state *s) {
 
 /* ---- FLUID_MAC_PROJECT (fluid.ts:863-911): face -= p(+1) - p, wall faces
  * 0, top face against the lid's ghost p_top (fluid.ts:893-898; the wave
- * coupling's, else 0 for the flat lid).  WORKLOAD: the reference's +-2 clamp
- * (lattice units) is not part of the workload. */
+ * coupling's, else 0 for the flat lid).  The reference's +-2 clamp (lattice
+ * units, fluid.ts:908) applies with the coupling's physics flag only: it is
+ * not part of the workload. */
 static void project(orc_state *s) {
   dims3 d = s->d[0];
+  const int clamp2 = s->wave_on && s->wp.physics;
 #pragma omp parallel for
   for (int k = 0; k < d.nz; ++k)
     for (int j = 0; j < d.ny; ++j)
@@ -555,6 +629,11 @@ static void project(orc_state *s) {
         double ptop = s->wave_on ? s->ptop[(long)k * d.nx + i] : 0.0;
         if (j < d.ny - 1) s->v[c] -= s->pr[IDX(d, i, j + 1, k)] - pc; else s->v[c] -= ptop - pc;
         if (k < d.nz - 1) s->w[c] -= s->pr[IDX(d, i, j, k + 1)] - pc; else s->w[c] = 0.0;
+        if (clamp2) {  /* the reference physics' +-2 clamp (fluid.ts:908) */
+          s->u[c] = clampd(s->u[c], -2.0, 2.0);
+          s->v[c] = clampd(s->v[c], -2.0, 2.0);
+          s->w[c] = clampd(s->w[c], -2.0, 2.0);
+        }
       }
 }
 
@@ -562,10 +641,15 @@ static void project(orc_state *s) {
  * centred velocity, uvw clamped to [0.001, 0.999].  WORKLOAD: the dye is a
  * cell field of the MAC grid; the static / single-stage speed shortcuts
  * (advect.ts:54-63) and the wall/top zeroing (advect.ts:42-45) are not part
- * of the workload. */
+ * of the workload: they apply with the coupling's physics flag only, as does
+ * the resolve's zeroing of the same cells (advect.ts:100-103). */
+static int conc_wall(dims3 d, int i, int j, int k) {
+  return i == 0 || i >= d.nx - 1 || k == 0 || k >= d.nz - 1 || j >= d.ny - 1;
+}
 static void dye_predict(orc_state *s, double dt) {
   dims3 d = s->d[0];
   double dom[3] = {d.nx * s->p.dx, d.ny * s->p.dx, d.nz * s->p.dx};
+  const int ref = s->wave_on && s->wp.physics;
 #pragma omp parallel for
   for (int k = 0; k < d.nz; ++k)
     for (int j = 0; j < d.ny; ++j)
@@ -575,6 +659,20 @@ static void dye_predict(orc_state *s, double dt) {
                         sample_cell(s->cv, d, uvw[0], uvw[1], uvw[2]),
                         sample_cell(s->cw, d, uvw[0], uvw[1], uvw[2])};
         double mid[3], back[3];
+        if (ref) {
+          /* the reference's branches: lateral walls and the top row zeroed
+           * (advect.ts:42-45); a static copy below speed^2 2.5e-5
+           * (advect.ts:54-58); a single-stage trace below 0.012 (:60-62) */
+          long c = IDX(d, i, j, k);
+          double sp2 = v0[0] * v0[0] + v0[1] * v0[1] + v0[2] * v0[2];
+          if (conc_wall(d, i, j, k)) { s->dpred[c] = 0.0; continue; }
+          if (sp2 < 2.5e-5) { s->dpred[c] = s->dye[c]; continue; }
+          if (sp2 < 0.012) {
+            for (int a = 0; a < 3; ++a) back[a] = clampd(uvw[a] - (v0[a] * dt) / dom[a], 0.001, 0.999);
+            s->dpred[c] = sample_cell(s->dye, d, back[0], back[1], back[2]);
+            continue;
+          }
+        }
         for (int a = 0; a < 3; ++a) mid[a] = clampd(uvw[a] - 0.5 * (v0[a] * dt) / dom[a], 0.001, 0.999);
         double vm[3] = {sample_cell(s->cu, d, mid[0], mid[1], mid[2]),
                         sample_cell(s->cv, d, mid[0], mid[1], mid[2]),
@@ -594,11 +692,13 @@ static void dye_resolve(orc_state *s, double dt) {
   double dom[3] = {d.nx * s->p.dx, d.ny * s->p.dx, d.nz * s->p.dx};
   double mc = s->p.maccormack;
   double *out = s->u2; /* scratch */
+  const int ref = s->wave_on && s->wp.physics;
 #pragma omp parallel for
   for (int k = 0; k < d.nz; ++k)
     for (int j = 0; j < d.ny; ++j)
       for (int i = 0; i < d.nx; ++i) {
         long c = IDX(d, i, j, k);
+        if (ref && conc_wall(d, i, j, k)) { out[c] = 0.0; continue; } /* advect.ts:100-103 */
         double uvw[3] = {(i + 0.5) / d.nx, (j + 0.5) / d.ny, (k + 0.5) / d.nz};
         double v0[3] = {sample_cell(s->cu, d, uvw[0], uvw[1], uvw[2]),
                         sample_cell(s->cv, d, uvw[0], uvw[1], uvw[2]),
@@ -1072,7 +1172,8 @@ static void casson_stage(orc_state *s, double dt) {
     face_to_centre(s);
     curl_pass(s);
   }
-  casson_buoyancy(s, dt);
+  if (s->wave_on && s->wp.physics) casson_ref(s, dt);
+  else casson_buoyancy(s, dt);
   if (scene) casson_scene(s, dt);
 }
 

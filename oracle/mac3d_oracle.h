@@ -46,6 +46,9 @@ typedef struct orc_wave_params {
   double half_extent, half_xz, depth, surface_y, marangoni;
   double vort_conf, sim_time, schlieren; /* Casson scene terms */
   double hemolysis;                      /* parcels' lysis rate */
+  int physics;  /* 1: the reference's own lattice physics (see casson_ref) */
+  int reserved;
+  double casson_scale, buoyancy;         /* lensAndRheology.z, fluidParams.y */
 } orc_wave_params;
 
 typedef struct orc_drop {

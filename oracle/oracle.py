@@ -37,7 +37,8 @@ class OrcBlob(C.Structure):
 class OrcWaveParams(C.Structure):
     _fields_ = [("n", C.c_int), ("scene", C.c_int)] + [(n, C.c_double) for n in (
         "half_extent", "half_xz", "depth", "surface_y", "marangoni", "vort_conf", "sim_time",
-        "schlieren", "hemolysis")]
+        "schlieren", "hemolysis")] + [("physics", C.c_int), ("reserved", C.c_int),
+                                      ("casson_scale", C.c_double), ("buoyancy", C.c_double)]
 
 
 class OrcDrop(C.Structure):
@@ -179,7 +180,7 @@ class Oracle:
 
     def wave_enable(self, n=512, half_extent=1.60, half_xz=0.0, depth=0.0, surface_y=0.30,
                     marangoni=0.0, scene=0, vort_conf=1.02, sim_time=0.0, schlieren=0.032,
-                    hemolysis=0.088, off=False):
+                    hemolysis=0.088, physics=0, casson_scale=1.0, buoyancy=0.48, off=False):
         """enable (flat surface, no drops) or, off=True, disable the wave coupling"""
         if off:
             lib().orc_wave_enable(self.h, None)
@@ -188,7 +189,8 @@ class Oracle:
                                          half_xz=half_xz, depth=depth, surface_y=surface_y,
                                          marangoni=marangoni, vort_conf=vort_conf,
                                          sim_time=sim_time, schlieren=schlieren,
-                                         hemolysis=hemolysis)
+                                         hemolysis=hemolysis, physics=int(physics),
+                                         casson_scale=casson_scale, buoyancy=buoyancy)
         lib().orc_wave_enable(self.h, C.byref(self.wave_params))
 
     def parcels_set(self, parcels):

@@ -16,19 +16,25 @@ def oracle_for(w, **kw):
                          max_cycles=30, **kw)
 
 
+# physical parameters shared by mac3d_config and the oracle's parameters
+CFG_KEYS = ("vel_damp", "maccormack", "rho_b", "tau_y", "mu_inf", "g", "alpha_max")
+
+
 def pair(name="tiny", seed=1, precision=None, warm_steps=2, **over):
     """HIP solver and oracle with identical state; warm_steps oracle steps
-    first so the velocity field is not the trivial initial one."""
+    first so the velocity field is not the trivial initial one.  Keywords
+    name workload fields, or (CFG_KEYS) physical parameters given to both."""
     w = scenarios.workload(name, seed)
     if precision is not None:
         w = scenarios.scaled(w, precision=precision)
+    cfg = {k: over.pop(k) for k in list(over) if k in CFG_KEYS}
     for k, v in over.items():
         setattr(w, k, v)
-    o = oracle_for(w)
+    o = oracle_for(w, **cfg)
     o.init_blobs(w.blobs)
     if warm_steps:
         o.step(w.dt, warm_steps)
-    s = MacSolver(config_for(w))
+    s = MacSolver(config_for(w, **cfg))
     s.init_blobs(w.blobs)
     copy_state(o, s)
     return w, s, o
