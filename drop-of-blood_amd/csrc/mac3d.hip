@@ -25,6 +25,7 @@
 #include "adjoint_kernels.hpp"
 #include "wave_kernels.hpp"
 #include "parcel_kernels.hpp"
+#include "conc_kernels.hpp"
 #include "profile.hpp"
 
 using namespace mac3d;
@@ -137,6 +138,13 @@ struct Ctx {
   // neighbours, the projection clamp, the dye predictor's branches)
   bool ref_physics = false;
   double casson_scale = 1.0, buoyancy = 0.48;
+  // the reference's 4-channel concentration (planes: intact, lysed,
+  // oxygenation, floor bed; FP32) and its twin, and the parcels' pending
+  // emission (device memory, read by the emission kernel of a parcel stage)
+  bool conc_on = false;
+  Buf conc, conc2;
+  void *conc_home = nullptr;
+  EmitCtl *emit = nullptr;
   WaveParams wp{};
   Buf wave, wz, wfilm, ptop;
   Buf curl;  // scene: curl of the post-advection centre velocity, 4 per cell
@@ -297,6 +305,23 @@ StepParams step_params(const Ctx *ctx, double dt) {
 // the coupling is on with the reference's own physics
 bool ref_physics(const Ctx *ctx) { return ctx->wave_on && ctx->ref_physics; }
 
+// the intact-RBC concentration the Casson pass, the buoyancy and the parcels
+// read (the dye, or the 4-channel concentration's .r), and the lysed one (.g;
+// null without it)
+const void *intact_conc(const Ctx *ctx) { return ctx->conc_on ? ctx->conc.ptr : ctx->dye.ptr; }
+const float *lysed_conc(const Ctx *ctx) {
+  return ctx->conc_on ? (const float *)ctx->conc.ptr + ctx->n[0] : nullptr;
+}
+ConcScene conc_scene(const Ctx *ctx) {
+  ConcScene S;
+  S.half_xz = ctx->wp.half_xz;
+  S.y_max = ctx->wp.surface_y;
+  S.y_min = ctx->wp.surface_y - ctx->wp.depth;
+  S.ext = ctx->wp.ext;
+  S.wave_n = ctx->wp.n;
+  return S;
+}
+
 void swap_vel(Ctx *ctx) {
   std::swap(ctx->u, ctx->u2);
   std::swap(ctx->v, ctx->v2);
@@ -323,6 +348,12 @@ WaveParams wave_params_dt(const Ctx *ctx, double dt) {
 // WV_MAX_DROPS DropPhase
 const size_t kDropBytes =
     WV_MAX_DROPS * sizeof(WaveDrop) + sizeof(SceneClock) + WV_MAX_DROPS * sizeof(DropPhase);
+const SceneClock *scene_clock(const Ctx *ctx) {
+  return (const SceneClock *)(ctx->drops + WV_MAX_DROPS);
+}
+const DropPhase *drop_phases(const Ctx *ctx) {
+  return (const DropPhase *)(scene_clock(ctx) + 1);
+}
 // One parcel step (BloodDropEngine.ts:2073-2093): both splat counters
 // cleared, then every parcel advanced and splatted.
 int launch_parcels(Ctx *ctx, double dt) {
@@ -334,6 +365,14 @@ int launch_parcels(Ctx *ctx, double dt) {
                      ctx->stream, ctx->splat, n2);  // (hipMalloc: 256-byte aligned)
   LAUNCHCHK();
   if (ctx->np == 0) return MAC3D_OK;
+  if (ctx->emit) {  // the pending emission, then mark it consumed
+    hipLaunchKernelGGL(k_parcel_emit, dim3(kMaxEmit / 256), dim3(256), 0, ctx->stream, d,
+                       conc_scene(ctx), (const EmitCtl *)ctx->emit, (const WaveDrop *)ctx->drops,
+                       scene_clock(ctx), (const float4 *)ctx->wave.ptr, ctx->np, ctx->parcels,
+                       ctx->splat + ctx->n[0]);
+    hipLaunchKernelGGL(k_emit_done, dim3(1), dim3(64), 0, ctx->stream, ctx->emit);
+    LAUNCHCHK();
+  }
   ParcelScene S;
   S.dt = (float)std::min(std::max(dt, 0.0), 0.025);
   S.half_xz = ctx->wp.half_xz;
@@ -343,17 +382,12 @@ int launch_parcels(Ctx *ctx, double dt) {
   S.k_lysis = ctx->clk_host.hemolysis;
   hipLaunchKernelGGL(k_parcel_advect_splat, dim3((unsigned)((ctx->np + 255) / 256)), dim3(256), 0,
                      ctx->stream, d, S, ctx->np, ctx->parcels, (const float3 *)ctx->cvel.ptr,
-                     (const float *)ctx->dye.ptr, ctx->splat, ctx->splat + ctx->n[0]);
+                     (const float *)intact_conc(ctx), lysed_conc(ctx), ctx->splat,
+                     ctx->splat + ctx->n[0]);
   LAUNCHCHK();
   return MAC3D_OK;
 }
 
-const SceneClock *scene_clock(const Ctx *ctx) {
-  return (const SceneClock *)(ctx->drops + WV_MAX_DROPS);
-}
-const DropPhase *drop_phases(const Ctx *ctx) {
-  return (const DropPhase *)(scene_clock(ctx) + 1);
-}
 
 int launch_advect(Ctx *ctx, double dt) {
   const Dims &d = ctx->d[0];
@@ -406,8 +440,8 @@ int launch_casson_scene(Ctx *ctx, double dt) {
       hipLaunchKernelGGL(k_casson_scene<T>, cell_grid(d), kCellBlock, 0, ctx->stream, d,
                          wave_params_dt(ctx, dt), scene_clock(ctx),
                          (const typename q4_of<T>::type *)ctx->curl.ptr,
-                         (const T *)ctx->dye.ptr, (T *)ctx->u.ptr, (T *)ctx->v.ptr,
-                         (T *)ctx->w.ptr);
+                         (const T *)intact_conc(ctx), (const T *)lysed_conc(ctx),
+                         (T *)ctx->u.ptr, (T *)ctx->v.ptr, (T *)ctx->w.ptr);
     });
   }
   LAUNCHCHK();
@@ -432,8 +466,8 @@ int launch_casson(Ctx *ctx, double dt) {
       auto kern = ref_physics(ctx) ? k_casson_zm<T, true> : k_casson_zm<T, false>;
       hipLaunchKernelGGL(kern, grid, dim3(CS_TX, CS_TY), 0, ctx->stream, d, P,
                          (const T *)ctx->u.ptr, (const T *)ctx->v.ptr, (const T *)ctx->w.ptr,
-                         (const T *)ctx->dye.ptr, (T *)ctx->u2.ptr, (T *)ctx->v2.ptr,
-                         (T *)ctx->w2.ptr, cz, ncx, ncy);
+                         (const T *)intact_conc(ctx), (const T *)lysed_conc(ctx),
+                         (T *)ctx->u2.ptr, (T *)ctx->v2.ptr, (T *)ctx->w2.ptr, cz, ncx, ncy);
     });
   }
   LAUNCHCHK();
@@ -574,6 +608,36 @@ int launch_dye_resolve(Ctx *ctx, double dt) {
 }
 
 // ---- multigrid
+// CONC_PREDICT on the lysed plane (the dye predictor with the reference's
+// branches) into dpred, then CONC_RESOLVE of the four planes into the twin
+int launch_conc_predict(Ctx *ctx, double dt) {
+  const Dims &d = ctx->d[0];
+  ProfScope ps(ctx->prof, ctx->stream, KC_DYE);
+  const bool buf = gather_buf<float>(d);
+  auto kern = buf ? k_dye_predict<float, true, true> : k_dye_predict<float, false, true>;
+  hipLaunchKernelGGL(kern, gather_grid(d), kGatherBlock, 0, ctx->stream, d,
+                     (float)(dt / ctx->cfg.dx), (const float3 *)ctx->cvel.ptr,
+                     (const float *)ctx->conc.ptr + ctx->n[0], (float *)ctx->dpred.ptr,
+                     (float2 *)nullptr);
+  LAUNCHCHK();
+  ctx->dmm_valid = false;
+  return MAC3D_OK;
+}
+int launch_conc_resolve(Ctx *ctx, double dt) {
+  const Dims &d = ctx->d[0];
+  {
+    ProfScope ps(ctx->prof, ctx->stream, KC_DYE);
+    hipLaunchKernelGGL(k_conc_resolve, cell_grid(d), kCellBlock, 0, ctx->stream, d,
+                       conc_scene(ctx), (float)dt, (float)ctx->cfg.maccormack,
+                       (const float3 *)ctx->cvel.ptr, (const float *)ctx->conc.ptr,
+                       (const float *)ctx->dpred.ptr, (const unsigned *)ctx->splat,
+                       (const unsigned *)ctx->splat + ctx->n[0], (float *)ctx->conc2.ptr);
+  }
+  LAUNCHCHK();
+  std::swap(ctx->conc, ctx->conc2);
+  return MAC3D_OK;
+}
+
 void swap_x(Ctx *ctx, int l) {
   std::swap(ctx->mgx[l], ctx->mgxa[l]);
   if (l == 0 && ctx->prec != MAC3D_PREC_MIXED && !ctx->adj_solving) ctx->p = ctx->mgx[0];
@@ -1026,6 +1090,7 @@ uint64_t parity_bits(const Ctx *ctx) {
   if (ctx->dye.ptr != ctx->dye_home) b |= 2;
   for (int l = 0; l <= ctx->L; ++l)
     if (ctx->mgx[l].ptr != ctx->mgx_home[l]) b |= (uint64_t)4 << l;
+  if (ctx->conc.ptr && ctx->conc.ptr != ctx->conc_home) b |= (uint64_t)1 << 12;
   for (int q = 0; q < 4; ++q) b |= (uint64_t)ctx->role[q] << (16 + 2 * q);
   return b;
 }
@@ -1035,6 +1100,8 @@ void set_parity_bits(Ctx *ctx, uint64_t b) {
   if (((ctx->dye.ptr != ctx->dye_home) ? 1u : 0u) != ((b >> 1) & 1)) std::swap(ctx->dye, ctx->dye2);
   for (int l = 0; l <= ctx->L; ++l)
     if (((ctx->mgx[l].ptr != ctx->mgx_home[l]) ? 1u : 0u) != ((b >> (2 + l)) & 1)) swap_x(ctx, l);
+  if (ctx->conc.ptr && ((ctx->conc.ptr != ctx->conc_home) ? 1u : 0u) != ((b >> 12) & 1))
+    std::swap(ctx->conc, ctx->conc2);
   for (int q = 0; q < 4; ++q) ctx->role[q] = (int)((b >> (16 + 2 * q)) & 3);
   sync_p(ctx);
 }
@@ -1100,6 +1167,11 @@ int seq_post(Ctx *ctx, double dt) {
   int rc;
   if ((rc = launch_project(ctx))) return rc;
   if ((rc = launch_face_to_centre(ctx))) return rc;
+  if (ctx->conc_on) {  // pass B (BloodDropEngine.ts:2073-2108)
+    if ((rc = launch_parcels(ctx, dt))) return rc;
+    if ((rc = launch_conc_predict(ctx, dt))) return rc;
+    return launch_conc_resolve(ctx, dt);
+  }
   if ((rc = launch_dye_predict(ctx, dt))) return rc;
   return launch_dye_resolve(ctx, dt);
 }
@@ -1209,6 +1281,7 @@ Buf *field_buf(Ctx *ctx, int field, int level) {
     case MAC3D_FIELD_WAVE: return l0 && ctx->wave_on ? &ctx->wave : nullptr;
     case MAC3D_FIELD_PTOP: return l0 && ctx->wave_on ? &ctx->ptop : nullptr;
     case MAC3D_FIELD_CURL: return l0 && ctx->curl.ptr ? &ctx->curl : nullptr;
+    case MAC3D_FIELD_CONC: return l0 && ctx->conc_on ? &ctx->conc : nullptr;
   }
   return nullptr;
 }
@@ -1728,9 +1801,10 @@ struct CkptHeader {
   int32_t abi, nx, ny, nz, levels, precision, elem;
   int32_t wave_n, np, hist, iter_n;
   int32_t iter_log[4];
+  int32_t conc, reserved;
   uint64_t bytes;
 };
-static_assert(sizeof(CkptHeader) == 80, "checkpoint header layout");
+static_assert(sizeof(CkptHeader) == 88, "checkpoint header layout");
 const char kCkptMagic[8] = {'M', 'A', 'C', '3', 'D', 'C', 'K', '1'};
 
 struct CkptSec {
@@ -1738,7 +1812,7 @@ struct CkptSec {
   size_t bytes;
 };
 // the device ranges of a checkpoint's sections, in order
-std::vector<CkptSec> ckpt_sections(Ctx *ctx, int wave_n, int np) {
+std::vector<CkptSec> ckpt_sections(Ctx *ctx, int wave_n, int np, bool conc) {
   const size_t n0 = ctx->n[0], es = type_size(ctx->t0);
   std::vector<CkptSec> v = {{ctx->u.ptr, es * n0}, {ctx->v.ptr, es * n0}, {ctx->w.ptr, es * n0},
                             {ctx->dye.ptr, es * n0}};
@@ -1748,11 +1822,12 @@ std::vector<CkptSec> ckpt_sections(Ctx *ctx, int wave_n, int np) {
     v.push_back({ctx->p.ptr, es * n0});
   if (wave_n > 0) v.push_back({ctx->wave.ptr, 4 * sizeof(float) * (size_t)wave_n * wave_n});
   if (np > 0) v.push_back({(void *)ctx->parcels, (size_t)np * sizeof(Parcel)});
+  if (conc) v.push_back({ctx->conc.ptr, 4 * sizeof(float) * n0});
   return v;
 }
-size_t ckpt_bytes(Ctx *ctx, int wave_n, int np) {
+size_t ckpt_bytes(Ctx *ctx, int wave_n, int np, bool conc) {
   size_t b = sizeof(CkptHeader);
-  for (const CkptSec &c : ckpt_sections(ctx, wave_n, np)) b += c.bytes;
+  for (const CkptSec &c : ckpt_sections(ctx, wave_n, np, conc)) b += c.bytes;
   return b;
 }
 int ckpt_wave_n(const Ctx *ctx) { return ctx->wave_on ? ctx->wp.n : 0; }
@@ -1832,6 +1907,19 @@ int mac3d_upload(void *h, int field, int level, const void *host, int dtype, siz
   }
   size_t hsz = dtype == MAC3D_F32 ? 4 : 8;
   HIPCHK(hipMemcpyAsync(ctx->stage, host, hsz * count, hipMemcpyHostToDevice, ctx->stream));
+  if (field == MAC3D_FIELD_CONC) {  // 4 values per cell -> the four planes
+    const size_t n0 = ctx->n[0];
+    dim3 g((unsigned)((n0 + 255) / 256));
+    if (dtype == MAC3D_F32)
+      hipLaunchKernelGGL((k_rgba_to_planes<float, float>), g, dim3(256), 0, ctx->stream,
+                         (const float *)ctx->stage, (float *)b->ptr, n0);
+    else
+      hipLaunchKernelGGL((k_rgba_to_planes<double, float>), g, dim3(256), 0, ctx->stream,
+                         (const double *)ctx->stage, (float *)b->ptr, n0);
+    LAUNCHCHK();
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    return MAC3D_OK;
+  }
   if (comp >= 0) {
     with_step_type(b->type, [&](auto tag) {
       using S = std::remove_pointer_t<decltype(tag)>;
@@ -1872,7 +1960,16 @@ int mac3d_download(void *h, int field, int level, void *host, int dtype, size_t 
   if (count != (comp >= 0 ? ctx->n[0] : b->n))
     return fail(ctx, MAC3D_EINVAL, "element count does not match the field");
   size_t hsz = dtype == MAC3D_F32 ? 4 : 8;
-  if (comp >= 0) {
+  if (field == MAC3D_FIELD_CONC) {  // the four planes -> 4 values per cell
+    const size_t n0 = ctx->n[0];
+    dim3 g((unsigned)((n0 + 255) / 256));
+    if (dtype == MAC3D_F32)
+      hipLaunchKernelGGL((k_planes_to_rgba<float, float>), g, dim3(256), 0, ctx->stream,
+                         (const float *)b->ptr, (float *)ctx->stage, n0);
+    else
+      hipLaunchKernelGGL((k_planes_to_rgba<float, double>), g, dim3(256), 0, ctx->stream,
+                         (const float *)b->ptr, (double *)ctx->stage, n0);
+  } else if (comp >= 0) {
     with_step_type(b->type, [&](auto tag) {
       using S = std::remove_pointer_t<decltype(tag)>;
       dim3 g((unsigned)((count + 255) / 256));
@@ -2074,6 +2171,14 @@ int mac3d_set_uniforms(void *h, const float *u, size_t count) {
     ctx->cfg.maccormack = mc;
   }
   if (changed) drop_graphs(ctx);  // (the captured launches hold the parameters by value)
+  // the parcels' emission of the next parcel stage: spawnCursor [61],
+  // spawnCount [62] (particleParams.y/.z, particles.ts:65-66,95)
+  if (ctx->np > 0) {
+    const int cursor = (int)std::max(0.f, u[61]), count = (int)std::max(0.f, u[62]);
+    if (count > kMaxEmit) return fail(ctx, MAC3D_EINVAL, "spawnCount above 8192");
+    int rc2 = mac3d_parcels_emit(h, cursor, count);
+    if (rc2) return rc2;
+  }
   ctx->wp.surface_y = u[40];
   ctx->wp.marangoni = u[41];
   ctx->wp.half_xz = half_xz;
@@ -2111,6 +2216,54 @@ int mac3d_parcels_upload(void *h, const float *parcels, int32_t n) {
                           ctx->stream));
   HIPCHK(hipStreamSynchronize(ctx->stream));
   ctx->np = n;
+  return MAC3D_OK;
+}
+
+int mac3d_conc_enable(void *h, int enable) {
+  if (!h) return MAC3D_EINVAL;
+  Ctx *ctx = (Ctx *)h;
+  drop_graphs(ctx);  // the step's launch sequence changes
+  if (!enable) {
+    ctx->conc_on = false;
+    return MAC3D_OK;
+  }
+  if (!ctx->wave_on) return fail(ctx, MAC3D_ESTATE, "the wave coupling is not enabled");
+  if (ctx->t0 != TF32)
+    return fail(ctx, MAC3D_EINVAL, "the 4-channel concentration needs the FP32 or mixed step");
+  int rc;
+  const size_t n0 = ctx->n[0];
+  if (!ctx->conc.ptr) {
+    if ((rc = alloc(ctx, ctx->conc, TF32, 4 * n0))) return rc;  // (zeroed)
+    if ((rc = alloc(ctx, ctx->conc2, TF32, 4 * n0))) return rc;
+    ctx->conc_home = ctx->conc.ptr;
+  }
+  if (!ctx->splat && (rc = dev_alloc(ctx, (void **)&ctx->splat, 2 * n0 * sizeof(unsigned))))
+    return rc;
+  if (4 * n0 > ctx->stage_n) {
+    if ((rc = dev_alloc(ctx, (void **)&ctx->stage, 4 * n0 * sizeof(double)))) return rc;
+    ctx->stage_n = 4 * n0;
+  }
+  HIPCHK(hipStreamSynchronize(ctx->stream));
+  ctx->conc_on = true;
+  return MAC3D_OK;
+}
+
+int mac3d_parcels_emit(void *h, int32_t cursor, int32_t count) {
+  if (!h) return MAC3D_EINVAL;
+  Ctx *ctx = (Ctx *)h;
+  if (!ctx->wave_on) return fail(ctx, MAC3D_ESTATE, "the wave coupling is not enabled");
+  if (cursor < 0 || count < 0 || count > kMaxEmit)
+    return fail(ctx, MAC3D_EINVAL, "need cursor >= 0 and 0 <= count <= 8192");
+  if (count > 0 && ctx->np == 0)
+    return fail(ctx, MAC3D_ESTATE, "upload the parcel ring (mac3d_parcels_upload) first");
+  int rc;
+  if (!ctx->emit) {
+    if ((rc = dev_alloc(ctx, (void **)&ctx->emit, sizeof(EmitCtl)))) return rc;
+    drop_graphs(ctx);  // (the parcel stage now runs the emission kernel)
+  }
+  const EmitCtl e{ctx->np > 0 ? cursor % ctx->np : 0, count};
+  HIPCHK(hipMemcpyAsync(ctx->emit, &e, sizeof(e), hipMemcpyHostToDevice, ctx->stream));
+  HIPCHK(hipStreamSynchronize(ctx->stream));
   return MAC3D_OK;
 }
 
@@ -2192,7 +2345,7 @@ int mac3d_checkpoint_size(void *h, size_t *bytes) {
   if (!h) return MAC3D_EINVAL;
   Ctx *ctx = (Ctx *)h;
   if (!bytes) return fail(ctx, MAC3D_EINVAL, "null size pointer");
-  *bytes = ckpt_bytes(ctx, ckpt_wave_n(ctx), ctx->np);
+  *bytes = ckpt_bytes(ctx, ckpt_wave_n(ctx), ctx->np, ctx->conc_on);
   return MAC3D_OK;
 }
 
@@ -2200,7 +2353,8 @@ int mac3d_checkpoint_save(void *h, void *host, size_t bytes) {
   if (!h) return MAC3D_EINVAL;
   Ctx *ctx = (Ctx *)h;
   const int wn = ckpt_wave_n(ctx), np = ctx->np;
-  const size_t need = ckpt_bytes(ctx, wn, np);
+  const bool conc = ctx->conc_on;
+  const size_t need = ckpt_bytes(ctx, wn, np, conc);
   if (!host || bytes < need)
     return fail(ctx, MAC3D_EINVAL, "checkpoint buffer is null or smaller than mac3d_checkpoint_size");
   CkptHeader hd;
@@ -2218,10 +2372,11 @@ int mac3d_checkpoint_save(void *h, void *host, size_t bytes) {
   hd.hist = ctx->hist;
   hd.iter_n = ctx->iter_n;
   for (int q = 0; q < 4; ++q) hd.iter_log[q] = ctx->iter_log[q];
+  hd.conc = conc ? 1 : 0;
   hd.bytes = need;
   std::memcpy(host, &hd, sizeof(hd));
   char *dst = (char *)host + sizeof(hd);
-  for (const CkptSec &c : ckpt_sections(ctx, wn, np)) {
+  for (const CkptSec &c : ckpt_sections(ctx, wn, np, conc)) {
     HIPCHK(hipMemcpyAsync(dst, c.dev, c.bytes, hipMemcpyDeviceToHost, ctx->stream));
     dst += c.bytes;
   }
@@ -2245,11 +2400,14 @@ int mac3d_checkpoint_load(void *h, const void *host, size_t bytes) {
                                    "(enable it with the same n first, or disable it)");
   if (hd.np < 0 || (hd.np > 0 && !ctx->wave_on))
     return fail(ctx, MAC3D_EINVAL, "checkpoint holds parcels: enable the wave coupling first");
-  if (hd.hist < 0 || hd.hist > 2 || hd.iter_n < 0 || hd.iter_n > 4)
+  if (hd.hist < 0 || hd.hist > 2 || hd.iter_n < 0 || hd.iter_n > 4 || hd.conc < 0 || hd.conc > 1)
     return fail(ctx, MAC3D_EINVAL, "checkpoint header is corrupt");
+  if ((hd.conc != 0) != ctx->conc_on)
+    return fail(ctx, MAC3D_EINVAL, "checkpoint's 4-channel concentration differs from the "
+                                   "context's (enable or disable it first)");
   for (int q = 0; q < 4; ++q)
     if (hd.iter_log[q] < 0) return fail(ctx, MAC3D_EINVAL, "checkpoint header is corrupt");
-  const size_t need = ckpt_bytes(ctx, hd.wave_n, hd.np);
+  const size_t need = ckpt_bytes(ctx, hd.wave_n, hd.np, hd.conc != 0);
   if (hd.bytes != need || bytes < need)
     return fail(ctx, MAC3D_EINVAL, "checkpoint size does not match its header");
   int rc;
@@ -2261,7 +2419,7 @@ int mac3d_checkpoint_load(void *h, const void *host, size_t bytes) {
       (rc = dev_alloc(ctx, (void **)&ctx->splat, 2 * ctx->n[0] * sizeof(unsigned))))
     return rc;
   const char *src = (const char *)host + sizeof(hd);
-  for (const CkptSec &c : ckpt_sections(ctx, hd.wave_n, hd.np)) {
+  for (const CkptSec &c : ckpt_sections(ctx, hd.wave_n, hd.np, hd.conc != 0)) {
     HIPCHK(hipMemcpyAsync(c.dev, src, c.bytes, hipMemcpyHostToDevice, ctx->stream));
     src += c.bytes;
   }
@@ -2324,6 +2482,12 @@ int mac3d_stage(void *h, int stage, double dt) {
     case MAC3D_STAGE_PROJECT: rc = launch_project(ctx); break;
     case MAC3D_STAGE_DYE_PREDICT: rc = launch_dye_predict(ctx, dt); break;
     case MAC3D_STAGE_DYE_RESOLVE: rc = launch_dye_resolve(ctx, dt); break;
+    case MAC3D_STAGE_CONC_PREDICT:
+    case MAC3D_STAGE_CONC_RESOLVE:
+      if (!ctx->conc_on) return fail(ctx, MAC3D_ESTATE, "the 4-channel concentration is off");
+      rc = stage == MAC3D_STAGE_CONC_PREDICT ? launch_conc_predict(ctx, dt)
+                                             : launch_conc_resolve(ctx, dt);
+      break;
     case MAC3D_STAGE_WAVE:
     case MAC3D_STAGE_TOP_PRESSURE:
       if (!ctx->wave_on) return fail(ctx, MAC3D_ESTATE, "the wave coupling is not enabled");

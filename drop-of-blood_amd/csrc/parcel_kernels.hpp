@@ -130,7 +130,8 @@ __device__ inline void pc_splat(const Dims &d, unsigned *__restrict__ acc, float
 __global__ void __launch_bounds__(256)
 k_parcel_advect_splat(Dims d, ParcelScene S, int np, Parcel *__restrict__ parcels,
                       const float3 *__restrict__ cvel, const float *__restrict__ dye,
-                      unsigned *__restrict__ intact, unsigned *__restrict__ lysed) {
+                      const float *__restrict__ lysed_conc, unsigned *__restrict__ intact,
+                      unsigned *__restrict__ lysed) {
   const int g = blockIdx.x * blockDim.x + threadIdx.x;
   if (g >= np) return;
   Parcel P = parcels[g];
@@ -144,8 +145,12 @@ k_parcel_advect_splat(Dims d, ParcelScene S, int np, Parcel *__restrict__ parcel
   float lysis = clampv(raw - ticks, 0.f, 0.98f);
   const float age = ticks * (1.f / 240.f);
   const float mature = pc_ss(8.5f, 18.f, age);
-  const float tr = __fmul_rn((float)g, 0.7548776662f);
-  const float tail_rank = tr - floorf(tr);
+  float tail_rank;
+  {  // each product and sum rounded on its own (no contraction into an FMA)
+#pragma clang fp contract(off)
+    const float tr = (float)g * 0.7548776662f;
+    tail_rank = tr - floorf(tr);
+  }
   const float hold = powf(pc_ss(0.36f, 0.995f, tail_rank), 1.65f) * (1.f - pc_ss(5.2f, 10.2f, age));
 
   const float3 v1 = pc_velocity(S, d, cvel, pos, mature, hold);
@@ -229,9 +234,16 @@ k_parcel_advect_splat(Dims d, ParcelScene S, int np, Parcel *__restrict__ parcel
     }
   }
   // hypotonic hemolysis (particles.ts:400-425)
-  const float cr = __fadd_rn(__fmul_rn((float)g, 0.6180339887f), 0.17f);
-  const float cell_res = cr - floorf(cr);
-  const float cn = sample_cell<float>(dye, d, uc.x, uc.y, uc.z);
+  float cell_res;
+  {
+#pragma clang fp contract(off)
+    const float cr = (float)g * 0.6180339887f + 0.17f;
+    cell_res = cr - floorf(cr);
+  }
+  // cSample.r + cSample.g (particles.ts:403-404): the dye (the intact
+  // concentration) plus the lysed one when the 4-channel concentration is on
+  float cn = sample_cell<float>(dye, d, uc.x, uc.y, uc.z);
+  if (lysed_conc) cn += sample_cell<float>(lysed_conc, d, uc.x, uc.y, uc.z);
   const float dilute = 1.f - pc_ss(0.012f, 0.22f, cn);
   const float disp = pc_ss(1.1f, 7.8f, age + dilute * 2.2f + lysis * 2.5f);
   const float late = pc_ss(7.2f, 16.f, age) * dilute;

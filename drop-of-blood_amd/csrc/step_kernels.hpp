@@ -241,14 +241,16 @@ __device__ inline void casson_cell(const StepParams &P, T m0, T m1, T m2, T xp0,
 //   alpha = clamp((sqrt(1e-4 (1 + 1.35 c)) + sqrt(0.0036 scale c^2 /
 //           max(gamma + 0.4, 0.4)))^2, 1e-4, 0.0044)      (fluid.ts:458-462)
 // with c = clamp(dye, 0, 1.5) and no dt (a per-step rate), then the
-// tanh-saturated buoyancy of the face concentration rho = 1.24 c_face, gated
+// tanh-saturated buoyancy of the face concentration rho = 1.24 c_face + 0.26
+// lysed_face (lc, lup: the lysed concentration, 0 without it), gated
 // by the face's height under the surface (fluid.ts:477-486); the neighbours
 // come clamped to the edge (fluid.ts:387-392).  j: the cell's row.
 template <class T>
 __device__ inline void casson_cell_ref(const StepParams &P, const Dims &d, int j, T m0, T m1, T m2,
                                        T xp0, T xp1, T xp2, T xn0, T xn1, T xn2, T yp0, T yp1,
                                        T yp2, T yn0, T yn1, T yn2, T zp0, T zp1, T zp2, T zn0,
-                                       T zn1, T zn2, T dc, T dup, T &o0, T &o1, T &o2) {
+                                       T zn1, T zn2, T dc, T dup, T lc, T lup, T &o0, T &o1,
+                                       T &o2) {
   T ss = T(0);
   {
     T a = xp0 - xn0, b = yp0 - yn0, e = zp0 - zn0;
@@ -268,7 +270,7 @@ __device__ inline void casson_cell_ref(const StepParams &P, const Dims &d, int j
   o0 = m0 + alpha * ((xp0 + xn0 + yp0 + yn0 + zp0 + zn0) - T(6) * m0);
   o1 = m1 + alpha * ((xp1 + xn1 + yp1 + yn1 + zp1 + zn1) - T(6) * m1);
   o2 = m2 + alpha * ((xp2 + xn2 + yp2 + yn2 + zp2 + zn2) - T(6) * m2);
-  const T rho = T(1.24) * (T(0.5) * (dc + dup));
+  const T rho = T(1.24) * (T(0.5) * (dc + dup)) + T(0.26) * (T(0.5) * (lc + lup));
   if (rho > T(0.0008)) {
     const T yf = T(P.surface_y - P.depth + (j + 1.0) / d.ny * P.depth);
     const T below = T(P.surface_y) - yf;
@@ -313,8 +315,9 @@ __device__ inline void casson_cell_ref(const StepParams &P, const Dims &d, int j
 template <class T, bool REF>
 __global__ void __launch_bounds__(CS_T, sizeof(T) == 4 ? 6 : 1)
 k_casson_zm(Dims d, StepParams P, const T *__restrict__ u, const T *__restrict__ v,
-            const T *__restrict__ w, const T *__restrict__ dye, T *__restrict__ uo,
-            T *__restrict__ vo, T *__restrict__ wo, int cz, int ncx, int ncy) {
+            const T *__restrict__ w, const T *__restrict__ dye, const T *__restrict__ lys,
+            T *__restrict__ uo, T *__restrict__ vo, T *__restrict__ wo, int cz, int ncx,
+            int ncy) {
   __shared__ T su[4][CS_EY][CS_EX], sv[4][CS_EY][CS_EX], sw[4][CS_EY][CS_EX];
   const int nx = d.nx, ny = d.ny, nz = d.nz;
   const int lb = xcd_remap(blockIdx.x, gridDim.x);  // one z-chunk of tiles per XCD
@@ -380,15 +383,25 @@ k_casson_zm(Dims d, StepParams P, const T *__restrict__ u, const T *__restrict__
   stage(k0);
   fetch(k0 + 1);
   T dn = dye[(size_t)k0 * plane + cc0], dnu = dye[(size_t)k0 * plane + cc0 + up];
+  // the lysed concentration (REF with the 4-channel concentration; else 0)
+  T ln = T(0), lnu = T(0);
+  if (REF && lys) {
+    ln = lys[(size_t)k0 * plane + cc0];
+    lnu = lys[(size_t)k0 * plane + cc0 + up];
+  }
   const int r = ty + 1, c = tx + 1;
   for (int k = k0; k < k1; ++k) {
     stage(k + 1);
-    T dc = dn, dup = dnu;
+    T dc = dn, dup = dnu, lc = ln, lup = lnu;
     __syncthreads();
     fetch(k + 2);
     if (k + 1 < k1) {
       dn = dye[(size_t)(k + 1) * plane + cc0];
       dnu = dye[(size_t)(k + 1) * plane + cc0 + up];
+      if (REF && lys) {
+        ln = lys[(size_t)(k + 1) * plane + cc0];
+        lnu = lys[(size_t)(k + 1) * plane + cc0 + up];
+      }
     }
     if (inside) {
       const int s0 = k & 3, sm = (k - 1) & 3, sp = (k + 1) & 3;
@@ -400,7 +413,8 @@ k_casson_zm(Dims d, StepParams P, const T *__restrict__ u, const T *__restrict__
                            su[s0][r + 1][c], sv[s0][r + 1][c], sw[s0][r + 1][c],
                            su[s0][r - 1][c], sv[s0][r - 1][c], sw[s0][r - 1][c],
                            su[sp][r][c], sv[sp][r][c], sw[sp][r][c],
-                           su[sm][r][c], sv[sm][r][c], sw[sm][r][c], dc, dup, o0, o1, o2);
+                           su[sm][r][c], sv[sm][r][c], sw[sm][r][c], dc, dup, lc, lup, o0, o1,
+                           o2);
       else
       casson_cell<T>(P, su[s0][r][c], sv[s0][r][c], sw[s0][r][c],
                      su[s0][r][c + 1], sv[s0][r][c + 1], sw[s0][r][c + 1],

@@ -667,6 +667,7 @@ template <class T>
 __global__ void __launch_bounds__(256)
 k_casson_scene(Dims d, WaveParams P, const SceneClock *__restrict__ clk,
                const typename q4_of<T>::type *__restrict__ curl, const T *__restrict__ dye,
+               const T *__restrict__ lys,
                T *__restrict__ u, T *__restrict__ v, T *__restrict__ w) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   const int j = blockIdx.y * blockDim.y + threadIdx.y;
@@ -691,7 +692,9 @@ k_casson_scene(Dims d, WaveParams P, const SceneClock *__restrict__ clk,
     T t = clampv((x - e0) / (e1 - e0), T(0), T(1));
     return t * t * (T(3) - T(2) * t);
   };
-  const T crbc = clampv(dye[c], T(0), T(1.5));
+  // cRbc + 0.45 concCell.g (fluid.ts:491): the lysed term with the
+  // 4-channel concentration only
+  const T crbc = clampv(dye[c], T(0), T(1.5)) + (lys ? T(0.45) * lys[c] : T(0));
   const T gate = ss(T(0.11), T(0.36), below) * ss(T(0.0008), T(0.015), crbc);
   if (gate > T(0.001)) {
     const T ts = (T)clk->sim_time * T(0.36) + (T)clk->schlieren * T(1.7);

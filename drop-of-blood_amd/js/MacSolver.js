@@ -19,7 +19,7 @@ const path = require('path');
 const addon = require(path.join(__dirname, '..', 'lib', 'mac3d.node'));
 
 const FIELD = Object.freeze({ U: 0, V: 1, W: 2, P: 3, DYE: 4, CU: 5, CV: 6, CW: 7, RHS: 8,
-  MGX: 9, MGB: 10, MGR: 11, DYE_PRED: 12, WAVE: 13, PTOP: 14, CURL: 15 });
+  MGX: 9, MGB: 10, MGR: 11, DYE_PRED: 12, WAVE: 13, PTOP: 14, CURL: 15, CONC: 16 });
 const PRECISION = Object.freeze({ fp32: 0, mixed: 1, fp64: 2 });
 
 /**
@@ -124,6 +124,25 @@ class MacSolver {
   step(dt, n = 1) {
     if (!Number.isFinite(dt) || dt <= 0) throw new RangeError('dt must be finite and > 0');
     return addon.step(this._live(), dt, n);
+  }
+
+  /**
+   * The reference's 4-channel concentration (field CONC, 4 per cell): each
+   * step then runs pass B (parcel emission, advection and splat,
+   * CONC_PREDICT, CONC_RESOLVE) after the projection.
+   * @param {boolean} on
+   */
+  concEnable(on = true) {
+    addon.concEnable(this._live(), on);
+  }
+
+  /**
+   * The next parcel stage first emits `count` parcels from ring slot
+   * `cursor` (the packer's spawnCursor / spawnCount).
+   * @param {number} cursor @param {number} count
+   */
+  parcelsEmit(cursor, count) {
+    addon.parcelsEmit(this._live(), cursor, count);
   }
 
   /** Copy the device fields into the reusable buffers. */

@@ -23,6 +23,10 @@ NODE_INCLUDE = "/usr/include/node"
 HIP_FLAGS = [
     "--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC", "-shared",
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+    # contract products into FMAs as hipcc does by default, except in blocks
+    # marked "#pragma clang fp contract(off)" (the hash ranks and R3 sequence,
+    # which must round every product as the reference's f32 does)
+    "-ffp-contract=fast-honor-pragmas",
     "-Wall", "-Wno-unused-result",
 ]
 

@@ -13,9 +13,10 @@ LIB_PATH = os.path.join(PKG, "lib", "libmac3d.so")
 F32, F64 = 0, 1
 PREC = {"fp32": 0, "mixed": 1, "fp64": 2}
 FIELD = dict(U=0, V=1, W=2, P=3, DYE=4, CU=5, CV=6, CW=7, RHS=8, MGX=9, MGB=10, MGR=11,
-             DYE_PRED=12, WAVE=13, PTOP=14, CURL=15)
+             DYE_PRED=12, WAVE=13, PTOP=14, CURL=15, CONC=16)
 STAGE = dict(FACE_TO_CENTRE=0, ADVECT=1, CASSON=2, DIVERGENCE=3, VCYCLE=4, PROJECT=5,
-             DYE_PREDICT=6, DYE_RESOLVE=7, WAVE=8, TOP_PRESSURE=9, PARCELS=10)
+             DYE_PREDICT=6, DYE_RESOLVE=7, WAVE=8, TOP_PRESSURE=9, PARCELS=10, CONC_PREDICT=11,
+             CONC_RESOLVE=12)
 MG = dict(SMOOTH=0, RESIDUAL=1, RESTRICT=2, PROLONG=3, COARSE=4, ZERO=5, SMOOTH_ZERO=6)
 KC = dict(FACE_TO_CENTRE=0, ADVECT=1, CASSON=2, DIVERGENCE=3, SMOOTH_L0=4, RESTRICT_L0=5,
           PROLONG_L0=6, MG_COARSE_LEVELS=7, COARSE_SOLVE=8, PROJECT=9, DYE=10, REFINE64=11,
@@ -86,7 +87,8 @@ EXPORTS = ("mac3d_default_config", "mac3d_create", "mac3d_destroy", "mac3d_uploa
            "mac3d_last_error", "mac3d_version", "mac3d_wave_default_params",
            "mac3d_wave_enable", "mac3d_set_drops", "mac3d_set_uniforms", "mac3d_parcels_upload",
            "mac3d_parcels_download", "mac3d_splat_download", "mac3d_checkpoint_size",
-           "mac3d_checkpoint_save", "mac3d_checkpoint_load", "mac3d_div_norms_noalloc")
+           "mac3d_checkpoint_save", "mac3d_checkpoint_load", "mac3d_div_norms_noalloc",
+           "mac3d_conc_enable", "mac3d_parcels_emit")
 
 
 def load_library(path: Optional[str] = None):
@@ -123,6 +125,8 @@ def load_library(path: Optional[str] = None):
     L.mac3d_checkpoint_size.argtypes = [vp, C.POINTER(C.c_size_t)]
     L.mac3d_checkpoint_save.argtypes = [vp, vp, C.c_size_t]
     L.mac3d_checkpoint_load.argtypes = [vp, vp, C.c_size_t]
+    L.mac3d_conc_enable.argtypes = [vp, C.c_int]
+    L.mac3d_parcels_emit.argtypes = [vp, C.c_int32, C.c_int32]
     L.mac3d_div_norms_noalloc.argtypes = [vp, vp, vp, C.c_int, C.c_int, C.c_int, C.c_int, vp, vp]
     L.mac3d_last_error.argtypes = [vp]
     L.mac3d_last_error.restype = C.c_char_p
@@ -198,7 +202,7 @@ class MacSolver:
             return (n, n, 4)
         if fid == FIELD["PTOP"]:
             return (self.cfg.nz, self.cfg.nx)
-        if fid == FIELD["CURL"]:
+        if fid in (FIELD["CURL"], FIELD["CONC"]):
             return self.level_shape(0) + (4,)
         return self.level_shape(level)
 
@@ -276,6 +280,16 @@ class MacSolver:
         p = np.ascontiguousarray(parcels, dtype=np.float32).reshape(-1, 8)
         self._check(self.L.mac3d_parcels_upload(self.h, p.ctypes.data, p.shape[0]))
         self.n_parcels = p.shape[0]
+
+    def parcels_emit(self, cursor: int, count: int):
+        """the next parcel stage first emits `count` parcels into the ring from
+        slot `cursor` (PARTICLE_EMIT)"""
+        self._check(self.L.mac3d_parcels_emit(self.h, int(cursor), int(count)))
+
+    def conc_enable(self, enable: bool = True):
+        """the reference's 4-channel concentration (field CONC; pass B in the
+        step: parcels, CONC_PREDICT, CONC_RESOLVE)"""
+        self._check(self.L.mac3d_conc_enable(self.h, int(enable)))
 
     def parcels_download(self, n=None) -> np.ndarray:
         n = self.n_parcels if n is None else n

@@ -445,6 +445,33 @@ napi_value Adjoint(napi_env env, napi_callback_info info) {
   return o;
 }
 
+// concEnable(h, on)
+napi_value ConcEnable(napi_env env, napi_callback_info info) {
+  napi_value argv[2];
+  if (!get_args(env, info, 2, argv)) return nullptr;
+  void *ctx = get_handle(env, argv[0]);
+  if (!ctx) return nullptr;
+  bool on = false;
+  napi_valuetype t;
+  NAPI_CALL(env, napi_typeof(env, argv[1], &t));
+  if (t == napi_boolean) napi_get_value_bool(env, argv[1], &on);
+  else on = get_int(env, argv[1]) != 0;
+  int rc = mac3d_conc_enable(ctx, on ? 1 : 0);
+  if (rc) return throw_status(env, ctx, rc);
+  return nullptr;
+}
+
+// parcelsEmit(h, cursor, count)
+napi_value ParcelsEmit(napi_env env, napi_callback_info info) {
+  napi_value argv[3];
+  if (!get_args(env, info, 3, argv)) return nullptr;
+  void *ctx = get_handle(env, argv[0]);
+  if (!ctx) return nullptr;
+  int rc = mac3d_parcels_emit(ctx, get_int(env, argv[1]), get_int(env, argv[2]));
+  if (rc) return throw_status(env, ctx, rc);
+  return nullptr;
+}
+
 // checkpointSave(h) -> ArrayBuffer holding mac3d_checkpoint_save's bytes
 napi_value CheckpointSave(napi_env env, napi_callback_info info) {
   napi_value argv[1];
@@ -519,6 +546,8 @@ napi_value Init(napi_env env, napi_value exports) {
       {"divNorms", nullptr, DivNorms, nullptr, nullptr, nullptr, napi_default, nullptr},
       {"adjoint", nullptr, Adjoint, nullptr, nullptr, nullptr, napi_default, nullptr},
       {"checkpointSave", nullptr, CheckpointSave, nullptr, nullptr, nullptr, napi_default, nullptr},
+      {"concEnable", nullptr, ConcEnable, nullptr, nullptr, nullptr, napi_default, nullptr},
+      {"parcelsEmit", nullptr, ParcelsEmit, nullptr, nullptr, nullptr, napi_default, nullptr},
       {"checkpointLoad", nullptr, CheckpointLoad, nullptr, nullptr, nullptr, napi_default, nullptr},
       {"version", nullptr, Version, nullptr, nullptr, nullptr, napi_default, nullptr},
   };

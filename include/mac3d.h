@@ -70,7 +70,12 @@ extern "C" {
 #define MAC3D_FIELD_CURL 15 /* scene only: curl of the post-advection centre velocity,
                                4 per cell (omega x, y, z, |omega|; grid units, 1/s
                                times h), FLUID_CURL (fluid.ts:331-369) */
-#define MAC3D_FIELD_COUNT 16
+#define MAC3D_FIELD_CONC 16 /* the reference's 4-channel concentration (needs
+                               mac3d_conc_enable): 4 per cell at 4*cell: intact RBC
+                               (.r), lysed Hb (.g), oxygenation/shear state (.b), floor
+                               bed (.a), the layout of its rgba texture
+                               (advect.ts:75-79) */
+#define MAC3D_FIELD_COUNT 17
 
 /* single stages of the step, for kernel-level parity tests */
 #define MAC3D_STAGE_FACE_TO_CENTRE 0 /* CU,CV,CW <- U,V,W */
@@ -85,7 +90,10 @@ extern "C" {
 #define MAC3D_STAGE_TOP_PRESSURE 9   /* PTOP <- top Dirichlet pressure from WAVE and drops */
 #define MAC3D_STAGE_PARCELS 10       /* parcels advanced through CU,CV,CW; splat counters <- their
                                         mass (cleared first); needs the coupling and parcels */
-#define MAC3D_STAGE_COUNT 11
+#define MAC3D_STAGE_CONC_PREDICT 11  /* DYE_PRED <- CONC_PREDICT of CONC's lysed channel */
+#define MAC3D_STAGE_CONC_RESOLVE 12  /* CONC <- CONC_RESOLVE from CONC, DYE_PRED, the splat
+                                        counters and CU,CV,CW */
+#define MAC3D_STAGE_COUNT 13
 
 /* single multigrid operators on one level, for parity tests */
 #define MAC3D_MG_SMOOTH 0   /* count RBGS sweeps on (MGX, MGB) */
@@ -202,13 +210,15 @@ int mac3d_step(void *ctx, double dt, int nsteps, mac3d_stats *stats);
  * replaces: the reference resumes by deterministic replay, resetSimulation +
  * seekToSimTime (BloodDropEngine.ts:1682-1763, 2142-2179); these let a host
  * keep the state itself and resume from it instead of replaying.
- * Layout: an 80-byte header (magic "MAC3DCK1", ABI version, nx, ny, nz, levels,
- * precision, element size, wave n (0: coupling off), parcel count, and the
- * mixed mode's pressure-history count and refinement-iteration log), then
+ * Layout: an 88-byte header (magic "MAC3DCK1", ABI version, nx, ny, nz, levels,
+ * precision, element size, wave n (0: coupling off), parcel count, the
+ * mixed mode's pressure-history count and refinement-iteration log, and
+ * whether the 4-channel concentration is on), then
  * U, V, W, DYE (element size each, n cells), the pressure (mixed: the FP64
  * pressures of the current and the two previous steps, which the warm start
- * extrapolates from; else P), the wave state (4 floats per texel) and the
- * parcels (8 floats each) when present.  A step after a load gives bit for
+ * extrapolates from; else P), the wave state (4 floats per texel), the
+ * parcels (8 floats each) and the 4-channel concentration (4 planes of n
+ * floats) when present.  A step after a load gives bit for
  * bit what the same step gives after the save.  Not in the checkpoint: the
  * per-step inputs (dt, the drop slots, the coupling's parameters and scene
  * clock), which the host sets each step as the reference packs them per
@@ -322,6 +332,26 @@ int mac3d_set_uniforms(void *ctx, const float *block, size_t count);
  * the FP32 and mixed modes (EINVAL in FP64: the parcels sample the FP32
  * centre velocity) */
 int mac3d_parcels_upload(void *ctx, const float *parcels, int32_t n);
+/* replaces: the spawnCursor / spawnCount of the packed block
+ * (particleParams.y/.z, BloodDropEngine.ts:1821-1844) consumed by
+ * PARTICLE_EMIT (particles.ts:53-186, dispatched at BloodDropEngine.ts:2082-2087):
+ * the next parcel stage (MAC3D_STAGE_PARCELS or the step's, with the 4-channel
+ * concentration on) first emits `count` parcels (<= 8192) into the uploaded
+ * parcel ring from slot cursor mod n, from the active drop slots in
+ * proportion to their drain rates, moving a live parcel's mass at a written
+ * slot into the lysed counter; the request is then consumed.  ESTATE without
+ * the coupling or (count > 0) without uploaded parcels. */
+int mac3d_parcels_emit(void *ctx, int32_t cursor, int32_t count);
+/* no reference counterpart as a call (the reference always runs its pass B):
+ * turns on or off the reference's 4-channel concentration at the MAC grid
+ * (MAC3D_FIELD_CONC, zero at first).  While on, each step runs the reference's
+ * pass B after the projection (BloodDropEngine.ts:2073-2108): the parcel stage
+ * (counters cleared, pending emission, advection and splat), CONC_PREDICT of
+ * the lysed channel and CONC_RESOLVE (advect.ts:28-298) in place of the dye
+ * transport, which then stands still; the Casson pass, the buoyancy and the
+ * parcels' hemolysis read the intact (.r) and lysed (.g) channels in place of
+ * the dye.  Needs the coupling and the FP32 or mixed step (ESTATE, EINVAL). */
+int mac3d_conc_enable(void *ctx, int enable);
 /* the parcels after MAC3D_STAGE_PARCELS (n <= the uploaded count) */
 int mac3d_parcels_download(void *ctx, float *parcels, int32_t n);
 /* the splat counters: which 0 intact, 1 lysed; count = cells */

@@ -27,9 +27,10 @@
 #define MAXLEV 8
 
 enum { F_U, F_V, F_W, F_P, F_DYE, F_CU, F_CV, F_CW, F_RHS, F_MGX, F_MGB,
-       F_MGR, F_DYE_PRED, F_WAVE, F_PTOP, F_CURL, F_COUNT };
+       F_MGR, F_DYE_PRED, F_WAVE, F_PTOP, F_CURL, F_CONC, F_COUNT };
 enum { ST_F2C, ST_ADVECT, ST_CASSON, ST_DIV, ST_VCYCLE, ST_PROJECT,
-       ST_DYE_PRED, ST_DYE_RES, ST_WAVE, ST_TOP_PRESSURE, ST_PARCELS };
+       ST_DYE_PRED, ST_DYE_RES, ST_WAVE, ST_TOP_PRESSURE, ST_PARCELS, ST_CONC_PRED,
+       ST_CONC_RES };
 #define MAX_DROPS 8
 enum { MG_SMOOTH, MG_RESIDUAL, MG_RESTRICT, MG_PROLONG, MG_COARSE, MG_ZERO };
 
@@ -55,9 +56,21 @@ struct orc_state {
   double *parcels;
   int np, np_cap;
   uint32_t *splat[2];
+  /* the reference's 4-channel concentration (planes: intact RBC, lysed Hb,
+   * oxygenation/shear state, floor bed; n[0] each) and its twin; the next
+   * parcel stage's emission (ring cursor, count) */
+  int conc_on;
+  double *conc, *conc2;
+  int emit_cursor, emit_count;
 };
 
 #define IDX(D, i, j, k) ((((long)(k)) * (D).ny + (j)) * (long)(D).nx + (i))
+
+/* the intact-RBC concentration the Casson pass, the buoyancy and the parcels
+ * read: the dye, or with the 4-channel concentration its .r plane; the lysed
+ * one (.g) is 0 without it (NULL) */
+static const double *intact_of(const orc_state *s) { return s->conc_on ? s->conc : s->dye; }
+static const double *lysed_of(const orc_state *s) { return s->conc_on ? s->conc + s->n[0] : NULL; }
 
 static double clampd(double x, double lo, double hi) {
   return x < lo ? lo : (x > hi ? hi : x);
@@ -266,7 +279,8 @@ static void casson_buoyancy(orc_state *s, double dt) {
           ss += du * du + dv * dv + dw * dw;
         }
         double gam = inv2h * sqrt(ss + P->shear_eps2);
-        double cc = clampd(s->dye[c], 0.0, 1.0);
+        const double *cr = intact_of(s);
+        double cc = clampd(cr[c], 0.0, 1.0);
         double sq_muinf = sqrt(P->mu_w + cc * (P->mu_inf - P->mu_w));
         double tau = P->tau_y * cc * cc;
         double yterm = sqrt(tau / fmax(gam + P->gamma_reg, P->gamma_reg));
@@ -279,7 +293,7 @@ static void casson_buoyancy(orc_state *s, double dt) {
           out[a] = m[a] + alpha * lap;
         }
         int jp = j + 1 < d.ny ? j + 1 : d.ny - 1;
-        double dye_face = 0.5 * (s->dye[c] + s->dye[IDX(d, i, jp, k)]);
+        double dye_face = 0.5 * (cr[c] + cr[IDX(d, i, jp, k)]);
         out[1] += buoy * dye_face;
         if (i >= d.nx - 1) out[0] = 0.0;
         if (k >= d.nz - 1) out[2] = 0.0;
@@ -296,11 +310,13 @@ static void casson_buoyancy(orc_state *s, double dt) {
  *   neighbours clamped to the edge (loadMac, fluid.ts:387-392);
  *   gamma = 50 sqrt(|du|^2 + |dv|^2 + |dw|^2 + 1e-6), du = mac(i+1) - mac(i-1)
  *     (all three components) etc. (fluid.ts:451-454);
- *   c = clamp(intact concentration, 0, 1.5) (fluid.ts:447-448; the dye here);
+ *   c = clamp(intact concentration, 0, 1.5) (fluid.ts:447-448; the dye, or
+ *     the 4-channel concentration's .r);
  *   alpha = clamp((sqrt(1e-4 (1 + 1.35 c)) + sqrt(0.0036 scale c^2 /
  *     max(gamma + 0.4, 0.4)))^2, 1e-4, 0.0044) (fluid.ts:458-462);
  *   mac += alpha (sum of 6 neighbours - 6 mac) (fluid.ts:464-465);
- *   y-faces: rho' = 1.24 c_face (+ 0.26 lysed, 0 here) sampled at the face;
+ *   y-faces: rho' = 1.24 c_face + 0.26 lysed_face (0 without the 4-channel
+ *     concentration) sampled at the face;
  *     if rho' > 0.0008: v += -buoyancy 0.345 1.24 tanh(rho'/1.24)
  *     smoothstep(0.002, 0.024, max(0, surface - y_face)) dt (fluid.ts:477-486);
  *   wall faces zeroed (fluid.ts:506-507).
@@ -335,7 +351,8 @@ static void casson_ref(orc_state *s, double dt) {
           ss += du * du + dv * dv + dw * dw;
         }
         double gam = 50.0 * sqrt(ss + 1e-6);
-        double cr = clampd(s->dye[c], 0.0, 1.5);
+        const double *ci = intact_of(s), *cl = lysed_of(s);
+        double cr = clampd(ci[c], 0.0, 1.5);
         double sq_muinf = sqrt(0.00010 * (1.0 + 1.35 * cr));
         double tau = 0.0036 * scale * cr * cr;
         double yterm = sqrt(tau / fmax(gam + 0.40, 0.40));
@@ -346,7 +363,8 @@ static void casson_ref(orc_state *s, double dt) {
           out[a] = m[a] + alpha * lap;
         }
         int jp = j + 1 < d.ny ? j + 1 : d.ny - 1;
-        double rho = 1.24 * (0.5 * (s->dye[c] + s->dye[IDX(d, i, jp, k)]));
+        long cu = IDX(d, i, jp, k);
+        double rho = 1.24 * (0.5 * (ci[c] + ci[cu])) + (cl ? 0.26 * (0.5 * (cl[c] + cl[cu])) : 0.0);
         if (rho > 0.0008) {
           double yf = y0 + (j + 1.0) / d.ny * s->wp.depth;
           double gate = sstep(0.002, 0.024, fmax(0.0, ytop - yf));
@@ -1144,7 +1162,9 @@ static void casson_scene(orc_state *s, double dt) {
         double px = -hx + (i + 0.5) / d.nx * 2.0 * hx, pz = -hx + (k + 0.5) / d.nz * 2.0 * hx;
         double py = y0 + (j + 0.5) / d.ny * s->wp.depth;
         double below = ytop - py;
-        double gate = sstep(0.11, 0.36, below) * sstep(0.0008, 0.015, clampd(s->dye[c], 0.0, 1.5));
+        const double *cl = lysed_of(s);
+        double gate = sstep(0.11, 0.36, below) *
+                      sstep(0.0008, 0.015, clampd(intact_of(s)[c], 0.0, 1.5) + (cl ? 0.45 * cl[c] : 0.0));
         if (gate > 0.001) {
           double q1 = 6.4 * px + 4.0 * py + 5.2 * pz + ts;
           double q2 = -5.6 * px + 4.8 * py + 7.2 * pz - ts * 0.77;
@@ -1322,7 +1342,10 @@ static void parcel_step(orc_state *s, int g, const parcel_scene *S) {
   }
   float crf = (float)g * 0.6180339887f + 0.17f;
   double cell_res = (double)(crf - floorf(crf));
-  double cn = sample_cell(s->dye, s->d[0], uc[0], uc[1], uc[2]);
+  /* cSample.r + cSample.g (particles.ts:403-404): the dye alone without the
+   * 4-channel concentration */
+  double cn = sample_cell(intact_of(s), s->d[0], uc[0], uc[1], uc[2]);
+  if (lysed_of(s)) cn += sample_cell(lysed_of(s), s->d[0], uc[0], uc[1], uc[2]);
   double dilute = 1.0 - sstep(0.012, 0.22, cn);
   double disp = sstep(1.1, 7.8, age + dilute * 2.2 + lysis * 2.5);
   double late = sstep(7.2, 16.0, age) * dilute;
@@ -1370,15 +1393,283 @@ static void parcel_step(orc_state *s, int g, const parcel_scene *S) {
   }
 }
 
+/* ---- PARTICLE_EMIT (particles.ts:53-186): emission `g` of the pending
+ * count writes a new parcel at ring slot (cursor + g) mod np, in the drop slot
+ * its share of the summed drain rates picks, after moving a live parcel's
+ * mass at that slot into the lysed counter.  The R3 sequence, hash33 and the
+ * fract() of the slot number are formed in single precision without fused
+ * multiply-adds, as the reference's f32 does (for slot numbers ~4e5 an FP64
+ * fract differs by up to 0.03); the rest is FP64. */
+static float fr32(float x) { return x - floorf(x); }
+static void hash33_f(float px, float py, float pz, float out[3]) {
+  float a = fr32(px * 0.1031f), b = fr32(py * 0.1030f), c = fr32(pz * 0.0973f);
+  /* p3 += dot(p3, p3.yxz + 33.33) */
+  float dt = a * (b + 33.33f) + b * (a + 33.33f) + c * (c + 33.33f);
+  a += dt; b += dt; c += dt;
+  /* fract((p3.xxy + p3.yxx) * p3.zyx) */
+  out[0] = fr32((a + b) * c);
+  out[1] = fr32((a + a) * b);
+  out[2] = fr32((b + a) * a);
+}
+static void parcel_emit(orc_state *s, int g, int count) {
+  const double hx = s->wp.half_xz, ytop = s->wp.surface_y, y0 = ytop - s->wp.depth;
+  dims3 d = s->d[0];
+  double total = 0.0;
+  for (int q = 0; q < MAX_DROPS; ++q)
+    if (s->drops[q].seed > 0.0 && s->drops[q].drain > 0.0001) total += s->drops[q].drain;
+  if (total <= 1e-6) return;
+  double target = (g + 0.5) / (double)(count > 1 ? count : 1) * total, acc = 0.0;
+  int pick = 0;
+  for (int q = 0; q < MAX_DROPS; ++q)
+    if (s->drops[q].seed > 0.0 && s->drops[q].drain > 0.0001) {
+      acc += s->drops[q].drain;
+      pick = q;
+      if (acc >= target) break;
+    }
+  const orc_drop *dr = &s->drops[pick];
+  const double seed = fmax(dr->seed, 0.5);
+  const int ring = (int)(((long)s->emit_cursor + g) % s->np);
+  double *P = s->parcels + 8L * ring;
+  if (P[3] > 1e-6) { /* the overwritten parcel's mass goes to the lysed counter */
+    double px = P[0], py = P[1], pz = P[2];
+    if (py < ytop - 0.32) py = y0 + 0.012;
+    double u[3] = {(px + hx) / (2.0 * hx), (py - y0) / (ytop - y0), (pz + hx) / (2.0 * hx)};
+    if (u[0] > 0.002 && u[1] > 0.002 && u[2] > 0.002 && u[0] < 0.998 && u[1] < 0.998 &&
+        u[2] < 0.998) {
+      int ci = clampi((int)floor(u[0] * d.nx), 0, d.nx - 1);
+      int cj = clampi((int)floor(u[1] * d.ny), 0, d.ny - 1);
+      int ck = clampi((int)floor(u[2] * d.nz), 0, d.nz - 1);
+      uint32_t q = (uint32_t)nearbyint(P[3] * 65536.0);
+      if (q > 0) s->splat[1][IDX(d, ci, cj, ck)] += q;
+    }
+  }
+  const float seedf = (float)seed;
+  const float seq = (float)ring + seedf * 137.0f;
+  float h[3];
+  hash33_f((float)g, (float)s->wp.sim_time * 60.0f, seedf, h);
+  const float r3x = fr32(seq * 0.8191725134f + h[0] * 0.032f);
+  const float r3y = fr32(seq * 0.6710436067f + h[1] * 0.032f);
+  const float r3z = fr32(seq * 0.5497004779f + h[2] * 0.032f);
+  const double th = 6.28318530718 * r3x;
+  const double az = 1.0 + 0.036 * cos(4.0 * th + seed * 1.73) + 0.026 * cos(6.0 * th - seed * 2.31) +
+                    0.018 * sin(8.0 * th + seed * 0.89);
+  const double rdrop = fmax(dr->radius, 0.010);
+  const double emerg = (dr->y - ytop) / fmax(rdrop, 0.018);
+  const double nasc = sstep(-0.32, 0.18, emerg);
+  const double slug = sstep(0.26, 0.85, dr->drain);
+  const double wake = mixd(0.46, 1.0, sstep(0.018, 0.26, dr->drain));
+  const double rneck = mixd(0.013 * wake, clampd(rdrop * 0.84 + 0.014, 0.020, 0.056), slug) * az;
+  const double inner = 0.37 * slug;
+  const int cap = slug > 0.08 && r3z < 0.16f;
+  const double rann = mixd(inner, 1.0, pow((double)r3y, mixd(0.50, 0.44, slug)));
+  const double rcap = mixd(0.14 * slug, fmax(inner * 1.32, 0.08), sqrt((double)r3y));
+  const double rfrac = mixd(cap ? rcap : rann, sqrt((double)r3y), nasc * 0.82);
+  const double r = rneck * rfrac;
+  const double wa = s->wp.sim_time * 1.85 + seed * 3.17;
+  const double wd = 0.0038 * (1.0 - slug) * (1.0 - wake * 0.45);
+  const double px = dr->x + cos(wa) * wd + cos(th) * r;
+  const double pz = dr->z + sin(wa * 0.83 + seed) * wd + sin(th) * r;
+  const double E = s->wp.half_extent;
+  const double eta = clampd(wv_sample(s->wave, s->wp.n, clampd((px + E) / (2.0 * E), 0.002, 0.998),
+                                      clampd((pz + E) / (2.0 * E), 0.002, 0.998), 0),
+                            -0.085, 0.015);
+  const double neck_top = ytop + fmin(0.0, eta) - 0.0038;
+  const double sag = (1.0 - rfrac * rfrac) * (0.0085 * slug + 0.0045 * nasc);
+  const double zs = cap ? r3z * 3.6 : (r3z - 0.16 * slug) / fmax(1.0 - 0.16 * slug, 0.5);
+  const double py = neck_top - sag - mixd(0.0014, 0.0135, clampd(zs, 0.0, 1.0));
+  const double frag = 1.0 - sstep(0.022, 0.056, rdrop);
+  const double rim = sstep(0.58, 1.0, rfrac);
+  const double lys = clampd(0.14 * rim + 0.28 * frag * (0.45 + 0.55 * rfrac), 0.0, 0.44);
+  double t[3] = {-sin(th) * fmax(rfrac, 0.25), -0.80, cos(th) * fmax(rfrac, 0.25)};
+  const double tl = sqrt(t[0] * t[0] + t[1] * t[1] + t[2] * t[2]);
+  P[0] = px; P[1] = py; P[2] = pz; P[3] = 0.13400;
+  for (int a = 0; a < 3; ++a) P[4 + a] = t[a] / tl * 0.0036;
+  P[7] = lys;
+}
+
 /* the parcel stage: both counters cleared (BloodDropEngine.ts:2073-2074),
- * then every parcel stepped and splatted (:2089-2093) */
+ * the pending emission (:2082-2087), then every parcel stepped and splatted
+ * (:2089-2093) */
 static void parcels_stage(orc_state *s, double dt) {
   if (!s->wave_on || !s->parcels) return;
   parcel_scene S = {s->wp.half_xz, s->wp.surface_y - s->wp.depth, s->wp.surface_y, s->wp.sim_time,
                     s->wp.hemolysis, clampd(dt, 0.0, 0.025)};
   memset(s->splat[0], 0, sizeof(uint32_t) * s->n[0]);
   memset(s->splat[1], 0, sizeof(uint32_t) * s->n[0]);
+  if (s->emit_count > 0 && s->np > 0)
+    for (int g = 0; g < s->emit_count; ++g) parcel_emit(s, g, s->emit_count);
+  s->emit_count = 0;
   for (int g = 0; g < s->np; ++g) parcel_step(s, g, &S);
+}
+
+/* ---- CONC_PREDICT (advect.ts:28-72) of the lysed channel: the dye
+ * predictor's code with the reference's branches, on .g */
+static void conc_predict(orc_state *s, double dt) {
+  double *keep = s->dye;
+  int phys = s->wp.physics;
+  s->dye = s->conc + s->n[0];
+  s->wp.physics = 1;
+  dye_predict(s, dt);
+  s->wp.physics = phys;
+  s->dye = keep;
+}
+
+/* ---- CONC_RESOLVE (advect.ts:81-298): the 4-channel composite.  Per cell
+ * (lateral walls and the top row 0):
+ *  - nothing to do where neither counter holds mass, the predictor's two
+ *    octant samples are below 1e-7 and (the 8 floor rows) the bed below
+ *    1e-6 (advect.ts:151-164), or (above the floor rows) the predicted lysed
+ *    value and its 6-neighbour mean are below 1.2e-6 (:176-179);
+ *  - lysed: MacCormack of the predictor (forward estimate at uvw + v0 dt) with
+ *    the 8-texel limiter of the source, a 6-neighbour diffusion blend
+ *    clamp(5.6 dt, 0, 0.105), an ambient dilution exp(-dt mix(0.016, 0.034,
+ *    height)), the lysed counter's mass (72% of it captured by the bed near
+ *    the floor), cut below 2.5e-6 (:195-237);
+ *  - bed (floor rows): self-drain, settling from above, radial spreading and
+ *    vertical diffusion, decay with radius, plus the captured mass, a
+ *    row-height fade, cut below 1e-6 (:241-272);
+ *  - intact: the intact counter blended with its 6-neighbour mean
+ *    (18% / 36% where it holds mass) plus the bed, under the radial wall
+ *    sponge; the bed also capped by a row profile (:276-285);
+ *  - oxygenation/shear state from the flow speed, the trace's convective
+ *    shear, the sheet thickness and the bed fraction (:290-297). */
+static double conc_resolve_cell(const orc_state *s, const double *src, double dt, int ix, int iy,
+                                int iz, double out[4]) {
+  dims3 d = s->d[0];
+  const long n = s->n[0];
+  const double *og = src + n, *oa = src + 3 * n, *pg = s->dpred;
+  const uint32_t *si = s->splat[0], *sl = s->splat[1];
+  out[0] = out[1] = out[2] = out[3] = 0.0;
+  if (ix == 0 || ix >= d.nx - 1 || iz == 0 || iz >= d.nz - 1 || iy >= d.ny - 1) return 0.0;
+  const double N[3] = {(double)d.nx, (double)d.ny, (double)d.nz};
+  const long c = IDX(d, ix, iy, iz);
+  const uint32_t ri = si[c], rl = sl[c];
+  const int floor_zone = iy <= 7;
+  if (ri == 0 && rl == 0) {
+    double o0 = sample_cell(pg, d, ix / N[0], iy / N[1], iz / N[2]);
+    double o1 = sample_cell(pg, d, (ix + 1.0) / N[0], (iy + 1.0) / N[1], (iz + 1.0) / N[2]);
+    double bed_on = floor_zone ? oa[c] : 0.0;
+    if (o0 + o1 <= 1.0e-7 && bed_on <= 1.0e-6) return 0.0;
+  }
+#define PG(i, j, k) pg[IDX(d, clampi(i, 0, d.nx - 1), clampi(j, 0, d.ny - 1), clampi(k, 0, d.nz - 1))]
+#define OA(i, j, k) oa[IDX(d, clampi(i, 0, d.nx - 1), clampi(j, 0, d.ny - 1), clampi(k, 0, d.nz - 1))]
+#define SI(i, j, k) (si[IDX(d, clampi(i, 0, d.nx - 1), clampi(j, 0, d.ny - 1), clampi(k, 0, d.nz - 1))] / 65536.0)
+  const double gpred = pg[c];
+  const double lapg = (PG(ix + 1, iy, iz) + PG(ix - 1, iy, iz) + PG(ix, iy + 1, iz) +
+                       PG(ix, iy - 1, iz) + PG(ix, iy, iz + 1) + PG(ix, iy, iz - 1)) * (1.0 / 6.0);
+  if (!floor_zone && ri == 0 && rl == 0 && gpred <= 1.2e-6 && lapg <= 1.2e-6) return 0.0;
+  double lapi = 0.0;
+  if (ri > 0 || gpred > 8.0e-5 || lapg > 8.0e-5)
+    lapi = (SI(ix + 1, iy, iz) + SI(ix - 1, iy, iz) + SI(ix, iy + 1, iz) + SI(ix, iy - 1, iz) +
+            SI(ix, iy, iz + 1) + SI(ix, iy, iz - 1)) * (1.0 / 6.0);
+  const double hx = s->wp.half_xz, ytop = s->wp.surface_y, y0 = ytop - s->wp.depth;
+  dt = clampd(dt, 0.0, 0.025);
+  const double uvw[3] = {(ix + 0.5) / N[0], (iy + 0.5) / N[1], (iz + 0.5) / N[2]};
+  const double dom[3] = {2.0 * hx, ytop - y0, 2.0 * hx};
+  const double p[3] = {-hx + uvw[0] * dom[0], y0 + uvw[1] * dom[1], -hx + uvw[2] * dom[2]};
+  double v0[3] = {sample_cell(s->cu, d, uvw[0], uvw[1], uvw[2]), sample_cell(s->cv, d, uvw[0], uvw[1], uvw[2]),
+                  sample_cell(s->cw, d, uvw[0], uvw[1], uvw[2])};
+  double mid[3], back[3], fwd[3];
+  for (int a = 0; a < 3; ++a) mid[a] = clampd(uvw[a] - 0.5 * (v0[a] * dt) / dom[a], 0.001, 0.999);
+  double vm[3] = {sample_cell(s->cu, d, mid[0], mid[1], mid[2]), sample_cell(s->cv, d, mid[0], mid[1], mid[2]),
+                  sample_cell(s->cw, d, mid[0], mid[1], mid[2])};
+  for (int a = 0; a < 3; ++a) {
+    back[a] = clampd(uvw[a] - (vm[a] * dt) / dom[a], 0.001, 0.999);
+    fwd[a] = clampd(uvw[a] + (v0[a] * dt) / dom[a], 0.001, 0.999);
+  }
+  const double est = sample_cell(pg, d, fwd[0], fwd[1], fwd[2]);
+  const double corr = gpred + s->p.maccormack * (og[c] - est);
+  int bi = (int)floor(back[0] * N[0] - 0.5), bj = (int)floor(back[1] * N[1] - 0.5),
+      bk = (int)floor(back[2] * N[2] - 0.5);
+  double lo = 1e300, hi = -1e300;
+  for (int dz = 0; dz < 2; ++dz)
+    for (int dy = 0; dy < 2; ++dy)
+      for (int dx = 0; dx < 2; ++dx) {
+        double v = og[IDX(d, clampi(bi + dx, 0, d.nx - 1), clampi(bj + dy, 0, d.ny - 1),
+                          clampi(bk + dz, 0, d.nz - 1))];
+        lo = fmin(lo, v);
+        hi = fmax(hi, v);
+      }
+  double cl = clampd(corr, lo, hi);
+  cl = mixd(cl, lapg, clampd(dt * 5.60, 0.0, 0.105));
+  cl *= exp(-dt * mixd(0.016, 0.034, sstep(y0 + 0.045, y0 + 0.26, p[1])));
+  const double dm = rl / 65536.0;
+  const double capture = 1.0 - sstep(y0 + 0.018, y0 + 0.058, p[1]);
+  cl += dm * (1.0 - capture * 0.72);
+  if (cl < 2.5e-6) cl = 0.0;
+  const double rh = sqrt(p[0] * p[0] + p[2] * p[2]);
+  double bed = 0.0;
+  if (floor_zone) {
+    const double bc = oa[c];
+    const double bh = 0.25 * (OA(ix + 1, iy, iz) + OA(ix - 1, iy, iz) + OA(ix, iy, iz + 1) + OA(ix, iy, iz - 1));
+    const double ba = OA(ix, iy + 1, iz), bb = OA(ix, iy > 0 ? iy - 1 : 0, iz);
+    const double spread = clampd(dt * (3.20 + 6.50 * fmin(bc, 0.65)), 0.0, 0.165);
+    const double vert = clampd(dt * 0.75, 0.0, 0.026);
+    const double settle = clampd(dt * 2.80, 0.0, 0.090);
+    const double decay = exp(-dt * mixd(0.095, 0.28, sstep(hx * 0.35, hx * 0.78, rh)));
+    const double drained = (iy == 0 ? bc : bc * (1.0 - settle)) * decay;
+    const double above = iy >= 7 ? 0.0 : ba * settle;
+    const double sm = mixd(mixd(drained + above, bh, spread), 0.5 * (ba + bb), vert);
+    const double fade = 1.0 - 0.38 * sstep(3.5, 7.0, (double)iy);
+    bed = (sm + dm * capture * 0.68) * fade;
+    if (bed < 1.0e-6) bed = 0.0;
+  }
+  const double wall = 1.0 - sstep(hx * 0.68, hx * 0.94, rh);
+  const double taper = 1.0 - sstep(hx * 0.52, hx * 0.88, rh);
+  bed = fmin(bed * wall, 0.54 * exp(-0.56 * iy) * taper);
+  const double craw = ri / 65536.0;
+  const double intact = mixd(craw, lapi, ri > 0 ? 0.36 : 0.18) * wall + bed;
+  cl *= wall;
+  const double tot = intact + cl;
+  double ox = 0.0;
+  if (tot > 1.0e-5) {
+    const double fl = clampd(cl / fmax(tot, 1.0e-5), 0.0, 1.0), fi = 1.0 - fl;
+    const double fb = sstep(0.0, 0.85, bed / fmax(tot, 1.0e-5));
+    const double speed = sqrt(v0[0] * v0[0] + v0[1] * v0[1] + v0[2] * v0[2]);
+    const double dvx = v0[0] - vm[0], dvy = v0[1] - vm[1], dvz = v0[2] - vm[2];
+    const double conv = sqrt(dvx * dvx + dvy * dvy + dvz * dvz) / fmax(0.5 * dt, 1.0e-3);
+    const double sh = sstep(0.014, 0.135, speed + 0.38 * conv);
+    const double thin = (1.0 - sstep(0.024, 0.24, intact)) * sstep(0.0006, 0.0065, intact);
+    const double act = thin * (0.38 + 0.62 * sh) * fi;
+    const double raw = clampd(0.08 + 0.62 * act + 0.36 * sh * fi, 0.0, 1.0);
+    ox = raw * (1.0 - 0.82 * fb);
+  }
+#undef PG
+#undef OA
+#undef SI
+  out[0] = intact; out[1] = cl; out[2] = ox; out[3] = bed;
+  return 0.0;
+}
+static void conc_resolve(orc_state *s, double dt) {
+  dims3 d = s->d[0];
+  const long n = s->n[0];
+#pragma omp parallel for
+  for (int k = 0; k < d.nz; ++k)
+    for (int j = 0; j < d.ny; ++j)
+      for (int i = 0; i < d.nx; ++i) {
+        double o[4];
+        conc_resolve_cell(s, s->conc, dt, i, j, k, o);
+        long c = IDX(d, i, j, k);
+        for (int a = 0; a < 4; ++a) s->conc2[a * n + c] = o[a];
+      }
+  double *t = s->conc; s->conc = s->conc2; s->conc2 = t;
+}
+
+void orc_conc_enable(orc_state *s, int on) {
+  if (on && !s->conc) {
+    s->conc = (double *)calloc(4 * (size_t)s->n[0], sizeof(double));
+    s->conc2 = (double *)calloc(4 * (size_t)s->n[0], sizeof(double));
+  }
+  if (on && !s->splat[0]) {
+    s->splat[0] = (uint32_t *)calloc(s->n[0], sizeof(uint32_t));
+    s->splat[1] = (uint32_t *)calloc(s->n[0], sizeof(uint32_t));
+  }
+  s->conc_on = on != 0;
+}
+
+void orc_parcels_emit(orc_state *s, int cursor, int count) {
+  s->emit_cursor = cursor;
+  s->emit_count = count;
 }
 
 void orc_parcels_set(orc_state *s, const double *p, int n) {
@@ -1462,7 +1753,7 @@ void orc_destroy(orc_state *s) {
   for (int l = 1; l <= s->p.levels; ++l) { free(s->mgx[l]); free(s->mgb[l]); free(s->mgr[l]); }
   for (int a = 0; a < 4; ++a) free(s->cg[a]);
   free(s->wave); free(s->wz); free(s->wz2); free(s->wfilm); free(s->ptop); free(s->curl);
-  free(s->parcels); free(s->splat[0]); free(s->splat[1]);
+  free(s->parcels); free(s->splat[0]); free(s->splat[1]); free(s->conc); free(s->conc2);
   free(s);
 }
 
@@ -1485,6 +1776,7 @@ double *orc_field(orc_state *s, int field, int level) {
     case F_WAVE: return s->wave_on && level == 0 ? s->wave : NULL;
     case F_PTOP: return s->wave_on && level == 0 ? s->ptop : NULL;
     case F_CURL: return level == 0 ? s->curl : NULL;
+    case F_CONC: return level == 0 && s->conc_on ? s->conc : NULL;
   }
   return NULL;
 }
@@ -1493,7 +1785,7 @@ long orc_field_len(orc_state *s, int field, int level) {
   if (field == F_MGX || field == F_MGB || field == F_MGR) return s->n[level];
   if (field == F_WAVE) return 4L * s->wp.n * s->wp.n;
   if (field == F_PTOP) return (long)s->d[0].nx * s->d[0].nz;
-  if (field == F_CURL) return 4L * s->n[0];
+  if (field == F_CURL || field == F_CONC) return 4L * s->n[0];
   return s->n[0];
 }
 
@@ -1540,6 +1832,8 @@ void orc_stage(orc_state *s, int stage, double dt) {
     case ST_WAVE: if (s->wave_on) wave_step(s, dt); break;
     case ST_TOP_PRESSURE: if (s->wave_on) top_pressure(s, dt); break;
     case ST_PARCELS: parcels_stage(s, dt); break;
+    case ST_CONC_PRED: if (s->conc_on) conc_predict(s, dt); break;
+    case ST_CONC_RES: if (s->conc_on) conc_resolve(s, dt); break;
   }
 }
 
@@ -1598,8 +1892,14 @@ int orc_step(orc_state *s, double dt, int nsteps, double *rel_res) {
     }
     project(s);
     face_to_centre(s);
-    dye_predict(s, dt);
-    dye_resolve(s, dt);
+    if (s->conc_on) {  /* pass B (:2073-2108): parcels, then the concentration */
+      parcels_stage(s, dt);
+      conc_predict(s, dt);
+      conc_resolve(s, dt);
+    } else {
+      dye_predict(s, dt);
+      dye_resolve(s, dt);
+    }
   }
   if (rel_res) *rel_res = rr;
   return cycles;

@@ -77,6 +77,11 @@ void orc_set_drops(orc_state *s, const orc_drop *d, int n);
  * age ticks + lysis); orc_parcels returns them after ST_PARCELS, orc_splat
  * the intact (0) and lysed (1) counters, n cells each */
 void orc_parcels_set(orc_state *s, const double *p, int n);
+/* the next parcel stage first emits `count` parcels from ring slot `cursor`
+ * (PARTICLE_EMIT); the 4-channel concentration on or off (stages 11, 12 and
+ * pass B in the step; field 16 = its planes r, g, b, a) */
+void orc_parcels_emit(orc_state *s, int cursor, int count);
+void orc_conc_enable(orc_state *s, int on);
 double *orc_parcels(orc_state *s);
 uint32_t *orc_splat(orc_state *s, int which);
 /* unitary in-place complex FFT of length n (a power of two) over interleaved

@@ -16,9 +16,9 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 LIB = os.path.join(HERE, "liborc.so")
 
 (F_U, F_V, F_W, F_P, F_DYE, F_CU, F_CV, F_CW, F_RHS, F_MGX, F_MGB, F_MGR, F_DYE_PRED, F_WAVE,
- F_PTOP, F_CURL) = range(16)
+ F_PTOP, F_CURL, F_CONC) = range(17)
 (ST_F2C, ST_ADVECT, ST_CASSON, ST_DIV, ST_VCYCLE, ST_PROJECT, ST_DYE_PRED, ST_DYE_RES, ST_WAVE,
- ST_TOP_PRESSURE, ST_PARCELS) = range(11)
+ ST_TOP_PRESSURE, ST_PARCELS, ST_CONC_PRED, ST_CONC_RES) = range(13)
 MG_SMOOTH, MG_RESIDUAL, MG_RESTRICT, MG_PROLONG, MG_COARSE, MG_ZERO = range(6)
 
 
@@ -78,6 +78,8 @@ def lib():
         L.orc_splat.argtypes = [C.c_void_p, C.c_int]
         L.orc_splat.restype = C.POINTER(C.c_uint32)
         L.orc_set_drops.argtypes = [C.c_void_p, C.POINTER(OrcDrop), C.c_int]
+        L.orc_parcels_emit.argtypes = [C.c_void_p, C.c_int, C.c_int]
+        L.orc_conc_enable.argtypes = [C.c_void_p, C.c_int]
         L.orc_fft.argtypes = [C.POINTER(C.c_double), C.c_int, C.c_int, C.c_int]
         L.orc_set_threads.restype = C.c_int
         L.orc_set_threads.argtypes = [C.c_int]
@@ -147,6 +149,8 @@ class Oracle:
             shape = (self.shape[0], self.shape[2])
         elif field == F_CURL:
             shape = tuple(self.shape) + (4,)
+        elif field == F_CONC:
+            shape = (4,) + tuple(self.shape)  # planes: intact, lysed, oxygenation, bed
         else:
             shape = self.shape
         return np.ctypeslib.as_array(ptr, shape=(n,)).reshape(shape)
@@ -209,6 +213,15 @@ class Oracle:
         ptr = lib().orc_splat(self.h, int(which))
         n = int(np.prod(self.shape))
         return np.ctypeslib.as_array(ptr, shape=(n,)).reshape(self.shape).copy()
+
+    def conc_enable(self, on=True):
+        """the reference's 4-channel concentration (pass B in the step)"""
+        lib().orc_conc_enable(self.h, int(on))
+
+    def parcels_emit(self, cursor, count):
+        """the next parcel stage first emits `count` parcels from ring slot
+        `cursor` (PARTICLE_EMIT)"""
+        lib().orc_parcels_emit(self.h, int(cursor), int(count))
 
     def set_drops(self, drops):
         """drops: sequences (x, y, z, radius, vel_y, osc, drain, seed)"""
