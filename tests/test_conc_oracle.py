@@ -142,6 +142,43 @@ def test_emission_lies_on_the_r3_sequence_in_the_neck():
     assert np.all((Q[:, 7] >= 0) & (Q[:, 7] <= 0.44))
 
 
+def _hash33_f32(px, py, pz):
+    """hash33 (src/shaders/common.ts:76-80) in numpy float32, every
+    operation rounded to single precision"""
+    f = np.float32
+
+    def fr(x):
+        return (x - np.floor(x)).astype(f)
+
+    a0, b0, c0 = fr(px * f(0.1031)), fr(py * f(0.1030)), fr(pz * f(0.0973))
+    t = (a0 * (b0 + f(33.33)) + b0 * (a0 + f(33.33))) + c0 * (c0 + f(33.33))
+    a, b, c = a0 + t, b0 + t, c0 + t
+    return fr((a + b) * c), fr((a + a) * b), fr((b + a) * a)
+
+
+def test_emission_angle_is_the_single_precision_r3_value():
+    """The same slug (no wake drift, so a parcel lies exactly on its ray
+    from the drop axis): each parcel's angle about the axis equals
+    2 pi r3.x with r3.x = fract(seq 0.8191725134 + hash33(g, 60 t, seed).x
+    0.032) formed in float32 as the reference's f32 shader does
+    (particles.ts:120-126) -- a one-ulp change of the sum at seq ~ 180
+    (1.5e-5) moves the angle by 1e-4 rad, far above the tolerance."""
+    f = np.float32
+    o = wide()
+    ring = 256
+    Q = emit(o, [DROP_A], cursor=0, count=ring, ring=ring)
+    x0, z0, seed = DROP_A[0], DROP_A[2], f(DROP_A[7])
+    g = np.arange(ring, dtype=f)
+    seq = g + seed * f(137.0)
+    h = _hash33_f32(g, np.full(ring, f(0.0)) * f(60.0), np.full(ring, seed))
+    r3x = (seq * f(0.8191725134) + h[0] * f(0.032))
+    r3x = (r3x - np.floor(r3x)).astype(f)
+    want = 2 * np.pi * r3x.astype(np.float64)
+    got = np.arctan2(Q[:, 2] - z0, Q[:, 0] - x0)
+    dev = (got - want + np.pi) % (2 * np.pi) - np.pi
+    assert np.abs(dev).max() <= 1e-9
+
+
 # ------------------------------------------------------------------ resolve
 def set_conc(o, r=None, g=None, b=None, a=None):
     c = o.view(oracle.F_CONC)
