@@ -27,6 +27,7 @@
 #include "parcel_kernels.hpp"
 #include "conc_kernels.hpp"
 #include "profile.hpp"
+#include "release.hpp"
 
 using namespace mac3d;
 
@@ -2452,6 +2453,146 @@ int mac3d_adjoint(void *h, const mac3d_blob *blobs, int nblobs, double dt, int n
   ctx->dmm_valid = false;
   ctx->prof.collect();
   return rc;
+}
+
+// ---- drop releases (release.hpp): the blobs of the releases, the adjoint
+// carried on to the release parameters, and a quasi-Newton fit
+int mac3d_release_default_map(const mac3d_config *cfg, mac3d_release_map *m) {
+  if (!cfg || !m) return noalloc_fail(MAC3D_EINVAL, "null config or map");
+  if (cfg->nx < 1 || cfg->ny < 1 || cfg->nz < 1)
+    return noalloc_fail(MAC3D_EINVAL, "grid sizes must be positive");
+  // DOMAIN_HALF_XZ 0.48, WATER_SURFACE_Y 0.30 (uniformLayout.ts:9-11,
+  // dropKinematics.ts:5); G_SLOW, MAX_DROP_OFFSET_XZ, PENDANT_MIN_RADIUS and
+  // the 0.095 radius cap (dropKinematics.ts:8-16, 364)
+  const double cell = 2.0 * 0.48 / cfg->nx;
+  m->cell = cell;
+  m->origin_x = -0.5 * cell * cfg->nx;
+  m->origin_z = -0.5 * cell * cfg->nz;
+  m->surface_y = 0.30;
+  m->origin_y = m->surface_y - cell * cfg->ny;
+  m->gravity = -0.155;
+  m->vel_scale = 1.0;
+  m->submerge = 1.0;
+  m->max_offset = 0.26;
+  m->r_min = 0.018;
+  m->r_max = 0.095;
+  return MAC3D_OK;
+}
+
+int mac3d_release_blobs(const mac3d_release_map *map, const mac3d_release *rel, int n,
+                        mac3d_blob *blobs, double *jac) {
+  if (!map || !rel || !blobs) return noalloc_fail(MAC3D_EINVAL, "null map, releases or blobs");
+  if (n < 1 || n > MAC3D_MAX_BLOBS) return noalloc_fail(MAC3D_EINVAL, "need 1..MAC3D_MAX_BLOBS releases");
+  if (!release_map_ok(*map)) return noalloc_fail(MAC3D_EINVAL, "release map: non-finite value, cell <= 0, gravity >= 0 or bad clamps");
+  for (int q = 0; q < n; ++q)
+    if (!release_ok(rel[q])) return noalloc_fail(MAC3D_EINVAL, "release parameters must be finite");
+  for (int q = 0; q < n; ++q) release_blob(*map, rel[q], blobs[q], jac ? jac + 25 * q : nullptr);
+  return MAC3D_OK;
+}
+
+int mac3d_adjoint_release(void *h, const mac3d_release_map *map, const mac3d_release *rel, int n,
+                          double dt, int nsteps, int checkpoint_every, const void *target,
+                          int dtype, mac3d_release_grad *out) {
+  if (!h) return MAC3D_EINVAL;
+  Ctx *ctx = (Ctx *)h;
+  if (!out) return fail(ctx, MAC3D_EINVAL, "null result");
+  mac3d_blob blobs[MAC3D_MAX_BLOBS];
+  double jac[25 * MAC3D_MAX_BLOBS];
+  int rc = mac3d_release_blobs(map, rel, n, blobs, jac);
+  if (rc) return fail(ctx, rc, g_create_err);
+  mac3d_grad g;
+  if ((rc = mac3d_adjoint(h, blobs, n, dt, nsteps, checkpoint_every, target, dtype, &g))) return rc;
+  std::memset(out, 0, sizeof *out);
+  out->loss = g.loss;
+  out->d_rho_b = g.d_rho_b;
+  out->n = n;
+  out->steps = g.steps;
+  for (int q = 0; q < n; ++q)
+    for (int b = 0; b < 5; ++b) {
+      double a = 0.0;  // (J^T dL/dblob)_b
+      for (int r = 0; r < 5; ++r) a += jac[25 * q + 5 * r + b] * g.d_blob[q][r];
+      out->d_release[q][b] = a;
+      out->d_blob[q][b] = g.d_blob[q][b];
+    }
+  return MAC3D_OK;
+}
+
+int mac3d_fit_default_opts(mac3d_fit_opts *o) {
+  if (!o) return noalloc_fail(MAC3D_EINVAL, "null options");
+  std::memset(o, 0, sizeof *o);
+  o->max_iters = 20;
+  o->checkpoint_every = 10;
+  o->mask = 0;
+  o->tol = 1e-6;
+  o->gtol = 1e-9;
+  return MAC3D_OK;
+}
+
+int mac3d_fit_release(void *h, const mac3d_release_map *map, mac3d_release *rel, int n,
+                      double dt, int nsteps, const void *target, int dtype,
+                      const mac3d_fit_opts *opts, mac3d_fit_report *report) {
+  if (!h) return MAC3D_EINVAL;
+  Ctx *ctx = (Ctx *)h;
+  mac3d_fit_opts o;
+  mac3d_fit_default_opts(&o);
+  if (opts) o = *opts;
+  if (o.max_iters < 0 || o.checkpoint_every < 1 || !(o.tol >= 0.0) || !(o.gtol >= 0.0))
+    return fail(ctx, MAC3D_EINVAL, "fit options: max_iters >= 0, checkpoint_every >= 1, tol and gtol >= 0");
+  if (n < 1 || n > MAC3D_MAX_BLOBS || !rel) return fail(ctx, MAC3D_EINVAL, "need 1..MAC3D_MAX_BLOBS releases");
+  if (!std::isfinite(dt) || !(dt > 0.0) || nsteps < 1)
+    return fail(ctx, MAC3D_EINVAL, "dt must be finite and > 0, nsteps >= 1");
+  uint64_t mask = o.mask;
+  if (!mask)
+    for (int q = 0; q < n; ++q)
+      mask |= (uint64_t)0x17 << (5 * q);  // x, z, radius, vel_y
+  if (5 * n < 64 && (mask >> (5 * n)))
+    return fail(ctx, MAC3D_EINVAL, "fit mask names parameters of releases past n");
+  // the fitted parameters and their scales (about one cell of blob motion)
+  mac3d_blob b0[MAC3D_MAX_BLOBS];
+  double jac[25 * MAC3D_MAX_BLOBS];
+  int rc = mac3d_release_blobs(map, rel, n, b0, jac);
+  if (rc) return fail(ctx, rc, g_create_err);
+  const double run = nsteps * dt;
+  std::vector<int> qi, bi;
+  std::vector<double> scale, p0;
+  for (int q = 0; q < n; ++q)
+    for (int b = 0; b < 5; ++b) {
+      if (!((mask >> (5 * q + b)) & 1)) continue;
+      double sc = map->cell;
+      if (b == RL_START_Y || b == RL_VEL_Y) {
+        const double dw = std::fabs(jac[25 * q + 5 * RB_W0 + b]);
+        const double slope = dw > 1e-12 ? dw : std::max(std::fabs(map->vel_scale), 1e-12);
+        sc = ctx->cfg.dx / (run * slope);
+      }
+      qi.push_back(q);
+      bi.push_back(b);
+      scale.push_back(sc);
+      p0.push_back(rel_param(rel[q], b));
+    }
+  std::vector<mac3d_release> cur(rel, rel + n);
+  auto eval = [&](const std::vector<double> &x, double &f, std::vector<double> &g) -> int {
+    for (size_t i = 0; i < x.size(); ++i) rel_param(cur[qi[i]], bi[i]) = p0[i] + scale[i] * x[i];
+    mac3d_release_grad rg;
+    int r = mac3d_adjoint_release(h, map, cur.data(), n, dt, nsteps, o.checkpoint_every,
+                                  target, dtype, &rg);
+    if (r) return r;
+    f = rg.loss;
+    for (size_t i = 0; i < x.size(); ++i) g[i] = rg.d_release[qi[i]][bi[i]] * scale[i];
+    return MAC3D_OK;
+  };
+  std::vector<double> x(qi.size(), 0.0);
+  FitResult R = bfgs_minimise(x, eval, o.max_iters, o.tol, o.gtol);
+  if (R.status) return R.status;
+  for (size_t i = 0; i < x.size(); ++i) rel_param(rel[qi[i]], bi[i]) = p0[i] + scale[i] * x[i];
+  if (report) {
+    std::memset(report, 0, sizeof *report);
+    report->iters = R.iters;
+    report->evals = R.evals;
+    report->converged = R.converged;
+    report->loss0 = R.f0;
+    report->loss = R.f;
+  }
+  return MAC3D_OK;
 }
 
 int mac3d_stage(void *h, int stage, double dt) {

@@ -27,6 +27,14 @@ export declare const FIELD: Readonly<{ U: 0; V: 1; W: 2; P: 3; DYE: 4; CU: 5; CV
   RHS: 8; MGX: 9; MGB: 10; MGR: 11; DYE_PRED: 12; WAVE: 13; PTOP: 14; CURL: 15; CONC: 16 }>;
 export declare const PRECISION: Readonly<{ fp32: 0; mixed: 1; fp64: 2 }>;
 export declare function version(): string;
+/** MultiDropSystem.spawnAerialDrop's parameters (scene metres, m/s) */
+export interface Release { x: number; z: number; radius: number; startY: number; velY: number; }
+export interface ReleaseMap {
+  originX: number; originY: number; originZ: number; cell: number; surfaceY: number;
+  gravity: number; velScale: number; submerge: number; maxOffset: number; rMin: number;
+  rMax: number;
+}
+export declare function releaseBlobs(map: ReleaseMap, releases: Release[]): Blob[];
 export declare class MacSolver {
   constructor(config?: MacConfig);
   readonly config: Required<MacConfig>;
@@ -51,6 +59,17 @@ export declare class MacSolver {
   backward(target: Float32Array | Float64Array,
            run: { blobs: Blob[]; dt: number; steps: number; checkpointEvery?: number }):
     { loss: number; dRhoB: number; dBlobs: Blob[] };
+  releaseMap(overrides?: Partial<ReleaseMap>): ReleaseMap;
+  backwardRelease(target: Float32Array | Float64Array,
+                  run: { releases: Release[]; dt: number; steps: number;
+                         checkpointEvery?: number; map?: ReleaseMap }):
+    { loss: number; dRhoB: number; dReleases: Release[]; dBlobs: Blob[] };
+  fitRelease(target: Float32Array | Float64Array,
+             run: { releases: Release[]; dt: number; steps: number; map?: ReleaseMap;
+                    maxIters?: number; checkpointEvery?: number; mask?: number; tol?: number;
+                    gtol?: number }):
+    { releases: Release[]; iters: number; evals: number; converged: number; loss0: number;
+      loss: number };
   divNorms(): { maxDiv: number; l2Div: number };
   saveCheckpoint(): Uint8Array;
   loadCheckpoint(bytes: Uint8Array | ArrayBuffer): void;

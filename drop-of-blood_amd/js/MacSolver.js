@@ -1,7 +1,8 @@
 // @ts-check
 'use strict';
 /**
- * TypeScript-checked host of the MI355X MAC step (Node N-API over the C ABI).
+ * JavaScript host of the MI355X MAC step (Node N-API over the C ABI), typed
+ * by JSDoc under @ts-check and by MacSolver.d.ts.
  *
  * Stands where the reference's WebGPU compute pass stands: the engine used to
  * record the MAC kernels in encodePhysicsStep (src/engine/BloodDropEngine.ts:
@@ -29,6 +30,11 @@ const PRECISION = Object.freeze({ fp32: 0, mixed: 1, fp64: 2 });
  *   tau_y?:number, mu_inf?:number, g?:number, gamma_reg?:number,
  *   shear_eps2?:number, alpha_max?:number, vel_damp?:number,
  *   maccormack?:number, ir_tol?:number, extrapolate?:number, sor_omega?:number}} MacConfig
+ * @typedef {{x:number, z:number, radius:number, startY:number, velY:number}} Release
+ *   spawnAerialDrop(offsetXZ = [x, z], radius, seed, startY, velY), scene metres and m/s
+ * @typedef {{originX:number, originY:number, originZ:number, cell:number,
+ *   surfaceY:number, gravity:number, velScale:number, submerge:number,
+ *   maxOffset:number, rMin:number, rMax:number}} ReleaseMap
  * @typedef {{steps:number, vcycles:number, irIters:number, converged:number,
  *   maxDiv:number, l2Div:number, maxDivRate:number, relRes:number, ms:number}} StepStats
  */
@@ -190,6 +196,48 @@ class MacSolver {
       target);
   }
 
+  /**
+   * The release map of this solver's grid (the reference tank's 0.96 m on nx
+   * cells, the top face at the surface), with any fields overridden.
+   * @param {Partial<ReleaseMap>} [overrides]
+   * @returns {ReleaseMap}
+   */
+  releaseMap(overrides = {}) {
+    return Object.assign(addon.releaseDefaultMap(this.config), overrides);
+  }
+
+  /**
+   * Gradient of L = 1/2 ||dye_N - target||^2 with respect to the parameters
+   * of MultiDropSystem.spawnAerialDrop (dropKinematics.ts:354-391) of each
+   * release, through the blobs the release map makes of them.
+   * @param {Float32Array|Float64Array} target one dye value per cell
+   * @param {{releases:Release[], dt:number, steps:number, checkpointEvery?:number,
+   *   map?:ReleaseMap}} run
+   * @returns {{loss:number, dRhoB:number, dReleases:Release[],
+   *   dBlobs:{cx:number, cy:number, cz:number, radius:number, w0:number}[]}}
+   */
+  backwardRelease(target, run) {
+    if (!Number.isFinite(run.dt) || run.dt <= 0) throw new RangeError('dt must be finite and > 0');
+    return addon.adjointRelease(this._live(), run.map || this.releaseMap(), run.releases, run.dt,
+      run.steps, run.checkpointEvery || 10, target);
+  }
+
+  /**
+   * Fits the releases to a target dye field from the given guess
+   * (quasi-Newton on x, z, radius and velY by default; startY held).
+   * @param {Float32Array|Float64Array} target
+   * @param {{releases:Release[], dt:number, steps:number, map?:ReleaseMap,
+   *   maxIters?:number, checkpointEvery?:number, mask?:number, tol?:number,
+   *   gtol?:number}} run
+   * @returns {{releases:Release[], iters:number, evals:number, converged:number,
+   *   loss0:number, loss:number}}
+   */
+  fitRelease(target, run) {
+    if (!Number.isFinite(run.dt) || run.dt <= 0) throw new RangeError('dt must be finite and > 0');
+    return addon.fitRelease(this._live(), run.map || this.releaseMap(), run.releases, run.dt,
+      run.steps, target, run);
+  }
+
   divNorms() {
     return addon.divNorms(this._live());
   }
@@ -217,4 +265,14 @@ class MacSolver {
   }
 }
 
-module.exports = { MacSolver, FIELD, PRECISION, version: addon.version };
+/**
+ * The blobs of releases under a map (host arithmetic, no solver needed).
+ * @param {ReleaseMap} map
+ * @param {Release[]} releases
+ * @returns {{cx:number, cy:number, cz:number, radius:number, w0:number}[]}
+ */
+function releaseBlobs(map, releases) {
+  return addon.releaseBlobs(map, releases);
+}
+
+module.exports = { MacSolver, FIELD, PRECISION, releaseBlobs, version: addon.version };

@@ -74,6 +74,79 @@ class Grad(C.Structure):
                 "d_blob": [[self.d_blob[q][p] for p in range(5)] for q in range(self.nblobs)]}
 
 
+RELEASE_PARAMS = ("x", "z", "radius", "start_y", "vel_y")
+BLOB_PARAMS = ("cx", "cy", "cz", "radius", "w0")
+
+
+class Release(C.Structure):
+    """MultiDropSystem.spawnAerialDrop's parameters (dropKinematics.ts:354-391)"""
+    _fields_ = [(n, C.c_double) for n in RELEASE_PARAMS]
+
+
+class ReleaseMap(C.Structure):
+    _fields_ = [(n, C.c_double) for n in ("origin_x", "origin_y", "origin_z", "cell", "surface_y",
+                                           "gravity", "vel_scale", "submerge", "max_offset",
+                                           "r_min", "r_max")]
+
+
+class ReleaseGrad(C.Structure):
+    _fields_ = [("loss", C.c_double), ("d_rho_b", C.c_double), ("n", C.c_int32),
+                ("steps", C.c_int32), ("d_release", (C.c_double * 5) * MAX_BLOBS),
+                ("d_blob", (C.c_double * 5) * MAX_BLOBS)]
+
+    def as_dict(self):
+        return {"loss": self.loss, "d_rho_b": self.d_rho_b, "steps": self.steps,
+                "d_release": [[self.d_release[q][p] for p in range(5)] for q in range(self.n)],
+                "d_blob": [[self.d_blob[q][p] for p in range(5)] for q in range(self.n)]}
+
+
+class FitOpts(C.Structure):
+    _fields_ = [("max_iters", C.c_int32), ("checkpoint_every", C.c_int32), ("mask", C.c_uint64),
+                ("tol", C.c_double), ("gtol", C.c_double)]
+
+
+class FitReport(C.Structure):
+    _fields_ = [("iters", C.c_int32), ("evals", C.c_int32), ("converged", C.c_int32),
+                ("reserved", C.c_int32), ("loss0", C.c_double), ("loss", C.c_double)]
+
+    def as_dict(self):
+        return {n: getattr(self, n) for n, _ in self._fields_ if n != "reserved"}
+
+
+def _releases(rel: Sequence):
+    """Release structs from objects or dicts with the five fields"""
+    def get(r, n):
+        return r[n] if isinstance(r, dict) else getattr(r, n)
+    return (Release * len(rel))(*[Release(*[float(get(r, n)) for n in RELEASE_PARAMS])
+                                  for r in rel])
+
+
+def release_default_map(cfg: "Config") -> ReleaseMap:
+    """the reference tank's width on nx cells, top face at the surface
+    (mac3d_release_default_map)"""
+    L = load_library()
+    m = ReleaseMap()
+    rc = L.mac3d_release_default_map(C.byref(cfg), C.byref(m))
+    if rc:
+        raise Mac3dError(rc, L.mac3d_last_error(None).decode())
+    return m
+
+
+def release_blobs(rmap: ReleaseMap, rel: Sequence):
+    """(blobs, jacobians): the blobs of the releases and, per release, the
+    5x5 d(cx, cy, cz, radius, w0)/d(x, z, radius, start_y, vel_y)
+    (mac3d_release_blobs; host arithmetic, no device)"""
+    L = load_library()
+    arr = _releases(rel)
+    n = len(rel)
+    blobs = (Blob * n)()
+    jac = np.zeros((n, 5, 5))
+    rc = L.mac3d_release_blobs(C.byref(rmap), arr, n, blobs, jac.ctypes.data)
+    if rc:
+        raise Mac3dError(rc, L.mac3d_last_error(None).decode())
+    return [blobs[q] for q in range(n)], jac
+
+
 class Mac3dError(RuntimeError):
     def __init__(self, code, msg):
         super().__init__(f"mac3d {STATUS.get(code, code)}: {msg}")
@@ -88,7 +161,9 @@ EXPORTS = ("mac3d_default_config", "mac3d_create", "mac3d_destroy", "mac3d_uploa
            "mac3d_wave_enable", "mac3d_set_drops", "mac3d_set_uniforms", "mac3d_parcels_upload",
            "mac3d_parcels_download", "mac3d_splat_download", "mac3d_checkpoint_size",
            "mac3d_checkpoint_save", "mac3d_checkpoint_load", "mac3d_div_norms_noalloc",
-           "mac3d_conc_enable", "mac3d_parcels_emit")
+           "mac3d_conc_enable", "mac3d_parcels_emit", "mac3d_release_default_map",
+           "mac3d_release_blobs", "mac3d_adjoint_release", "mac3d_fit_default_opts",
+           "mac3d_fit_release")
 
 
 def load_library(path: Optional[str] = None):
@@ -128,6 +203,16 @@ def load_library(path: Optional[str] = None):
     L.mac3d_conc_enable.argtypes = [vp, C.c_int]
     L.mac3d_parcels_emit.argtypes = [vp, C.c_int32, C.c_int32]
     L.mac3d_div_norms_noalloc.argtypes = [vp, vp, vp, C.c_int, C.c_int, C.c_int, C.c_int, vp, vp]
+    L.mac3d_release_default_map.argtypes = [C.POINTER(Config), C.POINTER(ReleaseMap)]
+    L.mac3d_release_blobs.argtypes = [C.POINTER(ReleaseMap), C.POINTER(Release), C.c_int,
+                                      C.POINTER(Blob), vp]
+    L.mac3d_adjoint_release.argtypes = [vp, C.POINTER(ReleaseMap), C.POINTER(Release), C.c_int,
+                                        C.c_double, C.c_int, C.c_int, vp, C.c_int,
+                                        C.POINTER(ReleaseGrad)]
+    L.mac3d_fit_default_opts.argtypes = [C.POINTER(FitOpts)]
+    L.mac3d_fit_release.argtypes = [vp, C.POINTER(ReleaseMap), C.POINTER(Release), C.c_int,
+                                    C.c_double, C.c_int, vp, C.c_int, C.POINTER(FitOpts),
+                                    C.POINTER(FitReport)]
     L.mac3d_last_error.argtypes = [vp]
     L.mac3d_last_error.restype = C.c_char_p
     L.mac3d_version.restype = C.c_char_p
@@ -329,6 +414,44 @@ class MacSolver:
         self._check(self.L.mac3d_adjoint(self.h, arr, len(blobs), dt, steps, checkpoint_every,
                                          t.ctypes.data, F64, C.byref(g)))
         return g.as_dict()
+
+    # -- drop releases
+    def release_map(self) -> ReleaseMap:
+        """mac3d_release_default_map for this solver's grid"""
+        return release_default_map(self.cfg)
+
+    def _target(self, target):
+        t = np.ascontiguousarray(target, dtype=np.float64)
+        if t.size != self.cfg.nx * self.cfg.ny * self.cfg.nz:
+            raise ValueError("target must have one value per cell")
+        return t
+
+    def adjoint_release(self, rmap: ReleaseMap, releases: Sequence, dt: float, steps: int,
+                        target, checkpoint_every: int = 10) -> dict:
+        """Gradient of 1/2 ||dye_N - target||^2 with respect to the release
+        parameters (x, z, radius, start_y, vel_y per release), from the
+        blobs the release map makes of them (mac3d_adjoint_release)."""
+        t = self._target(target)
+        g = ReleaseGrad()
+        self._check(self.L.mac3d_adjoint_release(self.h, C.byref(rmap), _releases(releases),
+                                                 len(releases), dt, steps, checkpoint_every,
+                                                 t.ctypes.data, F64, C.byref(g)))
+        return g.as_dict()
+
+    def fit_release(self, rmap: ReleaseMap, releases: Sequence, dt: float, steps: int, target,
+                    max_iters: int = 20, checkpoint_every: int = 10, mask: int = 0,
+                    tol: float = 1e-6, gtol: float = 1e-9):
+        """(fitted releases as dicts, report): quasi-Newton fit of the
+        releases to a target dye field from the given guess
+        (mac3d_fit_release)."""
+        t = self._target(target)
+        arr = _releases(releases)
+        o = FitOpts(max_iters, checkpoint_every, mask, tol, gtol)
+        rep = FitReport()
+        self._check(self.L.mac3d_fit_release(self.h, C.byref(rmap), arr, len(releases), dt, steps,
+                                             t.ctypes.data, F64, C.byref(o), C.byref(rep)))
+        out = [{n: getattr(arr[q], n) for n in RELEASE_PARAMS} for q in range(len(releases))]
+        return out, rep.as_dict()
 
     # -- checkpoints
     def checkpoint_save(self) -> np.ndarray:

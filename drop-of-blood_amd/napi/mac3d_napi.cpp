@@ -445,6 +445,206 @@ napi_value Adjoint(napi_env env, napi_callback_info info) {
   return o;
 }
 
+// ---- drop releases: {x, z, radius, startY, velY} objects, the release map
+// as {originX, originY, originZ, cell, surfaceY, gravity, velScale, submerge,
+// maxOffset, rMin, rMax}
+#define MAP_FIELDS(X) X(origin_x, "originX") X(origin_y, "originY") X(origin_z, "originZ") \
+  X(cell, "cell") X(surface_y, "surfaceY") X(gravity, "gravity") X(vel_scale, "velScale") \
+  X(submerge, "submerge") X(max_offset, "maxOffset") X(r_min, "rMin") X(r_max, "rMax")
+const char *kReleaseNames[5] = {"x", "z", "radius", "startY", "velY"};
+const char *kBlobNames[5] = {"cx", "cy", "cz", "radius", "w0"};
+
+napi_value map_to_js(napi_env env, const mac3d_release_map &m) {
+  napi_value o;
+  napi_create_object(env, &o);
+#define X(f, js) set_num(env, o, js, m.f);
+  MAP_FIELDS(X)
+#undef X
+  return o;
+}
+
+// every field must be given (a missing one is NaN, which the library rejects)
+void map_from_js(napi_env env, napi_value o, mac3d_release_map &m) {
+#define X(f, js) m.f = NAN; prop_num(env, o, js, &m.f);
+  MAP_FIELDS(X)
+#undef X
+}
+
+bool releases_from_js(napi_env env, napi_value arr, std::vector<mac3d_release> &rel) {
+  uint32_t n = 0;
+  if (napi_get_array_length(env, arr, &n) != napi_ok) {
+    napi_throw_type_error(env, nullptr, "releases must be an array");
+    return false;
+  }
+  rel.assign(n, mac3d_release{});
+  for (uint32_t q = 0; q < n; ++q) {
+    napi_value o;
+    napi_get_element(env, arr, q, &o);
+    double *f[5] = {&rel[q].x, &rel[q].z, &rel[q].radius, &rel[q].start_y, &rel[q].vel_y};
+    for (int b = 0; b < 5; ++b) {
+      *f[b] = NAN;
+      prop_num(env, o, kReleaseNames[b], f[b]);
+    }
+  }
+  return true;
+}
+
+napi_value releases_to_js(napi_env env, const mac3d_release *rel, size_t n) {
+  napi_value arr;
+  napi_create_array_with_length(env, n, &arr);
+  for (size_t q = 0; q < n; ++q) {
+    napi_value e;
+    napi_create_object(env, &e);
+    const double f[5] = {rel[q].x, rel[q].z, rel[q].radius, rel[q].start_y, rel[q].vel_y};
+    for (int b = 0; b < 5; ++b) set_num(env, e, kReleaseNames[b], f[b]);
+    napi_set_element(env, arr, (uint32_t)q, e);
+  }
+  return arr;
+}
+
+// a dye target: Float32Array or Float64Array
+bool target_arg(napi_env env, napi_value v, void **data, int *dtype) {
+  bool is_ta = false;
+  napi_is_typedarray(env, v, &is_ta);
+  napi_typedarray_type tt = napi_int8_array;
+  size_t len = 0, off = 0;
+  napi_value ab;
+  if (is_ta) napi_get_typedarray_info(env, v, &tt, &len, data, &ab, &off);
+  if (!is_ta || (tt != napi_float32_array && tt != napi_float64_array)) {
+    napi_throw_type_error(env, nullptr, "target must be a Float32Array or Float64Array");
+    return false;
+  }
+  *dtype = tt == napi_float32_array ? MAC3D_F32 : MAC3D_F64;
+  return true;
+}
+
+// releaseDefaultMap(config) -> map
+napi_value ReleaseDefaultMap(napi_env env, napi_callback_info info) {
+  napi_value argv[1];
+  if (!get_args(env, info, 1, argv)) return nullptr;
+  mac3d_config c;
+  mac3d_default_config(&c);
+  config_from_js(env, argv[0], c);
+  mac3d_release_map m;
+  int rc = mac3d_release_default_map(&c, &m);
+  if (rc) return throw_status(env, nullptr, rc);
+  return map_to_js(env, m);
+}
+
+// releaseBlobs(map, releases) -> [{cx, cy, cz, radius, w0}]
+napi_value ReleaseBlobs(napi_env env, napi_callback_info info) {
+  napi_value argv[2];
+  if (!get_args(env, info, 2, argv)) return nullptr;
+  mac3d_release_map m;
+  map_from_js(env, argv[0], m);
+  std::vector<mac3d_release> rel;
+  if (!releases_from_js(env, argv[1], rel)) return nullptr;
+  std::vector<mac3d_blob> blobs(rel.size() ? rel.size() : 1);
+  int rc = mac3d_release_blobs(&m, rel.data(), (int)rel.size(), blobs.data(), nullptr);
+  if (rc) return throw_status(env, nullptr, rc);
+  napi_value arr;
+  NAPI_CALL(env, napi_create_array_with_length(env, rel.size(), &arr));
+  for (size_t q = 0; q < rel.size(); ++q) {
+    napi_value e;
+    NAPI_CALL(env, napi_create_object(env, &e));
+    const double f[5] = {blobs[q].cx, blobs[q].cy, blobs[q].cz, blobs[q].radius, blobs[q].w0};
+    for (int b = 0; b < 5; ++b) set_num(env, e, kBlobNames[b], f[b]);
+    NAPI_CALL(env, napi_set_element(env, arr, (uint32_t)q, e));
+  }
+  return arr;
+}
+
+// adjointRelease(h, map, releases, dt, steps, checkpointEvery, target)
+//   -> {loss, dRhoB, dReleases: [{x, z, radius, startY, velY}], dBlobs: [...]}
+napi_value AdjointRelease(napi_env env, napi_callback_info info) {
+  napi_value argv[7];
+  if (!get_args(env, info, 7, argv)) return nullptr;
+  void *ctx = get_handle(env, argv[0]);
+  if (!ctx) return nullptr;
+  mac3d_release_map m;
+  map_from_js(env, argv[1], m);
+  std::vector<mac3d_release> rel;
+  if (!releases_from_js(env, argv[2], rel)) return nullptr;
+  void *data = nullptr;
+  int dtype = 0;
+  if (!target_arg(env, argv[6], &data, &dtype)) return nullptr;
+  mac3d_release_grad g;
+  std::memset(&g, 0, sizeof(g));
+  int rc = mac3d_adjoint_release(ctx, &m, rel.data(), (int)rel.size(), get_double(env, argv[3]),
+                                 get_int(env, argv[4]), get_int(env, argv[5]), data, dtype, &g);
+  if (rc) return throw_status(env, ctx, rc);
+  napi_value o, dr, db;
+  NAPI_CALL(env, napi_create_object(env, &o));
+  set_num(env, o, "loss", g.loss);
+  set_num(env, o, "dRhoB", g.d_rho_b);
+  NAPI_CALL(env, napi_create_array_with_length(env, g.n, &dr));
+  NAPI_CALL(env, napi_create_array_with_length(env, g.n, &db));
+  for (int q = 0; q < g.n; ++q) {
+    napi_value a, b;
+    NAPI_CALL(env, napi_create_object(env, &a));
+    NAPI_CALL(env, napi_create_object(env, &b));
+    for (int p = 0; p < 5; ++p) {
+      set_num(env, a, kReleaseNames[p], g.d_release[q][p]);
+      set_num(env, b, kBlobNames[p], g.d_blob[q][p]);
+    }
+    NAPI_CALL(env, napi_set_element(env, dr, q, a));
+    NAPI_CALL(env, napi_set_element(env, db, q, b));
+  }
+  NAPI_CALL(env, napi_set_named_property(env, o, "dReleases", dr));
+  NAPI_CALL(env, napi_set_named_property(env, o, "dBlobs", db));
+  return o;
+}
+
+// fitRelease(h, map, releases, dt, steps, target, {maxIters, checkpointEvery, mask,
+//   tol, gtol}) -> {releases, iters, evals, converged, loss0, loss}; mask is a
+//   Number (bits 5 q + b, q < 10)
+napi_value FitRelease(napi_env env, napi_callback_info info) {
+  napi_value argv[7];
+  if (!get_args(env, info, 7, argv)) return nullptr;
+  void *ctx = get_handle(env, argv[0]);
+  if (!ctx) return nullptr;
+  mac3d_release_map m;
+  map_from_js(env, argv[1], m);
+  std::vector<mac3d_release> rel;
+  if (!releases_from_js(env, argv[2], rel)) return nullptr;
+  void *data = nullptr;
+  int dtype = 0;
+  if (!target_arg(env, argv[5], &data, &dtype)) return nullptr;
+  mac3d_fit_opts o;
+  mac3d_fit_default_opts(&o);
+  napi_valuetype t;
+  NAPI_CALL(env, napi_typeof(env, argv[6], &t));
+  if (t == napi_object) {
+    double v;
+    if (prop_num(env, argv[6], "maxIters", &v)) o.max_iters = (int32_t)v;
+    if (prop_num(env, argv[6], "checkpointEvery", &v)) o.checkpoint_every = (int32_t)v;
+    if (prop_num(env, argv[6], "mask", &v)) {
+      if (!(v >= 0.0 && v < 9007199254740992.0) || v != std::floor(v)) {
+        napi_throw_range_error(env, nullptr, "mask must be a non-negative integer below 2^53");
+        return nullptr;
+      }
+      o.mask = (uint64_t)v;
+    }
+    if (prop_num(env, argv[6], "tol", &v)) o.tol = v;
+    if (prop_num(env, argv[6], "gtol", &v)) o.gtol = v;
+  }
+  mac3d_fit_report r;
+  std::memset(&r, 0, sizeof(r));
+  int rc = mac3d_fit_release(ctx, &m, rel.data(), (int)rel.size(), get_double(env, argv[3]),
+                             get_int(env, argv[4]), data, dtype, &o, &r);
+  if (rc) return throw_status(env, ctx, rc);
+  napi_value out;
+  NAPI_CALL(env, napi_create_object(env, &out));
+  NAPI_CALL(env, napi_set_named_property(env, out, "releases",
+                                         releases_to_js(env, rel.data(), rel.size())));
+  set_num(env, out, "iters", r.iters);
+  set_num(env, out, "evals", r.evals);
+  set_num(env, out, "converged", r.converged);
+  set_num(env, out, "loss0", r.loss0);
+  set_num(env, out, "loss", r.loss);
+  return out;
+}
+
 // concEnable(h, on)
 napi_value ConcEnable(napi_env env, napi_callback_info info) {
   napi_value argv[2];
@@ -549,6 +749,10 @@ napi_value Init(napi_env env, napi_value exports) {
       {"concEnable", nullptr, ConcEnable, nullptr, nullptr, nullptr, napi_default, nullptr},
       {"parcelsEmit", nullptr, ParcelsEmit, nullptr, nullptr, nullptr, napi_default, nullptr},
       {"checkpointLoad", nullptr, CheckpointLoad, nullptr, nullptr, nullptr, napi_default, nullptr},
+      {"releaseDefaultMap", nullptr, ReleaseDefaultMap, nullptr, nullptr, nullptr, napi_default, nullptr},
+      {"releaseBlobs", nullptr, ReleaseBlobs, nullptr, nullptr, nullptr, napi_default, nullptr},
+      {"adjointRelease", nullptr, AdjointRelease, nullptr, nullptr, nullptr, napi_default, nullptr},
+      {"fitRelease", nullptr, FitRelease, nullptr, nullptr, nullptr, napi_default, nullptr},
       {"version", nullptr, Version, nullptr, nullptr, nullptr, napi_default, nullptr},
   };
   napi_define_properties(env, exports, sizeof(props) / sizeof(props[0]), props);

@@ -241,6 +241,92 @@ int mac3d_checkpoint_load(void *ctx, const void *host, size_t bytes);
 int mac3d_adjoint(void *ctx, const mac3d_blob *blobs, int nblobs, double dt, int nsteps,
                   int checkpoint_every, const void *target, int dtype, mac3d_grad *out);
 
+/* Drop releases.  The reference lets a drop go with
+ * MultiDropSystem.spawnAerialDrop(offsetXZ, radius, seed, startY, velY)
+ * (src/sim/dropKinematics.ts:354-391): the offset is clamped to the disk of
+ * MAX_DROP_OFFSET_XZ, the radius to [PENDANT_MIN_RADIUS, 0.095], and the
+ * aerial branch of MultiDropSystem.step (:651-707) lets it fall freely under
+ * G_SLOW, solving each substep's contact time exactly, until its lowest point
+ * reaches WATER_SURFACE_Y.  The release map below takes that contact state in
+ * closed form and makes it a blob of the grid; the seed only shapes the
+ * drop's oscillation in the reference, so it has no part here. */
+typedef struct mac3d_release {
+  double x, z;    /* offsetXZ (scene m) */
+  double radius;  /* drop radius (scene m) */
+  double start_y; /* height of the centre at release (scene m; Y_START 0.495) */
+  double vel_y;   /* vertical velocity at release (scene m/s; V_START -0.075) */
+} mac3d_release;
+
+typedef struct mac3d_release_map {
+  double origin_x, origin_y, origin_z; /* scene position (m) of the grid's corner (0, 0, 0) */
+  double cell;       /* scene metres per grid cell */
+  double surface_y;  /* WATER_SURFACE_Y (0.30) */
+  double gravity;    /* G_SLOW (-0.155 m/s^2) */
+  double vel_scale;  /* blob w0 (m/s) per scene m/s of downward contact speed */
+  double submerge;   /* the blob's centre sits this many radii under the surface */
+  double max_offset; /* MAX_DROP_OFFSET_XZ (0.26) */
+  double r_min, r_max; /* radius clamp (PENDANT_MIN_RADIUS 0.018, 0.095) */
+} mac3d_release_map;
+
+/* dL/d(release parameters), chained from the blob gradient */
+typedef struct mac3d_release_grad {
+  double loss;
+  double d_rho_b;
+  int32_t n;
+  int32_t steps;
+  double d_release[MAC3D_MAX_BLOBS][5]; /* dL/d (x, z, radius, start_y, vel_y) */
+  double d_blob[MAC3D_MAX_BLOBS][5];    /* dL/d (cx, cy, cz, radius, w0) it came from */
+} mac3d_release_grad;
+
+/* The reference tank's width (2 x DOMAIN_HALF_XZ = 0.96 m, uniformLayout.ts:9-11) on
+ * nx cells, centred in x and z, the grid's top face at the surface, the
+ * reference's constants, vel_scale 1, submerge 1 (the blob's top at the
+ * surface). */
+int mac3d_release_default_map(const mac3d_config *cfg, mac3d_release_map *map);
+/* Blobs of n releases (host arithmetic only: no context, no device).  jac, if
+ * not NULL, receives n 5x5 Jacobians, d blob[q].(cx, cy, cz, radius, w0)[a] /
+ * d release[q].(x, z, radius, start_y, vel_y)[b] at jac[25 q + 5 a + b]
+ * (zero across a clamp that binds). */
+int mac3d_release_blobs(const mac3d_release_map *map, const mac3d_release *rel, int n,
+                        mac3d_blob *blobs, double *jac);
+/* mac3d_adjoint from the releases' blobs, with the gradient taken on to the
+ * release parameters through the map's Jacobian */
+int mac3d_adjoint_release(void *ctx, const mac3d_release_map *map, const mac3d_release *rel,
+                          int n, double dt, int nsteps, int checkpoint_every,
+                          const void *target, int dtype, mac3d_release_grad *out);
+
+/* Fitting releases to a target dye field: quasi-Newton (BFGS with a
+ * backtracking line search) on the release parameters, each evaluation one
+ * mac3d_adjoint_release.  Parameters are scaled so that one unit moves the
+ * blob by about a cell (x, z and radius by one cell; start_y and vel_y by
+ * the change of contact speed that moves the blob a cell over the run). */
+typedef struct mac3d_fit_opts {
+  int32_t max_iters;        /* quasi-Newton iterations (20) */
+  int32_t checkpoint_every; /* of each adjoint (10) */
+  uint64_t mask;            /* bit 5 q + b fits parameter b of release q; 0: x, z, radius,
+                               vel_y of every release (start_y and vel_y reach the dye only
+                               through the contact speed, so one of them is held) */
+  double tol;               /* stop when an iteration lowers the loss by less than
+                               tol of itself (1e-6) */
+  double gtol;              /* or when every scaled gradient component is below
+                               gtol times the loss (1e-9) */
+} mac3d_fit_opts;
+
+typedef struct mac3d_fit_report {
+  int32_t iters;     /* quasi-Newton iterations taken */
+  int32_t evals;     /* adjoint evaluations */
+  int32_t converged; /* 1: stopped by tol or gtol, 0: by max_iters or a failed line search */
+  int32_t reserved;
+  double loss0, loss; /* at the guess and at the result */
+} mac3d_fit_report;
+
+int mac3d_fit_default_opts(mac3d_fit_opts *opts);
+/* rel: the guess in, the fit out.  Leaves the context at an unspecified
+ * state of the last evaluation's run. */
+int mac3d_fit_release(void *ctx, const mac3d_release_map *map, mac3d_release *rel, int n,
+                      double dt, int nsteps, const void *target, int dtype,
+                      const mac3d_fit_opts *opts, mac3d_fit_report *report);
+
 /* Free-surface coupling.  The wave solver of the
  * reference scene -- a gravity-capillary spectral propagator on an n x n grid
  * with drop impulses, surfactant film, Marangoni pull and the MAC top layer's

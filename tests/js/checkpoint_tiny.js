@@ -1,6 +1,6 @@
 // @ts-check
 'use strict';
-// Checkpoint round trip through the TypeScript-checked host: the tiny
+// Checkpoint round trip through the JavaScript host: the tiny
 // workload (mixed precision when argv[3] is "mixed") steps 3 times, saves,
 // steps 2 more, loads and steps the same 2 again.  Prints whether the two
 // results are identical and writes the checkpoint and the final fields to

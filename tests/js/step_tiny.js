@@ -1,6 +1,6 @@
 // @ts-check
 'use strict';
-// Runs the tiny workload through the TypeScript-checked host (MacSolver over
+// Runs the tiny workload through the JavaScript host (MacSolver over
 // the N-API addon) and prints the fields' checksums and step stats as JSON.
 const path = require('path');
 const root = path.join(__dirname, '..', '..', 'drop-of-blood_amd', 'js');

@@ -1,4 +1,4 @@
-"""The TypeScript-checked host (js/MacSolver.js over the N-API addon) steps
+"""The JavaScript host (js/MacSolver.js over the N-API addon) steps
 the tiny workload on the GPU and agrees with the FP64 oracle."""
 import json
 import os
@@ -134,3 +134,49 @@ def test_node_checkpoint_round_trip(mode, tmp_path):
         mine = np.concatenate([s.download(f, dtype=np.float32).ravel()
                                for f in ("U", "V", "W", "P", "DYE")])
     assert np.array_equal(mine, js_fields)
+
+
+@pytest.mark.skipif(shutil.which("node") is None, reason="node not installed")
+def test_node_release_gradient_and_fit_match_c_abi():
+    """releaseMap / releaseBlobs / backwardRelease / fitRelease through the
+    N-API host against the same calls through the C ABI: the map and blobs
+    bit for bit, the gradient to 1e-10 (the adjoint's scatters are atomics,
+    so the last bits depend on the order), two fit iterations to 1e-8; a
+    release with missing fields is refused."""
+    from mac3d import MacSolver, lib
+    steps, iters = 6, 2
+    r = subprocess.run(["node", os.path.join(ROOT, "tests", "js", "release_tiny.js"), str(steps),
+                        str(iters)], capture_output=True, text=True, timeout=300)
+    assert r.returncode == 0, r.stderr
+    js = json.loads(r.stdout.strip().splitlines()[-1])
+    assert js["bad"] is not None and "status 1" in js["bad"]
+    w = scenarios.Workload("release", 32, 48, 32, 1e-3, 5e-4, steps, 3, 8, 2)
+    from mac3d.lib import config_for
+    cfg = config_for(w)
+    m = lib.release_default_map(cfg)
+    m.vel_scale = 2.5
+    names = dict(origin_x="originX", origin_y="originY", origin_z="originZ", cell="cell",
+                 surface_y="surfaceY", gravity="gravity", vel_scale="velScale",
+                 submerge="submerge", max_offset="maxOffset", r_min="rMin", r_max="rMax")
+    for c, j in names.items():
+        assert getattr(m, c) == js["map"][j], c
+    truth = dict(x=0.05, z=-0.03, radius=0.09, start_y=0.495, vel_y=-0.075)
+    guess = dict(x=0.08, z=-0.05, radius=0.1, start_y=0.495, vel_y=-0.15)
+    (bg,), _ = lib.release_blobs(m, [guess])
+    for n in lib.BLOB_PARAMS:
+        assert getattr(bg, n) == js["blobs"][0][n], n
+    blobs, _ = lib.release_blobs(m, [truth])
+    with MacSolver(cfg) as s:
+        s.init_blobs(blobs)
+        s.step(w.dt, steps)
+        target = s.download("DYE")
+        g = s.adjoint_release(m, [guess], w.dt, steps, target, checkpoint_every=3)
+        fit, rep = s.fit_release(m, [guess], w.dt, steps, target, max_iters=iters,
+                                 checkpoint_every=3)
+    jsn = dict(x="x", z="z", radius="radius", start_y="startY", vel_y="velY")
+    assert js["grad"]["loss"] == pytest.approx(g["loss"], rel=1e-12)
+    for b, n in enumerate(lib.RELEASE_PARAMS):
+        assert js["grad"]["dReleases"][0][jsn[n]] == pytest.approx(
+            g["d_release"][0][b], rel=1e-10, abs=1e-10 * max(map(abs, g["d_release"][0]))), n
+        assert js["fit"]["releases"][0][jsn[n]] == pytest.approx(fit[0][n], rel=1e-8, abs=1e-12), n
+    assert js["fit"]["iters"] == rep["iters"] and js["fit"]["loss"] == pytest.approx(rep["loss"], rel=1e-8)
