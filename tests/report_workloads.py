@@ -11,6 +11,9 @@
       256x384x256, 5-level V-cycle: FP32 and mixed steps/s and max-div, then
       the adjoint over the run with sqrt(N) checkpointing, and a central
       difference in rho_b (h = 1e-2) against it.
+  python tests/report_workloads.py rho_b [--steps 200]
+      the large case's dL/d rho_b (FP64) against central differences at
+      h = 1e-1, 1e-2, 1e-3, with the MacCormack correction on and off.
 
 Prints one JSON object per line.
 """
@@ -122,14 +125,49 @@ def large(steps):
          rel_err_rho_b=abs(g["d_rho_b"] - fd) / abs(fd), d_blob0=g["d_blob"][0])
 
 
+def rho_b_differences(steps):
+    """dL/d rho_b of the large case (FP64, sqrt(N) checkpoints) against
+    central differences at h = 1e-1, 1e-2 and 1e-3 kg/m^3, with the
+    MacCormack correction on and off."""
+    w = scenarios.scaled(scenarios.workload("large", 1), precision=2, steps=steps)
+    K = max(1, int(round(math.sqrt(steps))))
+    tb = copy.deepcopy(w.blobs)
+    tb[0].cx += 3.0
+    tb[0].w0 += 0.1
+    for mc in (None, 0.0):
+        kw = {} if mc is None else {"maccormack": mc}
+        with MacSolver(config_for(w, **kw)) as s:
+            s.init_blobs(tb)
+            s.step_nostats(w.dt, steps)
+            target = s.download("DYE")
+        with MacSolver(config_for(w, **kw)) as s:
+            g = s.adjoint(w.blobs, w.dt, steps, target, checkpoint_every=K)
+
+        def loss(rho_b):
+            with MacSolver(config_for(w, rho_b=rho_b, **kw)) as s:
+                s.init_blobs(w.blobs)
+                s.step_nostats(w.dt, steps)
+                d = s.download("DYE")
+            return 0.5 * float(np.sum((d.astype(np.float64) - target) ** 2))
+        l0 = loss(1060.0)
+        for h in (1e-1, 1e-2, 1e-3):
+            lp, lm = loss(1060.0 + h), loss(1060.0 - h)
+            fd = (lp - lm) / (2 * h)
+            emit(workload="large", steps=steps, maccormack="off" if mc == 0.0 else "on", h=h,
+                 d_rho_b=g["d_rho_b"], fd_rho_b=fd, rel_err=abs(g["d_rho_b"] - fd) / abs(fd),
+                 curvature=(lp - 2 * l0 + lm) / (h * h), loss=l0, adjoint_loss=g["loss"])
+
+
 def main():
     ap = argparse.ArgumentParser()
-    ap.add_argument("what", choices=["accuracy", "large"])
+    ap.add_argument("what", choices=["accuracy", "large", "rho_b"])
     ap.add_argument("--steps", type=int, default=200)
     ap.add_argument("--every", type=int, default=20)
     a = ap.parse_args()
     if a.what == "accuracy":
         accuracy(a.steps, a.every)
+    elif a.what == "rho_b":
+        rho_b_differences(a.steps)
     else:
         large(a.steps)
 
