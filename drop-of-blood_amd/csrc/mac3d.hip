@@ -1236,6 +1236,12 @@ int one_step(Ctx *ctx, double dt, int *cycles, int *iters, double *rel, int *con
     while ((q = hipEventQuery(ctx->norms_ready)) == hipErrorNotReady) {
     }
     if (q != hipSuccess) return fail(ctx, MAC3D_EHIP, std::string("norms: ") + hipGetErrorString(q));
+    {  // non-finite norms: the state went NaN or inf; stop refining it
+      const double *h = ctx->red_host + (first ? 3 * pre_it : 0);
+      if (!std::isfinite(h[0]) || !std::isfinite(h[1]) || !std::isfinite(h[2]))
+        return fail(ctx, MAC3D_ENONFINITE,
+                    "the pressure solve's residual norms are not finite (NaN or inf in the state)");
+    }
     if (first) {
       for (int j = 0; j <= pre_it && needed < 0; ++j)
         if (converged(ctx->red_host + 3 * j, nullptr)) needed = j;
@@ -2338,6 +2344,8 @@ int mac3d_step(void *h, double dt, int nsteps, mac3d_stats *stats) {
     stats->max_div = mx;
     stats->l2_div = l2;
     stats->max_div_rate = mx / ctx->cfg.dx;
+    if (!std::isfinite(mx) || !std::isfinite(l2))
+      return fail(ctx, MAC3D_ENONFINITE, "the divergence norms are not finite (NaN or inf in the velocity)");
   }
   return MAC3D_OK;
 }

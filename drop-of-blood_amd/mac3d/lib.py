@@ -21,7 +21,7 @@ MG = dict(SMOOTH=0, RESIDUAL=1, RESTRICT=2, PROLONG=3, COARSE=4, ZERO=5, SMOOTH_
 KC = dict(FACE_TO_CENTRE=0, ADVECT=1, CASSON=2, DIVERGENCE=3, SMOOTH_L0=4, RESTRICT_L0=5,
           PROLONG_L0=6, MG_COARSE_LEVELS=7, COARSE_SOLVE=8, PROJECT=9, DYE=10, REFINE64=11,
           SMOOTH2_L0=12, WAVE=13, PARCELS=14)
-STATUS = {0: "OK", 1: "EINVAL", 2: "EHIP", 3: "ENOMEM", 4: "ESTATE"}
+STATUS = {0: "OK", 1: "EINVAL", 2: "EHIP", 3: "ENOMEM", 4: "ESTATE", 5: "ENONFINITE"}
 
 
 class Config(C.Structure):

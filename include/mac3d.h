@@ -38,6 +38,10 @@ extern "C" {
 #define MAC3D_EHIP 2     /* a HIP runtime call failed */
 #define MAC3D_ENOMEM 3   /* device allocation failed */
 #define MAC3D_ESTATE 4   /* call not valid in the context's current state */
+#define MAC3D_ENONFINITE 5 /* the state went non-finite (NaN or inf): seen by the
+                              divergence norms of mac3d_step's stats, or by the mixed
+                              mode's residual norms; the state is left as computed, for
+                              the caller to restore from a checkpoint */
 
 /* precision modes of the step */
 #define MAC3D_PREC_FP32 0  /* plain FP32: fixed V-cycle count per step */
@@ -203,7 +207,11 @@ int mac3d_init_blobs(void *ctx, const mac3d_blob *blobs, int nblobs);
 /* replaces: encodePhysicsStep's MAC portion (BloodDropEngine.ts:1916-2069)
  * plus CONC_PREDICT/CONC_RESOLVE's dye core (advect.ts:28-72,196-222),
  * nsteps times at fixed dt, blocking until the stream completes.
- * dt must be finite and > 0 (EINVAL otherwise, state untouched). */
+ * dt must be finite and > 0 (EINVAL otherwise, state untouched).  With
+ * stats, a non-finite divergence norm after the last step returns
+ * MAC3D_ENONFINITE (stats still filled); the mixed mode returns it as soon
+ * as a residual norm is non-finite, with or without stats.  (The reference
+ * clamps dt and keeps stepping, dropKinematics.ts:605, fluid.ts:182.) */
 int mac3d_step(void *ctx, double dt, int nsteps, mac3d_stats *stats);
 
 /* Checkpoints of the evolving state, in a byte buffer the caller owns.

@@ -13,7 +13,9 @@ import subprocess
 import numpy as np
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-LIB = os.path.join(HERE, "liborc.so")
+# MAC3D_ORACLE_LIB: another build of the same oracle (the sanitizer build of
+# tests/test_sanitizers.py); by default the one beside this file
+LIB = os.environ.get("MAC3D_ORACLE_LIB") or os.path.join(HERE, "liborc.so")
 
 (F_U, F_V, F_W, F_P, F_DYE, F_CU, F_CV, F_CW, F_RHS, F_MGX, F_MGB, F_MGR, F_DYE_PRED, F_WAVE,
  F_PTOP, F_CURL, F_CONC) = range(17)
