@@ -9,8 +9,9 @@ fields resident in HBM.  The same line carries the mixed-precision run
 speed-up, the smoother's roofline figures (HIP events on the solver's
 stream), the FP32 step with the free-surface wave coupling on (a 512 x 512
 wave grid, two active drops), the same with the reference scene's advection
-and Casson terms, and the FP64 CPU oracle timed on this host as the CPU
-baseline.
+and Casson terms, and the CPU baseline timed on this host: the FP64
+JavaScript transcription of the step (Node worker threads) and the FP64 C
+oracle (all threads and one).
 
 python bench.py [--gpus 1] [--steps K] [--warmup W]
 """
@@ -19,6 +20,8 @@ from __future__ import annotations
 import argparse
 import json
 import os
+import shutil
+import subprocess
 import sys
 import time
 
@@ -168,6 +171,26 @@ def cpu_baseline(workload, steps, threads):
     return dt / steps, used
 
 
+def js_baseline(name, steps, threads):
+    """The FP64 JavaScript transcription of the step (oracle/js), the CPU path
+    in the reference's language family, on `threads` Node worker threads:
+    `steps` steps of the workload, the last one's time (the first includes
+    the compiler's warm-up).  None when node is missing or the run fails."""
+    node = shutil.which("node")
+    if node is None:
+        return None
+    try:
+        r = subprocess.run([node, "--max-old-space-size=8192",
+                            os.path.join(ROOT, "oracle", "js", "parallel.js"), name, str(steps),
+                            str(threads)], capture_output=True, text=True, timeout=600)
+    except subprocess.TimeoutExpired:
+        return None
+    if r.returncode != 0:
+        log(f"[bench] js baseline failed: {r.stderr[-400:]}")
+        return None
+    return json.loads(r.stdout.strip().splitlines()[-1])
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -281,15 +304,37 @@ def main():
         log(f"[bench] cpu baseline: FP64 oracle, 2 steps on {threads} threads, 1 on 1 thread")
         s_all, used = cpu_baseline(w32, 2, threads)
         s_one, _ = cpu_baseline(w32, 1, 1)
-        cpu = {"value": round(1.0 / s_all, 5), "unit": "steps/s", "cores": used, "kind": "port",
-               "cpu_model": cpu_model(), "host_threads_visible": os.cpu_count(),
-               "seconds_per_step_all_threads": round(s_all, 3),
-               "seconds_per_step_single_thread": round(s_one, 3),
-               "single_thread_steps_per_s": round(1.0 / s_one, 5),
-               "sample": f"the 192x256x192 FP32 workload's step (8 V-cycles) in the FP64 C "
-                         f"oracle after one untimed step: 2 steps on {used} OpenMP threads "
-                         f"({2 * s_all:.1f} s) and 1 step on 1 thread ({s_one:.1f} s)"}
-        log(f"[bench] cpu {s_all:.2f} s/step on {used} threads, {s_one:.2f} s/step on 1")
+        c_port = {"kind": "port", "language": "C (OpenMP), FP64",
+                  "seconds_per_step_all_threads": round(s_all, 3), "threads": used,
+                  "seconds_per_step_single_thread": round(s_one, 3),
+                  "single_thread_steps_per_s": round(1.0 / s_one, 5),
+                  "sample": f"2 steps on {used} OpenMP threads ({2 * s_all:.1f} s) and 1 step on "
+                            f"1 thread ({s_one:.1f} s), after one untimed step"}
+        log(f"[bench] cpu (C) {s_all:.2f} s/step on {used} threads, {s_one:.2f} s/step on 1")
+        js = js_baseline("ref", 2, threads)
+        if js is not None:
+            log(f"[bench] cpu (JavaScript) {js['seconds_per_step']:.2f} s/step on "
+                f"{js['threads']} worker threads")
+            cpu = {"value": round(1.0 / js["seconds_per_step"], 5), "unit": "steps/s",
+                   "cores": js["threads"], "kind": "port", "cpu_model": cpu_model(),
+                   "host_threads_visible": os.cpu_count(),
+                   "language": f"JavaScript (Node {js['node']}, worker_threads), FP64",
+                   "seconds_per_step_all_threads": round(js["seconds_per_step"], 3),
+                   "seconds_per_step_first": round(js["per_step"][0], 3),
+                   "sample": "the 192x256x192 FP32 workload's step (8 V-cycles) in the FP64 "
+                             "JavaScript transcription (oracle/js/parallel.js): 2 steps on "
+                             f"{js['threads']} worker threads, the second timed "
+                             f"({js['seconds_per_step']:.1f} s; the first, with the compiler's "
+                             f"warm-up, {js['per_step'][0]:.1f} s)",
+                   "c_oracle": c_port}
+        else:
+            cpu = {"value": round(1.0 / s_all, 5), "unit": "steps/s", "cores": used,
+                   "kind": "port", "cpu_model": cpu_model(),
+                   "host_threads_visible": os.cpu_count(), "c_oracle": c_port,
+                   "seconds_per_step_all_threads": round(s_all, 3),
+                   "seconds_per_step_single_thread": round(s_one, 3),
+                   "sample": "the 192x256x192 FP32 workload's step (8 V-cycles) in the FP64 C "
+                             "oracle (node is not installed): " + c_port["sample"]}
 
     line = {
         "metric": "MAC steps/sec @192x256x192 (mixed vs FP32, div<=1e-6); HBM GB/s % peak",
