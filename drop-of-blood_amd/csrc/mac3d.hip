@@ -2546,14 +2546,15 @@ int mac3d_fit_release(void *h, const mac3d_release_map *map, mac3d_release *rel,
   if (opts) o = *opts;
   if (o.max_iters < 0 || o.checkpoint_every < 1 || !(o.tol >= 0.0) || !(o.gtol >= 0.0))
     return fail(ctx, MAC3D_EINVAL, "fit options: max_iters >= 0, checkpoint_every >= 1, tol and gtol >= 0");
-  if (n < 1 || n > MAC3D_MAX_BLOBS || !rel) return fail(ctx, MAC3D_EINVAL, "need 1..MAC3D_MAX_BLOBS releases");
+  // the mask has five bits per release: at most 12 releases in 64 bits
+  if (n < 1 || n > 12 || !rel) return fail(ctx, MAC3D_EINVAL, "need 1..12 releases to fit");
   if (!std::isfinite(dt) || !(dt > 0.0) || nsteps < 1)
     return fail(ctx, MAC3D_EINVAL, "dt must be finite and > 0, nsteps >= 1");
   uint64_t mask = o.mask;
   if (!mask)
     for (int q = 0; q < n; ++q)
       mask |= (uint64_t)0x17 << (5 * q);  // x, z, radius, vel_y
-  if (5 * n < 64 && (mask >> (5 * n)))
+  if (mask >> (5 * n))
     return fail(ctx, MAC3D_EINVAL, "fit mask names parameters of releases past n");
   // the fitted parameters and their scales (about one cell of blob motion)
   mac3d_blob b0[MAC3D_MAX_BLOBS];

@@ -329,8 +329,8 @@ typedef struct mac3d_fit_report {
 } mac3d_fit_report;
 
 int mac3d_fit_default_opts(mac3d_fit_opts *opts);
-/* rel: the guess in, the fit out.  Leaves the context at an unspecified
- * state of the last evaluation's run. */
+/* rel: the guess in, the fit out (1..12 releases: five mask bits each).
+ * Leaves the context at an unspecified state of the last evaluation's run. */
 int mac3d_fit_release(void *ctx, const mac3d_release_map *map, mac3d_release *rel, int n,
                       double dt, int nsteps, const void *target, int dtype,
                       const mac3d_fit_opts *opts, mac3d_fit_report *report);

@@ -154,6 +154,8 @@ def test_fit_and_release_errors():
             s.adjoint_release(m, [dict(TRUE, radius=float("inf"))], w.dt, w.steps, target)
         with pytest.raises(lib.Mac3dError):
             s.fit_release(m, [TRUE], -w.dt, w.steps, target)
+        with pytest.raises(lib.Mac3dError):  # five mask bits per release: at most 12
+            s.fit_release(m, [TRUE] * 13, w.dt, w.steps, target)
         # nothing to move: a zero-iteration fit returns the guess and its loss
         fit, rep = s.fit_release(m, [TRUE], w.dt, w.steps, target, max_iters=0)
         assert fit[0] == TRUE and rep["iters"] == 0 and rep["evals"] == 1
