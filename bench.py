@@ -143,6 +143,35 @@ def copy_probe(workload, times=5):
             s.download_into("U", out)
 
 
+def smoother_cache_probe(workload, reps=20):
+    """The level-0 two-sweep pass with the device cache warm and cold: the
+    pass re-run on the same x and b (both fit the 256 MB Infinity Cache), and
+    the same with a 256x384x256 face->centre pass of another context (about
+    600 MB read and written) between launches, which evicts them.  Average
+    launch microseconds of each, from the library's HIP events."""
+    import numpy as np
+    from mac3d import MacSolver, scenarios
+    from mac3d.lib import config_for
+    rng = np.random.default_rng(3)
+    big = scenarios.scaled(scenarios.workload("large", 1), precision=0)
+    out = {}
+    with MacSolver(config_for(workload)) as s, MacSolver(config_for(big)) as flusher:
+        shp = s.level_shape(0)
+        s.upload("MGX", rng.standard_normal(shp).astype(np.float32), 0)
+        s.upload("MGB", rng.standard_normal(shp).astype(np.float32), 0)
+        for mode in ("warm", "cold"):
+            s.mg("SMOOTH", 0, 2)  # untimed
+            s.profile(True)
+            for _ in range(reps):
+                if mode == "cold":
+                    flusher.stage("FACE_TO_CENTRE")
+                s.mg("SMOOTH", 0, 2)
+            ms, n = s.profile_read("SMOOTH2_L0")
+            s.profile(False)
+            out[mode] = 1e3 * ms / max(n, 1)
+    return out
+
+
 def cpu_model():
     try:
         with open("/proc/cpuinfo") as f:
@@ -200,6 +229,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-mixed", action="store_true")
     ap.add_argument("--no-wave", action="store_true")
+    ap.add_argument("--no-cold", action="store_true",
+                    help="skip the smoothing pass's warm/cold cache probe")
     ap.add_argument("--copy-probe", action="store_true",
                     help="after the run, 5 device copies of one FP32 field (a plain copy "
                          "kernel of known size, to calibrate HBM counters)")
@@ -293,6 +324,17 @@ def main():
             if roof["traffic"]:
                 roof["traffic_gbps"] = round(roof["traffic"] / avg_s / 1e9, 1)
                 roof["traffic_frac"] = round(roof["traffic"] / avg_s / 1e9 / HBM_PEAK_GBPS, 4)
+    if roof is not None and not args.no_cold:
+        cc = smoother_cache_probe(w32)
+        kb = roof["bytes_per_launch"]
+        roof["cache_probe"] = {
+            "warm_avg_launch_us": round(cc["warm"], 2), "cold_avg_launch_us": round(cc["cold"], 2),
+            "cold_achieved": round(kb / (cc["cold"] * 1e-6) / 1e9, 1),
+            "cold_frac": round(kb / (cc["cold"] * 1e-6) / 1e9 / HBM_PEAK_GBPS, 4),
+            "how": "20 launches on the same x and b (warm), and 20 each after a 256x384x256 "
+                   "face->centre pass of another context (~600 MB) evicts the Infinity Cache "
+                   "(cold)"}
+        log(f"[bench] level-0 smoothing pass: warm {cc['warm']:.1f} us, cold {cc['cold']:.1f} us")
     kernel_ms = {k: round(v[0] / v[1], 4) for k, v in prof.items()}
     if args.copy_probe:
         copy_probe(w32)
