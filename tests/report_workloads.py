@@ -11,6 +11,10 @@
       256x384x256, 5-level V-cycle: FP32 and mixed steps/s and max-div, then
       the adjoint over the run with sqrt(N) checkpointing, and a central
       difference in rho_b (h = 1e-2) against it.
+  python tests/report_workloads.py shard [--steps 40]
+      one GPU's z-slab of the 192x256x192 workload split over 8 GPUs, at its
+      real shape with the smoothing pass's halo (192x256x32): the FP32 step,
+      its kernel classes, and the halo bytes per step.
   python tests/report_workloads.py rho_b [--steps 200]
       the large case's dL/d rho_b (FP64) against central differences at
       h = 1e-1, 1e-2, 1e-3, with the MacCormack correction on and off.
@@ -158,9 +162,61 @@ def rho_b_differences(steps):
                  curvature=(lp - 2 * l0 + lm) / (h * h), loss=l0, adjoint_loss=g["loss"])
 
 
+def shard(steps):
+    """One GPU's share of the 192x256x192 workload split in z-slabs over 8
+    GPUs, at its real shape: 24 planes plus the 4-plane halo the two-sweep
+    pass reads on each side (192x256x32, 3 levels).  The FP32 step and its
+    kernel classes timed there, and the halo bytes each GPU would exchange
+    per step."""
+    w = scenarios.workload("ref", 1)
+    sw = scenarios.scaled(w, nz=32, blobs=[scenarios.Blob(96.0, 200.0, 16.0, 7.0, 0.5)])
+    with MacSolver(config_for(sw)) as s:
+        s.init_blobs(sw.blobs)
+        s.step(sw.dt, 5)
+        t0 = time.perf_counter()
+        st = s.step(sw.dt, steps)
+        sec = time.perf_counter() - t0
+        s.profile(True)
+        s.step(sw.dt, 3)
+        cls = {}
+        for k in ("ADVECT", "CASSON", "FACE_TO_CENTRE", "DIVERGENCE", "PROJECT", "DYE",
+                  "SMOOTH2_L0", "RESTRICT_L0", "PROLONG_L0", "MG_COARSE_LEVELS", "COARSE_SOLVE"):
+            ms, n = s.profile_read(k)
+            cls[k] = {"launches_per_step": n / 3, "avg_us": round(1e3 * ms / max(n, 1), 2),
+                      "us_per_step": round(1e3 * ms / 3, 1)}
+        s.profile(False)
+    plane = sw.nx * sw.ny * 4  # one level-0 FP32 plane: 196,608 B
+    c = w.cycles
+    # planes of level-0 size each GPU sends to one z-neighbour per step (a
+    # level-1 plane is a quarter of one); level 2 (147k cells) and 3 are
+    # gathered onto every GPU and solved redundantly
+    halo_planes = {
+        "advection: u, v, w and the centre velocity's 3 components, 2 planes each": 2 * 6,
+        "Casson: u, v, w and the dye, 1 plane each": 4,
+        "divergence: w, 1 plane": 1,
+        "right-hand side for the two-sweep pass's b ring, 3 planes once per step": 3,
+        "level-0 two-sweep passes: x, 4 planes, 2 per cycle": c * 2 * 4,
+        "level-0 residual-restriction: x, 1 plane per cycle": c * 1,
+        "level-1 two-sweep passes and residual-restriction: 9 quarter planes per cycle": c * 9 / 4,
+        "level-1 correction for the level-0 prolongation: 1 quarter plane per cycle": c / 4,
+        "level-2 right-hand side gathered (147,456 cells, 3 planes' bytes) per cycle": c * 3,
+        "projection: p, 1 plane": 1,
+        "face->centre (twice): w, 1 plane": 2,
+        "dye predict: centre velocity (3) and dye, 2 planes each": 8,
+        "dye resolve: predictor 2 planes and dye 1": 3,
+    }
+    halo = plane * sum(halo_planes.values())
+    emit(workload="shard", shape=[sw.nx, sw.ny, sw.nz],
+         of="192x256x192 over 8 GPUs (24 planes + 2 x 4 halo)",
+         steps_per_s=steps / sec, ms_per_step=1e3 * sec / steps, max_div=st["max_div"],
+         kernel_classes=cls, halo_planes_per_step_per_side=halo_planes,
+         halo_bytes_per_step_per_side=halo,
+         halo_us_per_step_at_153GBps_per_link=round(halo / 153e9 * 1e6, 1))
+
+
 def main():
     ap = argparse.ArgumentParser()
-    ap.add_argument("what", choices=["accuracy", "large", "rho_b"])
+    ap.add_argument("what", choices=["accuracy", "large", "rho_b", "shard"])
     ap.add_argument("--steps", type=int, default=200)
     ap.add_argument("--every", type=int, default=20)
     a = ap.parse_args()
@@ -168,6 +224,8 @@ def main():
         accuracy(a.steps, a.every)
     elif a.what == "rho_b":
         rho_b_differences(a.steps)
+    elif a.what == "shard":
+        shard(a.steps)
     else:
         large(a.steps)
 
