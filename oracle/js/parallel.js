@@ -125,9 +125,11 @@ if (!isMainThread) {
     }
     // per step: the slowest worker's time of each step (the first one
     // includes the compiler's warm-up)
+    const cpus = require('os').cpus();
     console.log(JSON.stringify({ threads, steps, seconds, per_step: per,
       seconds_per_step: per[per.length - 1], maxDiv: oracle.divNorms().maxDiv,
-      node: process.version }));
+      node: process.version, cpu_model: cpus.length ? cpus[0].model : 'unknown',
+      cpus: cpus.length }));
   });
 }
 
