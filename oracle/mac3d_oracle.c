@@ -27,7 +27,7 @@
 #define MAXLEV 8
 
 enum { F_U, F_V, F_W, F_P, F_DYE, F_CU, F_CV, F_CW, F_RHS, F_MGX, F_MGB,
-       F_MGR, F_DYE_PRED, F_WAVE, F_PTOP, F_CURL, F_CONC, F_COUNT };
+       F_MGR, F_DYE_PRED, F_WAVE, F_PTOP, F_CURL, F_CONC, F_CONC_PRED, F_COUNT };
 enum { ST_F2C, ST_ADVECT, ST_CASSON, ST_DIV, ST_VCYCLE, ST_PROJECT,
        ST_DYE_PRED, ST_DYE_RES, ST_WAVE, ST_TOP_PRESSURE, ST_PARCELS, ST_CONC_PRED,
        ST_CONC_RES };
@@ -52,15 +52,21 @@ struct orc_state {
   orc_drop drops[MAX_DROPS];
   double *curl; /* scene: curl of the advected centre velocity, 4 per cell */
   /* Lagrangian parcels (8 per parcel: position, mass, stretch, age + lysis)
-   * and their two splat counters (intact, lysed; n[0] each) */
+   * and their two splat counters (intact, lysed; one per cell of the splat
+   * grid: the concentration grid while it is on, else the MAC grid) */
   double *parcels;
   int np, np_cap;
   uint32_t *splat[2];
+  long splat_cap;
   /* the reference's 4-channel concentration (planes: intact RBC, lysed Hb,
-   * oxygenation/shear state, floor bed; n[0] each) and its twin; the next
+   * oxygenation/shear state, floor bed) and its twin on its own grid, conc_r
+   * times the MAC grid per axis (the reference's CONC_GRID is twice its
+   * VEL_GRID, common.ts:4-15), the lysed channel's predictor there; the next
    * parcel stage's emission (ring cursor, count) */
-  int conc_on;
-  double *conc, *conc2;
+  int conc_on, conc_r;
+  dims3 dc;
+  long nc;
+  double *conc, *conc2, *cpred;
   int emit_cursor, emit_count;
 };
 
@@ -68,9 +74,11 @@ struct orc_state {
 
 /* the intact-RBC concentration the Casson pass, the buoyancy and the parcels
  * read: the dye, or with the 4-channel concentration its .r plane; the lysed
- * one (.g) is 0 without it (NULL) */
+ * one (.g) is 0 without it (NULL); conc_dims is the grid they live on */
 static const double *intact_of(const orc_state *s) { return s->conc_on ? s->conc : s->dye; }
-static const double *lysed_of(const orc_state *s) { return s->conc_on ? s->conc + s->n[0] : NULL; }
+static const double *lysed_of(const orc_state *s) { return s->conc_on ? s->conc + s->nc : NULL; }
+static dims3 conc_dims(const orc_state *s) { return s->conc_on ? s->dc : s->d[0]; }
+static long splat_cells(const orc_state *s) { return s->conc_on ? s->nc : s->n[0]; }
 
 static double clampd(double x, double lo, double hi) {
   return x < lo ? lo : (x > hi ? hi : x);
@@ -118,6 +126,28 @@ double orc_sample_cell(const double *f, int nx, int ny, int nz, double ux,
                        double uy, double uz) {
   dims3 d = {nx, ny, nz};
   return sample_cell(f, d, ux, uy, uz);
+}
+
+/* A concentration channel f (intact_of / lysed_of) as a MAC cell's Casson,
+ * buoyancy and unfurling terms read it: textureSampleLevel at the cell
+ * centre (uvwCell, fluid.ts:446-447) and at the top y-face (uvwFaceY
+ * clamped to [0.001, 0.999], :477-478).  On the MAC grid itself (the dye, or
+ * the concentration at conc_r 1) these are the cell's value and the mean of
+ * it and the cell above (the top row: its own); on a finer concentration
+ * grid, trilinear samples of it. */
+static double conc_at_cell(const orc_state *s, const double *f, int i, int j, int k) {
+  dims3 d = s->d[0];
+  if (!s->conc_on || s->conc_r == 1) return f[IDX(d, i, j, k)];
+  return sample_cell(f, s->dc, (i + 0.5) / d.nx, (j + 0.5) / d.ny, (k + 0.5) / d.nz);
+}
+static double conc_at_yface(const orc_state *s, const double *f, int i, int j, int k) {
+  dims3 d = s->d[0];
+  if (!s->conc_on || s->conc_r == 1) {
+    int jp = j + 1 < d.ny ? j + 1 : d.ny - 1;
+    return 0.5 * (f[IDX(d, i, j, k)] + f[IDX(d, i, jp, k)]);
+  }
+  return sample_cell(f, s->dc, clampd((i + 0.5) / d.nx, 0.001, 0.999),
+                     clampd((j + 1.0) / d.ny, 0.001, 0.999), clampd((k + 0.5) / d.nz, 0.001, 0.999));
 }
 
 static double top_stagnation_gate(const orc_state *s, int i, int j, int k);
@@ -280,7 +310,7 @@ static void casson_buoyancy(orc_state *s, double dt) {
         }
         double gam = inv2h * sqrt(ss + P->shear_eps2);
         const double *cr = intact_of(s);
-        double cc = clampd(cr[c], 0.0, 1.0);
+        double cc = clampd(conc_at_cell(s, cr, i, j, k), 0.0, 1.0);
         double sq_muinf = sqrt(P->mu_w + cc * (P->mu_inf - P->mu_w));
         double tau = P->tau_y * cc * cc;
         double yterm = sqrt(tau / fmax(gam + P->gamma_reg, P->gamma_reg));
@@ -292,9 +322,7 @@ static void casson_buoyancy(orc_state *s, double dt) {
           double lap = (xp[a] + xn[a] + yp[a] + yn[a] + zp[a] + zn[a]) - 6.0 * m[a];
           out[a] = m[a] + alpha * lap;
         }
-        int jp = j + 1 < d.ny ? j + 1 : d.ny - 1;
-        double dye_face = 0.5 * (cr[c] + cr[IDX(d, i, jp, k)]);
-        out[1] += buoy * dye_face;
+        out[1] += buoy * conc_at_yface(s, cr, i, j, k);
         if (i >= d.nx - 1) out[0] = 0.0;
         if (k >= d.nz - 1) out[2] = 0.0;
         s->u2[c] = out[0]; s->v2[c] = out[1]; s->w2[c] = out[2];
@@ -352,7 +380,7 @@ static void casson_ref(orc_state *s, double dt) {
         }
         double gam = 50.0 * sqrt(ss + 1e-6);
         const double *ci = intact_of(s), *cl = lysed_of(s);
-        double cr = clampd(ci[c], 0.0, 1.5);
+        double cr = clampd(conc_at_cell(s, ci, i, j, k), 0.0, 1.5);
         double sq_muinf = sqrt(0.00010 * (1.0 + 1.35 * cr));
         double tau = 0.0036 * scale * cr * cr;
         double yterm = sqrt(tau / fmax(gam + 0.40, 0.40));
@@ -362,9 +390,8 @@ static void casson_ref(orc_state *s, double dt) {
           double lap = (xp[a] + xn[a] + yp[a] + yn[a] + zp[a] + zn[a]) - 6.0 * m[a];
           out[a] = m[a] + alpha * lap;
         }
-        int jp = j + 1 < d.ny ? j + 1 : d.ny - 1;
-        long cu = IDX(d, i, jp, k);
-        double rho = 1.24 * (0.5 * (ci[c] + ci[cu])) + (cl ? 0.26 * (0.5 * (cl[c] + cl[cu])) : 0.0);
+        double rho = 1.24 * conc_at_yface(s, ci, i, j, k) +
+                     (cl ? 0.26 * conc_at_yface(s, cl, i, j, k) : 0.0);
         if (rho > 0.0008) {
           double yf = y0 + (j + 1.0) / d.ny * s->wp.depth;
           double gate = sstep(0.002, 0.024, fmax(0.0, ytop - yf));
@@ -1164,7 +1191,8 @@ static void casson_scene(orc_state *s, double dt) {
         double below = ytop - py;
         const double *cl = lysed_of(s);
         double gate = sstep(0.11, 0.36, below) *
-                      sstep(0.0008, 0.015, clampd(intact_of(s)[c], 0.0, 1.5) + (cl ? 0.45 * cl[c] : 0.0));
+                      sstep(0.0008, 0.015, clampd(conc_at_cell(s, intact_of(s), i, j, k), 0.0, 1.5) +
+                                               (cl ? 0.45 * conc_at_cell(s, cl, i, j, k) : 0.0));
         if (gate > 0.001) {
           double q1 = 6.4 * px + 4.0 * py + 5.2 * pz + ts;
           double q2 = -5.6 * px + 4.8 * py + 7.2 * pz - ts * 0.77;
@@ -1242,9 +1270,10 @@ static void pc_velocity(const orc_state *s, const parcel_scene *S, const double 
 }
 
 /* splatTrilinearIntact/Lysed (particles.ts:238-301): round-half-even of
- * ((mass 65536) w_x) w_y w_z added to the eight clamped cells */
+ * ((mass 65536) w_x) w_y w_z added to the eight clamped cells of the splat
+ * grid (CONC_GRID: the concentration's) */
 static void pc_splat(orc_state *s, int which, const double u[3], double mass) {
-  dims3 d = s->d[0];
+  dims3 d = conc_dims(s);
   int n[3] = {d.nx, d.ny, d.nz}, b0[3], b1[3];
   double f[3];
   for (int a = 0; a < 3; ++a) {
@@ -1344,8 +1373,8 @@ static void parcel_step(orc_state *s, int g, const parcel_scene *S) {
   double cell_res = (double)(crf - floorf(crf));
   /* cSample.r + cSample.g (particles.ts:403-404): the dye alone without the
    * 4-channel concentration */
-  double cn = sample_cell(intact_of(s), s->d[0], uc[0], uc[1], uc[2]);
-  if (lysed_of(s)) cn += sample_cell(lysed_of(s), s->d[0], uc[0], uc[1], uc[2]);
+  double cn = sample_cell(intact_of(s), conc_dims(s), uc[0], uc[1], uc[2]);
+  if (lysed_of(s)) cn += sample_cell(lysed_of(s), conc_dims(s), uc[0], uc[1], uc[2]);
   double dilute = 1.0 - sstep(0.012, 0.22, cn);
   double disp = sstep(1.1, 7.8, age + dilute * 2.2 + lysis * 2.5);
   double late = sstep(7.2, 16.0, age) * dilute;
@@ -1413,7 +1442,7 @@ static void hash33_f(float px, float py, float pz, float out[3]) {
 }
 static void parcel_emit(orc_state *s, int g, int count) {
   const double hx = s->wp.half_xz, ytop = s->wp.surface_y, y0 = ytop - s->wp.depth;
-  dims3 d = s->d[0];
+  dims3 d = conc_dims(s);
   double total = 0.0;
   for (int q = 0; q < MAX_DROPS; ++q)
     if (s->drops[q].seed > 0.0 && s->drops[q].drain > 0.0001) total += s->drops[q].drain;
@@ -1494,24 +1523,50 @@ static void parcels_stage(orc_state *s, double dt) {
   if (!s->wave_on || !s->parcels) return;
   parcel_scene S = {s->wp.half_xz, s->wp.surface_y - s->wp.depth, s->wp.surface_y, s->wp.sim_time,
                     s->wp.hemolysis, clampd(dt, 0.0, 0.025)};
-  memset(s->splat[0], 0, sizeof(uint32_t) * s->n[0]);
-  memset(s->splat[1], 0, sizeof(uint32_t) * s->n[0]);
+  memset(s->splat[0], 0, sizeof(uint32_t) * splat_cells(s));
+  memset(s->splat[1], 0, sizeof(uint32_t) * splat_cells(s));
   if (s->emit_count > 0 && s->np > 0)
     for (int g = 0; g < s->emit_count; ++g) parcel_emit(s, g, s->emit_count);
   s->emit_count = 0;
   for (int g = 0; g < s->np; ++g) parcel_step(s, g, &S);
 }
 
-/* ---- CONC_PREDICT (advect.ts:28-72) of the lysed channel: the dye
- * predictor's code with the reference's branches, on .g */
+/* ---- CONC_PREDICT (advect.ts:28-72) of the lysed channel (.g) on the
+ * concentration grid: uvw = (cell + 1/2) / N; the lateral walls and the top
+ * row 0 (:42-45); v0 the centre velocity (the MAC grid's) sampled at uvw;
+ * below speed^2 2.5e-5 the cell's own value (:54-58); below 0.012 a one-
+ * stage trace uvw - v0 dt / domainSize (:60-62); else the RK2 trace through
+ * the midpoint velocity (:63-67); uvw clamped to [0.001, 0.999]; dt clamped
+ * to [0, 0.025]; domainSize the world box (common.ts:52-54). */
 static void conc_predict(orc_state *s, double dt) {
-  double *keep = s->dye;
-  int phys = s->wp.physics;
-  s->dye = s->conc + s->n[0];
-  s->wp.physics = 1;
-  dye_predict(s, dt);
-  s->wp.physics = phys;
-  s->dye = keep;
+  dims3 d = s->d[0], dc = s->dc;
+  const double dom[3] = {2.0 * s->wp.half_xz, s->wp.depth, 2.0 * s->wp.half_xz};
+  const double *g = s->conc + s->nc;
+  dt = clampd(dt, 0.0, 0.025);
+#pragma omp parallel for
+  for (int k = 0; k < dc.nz; ++k)
+    for (int j = 0; j < dc.ny; ++j)
+      for (int i = 0; i < dc.nx; ++i) {
+        long c = IDX(dc, i, j, k);
+        if (conc_wall(dc, i, j, k)) { s->cpred[c] = 0.0; continue; }
+        double uvw[3] = {(i + 0.5) / dc.nx, (j + 0.5) / dc.ny, (k + 0.5) / dc.nz};
+        double v0[3] = {sample_cell(s->cu, d, uvw[0], uvw[1], uvw[2]),
+                        sample_cell(s->cv, d, uvw[0], uvw[1], uvw[2]),
+                        sample_cell(s->cw, d, uvw[0], uvw[1], uvw[2])};
+        double sp2 = v0[0] * v0[0] + v0[1] * v0[1] + v0[2] * v0[2], back[3];
+        if (sp2 < 2.5e-5) { s->cpred[c] = g[c]; continue; }
+        if (sp2 < 0.012) {
+          for (int a = 0; a < 3; ++a) back[a] = clampd(uvw[a] - (v0[a] * dt) / dom[a], 0.001, 0.999);
+        } else {
+          double mid[3];
+          for (int a = 0; a < 3; ++a) mid[a] = clampd(uvw[a] - 0.5 * (v0[a] * dt) / dom[a], 0.001, 0.999);
+          double vm[3] = {sample_cell(s->cu, d, mid[0], mid[1], mid[2]),
+                          sample_cell(s->cv, d, mid[0], mid[1], mid[2]),
+                          sample_cell(s->cw, d, mid[0], mid[1], mid[2])};
+          for (int a = 0; a < 3; ++a) back[a] = clampd(uvw[a] - (vm[a] * dt) / dom[a], 0.001, 0.999);
+        }
+        s->cpred[c] = sample_cell(g, dc, back[0], back[1], back[2]);
+      }
 }
 
 /* ---- CONC_RESOLVE (advect.ts:81-298): the 4-channel composite.  Per cell
@@ -1535,9 +1590,11 @@ static void conc_predict(orc_state *s, double dt) {
  *    shear, the sheet thickness and the bed fraction (:290-297). */
 static double conc_resolve_cell(const orc_state *s, const double *src, double dt, int ix, int iy,
                                 int iz, double out[4]) {
-  dims3 d = s->d[0];
-  const long n = s->n[0];
-  const double *og = src + n, *oa = src + 3 * n, *pg = s->dpred;
+  /* d: the concentration grid (its cells, counters and predictor); the
+   * centre velocity is sampled on the MAC grid dv */
+  dims3 d = s->dc, dv = s->d[0];
+  const long n = s->nc;
+  const double *og = src + n, *oa = src + 3 * n, *pg = s->cpred;
   const uint32_t *si = s->splat[0], *sl = s->splat[1];
   out[0] = out[1] = out[2] = out[3] = 0.0;
   if (ix == 0 || ix >= d.nx - 1 || iz == 0 || iz >= d.nz - 1 || iy >= d.ny - 1) return 0.0;
@@ -1567,12 +1624,12 @@ static double conc_resolve_cell(const orc_state *s, const double *src, double dt
   const double uvw[3] = {(ix + 0.5) / N[0], (iy + 0.5) / N[1], (iz + 0.5) / N[2]};
   const double dom[3] = {2.0 * hx, ytop - y0, 2.0 * hx};
   const double p[3] = {-hx + uvw[0] * dom[0], y0 + uvw[1] * dom[1], -hx + uvw[2] * dom[2]};
-  double v0[3] = {sample_cell(s->cu, d, uvw[0], uvw[1], uvw[2]), sample_cell(s->cv, d, uvw[0], uvw[1], uvw[2]),
-                  sample_cell(s->cw, d, uvw[0], uvw[1], uvw[2])};
+  double v0[3] = {sample_cell(s->cu, dv, uvw[0], uvw[1], uvw[2]), sample_cell(s->cv, dv, uvw[0], uvw[1], uvw[2]),
+                  sample_cell(s->cw, dv, uvw[0], uvw[1], uvw[2])};
   double mid[3], back[3], fwd[3];
   for (int a = 0; a < 3; ++a) mid[a] = clampd(uvw[a] - 0.5 * (v0[a] * dt) / dom[a], 0.001, 0.999);
-  double vm[3] = {sample_cell(s->cu, d, mid[0], mid[1], mid[2]), sample_cell(s->cv, d, mid[0], mid[1], mid[2]),
-                  sample_cell(s->cw, d, mid[0], mid[1], mid[2])};
+  double vm[3] = {sample_cell(s->cu, dv, mid[0], mid[1], mid[2]), sample_cell(s->cv, dv, mid[0], mid[1], mid[2]),
+                  sample_cell(s->cw, dv, mid[0], mid[1], mid[2])};
   for (int a = 0; a < 3; ++a) {
     back[a] = clampd(uvw[a] - (vm[a] * dt) / dom[a], 0.001, 0.999);
     fwd[a] = clampd(uvw[a] + (v0[a] * dt) / dom[a], 0.001, 0.999);
@@ -1641,8 +1698,8 @@ static double conc_resolve_cell(const orc_state *s, const double *src, double dt
   return 0.0;
 }
 static void conc_resolve(orc_state *s, double dt) {
-  dims3 d = s->d[0];
-  const long n = s->n[0];
+  dims3 d = s->dc;
+  const long n = s->nc;
 #pragma omp parallel for
   for (int k = 0; k < d.nz; ++k)
     for (int j = 0; j < d.ny; ++j)
@@ -1655,16 +1712,42 @@ static void conc_resolve(orc_state *s, double dt) {
   double *t = s->conc; s->conc = s->conc2; s->conc2 = t;
 }
 
+/* the two splat counters hold at least the splat grid's cells (zeroed when
+ * they grow) */
+static void ensure_splat(orc_state *s) {
+  long need = splat_cells(s);
+  if (s->splat_cap >= need) return;
+  free(s->splat[0]);
+  free(s->splat[1]);
+  s->splat[0] = (uint32_t *)calloc(need, sizeof(uint32_t));
+  s->splat[1] = (uint32_t *)calloc(need, sizeof(uint32_t));
+  s->splat_cap = need;
+}
+
+/* on: 0 off; 1 the concentration on the MAC grid; 2 on a grid twice as fine
+ * per axis (the reference's CONC_GRID over its VEL_GRID).  A change of grid
+ * starts from zero. */
 void orc_conc_enable(orc_state *s, int on) {
-  if (on && !s->conc) {
-    s->conc = (double *)calloc(4 * (size_t)s->n[0], sizeof(double));
-    s->conc2 = (double *)calloc(4 * (size_t)s->n[0], sizeof(double));
+  if (on != 1 && on != 2) {
+    s->conc_on = 0;
+    return;
   }
-  if (on && !s->splat[0]) {
-    s->splat[0] = (uint32_t *)calloc(s->n[0], sizeof(uint32_t));
-    s->splat[1] = (uint32_t *)calloc(s->n[0], sizeof(uint32_t));
+  if (s->conc && s->conc_r != on) {
+    free(s->conc); free(s->conc2); free(s->cpred);
+    s->conc = s->conc2 = s->cpred = NULL;
   }
-  s->conc_on = on != 0;
+  s->conc_r = on;
+  s->dc.nx = on * s->d[0].nx;
+  s->dc.ny = on * s->d[0].ny;
+  s->dc.nz = on * s->d[0].nz;
+  s->nc = (long)s->dc.nx * s->dc.ny * s->dc.nz;
+  if (!s->conc) {
+    s->conc = (double *)calloc(4 * (size_t)s->nc, sizeof(double));
+    s->conc2 = (double *)calloc(4 * (size_t)s->nc, sizeof(double));
+    s->cpred = (double *)calloc((size_t)s->nc, sizeof(double));
+  }
+  s->conc_on = 1;
+  ensure_splat(s);
 }
 
 void orc_parcels_emit(orc_state *s, int cursor, int count) {
@@ -1678,16 +1761,24 @@ void orc_parcels_set(orc_state *s, const double *p, int n) {
     s->parcels = (double *)malloc(sizeof(double) * 8 * (size_t)n);
     s->np_cap = n;
   }
-  if (!s->splat[0]) {
-    s->splat[0] = (uint32_t *)calloc(s->n[0], sizeof(uint32_t));
-    s->splat[1] = (uint32_t *)calloc(s->n[0], sizeof(uint32_t));
-  }
+  ensure_splat(s);
   memcpy(s->parcels, p, sizeof(double) * 8 * (size_t)n);
   s->np = n;
 }
 
 double *orc_parcels(orc_state *s) { return s->parcels; }
-uint32_t *orc_splat(orc_state *s, int which) { return (which == 0 || which == 1) ? s->splat[which] : NULL; }
+uint32_t *orc_splat(orc_state *s, int which) {
+  ensure_splat(s);
+  return (which == 0 || which == 1) ? s->splat[which] : NULL;
+}
+/* the splat grid (the concentration grid while it is on, else the MAC
+ * grid): out = nx, ny, nz */
+void orc_splat_dims(orc_state *s, int out[3]) {
+  dims3 d = conc_dims(s);
+  out[0] = d.nx;
+  out[1] = d.ny;
+  out[2] = d.nz;
+}
 
 void orc_wave_enable(orc_state *s, const orc_wave_params *p) {
   if (!p) {
@@ -1754,6 +1845,7 @@ void orc_destroy(orc_state *s) {
   for (int a = 0; a < 4; ++a) free(s->cg[a]);
   free(s->wave); free(s->wz); free(s->wz2); free(s->wfilm); free(s->ptop); free(s->curl);
   free(s->parcels); free(s->splat[0]); free(s->splat[1]); free(s->conc); free(s->conc2);
+  free(s->cpred);
   free(s);
 }
 
@@ -1777,6 +1869,7 @@ double *orc_field(orc_state *s, int field, int level) {
     case F_PTOP: return s->wave_on && level == 0 ? s->ptop : NULL;
     case F_CURL: return level == 0 ? s->curl : NULL;
     case F_CONC: return level == 0 && s->conc_on ? s->conc : NULL;
+    case F_CONC_PRED: return level == 0 && s->conc_on ? s->cpred : NULL;
   }
   return NULL;
 }
@@ -1785,7 +1878,9 @@ long orc_field_len(orc_state *s, int field, int level) {
   if (field == F_MGX || field == F_MGB || field == F_MGR) return s->n[level];
   if (field == F_WAVE) return 4L * s->wp.n * s->wp.n;
   if (field == F_PTOP) return (long)s->d[0].nx * s->d[0].nz;
-  if (field == F_CURL || field == F_CONC) return 4L * s->n[0];
+  if (field == F_CURL) return 4L * s->n[0];
+  if (field == F_CONC) return s->conc_on ? 4L * s->nc : 0;
+  if (field == F_CONC_PRED) return s->conc_on ? s->nc : 0;
   return s->n[0];
 }
 

@@ -75,15 +75,20 @@ void orc_wave_enable(orc_state *s, const orc_wave_params *p);
 void orc_set_drops(orc_state *s, const orc_drop *d, int n);
 /* Lagrangian parcels: n x 8 doubles (x, y, z, mass, stretch x, y, z,
  * age ticks + lysis); orc_parcels returns them after ST_PARCELS, orc_splat
- * the intact (0) and lysed (1) counters, n cells each */
+ * the intact (0) and lysed (1) counters, one per cell of the splat grid
+ * (orc_splat_dims: the concentration grid while it is on, else the MAC grid) */
 void orc_parcels_set(orc_state *s, const double *p, int n);
 /* the next parcel stage first emits `count` parcels from ring slot `cursor`
- * (PARTICLE_EMIT); the 4-channel concentration on or off (stages 11, 12 and
- * pass B in the step; field 16 = its planes r, g, b, a) */
+ * (PARTICLE_EMIT); the 4-channel concentration (stages 11, 12 and pass B in
+ * the step; field 16 = its planes r, g, b, a, field 17 = the lysed channel's
+ * predictor): on = 0 off, 1 on the MAC grid,
+ * 2 on a grid twice as fine per axis (the reference's CONC_GRID over its
+ * VEL_GRID); a change of grid starts from zero */
 void orc_parcels_emit(orc_state *s, int cursor, int count);
 void orc_conc_enable(orc_state *s, int on);
 double *orc_parcels(orc_state *s);
 uint32_t *orc_splat(orc_state *s, int which);
+void orc_splat_dims(orc_state *s, int out[3]);
 /* unitary in-place complex FFT of length n (a power of two) over interleaved
  * (re, im) doubles, sign -1 forward / +1 inverse: exposed for its pinning */
 void orc_fft(double *z, int n, int stride, int sign);

@@ -18,7 +18,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 LIB = os.environ.get("MAC3D_ORACLE_LIB") or os.path.join(HERE, "liborc.so")
 
 (F_U, F_V, F_W, F_P, F_DYE, F_CU, F_CV, F_CW, F_RHS, F_MGX, F_MGB, F_MGR, F_DYE_PRED, F_WAVE,
- F_PTOP, F_CURL, F_CONC) = range(17)
+ F_PTOP, F_CURL, F_CONC, F_CONC_PRED) = range(18)
 (ST_F2C, ST_ADVECT, ST_CASSON, ST_DIV, ST_VCYCLE, ST_PROJECT, ST_DYE_PRED, ST_DYE_RES, ST_WAVE,
  ST_TOP_PRESSURE, ST_PARCELS, ST_CONC_PRED, ST_CONC_RES) = range(13)
 MG_SMOOTH, MG_RESIDUAL, MG_RESTRICT, MG_PROLONG, MG_COARSE, MG_ZERO = range(6)
@@ -79,6 +79,7 @@ def lib():
         L.orc_parcels.restype = C.POINTER(C.c_double)
         L.orc_splat.argtypes = [C.c_void_p, C.c_int]
         L.orc_splat.restype = C.POINTER(C.c_uint32)
+        L.orc_splat_dims.argtypes = [C.c_void_p, C.POINTER(C.c_int)]
         L.orc_set_drops.argtypes = [C.c_void_p, C.POINTER(OrcDrop), C.c_int]
         L.orc_parcels_emit.argtypes = [C.c_void_p, C.c_int, C.c_int]
         L.orc_conc_enable.argtypes = [C.c_void_p, C.c_int]
@@ -152,7 +153,9 @@ class Oracle:
         elif field == F_CURL:
             shape = tuple(self.shape) + (4,)
         elif field == F_CONC:
-            shape = (4,) + tuple(self.shape)  # planes: intact, lysed, oxygenation, bed
+            shape = (4,) + self.splat_shape()  # planes: intact, lysed, oxygenation, bed
+        elif field == F_CONC_PRED:
+            shape = self.splat_shape()  # the lysed channel's predictor
         else:
             shape = self.shape
         return np.ctypeslib.as_array(ptr, shape=(n,)).reshape(shape)
@@ -210,15 +213,29 @@ class Oracle:
         ptr = lib().orc_parcels(self.h)
         return np.ctypeslib.as_array(ptr, shape=(self.n_parcels * 8,)).reshape(-1, 8).copy()
 
-    def splat(self, which):
-        """the intact (0) or lysed (1) splat counters, (nz, ny, nx) uint32"""
-        ptr = lib().orc_splat(self.h, int(which))
-        n = int(np.prod(self.shape))
-        return np.ctypeslib.as_array(ptr, shape=(n,)).reshape(self.shape).copy()
+    def splat_shape(self):
+        """(nz, ny, nx) of the splat grid: the concentration grid while it
+        is on, else the MAC grid"""
+        out = (C.c_int * 3)()
+        lib().orc_splat_dims(self.h, out)
+        return (out[2], out[1], out[0])
 
-    def conc_enable(self, on=True):
-        """the reference's 4-channel concentration (pass B in the step)"""
-        lib().orc_conc_enable(self.h, int(on))
+    def splat_view(self, which):
+        """numpy view (writes go through) of the intact (0) or lysed (1)
+        splat counters, splat_shape() uint32"""
+        ptr = lib().orc_splat(self.h, int(which))
+        shape = self.splat_shape()
+        return np.ctypeslib.as_array(ptr, shape=(int(np.prod(shape)),)).reshape(shape)
+
+    def splat(self, which):
+        """the intact (0) or lysed (1) splat counters, splat_shape() uint32"""
+        return self.splat_view(which).copy()
+
+    def conc_enable(self, on=True, refine=1):
+        """the reference's 4-channel concentration (pass B in the step), on
+        the MAC grid (refine 1) or a grid twice as fine per axis (refine 2,
+        the reference's CONC_GRID over its VEL_GRID)"""
+        lib().orc_conc_enable(self.h, int(refine) if on else 0)
 
     def parcels_emit(self, cursor, count):
         """the next parcel stage first emits `count` parcels from ring slot
