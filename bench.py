@@ -9,7 +9,9 @@ fields resident in HBM.  The same line carries the mixed-precision run
 speed-up, the smoother's roofline figures (HIP events on the solver's
 stream), the FP32 step with the free-surface wave coupling on (a 512 x 512
 wave grid, two active drops), the same with the reference scene's advection
-and Casson terms, and the CPU baseline timed on this host: the FP64
+and Casson terms, the parcel stage, the reference's whole scene at its own
+shape (96x128x96 MAC grid, 192x256x192 concentration, 393,216 parcels), and
+the CPU baseline timed on this host: the FP64
 JavaScript transcription of the step (Node worker threads) and the FP64 C
 oracle (all threads and one).
 
@@ -130,6 +132,48 @@ def run_parcels(workload, steps, order):
         s.profile(False)
         live = int((s.parcels_download()[:, 3] > 0).sum())
         return ms / max(n, 1), live
+
+
+# The reference's own scene at its shape: VEL_GRID 96x128x96 with 10 mm
+# cells in the 0.96 x 1.28 x 0.96 m box under the surface at y = 0.30, dt
+# 1/60 s, velDamp 0.9993, the 512-texel wave, the scene terms with the
+# reference's physics, the 393,216-parcel ring with 2000 emissions per step
+# from two draining drops, and the 4-channel concentration on CONC_GRID
+# 192x256x192 (twice the MAC grid per axis)
+REF_SCENE_DROPS = [(0.03, 0.285, -0.02, 0.024, -0.35, 0.0, 0.62, 1.3),
+                   (-0.11, 0.31, 0.07, 0.018, -0.9, 0.0, 0.21, 2.4)]
+REF_SCENE_EMIT = 2000
+
+
+def run_reference_scene(steps, warmup):
+    """steps/s and ms/step of the whole physics step of the reference's
+    scene (pass A and pass B) from a quiescent start, after `warmup` steps;
+    the emission request goes to the device before each step"""
+    import numpy as np
+    from mac3d import MacSolver, scenarios
+    from mac3d.lib import config_for
+    w = scenarios.scaled(scenarios.workload("tiny", 1), nx=96, ny=128, nz=96, dx=0.01,
+                         dt=1.0 / 60.0, levels=3, blobs=[])
+    with MacSolver(config_for(w, vel_damp=0.9993)) as s:
+        s.init_blobs([])
+        s.wave_enable(n=WAVE_N, half_xz=0.48, depth=1.28, surface_y=0.30, marangoni=1.0,
+                      scene=1, physics=1)
+        s.set_drops(REF_SCENE_DROPS)
+        s.conc_enable(True, refine=2)
+        s.parcels_upload(np.zeros((PARCELS, 8), dtype=np.float32))
+        q = 0
+        for _ in range(warmup):
+            s.parcels_emit(REF_SCENE_EMIT * q, REF_SCENE_EMIT)
+            s.step_nostats(w.dt, 1)
+            q += 1
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            s.parcels_emit(REF_SCENE_EMIT * q, REF_SCENE_EMIT)
+            s.step_nostats(w.dt, 1)
+            q += 1
+        wall = time.perf_counter() - t0
+        live = int((s.parcels_download()[:, 3] > 0).sum())
+        return steps / wall, live
 
 
 def copy_probe(workload, times=5):
@@ -297,6 +341,17 @@ def main():
                 "parcels_per_s": round(PARCELS / (ms_p * 1e-3), 1),
                 "state_gbps": round(PARCEL_BYTES * PARCELS / (ms_p * 1e-3) / 1e9, 1)}
             log(f"[bench] parcels {PARCELS} ({order} order): {1e3 * ms_p:.1f} us per step")
+        n_rs = max(10, min(args.steps, 40))
+        sps_rs, live_rs = run_reference_scene(n_rs, 5)
+        wave["reference_scene"] = {
+            "steps_per_s": round(sps_rs, 3), "ms_per_step": round(1e3 / sps_rs, 4),
+            "steps": n_rs, "warmup": 5, "live_parcels_after": live_rs,
+            "config": "96x128x96 MAC grid (10 mm), dt 1/60 s, 3-level V-cycle 2+2 x 8, "
+                      "512^2 wave, scene terms and the reference's physics, 393216-parcel "
+                      f"ring with {REF_SCENE_EMIT} emissions per step, 4-channel "
+                      "concentration on 192x256x192, FP32, quiescent start"}
+        log(f"[bench] reference scene at its shape: {sps_rs:.2f} steps/s "
+            f"({1e3 / sps_rs:.3f} ms/step), {live_rs} live parcels")
 
     roof = None
     if "SMOOTH2_L0" in prof:

@@ -1,12 +1,15 @@
-// conc_kernels.hpp — the reference's pass B on the MAC grid: parcel emission
-// from the draining necks (PARTICLE_EMIT, src/shaders/particles.ts:53-186) and
-// the 4-channel concentration resolve (CONC_RESOLVE,
-// src/shaders/advect.ts:81-298).  FP32, as the reference's parcels are (its
-// concentration is rgba16float; FP32 here).
+// conc_kernels.hpp — the reference's pass B: parcel emission from the
+// draining necks (PARTICLE_EMIT, src/shaders/particles.ts:53-186), the lysed
+// channel's predictor (CONC_PREDICT, src/shaders/advect.ts:28-72) and the
+// 4-channel concentration resolve (CONC_RESOLVE, advect.ts:81-298).  FP32, as
+// the reference's parcels are (its concentration is rgba16float; FP32 here).
 //
-// The concentration is four planes of n floats: intact RBC (.r), lysed Hb
-// (.g), oxygenation/shear state (.b), floor bed (.a).  The lysed plane's
-// semi-Lagrangian predictor is k_dye_predict<..., REF> on .g.
+// The concentration is four planes of n floats on its own grid: intact RBC
+// (.r), lysed Hb (.g), oxygenation/shear state (.b), floor bed (.a).  That
+// grid is the MAC grid or, as the reference holds it (CONC_GRID 192x256x192
+// over VEL_GRID 96x128x96, common.ts:4-15), twice as fine per axis; the
+// kernels address it in uvw = (cell + 1/2) / N and sample the MAC grid's
+// centre velocity there, as the reference's textureSampleLevel does.
 #pragma once
 
 #include "common.hpp"
@@ -54,7 +57,8 @@ __device__ inline void hash33_rn(float px, float py, float pz, float out[3]) {
 
 // PARTICLE_EMIT: thread g < ctl->count writes a new parcel at ring slot
 // (cursor + g) mod np, in the drop slot its share of the summed drain rates
-// picks, after moving a live parcel's mass there into the lysed counter.
+// picks, after moving a live parcel's mass there into the lysed counter (a
+// cell of the splat grid d).
 __global__ void __launch_bounds__(256)
 k_parcel_emit(Dims d, ConcScene S, const EmitCtl *__restrict__ ctl,
               const WaveDrop *__restrict__ drops, const SceneClock *__restrict__ clk,
@@ -149,13 +153,73 @@ __global__ void k_emit_done(EmitCtl *__restrict__ ctl) {
   if (threadIdx.x == 0 && blockIdx.x == 0) ctl->count = 0;
 }
 
-// CONC_RESOLVE (advect.ts:81-298): src/out the four planes (intact, lysed,
-// oxygenation, bed); pg the lysed predictor; si/sl the splat counters.
+// CONC_PREDICT (advect.ts:28-72) of the lysed channel g on the
+// concentration grid d: the lateral walls and the top row 0 (:42-45); v0 the
+// centre velocity (MAC grid dv) at uvw = (cell + 1/2) / N; below speed^2
+// 2.5e-5 the cell's own value (:54-58); below 0.012 the one-stage trace
+// uvw - v0 dt / domainSize (:60-62), else the RK2 trace through the midpoint
+// velocity (:63-67); uvw clamped to [0.001, 0.999], dt to [0, 0.025].
 __global__ void __launch_bounds__(256)
-k_conc_resolve(Dims d, ConcScene S, float dt, float strength, const float3 *__restrict__ cvel,
-               const float *__restrict__ src, const float *__restrict__ pg,
-               const unsigned *__restrict__ si, const unsigned *__restrict__ sl,
-               float *__restrict__ out) {
+k_conc_predict(Dims d, Dims dv, ConcScene S, float dt, const float3 *__restrict__ cvel,
+               const float *__restrict__ g, float *__restrict__ pred) {
+  MAC3D_CELL_PROLOGUE
+  if (conc_wall(d, i, j, k)) {
+    pred[c] = 0.f;
+    return;
+  }
+  const float dtc = clampv(dt, 0.f, 0.025f);
+  const float dom0 = 2.f * S.half_xz, dom1 = S.y_max - S.y_min;
+  const float u0 = (i + 0.5f) / (float)d.nx, u1 = (j + 0.5f) / (float)d.ny,
+              u2 = (k + 0.5f) / (float)d.nz;
+  float v0[3];
+  sample_vec<float>(cvel, dv, u0, u1, u2, v0);
+  const float sp2 = v0[0] * v0[0] + v0[1] * v0[1] + v0[2] * v0[2];
+  if (sp2 < 2.5e-5f) {
+    pred[c] = g[c];
+    return;
+  }
+  float v[3] = {v0[0], v0[1], v0[2]};
+  if (sp2 >= 0.012f) {
+    const float m0 = clampv(u0 - 0.5f * (v0[0] * dtc) / dom0, 0.001f, 0.999f);
+    const float m1 = clampv(u1 - 0.5f * (v0[1] * dtc) / dom1, 0.001f, 0.999f);
+    const float m2 = clampv(u2 - 0.5f * (v0[2] * dtc) / dom0, 0.001f, 0.999f);
+    sample_vec<float>(cvel, dv, m0, m1, m2, v);
+  }
+  const float b0 = clampv(u0 - (v[0] * dtc) / dom0, 0.001f, 0.999f);
+  const float b1 = clampv(u1 - (v[1] * dtc) / dom1, 0.001f, 0.999f);
+  const float b2 = clampv(u2 - (v[2] * dtc) / dom0, 0.001f, 0.999f);
+  pred[c] = sample_cell<float>(g, d, b0, b1, b2);
+}
+
+// The intact (.r) and lysed (.g) channels of a concentration grid dc finer
+// than the MAC grid d, as the Casson pass, its buoyancy and the unfurling
+// gate read them per MAC cell (fluid.ts:446-447, 477-478, 491): samples at
+// the cell centre and at the top y-face (uvw clamped to [0.001, 0.999]).
+// out: four MAC-grid planes, .r at the centre, .g at the centre, .r at the
+// face, .g at the face.
+__global__ void __launch_bounds__(256)
+k_conc_at_mac(Dims d, Dims dc, const float *__restrict__ conc, float *__restrict__ out) {
+  MAC3D_CELL_PROLOGUE
+  const size_t n = d.cells(), ncc = dc.cells();
+  const float *r = conc, *g = conc + ncc;
+  const float u0 = (i + 0.5f) / (float)d.nx, u1 = (j + 0.5f) / (float)d.ny,
+              u2 = (k + 0.5f) / (float)d.nz;
+  out[c] = sample_cell<float>(r, dc, u0, u1, u2);
+  out[n + c] = sample_cell<float>(g, dc, u0, u1, u2);
+  const float f0 = clampv(u0, 0.001f, 0.999f), f2 = clampv(u2, 0.001f, 0.999f);
+  const float f1 = clampv((j + 1.f) / (float)d.ny, 0.001f, 0.999f);
+  out[2 * n + c] = sample_cell<float>(r, dc, f0, f1, f2);
+  out[3 * n + c] = sample_cell<float>(g, dc, f0, f1, f2);
+}
+
+// CONC_RESOLVE (advect.ts:81-298) on the concentration grid d: src/out the
+// four planes (intact, lysed, oxygenation, bed); pg the lysed predictor;
+// si/sl the splat counters; cvel the centre velocity of the MAC grid dv.
+__global__ void __launch_bounds__(256)
+k_conc_resolve(Dims d, Dims dv, ConcScene S, float dt, float strength,
+               const float3 *__restrict__ cvel, const float *__restrict__ src,
+               const float *__restrict__ pg, const unsigned *__restrict__ si,
+               const unsigned *__restrict__ sl, float *__restrict__ out) {
   MAC3D_CELL_PROLOGUE
   const size_t n = d.cells();
   const float *og = src + n, *oa = src + 3 * n;
@@ -195,11 +259,11 @@ k_conc_resolve(Dims d, ConcScene S, float dt, float strength, const float3 *__re
   const float dom0 = 2.f * hx, dom1 = ytop - y0, dom2 = 2.f * hx;
   const float px = -hx + u0 * dom0, py = y0 + u1 * dom1, pz = -hx + u2 * dom2;
   float v0[3], vm[3];
-  sample_vec<float>(cvel, d, u0, u1, u2, v0);
+  sample_vec<float>(cvel, dv, u0, u1, u2, v0);
   const float m0 = clampv(u0 - 0.5f * (v0[0] * dtc) / dom0, 0.001f, 0.999f);
   const float m1 = clampv(u1 - 0.5f * (v0[1] * dtc) / dom1, 0.001f, 0.999f);
   const float m2 = clampv(u2 - 0.5f * (v0[2] * dtc) / dom2, 0.001f, 0.999f);
-  sample_vec<float>(cvel, d, m0, m1, m2, vm);
+  sample_vec<float>(cvel, dv, m0, m1, m2, vm);
   const float b0 = clampv(u0 - (vm[0] * dtc) / dom0, 0.001f, 0.999f);
   const float b1 = clampv(u1 - (vm[1] * dtc) / dom1, 0.001f, 0.999f);
   const float b2 = clampv(u2 - (vm[2] * dtc) / dom2, 0.001f, 0.999f);

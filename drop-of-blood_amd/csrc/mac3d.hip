@@ -140,20 +140,27 @@ struct Ctx {
   bool ref_physics = false;
   double casson_scale = 1.0, buoyancy = 0.48;
   // the reference's 4-channel concentration (planes: intact, lysed,
-  // oxygenation, floor bed; FP32) and its twin, and the parcels' pending
+  // oxygenation, floor bed; FP32) and its twin on their own grid dcg, conc_r
+  // times the MAC grid per axis; the lysed channel's predictor there; with
+  // conc_r 2 the channels as the MAC cells read them (cmac: .r and .g at the
+  // cell centres, then at the top y-faces, n[0] each); the parcels' pending
   // emission (device memory, read by the emission kernel of a parcel stage)
   bool conc_on = false;
-  Buf conc, conc2;
+  int conc_r = 0;
+  Dims dcg{};
+  Buf conc, conc2, cpred, cmac;
   void *conc_home = nullptr;
   EmitCtl *emit = nullptr;
   WaveParams wp{};
   Buf wave, wz, wfilm, ptop;
   Buf curl;  // scene: curl of the post-advection centre velocity, 4 per cell
   SceneClock clk_host{0.f, 0.032f, 1.02f, 0.088f};  // host copy of the device clock
-  // Lagrangian parcels and their two splat counters (intact, lysed; n[0] each)
+  // Lagrangian parcels and their two splat counters (intact, lysed; one per
+  // cell of the splat grid, room for splat_cap cells each)
   Parcel *parcels = nullptr;
   int np = 0, np_cap = 0;
   unsigned *splat = nullptr;
+  size_t splat_cap = 0;
   WaveDrop *drops = nullptr;  // WV_MAX_DROPS slots, then the SceneClock
   float2 *wtw = nullptr;
   // staging for upload/download (stage_n doubles)
@@ -213,6 +220,15 @@ int dev_alloc(Ctx *ctx, void **p, size_t bytes) {
   }
   ctx->allocs.push_back(*p);
   return MAC3D_OK;
+}
+
+// frees a dev_alloc'd block before the context ends (the caller drops any
+// graph that holds it first)
+void dev_free(Ctx *ctx, void *p) {
+  if (!p) return;
+  auto it = std::find(ctx->allocs.begin(), ctx->allocs.end(), p);
+  if (it != ctx->allocs.end()) ctx->allocs.erase(it);
+  (void)hipFree(p);
 }
 
 int alloc(Ctx *ctx, Buf &b, int type, size_t n) {
@@ -306,12 +322,44 @@ StepParams step_params(const Ctx *ctx, double dt) {
 // the coupling is on with the reference's own physics
 bool ref_physics(const Ctx *ctx) { return ctx->wave_on && ctx->ref_physics; }
 
-// the intact-RBC concentration the Casson pass, the buoyancy and the parcels
-// read (the dye, or the 4-channel concentration's .r), and the lysed one (.g;
-// null without it)
+// The splat grid (the concentration grid while it is on, else the MAC grid),
+// which also holds what the parcels read: the intact-RBC concentration (the
+// dye, or the 4-channel concentration's .r) and the lysed one (.g; null
+// without it).
+const Dims &splat_dims(const Ctx *ctx) { return ctx->conc_on ? ctx->dcg : ctx->d[0]; }
+size_t splat_cells(const Ctx *ctx) { return splat_dims(ctx).cells(); }
 const void *intact_conc(const Ctx *ctx) { return ctx->conc_on ? ctx->conc.ptr : ctx->dye.ptr; }
 const float *lysed_conc(const Ctx *ctx) {
-  return ctx->conc_on ? (const float *)ctx->conc.ptr + ctx->n[0] : nullptr;
+  return ctx->conc_on ? (const float *)ctx->conc.ptr + ctx->conc.n / 4 : nullptr;
+}
+// The same two as the MAC cells' Casson pass, buoyancy and unfurling gate
+// read them, and at the cells' top y-faces (null: the mean of the cell and
+// the cell above): on the MAC grid the fields themselves, from a finer
+// concentration grid the samples k_conc_at_mac forms.
+const void *intact_mac(const Ctx *ctx) {
+  return ctx->conc_on && ctx->conc_r != 1 ? ctx->cmac.ptr : intact_conc(ctx);
+}
+const float *lysed_mac(const Ctx *ctx) {
+  return ctx->conc_on && ctx->conc_r != 1 ? (const float *)ctx->cmac.ptr + ctx->n[0]
+                                          : lysed_conc(ctx);
+}
+const float *face_mac(const Ctx *ctx, int which) {
+  return ctx->conc_on && ctx->conc_r != 1 ? (const float *)ctx->cmac.ptr + (2 + which) * ctx->n[0]
+                                          : nullptr;
+}
+// the two splat counters hold at least the splat grid's cells each
+// (allocated zeroed; the caller drops graphs that hold the old block)
+int ensure_splat(Ctx *ctx) {
+  const size_t need = splat_cells(ctx);
+  if (ctx->splat && ctx->splat_cap >= need) return MAC3D_OK;
+  dev_free(ctx, ctx->splat);
+  ctx->splat = nullptr;
+  ctx->splat_cap = 0;
+  int rc;
+  if ((rc = dev_alloc(ctx, (void **)&ctx->splat, 2 * need * sizeof(unsigned)))) return rc;
+  HIPCHK(hipMemsetAsync(ctx->splat, 0, 2 * need * sizeof(unsigned), ctx->stream));
+  ctx->splat_cap = need;
+  return MAC3D_OK;
 }
 ConcScene conc_scene(const Ctx *ctx) {
   ConcScene S;
@@ -358,19 +406,20 @@ const DropPhase *drop_phases(const Ctx *ctx) {
 // One parcel step (BloodDropEngine.ts:2073-2093): both splat counters
 // cleared, then every parcel advanced and splatted.
 int launch_parcels(Ctx *ctx, double dt) {
-  const Dims &d = ctx->d[0];
+  const Dims &d = ctx->d[0], &ds = splat_dims(ctx);
   if (ctx->t0 != TF32) return fail(ctx, MAC3D_EINVAL, "parcels need the FP32 or mixed step");
-  const size_t n2 = 2 * ctx->n[0];
+  const size_t ns = ds.cells(), n2 = 2 * ns;
+  if (!ctx->splat || ctx->splat_cap < ns) return fail(ctx, MAC3D_ESTATE, "no splat counters");
   ProfScope ps(ctx->prof, ctx->stream, KC_PARCELS);
   hipLaunchKernelGGL(k_zero_u32, dim3((unsigned)((n2 / 4 + 3 + 255) / 256)), dim3(256), 0,
                      ctx->stream, ctx->splat, n2);  // (hipMalloc: 256-byte aligned)
   LAUNCHCHK();
   if (ctx->np == 0) return MAC3D_OK;
   if (ctx->emit) {  // the pending emission, then mark it consumed
-    hipLaunchKernelGGL(k_parcel_emit, dim3(kMaxEmit / 256), dim3(256), 0, ctx->stream, d,
+    hipLaunchKernelGGL(k_parcel_emit, dim3(kMaxEmit / 256), dim3(256), 0, ctx->stream, ds,
                        conc_scene(ctx), (const EmitCtl *)ctx->emit, (const WaveDrop *)ctx->drops,
                        scene_clock(ctx), (const float4 *)ctx->wave.ptr, ctx->np, ctx->parcels,
-                       ctx->splat + ctx->n[0]);
+                       ctx->splat + ns);
     hipLaunchKernelGGL(k_emit_done, dim3(1), dim3(64), 0, ctx->stream, ctx->emit);
     LAUNCHCHK();
   }
@@ -382,9 +431,9 @@ int launch_parcels(Ctx *ctx, double dt) {
   S.sim_time = ctx->clk_host.sim_time;
   S.k_lysis = ctx->clk_host.hemolysis;
   hipLaunchKernelGGL(k_parcel_advect_splat, dim3((unsigned)((ctx->np + 255) / 256)), dim3(256), 0,
-                     ctx->stream, d, S, ctx->np, ctx->parcels, (const float3 *)ctx->cvel.ptr,
+                     ctx->stream, d, ds, S, ctx->np, ctx->parcels, (const float3 *)ctx->cvel.ptr,
                      (const float *)intact_conc(ctx), lysed_conc(ctx), ctx->splat,
-                     ctx->splat + ctx->n[0]);
+                     ctx->splat + ns);
   LAUNCHCHK();
   return MAC3D_OK;
 }
@@ -441,7 +490,7 @@ int launch_casson_scene(Ctx *ctx, double dt) {
       hipLaunchKernelGGL(k_casson_scene<T>, cell_grid(d), kCellBlock, 0, ctx->stream, d,
                          wave_params_dt(ctx, dt), scene_clock(ctx),
                          (const typename q4_of<T>::type *)ctx->curl.ptr,
-                         (const T *)intact_conc(ctx), (const T *)lysed_conc(ctx),
+                         (const T *)intact_mac(ctx), (const T *)lysed_mac(ctx),
                          (T *)ctx->u.ptr, (T *)ctx->v.ptr, (T *)ctx->w.ptr);
     });
   }
@@ -455,6 +504,12 @@ int launch_casson(Ctx *ctx, double dt) {
   const bool scene = ctx->wave_on && ctx->wave_scene;
   int rc;
   if (scene && (rc = launch_curl(ctx))) return rc;
+  if (ctx->conc_on && ctx->conc_r != 1) {  // the finer grid's samples the pass reads
+    ProfScope ps(ctx->prof, ctx->stream, KC_CASSON);
+    hipLaunchKernelGGL(k_conc_at_mac, cell_grid(d), kCellBlock, 0, ctx->stream, d, ctx->dcg,
+                       (const float *)ctx->conc.ptr, (float *)ctx->cmac.ptr);
+    LAUNCHCHK();
+  }
   {
     ProfScope ps(ctx->prof, ctx->stream, KC_CASSON);
     with_step_type(ctx->t0, [&](auto tag) {
@@ -467,7 +522,8 @@ int launch_casson(Ctx *ctx, double dt) {
       auto kern = ref_physics(ctx) ? k_casson_zm<T, true> : k_casson_zm<T, false>;
       hipLaunchKernelGGL(kern, grid, dim3(CS_TX, CS_TY), 0, ctx->stream, d, P,
                          (const T *)ctx->u.ptr, (const T *)ctx->v.ptr, (const T *)ctx->w.ptr,
-                         (const T *)intact_conc(ctx), (const T *)lysed_conc(ctx),
+                         (const T *)intact_mac(ctx), (const T *)lysed_mac(ctx),
+                         (const T *)face_mac(ctx, 0), (const T *)face_mac(ctx, 1),
                          (T *)ctx->u2.ptr, (T *)ctx->v2.ptr, (T *)ctx->w2.ptr, cz, ncx, ncy);
     });
   }
@@ -612,27 +668,24 @@ int launch_dye_resolve(Ctx *ctx, double dt) {
 // CONC_PREDICT on the lysed plane (the dye predictor with the reference's
 // branches) into dpred, then CONC_RESOLVE of the four planes into the twin
 int launch_conc_predict(Ctx *ctx, double dt) {
-  const Dims &d = ctx->d[0];
+  const Dims &dc = ctx->dcg;
   ProfScope ps(ctx->prof, ctx->stream, KC_DYE);
-  const bool buf = gather_buf<float>(d);
-  auto kern = buf ? k_dye_predict<float, true, true> : k_dye_predict<float, false, true>;
-  hipLaunchKernelGGL(kern, gather_grid(d), kGatherBlock, 0, ctx->stream, d,
-                     (float)(dt / ctx->cfg.dx), (const float3 *)ctx->cvel.ptr,
-                     (const float *)ctx->conc.ptr + ctx->n[0], (float *)ctx->dpred.ptr,
-                     (float2 *)nullptr);
+  hipLaunchKernelGGL(k_conc_predict, cell_grid(dc), kCellBlock, 0, ctx->stream, dc, ctx->d[0],
+                     conc_scene(ctx), (float)dt, (const float3 *)ctx->cvel.ptr,
+                     (const float *)ctx->conc.ptr + dc.cells(), (float *)ctx->cpred.ptr);
   LAUNCHCHK();
-  ctx->dmm_valid = false;
   return MAC3D_OK;
 }
 int launch_conc_resolve(Ctx *ctx, double dt) {
-  const Dims &d = ctx->d[0];
+  const Dims &dc = ctx->dcg;
+  if (!ctx->splat || ctx->splat_cap < dc.cells()) return fail(ctx, MAC3D_ESTATE, "no splat counters");
   {
     ProfScope ps(ctx->prof, ctx->stream, KC_DYE);
-    hipLaunchKernelGGL(k_conc_resolve, cell_grid(d), kCellBlock, 0, ctx->stream, d,
+    hipLaunchKernelGGL(k_conc_resolve, cell_grid(dc), kCellBlock, 0, ctx->stream, dc, ctx->d[0],
                        conc_scene(ctx), (float)dt, (float)ctx->cfg.maccormack,
                        (const float3 *)ctx->cvel.ptr, (const float *)ctx->conc.ptr,
-                       (const float *)ctx->dpred.ptr, (const unsigned *)ctx->splat,
-                       (const unsigned *)ctx->splat + ctx->n[0], (float *)ctx->conc2.ptr);
+                       (const float *)ctx->cpred.ptr, (const unsigned *)ctx->splat,
+                       (const unsigned *)ctx->splat + dc.cells(), (float *)ctx->conc2.ptr);
   }
   LAUNCHCHK();
   std::swap(ctx->conc, ctx->conc2);
@@ -1289,6 +1342,7 @@ Buf *field_buf(Ctx *ctx, int field, int level) {
     case MAC3D_FIELD_PTOP: return l0 && ctx->wave_on ? &ctx->ptop : nullptr;
     case MAC3D_FIELD_CURL: return l0 && ctx->curl.ptr ? &ctx->curl : nullptr;
     case MAC3D_FIELD_CONC: return l0 && ctx->conc_on ? &ctx->conc : nullptr;
+    case MAC3D_FIELD_CONC_PRED: return l0 && ctx->conc_on ? &ctx->cpred : nullptr;
   }
   return nullptr;
 }
@@ -1819,7 +1873,8 @@ struct CkptSec {
   size_t bytes;
 };
 // the device ranges of a checkpoint's sections, in order
-std::vector<CkptSec> ckpt_sections(Ctx *ctx, int wave_n, int np, bool conc) {
+// (conc: the concentration grid's refinement, 0 without it)
+std::vector<CkptSec> ckpt_sections(Ctx *ctx, int wave_n, int np, int conc) {
   const size_t n0 = ctx->n[0], es = type_size(ctx->t0);
   std::vector<CkptSec> v = {{ctx->u.ptr, es * n0}, {ctx->v.ptr, es * n0}, {ctx->w.ptr, es * n0},
                             {ctx->dye.ptr, es * n0}};
@@ -1829,15 +1884,19 @@ std::vector<CkptSec> ckpt_sections(Ctx *ctx, int wave_n, int np, bool conc) {
     v.push_back({ctx->p.ptr, es * n0});
   if (wave_n > 0) v.push_back({ctx->wave.ptr, 4 * sizeof(float) * (size_t)wave_n * wave_n});
   if (np > 0) v.push_back({(void *)ctx->parcels, (size_t)np * sizeof(Parcel)});
-  if (conc) v.push_back({ctx->conc.ptr, 4 * sizeof(float) * n0});
+  if (conc) {
+    const size_t nc = (size_t)conc * conc * conc * n0;
+    v.push_back({ctx->conc.ptr, 4 * sizeof(float) * nc});
+  }
   return v;
 }
-size_t ckpt_bytes(Ctx *ctx, int wave_n, int np, bool conc) {
+size_t ckpt_bytes(Ctx *ctx, int wave_n, int np, int conc) {
   size_t b = sizeof(CkptHeader);
   for (const CkptSec &c : ckpt_sections(ctx, wave_n, np, conc)) b += c.bytes;
   return b;
 }
 int ckpt_wave_n(const Ctx *ctx) { return ctx->wave_on ? ctx->wp.n : 0; }
+int ckpt_conc(const Ctx *ctx) { return ctx->conc_on ? ctx->conc_r : 0; }
 
 }  // namespace
 
@@ -1915,7 +1974,7 @@ int mac3d_upload(void *h, int field, int level, const void *host, int dtype, siz
   size_t hsz = dtype == MAC3D_F32 ? 4 : 8;
   HIPCHK(hipMemcpyAsync(ctx->stage, host, hsz * count, hipMemcpyHostToDevice, ctx->stream));
   if (field == MAC3D_FIELD_CONC) {  // 4 values per cell -> the four planes
-    const size_t n0 = ctx->n[0];
+    const size_t n0 = b->n / 4;
     dim3 g((unsigned)((n0 + 255) / 256));
     if (dtype == MAC3D_F32)
       hipLaunchKernelGGL((k_rgba_to_planes<float, float>), g, dim3(256), 0, ctx->stream,
@@ -1968,7 +2027,7 @@ int mac3d_download(void *h, int field, int level, void *host, int dtype, size_t 
     return fail(ctx, MAC3D_EINVAL, "element count does not match the field");
   size_t hsz = dtype == MAC3D_F32 ? 4 : 8;
   if (field == MAC3D_FIELD_CONC) {  // the four planes -> 4 values per cell
-    const size_t n0 = ctx->n[0];
+    const size_t n0 = b->n / 4;
     dim3 g((unsigned)((n0 + 255) / 256));
     if (dtype == MAC3D_F32)
       hipLaunchKernelGGL((k_planes_to_rgba<float, float>), g, dim3(256), 0, ctx->stream,
@@ -2216,8 +2275,10 @@ int mac3d_parcels_upload(void *h, const float *parcels, int32_t n) {
     if ((rc = dev_alloc(ctx, (void **)&ctx->parcels, (size_t)n * sizeof(Parcel)))) return rc;
     ctx->np_cap = n;
   }
-  if (!ctx->splat && (rc = dev_alloc(ctx, (void **)&ctx->splat, 2 * ctx->n[0] * sizeof(unsigned))))
-    return rc;
+  if (!ctx->splat || ctx->splat_cap < splat_cells(ctx)) {
+    drop_graphs(ctx);  // (a captured parcel stage holds the old counters)
+    if ((rc = ensure_splat(ctx))) return rc;
+  }
   if (n)
     HIPCHK(hipMemcpyAsync(ctx->parcels, parcels, (size_t)n * sizeof(Parcel), hipMemcpyHostToDevice,
                           ctx->stream));
@@ -2229,6 +2290,7 @@ int mac3d_parcels_upload(void *h, const float *parcels, int32_t n) {
 int mac3d_conc_enable(void *h, int enable) {
   if (!h) return MAC3D_EINVAL;
   Ctx *ctx = (Ctx *)h;
+  if (enable < 0 || enable > 2) return fail(ctx, MAC3D_EINVAL, "enable is 0, 1 or 2");
   drop_graphs(ctx);  // the step's launch sequence changes
   if (!enable) {
     ctx->conc_on = false;
@@ -2238,20 +2300,39 @@ int mac3d_conc_enable(void *h, int enable) {
   if (ctx->t0 != TF32)
     return fail(ctx, MAC3D_EINVAL, "the 4-channel concentration needs the FP32 or mixed step");
   int rc;
-  const size_t n0 = ctx->n[0];
+  const Dims &d = ctx->d[0];
+  const Dims dc{d.nx * enable, d.ny * enable, d.nz * enable};
+  const size_t nc = dc.cells();
+  // the kernels index the concentration grid and its sampled planes with
+  // 32-bit cell coordinates and size_t offsets; the reference's is 9.4 M cells
+  if (dc.nz > 65535 || nc > ((size_t)1 << 32))
+    return fail(ctx, MAC3D_EINVAL, "concentration grid too large");
+  if (ctx->conc.ptr && ctx->conc_r != enable) {  // another grid: start from zero
+    Buf *bs[] = {&ctx->conc, &ctx->conc2, &ctx->cpred, &ctx->cmac};
+    for (Buf *b : bs) {
+      dev_free(ctx, b->ptr);
+      *b = Buf{};
+    }
+  }
   if (!ctx->conc.ptr) {
-    if ((rc = alloc(ctx, ctx->conc, TF32, 4 * n0))) return rc;  // (zeroed)
-    if ((rc = alloc(ctx, ctx->conc2, TF32, 4 * n0))) return rc;
+    if ((rc = alloc(ctx, ctx->conc, TF32, 4 * nc))) return rc;  // (zeroed)
+    if ((rc = alloc(ctx, ctx->conc2, TF32, 4 * nc))) return rc;
+    if ((rc = alloc(ctx, ctx->cpred, TF32, nc))) return rc;
+    if (enable != 1 && (rc = alloc(ctx, ctx->cmac, TF32, 4 * d.cells()))) return rc;
     ctx->conc_home = ctx->conc.ptr;
   }
-  if (!ctx->splat && (rc = dev_alloc(ctx, (void **)&ctx->splat, 2 * n0 * sizeof(unsigned))))
-    return rc;
-  if (4 * n0 > ctx->stage_n) {
-    if ((rc = dev_alloc(ctx, (void **)&ctx->stage, 4 * n0 * sizeof(double)))) return rc;
-    ctx->stage_n = 4 * n0;
+  ctx->conc_r = enable;
+  ctx->dcg = dc;
+  ctx->conc_on = true;
+  if ((rc = ensure_splat(ctx))) return rc;
+  if (4 * nc > ctx->stage_n) {
+    dev_free(ctx, ctx->stage);
+    ctx->stage = nullptr;
+    ctx->stage_n = 0;
+    if ((rc = dev_alloc(ctx, (void **)&ctx->stage, 4 * nc * sizeof(double)))) return rc;
+    ctx->stage_n = 4 * nc;
   }
   HIPCHK(hipStreamSynchronize(ctx->stream));
-  ctx->conc_on = true;
   return MAC3D_OK;
 }
 
@@ -2289,10 +2370,10 @@ int mac3d_parcels_download(void *h, float *parcels, int32_t n) {
 int mac3d_splat_download(void *h, int which, uint32_t *out, size_t count) {
   if (!h) return MAC3D_EINVAL;
   Ctx *ctx = (Ctx *)h;
-  if ((which != 0 && which != 1) || !out || count != ctx->n[0])
-    return fail(ctx, MAC3D_EINVAL, "which is 0 or 1 and count the cells");
-  if (!ctx->splat) return fail(ctx, MAC3D_ESTATE, "no parcels uploaded");
-  HIPCHK(hipMemcpyAsync(out, ctx->splat + which * ctx->n[0], count * sizeof(unsigned),
+  if ((which != 0 && which != 1) || !out || count != splat_cells(ctx))
+    return fail(ctx, MAC3D_EINVAL, "which is 0 or 1 and count the splat grid's cells");
+  if (!ctx->splat || ctx->splat_cap < count) return fail(ctx, MAC3D_ESTATE, "no parcels uploaded");
+  HIPCHK(hipMemcpyAsync(out, ctx->splat + which * count, count * sizeof(unsigned),
                         hipMemcpyDeviceToHost, ctx->stream));
   HIPCHK(hipStreamSynchronize(ctx->stream));
   return MAC3D_OK;
@@ -2354,15 +2435,14 @@ int mac3d_checkpoint_size(void *h, size_t *bytes) {
   if (!h) return MAC3D_EINVAL;
   Ctx *ctx = (Ctx *)h;
   if (!bytes) return fail(ctx, MAC3D_EINVAL, "null size pointer");
-  *bytes = ckpt_bytes(ctx, ckpt_wave_n(ctx), ctx->np, ctx->conc_on);
+  *bytes = ckpt_bytes(ctx, ckpt_wave_n(ctx), ctx->np, ckpt_conc(ctx));
   return MAC3D_OK;
 }
 
 int mac3d_checkpoint_save(void *h, void *host, size_t bytes) {
   if (!h) return MAC3D_EINVAL;
   Ctx *ctx = (Ctx *)h;
-  const int wn = ckpt_wave_n(ctx), np = ctx->np;
-  const bool conc = ctx->conc_on;
+  const int wn = ckpt_wave_n(ctx), np = ctx->np, conc = ckpt_conc(ctx);
   const size_t need = ckpt_bytes(ctx, wn, np, conc);
   if (!host || bytes < need)
     return fail(ctx, MAC3D_EINVAL, "checkpoint buffer is null or smaller than mac3d_checkpoint_size");
@@ -2381,7 +2461,7 @@ int mac3d_checkpoint_save(void *h, void *host, size_t bytes) {
   hd.hist = ctx->hist;
   hd.iter_n = ctx->iter_n;
   for (int q = 0; q < 4; ++q) hd.iter_log[q] = ctx->iter_log[q];
-  hd.conc = conc ? 1 : 0;
+  hd.conc = conc;
   hd.bytes = need;
   std::memcpy(host, &hd, sizeof(hd));
   char *dst = (char *)host + sizeof(hd);
@@ -2409,14 +2489,14 @@ int mac3d_checkpoint_load(void *h, const void *host, size_t bytes) {
                                    "(enable it with the same n first, or disable it)");
   if (hd.np < 0 || (hd.np > 0 && !ctx->wave_on))
     return fail(ctx, MAC3D_EINVAL, "checkpoint holds parcels: enable the wave coupling first");
-  if (hd.hist < 0 || hd.hist > 2 || hd.iter_n < 0 || hd.iter_n > 4 || hd.conc < 0 || hd.conc > 1)
+  if (hd.hist < 0 || hd.hist > 2 || hd.iter_n < 0 || hd.iter_n > 4 || hd.conc < 0 || hd.conc > 2)
     return fail(ctx, MAC3D_EINVAL, "checkpoint header is corrupt");
-  if ((hd.conc != 0) != ctx->conc_on)
+  if (hd.conc != ckpt_conc(ctx))
     return fail(ctx, MAC3D_EINVAL, "checkpoint's 4-channel concentration differs from the "
-                                   "context's (enable or disable it first)");
+                                   "context's (enable it on the same grid, or disable it, first)");
   for (int q = 0; q < 4; ++q)
     if (hd.iter_log[q] < 0) return fail(ctx, MAC3D_EINVAL, "checkpoint header is corrupt");
-  const size_t need = ckpt_bytes(ctx, hd.wave_n, hd.np, hd.conc != 0);
+  const size_t need = ckpt_bytes(ctx, hd.wave_n, hd.np, hd.conc);
   if (hd.bytes != need || bytes < need)
     return fail(ctx, MAC3D_EINVAL, "checkpoint size does not match its header");
   int rc;
@@ -2424,11 +2504,12 @@ int mac3d_checkpoint_load(void *h, const void *host, size_t bytes) {
     if ((rc = dev_alloc(ctx, (void **)&ctx->parcels, (size_t)hd.np * sizeof(Parcel)))) return rc;
     ctx->np_cap = hd.np;
   }
-  if (hd.np > 0 && !ctx->splat &&
-      (rc = dev_alloc(ctx, (void **)&ctx->splat, 2 * ctx->n[0] * sizeof(unsigned))))
-    return rc;
+  if (hd.np > 0 && (!ctx->splat || ctx->splat_cap < splat_cells(ctx))) {
+    drop_graphs(ctx);  // (a captured parcel stage holds the old counters)
+    if ((rc = ensure_splat(ctx))) return rc;
+  }
   const char *src = (const char *)host + sizeof(hd);
-  for (const CkptSec &c : ckpt_sections(ctx, hd.wave_n, hd.np, hd.conc != 0)) {
+  for (const CkptSec &c : ckpt_sections(ctx, hd.wave_n, hd.np, hd.conc)) {
     HIPCHK(hipMemcpyAsync(c.dev, src, c.bytes, hipMemcpyHostToDevice, ctx->stream));
     src += c.bytes;
   }

@@ -125,10 +125,12 @@ __device__ inline void pc_splat(const Dims &d, unsigned *__restrict__ acc, float
 }
 
 // One parcel step (PARTICLE_ADVECT_SPLAT main, particles.ts:303-462).
-// intact/lysed: the two splat counters (n cells each); dye stands for the
-// concentration's intact + lysed channels in the hemolysis exposure.
+// cvel: the centre velocity of the MAC grid d.  ds: the splat grid (the
+// concentration's, CONC_GRID, while it is on; else d), which holds the two
+// counters intact/lysed and the hemolysis exposure: dye, standing for the
+// concentration's intact channel, plus lysed_conc (nullable).
 __global__ void __launch_bounds__(256)
-k_parcel_advect_splat(Dims d, ParcelScene S, int np, Parcel *__restrict__ parcels,
+k_parcel_advect_splat(Dims d, Dims ds, ParcelScene S, int np, Parcel *__restrict__ parcels,
                       const float3 *__restrict__ cvel, const float *__restrict__ dye,
                       const float *__restrict__ lysed_conc, unsigned *__restrict__ intact,
                       unsigned *__restrict__ lysed) {
@@ -225,7 +227,7 @@ k_parcel_advect_splat(Dims d, ParcelScene S, int np, Parcel *__restrict__ parcel
     const float dm = mass * clampv(w * dt * 1.65f, 0.f, 0.095f);
     if (dm > 1e-7f) {
       mass = fmaxf(0.f, mass - dm);
-      if (dm * 0.68f > 1e-7f) pc_splat(d, lysed, uc, dm * 0.68f);
+      if (dm * 0.68f > 1e-7f) pc_splat(ds, lysed, uc, dm * 0.68f);
     }
     if (mass <= 1e-5f) {
       P.pm.w = 0.f;
@@ -242,8 +244,8 @@ k_parcel_advect_splat(Dims d, ParcelScene S, int np, Parcel *__restrict__ parcel
   }
   // cSample.r + cSample.g (particles.ts:403-404): the dye (the intact
   // concentration) plus the lysed one when the 4-channel concentration is on
-  float cn = sample_cell<float>(dye, d, uc.x, uc.y, uc.z);
-  if (lysed_conc) cn += sample_cell<float>(lysed_conc, d, uc.x, uc.y, uc.z);
+  float cn = sample_cell<float>(dye, ds, uc.x, uc.y, uc.z);
+  if (lysed_conc) cn += sample_cell<float>(lysed_conc, ds, uc.x, uc.y, uc.z);
   const float dilute = 1.f - pc_ss(0.012f, 0.22f, cn);
   const float disp = pc_ss(1.1f, 7.8f, age + dilute * 2.2f + lysis * 2.5f);
   const float late = pc_ss(7.2f, 16.f, age) * dilute;
@@ -260,7 +262,7 @@ k_parcel_advect_splat(Dims d, ParcelScene S, int np, Parcel *__restrict__ parcel
     const float dm = fminf(headroom, mass * clampv(frag * S.k_lysis * 1.95f * dt, 0.f, 0.16f));
     if (dm > 1e-7f) {
       mass = fmaxf(0.f, mass - dm);
-      pc_splat(d, lysed, uc, dm);
+      pc_splat(ds, lysed, uc, dm);
     }
   }
   P.pm = make_float4(pos.x, pos.y, pos.z, mass);
@@ -271,7 +273,7 @@ k_parcel_advect_splat(Dims d, ParcelScene S, int np, Parcel *__restrict__ parcel
   // tapered ribbon along the stretch and a sheet tangent (particles.ts:430-461)
   const float sl = sqrtf(st.x * st.x + st.y * st.y + st.z * st.z);
   if (sl < 0.0038f) {
-    if (mass > 1e-6f) pc_splat(d, intact, uc, mass);
+    if (mass > 1e-6f) pc_splat(ds, intact, uc, mass);
     return;
   }
   const float sxz = sqrtf(st.x * st.x + st.z * st.z);
@@ -300,7 +302,7 @@ k_parcel_advect_splat(Dims d, ParcelScene S, int np, Parcel *__restrict__ parcel
     const float3 pq = make_float3(pos.x + a[q] * st.x + b[q] * sv.x, pos.y + a[q] * st.y + b[q] * sv.y,
                                   pos.z + a[q] * st.z + b[q] * sv.z);
     const float mq = wt[q] * mass;
-    if (mq > 1e-6f) pc_splat(d, intact, pc_clamp_uvw(pc_uvw(S, pq)), mq);
+    if (mq > 1e-6f) pc_splat(ds, intact, pc_clamp_uvw(pc_uvw(S, pq)), mq);
   }
 }
 

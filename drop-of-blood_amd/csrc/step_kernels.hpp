@@ -200,11 +200,11 @@ __device__ inline T face_w(const Dims &d, const T *__restrict__ w, int i, int j,
 // One cell of the Casson update and buoyancy (fluid.ts:451-486 with the
 // workload's SI Casson law): m the cell's three faces, x/y/z p/n the face
 // values one cell up/down each axis (wall ghosts applied), dc the cell's
-// dye, dup the dye one row up (the cell's own at the top row).
+// dye, df the dye at its top y-face.
 template <class T>
 __device__ inline void casson_cell(const StepParams &P, T m0, T m1, T m2, T xp0, T xp1, T xp2,
                                    T xn0, T xn1, T xn2, T yp0, T yp1, T yp2, T yn0, T yn1, T yn2,
-                                   T zp0, T zp1, T zp2, T zn0, T zn1, T zn2, T dc, T dup, T &o0,
+                                   T zp0, T zp1, T zp2, T zn0, T zn1, T zn2, T dc, T df, T &o0,
                                    T &o1, T &o2) {
   const T inv2h = T(1.0 / (2.0 * P.dx));
   const T dtoh2 = T(P.dt / (P.dx * P.dx));
@@ -231,8 +231,7 @@ __device__ inline void casson_cell(const StepParams &P, T m0, T m1, T m2, T xp0,
   o0 = m0 + alpha * ((xp0 + xn0 + yp0 + yn0 + zp0 + zn0) - T(6) * m0);
   o1 = m1 + alpha * ((xp1 + xn1 + yp1 + yn1 + zp1 + zn1) - T(6) * m1);
   o2 = m2 + alpha * ((xp2 + xn2 + yp2 + yn2 + zp2 + zn2) - T(6) * m2);
-  T dyef = T(0.5) * (dc + dup);
-  o1 += buoy * dyef;
+  o1 += buoy * df;
 }
 
 // One cell of FLUID_CASSON_FORCES parts (1) and (3) with the reference's own
@@ -240,17 +239,17 @@ __device__ inline void casson_cell(const StepParams &P, T m0, T m1, T m2, T xp0,
 //   gamma = 50 sqrt(|du|^2 + |dv|^2 + |dw|^2 + 1e-6)        (fluid.ts:451-454)
 //   alpha = clamp((sqrt(1e-4 (1 + 1.35 c)) + sqrt(0.0036 scale c^2 /
 //           max(gamma + 0.4, 0.4)))^2, 1e-4, 0.0044)      (fluid.ts:458-462)
-// with c = clamp(dye, 0, 1.5) and no dt (a per-step rate), then the
-// tanh-saturated buoyancy of the face concentration rho = 1.24 c_face + 0.26
-// lysed_face (lc, lup: the lysed concentration, 0 without it), gated
-// by the face's height under the surface (fluid.ts:477-486); the neighbours
-// come clamped to the edge (fluid.ts:387-392).  j: the cell's row.
+// with c = clamp(dc, 0, 1.5) and no dt (a per-step rate), then the
+// tanh-saturated buoyancy of the face concentration rho = 1.24 fr + 0.26 fg
+// (fr, fg: the intact and lysed concentration at the cell's top y-face, fg 0
+// without the 4-channel concentration), gated by the face's height under the
+// surface (fluid.ts:477-486); the neighbours come clamped to the edge
+// (fluid.ts:387-392).  j: the cell's row.
 template <class T>
 __device__ inline void casson_cell_ref(const StepParams &P, const Dims &d, int j, T m0, T m1, T m2,
                                        T xp0, T xp1, T xp2, T xn0, T xn1, T xn2, T yp0, T yp1,
                                        T yp2, T yn0, T yn1, T yn2, T zp0, T zp1, T zp2, T zn0,
-                                       T zn1, T zn2, T dc, T dup, T lc, T lup, T &o0, T &o1,
-                                       T &o2) {
+                                       T zn1, T zn2, T dc, T fr, T fg, T &o0, T &o1, T &o2) {
   T ss = T(0);
   {
     T a = xp0 - xn0, b = yp0 - yn0, e = zp0 - zn0;
@@ -270,7 +269,7 @@ __device__ inline void casson_cell_ref(const StepParams &P, const Dims &d, int j
   o0 = m0 + alpha * ((xp0 + xn0 + yp0 + yn0 + zp0 + zn0) - T(6) * m0);
   o1 = m1 + alpha * ((xp1 + xn1 + yp1 + yn1 + zp1 + zn1) - T(6) * m1);
   o2 = m2 + alpha * ((xp2 + xn2 + yp2 + yn2 + zp2 + zn2) - T(6) * m2);
-  const T rho = T(1.24) * (T(0.5) * (dc + dup)) + T(0.26) * (T(0.5) * (lc + lup));
+  const T rho = T(1.24) * fr + T(0.26) * fg;
   if (rho > T(0.0008)) {
     const T yf = T(P.surface_y - P.depth + (j + 1.0) / d.ny * P.depth);
     const T below = T(P.surface_y) - yf;
@@ -316,8 +315,8 @@ template <class T, bool REF>
 __global__ void __launch_bounds__(CS_T, sizeof(T) == 4 ? 6 : 1)
 k_casson_zm(Dims d, StepParams P, const T *__restrict__ u, const T *__restrict__ v,
             const T *__restrict__ w, const T *__restrict__ dye, const T *__restrict__ lys,
-            T *__restrict__ uo, T *__restrict__ vo, T *__restrict__ wo, int cz, int ncx,
-            int ncy) {
+            const T *__restrict__ dface, const T *__restrict__ lface, T *__restrict__ uo,
+            T *__restrict__ vo, T *__restrict__ wo, int cz, int ncx, int ncy) {
   __shared__ T su[4][CS_EY][CS_EX], sv[4][CS_EY][CS_EX], sw[4][CS_EY][CS_EX];
   const int nx = d.nx, ny = d.ny, nz = d.nz;
   const int lb = xcd_remap(blockIdx.x, gridDim.x);  // one z-chunk of tiles per XCD
@@ -382,25 +381,31 @@ k_casson_zm(Dims d, StepParams P, const T *__restrict__ u, const T *__restrict__
   fetch(k0);
   stage(k0);
   fetch(k0 + 1);
-  T dn = dye[(size_t)k0 * plane + cc0], dnu = dye[(size_t)k0 * plane + cc0 + up];
-  // the lysed concentration (REF with the 4-channel concentration; else 0)
+  // the concentration at the cell (dye; the lysed one, lys, with REF and the
+  // 4-channel concentration, else 0) and at its top y-face: the face planes
+  // when given (samples of a finer concentration grid), else the mean of the
+  // cell and the cell above
+  const T *fd = dface ? dface : dye, *fl = dface ? lface : lys;
+  const size_t fo = dface ? 0 : up;
+  T dn = dye[(size_t)k0 * plane + cc0], dnu = fd[(size_t)k0 * plane + cc0 + fo];
   T ln = T(0), lnu = T(0);
   if (REF && lys) {
     ln = lys[(size_t)k0 * plane + cc0];
-    lnu = lys[(size_t)k0 * plane + cc0 + up];
+    lnu = fl[(size_t)k0 * plane + cc0 + fo];
   }
   const int r = ty + 1, c = tx + 1;
   for (int k = k0; k < k1; ++k) {
     stage(k + 1);
-    T dc = dn, dup = dnu, lc = ln, lup = lnu;
+    const T dc = dn;
+    const T fr = dface ? dnu : T(0.5) * (dn + dnu), fg = dface ? lnu : T(0.5) * (ln + lnu);
     __syncthreads();
     fetch(k + 2);
     if (k + 1 < k1) {
       dn = dye[(size_t)(k + 1) * plane + cc0];
-      dnu = dye[(size_t)(k + 1) * plane + cc0 + up];
+      dnu = fd[(size_t)(k + 1) * plane + cc0 + fo];
       if (REF && lys) {
         ln = lys[(size_t)(k + 1) * plane + cc0];
-        lnu = lys[(size_t)(k + 1) * plane + cc0 + up];
+        lnu = fl[(size_t)(k + 1) * plane + cc0 + fo];
       }
     }
     if (inside) {
@@ -413,8 +418,7 @@ k_casson_zm(Dims d, StepParams P, const T *__restrict__ u, const T *__restrict__
                            su[s0][r + 1][c], sv[s0][r + 1][c], sw[s0][r + 1][c],
                            su[s0][r - 1][c], sv[s0][r - 1][c], sw[s0][r - 1][c],
                            su[sp][r][c], sv[sp][r][c], sw[sp][r][c],
-                           su[sm][r][c], sv[sm][r][c], sw[sm][r][c], dc, dup, lc, lup, o0, o1,
-                           o2);
+                           su[sm][r][c], sv[sm][r][c], sw[sm][r][c], dc, fr, fg, o0, o1, o2);
       else
       casson_cell<T>(P, su[s0][r][c], sv[s0][r][c], sw[s0][r][c],
                      su[s0][r][c + 1], sv[s0][r][c + 1], sw[s0][r][c + 1],
@@ -422,7 +426,7 @@ k_casson_zm(Dims d, StepParams P, const T *__restrict__ u, const T *__restrict__
                      su[s0][r + 1][c], sv[s0][r + 1][c], sw[s0][r + 1][c],
                      su[s0][r - 1][c], sv[s0][r - 1][c], sw[s0][r - 1][c],
                      su[sp][r][c], sv[sp][r][c], sw[sp][r][c],
-                     su[sm][r][c], sv[sm][r][c], sw[sm][r][c], dc, dup, o0, o1, o2);
+                     su[sm][r][c], sv[sm][r][c], sw[sm][r][c], dc, fr, o0, o1, o2);
       const size_t ci = (size_t)k * plane + cc0;
       uo[ci] = (gx >= nx - 1) ? T(0) : o0;
       vo[ci] = o1;

@@ -24,7 +24,8 @@ export interface StepStats {
   maxDiv: number; l2Div: number; maxDivRate: number; relRes: number; ms: number;
 }
 export declare const FIELD: Readonly<{ U: 0; V: 1; W: 2; P: 3; DYE: 4; CU: 5; CV: 6; CW: 7;
-  RHS: 8; MGX: 9; MGB: 10; MGR: 11; DYE_PRED: 12; WAVE: 13; PTOP: 14; CURL: 15; CONC: 16 }>;
+  RHS: 8; MGX: 9; MGB: 10; MGR: 11; DYE_PRED: 12; WAVE: 13; PTOP: 14; CURL: 15; CONC: 16;
+  CONC_PRED: 17 }>;
 export declare const PRECISION: Readonly<{ fp32: 0; mixed: 1; fp64: 2 }>;
 export declare function version(): string;
 /** MultiDropSystem.spawnAerialDrop's parameters (scene metres, m/s) */
@@ -48,7 +49,9 @@ export declare class MacSolver {
   setUniforms(block: Float32Array): void;
   parcelsUpload(parcels: Float32Array): void;
   parcelsEmit(cursor: number, count: number): void;
-  concEnable(on?: boolean): void;
+  concEnable(on?: boolean, refine?: 1 | 2): void;
+  readonly concCells: number;
+  concRefine: 0 | 1 | 2;
   parcelsStep(dt: number): void;
   parcelsDownload(out: Float32Array): Float32Array;
   splatDownload(which: 0 | 1, out: Uint32Array): Uint32Array;

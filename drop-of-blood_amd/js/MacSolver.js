@@ -20,7 +20,8 @@ const path = require('path');
 const addon = require(path.join(__dirname, '..', 'lib', 'mac3d.node'));
 
 const FIELD = Object.freeze({ U: 0, V: 1, W: 2, P: 3, DYE: 4, CU: 5, CV: 6, CW: 7, RHS: 8,
-  MGX: 9, MGB: 10, MGR: 11, DYE_PRED: 12, WAVE: 13, PTOP: 14, CURL: 15, CONC: 16 });
+  MGX: 9, MGB: 10, MGR: 11, DYE_PRED: 12, WAVE: 13, PTOP: 14, CURL: 15, CONC: 16,
+  CONC_PRED: 17 });
 const PRECISION = Object.freeze({ fp32: 0, mixed: 1, fp64: 2 });
 
 /**
@@ -53,6 +54,8 @@ class MacSolver {
     this.w = new Float32Array(this.cells);
     this.p = new Float32Array(this.cells);
     this.dye = new Float32Array(this.cells);
+    /** the concentration grid per axis (concEnable), 0 while it is off */
+    this.concRefine = 0;
     this.destroyed = false;
   }
 
@@ -116,7 +119,7 @@ class MacSolver {
     return out;
   }
 
-  /** @param {0|1} which intact or lysed @param {Uint32Array} out one per cell */
+  /** @param {0|1} which intact or lysed @param {Uint32Array} out one per cell of the splat grid (concCells) */
   splatDownload(which, out) {
     addon.splatDownload(this._live(), which, out);
     return out;
@@ -133,13 +136,23 @@ class MacSolver {
   }
 
   /**
-   * The reference's 4-channel concentration (field CONC, 4 per cell): each
-   * step then runs pass B (parcel emission, advection and splat,
-   * CONC_PREDICT, CONC_RESOLVE) after the projection.
-   * @param {boolean} on
+   * The reference's 4-channel concentration (field CONC, 4 per cell of its
+   * grid): each step then runs pass B (parcel emission, advection and splat,
+   * CONC_PREDICT, CONC_RESOLVE) after the projection.  refine 1 holds it on
+   * the MAC grid, 2 on a grid twice as fine per axis (the reference's
+   * CONC_GRID over its VEL_GRID); the parcels splat into that grid.
+   * @param {boolean} on @param {1|2} [refine]
    */
-  concEnable(on = true) {
-    addon.concEnable(this._live(), on);
+  concEnable(on = true, refine = 1) {
+    if (refine !== 1 && refine !== 2) throw new RangeError('refine is 1 or 2');
+    addon.concEnable(this._live(), on ? refine : 0);
+    this.concRefine = on ? refine : 0;
+  }
+
+  /** cells of the concentration grid (the splat grid; the MAC grid while it is off) */
+  get concCells() {
+    const r = Math.max(this.concRefine || 0, 1);
+    return this.cells * r * r * r;
   }
 
   /**

@@ -13,7 +13,7 @@ LIB_PATH = os.path.join(PKG, "lib", "libmac3d.so")
 F32, F64 = 0, 1
 PREC = {"fp32": 0, "mixed": 1, "fp64": 2}
 FIELD = dict(U=0, V=1, W=2, P=3, DYE=4, CU=5, CV=6, CW=7, RHS=8, MGX=9, MGB=10, MGR=11,
-             DYE_PRED=12, WAVE=13, PTOP=14, CURL=15, CONC=16)
+             DYE_PRED=12, WAVE=13, PTOP=14, CURL=15, CONC=16, CONC_PRED=17)
 STAGE = dict(FACE_TO_CENTRE=0, ADVECT=1, CASSON=2, DIVERGENCE=3, VCYCLE=4, PROJECT=5,
              DYE_PREDICT=6, DYE_RESOLVE=7, WAVE=8, TOP_PRESSURE=9, PARCELS=10, CONC_PREDICT=11,
              CONC_RESOLVE=12)
@@ -253,6 +253,7 @@ class MacSolver:
             raise Mac3dError(rc, self.L.mac3d_last_error(None).decode())
         self.h = h
         self.shape = (self.cfg.nz, self.cfg.ny, self.cfg.nx)
+        self.conc_refine = 0  # the concentration grid per axis, 0 while it is off
 
     # -- lifetime
     def close(self):
@@ -287,9 +288,19 @@ class MacSolver:
             return (n, n, 4)
         if fid == FIELD["PTOP"]:
             return (self.cfg.nz, self.cfg.nx)
-        if fid in (FIELD["CURL"], FIELD["CONC"]):
+        if fid == FIELD["CURL"]:
             return self.level_shape(0) + (4,)
+        if fid == FIELD["CONC"]:
+            return self.conc_shape() + (4,)
+        if fid == FIELD["CONC_PRED"]:
+            return self.conc_shape()
         return self.level_shape(level)
+
+    def conc_shape(self):
+        """(nz, ny, nx) of the concentration grid (the splat grid while it is
+        on; else the MAC grid)"""
+        r = max(self.conc_refine, 1)
+        return tuple(r * a for a in self.shape)
 
     def download(self, field, level: int = 0, dtype=np.float64) -> np.ndarray:
         fid = FIELD[field] if isinstance(field, str) else field
@@ -371,10 +382,14 @@ class MacSolver:
         slot `cursor` (PARTICLE_EMIT)"""
         self._check(self.L.mac3d_parcels_emit(self.h, int(cursor), int(count)))
 
-    def conc_enable(self, enable: bool = True):
+    def conc_enable(self, enable: bool = True, refine: int = 1):
         """the reference's 4-channel concentration (field CONC; pass B in the
-        step: parcels, CONC_PREDICT, CONC_RESOLVE)"""
-        self._check(self.L.mac3d_conc_enable(self.h, int(enable)))
+        step: parcels, CONC_PREDICT, CONC_RESOLVE) on the MAC grid (refine 1)
+        or a grid twice as fine per axis (refine 2, as the reference holds
+        it); a change of grid starts from zero"""
+        e = int(refine) if enable else 0
+        self._check(self.L.mac3d_conc_enable(self.h, e))
+        self.conc_refine = e
 
     def parcels_download(self, n=None) -> np.ndarray:
         n = self.n_parcels if n is None else n
@@ -383,9 +398,10 @@ class MacSolver:
         return out
 
     def splat_download(self, which) -> np.ndarray:
-        """the splat counters (nz, ny, nx) uint32: which 0 or "INTACT", 1 or "LYSED"."""
+        """the splat counters conc_shape() uint32: which 0 or "INTACT", 1 or
+        "LYSED"."""
         w = {"INTACT": 0, "LYSED": 1}.get(which, which)
-        out = np.empty(self.level_shape(0), dtype=np.uint32)
+        out = np.empty(self.conc_shape(), dtype=np.uint32)
         self._check(self.L.mac3d_splat_download(self.h, int(w), out.ctypes.data, out.size))
         return out
 

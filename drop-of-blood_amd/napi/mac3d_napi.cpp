@@ -323,7 +323,7 @@ napi_value ParcelsDownload(napi_env env, napi_callback_info info) {
   return nullptr;
 }
 
-// splatDownload(h, which, Uint32Array(cells)): 0 intact, 1 lysed
+// splatDownload(h, which, Uint32Array(splat grid cells)): 0 intact, 1 lysed
 napi_value SplatDownload(napi_env env, napi_callback_info info) {
   napi_value argv[3];
   if (!get_args(env, info, 3, argv)) return nullptr;
@@ -645,18 +645,24 @@ napi_value FitRelease(napi_env env, napi_callback_info info) {
   return out;
 }
 
-// concEnable(h, on)
+// concEnable(h, on): false/0 off, true/1 on the MAC grid, 2 on the grid
+// twice as fine per axis
 napi_value ConcEnable(napi_env env, napi_callback_info info) {
   napi_value argv[2];
   if (!get_args(env, info, 2, argv)) return nullptr;
   void *ctx = get_handle(env, argv[0]);
   if (!ctx) return nullptr;
-  bool on = false;
+  int e = 0;
   napi_valuetype t;
   NAPI_CALL(env, napi_typeof(env, argv[1], &t));
-  if (t == napi_boolean) napi_get_value_bool(env, argv[1], &on);
-  else on = get_int(env, argv[1]) != 0;
-  int rc = mac3d_conc_enable(ctx, on ? 1 : 0);
+  if (t == napi_boolean) {
+    bool on = false;
+    napi_get_value_bool(env, argv[1], &on);
+    e = on ? 1 : 0;
+  } else {
+    e = get_int(env, argv[1]);
+  }
+  int rc = mac3d_conc_enable(ctx, e);
   if (rc) return throw_status(env, ctx, rc);
   return nullptr;
 }

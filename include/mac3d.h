@@ -75,11 +75,14 @@ extern "C" {
                                4 per cell (omega x, y, z, |omega|; grid units, 1/s
                                times h), FLUID_CURL (fluid.ts:331-369) */
 #define MAC3D_FIELD_CONC 16 /* the reference's 4-channel concentration (needs
-                               mac3d_conc_enable): 4 per cell at 4*cell: intact RBC
-                               (.r), lysed Hb (.g), oxygenation/shear state (.b), floor
-                               bed (.a), the layout of its rgba texture
-                               (advect.ts:75-79) */
-#define MAC3D_FIELD_COUNT 17
+                               mac3d_conc_enable): 4 per cell of the concentration
+                               grid at 4*cell: intact RBC (.r), lysed Hb (.g),
+                               oxygenation/shear state (.b), floor bed (.a), the
+                               layout of its rgba texture (advect.ts:75-79) */
+#define MAC3D_FIELD_CONC_PRED 17 /* the lysed channel's semi-Lagrangian predictor
+                                    (CONC_PREDICT's concPredOut .g), one per cell of
+                                    the concentration grid */
+#define MAC3D_FIELD_COUNT 18
 
 /* single stages of the step, for kernel-level parity tests */
 #define MAC3D_STAGE_FACE_TO_CENTRE 0 /* CU,CV,CW <- U,V,W */
@@ -437,8 +440,15 @@ int mac3d_parcels_upload(void *ctx, const float *parcels, int32_t n);
  * the coupling or (count > 0) without uploaded parcels. */
 int mac3d_parcels_emit(void *ctx, int32_t cursor, int32_t count);
 /* no reference counterpart as a call (the reference always runs its pass B):
- * turns on or off the reference's 4-channel concentration at the MAC grid
- * (MAC3D_FIELD_CONC, zero at first).  While on, each step runs the reference's
+ * turns on or off the reference's 4-channel concentration (MAC3D_FIELD_CONC,
+ * zero at first) on its own grid: enable 1, the MAC grid; enable 2, a grid
+ * twice as fine per axis, as the reference holds it (CONC_GRID 192x256x192
+ * over VEL_GRID 96x128x96, common.ts:4-15), which the kernels address in uvw
+ * and sample the MAC grid's centre velocity at, and whose channels the Casson
+ * pass reads as samples at the MAC cell centres and y-faces (fluid.ts:446-447,
+ * 477-478); enable 0, off.  A change of grid starts from zero.  The parcels
+ * splat into the concentration grid while it is on (else the MAC grid).
+ * While on, each step runs the reference's
  * pass B after the projection (BloodDropEngine.ts:2073-2108): the parcel stage
  * (counters cleared, pending emission, advection and splat), CONC_PREDICT of
  * the lysed channel and CONC_RESOLVE (advect.ts:28-298) in place of the dye
@@ -448,7 +458,8 @@ int mac3d_parcels_emit(void *ctx, int32_t cursor, int32_t count);
 int mac3d_conc_enable(void *ctx, int enable);
 /* the parcels after MAC3D_STAGE_PARCELS (n <= the uploaded count) */
 int mac3d_parcels_download(void *ctx, float *parcels, int32_t n);
-/* the splat counters: which 0 intact, 1 lysed; count = cells */
+/* the splat counters: which 0 intact, 1 lysed; count = the splat grid's cells
+ * (the concentration grid while it is on, else the MAC grid) */
 int mac3d_splat_download(void *ctx, int which, uint32_t *out, size_t count);
 
 /* one stage of the step (MAC3D_STAGE_*), blocking */

@@ -70,6 +70,31 @@ if (process.argv[3] === 'parcels') {
   s4.destroy();
 }
 
+// the 4-channel concentration on the grid twice as fine, through the same
+// host: coupled steps with emission; checksums of the fields, the
+// concentration (4 per fine cell) and both fine-grid splat counters
+if (process.argv[3] === 'conc2') {
+  const s5 = new MacSolver({ nx: w.nx, ny: w.ny, nz: w.nz, dx: w.dx, levels: w.levels,
+    cycles: w.cycles, precision: 0 });
+  s5.initBlobs(w.blobs);
+  s5.waveEnable({ n: 64, halfExtent: 0.25, halfXZ: 0.1, depth: 0.2, surfaceY: 0.0,
+    marangoni: 0.8, scene: 1, physics: 1 });
+  s5.setDrops([{ x: 0.004, y: -0.003, z: -0.002, radius: 0.02, velY: -0.4, drain: 0.85, seed: 1.3 }]);
+  s5.concEnable(true, 2);
+  s5.parcelsUpload(new Float32Array(8 * 4096));
+  for (let q = 0; q < steps; q++) {
+    s5.parcelsEmit(300 * q, 300);
+    s5.step(w.dt, 1);
+  }
+  s5.sync();
+  const nc = s5.concCells;
+  const conc = s5.read('CONC', new Float64Array(4 * nc));
+  const a = s5.splatDownload(0, new Uint32Array(nc)), b = s5.splatDownload(1, new Uint32Array(nc));
+  console.log(JSON.stringify({ conc2: { cells: nc, v: sum(s5.v), conc: sum(conc),
+    intact: a.reduce((x, y) => x + y, 0), lysed: b.reduce((x, y) => x + y, 0) } }));
+  s5.destroy();
+}
+
 if (process.argv[3] === 'backward') {
   const s2 = new MacSolver({ nx: w.nx, ny: w.ny, nz: w.nz, dx: w.dx, levels: w.levels,
     cycles: w.cycles, precision: 2 });

@@ -30,12 +30,14 @@ DROPS = [(0.004, -0.003, -0.002, 0.020, -0.4, 0.0, 0.85, 1.3),
          (-0.045, 0.010, 0.030, 0.018, -0.2, 0.0, 0.30, 2.1)]
 
 
-def conc_pair(prec=0, physics=1, scene=1, ring=4096, warm=0):
+def conc_pair(prec=0, physics=1, scene=1, ring=4096, warm=0, refine=1):
+    """the tiny workload's solver and oracle with the coupling, two drops
+    and the 4-channel concentration on its refine-times grid"""
     w, s, o = pair(precision=prec, warm_steps=warm)
     for x in (s, o):
         x.wave_enable(n=64, scene=scene, physics=physics, **GEOM)
         x.set_drops(DROPS)
-        x.conc_enable(True)
+        x.conc_enable(True, refine=refine)
     P = np.zeros((ring, 8))
     s.parcels_upload(P)
     o.parcels_set(P)
@@ -51,13 +53,15 @@ def _parcels_close(a, b, pos_tol=2e-6):
     assert dp <= pos_tol and dm <= 1e-6
 
 
-def test_emission_against_oracle():
+@pytest.mark.parametrize("refine", [1, 2])
+def test_emission_against_oracle(refine):
     """600 emissions into a 4096-slot ring from slot 3900 (wrapping), two
     drops sharing by drain rate, over a ring whose written slots hold 50 live
     parcels: the new parcels' positions within 2 um of the oracle's, masses,
     stretch and lysis to FP32 rounding, the transfers into the lysed counter
-    equal, and the untouched slots unchanged."""
-    w, s, o = conc_pair()
+    (on the concentration grid, the MAC grid's or twice as fine) equal, and
+    the untouched slots unchanged."""
+    w, s, o = conc_pair(refine=refine)
     rng = np.random.default_rng(4)
     P = np.zeros((4096, 8))
     live = (3900 + rng.choice(600, 50, replace=False)) % 4096
@@ -92,22 +96,27 @@ def test_emission_against_oracle():
     o.close()
 
 
-@pytest.mark.parametrize("prec", [0])
-def test_resolve_against_oracle(prec):
+@pytest.mark.parametrize("refine", [1, 2])
+def test_resolve_against_oracle(refine):
     """Predict + resolve of a smooth random 4-channel state in a swirling
     flow, with the counters of 3000 splatted parcels (intact and lysed, floor
     bed near the bottom): intact, lysed and bed within 1e-5 relative L2; the
     oxygenation/shear channel within 1e-4 -- it is a chain of smoothsteps of
     the convective shear |v0 - v_mid| / (dt/2), which divides an FP32
-    difference of two velocity samples by half a step (measured 3.6e-5)."""
-    w, s, o = conc_pair(prec)
+    difference of two velocity samples by half a step (measured 3.6e-5).
+    On the grid twice as fine the lysed channel's predictor is compared as
+    well (its own field), at 1e-5."""
+    w, s, o = conc_pair(refine=refine)
     rng = np.random.default_rng(7)
+    nz, ny, nx = o.splat_shape()
+    k, j, i = np.meshgrid(np.arange(nz), np.arange(ny), np.arange(nx), indexing="ij")
+    i, j, k = i / refine, j / refine, k / refine  # the same fields in MAC-cell units
+    C = np.stack([0.2 * (1 + np.sin(i / 3.0) * np.cos(j / 5.0)) * (k > 2) * (k < nz / refine - 3),
+                  0.05 * (1 + np.cos(i / 4.0 + j / 6.0)),
+                  np.zeros(i.shape),
+                  0.3 * np.exp(-0.5 * j) * (1 + np.sin(k / 4.0))], axis=-1)
     nz, ny, nx = o.shape
     k, j, i = np.meshgrid(np.arange(nz), np.arange(ny), np.arange(nx), indexing="ij")
-    C = np.stack([0.2 * (1 + np.sin(i / 3.0) * np.cos(j / 5.0)) * (k > 2) * (k < nz - 3),
-                  0.05 * (1 + np.cos(i / 4.0 + j / 6.0)),
-                  np.zeros(o.shape),
-                  0.3 * np.exp(-0.5 * j) * (1 + np.sin(k / 4.0))], axis=-1)
     P = np.zeros((3000, 8))
     P[:, 0], P[:, 2] = rng.uniform(-0.05, 0.05, (2, 3000))
     P[:, 1] = rng.uniform(-0.195, -0.02, 3000)
@@ -130,6 +139,9 @@ def test_resolve_against_oracle(prec):
         o.stage(oracle.ST_PARCELS, 1e-3)
         s.stage("CONC_PREDICT", 1e-3)
         o.stage(oracle.ST_CONC_PRED, 1e-3)
+        e = rel_l2(s.download("CONC_PRED"), o.get(oracle.F_CONC_PRED))
+        print(f"[conc] predictor: rel L2 {e:.3e}", flush=True)
+        assert e <= 1e-5
         s.stage("CONC_RESOLVE", 1e-3)
         o.stage(oracle.ST_CONC_RES, 1e-3)
         got = s.download("CONC")
@@ -143,12 +155,14 @@ def test_resolve_against_oracle(prec):
     o.close()
 
 
-def test_steps_with_the_concentration_against_oracle():
+@pytest.mark.parametrize("refine", [1, 2])
+def test_steps_with_the_concentration_against_oracle(refine):
     """Three full FP32 steps of the reference scene with the 4-channel
-    concentration: wave, drops, scene terms and reference physics, 400
-    emissions per step into a 4096-slot ring, parcels, predict and resolve;
-    the Casson pass, buoyancy and hemolysis read the concentration."""
-    w, s, o = conc_pair(0, warm=0)
+    concentration on the MAC grid or twice as fine: wave, drops, scene terms
+    and reference physics, 400 emissions per step into a 4096-slot ring,
+    parcels, predict and resolve; the Casson pass, buoyancy and hemolysis
+    read the concentration."""
+    w, s, o = conc_pair(0, warm=0, refine=refine)
     with s:
         for step in range(3):
             for x in (s, o):
@@ -195,6 +209,28 @@ def test_conc_calls_and_checkpoint():
         s.upload("CONC", C * 0)
         s.checkpoint_load(ck)
         assert np.allclose(s.download("CONC"), C.astype(np.float32))
+    with MacSolver(config_for(w)) as s:
+        # on the grid twice as fine: its own field sizes, counters and
+        # checkpoint section; a checkpoint of one grid does not load into the
+        # other; a change of grid starts from zero
+        s.wave_enable(n=64, **GEOM)
+        s.conc_enable(True, refine=2)
+        assert s.conc_shape() == tuple(2 * a for a in s.shape)
+        s.parcels_upload(np.zeros((16, 8)))
+        assert s.splat_download(0).shape == s.conc_shape()
+        C = np.random.default_rng(2).random(s.conc_shape() + (4,)) * 0.1
+        s.upload("CONC", C)
+        ck = s.checkpoint_save()
+        assert ck.nbytes > 8 * 4 * int(np.prod(s.shape)) * 4
+        s.conc_enable(True, refine=1)
+        assert np.all(s.download("CONC") == 0)
+        with pytest.raises(Mac3dError, match="EINVAL"):
+            s.checkpoint_load(ck)
+        s.conc_enable(True, refine=2)
+        s.checkpoint_load(ck)
+        assert np.allclose(s.download("CONC"), C.astype(np.float32))
+        with pytest.raises(Mac3dError, match="EINVAL"):
+            s.conc_enable(True, refine=3)
     w64 = scenarios.scaled(w, precision=2)
     with MacSolver(config_for(w64)) as s:
         s.wave_enable(n=64, **GEOM)

@@ -112,6 +112,38 @@ def test_node_parcel_stage_matches_c_abi():
 
 
 @pytest.mark.skipif(shutil.which("node") is None, reason="node not installed")
+def test_node_concentration_on_the_finer_grid_matches_c_abi():
+    """concEnable(true, 2) through the N-API host: the concentration, its
+    splat counters and the fields after coupled steps with emission equal the
+    C ABI's run of the same calls (the counters bit for bit)."""
+    from mac3d import MacSolver
+    from mac3d.lib import config_for
+    steps = 2
+    r = subprocess.run(["node", os.path.join(ROOT, "tests", "js", "step_tiny.js"), str(steps),
+                        "conc2"], capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stderr
+    js = json.loads(r.stdout.strip().splitlines()[-1])["conc2"]
+    w = scenarios.workload("tiny", 1)
+    with MacSolver(config_for(w)) as s:
+        s.init_blobs(w.blobs)
+        s.wave_enable(n=64, half_extent=0.25, half_xz=0.1, depth=0.2, surface_y=0.0,
+                      marangoni=0.8, scene=1, physics=1)
+        s.set_drops([(0.004, -0.003, -0.002, 0.02, -0.4, 0.0, 0.85, 1.3)])
+        s.conc_enable(True, refine=2)
+        s.parcels_upload(np.zeros((4096, 8)))
+        for q in range(steps):
+            s.parcels_emit(300 * q, 300)
+            s.step(w.dt, 1)
+        conc = s.download("CONC")
+        v = s.download("V").astype(np.float32).astype(np.float64)
+        intact, lysed = int(s.splat_download(0).sum()), int(s.splat_download(1).sum())
+        assert js["cells"] == 8 * int(np.prod(s.shape)) == conc.shape[0] * conc.shape[1] * conc.shape[2]
+    assert intact > 0 and js["intact"] == intact and js["lysed"] == lysed
+    assert js["conc"] == pytest.approx(float(np.sqrt(np.sum(conc * conc))), rel=1e-12)
+    assert js["v"] == pytest.approx(float(np.sqrt(np.sum(v * v))), rel=1e-12)
+
+
+@pytest.mark.skipif(shutil.which("node") is None, reason="node not installed")
 @pytest.mark.parametrize("mode", ["fp32", "mixed"])
 def test_node_checkpoint_round_trip(mode, tmp_path):
     """saveCheckpoint -> step -> loadCheckpoint -> step through the N-API
