@@ -799,10 +799,83 @@ int sweep2_cz(const Dims &d, int tx) {
   return cz;
 }
 
+// Tile shapes of the coarse levels' two-sweep pass (k_rbgs_tile2): interior
+// TX x TY x TZ cells and NT threads per block (at least one per column of the
+// staged x ring and of R1's region).
+struct Tile2Shape { int tx, ty, tz, nt; };
+static constexpr Tile2Shape kTile2Shapes[] = {{32, 8, 8, 320}, {16, 8, 8, 192}, {48, 8, 8, 448}};
+
+// Measured on 192x256x192's coarse levels (us per launch, given / zero guess;
+// z-marching pass 16.3 / 14.5 on level 1, 11.0 / 10.1 on level 2):
+//   96x128x96: 48x8x8 14.5 / 12.5, 16x8x8 16.1 / 13.4, 32x8x8 20.0 / 13.6;
+//   48x64x48:  16x8x8  8.1 / 7.5, 48x8x8 9.6 / 8.6, 32x8x8 8.8 / 8.1.
+// Small grids take the narrow tile (more blocks); larger ones the widest tile
+// that divides the rows (less ring work per cell).
+static int tile2_shape(const Dims &d) {
+  if (d.cells() <= ((size_t)1 << 19)) return 1;
+  if (d.nx % 48 == 0) return 2;
+  if (d.nx % 32 == 0) return 0;
+  return 1;
+}
+
+// The coarse levels' two-sweep pass: one 3-D tile per block (k_rbgs_tile2).
+int mg_tile2(Ctx *ctx, int l, bool zero) {
+  const Dims &d = ctx->d[l];
+  const int shape = tile2_shape(d);
+  const Tile2Shape sh = kTile2Shapes[shape];
+  const int ncx = (d.nx + sh.tx - 1) / sh.tx, ncy = (d.ny + sh.ty - 1) / sh.ty;
+  const int ncz = (d.nz + sh.tz - 1) / sh.tz;
+  dim3 grid(ncx * ncy * ncz);
+  {
+    ProfScope ps(ctx->prof, ctx->stream, KC_MG);
+    auto launch = [&](auto stag, auto w1tag, auto ztag, auto shtag) {
+      using S = std::remove_pointer_t<decltype(stag)>;
+      constexpr bool W1 = decltype(w1tag)::value, Z = decltype(ztag)::value;
+      constexpr int SH = decltype(shtag)::value;
+      constexpr Tile2Shape T = kTile2Shapes[SH];
+      hipLaunchKernelGGL((k_rbgs_tile2<Z, S, W1, T.tx, T.ty, T.tz, T.nt>), grid, dim3(T.nt), 0,
+                         ctx->stream, d, (const S *)ctx->mgx[l].ptr, (S *)ctx->mgxa[l].ptr,
+                         (const S *)ctx->mgb[l].ptr, (float)level_h2(l),
+                         (float)(l == 0 ? 1.0 : top_coeff(l)), (float)ctx->omega,
+                         l == 0 ? (const double *)ctx->scale : nullptr, ncx, ncy);
+    };
+    auto by_shape = [&](auto stag, auto w1tag, auto ztag) {
+      switch (shape) {
+        case 1: launch(stag, w1tag, ztag, std::integral_constant<int, 1>{}); break;
+        case 2: launch(stag, w1tag, ztag, std::integral_constant<int, 2>{}); break;
+        default: launch(stag, w1tag, ztag, std::integral_constant<int, 0>{}); break;
+      }
+    };
+    auto by_zero = [&](auto stag, auto w1tag) {
+      if (zero) by_shape(stag, w1tag, std::true_type{});
+      else by_shape(stag, w1tag, std::false_type{});
+    };
+    const bool w1 = (float)ctx->omega == 1.0f;
+    if (ctx->mgx[l].type == TF16) {
+      if (w1) by_zero((half_t *)nullptr, std::true_type{});
+      else by_zero((half_t *)nullptr, std::false_type{});
+    } else {
+      if (w1) by_zero((float *)nullptr, std::true_type{});
+      else by_zero((float *)nullptr, std::false_type{});
+    }
+  }
+  LAUNCHCHK();
+  swap_x(ctx, l);
+  return MAC3D_OK;
+}
+
 // Two red-black sweeps on level l in one pass (FP32 or FP16 storage): mgx -> mgxa,
 // then the two swap roles.  zero: the first sweep starts from a zero guess.
-int mg_sweep2(Ctx *ctx, int l, bool zero) {
+// Levels >= 1 (and grids a z-march would not fill) take the tile pass; march:
+// the z-marching pass on any level (parity tests of the two).
+int mg_sweep2(Ctx *ctx, int l, bool zero, bool march = false) {
   const Dims &d = ctx->d[l];
+  // (the tile pass takes x-pairs and 32-bit byte offsets, a level and its
+  // ring planes under 2^30 bytes)
+  const size_t es = type_size(ctx->mgx[l].type);
+  if (!march && l > 0 && d.nx % 2 == 0 && d.nz >= 2 &&
+      (d.cells() + (size_t)8 * d.nx * d.ny) * es < ((size_t)1 << 30))
+    return mg_tile2(ctx, l, zero);
   const int tx = sweep2_tx(d);
   int cz = sweep2_cz(d, tx);
   int ncx = (d.nx + tx - 1) / tx, ncy = (d.ny + SWEEP2_TY - 1) / SWEEP2_TY;
@@ -2760,6 +2833,12 @@ int mac3d_mg_op(void *h, int op, int level, int count) {
     case MAC3D_MG_ZERO: rc = mg_fill_zero(ctx, ctx->mgx[level]); break;
     case MAC3D_MG_SMOOTH_ZERO:
       rc = reset_scale(ctx) ? MAC3D_EHIP : mg_smooth(ctx, level, count, true);
+      break;
+    case MAC3D_MG_SMOOTH2_MARCH:
+      if (ctx->mgx[level].type != TF32 && ctx->mgx[level].type != TF16)
+        return fail(ctx, MAC3D_EINVAL, "the z-marching pass takes FP32 or FP16 levels");
+      if (count > 1) return fail(ctx, MAC3D_EINVAL, "count: 0 (given guess) or 1 (zero guess)");
+      rc = reset_scale(ctx) ? MAC3D_EHIP : mg_sweep2(ctx, level, count == 1, true);
       break;
     default: return fail(ctx, MAC3D_EINVAL, "unknown multigrid op");
   }

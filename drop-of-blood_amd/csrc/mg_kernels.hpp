@@ -1110,4 +1110,204 @@ k_rbgs_sweep2(Dims d, const S *__restrict__ xin, S *__restrict__ xout,
   if (j < k1) iter(odd, chk, j, xA, bA);
 }
 
+// ---------------------------------------------------------------------------
+// Two red-black sweeps on one 3-D tile per block, for the coarse levels: the
+// same result as k_rbgs_sweep2 bit for bit (the cell arithmetic of rb_cell,
+// the x neighbours summed pair cell first), without its z-march.  On a grid
+// of 1-3 M cells a marching pass is a chain of ~13 dependent plane iterations
+// (7 of them filling the stage pipeline) per block; here a block loads its
+// tile of x with a four-cell ring in every direction (TX+8 x TY+8 x TZ+8
+// cells) into LDS at once, runs R1 (tile plus three cells), B1 (plus two), R2
+// (plus one) and B2 (the tile) with one barrier between stages, and writes
+// the tile: one memory round trip and four short stages.  Cells outside the
+// grid stage as 0 and are never updated, so the stencil sums its six
+// neighbours without branches; only the diagonal depends on where the cell is.
+//
+// In a stage each thread owns one column of the stage's region, a (pair,
+// row), and walks its planes: everything about the column (LDS slot, side of
+// the stage's colour on even and odd planes, in-grid test, 1/diag inside and
+// on the z walls) is formed once; the plane enters only as an immediate LDS
+// offset and a wave-uniform z test.  The right-hand sides of a thread's
+// updates, one per plane of each stage, are loaded into registers with the
+// staging.  LDS holds each staged row split by colour (TX/2+4 red pairs, then
+// as many black ones): a cell's six neighbours are of the other colour, at
+// pair p (the pair cell, and the rows and planes either side) and at pair p-1
+// or p+1 (by the cell's side).  ZERO: zero initial guess (x is not read).
+// scale (nullable): the right-hand side is b scale[0].  Storage S: FP32 or
+// FP16 (widened as staged, rounded once on the store).
+
+// Stage geometry of k_rbgs_tile2: a stage on the tile plus an R-cell ring
+// takes pairs [PLO, PLO + NPP) of a staged row and rows [4 - R, 4 + TY + R)
+// (NC columns), planes [4 - R, 4 + TZ + R) (NZ).
+template <int TX, int TY, int TZ, int R> struct Tile2Stage {
+  static constexpr int PLO = (4 - R) / 2, NPP = (TX + 3 + R) / 2 + 1 - PLO;
+  static constexpr int NRY = TY + 2 * R, NC = NPP * NRY, NZ = TZ + 2 * R;
+};
+
+template <bool ZERO, class S, bool W1, int TX, int TY, int TZ, int NT>
+__global__ void __launch_bounds__(NT)
+k_rbgs_tile2(Dims d, const S *__restrict__ xin, S *__restrict__ xout, const S *__restrict__ b,
+             float h2, float topc, float omega, const double *__restrict__ scale, int ncx,
+             int ncy) {
+  typedef float C;
+  constexpr int PX = TX / 2 + 4;           // staged pairs per row (four-cell ring)
+  constexpr int EY = TY + 8, EZ = TZ + 8;  // staged rows and planes
+  constexpr int RS = 2 * PX;               // floats per staged row: red, then black
+  constexpr int PS = EY * RS;              // floats per staged plane
+  constexpr int es = (int)sizeof(S);
+  static_assert(TX % 2 == 0 && TZ % 2 == 0, "x-pairs; tiles start on even planes");
+  static_assert(Tile2Stage<TX, TY, TZ, 3>::NC <= NT, "one column per thread and stage");
+  __shared__ C lds[EZ * PS];
+  const int nx = d.nx, ny = d.ny, nz = d.nz;
+  const int lb = xcd_remap(blockIdx.x, gridDim.x);
+  const int x0 = (lb % ncx) * TX, y0 = ((lb / ncx) % ncy) * TY, z0 = (lb / (ncx * ncy)) * TZ;
+  const int tid = threadIdx.x;
+  const C bs = scale ? (C)scale[0] : C(1);
+  const int pst = nx * ny * es;  // bytes per plane
+  const uint32_t bytes = (uint32_t)(d.cells() * es);
+  const BufRsrc rx = buf_rsrc(xin, bytes), rb = buf_rsrc(b, bytes);
+
+  // ---- per stage: this thread's column ----------------------------------------
+  // a: LDS slot of the column's colour-X cell on plane 0; fo[h]: offset of its
+  // x neighbour in the next pair over on planes of parity h (sides alternate
+  // plane by plane); ok bit h: the cell lies in the ring and in the grid in x
+  // and y; rd[h] / rw[h]: 1/diag on planes with both z neighbours / with one;
+  // bo[h]: byte offset of the cell in its plane (b's loads)
+  struct Col { int a, fo[2], bo[2]; unsigned ok; C rd[2], rw[2]; bool act; };
+  auto column = [&](auto Xc, auto Rc) -> Col {
+    constexpr int X = decltype(Xc)::value, R = decltype(Rc)::value;
+    typedef Tile2Stage<TX, TY, TZ, R> T;
+    Col c;
+    c.act = tid < T::NC;
+    const int w = c.act ? tid : 0;
+    const int pp = T::PLO + w % T::NPP, py = 4 - R + w / T::NPP;
+    const int gy = y0 - 4 + py, gxl = x0 - 4 + 2 * pp;
+    c.a = py * RS + X * PX + pp;
+    c.ok = 0;
+    const C top = (gy < ny - 1) ? C(1) : (C)topc;
+#pragma unroll
+    for (int h = 0; h < 2; ++h) {
+      // plane parity h: gz = z0 - 4 + pz with z0 even, so gz & 1 = pz & 1
+      const int sd = ((gy + h) & 1) ^ X;  // side of the colour-X cell in its pair
+      const int gx = gxl + sd;
+      c.fo[h] = sd ? 1 : -1;
+      bool in = c.act && gx >= 0 && gx < nx && gy >= 0 && gy < ny;
+      if (R & 1) in = in && gx >= x0 - R && gx < x0 + TX + R;
+      c.ok |= (in ? 1u : 0u) << h;
+      const int nxy = (gx > 0) + (gx < nx - 1) + (gy > 0);
+      c.rd[h] = fast_rcp(top + C(nxy + 2));
+      c.rw[h] = fast_rcp(top + C(nxy + 1));
+      c.bo[h] = (gy * nx + gx) * es;
+    }
+    return c;
+  };
+  const std::integral_constant<int, 0> red, r0;
+  const std::integral_constant<int, 1> black, r1;
+  const std::integral_constant<int, 2> r2;
+  const std::integral_constant<int, 3> r3;
+  const Col c1 = column(red, r3), c2 = column(black, r2), c3 = column(red, r1),
+            c4 = column(black, r0);
+
+  // ---- loads: the staged x pairs by columns (thread t stages the column of
+  // pairs (t mod PX, t / PX) through all planes), then the right-hand side of
+  // every update
+  constexpr int NXC = PX * EY;
+  static_assert(NXC <= NT, "one staged column per thread");
+  const bool xact = tid < NXC;
+  const int xpp = xact ? tid % PX : 0, xpy = xact ? tid / PX : 0;
+  const int xgx = x0 - 4 + 2 * xpp, xgy = y0 - 4 + xpy;
+  // in-plane byte offset of the staged pair, or out of every plane's range
+  // (with the level under 2^30 bytes, no plane offset brings it back)
+  const int xo = (xgx >= 0 && xgx < nx && xgy >= 0 && xgy < ny) ? (xgy * nx + xgx) * es
+                                                                 : MAC3D_OFF_NONE;
+  float2 xs[ZERO ? 1 : EZ];
+  if (!ZERO) {
+#pragma unroll
+    for (int pz = 0; pz < EZ; ++pz) {
+      const int gz = z0 - 4 + pz;
+      // planes outside the grid lie outside the buffer's range (zeros)
+      xs[pz] = buf_load_pair<S>(rx, xo + gz * pst);
+    }
+  }
+  // b of a column's cell on plane pz (0 outside the grid in z: the plane's
+  // offset then lies outside the buffer's range)
+  auto ld_b = [&](const Col &c, int pz) -> C {
+    const int gz = z0 - 4 + pz;
+    const int o = c.bo[pz & 1] + gz * pst;
+    if constexpr (sizeof(S) == 4) return buf_load<float>(rb, o) * bs;
+    else
+      return (C)__builtin_bit_cast(
+                 half_t, (unsigned short)__builtin_amdgcn_raw_buffer_load_b16(rb, o, 0, 0)) *
+             bs;
+  };
+  C b1[TZ + 6], b2[TZ + 4], b3[TZ + 2], b4[TZ];
+#pragma unroll
+  for (int z = 0; z < TZ + 6; ++z) b1[z] = ld_b(c1, 1 + z);
+#pragma unroll
+  for (int z = 0; z < TZ + 4; ++z) b2[z] = ld_b(c2, 2 + z);
+#pragma unroll
+  for (int z = 0; z < TZ + 2; ++z) b3[z] = ld_b(c3, 3 + z);
+#pragma unroll
+  for (int z = 0; z < TZ; ++z) b4[z] = ld_b(c4, 4 + z);
+  // staging: pair (x, y) to its row's red and black slots; red on the right
+  // where (gy + gz) is odd (the ring offsets are even)
+  if (xact) {
+    const int a0 = xpy * RS + xpp, rs0 = (y0 + xpy + z0) & 1;
+#pragma unroll
+    for (int pz = 0; pz < EZ; ++pz) {
+      const int a = a0 + pz * PS;
+      if (ZERO) {
+        lds[a] = C(0);
+        lds[a + PX] = C(0);
+      } else {
+        const bool rs = rs0 ^ (pz & 1);
+        lds[a] = rs ? xs[pz].y : xs[pz].x;
+        lds[a + PX] = rs ? xs[pz].x : xs[pz].y;
+      }
+    }
+  }
+  __syncthreads();
+
+  // One colour stage over the column's planes: x neighbours (pair cell first,
+  // as k_rbgs_sweep2 sums them), y-, y+, z-, z+, then the update.  OUT (B2):
+  // the new black cell goes to xout with the pair's red cell from R2.
+  auto stage = [&](auto Xc, auto Rc, auto OUTc, const Col &c, const auto &bv) {
+    constexpr int X = decltype(Xc)::value, R = decltype(Rc)::value;
+    constexpr bool OUT = decltype(OUTc)::value;
+    typedef Tile2Stage<TX, TY, TZ, R> T;
+    // whole waves past the stage's columns skip it (a scalar branch)
+    if (__builtin_amdgcn_readfirstlane(tid & ~63) >= T::NC) return;
+    constexpr int oc = X ? -PX : PX;  // the pair's other-colour cell
+#pragma unroll
+    for (int z = 0; z < T::NZ; ++z) {
+      const int pz = 4 - R + z, h = pz & 1;
+      const int gz = z0 - 4 + pz;  // uniform over the block
+      const int a = c.a + pz * PS, o = a + oc;
+      const C sm = lds[o], fx = lds[o + c.fo[h]];
+      const C ym = lds[o - RS], yp = lds[o + RS], zm = lds[o - PS], zp = lds[o + PS];
+      const C rdg = (gz > 0 && gz < nz - 1) ? c.rd[h] : c.rw[h];
+      C nv;
+      if (W1) nv = rb_cell_r1<C>(sm, fx, ym, yp, zm, zp, bv[z], (C)h2, rdg);
+      else nv = rb_cell_r<C>(sm, fx, ym, yp, zm, zp, bv[z], lds[a], (C)h2, rdg, (C)omega);
+      if (gz < 0 || gz >= nz || !((c.ok >> h) & 1u)) continue;
+      if (OUT) {
+        const int sd = c.fo[h] > 0;
+        float2 ov;
+        ov.x = sd ? sm : nv;
+        ov.y = sd ? nv : sm;
+        store_pair(xout + (size_t)gz * nx * ny + ((c.bo[h] / es) - sd), ov);
+      } else {
+        lds[a] = nv;
+      }
+    }
+  };
+  stage(red, r3, std::false_type{}, c1, b1);
+  __syncthreads();
+  stage(black, r2, std::false_type{}, c2, b2);
+  __syncthreads();
+  stage(red, r1, std::false_type{}, c3, b3);
+  __syncthreads();
+  stage(black, r0, std::true_type{}, c4, b4);
+}
+
 }  // namespace mac3d

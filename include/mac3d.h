@@ -111,6 +111,10 @@ extern "C" {
 #define MAC3D_MG_ZERO 5     /* MGX[l] <- 0 */
 #define MAC3D_MG_SMOOTH_ZERO 6 /* count RBGS sweeps on MGB from a zero guess (MGX unread;
                                   MG_RBGS_RED_ZERO, src/shaders/fluid.ts:605-641) */
+#define MAC3D_MG_SMOOTH2_MARCH 7 /* two RBGS sweeps on (MGX, MGB) by the z-marching two-sweep
+                                    pass (level 0's) on any FP32/FP16 level; count 0: from
+                                    MGX, 1: from a zero guess.  Coarse levels smooth with a
+                                    tile pass, which must equal this one bit for bit */
 
 /* kernel classes timed by mac3d_profile (HIP events on the stream) */
 #define MAC3D_KC_FACE_TO_CENTRE 0

@@ -62,3 +62,15 @@ def test_null_handles():
     assert lib.mac3d_destroy(None) == 0
     assert lib.mac3d_step(None, 1e-3, 1, None) == 1
     assert lib.mac3d_upload(None, 0, 0, None, 0, 0) == 1
+
+
+@pytest.mark.parametrize("table,prefix", [("FIELD", "MAC3D_FIELD_"), ("STAGE", "MAC3D_STAGE_"),
+                                          ("MG", "MAC3D_MG_"), ("KC", "MAC3D_KC_")])
+def test_host_id_tables_match_the_header(table, prefix):
+    """The Python host's field, stage, multigrid-op and kernel-class ids are
+    the header's #defines, name for name."""
+    src = open(os.path.join(ROOT, "include", "mac3d.h")).read()
+    defs = {m.group(1): int(m.group(2))
+            for m in re.finditer(r"#define " + prefix + r"([A-Z0-9_]+)\s+(\d+)", src)}
+    defs.pop("COUNT", None)
+    assert getattr(L, table) == defs
