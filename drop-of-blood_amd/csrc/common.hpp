@@ -133,18 +133,31 @@ template <class T> struct TPos {
   T o[3];
 };
 
+// Every sample point of a cell's backtrace keeps the cell's own texel index
+// as its base, so a clamp to [lo, hi] texels acts on the offset alone,
+// against the bounds taken relative to that index (tpos_bounds).  Where a
+// clamp takes effect the bound lies within a few texels of the index, and
+// their difference is exact (Sterbenz), so the clamped point is the bound
+// itself; two compares and two selects per axis, no branch.  A NaN offset
+// compares false and stays NaN.
+template <class T> struct TBounds {
+  T lo[3], hi[3];
+};
 template <class T>
-__device__ inline void tpos_clamp(TPos<T> &p, const T lo[3], const T hi[3]) {
+__device__ inline TBounds<T> tpos_bounds(const T lo[3], const T hi[3], int i, int j, int k) {
+  TBounds<T> r;
+  const int b[3] = {i, j, k};
 #pragma unroll
   for (int a = 0; a < 3; ++a) {
-    T t = (T)p.b[a] + p.o[a];
-    if (t < lo[a] || t > hi[a]) {
-      T q = t < lo[a] ? lo[a] : hi[a];
-      T fq = floor(q);
-      p.b[a] = (int)fq;
-      p.o[a] = q - fq;
-    }
+    r.lo[a] = lo[a] - (T)b[a];
+    r.hi[a] = hi[a] - (T)b[a];
   }
+  return r;
+}
+template <class T>
+__device__ inline void tpos_clamp(TPos<T> &p, const TBounds<T> &r) {
+#pragma unroll
+  for (int a = 0; a < 3; ++a) p.o[a] = clampv(p.o[a], r.lo[a], r.hi[a]);
 }
 
 template <class T, class I = size_t> struct TriIdx {

@@ -77,19 +77,19 @@ __device__ inline void trace_back(const Dims &d,
   out[2] = clampv(f[2] - vm[2] * sc[2], lo, hi);
 }
 
-// traceBackUVW in texel space: the face point f (texel base + offset), RK2
-// displacement in texels v dt/h, clamps at [0.002 N - 1/2, 0.998 N - 1/2].
-// f is the +1/2 face of cell c along axis a (c and the axis stride sa in the
-// reader's offset unit): unless the clamp
-// moves it, the first sample's weights are 1/2 along a and 0 on the other
-// axes, so it is mixv(cvel[c], cvel[c + sa], 1/2) -- the value of the
+// traceBackUVW in texel space: the face point f (the cell's texel index +
+// offset), RK2 displacement in texels v dt/h, clamps at [0.002 N - 1/2,
+// 0.998 N - 1/2] (bw, relative to the cell).  f is the +1/2 face of cell c
+// along axis a (c and the axis stride sa in the reader's offset unit): unless
+// the clamp moves it, the first sample's weights are 1/2 along a and 0 on the
+// other axes, so it is mixv(cvel[c], cvel[c + sa], 1/2) -- the value of the
 // 8-texel sample (mixv(x, y, 0) = x exactly) from 2 loads.
 template <class T, class CR>
 __device__ inline void trace_back_tp(const Dims &d, const CR &cvel, T dth, const TPos<T> &f,
-                                     const T lo[3], const T hi[3], typename CR::index c,
+                                     const TBounds<T> &bw, typename CR::index c,
                                      typename CR::index sa, TPos<T> &out) {
   TPos<T> p0 = f;
-  tpos_clamp(p0, lo, hi);
+  tpos_clamp(p0, bw);
   T v0[3], vm[3];
   bool moved = false;
 #pragma unroll
@@ -104,11 +104,11 @@ __device__ inline void trace_back_tp(const Dims &d, const CR &cvel, T dth, const
   }
   TPos<T> m = f;
   for (int a = 0; a < 3; ++a) m.o[a] -= T(0.5) * (v0[a] * dth);
-  tpos_clamp(m, lo, hi);
+  tpos_clamp(m, bw);
   sample_vec_tp_r<T>(cvel, d, m, vm);
   out = f;
   for (int a = 0; a < 3; ++a) out.o[a] -= vm[a] * dth;
-  tpos_clamp(out, lo, hi);
+  tpos_clamp(out, bw);
 }
 
 // FLUID_ADVECT_MAC (fluid.ts:144-192, 323-325): each staggered component is
@@ -143,6 +143,7 @@ k_advect_mac(Dims d, T dth, T damp, const T *__restrict__ u, const T *__restrict
     lo1[a] = T(0.001) * N[a] - T(0.5);
     hi1[a] = T(0.999) * N[a] - T(0.5);
   }
+  const TBounds<T> bw2 = tpos_bounds(lo2, hi2, i, j, k), bw1 = tpos_bounds(lo1, hi1, i, j, k);
   T res[3];
 #pragma unroll
   for (int a = 0; a < 3; ++a) {
@@ -153,9 +154,9 @@ k_advect_mac(Dims d, T dth, T damp, const T *__restrict__ u, const T *__restrict
     f.o[a] = T(0.5);
     TPos<T> b;
     const I sa = (a == 0 ? 1 : (a == 1 ? (I)d.nx : (I)d.nx * d.ny)) * decltype(cv)::unit;
-    trace_back_tp<T>(d, cv, dth, f, lo2, hi2, cu, sa, b);
+    trace_back_tp<T>(d, cv, dth, f, bw2, cu, sa, b);
     b.o[a] -= T(0.5);  // the -1/2 texel shift of sampleMacU/V/W
-    tpos_clamp(b, lo1, hi1);
+    tpos_clamp(b, bw1);
     res[a] = sample_tp_r<T>(rf[a], d, b) * damp;
   }
   if constexpr (SCENE)
@@ -514,14 +515,15 @@ __device__ inline void dye_back_tp(const Dims &d, const CR &cvel, T dth, int i, 
     v0[1] = a0.y;
     v0[2] = a0.z;
   }
+  const TBounds<T> bw = tpos_bounds(lo, hi, i, j, k);
   TPos<T> m = c;
   for (int a = 0; a < 3; ++a) m.o[a] -= T(0.5) * (v0[a] * dth);
-  tpos_clamp(m, lo, hi);
+  tpos_clamp(m, bw);
   T vm[3];
   sample_vec_tp_r<T>(cvel, d, m, vm);
   back = c;
   for (int a = 0; a < 3; ++a) back.o[a] -= vm[a] * dth;
-  tpos_clamp(back, lo, hi);
+  tpos_clamp(back, bw);
 }
 
 template <class T>
@@ -575,7 +577,7 @@ k_dye_predict(Dims d, T dth, const typename cvec_of<T>::type *__restrict__ cvel,
       b1.o[0] = -a0.x * dth;
       b1.o[1] = -a0.y * dth;
       b1.o[2] = -a0.z * dth;
-      tpos_clamp(b1, lo, hi);
+      tpos_clamp(b1, tpos_bounds(lo, hi, i, j, k));
       pred[c] = sample_tp_r<T>(rd, d, b1);
       return;
     }
@@ -612,7 +614,7 @@ k_dye_resolve(Dims d, T dth, T strength, const typename cvec_of<T>::type *__rest
   TPos<T> fw;
   fw.b[0] = i; fw.b[1] = j; fw.b[2] = k;
   for (int a = 0; a < 3; ++a) fw.o[a] = v0[a] * dth;
-  tpos_clamp(fw, lo, hi);
+  tpos_clamp(fw, tpos_bounds(lo, hi, i, j, k));
   T est = sample_tp_r<T>(make_reader<BUF>(pred, n), d, fw);
   T corr = pred[c] + strength * (dye[c] - est);
   int bi = back.b[0] + (int)floor(back.o[0]);
@@ -646,7 +648,7 @@ k_dye_resolve_mm(Dims d, T dth, T strength, const typename cvec_of<T>::type *__r
   fw.o[0] = a0.x * dth;
   fw.o[1] = a0.y * dth;
   fw.o[2] = a0.z * dth;
-  tpos_clamp(fw, lo, hi);
+  tpos_clamp(fw, tpos_bounds(lo, hi, i, j, k));
   T est = sample_tp_r<T>(make_reader<BUF>(pred, d.cells()), d, fw);
   T corr = pred[c] + strength * (dye[c] - est);
   const auto b = mm[c];
