@@ -138,8 +138,9 @@ template <class T> struct TPos {
 // against the bounds taken relative to that index (tpos_bounds).  Where a
 // clamp takes effect the bound lies within a few texels of the index, and
 // their difference is exact (Sterbenz), so the clamped point is the bound
-// itself; two compares and two selects per axis, no branch.  A NaN offset
-// compares false and stays NaN.
+// itself.  One median-of-three per axis in FP32 (a NaN offset clamps to a
+// bound: the advection carries a non-finite trace on by itself, trace_nan),
+// two compares and two selects in FP64.
 template <class T> struct TBounds {
   T lo[3], hi[3];
 };
@@ -157,7 +158,16 @@ __device__ inline TBounds<T> tpos_bounds(const T lo[3], const T hi[3], int i, in
 template <class T>
 __device__ inline void tpos_clamp(TPos<T> &p, const TBounds<T> &r) {
 #pragma unroll
-  for (int a = 0; a < 3; ++a) p.o[a] = clampv(p.o[a], r.lo[a], r.hi[a]);
+  for (int a = 0; a < 3; ++a) {
+    if constexpr (sizeof(T) == 4) p.o[a] = __builtin_amdgcn_fmed3f(p.o[a], r.lo[a], r.hi[a]);
+    else p.o[a] = clampv(p.o[a], r.lo[a], r.hi[a]);
+  }
+}
+// 0, or NaN when a velocity of a backtrace is not finite: added to the
+// advected value, so a non-finite velocity yields a non-finite result even
+// where its trace was clamped back into the grid
+template <class T> __device__ inline T trace_nan(const T a[3], const T b[3]) {
+  return T(0) * (((a[0] + a[1]) + (a[2] + b[0])) + (b[1] + b[2]));
 }
 
 template <class T, class I = size_t> struct TriIdx {

@@ -87,7 +87,7 @@ __device__ inline void trace_back(const Dims &d,
 template <class T, class CR>
 __device__ inline void trace_back_tp(const Dims &d, const CR &cvel, T dth, const TPos<T> &f,
                                      const TBounds<T> &bw, typename CR::index c,
-                                     typename CR::index sa, TPos<T> &out) {
+                                     typename CR::index sa, TPos<T> &out, T &nanc) {
   TPos<T> p0 = f;
   tpos_clamp(p0, bw);
   T v0[3], vm[3];
@@ -109,20 +109,22 @@ __device__ inline void trace_back_tp(const Dims &d, const CR &cvel, T dth, const
   out = f;
   for (int a = 0; a < 3; ++a) out.o[a] -= vm[a] * dth;
   tpos_clamp(out, bw);
+  nanc = trace_nan(v0, vm);
 }
 
 // FLUID_ADVECT_MAC (fluid.ts:144-192, 323-325): each staggered component is
 // traced back from its own face and sampled with the -1/2 texel shift of
 // sampleMacU/V/W (fluid.ts:154-167), then multiplied by the damping factor;
 // the two solid wall faces stay 0.  sc = dt / domain size per axis.
-// (FP32: at least 6 waves per SIMD, more gathers in flight, measured 248
-// against 264 us at 192x256x192; 8 waves forces too few registers and was
-// slower.  FP64 needs its registers.)  BUF: fields read through buffer
+// (FP32: at least 5 waves per SIMD, more gathers in flight; 6 spill since
+// the traces carry their non-finite check, and 8 forced too few registers.
+// FP64 needs its registers.  The plain FP32 step takes k_advect_mac_lds.)
+// BUF: fields read through buffer
 // resources with 32-bit byte offsets (fields under 2^31 bytes, planes under
 // 2^24).  SCENE: the coupling's scene terms (advect_scene_terms) from the
 // drop slots and the wave state.
 template <class T, bool BUF, bool SCENE>
-__global__ void __launch_bounds__(256, sizeof(T) == 4 ? 6 : 1)
+__global__ void __launch_bounds__(256, sizeof(T) == 4 ? 5 : 1)
 k_advect_mac(Dims d, T dth, T damp, const T *__restrict__ u, const T *__restrict__ v,
              const T *__restrict__ w, const typename cvec_of<T>::type *__restrict__ cvel,
              T *__restrict__ uo, T *__restrict__ vo, T *__restrict__ wo, WaveParams wp,
@@ -154,16 +156,163 @@ k_advect_mac(Dims d, T dth, T damp, const T *__restrict__ u, const T *__restrict
     f.o[a] = T(0.5);
     TPos<T> b;
     const I sa = (a == 0 ? 1 : (a == 1 ? (I)d.nx : (I)d.nx * d.ny)) * decltype(cv)::unit;
-    trace_back_tp<T>(d, cv, dth, f, bw2, cu, sa, b);
+    T nanc;
+    trace_back_tp<T>(d, cv, dth, f, bw2, cu, sa, b, nanc);
     b.o[a] -= T(0.5);  // the -1/2 texel shift of sampleMacU/V/W
     tpos_clamp(b, bw1);
-    res[a] = sample_tp_r<T>(rf[a], d, b) * damp;
+    res[a] = sample_tp_r<T>(rf[a], d, b) * damp + nanc;
   }
   if constexpr (SCENE)
     advect_scene_terms<T>(d, i, j, k, wp, drops, phs, wave, res[0], res[1], res[2]);
   uo[c] = (i >= d.nx - 1) ? T(0) : res[0];
   vo[c] = res[1];
   wo[c] = (k >= d.nz - 1) ? T(0) : res[2];
+}
+
+// ---------------------------------------------------------------------------
+// k_advect_mac with its samples taken from LDS (FP32).  The trilinear
+// gathers of the backtraces (30 of 12 B and 24 of 4 B per cell) saturate the
+// texture addresser; at the step's Courant numbers (|v| dt/h under 1) every
+// sample of a cell has its corners within one cell of it.  A block stages
+// its tile (AL_TX x AL_TY x AL_TZ cells) with a one-cell ring of the centre
+// velocity and of u, v, w in LDS (structure of arrays, coalesced loads), the
+// ring's cells outside the grid replicating the edge cells, which is the
+// sampler's clamp-to-edge rule, and samples from there: the corner offsets
+// are immediates and need no edge logic.  A sample whose corners leave the
+// staged box falls back to the buffer gathers of k_advect_mac.  The sample
+// arithmetic (weights, lerp order) is k_advect_mac's, so the results are
+// identical.
+#define AL_TX 32
+#define AL_TY 8
+#define AL_TZ 4
+#define AL_EX (AL_TX + 2)
+#define AL_EY (AL_TY + 2)
+#define AL_EZ (AL_TZ + 2)
+#define AL_N (AL_EX * AL_EY * AL_EZ)
+
+// lower corner (staged index o) and weights of p, a sample point of the cell
+// at staged index (lx, ly, lz); false when a corner lies outside the box
+__device__ inline bool al_corner(const TPos<float> &p, int lx, int ly, int lz, int &o,
+                                 float &fx, float &fy, float &fz) {
+  const float f0 = floor(p.o[0]), f1 = floor(p.o[1]), f2 = floor(p.o[2]);
+  fx = p.o[0] - f0;
+  fy = p.o[1] - f1;
+  fz = p.o[2] - f2;
+  const int cx = lx + (int)f0, cy = ly + (int)f1, cz = lz + (int)f2;
+  o = (cz * AL_EY + cy) * AL_EX + cx;
+  return (unsigned)cx < AL_EX - 1 && (unsigned)cy < AL_EY - 1 && (unsigned)cz < AL_EZ - 1;
+}
+// trilinear sample of a staged scalar at corner o (the order of sample_tp_r)
+__device__ inline float al_tri(const float *f, int o, float fx, float fy, float fz) {
+  constexpr int dy = AL_EX, dz = AL_EX * AL_EY;
+  const float x00 = mixv(f[o], f[o + 1], fx);
+  const float x10 = mixv(f[o + dy], f[o + dy + 1], fx);
+  const float x01 = mixv(f[o + dz], f[o + dz + 1], fx);
+  const float x11 = mixv(f[o + dz + dy], f[o + dz + dy + 1], fx);
+  return mixv(mixv(x00, x10, fy), mixv(x01, x11, fy), fz);
+}
+
+template <bool SCENE>
+__global__ void __launch_bounds__(AL_TX * AL_TY)
+k_advect_mac_lds(Dims d, float dth, float damp, const float *__restrict__ u,
+                 const float *__restrict__ v, const float *__restrict__ w,
+                 const float3 *__restrict__ cvel, float *__restrict__ uo, float *__restrict__ vo,
+                 float *__restrict__ wo, WaveParams wp, const WaveDrop *__restrict__ drops,
+                 const DropPhase *__restrict__ phs, const float4 *__restrict__ wave) {
+  __shared__ float sc[3][AL_N];  // centre velocity x, y, z
+  __shared__ float sf[3][AL_N];  // u, v, w
+  const int x0 = blockIdx.x * AL_TX, y0 = blockIdx.y * AL_TY, z0 = blockIdx.z * AL_TZ;
+  const int tid = threadIdx.x;
+  for (int t = tid; t < AL_N; t += AL_TX * AL_TY) {
+    const int lx = t % AL_EX, ly = (t / AL_EX) % AL_EY, lz = t / (AL_EX * AL_EY);
+    const size_t c = idx3(d, clampi(x0 - 1 + lx, 0, d.nx - 1), clampi(y0 - 1 + ly, 0, d.ny - 1),
+                          clampi(z0 - 1 + lz, 0, d.nz - 1));
+    const float3 cv = cvel[c];
+    sc[0][t] = cv.x;
+    sc[1][t] = cv.y;
+    sc[2][t] = cv.z;
+    sf[0][t] = u[c];
+    sf[1][t] = v[c];
+    sf[2][t] = w[c];
+  }
+  __syncthreads();
+  const int i = x0 + tid % AL_TX, j = y0 + tid / AL_TX;
+  if (i >= d.nx || j >= d.ny) return;
+  const size_t n = d.cells();
+  const auto cvr = make_reader<true>(cvel, n);
+  const BufRead<float> rf[3] = {make_reader<true>(u, n), make_reader<true>(v, n),
+                                make_reader<true>(w, n)};
+  const float N[3] = {(float)d.nx, (float)d.ny, (float)d.nz};
+  float lo2[3], hi2[3], lo1[3], hi1[3];
+  for (int a = 0; a < 3; ++a) {
+    lo2[a] = 0.002f * N[a] - 0.5f;
+    hi2[a] = 0.998f * N[a] - 0.5f;
+    lo1[a] = 0.001f * N[a] - 0.5f;
+    hi1[a] = 0.999f * N[a] - 0.5f;
+  }
+  const int lx = tid % AL_TX + 1, ly = tid / AL_TX + 1;
+  // centre velocity at a point: from LDS, or gathered where it leaves the box
+  auto vel_at = [&](const TPos<float> &p, int lz, float out[3]) {
+    int o;
+    float fx, fy, fz;
+    if (al_corner(p, lx, ly, lz, o, fx, fy, fz)) {
+#pragma unroll
+      for (int a = 0; a < 3; ++a) out[a] = al_tri(sc[a], o, fx, fy, fz);
+    } else {
+      sample_vec_tp_r<float>(cvr, d, p, out);
+    }
+  };
+#pragma unroll 1
+  for (int kz = 0; kz < AL_TZ; ++kz) {
+    const int k = z0 + kz;
+    if (k >= d.nz) break;
+    const int lz = kz + 1;
+    const int l0 = (lz * AL_EY + ly) * AL_EX + lx;  // the cell's staged index
+    const TBounds<float> bw2 = tpos_bounds(lo2, hi2, i, j, k), bw1 = tpos_bounds(lo1, hi1, i, j, k);
+    float res[3];
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+      // face +1/2 along axis a: the RK2 trace of trace_back_tp
+      TPos<float> f;
+      f.b[0] = i; f.b[1] = j; f.b[2] = k;
+      f.o[0] = f.o[1] = f.o[2] = 0.f;
+      f.o[a] = 0.5f;
+      TPos<float> p0 = f;
+      tpos_clamp(p0, bw2);
+      bool moved = false;
+#pragma unroll
+      for (int q = 0; q < 3; ++q) moved |= p0.o[q] != f.o[q];
+      float v0[3], vm[3];
+      if (!moved) {
+        const int sa = a == 0 ? 1 : (a == 1 ? AL_EX : AL_EX * AL_EY);
+#pragma unroll
+        for (int q = 0; q < 3; ++q) v0[q] = mixv(sc[q][l0], sc[q][l0 + sa], 0.5f);
+      } else {
+        vel_at(p0, lz, v0);
+      }
+      TPos<float> m = f;
+      for (int q = 0; q < 3; ++q) m.o[q] -= 0.5f * (v0[q] * dth);
+      tpos_clamp(m, bw2);
+      vel_at(m, lz, vm);
+      TPos<float> b = f;
+      for (int q = 0; q < 3; ++q) b.o[q] -= vm[q] * dth;
+      tpos_clamp(b, bw2);
+      b.o[a] -= 0.5f;  // the -1/2 texel shift of sampleMacU/V/W
+      tpos_clamp(b, bw1);
+      int o;
+      float fx, fy, fz;
+      float sv;
+      if (al_corner(b, lx, ly, lz, o, fx, fy, fz)) sv = al_tri(sf[a], o, fx, fy, fz);
+      else sv = sample_tp_r<float>(rf[a], d, b);
+      res[a] = sv * damp + trace_nan(v0, vm);
+    }
+    if constexpr (SCENE)
+      advect_scene_terms<float>(d, i, j, k, wp, drops, phs, wave, res[0], res[1], res[2]);
+    const size_t c = idx3(d, i, j, k);
+    uo[c] = (i >= d.nx - 1) ? 0.f : res[0];
+    vo[c] = res[1];
+    wo[c] = (k >= d.nz - 1) ? 0.f : res[2];
+  }
 }
 
 // Neighbour face values with the workload's wall rules: no-slip solid walls

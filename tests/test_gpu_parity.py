@@ -57,6 +57,22 @@ def test_stage(stage, fields, prec, tol):
         _cmp(s, o, fields, tol)
 
 
+@pytest.mark.parametrize("scale", [1.0, 6.0])
+def test_advect_stage_large_steps(scale):
+    """FP32 advection samples from each block's staged tile and gathers from
+    memory where a sample's corners leave it (displacements over a cell):
+    the advection stage at the 192x256x192 workload's step and at 6 times
+    it (traces of up to ~2.5 cells), against the oracle."""
+    w, s, o = pair(name="tiny", precision=0)
+    with s:
+        dt = w.dt * scale
+        o.stage(oracle.ST_F2C)
+        s.stage("FACE_TO_CENTRE")
+        o.stage(oracle.ST_ADVECT, dt)
+        s.stage("ADVECT", dt)
+        _cmp(s, o, ("U", "V", "W"), TOL32)
+
+
 @pytest.mark.parametrize("prec,tol", [(0, TOL32), (2, TOL64)])
 def test_dye_stages(prec, tol):
     w, s, o = pair(precision=prec)
