@@ -194,9 +194,9 @@ def _full_size_rhs(precision):
 
 
 def test_vcycle_full_size():
-    """FP32 V-cycles at 192x256x192 against the oracle's: level 0 and level 1
-    (96x128x96) smooth in two-sweep passes, level 2 in 2-plane chunks, and
-    the residual-restriction of levels 0 and 1 marches in z.
+    """FP32 V-cycles at 192x256x192 against the oracle's: level 0 smooths in
+    z-marching two-sweep passes, levels 1 (96x128x96) and 2 (48x64x48) in the
+    tile pass, and the residual-restriction of levels 0 and 1 marches in z.
 
     The pressure is held to the FP32 bar after every one of the workload's
     8 cycles from a cold start, and the projected velocities after the last.
@@ -393,9 +393,8 @@ def test_two_sweep_pass_against_oracle():
 @pytest.mark.parametrize("omega", [1.0, 1.2])
 def test_two_sweep_pass_coarse_levels_against_oracle(level, omega):
     """Levels 1 and 2 of the 192x256x192 grid (96x128x96 and 48x64x48, at or
-    above 2^17 cells; level 2 in 2-plane chunks) also smooth in two-sweep
-    passes: against the oracle's two FP64 sweeps (h^2 = 4^l, lid coefficient
-    2 2^l / (2^l + 1))."""
+    above 2^17 cells) smooth pairs of sweeps in the tile pass: against the
+    oracle's two FP64 sweeps (h^2 = 4^l, lid coefficient 2 2^l / (2^l + 1))."""
     from mac3d import MacSolver
     from mac3d.lib import config_for
     from mac3d import scenarios
