@@ -393,7 +393,7 @@ def test_two_sweep_pass_against_oracle():
 @pytest.mark.parametrize("omega", [1.0, 1.2])
 def test_two_sweep_pass_coarse_levels_against_oracle(level, omega):
     """Levels 1 and 2 of the 192x256x192 grid (96x128x96 and 48x64x48, at or
-    above 2^17 cells) smooth pairs of sweeps in the tile pass: against the
+    the tile pass's sizes) smooth pairs of sweeps in the tile pass: against the
     oracle's two FP64 sweeps (h^2 = 4^l, lid coefficient 2 2^l / (2^l + 1))."""
     from mac3d import MacSolver
     from mac3d.lib import config_for

@@ -1,7 +1,7 @@
 """The coarse levels' two-sweep tile pass against the z-marching two-sweep
 pass, bit for bit, through the C ABI (needs an MI355X).
 
-Levels >= 1 of at least 2^17 cells smooth pairs of red-black sweeps with one
+Levels >= 1 of at least 512 cells smooth pairs of red-black sweeps with one
 3-D tile per block (16x8x8, 32x8x8 or 48x8x8 cells by the level's size and
 width); level 0's z-marching pass (MAC3D_MG_SMOOTH2_MARCH) runs the same cell
 arithmetic, so the two must agree exactly: from a given guess and from a zero
@@ -24,6 +24,9 @@ CASES = [
     ((192, 264, 200), 3, 1, 0),   # 96x132x100: 48x8x8 tiles, partial in y and z
     ((256, 384, 256), 5, 1, 1),   # 128x192x128 (the large case's level 1, mixed: FP32):
                                   # 32x8x8 tiles
+    ((256, 384, 256), 5, 3, 1),   # 32x48x32, FP16: the large case's level 3
+    ((32, 48, 32), 3, 1, 0),      # 16x24x16: the tiny workload's level 1
+    ((32, 48, 32), 3, 2, 1),      # 8x12x8, FP16: one tile of 16x8x8 over the whole level
 ]
 
 
@@ -40,7 +43,7 @@ def test_tile_pass_equals_marching_pass(dims, levels, level, prec, omega):
     rng = np.random.default_rng(level * 7 + prec + int(omega * 10))
     with MacSolver(config_for(w, sor_omega=omega)) as s:
         shp = s.level_shape(level)
-        assert np.prod(shp) >= 1 << 17
+        assert np.prod(shp) >= 512
         x = rng.standard_normal(shp)
         b = rng.standard_normal(shp)
         res = {}
