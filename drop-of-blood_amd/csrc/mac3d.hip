@@ -846,7 +846,7 @@ int mg_tile2(Ctx *ctx, int l, bool zero) {
   const int ncz = (d.nz + sh.tz - 1) / sh.tz;
   dim3 grid(ncx * ncy * ncz);
   {
-    ProfScope ps(ctx->prof, ctx->stream, KC_MG);
+    ProfScope ps(ctx->prof, ctx->stream, l == 0 ? KC_SMOOTH0X2 : KC_MG);
     auto launch = [&](auto stag, auto w1tag, auto ztag, auto shtag) {
       using S = std::remove_pointer_t<decltype(stag)>;
       constexpr bool W1 = decltype(w1tag)::value, Z = decltype(ztag)::value;
@@ -885,14 +885,16 @@ int mg_tile2(Ctx *ctx, int l, bool zero) {
 
 // Two red-black sweeps on level l in one pass (FP32 or FP16 storage): mgx -> mgxa,
 // then the two swap roles.  zero: the first sweep starts from a zero guess.
-// Levels >= 1 (and grids a z-march would not fill) take the tile pass; march:
-// the z-marching pass on any level (parity tests of the two).
+// Levels >= 1, and level 0 of grids a z-march would not fill, take the tile
+// pass; march: the z-marching pass on any level (parity tests of the two).
 int mg_sweep2(Ctx *ctx, int l, bool zero, bool march = false) {
   const Dims &d = ctx->d[l];
   // (the tile pass takes x-pairs and 32-bit byte offsets, a level and its
-  // ring planes under 2^30 bytes)
+  // ring planes under 2^30 bytes; on level 0 it takes grids of up to 2^21
+  // cells, where a z-march has few planes per block to fill its pipeline:
+  // 96x128x96 14.1 / 12.2 against 16.3 / 14.5 us)
   const size_t es = type_size(ctx->mgx[l].type);
-  if (!march && l > 0 && d.nx % 2 == 0 && d.nz >= 2 &&
+  if (!march && (l > 0 || d.cells() <= ((size_t)1 << 21)) && d.nx % 2 == 0 && d.nz >= 2 &&
       (d.cells() + (size_t)8 * d.nx * d.ny) * es < ((size_t)1 << 30))
     return mg_tile2(ctx, l, zero);
   const int tx = sweep2_tx(d);
