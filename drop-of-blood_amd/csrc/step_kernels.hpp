@@ -202,6 +202,36 @@ __device__ inline bool al_corner(const TPos<float> &p, int lx, int ly, int lz, i
   o = (cz * AL_EY + cy) * AL_EX + cx;
   return (unsigned)cx < AL_EX - 1 && (unsigned)cy < AL_EY - 1 && (unsigned)cz < AL_EZ - 1;
 }
+// The staged centre velocity interleaves its components by row
+// ([plane][row][component][x]): a plane's 24 corner values of a sample then
+// lie within the 8-bit dword offsets of paired LDS reads from one base.
+#define AL_VR (3 * AL_EX)       // floats per staged row of the velocity
+#define AL_VP (AL_EY * AL_VR)   // floats per staged plane
+// lower corner in the staged velocity's layout (index v of component 0)
+// and weights of p; false when a corner lies outside the box
+__device__ inline bool al_corner3(const TPos<float> &p, int lx, int ly, int lz, int &v,
+                                  float &fx, float &fy, float &fz) {
+  const float f0 = floor(p.o[0]), f1 = floor(p.o[1]), f2 = floor(p.o[2]);
+  fx = p.o[0] - f0;
+  fy = p.o[1] - f1;
+  fz = p.o[2] - f2;
+  const int cx = lx + (int)f0, cy = ly + (int)f1, cz = lz + (int)f2;
+  v = cz * AL_VP + cy * AL_VR + cx;
+  return (unsigned)cx < AL_EX - 1 && (unsigned)cy < AL_EY - 1 && (unsigned)cz < AL_EZ - 1;
+}
+// trilinear sample of the staged velocity at corner v, the order of
+// sample_vec_tp_r
+__device__ inline void al_tri3(const float *f, int v, float fx, float fy, float fz, float out[3]) {
+#pragma unroll
+  for (int c = 0; c < 3; ++c) {
+    const float *g = f + v + c * AL_EX;
+    const float x00 = mixv(g[0], g[1], fx);
+    const float x10 = mixv(g[AL_VR], g[AL_VR + 1], fx);
+    const float x01 = mixv(g[AL_VP], g[AL_VP + 1], fx);
+    const float x11 = mixv(g[AL_VP + AL_VR], g[AL_VP + AL_VR + 1], fx);
+    out[c] = mixv(mixv(x00, x10, fy), mixv(x01, x11, fy), fz);
+  }
+}
 // trilinear sample of a staged scalar at corner o (the order of sample_tp_r)
 __device__ inline float al_tri(const float *f, int o, float fx, float fy, float fz) {
   constexpr int dy = AL_EX, dz = AL_EX * AL_EY;
@@ -219,21 +249,29 @@ k_advect_mac_lds(Dims d, float dth, float damp, const float *__restrict__ u,
                  const float3 *__restrict__ cvel, float *__restrict__ uo, float *__restrict__ vo,
                  float *__restrict__ wo, WaveParams wp, const WaveDrop *__restrict__ drops,
                  const DropPhase *__restrict__ phs, const float4 *__restrict__ wave) {
-  __shared__ float sc[3][AL_N];  // centre velocity x, y, z
-  __shared__ float sf[3][AL_N];  // u, v, w
+  __shared__ float sc[3 * AL_N];  // centre velocity, components interleaved by row
+  __shared__ float sf[3][AL_N];   // u, v, w
   const int x0 = blockIdx.x * AL_TX, y0 = blockIdx.y * AL_TY, z0 = blockIdx.z * AL_TZ;
   const int tid = threadIdx.x;
-  for (int t = tid; t < AL_N; t += AL_TX * AL_TY) {
-    const int lx = t % AL_EX, ly = (t / AL_EX) % AL_EY, lz = t / (AL_EX * AL_EY);
-    const size_t c = idx3(d, clampi(x0 - 1 + lx, 0, d.nx - 1), clampi(y0 - 1 + ly, 0, d.ny - 1),
-                          clampi(z0 - 1 + lz, 0, d.nz - 1));
-    const float3 cv = cvel[c];
-    sc[0][t] = cv.x;
-    sc[1][t] = cv.y;
-    sc[2][t] = cv.z;
-    sf[0][t] = u[c];
-    sf[1][t] = v[c];
-    sf[2][t] = w[c];
+  // staging by (x, y) columns through the box's planes: the column's
+  // position is formed once, a plane adds the (uniform) plane offset
+  const size_t plane = (size_t)d.nx * d.ny;
+  for (int col = tid; col < AL_EX * AL_EY; col += AL_TX * AL_TY) {
+    const int lx = col % AL_EX, ly = col / AL_EX;
+    const size_t cxy = (size_t)clampi(y0 - 1 + ly, 0, d.ny - 1) * d.nx +
+                       clampi(x0 - 1 + lx, 0, d.nx - 1);
+#pragma unroll
+    for (int lz = 0; lz < AL_EZ; ++lz) {
+      const size_t c = cxy + (size_t)clampi(z0 - 1 + lz, 0, d.nz - 1) * plane;
+      const float3 cv = cvel[c];
+      const int t = lz * (AL_EX * AL_EY) + col, tv = lz * AL_VP + ly * AL_VR + lx;
+      sc[tv] = cv.x;
+      sc[tv + AL_EX] = cv.y;
+      sc[tv + 2 * AL_EX] = cv.z;
+      sf[0][t] = u[c];
+      sf[1][t] = v[c];
+      sf[2][t] = w[c];
+    }
   }
   __syncthreads();
   const int i = x0 + tid % AL_TX, j = y0 + tid / AL_TX;
@@ -255,9 +293,8 @@ k_advect_mac_lds(Dims d, float dth, float damp, const float *__restrict__ u,
   auto vel_at = [&](const TPos<float> &p, int lz, float out[3]) {
     int o;
     float fx, fy, fz;
-    if (al_corner(p, lx, ly, lz, o, fx, fy, fz)) {
-#pragma unroll
-      for (int a = 0; a < 3; ++a) out[a] = al_tri(sc[a], o, fx, fy, fz);
+    if (al_corner3(p, lx, ly, lz, o, fx, fy, fz)) {
+      al_tri3(sc, o, fx, fy, fz, out);
     } else {
       sample_vec_tp_r<float>(cvr, d, p, out);
     }
@@ -267,7 +304,7 @@ k_advect_mac_lds(Dims d, float dth, float damp, const float *__restrict__ u,
     const int k = z0 + kz;
     if (k >= d.nz) break;
     const int lz = kz + 1;
-    const int l0 = (lz * AL_EY + ly) * AL_EX + lx;  // the cell's staged index
+    const int l0 = lz * AL_VP + ly * AL_VR + lx;  // the cell's staged velocity index
     const TBounds<float> bw2 = tpos_bounds(lo2, hi2, i, j, k), bw1 = tpos_bounds(lo1, hi1, i, j, k);
     float res[3];
 #pragma unroll
@@ -284,9 +321,10 @@ k_advect_mac_lds(Dims d, float dth, float damp, const float *__restrict__ u,
       for (int q = 0; q < 3; ++q) moved |= p0.o[q] != f.o[q];
       float v0[3], vm[3];
       if (!moved) {
-        const int sa = a == 0 ? 1 : (a == 1 ? AL_EX : AL_EX * AL_EY);
+        const int sa = a == 0 ? 1 : (a == 1 ? AL_VR : AL_VP);
 #pragma unroll
-        for (int q = 0; q < 3; ++q) v0[q] = mixv(sc[q][l0], sc[q][l0 + sa], 0.5f);
+        for (int q = 0; q < 3; ++q)
+          v0[q] = mixv(sc[l0 + q * AL_EX], sc[l0 + q * AL_EX + sa], 0.5f);
       } else {
         vel_at(p0, lz, v0);
       }
