@@ -640,6 +640,18 @@ int launch_dye_predict(Ctx *ctx, double dt) {
   with_step_type(ctx->t0, [&](auto tag) {
     using T = std::remove_pointer_t<decltype(tag)>;
     const bool buf = gather_buf<T>(d), ref = ref_physics(ctx);
+    if constexpr (std::is_same<T, float>::value) {
+      // FP32, plain step: samples from the block's staged tile
+      if (buf && !ref) {
+        hipLaunchKernelGGL(k_dye_predict_lds,
+                           dim3((d.nx + AL_TX - 1) / AL_TX, (d.ny + AL_TY - 1) / AL_TY,
+                                (d.nz + AL_TZ - 1) / AL_TZ),
+                           dim3(AL_TX * AL_TY), 0, ctx->stream, d, (float)(dt / ctx->cfg.dx),
+                           (const float3 *)ctx->cvel.ptr, (const float *)ctx->dye.ptr,
+                           (float *)ctx->dpred.ptr, (float2 *)ctx->dmm.ptr);
+        return;
+      }
+    }
     auto kern = ref ? (buf ? k_dye_predict<T, true, true> : k_dye_predict<T, false, true>)
                     : (buf ? k_dye_predict<T, true, false> : k_dye_predict<T, false, false>);
     // (the reference's branches sample elsewhere than the resolve's trace:

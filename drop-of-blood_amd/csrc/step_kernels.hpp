@@ -780,6 +780,96 @@ k_dye_predict(Dims d, T dth, const typename cvec_of<T>::type *__restrict__ cvel,
   }
 }
 
+// k_dye_predict for the plain FP32 step (no reference branches; the limiter
+// bounds emitted) with its samples taken from LDS: a block stages its tile
+// (AL_TX x AL_TY x AL_TZ cells) with a one-cell ring of the centre velocity
+// (components interleaved by row) and of the dye, as k_advect_mac_lds does,
+// the ring's cells outside the grid replicating the edge cells (the
+// sampler's clamp-to-edge).  A sample whose corners leave the box gathers
+// from memory.  Weights, lerp order and the min/max order are those of
+// k_dye_predict, so the values are identical.
+__device__ inline float al_tri_mm(const float *f, int o, float fx, float fy, float fz,
+                                  float &mn, float &mx) {
+  constexpr int dy = AL_EX, dz = AL_EX * AL_EY;
+  const float s0 = f[o], s1 = f[o + 1], s2 = f[o + dy], s3 = f[o + dy + 1];
+  const float s4 = f[o + dz], s5 = f[o + dz + 1], s6 = f[o + dz + dy], s7 = f[o + dz + dy + 1];
+  mn = fmin(fmin(fmin(s0, s1), fmin(s2, s3)), fmin(fmin(s4, s5), fmin(s6, s7)));
+  mx = fmax(fmax(fmax(s0, s1), fmax(s2, s3)), fmax(fmax(s4, s5), fmax(s6, s7)));
+  const float x00 = mixv(s0, s1, fx);
+  const float x10 = mixv(s2, s3, fx);
+  const float x01 = mixv(s4, s5, fx);
+  const float x11 = mixv(s6, s7, fx);
+  return mixv(mixv(x00, x10, fy), mixv(x01, x11, fy), fz);
+}
+
+__global__ void __launch_bounds__(AL_TX * AL_TY)
+k_dye_predict_lds(Dims d, float dth, const float3 *__restrict__ cvel,
+                  const float *__restrict__ dye, float *__restrict__ pred,
+                  float2 *__restrict__ mm) {
+  __shared__ float sc[3 * AL_N];  // centre velocity, components interleaved by row
+  __shared__ float sd[AL_N];      // dye
+  const int x0 = blockIdx.x * AL_TX, y0 = blockIdx.y * AL_TY, z0 = blockIdx.z * AL_TZ;
+  const int tid = threadIdx.x;
+  const size_t plane = (size_t)d.nx * d.ny;
+  for (int col = tid; col < AL_EX * AL_EY; col += AL_TX * AL_TY) {
+    const int lx = col % AL_EX, ly = col / AL_EX;
+    const size_t cxy = (size_t)clampi(y0 - 1 + ly, 0, d.ny - 1) * d.nx +
+                       clampi(x0 - 1 + lx, 0, d.nx - 1);
+#pragma unroll
+    for (int lz = 0; lz < AL_EZ; ++lz) {
+      const size_t c = cxy + (size_t)clampi(z0 - 1 + lz, 0, d.nz - 1) * plane;
+      const float3 cv = cvel[c];
+      const int tv = lz * AL_VP + ly * AL_VR + lx;
+      sc[tv] = cv.x;
+      sc[tv + AL_EX] = cv.y;
+      sc[tv + 2 * AL_EX] = cv.z;
+      sd[lz * (AL_EX * AL_EY) + col] = dye[c];
+    }
+  }
+  __syncthreads();
+  const int i = x0 + tid % AL_TX, j = y0 + tid / AL_TX;
+  if (i >= d.nx || j >= d.ny) return;
+  const size_t n = d.cells();
+  const auto cvr = make_reader<true>(cvel, n);
+  const auto rd = make_reader<true>(dye, n);
+  float lo[3], hi[3];
+  dye_bounds(d, lo, hi);
+  const int lx = tid % AL_TX + 1, ly = tid / AL_TX + 1;
+#pragma unroll 1
+  for (int kz = 0; kz < AL_TZ; ++kz) {
+    const int k = z0 + kz;
+    if (k >= d.nz) break;
+    const int lz = kz + 1;
+    const int l0 = lz * AL_VP + ly * AL_VR + lx;
+    // dye_back_tp: the first sample is the cell's own centre velocity
+    float v0[3], vm[3];
+#pragma unroll
+    for (int q = 0; q < 3; ++q) v0[q] = sc[l0 + q * AL_EX];
+    const TBounds<float> bw = tpos_bounds(lo, hi, i, j, k);
+    TPos<float> m;
+    m.b[0] = i; m.b[1] = j; m.b[2] = k;
+    m.o[0] = m.o[1] = m.o[2] = 0.f;
+    for (int a = 0; a < 3; ++a) m.o[a] -= 0.5f * (v0[a] * dth);
+    tpos_clamp(m, bw);
+    int o;
+    float fx, fy, fz;
+    if (al_corner3(m, lx, ly, lz, o, fx, fy, fz)) al_tri3(sc, o, fx, fy, fz, vm);
+    else sample_vec_tp_r<float>(cvr, d, m, vm);
+    TPos<float> back;
+    back.b[0] = i; back.b[1] = j; back.b[2] = k;
+    back.o[0] = back.o[1] = back.o[2] = 0.f;
+    for (int a = 0; a < 3; ++a) back.o[a] -= vm[a] * dth;
+    tpos_clamp(back, bw);
+    float2 b;
+    float s;
+    if (al_corner(back, lx, ly, lz, o, fx, fy, fz)) s = al_tri_mm(sd, o, fx, fy, fz, b.x, b.y);
+    else s = sample_tp_mm_r<float>(rd, d, back, b.x, b.y);
+    const size_t c = idx3(d, i, j, k);
+    pred[c] = s;
+    mm[c] = b;
+  }
+}
+
 // CONC_RESOLVE's MacCormack core (advect.ts:196-222): forward estimate from
 // the predictor at uvw + v0 dt, corrected value pred + s (orig - estimate),
 // clamped to the min/max of the 8 source texels around the backtrace point.

@@ -87,6 +87,24 @@ def test_dye_stages(prec, tol):
         _cmp(s, o, ("DYE",), tol)
 
 
+@pytest.mark.parametrize("scale", [1.0, 6.0])
+def test_dye_stages_large_steps(scale):
+    """The FP32 dye predictor samples from each block's staged tile and
+    gathers from memory where a sample's corners leave it: the dye stages at
+    the workload's step and at 6 times it, against the oracle."""
+    w, s, o = pair(name="tiny", precision=0)
+    with s:
+        dt = w.dt * scale
+        o.stage(oracle.ST_F2C)
+        s.stage("FACE_TO_CENTRE")
+        o.stage(oracle.ST_DYE_PRED, dt)
+        s.stage("DYE_PREDICT", dt)
+        _cmp(s, o, ("DYE_PRED",), TOL32)
+        o.stage(oracle.ST_DYE_RES, dt)
+        s.stage("DYE_RESOLVE", dt)
+        _cmp(s, o, ("DYE",), TOL32)
+
+
 @pytest.mark.parametrize("prec,tol", [(0, TOL32), (2, TOL64)])
 def test_dye_resolve_after_dye_change(prec, tol):
     """The resolve stage after the dye changed since the predict (an upload
