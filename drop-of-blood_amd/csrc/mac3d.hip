@@ -67,6 +67,7 @@ struct Ctx {
   Dims d[kMaxLev + 1];
   size_t n[kMaxLev + 1];
   hipStream_t stream = nullptr;
+  int ncu = 256;                     // the device's compute units
   hipEvent_t norms_ready = nullptr;  // mixed mode: the residual norms reached the host
   // cvel holds the centre velocity of the current u, v, w (set by a step's
   // last face-to-centre pass, cleared by anything else that may change them)
@@ -834,25 +835,36 @@ int sweep2_cz(const Dims &d, int tx) {
 // TX x TY x TZ cells and NT threads per block (at least one per column of the
 // staged x ring and of R1's region).
 struct Tile2Shape { int tx, ty, tz, nt; };
-static constexpr Tile2Shape kTile2Shapes[] = {{32, 8, 8, 320}, {16, 8, 8, 192}, {48, 8, 8, 448}};
+// three x-y families (32 x 8, 16 x 8, 48 x 8), each 8, 6 or 2 planes deep
+static constexpr Tile2Shape kTile2Shapes[] = {
+    {32, 8, 8, 320}, {16, 8, 8, 192}, {48, 8, 8, 448}, {32, 8, 6, 320}, {16, 8, 6, 192},
+    {48, 8, 6, 448}, {32, 8, 2, 320}, {16, 8, 2, 192}, {48, 8, 2, 448}};
 
 // Measured on 192x256x192's coarse levels (us per launch, given / zero guess;
 // z-marching pass 16.3 / 14.5 on level 1, 11.0 / 10.1 on level 2):
 //   96x128x96: 48x8x8 14.5 / 12.5, 16x8x8 16.1 / 13.4, 32x8x8 20.0 / 13.6;
 //   48x64x48:  16x8x8  8.1 / 7.5, 48x8x8 9.6 / 8.6, 32x8x8 8.8 / 8.1.
 // Small grids take the narrow tile (more blocks); larger ones the widest tile
-// that divides the rows (less ring work per cell).
-static int tile2_shape(const Dims &d) {
-  if (d.cells() <= ((size_t)1 << 19)) return 1;
-  if (d.nx % 48 == 0) return 2;
-  if (d.nx % 32 == 0) return 0;
-  return 1;
+// that divides the rows (less ring work per cell).  The depth is then the
+// largest of 8, 6, 2 planes that still gives every CU two blocks (else 2):
+// levels 1-3 of 192x256x192 per FP32 step, 643 us with 8-plane tiles, 601
+// with 48x8x6 on 96x128x96 (512 blocks; 626 with 4 planes, 707 with 2) and
+// 16x8x2 on 48x64x48 (576 blocks; 608 with 6 planes, 608 with 4).
+static int tile2_shape(const Dims &d, int ncu) {
+  const int fam = d.cells() <= ((size_t)1 << 19) ? 1 : (d.nx % 48 == 0 ? 2 : (d.nx % 32 == 0 ? 0 : 1));
+  const Tile2Shape &f = kTile2Shapes[fam];
+  const size_t cols = (size_t)((d.nx + f.tx - 1) / f.tx) * ((d.ny + f.ty - 1) / f.ty);
+  for (int q = 0; q < 2; ++q) {  // depths 8, then 6
+    const int tz = kTile2Shapes[fam + 3 * q].tz;
+    if (cols * ((d.nz + tz - 1) / tz) >= (size_t)2 * ncu) return fam + 3 * q;
+  }
+  return fam + 6;
 }
 
 // The coarse levels' two-sweep pass: one 3-D tile per block (k_rbgs_tile2).
 int mg_tile2(Ctx *ctx, int l, bool zero) {
   const Dims &d = ctx->d[l];
-  const int shape = tile2_shape(d);
+  const int shape = tile2_shape(d, ctx->ncu);
   const Tile2Shape sh = kTile2Shapes[shape];
   const int ncx = (d.nx + sh.tx - 1) / sh.tx, ncy = (d.ny + sh.ty - 1) / sh.ty;
   const int ncz = (d.nz + sh.tz - 1) / sh.tz;
@@ -874,6 +886,12 @@ int mg_tile2(Ctx *ctx, int l, bool zero) {
       switch (shape) {
         case 1: launch(stag, w1tag, ztag, std::integral_constant<int, 1>{}); break;
         case 2: launch(stag, w1tag, ztag, std::integral_constant<int, 2>{}); break;
+        case 3: launch(stag, w1tag, ztag, std::integral_constant<int, 3>{}); break;
+        case 4: launch(stag, w1tag, ztag, std::integral_constant<int, 4>{}); break;
+        case 5: launch(stag, w1tag, ztag, std::integral_constant<int, 5>{}); break;
+        case 6: launch(stag, w1tag, ztag, std::integral_constant<int, 6>{}); break;
+        case 7: launch(stag, w1tag, ztag, std::integral_constant<int, 7>{}); break;
+        case 8: launch(stag, w1tag, ztag, std::integral_constant<int, 8>{}); break;
         default: launch(stag, w1tag, ztag, std::integral_constant<int, 0>{}); break;
       }
     };
@@ -1495,6 +1513,7 @@ int create_impl(Ctx *ctx, const mac3d_config *cfg) {
   ctx->prec = cfg->precision;
   ctx->t0 = ctx->prec == MAC3D_PREC_FP64 ? TF64 : TF32;
   HIPCHK(hipSetDevice(cfg->device));
+  HIPCHK(hipDeviceGetAttribute(&ctx->ncu, hipDeviceAttributeMultiprocessorCount, cfg->device));
   HIPCHK(hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking));
   HIPCHK(hipEventCreateWithFlags(&ctx->norms_ready, hipEventDisableTiming));
   for (int l = 0; l <= ctx->L; ++l) {
