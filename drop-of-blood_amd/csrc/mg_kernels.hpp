@@ -249,14 +249,26 @@ k_prolong_add_2x(Dims df, Dims dc, const Sc *__restrict__ ec, Sf *__restrict__ x
   __shared__ C se[3][PR_BY + 2][PR_BX + 2];
   const int tx = threadIdx.x, ty = threadIdx.y, tid = ty * PR_BX + tx;
   const int I0 = blockIdx.x * PR_BX, J0 = blockIdx.y * PR_BY, K = blockIdx.z;
+  const int I = I0 + tx, J = J0 + ty;
+  const bool in = I < dc.nx && J < dc.ny;
+  // the thread's fine cells are loaded first, so their round trip overlaps
+  // the staging's (on small grids few blocks hide latency)
+  struct alignas(2 * sizeof(Sf)) P2 { Sf a, b; };
+  P2 fv[2][2];
+  if (in) {
+#pragma unroll
+    for (int oz = 0; oz < 2; ++oz)
+#pragma unroll
+      for (int oy = 0; oy < 2; ++oy)
+        fv[oz][oy] = *(const P2 *)(xf + idx3(df, 2 * I, 2 * J + oy, 2 * K + oz));
+  }
   for (int t = tid; t < 3 * (PR_BY + 2) * (PR_BX + 2); t += PR_BX * PR_BY) {
     const int q = t % (PR_BX + 2), r = (t / (PR_BX + 2)) % (PR_BY + 2);
     const int a = t / ((PR_BX + 2) * (PR_BY + 2));
     se[a][r][q] = (C)coarse_at(ec, dc, I0 + q - 1, J0 + r - 1, K + a - 1);
   }
   __syncthreads();
-  const int I = I0 + tx, J = J0 + ty;
-  if (I >= dc.nx || J >= dc.ny) return;
+  if (!in) return;
   C e[3][3][3];  // [dz][dy][dx], offsets -1..1
 #pragma unroll
   for (int a = 0; a < 3; ++a)
@@ -280,9 +292,8 @@ k_prolong_add_2x(Dims df, Dims dc, const Sc *__restrict__ ec, Sf *__restrict__ x
         yz[q] = wl[oy] * zc[0] + wm * zc[1] + wu[oy] * zc[2];
       }
       // the two x-children as one aligned pair (2I is even)
-      struct alignas(2 * sizeof(Sf)) P2 { Sf a, b; };
       P2 *px = (P2 *)(xf + idx3(df, 2 * I, 2 * J + oy, 2 * K + oz));
-      P2 f = *px;
+      P2 f = fv[oz][oy];
       C c0 = wl[0] * yz[0] + wm * yz[1] + wu[0] * yz[2];
       C c1 = wl[1] * yz[0] + wm * yz[1] + wu[1] * yz[2];
       f.a = store_as<Sf, C>((C)f.a + c0);
