@@ -1112,11 +1112,11 @@ int mg_coarse(Ctx *ctx) {
       using C = std::remove_pointer_t<decltype(ctag)>;
       constexpr bool QL = decltype(qtag)::value;
       size_t sh_xz = QL ? qxz * sizeof(C) : 0, sh_y = QL ? qyy * sizeof(C) : 0;
-      hipLaunchKernelGGL((k_coarse_xz_fwd<S, C, QL>), dim3(d.ny), dim3(1024), sh_xz, ctx->stream,
+      hipLaunchKernelGGL((k_coarse_xz_fwd<S, C, QL>), dim3(d.ny), dim3(COARSE_T), sh_xz, ctx->stream,
                          d, (const S *)ctx->mgb[L].ptr, qx, qz, ctx->cwork);
-      hipLaunchKernelGGL((k_coarse_y<C, QL>), dim3(d.nz), dim3(1024), sh_y, ctx->stream, d, qy,
+      hipLaunchKernelGGL((k_coarse_y<C, QL>), dim3(d.nz), dim3(COARSE_T), sh_y, ctx->stream, d, qy,
                          lx, ly, lz, h2, ctx->cwork);
-      hipLaunchKernelGGL((k_coarse_xz_inv<S, C, QL>), dim3(d.ny), dim3(1024), sh_xz, ctx->stream,
+      hipLaunchKernelGGL((k_coarse_xz_inv<S, C, QL>), dim3(d.ny), dim3(COARSE_T), sh_xz, ctx->stream,
                          d, (const double *)ctx->cwork, qx, qz, (S *)ctx->mgx[L].ptr);
     };
     if (c64) {

@@ -321,12 +321,26 @@ k_prolong_add_2x(Dims df, Dims dc, const Sc *__restrict__ ec, Sf *__restrict__ x
 // the compute type C, as every use converts them): the dot products then
 // chain LDS reads rather than global loads.  The host passes QL when they
 // fit in 48 KB.
-template <bool QL, class C>
-__device__ inline const C *coarse_q(C *dst, const double *__restrict__ q, int n) {
-  if (!QL) return nullptr;
-  for (int t = threadIdx.x; t < n * n; t += blockDim.x) dst[t] = (C)q[t];
-  return dst;
-}
+//
+// The three kernels run COARSE_T threads per block.  Each stages its inputs
+// (the eigenvector matrices and one plane) with every thread's first loads
+// issued together before its LDS stores, so the prologue costs one memory
+// round trip rather than one per array (arrays of up to COARSE_T values, as
+// at 24 x 32 x 24, need no more; larger ones copy the rest afterwards).
+#define COARSE_T 1024
+
+// a thread's first element of the block-wide copy of n >= 1 values f(t),
+// loaded and stored with no branch (the index clamped to n - 1: threads past
+// n store element n - 1's own value there), so that the compiler keeps the
+// loads of all the copies together; the elements past COARSE_T follow in a loop
+template <class C> struct CoarseFirst {
+  C v;
+  template <class F> __device__ void load(int n, F f) { v = f(min((int)threadIdx.x, n - 1)); }
+  template <class F> __device__ void store(C *dst, int n, F f) const {
+    dst[min((int)threadIdx.x, n - 1)] = v;
+    for (int t = (int)threadIdx.x + COARSE_T; t < n; t += COARSE_T) dst[t] = f(t);
+  }
+};
 
 // sum of f(0) .. f(n-1) in four interleaved partial sums (a quarter of the
 // dependent chain of one running sum), combined pairwise
@@ -346,21 +360,30 @@ __device__ inline auto dot4(int n, F f) -> decltype(f(0)) {
 }
 
 template <class S, class C, bool QL>
-__global__ void __launch_bounds__(1024)
+__global__ void __launch_bounds__(COARSE_T)
 k_coarse_xz_fwd(Dims d, const S *__restrict__ b, const double *__restrict__ qx,
                 const double *__restrict__ qz, double *__restrict__ work) {
   __shared__ C s0[COARSE_MAX_PLANE], s1[COARSE_MAX_PLANE];
   extern __shared__ unsigned char dyn_q[];
   int j = blockIdx.x;
   int nx = d.nx, nz = d.nz, np = nx * nz;
-  const C *lqx = coarse_q<QL, C>((C *)dyn_q, qx, nx);
-  const C *lqz = coarse_q<QL, C>((C *)dyn_q + nx * nx, qz, nz);
+  C *const lqx = (C *)dyn_q, *const lqz = lqx + nx * nx;
   auto QX = [&](int r) { return QL ? lqx[r] : (C)qx[r]; };
   auto QZ = [&](int r) { return QL ? lqz[r] : (C)qz[r]; };
-  for (int t = threadIdx.x; t < np; t += blockDim.x) {
-    int i = t % nx, k = t / nx;
-    s0[t] = (C)b[idx3(d, i, j, k)];
+  auto fx = [&](int t) { return (C)qx[t]; };
+  auto fz = [&](int t) { return (C)qz[t]; };
+  auto fb = [&](int t) { return (C)b[idx3(d, t % nx, j, t / nx)]; };
+  CoarseFirst<C> hx, hz, hb;
+  if (QL) {
+    hx.load(nx * nx, fx);
+    hz.load(nz * nz, fz);
   }
+  hb.load(np, fb);
+  if (QL) {
+    hx.store(lqx, nx * nx, fx);
+    hz.store(lqz, nz * nz, fz);
+  }
+  hb.store(s0, np, fb);
   __syncthreads();
   // s1[k][a] = sum_m Qx[m][a] s0[k][m]
   for (int t = threadIdx.x; t < np; t += blockDim.x) {
@@ -376,7 +399,7 @@ k_coarse_xz_fwd(Dims d, const S *__restrict__ b, const double *__restrict__ qx,
 }
 
 template <class C, bool QL>
-__global__ void __launch_bounds__(1024)
+__global__ void __launch_bounds__(COARSE_T)
 k_coarse_y(Dims d, const double *__restrict__ qy, const double *__restrict__ lx,
            const double *__restrict__ ly, const double *__restrict__ lz, double h2,
            double *__restrict__ work) {
@@ -384,18 +407,24 @@ k_coarse_y(Dims d, const double *__restrict__ qy, const double *__restrict__ lx,
   extern __shared__ unsigned char dyn_q[];
   int cz = blockIdx.x;
   int nx = d.nx, ny = d.ny, np = nx * ny;
-  const C *lqy = coarse_q<QL, C>((C *)dyn_q, qy, ny);
+  C *const lqy = (C *)dyn_q;
   auto QY = [&](int r) { return QL ? lqy[r] : (C)qy[r]; };
-  for (int t = threadIdx.x; t < np; t += blockDim.x) {
-    int a = t % nx, m = t / nx;
-    s0[t] = (C)work[idx3(d, a, m, cz)];
-  }
+  auto fy = [&](int t) { return (C)qy[t]; };
+  auto fw = [&](int t) { return (C)work[idx3(d, t % nx, t / nx, cz)]; };
+  // the eigenvalue sum of output t = a + nx bb
+  auto fe = [&](int t) { return (C)(lx[t % nx] + ly[t / nx] + lz[cz]); };
+  CoarseFirst<C> hy, hw, he;
+  if (QL) hy.load(ny * ny, fy);
+  hw.load(np, fw);
+  he.load(np, fe);
+  if (QL) hy.store(lqy, ny * ny, fy);
+  hw.store(s0, np, fw);
   __syncthreads();
   // s1[bb][a] = (sum_m Qy[m][bb] s0[m][a]) / (lx_a + ly_bb + lz_cz) * (-h2)
-  for (int t = threadIdx.x; t < np; t += blockDim.x) {
+  for (int t = threadIdx.x; t < np; t += COARSE_T) {
     int a = t % nx, bb = t / nx;
     C acc = dot4(ny, [&](int m) { return QY(m * ny + bb) * s0[m * nx + a]; });
-    s1[t] = acc * (C)(-h2) / (C)(lx[a] + ly[bb] + lz[cz]);
+    s1[t] = acc * (C)(-h2) / (t == (int)threadIdx.x ? he.v : fe(t));
   }
   __syncthreads();
   // back: work[m][a] = sum_bb Qy[m][bb] s1[bb][a]
@@ -406,21 +435,30 @@ k_coarse_y(Dims d, const double *__restrict__ qy, const double *__restrict__ lx,
 }
 
 template <class S, class C, bool QL>
-__global__ void __launch_bounds__(1024)
+__global__ void __launch_bounds__(COARSE_T)
 k_coarse_xz_inv(Dims d, const double *__restrict__ work, const double *__restrict__ qx,
                 const double *__restrict__ qz, S *__restrict__ x) {
   __shared__ C s0[COARSE_MAX_PLANE], s1[COARSE_MAX_PLANE];
   extern __shared__ unsigned char dyn_q[];
   int j = blockIdx.x;
   int nx = d.nx, nz = d.nz, np = nx * nz;
-  const C *lqx = coarse_q<QL, C>((C *)dyn_q, qx, nx);
-  const C *lqz = coarse_q<QL, C>((C *)dyn_q + nx * nx, qz, nz);
+  C *const lqx = (C *)dyn_q, *const lqz = lqx + nx * nx;
   auto QX = [&](int r) { return QL ? lqx[r] : (C)qx[r]; };
   auto QZ = [&](int r) { return QL ? lqz[r] : (C)qz[r]; };
-  for (int t = threadIdx.x; t < np; t += blockDim.x) {
-    int a = t % nx, cc = t / nx;
-    s0[t] = (C)work[idx3(d, a, j, cc)];
+  auto fx = [&](int t) { return (C)qx[t]; };
+  auto fz = [&](int t) { return (C)qz[t]; };
+  auto fw = [&](int t) { return (C)work[idx3(d, t % nx, j, t / nx)]; };
+  CoarseFirst<C> hx, hz, hw;
+  if (QL) {
+    hx.load(nx * nx, fx);
+    hz.load(nz * nz, fz);
   }
+  hw.load(np, fw);
+  if (QL) {
+    hx.store(lqx, nx * nx, fx);
+    hz.store(lqz, nz * nz, fz);
+  }
+  hw.store(s0, np, fw);
   __syncthreads();
   // s1[n][a] = sum_c Qz[n][c] s0[c][a]
   for (int t = threadIdx.x; t < np; t += blockDim.x) {
