@@ -254,23 +254,44 @@ k_advect_mac_lds(Dims d, float dth, float damp, const float *__restrict__ u,
   const int x0 = blockIdx.x * AL_TX, y0 = blockIdx.y * AL_TY, z0 = blockIdx.z * AL_TZ;
   const int tid = threadIdx.x;
   // staging by (x, y) columns through the box's planes: the column's
-  // position is formed once, a plane adds the (uniform) plane offset
+  // position is formed once, a plane adds the (uniform) plane offset.  A
+  // thread's two columns load together (one round trip); the second is
+  // clamped to the box's last column, which threads past it restage with
+  // that column's own values.
+  static_assert(AL_EX * AL_EY <= 2 * AL_TX * AL_TY, "two columns per thread");
   const size_t plane = (size_t)d.nx * d.ny;
-  for (int col = tid; col < AL_EX * AL_EY; col += AL_TX * AL_TY) {
-    const int lx = col % AL_EX, ly = col / AL_EX;
-    const size_t cxy = (size_t)clampi(y0 - 1 + ly, 0, d.ny - 1) * d.nx +
-                       clampi(x0 - 1 + lx, 0, d.nx - 1);
+  {
+    int cl[2];
+    float3 cvv[2][AL_EZ];
+    float fu[2][AL_EZ], fv[2][AL_EZ], fw[2][AL_EZ];
 #pragma unroll
-    for (int lz = 0; lz < AL_EZ; ++lz) {
-      const size_t c = cxy + (size_t)clampi(z0 - 1 + lz, 0, d.nz - 1) * plane;
-      const float3 cv = cvel[c];
-      const int t = lz * (AL_EX * AL_EY) + col, tv = lz * AL_VP + ly * AL_VR + lx;
-      sc[tv] = cv.x;
-      sc[tv + AL_EX] = cv.y;
-      sc[tv + 2 * AL_EX] = cv.z;
-      sf[0][t] = u[c];
-      sf[1][t] = v[c];
-      sf[2][t] = w[c];
+    for (int h = 0; h < 2; ++h) {
+      const int col = min(tid + h * AL_TX * AL_TY, AL_EX * AL_EY - 1);
+      cl[h] = col;
+      const size_t cxy = (size_t)clampi(y0 - 1 + col / AL_EX, 0, d.ny - 1) * d.nx +
+                         clampi(x0 - 1 + col % AL_EX, 0, d.nx - 1);
+#pragma unroll
+      for (int lz = 0; lz < AL_EZ; ++lz) {
+        const size_t c = cxy + (size_t)clampi(z0 - 1 + lz, 0, d.nz - 1) * plane;
+        cvv[h][lz] = cvel[c];
+        fu[h][lz] = u[c];
+        fv[h][lz] = v[c];
+        fw[h][lz] = w[c];
+      }
+    }
+#pragma unroll
+    for (int h = 0; h < 2; ++h) {
+      const int col = cl[h], lx = col % AL_EX, ly = col / AL_EX;
+#pragma unroll
+      for (int lz = 0; lz < AL_EZ; ++lz) {
+        const int t = lz * (AL_EX * AL_EY) + col, tv = lz * AL_VP + ly * AL_VR + lx;
+        sc[tv] = cvv[h][lz].x;
+        sc[tv + AL_EX] = cvv[h][lz].y;
+        sc[tv + 2 * AL_EX] = cvv[h][lz].z;
+        sf[0][t] = fu[h][lz];
+        sf[1][t] = fv[h][lz];
+        sf[2][t] = fw[h][lz];
+      }
     }
   }
   __syncthreads();
@@ -810,20 +831,36 @@ k_dye_predict_lds(Dims d, float dth, const float3 *__restrict__ cvel,
   __shared__ float sd[AL_N];      // dye
   const int x0 = blockIdx.x * AL_TX, y0 = blockIdx.y * AL_TY, z0 = blockIdx.z * AL_TZ;
   const int tid = threadIdx.x;
+  // (staged as in k_advect_mac_lds: a thread's two columns load together)
   const size_t plane = (size_t)d.nx * d.ny;
-  for (int col = tid; col < AL_EX * AL_EY; col += AL_TX * AL_TY) {
-    const int lx = col % AL_EX, ly = col / AL_EX;
-    const size_t cxy = (size_t)clampi(y0 - 1 + ly, 0, d.ny - 1) * d.nx +
-                       clampi(x0 - 1 + lx, 0, d.nx - 1);
+  {
+    int cl[2];
+    float3 cvv[2][AL_EZ];
+    float fd[2][AL_EZ];
 #pragma unroll
-    for (int lz = 0; lz < AL_EZ; ++lz) {
-      const size_t c = cxy + (size_t)clampi(z0 - 1 + lz, 0, d.nz - 1) * plane;
-      const float3 cv = cvel[c];
-      const int tv = lz * AL_VP + ly * AL_VR + lx;
-      sc[tv] = cv.x;
-      sc[tv + AL_EX] = cv.y;
-      sc[tv + 2 * AL_EX] = cv.z;
-      sd[lz * (AL_EX * AL_EY) + col] = dye[c];
+    for (int h = 0; h < 2; ++h) {
+      const int col = min(tid + h * AL_TX * AL_TY, AL_EX * AL_EY - 1);
+      cl[h] = col;
+      const size_t cxy = (size_t)clampi(y0 - 1 + col / AL_EX, 0, d.ny - 1) * d.nx +
+                         clampi(x0 - 1 + col % AL_EX, 0, d.nx - 1);
+#pragma unroll
+      for (int lz = 0; lz < AL_EZ; ++lz) {
+        const size_t c = cxy + (size_t)clampi(z0 - 1 + lz, 0, d.nz - 1) * plane;
+        cvv[h][lz] = cvel[c];
+        fd[h][lz] = dye[c];
+      }
+    }
+#pragma unroll
+    for (int h = 0; h < 2; ++h) {
+      const int col = cl[h], lx = col % AL_EX, ly = col / AL_EX;
+#pragma unroll
+      for (int lz = 0; lz < AL_EZ; ++lz) {
+        const int tv = lz * AL_VP + ly * AL_VR + lx;
+        sc[tv] = cvv[h][lz].x;
+        sc[tv + AL_EX] = cvv[h][lz].y;
+        sc[tv + 2 * AL_EX] = cvv[h][lz].z;
+        sd[lz * (AL_EX * AL_EY) + col] = fd[h][lz];
+      }
     }
   }
   __syncthreads();
