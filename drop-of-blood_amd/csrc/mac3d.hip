@@ -1082,12 +1082,15 @@ int mg_prolong(Ctx *ctx, int l) {
     using Sc = std::remove_pointer_t<decltype(ctag)>;
     with_type(ctx->mgx[l].type, [&](auto ftag) {
       using Sf = std::remove_pointer_t<decltype(ftag)>;
-      // 32 x 8 threads over the coarse grid (coarse rows of 96 fill three blocks)
-      hipLaunchKernelGGL((k_prolong_add_2x<Sc, Sf>),
-                         dim3((dc.nx + PR_BX - 1) / PR_BX, (dc.ny + PR_BY - 1) / PR_BY, dc.nz),
-                         dim3(PR_BX, PR_BY), 0,
-                         ctx->stream, df, dc, (const Sc *)ctx->mgx[l + 1].ptr,
-                         (Sf *)ctx->mgx[l].ptr);
+      // 32 x 8 threads over the coarse grid (coarse rows of 96 fill three
+      // blocks); the fine cells loaded early on grids of under 8 blocks per CU
+      // (192x256x192's levels 1-3: 6 us per step faster; onto its level 0,
+      // 4608 blocks, about 1 us per launch slower)
+      const dim3 grid((dc.nx + PR_BX - 1) / PR_BX, (dc.ny + PR_BY - 1) / PR_BY, dc.nz);
+      const bool pre = (size_t)grid.x * grid.y * grid.z < (size_t)8 * ctx->ncu;
+      auto kern = pre ? k_prolong_add_2x<Sc, Sf, true> : k_prolong_add_2x<Sc, Sf, false>;
+      hipLaunchKernelGGL(kern, grid, dim3(PR_BX, PR_BY), 0, ctx->stream, df, dc,
+                         (const Sc *)ctx->mgx[l + 1].ptr, (Sf *)ctx->mgx[l].ptr);
     });
   });
   LAUNCHCHK();

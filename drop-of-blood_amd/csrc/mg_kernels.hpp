@@ -242,7 +242,10 @@ coarse_at(const Sc *__restrict__ e, const Dims &dc, int i, int j, int k) {
 // from LDS rather than issuing 27 gathers.
 #define PR_BX 32
 #define PR_BY 8
-template <class Sc, class Sf>
+// PRE: the thread's fine cells are loaded before the staging, so the two
+// round trips overlap (grids of few blocks per CU, where no other block
+// hides them; on large grids the early loads cost more than they save).
+template <class Sc, class Sf, bool PRE>
 __global__ void __launch_bounds__(PR_BX * PR_BY)
 k_prolong_add_2x(Dims df, Dims dc, const Sc *__restrict__ ec, Sf *__restrict__ xf) {
   typedef typename compute_of<Sf>::type C;
@@ -251,11 +254,9 @@ k_prolong_add_2x(Dims df, Dims dc, const Sc *__restrict__ ec, Sf *__restrict__ x
   const int I0 = blockIdx.x * PR_BX, J0 = blockIdx.y * PR_BY, K = blockIdx.z;
   const int I = I0 + tx, J = J0 + ty;
   const bool in = I < dc.nx && J < dc.ny;
-  // the thread's fine cells are loaded first, so their round trip overlaps
-  // the staging's (on small grids few blocks hide latency)
   struct alignas(2 * sizeof(Sf)) P2 { Sf a, b; };
   P2 fv[2][2];
-  if (in) {
+  if (PRE && in) {
 #pragma unroll
     for (int oz = 0; oz < 2; ++oz)
 #pragma unroll
@@ -293,7 +294,7 @@ k_prolong_add_2x(Dims df, Dims dc, const Sc *__restrict__ ec, Sf *__restrict__ x
       }
       // the two x-children as one aligned pair (2I is even)
       P2 *px = (P2 *)(xf + idx3(df, 2 * I, 2 * J + oy, 2 * K + oz));
-      P2 f = fv[oz][oy];
+      P2 f = PRE ? fv[oz][oy] : *px;
       C c0 = wl[0] * yz[0] + wm * yz[1] + wu[0] * yz[2];
       C c1 = wl[1] * yz[0] + wm * yz[1] + wu[1] * yz[2];
       f.a = store_as<Sf, C>((C)f.a + c0);
