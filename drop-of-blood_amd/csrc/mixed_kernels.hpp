@@ -219,11 +219,24 @@ __global__ void __launch_bounds__(256)
 k_fold3(const double *__restrict__ part, size_t n, double *__restrict__ out,
         double *__restrict__ scale) {
   double mx = 0.0, e = 0.0, f = 0.0;
-  for (size_t q = threadIdx.x; q < n; q += 256) {
-    double a = part[3 * q];
-    mx = a > mx ? a : mx;
-    e += part[3 * q + 1];
-    f += part[3 * q + 2];
+  // four of the thread's triples loaded together (indices past n clamped to
+  // n - 1, not summed), then summed in the order q = t, t + 256, ...
+  for (size_t q0 = threadIdx.x; q0 < n; q0 += 4 * 256) {
+    double a[4], b[4], c[4];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      const size_t q = min(q0 + (size_t)u * 256, n - 1);
+      a[u] = part[3 * q];
+      b[u] = part[3 * q + 1];
+      c[u] = part[3 * q + 2];
+    }
+#pragma unroll
+    for (int u = 0; u < 4; ++u)
+      if (q0 + (size_t)u * 256 < n) {
+        mx = a[u] > mx ? a[u] : mx;
+        e += b[u];
+        f += c[u];
+      }
   }
   mx = wave_max(mx);
   e = wave_sum(e);
