@@ -85,33 +85,38 @@ __device__ inline void residual64_zm(const Dims &d, const Src &src, int kbase, i
   const int gx = x0 + tx, gy = y0 + ty;
   const bool inside = gx < nx && gy < ny;
   const int bo = inside ? (gy * nx + gx) * 8 : MAC3D_OFF_NONE;
-  typename Src::Reg pre[R64_LPT];
-  auto fetch = [&](int k) {
+  // two register sets of staged values: plane k + 1 is staged from one while
+  // plane k + 2's loads are in flight in the other
+  typename Src::Reg pre[2][R64_LPT];
+  auto fetch = [&](int k, int h) {
 #pragma unroll
     for (int q = 0; q < R64_LPT; ++q)
-      pre[q] = src.load(so8[q] + (k - kbase) * pb8, so4[q] + (k - kbase) * pb4);
+      pre[h][q] = src.load(so8[q] + (k - kbase) * pb8, so4[q] + (k - kbase) * pb4);
   };
-  auto stage = [&](int slot) {
+  auto stage = [&](int slot, int h) {
 #pragma unroll
     for (int q = 0; q < R64_LPT; ++q)
-      if (tid + q * R64_T < R64_PL) sq[slot][sr[q]][sc[q]] = src.value(pre[q]);
+      if (tid + q * R64_T < R64_PL) sq[slot][sr[q]][sc[q]] = src.value(pre[h][q]);
   };
   // diagonal at this column: the lid adds 1 whether or not j is the top row
   const double dgxy = 1.0 + (gx > 0) + (gx < nx - 1) + (gy > 0);
   double mx = 0.0, rr = 0.0, bb = 0.0;
-  // prologue: planes k0 - 1 and k0 staged, k0 + 1 in flight
-  fetch(k0 - 1);
-  stage(0);
-  fetch(k0);
-  stage(1);
-  fetch(k0 + 1);
+  // prologue: planes k0 - 1 and k0 staged, k0 + 1 and k0 + 2 in flight
+  fetch(k0 - 1, 0);
+  fetch(k0, 1);
+  stage(0, 0);
+  stage(1, 1);
+  fetch(k0 + 1, 1);
+  fetch(k0 + 2, 0);
   double bnext = buf_load<double>(rbv, bo + (k0 - kbase) * pb8);
   __syncthreads();
   double qp = sq[0][ty + 1][tx + 1], qc = sq[1][ty + 1][tx + 1];
   int s0 = 0, s1 = 1, s2 = 2;  // slots of planes k-1, k, k+1
-  for (int k = k0; k < k1; ++k) {
-    stage(s2);
-    fetch(k + 2);
+  // iteration k stages plane k + 1 from set (k - k0 + 1) & 1 and refills it
+  // with plane k + 3
+  auto body = [&](int k, int h) {
+    stage(s2, h);
+    fetch(k + 3, h);
     double bc = bnext;
     bnext = buf_load<double>(rbv, bo + (k + 1 - kbase) * pb8);
     __syncthreads();
@@ -141,7 +146,13 @@ __device__ inline void residual64_zm(const Dims &d, const Src &src, int kbase, i
     s0 = s1;
     s1 = s2;
     s2 = t;
+  };
+  int k = k0;
+  for (; k + 1 < k1; k += 2) {
+    body(k, 1);
+    body(k + 1, 0);
   }
+  if (k < k1) body(k, 1);
   mx = wave_max(mx);
   rr = wave_sum(rr);
   bb = wave_sum(bb);
