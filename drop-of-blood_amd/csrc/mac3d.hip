@@ -1532,7 +1532,9 @@ int create_impl(Ctx *ctx, const mac3d_config *cfg) {
   if ((rc = alloc(ctx, ctx->cvel, t0, 3 * n0))) return rc;
   if ((rc = alloc(ctx, ctx->dmm, t0, 2 * n0))) return rc;
   bool mixed = ctx->prec == MAC3D_PREC_MIXED;
-  ctx->omega = cfg->sor_omega > 0.0 ? cfg->sor_omega : (mixed ? 1.2 : 1.0);
+  // mixed: 1.25, measured fastest (cycles per step at 192x256x192 over 100
+  // steps: 3.34 at 1.2, 3.06 at 1.24-1.25, 3.17 at 1.28)
+  ctx->omega = cfg->sor_omega > 0.0 ? cfg->sor_omega : (mixed ? 1.25 : 1.0);
   ctx->extrap = mixed ? (cfg->extrapolate < 0 ? 2 : std::min(cfg->extrapolate, 2)) : 0;
   if (mixed) {
     for (int q = 0; q < 4; ++q)

@@ -156,7 +156,7 @@ typedef struct mac3d_config {
   double maccormack;    /* dye MacCormack correction strength */
   double ir_tol;        /* mixed: relative divergence target ||r||/||b|| */
   double sor_omega;     /* smoother over-relaxation (0: mode default, 1 in the
-                           FP32/FP64 modes as the reference, 1.2 in mixed) */
+                           FP32/FP64 modes as the reference, 1.25 in mixed) */
 } mac3d_config;
 
 /* one spherical blood blob of the seeded initial condition (cell units:

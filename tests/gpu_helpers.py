@@ -8,9 +8,9 @@ from mac3d.lib import config_for
 
 
 def oracle_for(w, **kw):
-    # the mixed mode smooths with over-relaxation 1.2 (the device default)
+    # the mixed mode smooths with over-relaxation 1.25 (the device default)
     if w.precision == 1:
-        kw.setdefault("omega", 1.2)
+        kw.setdefault("omega", 1.25)
     return oracle.Oracle(w.nx, w.ny, w.nz, w.dx, levels=w.levels, nu1=w.nu1, nu2=w.nu2,
                          cycles=w.cycles, solve_to_tol=(w.precision == 1), tol=w.ir_tol,
                          max_cycles=30, **kw)

@@ -53,7 +53,7 @@ def accuracy(steps, every):
     threads = oracle.set_threads(int(os.environ.get("OMP_NUM_THREADS", "0") or 0))
     for name in ("ref", "mixed"):
         w = scenarios.workload(name, 1)
-        kw = dict(solve_to_tol=True, tol=w.ir_tol, max_cycles=30, omega=1.2) if w.precision == 1 \
+        kw = dict(solve_to_tol=True, tol=w.ir_tol, max_cycles=30, omega=1.25) if w.precision == 1 \
             else dict(cycles=w.cycles)
         o = oracle.Oracle(w.nx, w.ny, w.nz, w.dx, levels=w.levels, **kw)
         o.init_blobs(w.blobs)

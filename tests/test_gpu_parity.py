@@ -305,8 +305,8 @@ def test_graph_replays_match_eager(prec):
 
 @pytest.mark.parametrize("level", [0, 1])
 def test_over_relaxed_smoother_fp64(level):
-    """The mixed mode's over-relaxed red-black sweep (omega 1.2) against the
-    oracle's, in the FP64 device mode."""
+    """The over-relaxed red-black sweep (omega 1.2; the mixed mode smooths
+    with 1.25) against the oracle's, in the FP64 device mode."""
     from mac3d import MacSolver
     from mac3d.lib import config_for
     from mac3d import scenarios
@@ -329,16 +329,17 @@ def test_over_relaxed_smoother_fp64(level):
 def test_two_sweep_pass_fp16_level_against_oracle(op):
     """Mixed mode, 192x256x192: level 2 (48x64x48, FP16 storage) smooths a
     pair of sweeps in one two-sweep pass, FP32 in between and rounded to FP16
-    once on the store.  Against the oracle's two over-relaxed sweeps from the
-    same FP16 values: within 1e-3 relative (FP16 rounds to 2^-11)."""
+    once on the store.  Against the oracle's two over-relaxed sweeps (omega
+    1.25, the mode's default) from the same FP16 values: within 1e-3 relative
+    (FP16 rounds to 2^-11)."""
     from mac3d import MacSolver
     from mac3d.lib import config_for
     from mac3d import scenarios
     w = scenarios.scaled(scenarios.workload("ref", 1), precision=1)
-    o = oracle.Oracle(w.nx, w.ny, w.nz, w.dx, levels=w.levels, omega=1.2)
+    o = oracle.Oracle(w.nx, w.ny, w.nz, w.dx, levels=w.levels, omega=1.25)
     rng = np.random.default_rng(5)
     level = 2
-    with MacSolver(config_for(w)) as s:
+    with MacSolver(config_for(w, sor_omega=1.25)) as s:
         shp = s.level_shape(level)
         x = rng.standard_normal(shp).astype(np.float16).astype(np.float64)
         b = rng.standard_normal(shp).astype(np.float16).astype(np.float64)
@@ -387,8 +388,8 @@ def test_two_sweep_pass_equals_two_sweeps(dims, levels, omega):
 
 
 def test_two_sweep_pass_against_oracle():
-    """The FP32 two-sweep pass (omega 1.2, the mixed mode's smoother) against
-    the oracle's two FP64 sweeps on the tiny grid."""
+    """The FP32 two-sweep pass with over-relaxation (omega 1.2) against the
+    oracle's two FP64 sweeps on the tiny grid."""
     from mac3d import MacSolver
     from mac3d.lib import config_for
     from mac3d import scenarios
